@@ -1,0 +1,175 @@
+/*
+ * ll_gpu.h - C ABI of the MI355X (gfx950) read-only analytics path over LLAMA's
+ * multi-level snapshot CSR.
+ *
+ * A graph handle mirrors, in HBM, the frozen levels of both directions of an
+ * ll_mlcsr_ro_graph (llama/include/llama/ll_mlcsr_graph.h:83-2003): per-level
+ * vertex tables (16-byte {adj_list_start, level_length, degree} entries,
+ * ll_mlcsr_helpers.h:141-149, flattened from the copy-on-write pages of
+ * ll_mem_array.h:918-1485), per-level edge tables of 64-bit words with their
+ * continuation records (ll_mlcsr_sp.h:1474-1497), and the deletion vector kept
+ * in the high 26 bits of each word (ll_mlcsr_helpers.h:90-107).  Each newly
+ * frozen level is pushed with an asynchronous host-to-device copy; deletions
+ * that rewrote words of older levels are replayed with
+ * ll_gpu_graph_patch_deletions.
+ *
+ * Conventions (all entry points):
+ *   - return 0 on success or a negative ll_gpu_status; the library never aborts;
+ *     ll_gpu_last_error() gives a message for the last failure on that context;
+ *   - host arrays are caller-owned and only borrowed for the call; device memory
+ *     is owned by the context or graph; one host thread per context at a time;
+ *   - `level` selects the snapshot: -1 = latest frozen level, otherwise 0..L;
+ *     deleted words are those whose max-visible level <= level;
+ *   - node IDs are 0..max_nodes-1 (uint32 on the device, int64 node_t at the
+ *     boundary where the reference uses node_t).
+ */
+#ifndef LL_GPU_H
+#define LL_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  LL_GPU_OK = 0,
+  LL_GPU_EINVAL = -1,   /* bad argument                                       */
+  LL_GPU_ENOMEM = -2,   /* device or pinned allocation failed                 */
+  LL_GPU_EHIP = -3,     /* HIP runtime error                                  */
+  LL_GPU_ESTATE = -4,   /* mirror inconsistent (level order, degree mismatch) */
+  LL_GPU_ENOTSUP = -5,  /* unsupported flag or layout                         */
+  LL_GPU_ERANGE = -6    /* size beyond a documented limit                     */
+} ll_gpu_status;
+
+#define LL_GPU_DIR_OUT 0u
+#define LL_GPU_DIR_IN 1u
+
+/* ll_gpu_level_view.flags / ll_gpu_graph_create flags */
+#define LL_GPU_HAS_DELETIONS 0x1u   /* words carry max-visible level (LL_DELETIONS)   */
+#define LL_GPU_HAS_CONT 0x2u        /* levels > 0 end runs with continuation records  */
+
+typedef struct ll_gpu_ctx ll_gpu_ctx;
+typedef struct ll_gpu_graph ll_gpu_graph;
+
+/* One frozen level of one direction, as laid out by init_level_from_degrees +
+ * init_node + finish_level_vertices (ll_mlcsr_sp.h:1375-1607, 786-808). */
+typedef struct {
+  uint32_t dir;              /* LL_GPU_DIR_OUT or LL_GPU_DIR_IN                  */
+  uint32_t level;            /* must equal the number of levels already pushed  */
+  uint64_t num_nodes;        /* max_nodes of this level                          */
+  uint64_t num_vt_entries;   /* num_nodes + 1 (sentinel entry at num_nodes)      */
+  uint64_t num_et_words;     /* edge-table words incl. continuation records      */
+  const void* vt;            /* 16-byte entries {int64 start; uint32 len; uint32 degree} */
+  const uint64_t* et;        /* edge-table words                                 */
+  uint32_t flags;            /* LL_GPU_HAS_DELETIONS | LL_GPU_HAS_CONT           */
+  const uint32_t* page_table;/* reserved for COW page upload; must be NULL       */
+} ll_gpu_level_view;
+
+/* Per-snapshot statistics (materialised on first use of a level). */
+typedef struct {
+  uint64_t num_nodes;
+  uint64_t live_edges;       /* E_live: words visible at the snapshot level      */
+  uint64_t stored_words;     /* words in all reachable segments (E_st)           */
+  uint64_t segments;         /* reachable segments                               */
+  uint64_t continuations;    /* C = sum over vertices of (segments - 1)          */
+  double materialize_ms;     /* device time of the last materialisation          */
+} ll_gpu_snapshot_info;
+
+/* --- context and mirror ------------------------------------------------------- */
+int ll_gpu_create(int device, ll_gpu_ctx** out);
+void ll_gpu_destroy(ll_gpu_ctx* ctx);
+const char* ll_gpu_last_error(const ll_gpu_ctx* ctx);
+int ll_gpu_sync(ll_gpu_ctx* ctx);
+/* Device stream the context launches on (a hipStream_t). */
+void* ll_gpu_stream(ll_gpu_ctx* ctx);
+
+/* Replaces: ll_database/ll_mlcsr_ro_graph ownership of the RO levels
+ * (ll_database.h:61-80, ll_mlcsr_graph.h:180-309). */
+int ll_gpu_graph_create(ll_gpu_ctx* ctx, uint64_t max_nodes, uint32_t flags, ll_gpu_graph** out);
+void ll_gpu_graph_destroy(ll_gpu_graph* g);
+/* Replaces: the level-freeze event ll_mlcsr_ro_graph::checkpoint
+ * (ll_mlcsr_graph.h:1501-1831) / load_direct (loaders/ll_load_utils.h:289-880),
+ * whose new VT_k / ET_k this call copies to HBM (async, pinned staging). */
+int ll_gpu_graph_push_level(ll_gpu_graph* g, const ll_gpu_level_view* view);
+/* Replaces: update_max_visible_level_lower_only on old levels
+ * (ll_mlcsr_sp.h:849-863; ll_mlcsr_graph.h:1091-1108).  edge_ids use the
+ * level<<38|index encoding (ll_mlcsr_helpers.h:77-79). */
+int ll_gpu_graph_patch_deletions(ll_gpu_graph* g, uint32_t dir, const uint64_t* edge_ids,
+                                 const uint32_t* max_level, uint64_t n);
+/* Replaces: ll_mlcsr_ro_graph::num_levels / max_nodes (ll_mlcsr_graph.h:626-659). */
+int ll_gpu_graph_num_levels(const ll_gpu_graph* g, uint32_t dir);
+uint64_t ll_gpu_graph_max_nodes(const ll_gpu_graph* g);
+/* Total bytes copied host-to-device and device time of the last push (ms). */
+int ll_gpu_graph_upload_stats(const ll_gpu_graph* g, uint64_t* bytes, double* last_ms);
+/* Materialise (or reuse) the snapshot at `level` and report its statistics. */
+int ll_gpu_snapshot_stats(ll_gpu_graph* g, int level, ll_gpu_snapshot_info* info);
+/* Snapshot adjacency in iterator order (newest segment first, stored order
+ * within a segment, deleted words skipped), copied to host arrays of
+ * max_nodes+1 offsets and live_edges neighbours.  Replaces iterating
+ * out_iter_begin/out_iter_next or inm_iter_* (ll_mlcsr_graph.h:751-915). */
+int ll_gpu_snapshot_adjacency(ll_gpu_graph* g, int level, uint32_t dir, uint64_t* offsets,
+                              uint32_t* neighbours);
+/* Live degrees at the snapshot (precomputed VT degree, ll_mlcsr_sp.h:1900-1983). */
+int ll_gpu_snapshot_degrees(ll_gpu_graph* g, int level, uint32_t dir, uint32_t* degrees);
+
+/* --- algorithms -------------------------------------------------------------- */
+/* Replaces: task `pagerank_double` / `pagerank` = ll_b_pagerank_pull_ext::run
+ * (benchmark/benchmarks/pagerank.h:98-182): r0 = 1/N, N = max_nodes,
+ * r' = (1-d)/N + d * sum_{w in in(t)} r[w]/outdeg(w), fixed `iters`, no dangling
+ * redistribution.  Computed in FP32 with compensated sums; ranks_out[max_nodes]
+ * receives the FP64-widened ranks; last_diff (may be NULL) the L1 change of
+ * the final iteration (the reference's unused `diff`). */
+int ll_gpu_pagerank(ll_gpu_graph* g, int level, int iters, double damping, double* ranks_out,
+                    double* last_diff);
+/* Replaces: task `sssp_unweighted` (benchmark/benchmarks/sssp.h:220-333) and
+ * `bfs` (bfs.h:85-128): levels_out[max_nodes] (may be NULL) gets the BFS level,
+ * INT_MAX-1 for unreached; reached = number of reached vertices (bfs_count);
+ * edges_examined = edges the direction-optimising traversal inspected. */
+int ll_gpu_bfs(ll_gpu_graph* g, int level, int64_t root, int32_t* levels_out, uint64_t* reached,
+               uint64_t* edges_examined);
+/* Multi-source BFS, sources in lanes of lane_width bits (64, or 0 = autotune
+ * among 16/32/64/128).  levels_out (may be NULL) is n x max_nodes int32,
+ * row-major by source; reached_out[n] (may be NULL) per-source reached counts. */
+int ll_gpu_bfs_batch(ll_gpu_graph* g, int level, const int64_t* roots, int n, int lane_width,
+                     int32_t* levels_out, uint64_t* reached_out);
+/* Replaces: tasks `tc_u` / `tc_od` (benchmark/benchmarks/triangle_counting.h:343-470,
+ * 682-831), extended to multi-level snapshots: triangles of the simple
+ * undirected graph of live edges (out and in, self-loops and duplicates removed). */
+int ll_gpu_tc(ll_gpu_graph* g, int level, uint64_t* triangles);
+/* Replaces: task `tarjan_scc` (benchmark/benchmarks/tarjan_scc.h:121-206) with
+ * a correct SCC decomposition; labels_out[max_nodes] = smallest vertex ID of
+ * each vertex's strongly connected component.  num_scc may be NULL. */
+int ll_gpu_scc(ll_gpu_graph* g, int level, int64_t* labels_out, uint64_t* num_scc);
+/* Batched personalized PageRank: seeds[n] single-vertex teleports,
+ * r0 = e_s, r' = (1-d) e_s + d * sum_{w in in(t)} r[w]/outdeg(w), at most
+ * `iters` iterations, stopping once every column's L1 change <= tol.
+ * ranks_out (may be NULL) is max_nodes x n FP32 row-major (vertex-major);
+ * batch_width 0 = autotune among 16/32/64/128.  iters_done may be NULL. */
+int ll_gpu_ppr_batch(ll_gpu_graph* g, int level, const int64_t* seeds, int n, int iters,
+                     double damping, double tol, int batch_width, float* ranks_out,
+                     int* iters_done);
+
+/* Weighted PageRank on the snapshot's out-edges, weights in the snapshot's
+ * out-edge order (ll_gpu_snapshot_adjacency(..., LL_GPU_DIR_OUT, ...)).
+ * Pointers below are DEVICE pointers on this context's device; the call is
+ * ordered on ll_gpu_stream(ctx) and returns after it completes.
+ *   forward: r0 = p, r' = (1-d) p + d * W^T r, `iters` iterations.
+ *   adjoint: given dL/dr of the returned r, solves the transposed system
+ *     lambda = dL/dr + d * W lambda with the same number of iterations
+ *     (a pull over out-edges = the transposed push sweep of
+ *     ll_b_pagerank_push_ext, pagerank.h:217-360) and returns
+ *     dL/dw[e=(u,v)] = d * r[u] * lambda[v] and dL/dp = (1-d) * lambda.
+ * The adjoint is that of the fixed point; it equals the derivative of the
+ * iterated forward once the iteration has converged. */
+int ll_gpu_wpr_forward(ll_gpu_graph* g, int level, const float* edge_w, const float* teleport,
+                       int iters, double damping, float* r_out);
+int ll_gpu_wpr_adjoint(ll_gpu_graph* g, int level, const float* edge_w, const float* r,
+                       const float* dl_dr, int iters, double damping, float* dl_dw_out,
+                       float* dl_dteleport_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LL_GPU_H */

@@ -1,0 +1,271 @@
+// C ABI over the host level builder, generator and .xs1 files (ll_host.h).
+#include "ll_host.h"
+
+#include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <new>
+#include <string>
+
+#include "ll_layout.h"
+
+struct ll_host_edges {
+  llhost::EdgeList e;
+  llhost::vec<float> w;
+};
+
+struct ll_host_graph {
+  std::unique_ptr<llhost::Graph> g;
+  std::unique_ptr<llhost::Updater> upd;
+  ll_host_edges last_ins, last_del;
+  llhost::vec<uint64_t> flat_ofs;
+  llhost::vec<uint32_t> flat_adj;
+};
+
+namespace {
+thread_local std::string g_err;
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+}  // namespace
+
+extern "C" {
+
+const char* ll_host_last_error(void) { return g_err.c_str(); }
+
+int ll_host_graph_create(uint64_t num_nodes, int deletions, ll_host_graph** out) {
+  if (!out) return fail(LL_GPU_EINVAL, "out is NULL");
+  if (num_nodes == 0 || num_nodes >= (uint64_t(1) << 32))
+    return fail(LL_GPU_ERANGE, "num_nodes must be in [1, 2^32)");
+  auto* h = new (std::nothrow) ll_host_graph();
+  if (!h) return fail(LL_GPU_ENOMEM, "out of host memory");
+  h->g.reset(new llhost::Graph(num_nodes, deletions != 0));
+  *out = h;
+  return LL_GPU_OK;
+}
+
+void ll_host_graph_destroy(ll_host_graph* g) { delete g; }
+
+uint64_t ll_host_graph_num_nodes(const ll_host_graph* g) { return g ? g->g->num_nodes() : 0; }
+int ll_host_graph_num_levels(const ll_host_graph* g) { return g ? g->g->num_levels() : 0; }
+int ll_host_graph_has_deletions(const ll_host_graph* g) { return g ? int(g->g->deletions()) : 0; }
+
+int ll_host_graph_add_level(ll_host_graph* g, const uint32_t* ins_src, const uint32_t* ins_dst,
+                            uint64_t n_ins, const uint32_t* del_src, const uint32_t* del_dst,
+                            uint64_t n_del) {
+  if (!g) return fail(LL_GPU_EINVAL, "graph is NULL");
+  if ((n_ins && (!ins_src || !ins_dst)) || (n_del && (!del_src || !del_dst)))
+    return fail(LL_GPU_EINVAL, "NULL edge array");
+  try {
+    if (!g->g->add_level(ins_src, ins_dst, n_ins, del_src, del_dst, n_del))
+      return fail(LL_GPU_EINVAL, g->g->last_error());
+    g->upd.reset();  // the sampling snapshot is rebuilt lazily from the graph
+  } catch (const std::bad_alloc&) {
+    return fail(LL_GPU_ENOMEM, "out of host memory while freezing a level");
+  }
+  return LL_GPU_OK;
+}
+
+int ll_host_graph_add_level_edges(ll_host_graph* g, const ll_host_edges* ins,
+                                  const ll_host_edges* del) {
+  if (!g) return fail(LL_GPU_EINVAL, "graph is NULL");
+  const uint64_t ni = ins ? ins->e.src.size() : 0, nd = del ? del->e.src.size() : 0;
+  return ll_host_graph_add_level(g, ni ? ins->e.src.data() : nullptr, ni ? ins->e.dst.data() : nullptr,
+                                 ni, nd ? del->e.src.data() : nullptr,
+                                 nd ? del->e.dst.data() : nullptr, nd);
+}
+
+int ll_host_graph_add_random_level(ll_host_graph* g, double ins_frac, double del_frac,
+                                   uint64_t seed) {
+  if (!g) return fail(LL_GPU_EINVAL, "graph is NULL");
+  if (g->g->num_levels() == 0) return fail(LL_GPU_ESTATE, "random levels need a base level");
+  if (del_frac > 0 && !g->g->deletions())
+    return fail(LL_GPU_EINVAL, "deletions need a graph created with deletions on");
+  if (ins_frac < 0 || del_frac < 0 || ins_frac > 1 || del_frac > 1)
+    return fail(LL_GPU_EINVAL, "fractions must be in [0, 1]");
+  try {
+    if (!g->upd) g->upd.reset(new llhost::Updater(*g->g));
+    llhost::EdgeList ins, del;
+    g->upd->next(ins_frac, del_frac, seed, ins, del);
+    if (!g->g->add_level(ins.src.data(), ins.dst.data(), ins.src.size(), del.src.data(),
+                         del.dst.data(), del.src.size())) {
+      g->upd.reset();
+      return fail(LL_GPU_ESTATE, g->g->last_error());
+    }
+    g->last_ins.e = std::move(ins);
+    g->last_ins.w.clear();
+    g->last_del.e = std::move(del);
+    g->last_del.w.clear();
+  } catch (const std::bad_alloc&) {
+    return fail(LL_GPU_ENOMEM, "out of host memory while drawing a level");
+  }
+  return LL_GPU_OK;
+}
+
+int ll_host_graph_last_batch(const ll_host_graph* g, const ll_host_edges** ins,
+                             const ll_host_edges** del) {
+  if (!g || !ins || !del) return fail(LL_GPU_EINVAL, "NULL argument");
+  *ins = &g->last_ins;
+  *del = &g->last_del;
+  return LL_GPU_OK;
+}
+
+int ll_host_graph_level_view(const ll_host_graph* g, uint32_t dir, uint32_t level,
+                             ll_gpu_level_view* out) {
+  if (!g || !out) return fail(LL_GPU_EINVAL, "NULL argument");
+  if (dir > 1) return fail(LL_GPU_EINVAL, "dir must be 0 (out) or 1 (in)");
+  if (int(level) >= g->g->num_levels()) return fail(LL_GPU_EINVAL, "no such level");
+  const auto& lv = g->g->level(int(dir), int(level));
+  std::memset(out, 0, sizeof(*out));
+  out->dir = dir;
+  out->level = level;
+  out->num_nodes = g->g->num_nodes();
+  out->num_vt_entries = lv.vt.size();
+  out->num_et_words = lv.et.size();
+  out->vt = lv.vt.data();
+  out->et = lv.et.data();
+  out->flags = LL_GPU_HAS_CONT | (g->g->deletions() ? LL_GPU_HAS_DELETIONS : 0u);
+  out->page_table = nullptr;
+  return LL_GPU_OK;
+}
+
+int ll_host_graph_patches(const ll_host_graph* g, uint32_t dir, uint32_t level,
+                          const uint64_t** edge_ids, const uint32_t** max_levels, uint64_t* n) {
+  if (!g || !edge_ids || !max_levels || !n) return fail(LL_GPU_EINVAL, "NULL argument");
+  if (dir > 1 || int(level) >= g->g->num_levels()) return fail(LL_GPU_EINVAL, "no such level");
+  const auto& p = g->g->patches(int(dir), int(level));
+  *n = p.edge_ids.size();
+  *edge_ids = p.edge_ids.data();
+  *max_levels = p.maxlev.data();
+  return LL_GPU_OK;
+}
+
+int ll_host_graph_flatten(ll_host_graph* g, uint32_t dir, int level, const uint64_t** offsets,
+                          const uint32_t** neighbours, uint64_t* num_edges) {
+  if (!g || !offsets || !neighbours || !num_edges) return fail(LL_GPU_EINVAL, "NULL argument");
+  const int L = g->g->num_levels();
+  if (L == 0) return fail(LL_GPU_ESTATE, "graph has no levels");
+  if (level < 0) level = L - 1;
+  if (dir > 1 || level >= L) return fail(LL_GPU_EINVAL, "no such level");
+  g->g->flatten(int(dir), level, g->flat_ofs, g->flat_adj);
+  *offsets = g->flat_ofs.data();
+  *neighbours = g->flat_adj.data();
+  *num_edges = g->flat_adj.size();
+  return LL_GPU_OK;
+}
+
+int ll_host_gen_rmat(uint32_t scale, double edge_factor, double a, double b, double c,
+                     uint64_t seed, uint64_t perm_seed, ll_host_edges** out) {
+  if (!out) return fail(LL_GPU_EINVAL, "out is NULL");
+  if (scale < 1 || scale > 31) return fail(LL_GPU_ERANGE, "scale must be in [1, 31]");
+  if (!(a >= 0 && b >= 0 && c >= 0 && a + b + c <= 1.0 + 1e-12) || edge_factor <= 0)
+    return fail(LL_GPU_EINVAL, "bad R-MAT parameters");
+  try {
+    auto* e = new ll_host_edges();
+    llhost::gen_rmat(scale, edge_factor, a, b, c, seed, perm_seed, e->e);
+    *out = e;
+  } catch (const std::bad_alloc&) {
+    return fail(LL_GPU_ENOMEM, "out of host memory in the R-MAT generator");
+  }
+  return LL_GPU_OK;
+}
+
+int ll_host_edges_create(const uint32_t* src, const uint32_t* dst, const float* weight,
+                         uint64_t n, ll_host_edges** out) {
+  if (!out || (n && (!src || !dst))) return fail(LL_GPU_EINVAL, "NULL argument");
+  auto* e = new (std::nothrow) ll_host_edges();
+  if (!e) return fail(LL_GPU_ENOMEM, "out of host memory");
+  e->e.src.assign(src, src + n);
+  e->e.dst.assign(dst, dst + n);
+  if (weight) e->w.assign(weight, weight + n);
+  *out = e;
+  return LL_GPU_OK;
+}
+
+void ll_host_edges_destroy(ll_host_edges* e) { delete e; }
+uint64_t ll_host_edges_count(const ll_host_edges* e) { return e ? e->e.src.size() : 0; }
+const uint32_t* ll_host_edges_src(const ll_host_edges* e) { return e ? e->e.src.data() : nullptr; }
+const uint32_t* ll_host_edges_dst(const ll_host_edges* e) { return e ? e->e.dst.data() : nullptr; }
+const float* ll_host_edges_weight(const ll_host_edges* e) {
+  return e && !e->w.empty() ? e->w.data() : nullptr;
+}
+
+// X-Stream type 1: packed {u32 tail, u32 head, f32 weight}; the .ini side file
+// carries vertices= and edges= (loaders/ll_load_xstream1.h:146-152, 243-273).
+int ll_host_xs1_write(const char* path, const ll_host_edges* e, uint64_t num_nodes) {
+  if (!path || !e) return fail(LL_GPU_EINVAL, "NULL argument");
+  FILE* f = std::fopen(path, "wb");
+  if (!f) return fail(LL_GPU_EINVAL, std::string("cannot open ") + path);
+  const uint64_t n = e->e.src.size();
+  const size_t chunk = 1 << 16;
+  std::vector<unsigned char> buf(chunk * 12);
+  for (uint64_t i = 0; i < n; i += chunk) {
+    size_t m = size_t(std::min<uint64_t>(chunk, n - i));
+    for (size_t j = 0; j < m; j++) {
+      float w = e->w.empty() ? 1.0f : e->w[i + j];
+      std::memcpy(&buf[j * 12 + 0], &e->e.src[i + j], 4);
+      std::memcpy(&buf[j * 12 + 4], &e->e.dst[i + j], 4);
+      std::memcpy(&buf[j * 12 + 8], &w, 4);
+    }
+    if (std::fwrite(buf.data(), 12, m, f) != m) {
+      std::fclose(f);
+      return fail(LL_GPU_EINVAL, "short write");
+    }
+  }
+  std::fclose(f);
+  std::string ini = std::string(path) + ".ini";
+  FILE* fi = std::fopen(ini.c_str(), "w");
+  if (!fi) return fail(LL_GPU_EINVAL, "cannot write .ini");
+  std::fprintf(fi, "[graph]\ntype=1\nname=%s\nvertices=%llu\nedges=%llu\n", path,
+               (unsigned long long)num_nodes, (unsigned long long)n);
+  std::fclose(fi);
+  return LL_GPU_OK;
+}
+
+int ll_host_xs1_read(const char* path, ll_host_edges** out, uint64_t* num_nodes) {
+  if (!path || !out) return fail(LL_GPU_EINVAL, "NULL argument");
+  FILE* f = std::fopen(path, "rb");
+  if (!f) return fail(LL_GPU_EINVAL, std::string("cannot open ") + path);
+  std::fseek(f, 0, SEEK_END);
+  long sz = std::ftell(f);
+  std::fseek(f, 0, SEEK_SET);
+  if (sz < 0 || sz % 12 != 0) {
+    std::fclose(f);
+    return fail(LL_GPU_EINVAL, "file size is not a multiple of 12 bytes");
+  }
+  auto* e = new (std::nothrow) ll_host_edges();
+  if (!e) { std::fclose(f); return fail(LL_GPU_ENOMEM, "out of host memory"); }
+  uint64_t n = uint64_t(sz) / 12, vmax = 0;
+  e->e.src.resize(n);
+  e->e.dst.resize(n);
+  e->w.resize(n);
+  std::vector<unsigned char> buf(12);
+  for (uint64_t i = 0; i < n; i++) {
+    if (std::fread(buf.data(), 12, 1, f) != 1) {
+      std::fclose(f);
+      delete e;
+      return fail(LL_GPU_EINVAL, "short read");
+    }
+    std::memcpy(&e->e.src[i], &buf[0], 4);
+    std::memcpy(&e->e.dst[i], &buf[4], 4);
+    std::memcpy(&e->w[i], &buf[8], 4);
+    vmax = std::max<uint64_t>(vmax, std::max(e->e.src[i], e->e.dst[i]) + uint64_t(1));
+  }
+  std::fclose(f);
+  uint64_t v_ini = 0;
+  FILE* fi = std::fopen((std::string(path) + ".ini").c_str(), "r");
+  if (fi) {
+    char line[512];
+    while (std::fgets(line, sizeof line, fi))
+      if (std::strncmp(line, "vertices=", 9) == 0) v_ini = std::strtoull(line + 9, nullptr, 10);
+    std::fclose(fi);
+  }
+  if (num_nodes) *num_nodes = std::max(vmax, v_ini);
+  *out = e;
+  return LL_GPU_OK;
+}
+
+}  // extern "C"

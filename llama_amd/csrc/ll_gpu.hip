@@ -1,0 +1,342 @@
+// Context, device mirror of the frozen levels, uploads and deletion patches.
+//
+// The mirror keeps, per direction and level, the flattened vertex table
+// VT_k[V+1] and the edge table ET_k exactly as the host level builder froze
+// them (ll_mlcsr_sp.h:624-808, 1375-1607).  A device table of per-level ET base
+// pointers lets kernels follow a continuation record's edge ID
+// (level<<38 | index) into any older level, as iter_descend does
+// (ll_mlcsr_sp.h:2090-2134).
+#include <chrono>
+#include <cstring>
+
+#include "ll_gpu_internal.h"
+
+using llgpu::DevBuf;
+using llgpu::Error;
+
+namespace {
+
+constexpr size_t kStageBytes = size_t(64) << 20;  // two halves of pinned staging
+
+int fail(ll_gpu_ctx* c, const Error& e) {
+  if (c) c->err = e.msg;
+  return e.code;
+}
+
+template <typename F>
+int guarded(ll_gpu_ctx* c, F&& f) {
+  try {
+    f();
+    return LL_GPU_OK;
+  } catch (const Error& e) {
+    return fail(c, e);
+  } catch (const std::bad_alloc&) {
+    return fail(c, Error{LL_GPU_ENOMEM, "out of host memory"});
+  }
+}
+
+__global__ void k_apply_patches(const uint64_t* const* et_table, const uint64_t* ids,
+                                const uint32_t* maxlev, uint64_t n) {
+  uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint64_t id = ids[i];
+  uint64_t* et = const_cast<uint64_t*>(et_table[llgpu::edge_level(id)]);
+  uint64_t idx = llgpu::edge_index(id);
+  uint64_t w = et[idx];
+  uint32_t ml = maxlev[i];
+  // lower only, as update_max_visible_level_lower_only (ll_mlcsr_sp.h:849-863)
+  if (ml < llgpu::word_maxlev(w))
+    et[idx] = (w & llgpu::kIndexMask) | (uint64_t(ml) << llgpu::kBitsIndex);
+}
+
+// Copy host bytes to device through the double-buffered pinned staging area.
+void upload(ll_gpu_ctx* c, void* dst, const void* src, size_t bytes) {
+  const char* s = static_cast<const char*>(src);
+  char* d = static_cast<char*>(dst);
+  const size_t half = c->pinned_bytes / 2;
+  int slot = 0;
+  for (size_t off = 0; off < bytes; off += half, slot ^= 1) {
+    size_t m = std::min(half, bytes - off);
+    LLG_HIP(hipEventSynchronize(c->pinned_ev[slot]));
+    char* stage = static_cast<char*>(c->pinned) + slot * half;
+    std::memcpy(stage, s + off, m);
+    LLG_HIP(hipMemcpyAsync(d + off, stage, m, hipMemcpyHostToDevice, c->stream));
+    LLG_HIP(hipEventRecord(c->pinned_ev[slot], c->stream));
+  }
+}
+
+}  // namespace
+
+namespace llgpu {
+
+TimedLaunch::TimedLaunch(ll_gpu_ctx* c, const char* n) : ctx(c), name(n) {
+  if (!ctx->timer.on) return;
+  auto take = [&]() {
+    if (!ctx->timer.pool.empty()) {
+      hipEvent_t e = ctx->timer.pool.back();
+      ctx->timer.pool.pop_back();
+      return e;
+    }
+    hipEvent_t e;
+    LLG_HIP(hipEventCreate(&e));
+    return e;
+  };
+  a = take();
+  b = take();
+  LLG_HIP(hipEventRecord(a, ctx->stream));
+}
+
+TimedLaunch::~TimedLaunch() {
+  if (!a) return;
+  if (hipEventRecord(b, ctx->stream) == hipSuccess)
+    ctx->timer.pending.push_back({name, a, b});
+}
+
+int resolve_level(const ll_gpu_graph* g, int level) {
+  int L = int(g->dir[0].vt.size());
+  LLG_CHECK(L > 0 && int(g->dir[1].vt.size()) == L, LL_GPU_ESTATE,
+            "graph mirror needs the same number of out- and in-levels (>= 1)");
+  if (level < 0) level = L - 1;
+  LLG_CHECK(level < L, LL_GPU_EINVAL, "snapshot level beyond the pushed levels");
+  return level;
+}
+
+void drop_snapshots(ll_gpu_graph* g) {
+  for (int d = 0; d < 2; d++) {
+    g->deg_level[d] = -1;
+    g->deg[d].release();
+    g->csr[d] = Csr();
+    g->rb[d] = RowBlocks();
+  }
+  g->sym_level = -1;
+  g->sym_ofs.release();
+  g->sym_adj.release();
+  g->w_level = -1;
+  g->in2out.release();
+}
+
+}  // namespace llgpu
+
+extern "C" {
+
+int ll_gpu_create(int device, ll_gpu_ctx** out) {
+  if (!out) return LL_GPU_EINVAL;
+  *out = nullptr;
+  auto* c = new (std::nothrow) ll_gpu_ctx();
+  if (!c) return LL_GPU_ENOMEM;
+  int rc = guarded(c, [&]() {
+    int n = 0;
+    LLG_HIP(hipGetDeviceCount(&n));
+    LLG_CHECK(device >= 0 && device < n, LL_GPU_EINVAL, "no such device");
+    LLG_HIP(hipSetDevice(device));
+    c->device = device;
+    hipDeviceProp_t prop;
+    LLG_HIP(hipGetDeviceProperties(&prop, device));
+    c->num_cus = prop.multiProcessorCount;
+    LLG_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    hipError_t e = hipHostMalloc(&c->pinned, kStageBytes, hipHostMallocDefault);
+    LLG_CHECK(e == hipSuccess, LL_GPU_ENOMEM, "pinned staging allocation failed");
+    c->pinned_bytes = kStageBytes;
+    for (auto& ev : c->pinned_ev) {
+      LLG_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+      LLG_HIP(hipEventRecord(ev, c->stream));
+    }
+  });
+  if (rc != LL_GPU_OK) {
+    ll_gpu_destroy(c);
+    return rc;
+  }
+  *out = c;
+  return LL_GPU_OK;
+}
+
+void ll_gpu_destroy(ll_gpu_ctx* c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (auto& r : c->timer.pending) {
+    (void)hipEventDestroy(r.a);
+    (void)hipEventDestroy(r.b);
+  }
+  for (auto e : c->timer.pool) (void)hipEventDestroy(e);
+  for (auto& ev : c->pinned_ev)
+    if (ev) (void)hipEventDestroy(ev);
+  if (c->pinned) (void)hipHostFree(c->pinned);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+const char* ll_gpu_last_error(const ll_gpu_ctx* c) { return c ? c->err.c_str() : "NULL context"; }
+
+int ll_gpu_sync(ll_gpu_ctx* c) {
+  if (!c) return LL_GPU_EINVAL;
+  return guarded(c, [&]() { LLG_HIP(hipStreamSynchronize(c->stream)); });
+}
+
+void* ll_gpu_stream(ll_gpu_ctx* c) { return c ? static_cast<void*>(c->stream) : nullptr; }
+
+int ll_gpu_profile(ll_gpu_ctx* c, int on) {
+  if (!c) return LL_GPU_EINVAL;
+  return guarded(c, [&]() {
+    LLG_HIP(hipStreamSynchronize(c->stream));
+    for (auto& r : c->timer.pending) {
+      c->timer.pool.push_back(r.a);
+      c->timer.pool.push_back(r.b);
+    }
+    c->timer.pending.clear();
+    c->timer.acc.clear();
+    c->timer.on = on != 0;
+  });
+}
+
+int ll_gpu_profile_get(ll_gpu_ctx* c, const char* kernel, uint64_t* launches, double* total_ms) {
+  if (!c || !kernel) return LL_GPU_EINVAL;
+  return guarded(c, [&]() {
+    LLG_HIP(hipStreamSynchronize(c->stream));
+    for (auto& r : c->timer.pending) {
+      float ms = 0;
+      LLG_HIP(hipEventElapsedTime(&ms, r.a, r.b));
+      auto& a = c->timer.acc[r.name];
+      a.first += 1;
+      a.second += ms;
+      c->timer.pool.push_back(r.a);
+      c->timer.pool.push_back(r.b);
+    }
+    c->timer.pending.clear();
+    auto it = c->timer.acc.find(kernel);
+    if (launches) *launches = it == c->timer.acc.end() ? 0 : it->second.first;
+    if (total_ms) *total_ms = it == c->timer.acc.end() ? 0.0 : it->second.second;
+  });
+}
+
+int ll_gpu_graph_create(ll_gpu_ctx* c, uint64_t max_nodes, uint32_t flags, ll_gpu_graph** out) {
+  if (!c || !out) return LL_GPU_EINVAL;
+  *out = nullptr;
+  return guarded(c, [&]() {
+    LLG_CHECK(max_nodes > 0 && max_nodes < (uint64_t(1) << 32), LL_GPU_ERANGE,
+              "max_nodes must be in [1, 2^32) (uint32 vertex IDs on the device)");
+    LLG_CHECK((flags & ~(LL_GPU_HAS_DELETIONS | LL_GPU_HAS_CONT)) == 0, LL_GPU_ENOTSUP,
+              "unknown graph flags");
+    auto* g = new ll_gpu_graph();
+    g->ctx = c;
+    g->V = max_nodes;
+    g->flags = flags;
+    *out = g;
+  });
+}
+
+void ll_gpu_graph_destroy(ll_gpu_graph* g) {
+  if (!g) return;
+  (void)hipSetDevice(g->ctx->device);
+  (void)hipStreamSynchronize(g->ctx->stream);
+  delete g;
+  // stream-ordered frees were queued by the buffers' destructors
+}
+
+int ll_gpu_graph_push_level(ll_gpu_graph* g, const ll_gpu_level_view* v) {
+  if (!g || !v) return LL_GPU_EINVAL;
+  ll_gpu_ctx* c = g->ctx;
+  return guarded(c, [&]() {
+    LLG_HIP(hipSetDevice(c->device));
+    LLG_CHECK(v->dir <= 1, LL_GPU_EINVAL, "dir must be LL_GPU_DIR_OUT or LL_GPU_DIR_IN");
+    auto& d = g->dir[v->dir];
+    LLG_CHECK(v->level == d.vt.size(), LL_GPU_ESTATE,
+              "levels must be pushed in order (expected level " + std::to_string(d.vt.size()) + ")");
+    LLG_CHECK(v->page_table == nullptr, LL_GPU_ENOTSUP, "COW page-table upload is not supported");
+    LLG_CHECK(v->num_nodes == g->V && v->num_vt_entries == g->V + 1, LL_GPU_EINVAL,
+              "vertex table must have max_nodes+1 entries");
+    LLG_CHECK(v->flags == g->flags, LL_GPU_EINVAL, "level flags differ from the graph flags");
+    LLG_CHECK(v->vt != nullptr && (v->num_et_words == 0 || v->et != nullptr), LL_GPU_EINVAL,
+              "NULL level arrays");
+    LLG_CHECK(v->level < llgpu::kLevelMask - 2, LL_GPU_ERANGE, "level IDs exhausted");
+    hipEvent_t e0, e1;
+    LLG_HIP(hipEventCreate(&e0));
+    LLG_HIP(hipEventCreate(&e1));
+    LLG_HIP(hipEventRecord(e0, c->stream));
+    DevBuf vt, et;
+    const size_t vt_bytes = size_t(v->num_vt_entries) * sizeof(llgpu::VtEntry);
+    const size_t et_bytes = size_t(v->num_et_words) * sizeof(uint64_t);
+    vt.alloc(vt_bytes, c->stream);
+    et.alloc(std::max<size_t>(et_bytes, 8), c->stream);
+    upload(c, vt.p, v->vt, vt_bytes);
+    if (et_bytes) upload(c, et.p, v->et, et_bytes);
+    d.vt.push_back(std::move(vt));
+    d.et.push_back(std::move(et));
+    d.et_words.push_back(v->num_et_words);
+    std::vector<const uint64_t*> table;
+    for (auto& b : d.et) table.push_back(b.as<const uint64_t>());
+    DevBuf t;
+    t.alloc(table.size() * sizeof(void*), c->stream);
+    LLG_HIP(hipMemcpyAsync(t.p, table.data(), table.size() * sizeof(void*),
+                           hipMemcpyHostToDevice, c->stream));
+    d.et_table = std::move(t);
+    DevBuf wt;
+    wt.alloc(d.et_words.size() * sizeof(uint64_t), c->stream);
+    LLG_HIP(hipMemcpyAsync(wt.p, d.et_words.data(), d.et_words.size() * sizeof(uint64_t),
+                           hipMemcpyHostToDevice, c->stream));
+    d.et_words_dev = std::move(wt);
+    LLG_HIP(hipEventRecord(e1, c->stream));
+    LLG_HIP(hipEventSynchronize(e1));  // host arrays and the table are released on return
+    float ms = 0;
+    LLG_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    g->last_upload_ms = ms;
+    g->upload_bytes += vt_bytes + et_bytes;
+    llgpu::drop_snapshots(g);
+  });
+}
+
+int ll_gpu_graph_patch_deletions(ll_gpu_graph* g, uint32_t dir, const uint64_t* ids,
+                                 const uint32_t* maxlev, uint64_t n) {
+  if (!g) return LL_GPU_EINVAL;
+  ll_gpu_ctx* c = g->ctx;
+  return guarded(c, [&]() {
+    LLG_CHECK(dir <= 1, LL_GPU_EINVAL, "dir must be LL_GPU_DIR_OUT or LL_GPU_DIR_IN");
+    if (n == 0) return;
+    LLG_CHECK(ids && maxlev, LL_GPU_EINVAL, "NULL patch arrays");
+    LLG_CHECK(g->flags & LL_GPU_HAS_DELETIONS, LL_GPU_ENOTSUP,
+              "graph was created without deletions");
+    auto& d = g->dir[dir];
+    for (uint64_t i = 0; i < n; i++) {
+      uint32_t l = llgpu::edge_level(ids[i]);
+      LLG_CHECK(l < d.et.size() && llgpu::edge_index(ids[i]) < d.et_words[l], LL_GPU_EINVAL,
+                "patch edge ID outside the pushed levels");
+      LLG_CHECK(maxlev[i] <= llgpu::kLevelMask, LL_GPU_EINVAL, "patch level out of range");
+    }
+    LLG_HIP(hipSetDevice(c->device));
+    DevBuf di, dm;
+    di.alloc(n * 8, c->stream);
+    dm.alloc(n * 4, c->stream);
+    upload(c, di.p, ids, n * 8);
+    upload(c, dm.p, maxlev, n * 4);
+    k_apply_patches<<<llgpu::grid_for(n, 256), 256, 0, c->stream>>>(
+        d.et_table.as<const uint64_t* const>(), di.as<uint64_t>(), dm.as<uint32_t>(), n);
+    LLG_HIP(hipGetLastError());
+    LLG_HIP(hipStreamSynchronize(c->stream));
+    g->upload_bytes += n * 12;
+    llgpu::drop_snapshots(g);
+  });
+}
+
+int ll_gpu_graph_num_levels(const ll_gpu_graph* g, uint32_t dir) {
+  if (!g || dir > 1) return LL_GPU_EINVAL;
+  return int(g->dir[dir].vt.size());
+}
+
+uint64_t ll_gpu_graph_max_nodes(const ll_gpu_graph* g) { return g ? g->V : 0; }
+
+int ll_gpu_graph_upload_stats(const ll_gpu_graph* g, uint64_t* bytes, double* last_ms) {
+  if (!g) return LL_GPU_EINVAL;
+  if (bytes) *bytes = g->upload_bytes;
+  if (last_ms) *last_ms = g->last_upload_ms;
+  return LL_GPU_OK;
+}
+
+void ll_gpu_graph_drop_snapshots(ll_gpu_graph* g) {
+  if (!g) return;
+  (void)hipSetDevice(g->ctx->device);
+  llgpu::drop_snapshots(g);
+}
+
+}  // extern "C"

@@ -1,0 +1,194 @@
+// Internal structures shared by the HIP translation units of libll_gpu.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "ll_gpu.h"
+
+namespace llgpu {
+
+// LLAMA edge-ID / edge-table word encoding (ll_mlcsr_helpers.h:55-107).
+constexpr int kBitsIndex = 38;
+constexpr uint64_t kIndexMask = (uint64_t(1) << kBitsIndex) - 1;
+constexpr uint32_t kLevelMask = (1u << 26) - 1;
+
+struct VtEntry {
+  int64_t start;
+  uint32_t len;
+  uint32_t degree;
+};
+static_assert(sizeof(VtEntry) == 16, "VT entry must be 16 bytes");
+
+__host__ __device__ inline uint32_t edge_level(uint64_t e) { return uint32_t(e >> kBitsIndex); }
+__host__ __device__ inline uint64_t edge_index(uint64_t e) { return e & kIndexMask; }
+__host__ __device__ inline uint32_t word_maxlev(uint64_t w) {
+  return uint32_t(w >> kBitsIndex) & kLevelMask;
+}
+
+// Error carried out of helpers; converted to a status at the C boundary.
+struct Error {
+  int code;
+  std::string msg;
+};
+
+#define LLG_HIP(expr)                                                                   \
+  do {                                                                                  \
+    hipError_t e_ = (expr);                                                             \
+    if (e_ != hipSuccess)                                                               \
+      throw ::llgpu::Error{LL_GPU_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_)}; \
+  } while (0)
+
+#define LLG_CHECK(cond, code, msg)                       \
+  do {                                                   \
+    if (!(cond)) throw ::llgpu::Error{(code), (msg)};    \
+  } while (0)
+
+// Stream-ordered device buffer (hipMallocAsync / hipFreeAsync on the context stream).
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  hipStream_t s = nullptr;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes), s(o.s) { o.p = nullptr; o.bytes = 0; }
+  DevBuf& operator=(DevBuf&& o) noexcept {
+    if (this != &o) { release(); p = o.p; bytes = o.bytes; s = o.s; o.p = nullptr; o.bytes = 0; }
+    return *this;
+  }
+  ~DevBuf() { release(); }
+  void alloc(size_t n, hipStream_t stream) {
+    release();
+    s = stream;
+    bytes = n;
+    if (n == 0) return;
+    hipError_t e = hipMallocAsync(&p, n, stream);
+    if (e != hipSuccess) {
+      p = nullptr;
+      bytes = 0;
+      (void)hipGetLastError();
+      throw Error{LL_GPU_ENOMEM, "device allocation of " + std::to_string(n) + " bytes failed"};
+    }
+  }
+  void release() {
+    if (p) (void)hipFreeAsync(p, s);
+    p = nullptr;
+    bytes = 0;
+  }
+  template <typename T> T* as() const { return static_cast<T*>(p); }
+};
+
+// Per-kernel device time accumulated from HIP events (profiling mode).
+struct KernelTimer {
+  struct Rec { std::string name; hipEvent_t a, b; };
+  bool on = false;
+  std::vector<Rec> pending;
+  std::vector<hipEvent_t> pool;
+  std::map<std::string, std::pair<uint64_t, double>> acc;
+};
+
+}  // namespace llgpu
+
+struct ll_gpu_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  int num_cus = 0;
+  llgpu::KernelTimer timer;
+  void* pinned = nullptr;      // staging for uploads
+  size_t pinned_bytes = 0;
+  hipEvent_t pinned_ev[2] = {nullptr, nullptr};
+  std::map<std::string, int> tuned;  // autotuner decisions
+};
+
+namespace llgpu {
+
+// Compact CSR of one direction at one snapshot level (iterator order).
+struct Csr {
+  bool valid = false;
+  int level = -1;
+  uint64_t V = 0, E = 0, stored = 0, segments = 0;
+  DevBuf ofs;   // uint64_t[V+1]
+  DevBuf adj;   // uint32_t[E]
+  double ms = 0;
+};
+
+// Row blocks of a CSR for the load-balanced row-reduction kernels.
+struct RowItem {
+  uint32_t row_begin, row_end, piece, npieces;
+};
+struct RowBlocks {
+  bool valid = false;
+  uint32_t cap = 0;
+  uint64_t n = 0;
+  DevBuf items;  // RowItem[n]
+};
+
+struct DirMirror {
+  std::vector<DevBuf> vt;   // VtEntry[V+1] per level
+  std::vector<DevBuf> et;   // uint64_t[words] per level
+  std::vector<uint64_t> et_words;
+  DevBuf et_table;          // const uint64_t*[levels] on device
+  DevBuf et_words_dev;      // uint64_t[levels]: words per level (bounds of the walk)
+};
+
+}  // namespace llgpu
+
+struct ll_gpu_graph {
+  ll_gpu_ctx* ctx = nullptr;
+  uint64_t V = 0;
+  uint32_t flags = 0;
+  llgpu::DirMirror dir[2];
+  uint64_t upload_bytes = 0;
+  double last_upload_ms = 0;
+  // snapshot caches
+  int deg_level[2] = {-1, -1};
+  llgpu::DevBuf deg[2];       // uint32_t[V]
+  llgpu::Csr csr[2];
+  llgpu::RowBlocks rb[2];
+  // derived structures (built on demand per snapshot level)
+  int sym_level = -1;
+  llgpu::DevBuf sym_ofs, sym_adj;  // degree-oriented DAG for triangle counting
+  uint64_t sym_edges = 0;
+  int w_level = -1;
+  llgpu::DevBuf in2out;       // uint32/64 map in-edge position -> out-edge position
+};
+
+namespace llgpu {
+
+// Launch helpers that record per-kernel device time when profiling is on.
+struct TimedLaunch {
+  ll_gpu_ctx* ctx;
+  const char* name;
+  hipEvent_t a = nullptr, b = nullptr;
+  TimedLaunch(ll_gpu_ctx* c, const char* n);
+  ~TimedLaunch();
+};
+
+int resolve_level(const ll_gpu_graph* g, int level);
+const uint32_t* ensure_degrees(ll_gpu_graph* g, uint32_t dir, int level);
+const Csr& ensure_csr(ll_gpu_graph* g, uint32_t dir, int level);
+const RowBlocks& ensure_rowblocks(ll_gpu_graph* g, uint32_t dir, int level, uint32_t cap);
+void drop_snapshots(ll_gpu_graph* g);
+
+// Device-wide exclusive scan of n values into out (n+1 entries, total last).
+void scan_u32_to_u64(const uint32_t* in, uint64_t* out, uint64_t n, hipStream_t s);
+void scan_u64(const uint64_t* in, uint64_t* out, uint64_t n, hipStream_t s);
+void sort_pairs_u64_u32(const uint64_t* keys_in, uint64_t* keys_out, const uint32_t* vals_in,
+                        uint32_t* vals_out, uint64_t n, int end_bit, hipStream_t s);
+void sort_keys_u64(const uint64_t* keys_in, uint64_t* keys_out, uint64_t n, int end_bit,
+                   hipStream_t s);
+uint64_t unique_u64(const uint64_t* in, uint64_t* out, uint64_t n, hipStream_t s);
+
+inline unsigned grid_for(uint64_t n, unsigned block) {
+  uint64_t g = (n + block - 1) / block;
+  return unsigned(g == 0 ? 1 : g);
+}
+
+}  // namespace llgpu

@@ -1,0 +1,254 @@
+// Pull PageRank over the snapshot in-CSR (task pagerank / pagerank_double,
+// benchmark/benchmarks/pagerank.h:98-182).
+//
+// Per iteration one launch of k_pr_pull reduces, for every vertex t,
+//   S[t] = sum_{w in in(t)} contrib[w],   contrib[w] = rank[w] / outdeg(w)
+// and writes rank'[t] = (1-d)/N + d*S[t] and contrib'[t] = rank'[t]/outdeg(t)
+// (0 for outdeg 0; such a vertex is never an in-neighbour).  The reference
+// divides per edge (pagerank.h:147); dividing once per vertex moves the
+// division but keeps the same terms.  Sums are FP32 with Neumaier
+// compensation; rows split over several workgroups are combined in piece order
+// by k_pr_fixup.  Bytes per iteration: 4 B in-neighbour ID + 4 B contribution
+// gather per edge, 8 B offset + 4 B out-degree + 8 B rank/contrib writes per
+// vertex.
+#include "ll_gpu_internal.h"
+#include "rowblocks.h"
+
+namespace llgpu {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+struct PrArgs {
+  const RowItem* items;
+  const uint64_t* ofs;
+  const uint32_t* adj;
+  const float* contrib_in;
+  const float* rank_in;   // only read when diff != nullptr
+  const uint32_t* outdeg;
+  float* rank_out;
+  float* contrib_out;
+  Neu* part;              // per item partials of split rows
+  double* diff;           // L1 change accumulator (last iteration only)
+  float base, d;
+};
+
+__device__ inline float finalize(const PrArgs& a, uint32_t r, float S) {
+  float v = a.base + a.d * S;
+  a.rank_out[r] = v;
+  uint32_t od = a.outdeg[r];
+  a.contrib_out[r] = od ? v / float(od) : 0.f;
+  return a.diff ? fabsf(v - a.rank_in[r]) : 0.f;
+}
+
+__device__ inline void block_add_diff(const PrArgs& a, float x) {
+  if (!a.diff) return;
+  __shared__ float red[kBlock / 64];
+  for (int o = 32; o > 0; o >>= 1) x += __shfl_down(x, o, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = x;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float t = 0;
+    for (int i = 0; i < kBlock / 64; i++) t += red[i];
+    atomicAdd(a.diff, double(t));
+  }
+}
+
+template <uint32_t CAP>
+__global__ __launch_bounds__(kBlock) void k_pr_pull(PrArgs a) {
+  __shared__ float vals[CAP + CAP / 2];
+  __shared__ uint64_t rofs[kRowsPerItem + 1];
+  __shared__ Neu red[kBlock / 64];
+  const RowItem it = a.items[blockIdx.x];
+  float dsum = 0.f;
+  if (it.npieces == 0) {
+    const uint32_t nrows = it.row_end - it.row_begin;
+    for (uint32_t i = threadIdx.x; i <= nrows; i += kBlock) rofs[i] = a.ofs[it.row_begin + i];
+    __syncthreads();
+    const uint64_t e0 = rofs[0];
+    const uint32_t ne = uint32_t(rofs[nrows] - e0);
+    // coalesced neighbour IDs first, then all gathers in flight together
+    constexpr uint32_t kPer = (CAP + CAP / 2) / kBlock;
+    uint32_t idx[kPer];
+    float v[kPer];
+#pragma unroll
+    for (uint32_t k = 0; k < kPer; k++) {
+      uint32_t i = threadIdx.x + k * kBlock;
+      idx[k] = i < ne ? a.adj[e0 + i] : 0u;
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < kPer; k++) {
+      uint32_t i = threadIdx.x + k * kBlock;
+      v[k] = i < ne ? a.contrib_in[idx[k]] : 0.f;
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < kPer; k++) {
+      uint32_t i = threadIdx.x + k * kBlock;
+      if (i < ne) vals[i] = v[k];
+    }
+    __syncthreads();
+    if (threadIdx.x < nrows) {
+      const uint32_t lo = uint32_t(rofs[threadIdx.x] - e0), hi = uint32_t(rofs[threadIdx.x + 1] - e0);
+      Neu s = Neu::zero();
+      for (uint32_t j = lo; j < hi; j++) s.add(vals[j]);
+      dsum = finalize(a, it.row_begin + threadIdx.x, s.value());
+    }
+  } else {
+    const uint32_t r = it.row_begin;
+    const uint64_t rb = a.ofs[r], re = a.ofs[r + 1];
+    const uint64_t lo = rb + uint64_t(it.piece) * kPiece;
+    const uint64_t hi = min(re, lo + kPiece);
+    Neu s = Neu::zero();
+    for (uint64_t e = lo + threadIdx.x; e < hi; e += kBlock) s.add(a.contrib_in[a.adj[e]]);
+    s = wave_reduce(s);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      Neu t = red[0];
+      for (int i = 1; i < kBlock / 64; i++) t.merge(red[i]);
+      if (it.npieces == 1) dsum = finalize(a, r, t.value());
+      else a.part[blockIdx.x] = t;
+    }
+  }
+  block_add_diff(a, dsum);
+}
+
+// Rows split into several pieces: combine the partials in piece order.
+__global__ void k_pr_fixup(PrArgs a, uint64_t nitems) {
+  uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  float dsum = 0.f;
+  if (i < nitems) {
+    const RowItem it = a.items[i];
+    if (it.npieces > 1 && it.piece == 0) {
+      Neu t = a.part[i];
+      for (uint32_t p = 1; p < it.npieces; p++) t.merge(a.part[i + p]);
+      dsum = finalize(a, it.row_begin, t.value());
+    }
+  }
+  block_add_diff(a, dsum);
+}
+
+__global__ void k_pr_init(const uint32_t* outdeg, float* rank, float* contrib, uint64_t V,
+                          float r0) {
+  uint64_t v = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (v >= V) return;
+  rank[v] = r0;
+  uint32_t od = outdeg[v];
+  contrib[v] = od ? r0 / float(od) : 0.f;
+}
+
+__global__ void k_widen(const float* in, double* out, uint64_t n) {
+  uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = double(in[i]);
+}
+
+__global__ void k_zero_f64(double* p) { *p = 0.0; }
+
+}  // namespace
+
+void launch_pr_pull(uint32_t cap, unsigned nitems, const PrArgs& a, hipStream_t s) {
+  switch (cap) {
+    case 512: k_pr_pull<512><<<nitems, kBlock, 0, s>>>(a); break;
+    case 1024: k_pr_pull<1024><<<nitems, kBlock, 0, s>>>(a); break;
+    case 2048: k_pr_pull<2048><<<nitems, kBlock, 0, s>>>(a); break;
+    case 4096: k_pr_pull<4096><<<nitems, kBlock, 0, s>>>(a); break;
+    default: throw Error{LL_GPU_EINVAL, "unsupported PageRank row-block capacity"};
+  }
+}
+
+// Run `iters` pull iterations; leaves FP32 ranks in rank_final (device).
+void pagerank_device(ll_gpu_graph* g, int level, int iters, double damping, uint32_t cap,
+                     DevBuf& rank_final, double* last_diff) {
+  ll_gpu_ctx* ctx = g->ctx;
+  hipStream_t s = ctx->stream;
+  const uint64_t V = g->V;
+  const Csr& in = ensure_csr(g, LL_GPU_DIR_IN, level);
+  const uint32_t* outdeg = ensure_degrees(g, LL_GPU_DIR_OUT, level);
+  const RowBlocks& rb = ensure_rowblocks(g, LL_GPU_DIR_IN, level, cap);
+  DevBuf rank[2], contrib[2], part, diff;
+  for (int i = 0; i < 2; i++) {
+    rank[i].alloc(V * 4, s);
+    contrib[i].alloc(V * 4, s);
+  }
+  part.alloc(std::max<uint64_t>(rb.n, 1) * sizeof(Neu), s);
+  diff.alloc(8, s);
+  const float N = float(V);
+  k_pr_init<<<grid_for(V, 256), 256, 0, s>>>(outdeg, rank[0].as<float>(), contrib[0].as<float>(),
+                                             V, float(1.0 / double(V)));
+  k_zero_f64<<<1, 1, 0, s>>>(diff.as<double>());
+  LLG_HIP(hipGetLastError());
+  int cur = 0;
+  for (int it = 0; it < iters; it++) {
+    PrArgs a;
+    a.items = rb.items.as<RowItem>();
+    a.ofs = in.ofs.as<uint64_t>();
+    a.adj = in.adj.as<uint32_t>();
+    a.contrib_in = contrib[cur].as<float>();
+    a.rank_in = rank[cur].as<float>();
+    a.outdeg = outdeg;
+    a.rank_out = rank[cur ^ 1].as<float>();
+    a.contrib_out = contrib[cur ^ 1].as<float>();
+    a.part = part.as<Neu>();
+    a.diff = (it == iters - 1 && last_diff) ? diff.as<double>() : nullptr;
+    a.base = float((1.0 - damping) / double(N));
+    a.d = float(damping);
+    {
+      TimedLaunch t(ctx, "k_pr_pull");
+      launch_pr_pull(cap, unsigned(rb.n), a, s);
+    }
+    {
+      TimedLaunch t(ctx, "k_pr_fixup");
+      k_pr_fixup<<<grid_for(rb.n, 256), 256, 0, s>>>(a, rb.n);
+    }
+    LLG_HIP(hipGetLastError());
+    cur ^= 1;
+  }
+  rank_final = std::move(rank[cur]);
+  if (last_diff) {
+    double h = 0;
+    LLG_HIP(hipMemcpyAsync(&h, diff.p, 8, hipMemcpyDeviceToHost, s));
+    LLG_HIP(hipStreamSynchronize(s));
+    *last_diff = iters > 0 ? h : 0.0;
+  }
+}
+
+void widen_to_host(ll_gpu_graph* g, const float* dev, double* host, uint64_t n) {
+  hipStream_t s = g->ctx->stream;
+  DevBuf w;
+  w.alloc(n * 8, s);
+  k_widen<<<grid_for(n, 256), 256, 0, s>>>(dev, w.as<double>(), n);
+  LLG_HIP(hipGetLastError());
+  LLG_HIP(hipMemcpyAsync(host, w.p, n * 8, hipMemcpyDeviceToHost, s));
+  LLG_HIP(hipStreamSynchronize(s));
+}
+
+}  // namespace llgpu
+
+extern "C" {
+
+int ll_gpu_pagerank_ex(ll_gpu_graph* g, int level, int iters, double damping, uint32_t cap,
+                       double* ranks_out, double* last_diff) {
+  if (!g) return LL_GPU_EINVAL;
+  try {
+    LLG_HIP(hipSetDevice(g->ctx->device));
+    LLG_CHECK(iters >= 1, LL_GPU_EINVAL, "iters must be >= 1 (the reference runs do/while)");
+    LLG_CHECK(damping >= 0 && damping <= 1, LL_GPU_EINVAL, "damping must be in [0, 1]");
+    if (cap == 0) cap = 1024;
+    llgpu::DevBuf r;
+    llgpu::pagerank_device(g, level, iters, damping, cap, r, last_diff);
+    if (ranks_out) llgpu::widen_to_host(g, r.as<float>(), ranks_out, g->V);
+    else LLG_HIP(hipStreamSynchronize(g->ctx->stream));
+    return LL_GPU_OK;
+  } catch (const llgpu::Error& e) {
+    g->ctx->err = e.msg;
+    return e.code;
+  }
+}
+
+int ll_gpu_pagerank(ll_gpu_graph* g, int level, int iters, double damping, double* ranks_out,
+                    double* last_diff) {
+  return ll_gpu_pagerank_ex(g, level, iters, damping, 0, ranks_out, last_diff);
+}
+
+}  // extern "C"

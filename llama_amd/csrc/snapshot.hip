@@ -1,0 +1,418 @@
+// Snapshot materialisation: the multi-level iterator, run once per snapshot.
+//
+// For snapshot level L the adjacency of v is the chain of segments reached
+// from VT_L[v] by following continuation records (iter_begin / iter_next /
+// iter_descend, ll_mlcsr_sp.h:1994-2196), minus the words whose max-visible
+// level is <= L (is_edge_deleted, :1290-1310).  Concatenated over v in ID
+// order, chain order and stored order, the stored words form one "virtual"
+// word sequence; the live words of that sequence, in order, are exactly the
+// compact CSR adjacency in iterator order.  So:
+//   1. thread per vertex: walk the chain through the continuation records only
+//      (one dependent 16-byte load per segment), emit the segment count and
+//      stored-word count;                                    k_seg_count
+//   2. scans give each segment its slot and virtual word offset;
+//   3. thread per vertex: write segment records (edge ID of the run start and
+//      virtual offset);                                      k_seg_fill
+//   4. block per 2048 virtual words: find each word's segment by binary search
+//      in an LDS copy of the block's segment range, read the word (coalesced
+//      within segments), test liveness, count;               k_live_count
+//   5. scan of the block counts; second pass writes the live payloads with a
+//      block-wide ordered compaction (wave ballots).          k_compact
+// Without deletions every stored word is live and step 4 is skipped.
+// The row offsets are the scan of the precomputed VT_L degrees; the live-word
+// total is checked against them.
+#include "ll_gpu_internal.h"
+
+namespace llgpu {
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kWordsPerBlock = 2048;
+constexpr int kWaves = kBlock / 64;
+
+__global__ void k_degrees(const VtEntry* vt, uint32_t* deg, uint64_t V) {
+  uint64_t v = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (v < V) deg[v] = vt[v].degree;
+}
+
+// Chain walk shared by the two per-vertex passes.  Levels strictly decrease
+// along a chain and every run (plus its continuation record on levels > 0)
+// lies inside its level's edge table; a violation flags the mirror as corrupt
+// and stops the walk instead of reading out of bounds.
+template <typename F>
+__device__ inline void walk_chain(const VtEntry* vt, const uint64_t* const* et,
+                                  const uint64_t* et_words, uint32_t level, uint64_t v,
+                                  uint32_t* bad, F&& f) {
+  VtEntry e = vt[v];
+  int64_t start = e.start;
+  uint32_t len = e.len;
+  uint32_t kprev = level + 1;
+  while (start != -1 && len > 0) {
+    uint32_t k = edge_level(uint64_t(start));
+    uint64_t idx = edge_index(uint64_t(start));
+    if (k >= kprev || idx + len + (k > 0 ? 2 : 0) > et_words[k]) {
+      atomicOr(bad, 1u);
+      return;
+    }
+    f(uint64_t(start), len);
+    if (k == 0) return;
+    const uint64_t* c = et[k] + idx + len;
+    start = int64_t(c[0]);
+    len = uint32_t(c[1]);
+    kprev = k;
+  }
+}
+
+__global__ void k_seg_count(const VtEntry* vt, const uint64_t* const* et,
+                            const uint64_t* et_words, uint32_t level, uint64_t V,
+                            uint32_t* nseg, uint64_t* nwords, uint32_t* bad) {
+  uint64_t v = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (v >= V) return;
+  uint32_t cnt = 0;
+  uint64_t words = 0;
+  walk_chain(vt, et, et_words, level, v, bad, [&](uint64_t, uint32_t len) {
+    cnt++;
+    words += len;
+  });
+  nseg[v] = cnt;
+  nwords[v] = words;
+}
+
+__global__ void k_seg_fill(const VtEntry* vt, const uint64_t* const* et,
+                           const uint64_t* et_words, uint32_t level, uint64_t V,
+                           const uint64_t* seg_base, const uint64_t* word_base,
+                           uint64_t* seg_addr, uint64_t* seg_vofs, uint32_t* bad) {
+  uint64_t v = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (v >= V) return;
+  uint64_t s = seg_base[v], w = word_base[v];
+  walk_chain(vt, et, et_words, level, v, bad, [&](uint64_t start, uint32_t len) {
+    seg_addr[s] = start;
+    seg_vofs[s] = w;
+    s++;
+    w += len;
+  });
+}
+
+__global__ void k_seg_total(uint64_t* seg_vofs, uint64_t nseg, uint64_t total) {
+  seg_vofs[nseg] = total;
+}
+
+// Largest s in [lo, hi) with vofs[s] <= x (vofs ascending, vofs[lo] <= x).
+__device__ inline uint64_t seg_search_global(const uint64_t* vofs, uint64_t lo, uint64_t hi,
+                                             uint64_t x) {
+  while (hi - lo > 1) {
+    uint64_t mid = lo + (hi - lo) / 2;
+    if (vofs[mid] <= x) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+struct BlockSegs {
+  uint64_t s_lo;
+  uint32_t n;  // segments overlapping the block
+};
+
+// Load the block's segment range into LDS; returns it.
+__device__ inline BlockSegs load_block_segs(const uint64_t* seg_vofs, const uint64_t* seg_addr,
+                                            uint64_t nseg, uint64_t x0, uint64_t x1,
+                                            uint64_t* l_vofs, uint64_t* l_addr) {
+  __shared__ uint64_t sh_lo, sh_hi;
+  if (threadIdx.x == 0) {
+    sh_lo = seg_search_global(seg_vofs, 0, nseg, x0);
+    sh_hi = seg_search_global(seg_vofs, 0, nseg, x1 - 1);
+  }
+  __syncthreads();
+  uint64_t lo = sh_lo;
+  uint32_t n = uint32_t(sh_hi - lo + 1);
+  for (uint32_t i = threadIdx.x; i <= n; i += blockDim.x) {
+    l_vofs[i] = seg_vofs[lo + i];
+    if (i < n) l_addr[i] = seg_addr[lo + i];
+  }
+  __syncthreads();
+  return BlockSegs{lo, n};
+}
+
+__device__ inline uint32_t seg_search_lds(const uint64_t* l_vofs, uint32_t n, uint64_t x) {
+  uint32_t lo = 0, hi = n;
+  while (hi - lo > 1) {
+    uint32_t mid = (lo + hi) >> 1;
+    if (l_vofs[mid] <= x) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+__device__ inline uint64_t load_word(const uint64_t* const* et, const uint64_t* l_vofs,
+                                     const uint64_t* l_addr, uint32_t n, uint64_t x) {
+  uint32_t s = seg_search_lds(l_vofs, n, x);
+  uint64_t a = l_addr[s];
+  return et[edge_level(a)][edge_index(a) + (x - l_vofs[s])];
+}
+
+__global__ __launch_bounds__(kBlock) void k_live_count(const uint64_t* const* et,
+                                                       const uint64_t* seg_vofs,
+                                                       const uint64_t* seg_addr, uint64_t nseg,
+                                                       uint64_t total, uint32_t level,
+                                                       uint32_t* blk_cnt) {
+  __shared__ uint64_t l_vofs[kWordsPerBlock + 1];
+  __shared__ uint64_t l_addr[kWordsPerBlock];
+  __shared__ uint32_t wsum[kWaves];
+  const uint64_t x0 = uint64_t(blockIdx.x) * kWordsPerBlock;
+  const uint64_t x1 = min(total, x0 + kWordsPerBlock);
+  BlockSegs bs = load_block_segs(seg_vofs, seg_addr, nseg, x0, x1, l_vofs, l_addr);
+  uint32_t cnt = 0;
+  for (uint64_t x = x0 + threadIdx.x; x < x1; x += kBlock) {
+    uint64_t w = load_word(et, l_vofs, l_addr, bs.n, x);
+    cnt += word_maxlev(w) > level ? 1u : 0u;
+  }
+  for (int o = 32; o > 0; o >>= 1) cnt += __shfl_down(cnt, o, 64);
+  if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = cnt;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t t = 0;
+    for (int i = 0; i < kWaves; i++) t += wsum[i];
+    blk_cnt[blockIdx.x] = t;
+  }
+}
+
+template <bool kDeletions>
+__global__ __launch_bounds__(kBlock) void k_compact(const uint64_t* const* et,
+                                                    const uint64_t* seg_vofs,
+                                                    const uint64_t* seg_addr, uint64_t nseg,
+                                                    uint64_t total, uint32_t level,
+                                                    const uint64_t* blk_ofs, uint32_t* adj) {
+  __shared__ uint64_t l_vofs[kWordsPerBlock + 1];
+  __shared__ uint64_t l_addr[kWordsPerBlock];
+  __shared__ uint32_t wcnt[kWaves];
+  const uint64_t x0 = uint64_t(blockIdx.x) * kWordsPerBlock;
+  const uint64_t x1 = min(total, x0 + kWordsPerBlock);
+  BlockSegs bs = load_block_segs(seg_vofs, seg_addr, nseg, x0, x1, l_vofs, l_addr);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  uint64_t out = kDeletions ? blk_ofs[blockIdx.x] : x0;
+  // Rounds of 256 consecutive words keep the output in virtual order.
+  for (uint64_t base = x0; base < x1; base += kBlock) {
+    uint64_t x = base + threadIdx.x;
+    bool valid = x < x1;
+    uint64_t w = valid ? load_word(et, l_vofs, l_addr, bs.n, x) : 0;
+    bool live = valid && (!kDeletions || word_maxlev(w) > level);
+    uint64_t mask = __ballot(live);
+    uint32_t rank = __popcll(mask & ((uint64_t(1) << lane) - 1));
+    if (lane == 0) wcnt[wave] = __popcll(mask);
+    __syncthreads();
+    uint32_t before = 0, round = 0;
+    for (int i = 0; i < kWaves; i++) {
+      if (i < wave) before += wcnt[i];
+      round += wcnt[i];
+    }
+    if (live) adj[out + before + rank] = uint32_t(w & kIndexMask);
+    out += round;
+    __syncthreads();
+  }
+}
+
+}  // namespace
+
+const uint32_t* ensure_degrees(ll_gpu_graph* g, uint32_t dir, int level) {
+  level = resolve_level(g, level);
+  if (g->deg_level[dir] == level) return g->deg[dir].as<uint32_t>();
+  hipStream_t s = g->ctx->stream;
+  g->deg[dir].alloc(g->V * sizeof(uint32_t), s);
+  {
+    TimedLaunch t(g->ctx, "k_degrees");
+    k_degrees<<<grid_for(g->V, 256), 256, 0, s>>>(g->dir[dir].vt[level].as<VtEntry>(),
+                                                   g->deg[dir].as<uint32_t>(), g->V);
+  }
+  LLG_HIP(hipGetLastError());
+  g->deg_level[dir] = level;
+  return g->deg[dir].as<uint32_t>();
+}
+
+const Csr& ensure_csr(ll_gpu_graph* g, uint32_t dir, int level) {
+  level = resolve_level(g, level);
+  Csr& c = g->csr[dir];
+  if (c.valid && c.level == level) return c;
+  c = Csr();
+  ll_gpu_ctx* ctx = g->ctx;
+  hipStream_t s = ctx->stream;
+  const uint64_t V = g->V;
+  const VtEntry* vt = g->dir[dir].vt[level].as<VtEntry>();
+  const uint64_t* const* et = g->dir[dir].et_table.as<const uint64_t* const>();
+  hipEvent_t e0, e1;
+  LLG_HIP(hipEventCreate(&e0));
+  LLG_HIP(hipEventCreate(&e1));
+  LLG_HIP(hipEventRecord(e0, s));
+
+  // 1-3: segments
+  const uint64_t* et_words = g->dir[dir].et_words_dev.as<const uint64_t>();
+  DevBuf nseg, nwords, seg_base, word_base, bad;
+  nseg.alloc(V * 4, s);
+  nwords.alloc(V * 8, s);
+  bad.alloc(4, s);
+  LLG_HIP(hipMemsetAsync(bad.p, 0, 4, s));
+  {
+    TimedLaunch t(ctx, "k_seg_count");
+    k_seg_count<<<grid_for(V, 256), 256, 0, s>>>(vt, et, et_words, uint32_t(level), V,
+                                                 nseg.as<uint32_t>(), nwords.as<uint64_t>(),
+                                                 bad.as<uint32_t>());
+  }
+  LLG_HIP(hipGetLastError());
+  seg_base.alloc((V + 1) * 8, s);
+  word_base.alloc((V + 1) * 8, s);
+  scan_u32_to_u64(nseg.as<uint32_t>(), seg_base.as<uint64_t>(), V, s);
+  scan_u64(nwords.as<uint64_t>(), word_base.as<uint64_t>(), V, s);
+  uint64_t tot[2];
+  uint32_t hbad = 0;
+  LLG_HIP(hipMemcpyAsync(&tot[0], seg_base.as<uint64_t>() + V, 8, hipMemcpyDeviceToHost, s));
+  LLG_HIP(hipMemcpyAsync(&tot[1], word_base.as<uint64_t>() + V, 8, hipMemcpyDeviceToHost, s));
+  LLG_HIP(hipMemcpyAsync(&hbad, bad.p, 4, hipMemcpyDeviceToHost, s));
+  LLG_HIP(hipStreamSynchronize(s));
+  LLG_CHECK(hbad == 0, LL_GPU_ESTATE,
+            "corrupt mirror: a continuation chain leaves its edge table or does not descend");
+  const uint64_t S = tot[0], W = tot[1];
+  nseg.release();
+  nwords.release();
+  DevBuf seg_addr, seg_vofs;
+  seg_addr.alloc(std::max<uint64_t>(S, 1) * 8, s);
+  seg_vofs.alloc((S + 1) * 8, s);
+  {
+    TimedLaunch t(ctx, "k_seg_fill");
+    k_seg_fill<<<grid_for(V, 256), 256, 0, s>>>(vt, et, et_words, uint32_t(level), V,
+                                                seg_base.as<uint64_t>(), word_base.as<uint64_t>(),
+                                                seg_addr.as<uint64_t>(), seg_vofs.as<uint64_t>(),
+                                                bad.as<uint32_t>());
+  }
+  k_seg_total<<<1, 1, 0, s>>>(seg_vofs.as<uint64_t>(), S, W);
+  LLG_HIP(hipGetLastError());
+  seg_base.release();
+  word_base.release();
+
+  // Row offsets from the precomputed degrees.
+  const uint32_t* deg = ensure_degrees(g, dir, level);
+  c.ofs.alloc((V + 1) * 8, s);
+  scan_u32_to_u64(deg, c.ofs.as<uint64_t>(), V, s);
+  uint64_t E = 0;
+  LLG_HIP(hipMemcpyAsync(&E, c.ofs.as<uint64_t>() + V, 8, hipMemcpyDeviceToHost, s));
+  LLG_HIP(hipStreamSynchronize(s));
+
+  // 4-5: compaction of live words
+  const uint64_t nblk = (W + kWordsPerBlock - 1) / kWordsPerBlock;
+  const bool dels = (g->flags & LL_GPU_HAS_DELETIONS) != 0;
+  c.adj.alloc(std::max<uint64_t>(E, 1) * 4, s);
+  uint64_t live = W;
+  if (nblk > 0) {
+    DevBuf blk_cnt, blk_ofs;
+    if (dels) {
+      blk_cnt.alloc(nblk * 4, s);
+      blk_ofs.alloc((nblk + 1) * 8, s);
+      {
+        TimedLaunch t(ctx, "k_live_count");
+        k_live_count<<<unsigned(nblk), kBlock, 0, s>>>(et, seg_vofs.as<uint64_t>(),
+                                                        seg_addr.as<uint64_t>(), S, W,
+                                                        uint32_t(level), blk_cnt.as<uint32_t>());
+      }
+      LLG_HIP(hipGetLastError());
+      scan_u32_to_u64(blk_cnt.as<uint32_t>(), blk_ofs.as<uint64_t>(), nblk, s);
+      LLG_HIP(hipMemcpyAsync(&live, blk_ofs.as<uint64_t>() + nblk, 8, hipMemcpyDeviceToHost, s));
+      LLG_HIP(hipStreamSynchronize(s));
+    }
+    LLG_CHECK(live == E, LL_GPU_ESTATE,
+              "snapshot has " + std::to_string(live) + " live words but its degrees sum to " +
+                  std::to_string(E) + " (missing deletion patches?)");
+    {
+      TimedLaunch t(ctx, "k_compact");
+      if (dels)
+        k_compact<true><<<unsigned(nblk), kBlock, 0, s>>>(et, seg_vofs.as<uint64_t>(),
+                                                           seg_addr.as<uint64_t>(), S, W,
+                                                           uint32_t(level), blk_ofs.as<uint64_t>(),
+                                                           c.adj.as<uint32_t>());
+      else
+        k_compact<false><<<unsigned(nblk), kBlock, 0, s>>>(et, seg_vofs.as<uint64_t>(),
+                                                            seg_addr.as<uint64_t>(), S, W,
+                                                            uint32_t(level), nullptr,
+                                                            c.adj.as<uint32_t>());
+    }
+    LLG_HIP(hipGetLastError());
+  } else {
+    LLG_CHECK(E == 0, LL_GPU_ESTATE, "degrees are non-zero but no segments are reachable");
+  }
+  LLG_HIP(hipEventRecord(e1, s));
+  LLG_HIP(hipEventSynchronize(e1));
+  float ms = 0;
+  LLG_HIP(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  c.valid = true;
+  c.level = level;
+  c.V = V;
+  c.E = E;
+  c.stored = W;
+  c.segments = S;
+  c.ms = ms;
+  return c;
+}
+
+}  // namespace llgpu
+
+extern "C" {
+
+int ll_gpu_snapshot_stats(ll_gpu_graph* g, int level, ll_gpu_snapshot_info* info) {
+  if (!g || !info) return LL_GPU_EINVAL;
+  try {
+    LLG_HIP(hipSetDevice(g->ctx->device));
+    const llgpu::Csr& c = llgpu::ensure_csr(g, LL_GPU_DIR_OUT, level);
+    info->num_nodes = c.V;
+    info->live_edges = c.E;
+    info->stored_words = c.stored;
+    info->segments = c.segments;
+    // vertices with >= 1 segment contribute (segments - 1) continuations
+    uint64_t with_edges = 0;
+    {
+      const uint32_t* deg = llgpu::ensure_degrees(g, LL_GPU_DIR_OUT, level);
+      std::vector<uint32_t> h(g->V);
+      LLG_HIP(hipMemcpyAsync(h.data(), deg, g->V * 4, hipMemcpyDeviceToHost, g->ctx->stream));
+      LLG_HIP(hipStreamSynchronize(g->ctx->stream));
+      for (uint32_t d : h) with_edges += d > 0;
+    }
+    info->continuations = c.segments >= with_edges ? c.segments - with_edges : 0;
+    info->materialize_ms = c.ms;
+    return LL_GPU_OK;
+  } catch (const llgpu::Error& e) {
+    g->ctx->err = e.msg;
+    return e.code;
+  }
+}
+
+int ll_gpu_snapshot_adjacency(ll_gpu_graph* g, int level, uint32_t dir, uint64_t* offsets,
+                              uint32_t* neighbours) {
+  if (!g || dir > 1) return LL_GPU_EINVAL;
+  try {
+    LLG_HIP(hipSetDevice(g->ctx->device));
+    const llgpu::Csr& c = llgpu::ensure_csr(g, dir, level);
+    hipStream_t s = g->ctx->stream;
+    if (offsets)
+      LLG_HIP(hipMemcpyAsync(offsets, c.ofs.p, (c.V + 1) * 8, hipMemcpyDeviceToHost, s));
+    if (neighbours && c.E)
+      LLG_HIP(hipMemcpyAsync(neighbours, c.adj.p, c.E * 4, hipMemcpyDeviceToHost, s));
+    LLG_HIP(hipStreamSynchronize(s));
+    return LL_GPU_OK;
+  } catch (const llgpu::Error& e) {
+    g->ctx->err = e.msg;
+    return e.code;
+  }
+}
+
+int ll_gpu_snapshot_degrees(ll_gpu_graph* g, int level, uint32_t dir, uint32_t* degrees) {
+  if (!g || dir > 1 || !degrees) return LL_GPU_EINVAL;
+  try {
+    LLG_HIP(hipSetDevice(g->ctx->device));
+    const uint32_t* d = llgpu::ensure_degrees(g, dir, level);
+    LLG_HIP(hipMemcpyAsync(degrees, d, g->V * 4, hipMemcpyDeviceToHost, g->ctx->stream));
+    LLG_HIP(hipStreamSynchronize(g->ctx->stream));
+    return LL_GPU_OK;
+  } catch (const llgpu::Error& e) {
+    g->ctx->err = e.msg;
+    return e.code;
+  }
+}
+
+}  // extern "C"

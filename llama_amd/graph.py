@@ -1,0 +1,332 @@
+"""Host-side graph objects: the LLAMA-layout multi-level graph and its HBM mirror.
+
+``HostGraph`` wraps the native level builder (include/ll_host.h); it plays the
+role of ``ll_database`` / ``ll_mlcsr_ro_graph`` ownership of the frozen levels
+(llama/include/llama/ll_database.h:61-80, ll_mlcsr_graph.h:180-309).
+``DeviceGraph`` is the HBM mirror (include/ll_gpu.h) kept current by pushing
+each newly frozen level and replaying its deletion patches, the device side of
+the level-freeze event ll_mlcsr_ro_graph::checkpoint (ll_mlcsr_graph.h:1501-1831).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import weakref
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _native as N
+
+DIR_OUT, DIR_IN = 0, 1
+HAS_DELETIONS, HAS_CONT = 0x1, 0x2
+UNREACHED = 2**31 - 2  # INT_MAX-1, sssp.h:291
+
+
+def _u32p(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_uint32))
+
+
+@dataclass
+class LevelArrays:
+    """Borrowed numpy views of one frozen level of one direction."""
+
+    vt: np.ndarray      # structured (start int64, len uint32, degree uint32), V+1 rows
+    et: np.ndarray      # uint64 words
+    flags: int
+
+
+VT_DTYPE = np.dtype([("start", "<i8"), ("len", "<u4"), ("degree", "<u4")])
+
+
+class Edges:
+    """Owned native edge list (ll_host_edges)."""
+
+    def __init__(self, handle: int):
+        self._h = C.c_void_p(handle)
+
+    @classmethod
+    def from_arrays(cls, src, dst, weight=None) -> "Edges":
+        s = np.ascontiguousarray(src, dtype=np.uint32)
+        d = np.ascontiguousarray(dst, dtype=np.uint32)
+        if s.shape != d.shape:
+            raise ValueError("src and dst differ in length")
+        w = None if weight is None else np.ascontiguousarray(weight, dtype=np.float32)
+        h = C.c_void_p()
+        N.check_host(N.host().ll_host_edges_create(
+            _u32p(s), _u32p(d),
+            None if w is None else w.ctypes.data_as(C.POINTER(C.c_float)), s.size, C.byref(h)))
+        return cls(h.value)
+
+    @classmethod
+    def rmat(cls, scale: int, edge_factor: float = 16.0, a: float = 0.57, b: float = 0.19,
+             c: float = 0.19, seed: int = 1, perm_seed: int = 2) -> "Edges":
+        """Seeded Graph500-style R-MAT, permuted, no self-loops, deduplicated."""
+        h = C.c_void_p()
+        N.check_host(N.host().ll_host_gen_rmat(scale, edge_factor, a, b, c, seed, perm_seed,
+                                               C.byref(h)))
+        return cls(h.value)
+
+    def __len__(self) -> int:
+        return int(N.host().ll_host_edges_count(self._h))
+
+    def arrays(self) -> tuple[np.ndarray, np.ndarray]:
+        n = len(self)
+        if n == 0:
+            return np.zeros(0, np.uint32), np.zeros(0, np.uint32)
+        s = np.ctypeslib.as_array(N.host().ll_host_edges_src(self._h), shape=(n,))
+        d = np.ctypeslib.as_array(N.host().ll_host_edges_dst(self._h), shape=(n,))
+        return s.copy(), d.copy()
+
+    def write_xs1(self, path: str, num_nodes: int) -> None:
+        N.check_host(N.host().ll_host_xs1_write(path.encode(), self._h, num_nodes))
+
+    @classmethod
+    def read_xs1(cls, path: str) -> tuple["Edges", int]:
+        h, nn = C.c_void_p(), C.c_uint64()
+        N.check_host(N.host().ll_host_xs1_read(path.encode(), C.byref(h), C.byref(nn)))
+        return cls(h.value), int(nn.value)
+
+    def __del__(self):
+        if getattr(self, "_h", None) and self._h.value:
+            N.host().ll_host_edges_destroy(self._h)
+            self._h = C.c_void_p()
+
+
+class HostGraph:
+    """Multi-level snapshot CSR in LLAMA's read-only layout (host memory)."""
+
+    def __init__(self, num_nodes: int, deletions: bool = True):
+        h = C.c_void_p()
+        N.check_host(N.host().ll_host_graph_create(num_nodes, int(deletions), C.byref(h)))
+        self._h = h
+
+    # ---- building --------------------------------------------------------------
+    def add_level(self, ins_src=(), ins_dst=(), del_src=(), del_dst=()) -> int:
+        """Freeze one level (inserts in insertion order, deletions of live edges)."""
+        s = np.ascontiguousarray(ins_src, dtype=np.uint32)
+        d = np.ascontiguousarray(ins_dst, dtype=np.uint32)
+        ds = np.ascontiguousarray(del_src, dtype=np.uint32)
+        dd = np.ascontiguousarray(del_dst, dtype=np.uint32)
+        if s.shape != d.shape or ds.shape != dd.shape:
+            raise ValueError("edge arrays differ in length")
+        N.check_host(N.host().ll_host_graph_add_level(self._h, _u32p(s), _u32p(d), s.size,
+                                                      _u32p(ds), _u32p(dd), ds.size))
+        return self.num_levels - 1
+
+    def add_level_edges(self, ins: Edges | None, dels: Edges | None = None) -> int:
+        N.check_host(N.host().ll_host_graph_add_level_edges(
+            self._h, ins._h if ins is not None else None, dels._h if dels is not None else None))
+        return self.num_levels - 1
+
+    def add_random_level(self, ins_frac: float, del_frac: float, seed: int) -> int:
+        N.check_host(N.host().ll_host_graph_add_random_level(self._h, ins_frac, del_frac, seed))
+        return self.num_levels - 1
+
+    def last_batch(self) -> tuple[tuple[np.ndarray, np.ndarray], tuple[np.ndarray, np.ndarray]]:
+        pi, pd = C.c_void_p(), C.c_void_p()
+        N.check_host(N.host().ll_host_graph_last_batch(self._h, C.byref(pi), C.byref(pd)))
+        out = []
+        for p in (pi, pd):
+            n = int(N.host().ll_host_edges_count(p))
+            if n == 0:
+                out.append((np.zeros(0, np.uint32), np.zeros(0, np.uint32)))
+                continue
+            s = np.ctypeslib.as_array(N.host().ll_host_edges_src(p), shape=(n,)).copy()
+            d = np.ctypeslib.as_array(N.host().ll_host_edges_dst(p), shape=(n,)).copy()
+            out.append((s, d))
+        return out[0], out[1]
+
+    # ---- reading ---------------------------------------------------------------
+    @property
+    def num_nodes(self) -> int:
+        return int(N.host().ll_host_graph_num_nodes(self._h))
+
+    @property
+    def num_levels(self) -> int:
+        return int(N.host().ll_host_graph_num_levels(self._h))
+
+    @property
+    def deletions(self) -> bool:
+        return bool(N.host().ll_host_graph_has_deletions(self._h))
+
+    def level_view(self, direction: int, level: int) -> N.LevelView:
+        v = N.LevelView()
+        N.check_host(N.host().ll_host_graph_level_view(self._h, direction, level, C.byref(v)))
+        return v
+
+    def level_arrays(self, direction: int, level: int) -> LevelArrays:
+        v = self.level_view(direction, level)
+        vt_raw = (C.c_uint8 * (v.num_vt_entries * 16)).from_address(v.vt)
+        vt = np.frombuffer(vt_raw, dtype=VT_DTYPE)
+        if v.num_et_words:
+            et = np.ctypeslib.as_array(v.et, shape=(v.num_et_words,))
+        else:
+            et = np.zeros(0, np.uint64)
+        return LevelArrays(vt=vt, et=et, flags=v.flags)
+
+    def patches(self, direction: int, level: int) -> tuple[np.ndarray, np.ndarray]:
+        ids, ml, n = C.POINTER(C.c_uint64)(), C.POINTER(C.c_uint32)(), C.c_uint64()
+        N.check_host(N.host().ll_host_graph_patches(self._h, direction, level, C.byref(ids),
+                                                    C.byref(ml), C.byref(n)))
+        if n.value == 0:
+            return np.zeros(0, np.uint64), np.zeros(0, np.uint32)
+        return (np.ctypeslib.as_array(ids, shape=(n.value,)).copy(),
+                np.ctypeslib.as_array(ml, shape=(n.value,)).copy())
+
+    def flatten(self, direction: int, level: int = -1) -> tuple[np.ndarray, np.ndarray]:
+        """Snapshot CSR in iterator order (host restatement, for update sampling)."""
+        o, a, e = C.POINTER(C.c_uint64)(), C.POINTER(C.c_uint32)(), C.c_uint64()
+        N.check_host(N.host().ll_host_graph_flatten(self._h, direction, level, C.byref(o),
+                                                    C.byref(a), C.byref(e)))
+        ofs = np.ctypeslib.as_array(o, shape=(self.num_nodes + 1,)).copy()
+        adj = (np.ctypeslib.as_array(a, shape=(e.value,)).copy() if e.value
+               else np.zeros(0, np.uint32))
+        return ofs, adj
+
+    def __del__(self):
+        if getattr(self, "_h", None) and self._h.value:
+            N.host().ll_host_graph_destroy(self._h)
+            self._h = C.c_void_p()
+
+
+def build_multilevel(scale: int, levels: int = 1, edge_factor: float = 16.0,
+                     ins_frac: float = 0.01, del_frac: float = 0.001, seed: int = 1,
+                     deletions: bool = True, rmat=(0.57, 0.19, 0.19)) -> HostGraph:
+    """R-MAT base level (seed, seed+1 permutation) plus `levels-1` random levels
+    (seed+2+k), the multi-level snapshot configuration."""
+    base = Edges.rmat(scale, edge_factor, *rmat, seed=seed, perm_seed=seed + 1)
+    g = HostGraph(1 << scale, deletions=deletions)
+    g.add_level_edges(base)
+    del base
+    for k in range(1, levels):
+        g.add_random_level(ins_frac, del_frac if deletions else 0.0, seed + 2 + k)
+    return g
+
+
+class GpuError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+class GpuContext:
+    """One HIP device + stream (ll_gpu_ctx)."""
+
+    def __init__(self, device: int = 0):
+        self.lib = N.gpu()
+        h = C.c_void_p()
+        rc = self.lib.ll_gpu_create(device, C.byref(h))
+        if rc != 0:
+            raise GpuError(rc, f"ll_gpu_create({device}) failed")
+        self._h = h
+        self._graphs = weakref.WeakSet()
+
+    def check(self, rc: int) -> None:
+        if rc != 0:
+            raise GpuError(rc, self.lib.ll_gpu_last_error(self._h).decode(errors="replace"))
+
+    def sync(self) -> None:
+        self.check(self.lib.ll_gpu_sync(self._h))
+
+    @property
+    def stream(self) -> int:
+        return int(self.lib.ll_gpu_stream(self._h) or 0)
+
+    def profile(self, on: bool) -> None:
+        self.check(self.lib.ll_gpu_profile(self._h, int(on)))
+
+    def kernel_time(self, name: str) -> tuple[int, float]:
+        n, ms = C.c_uint64(), C.c_double()
+        self.check(self.lib.ll_gpu_profile_get(self._h, name.encode(), C.byref(n), C.byref(ms)))
+        return int(n.value), float(ms.value)
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) and self._h.value:
+            for gr in list(getattr(self, "_graphs", ())):
+                gr.close()  # graphs free their buffers on this context's stream
+            self.lib.ll_gpu_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        self.close()
+
+
+class DeviceGraph:
+    """HBM mirror of a HostGraph (ll_gpu_graph)."""
+
+    def __init__(self, ctx: GpuContext, host: HostGraph):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        self.host = host
+        flags = HAS_CONT | (HAS_DELETIONS if host.deletions else 0)
+        h = C.c_void_p()
+        ctx.check(self.lib.ll_gpu_graph_create(ctx._h, host.num_nodes, flags, C.byref(h)))
+        self._h = h
+        self.V = host.num_nodes
+        ctx._graphs.add(self)
+        self.sync_levels()
+
+    def sync_levels(self) -> None:
+        """Push every frozen host level not yet mirrored, with its patches."""
+        have = self.lib.ll_gpu_graph_num_levels(self._h, DIR_OUT)
+        for k in range(have, self.host.num_levels):
+            for d in (DIR_OUT, DIR_IN):
+                ids, ml = self.host.patches(d, k)
+                if ids.size:
+                    self.ctx.check(self.lib.ll_gpu_graph_patch_deletions(
+                        self._h, d, ids.ctypes.data_as(C.POINTER(C.c_uint64)),
+                        ml.ctypes.data_as(C.POINTER(C.c_uint32)), ids.size))
+                v = self.host.level_view(d, k)
+                self.ctx.check(self.lib.ll_gpu_graph_push_level(self._h, C.byref(v)))
+
+    @property
+    def num_levels(self) -> int:
+        return int(self.lib.ll_gpu_graph_num_levels(self._h, DIR_OUT))
+
+    def drop_snapshots(self) -> None:
+        self.lib.ll_gpu_graph_drop_snapshots(self._h)
+
+    def upload_stats(self) -> tuple[int, float]:
+        b, ms = C.c_uint64(), C.c_double()
+        self.ctx.check(self.lib.ll_gpu_graph_upload_stats(self._h, C.byref(b), C.byref(ms)))
+        return int(b.value), float(ms.value)
+
+    def snapshot_info(self, level: int = -1) -> N.SnapshotInfo:
+        info = N.SnapshotInfo()
+        self.ctx.check(self.lib.ll_gpu_snapshot_stats(self._h, level, C.byref(info)))
+        return info
+
+    def adjacency(self, direction: int, level: int = -1) -> tuple[np.ndarray, np.ndarray]:
+        info = self.snapshot_info(level)
+        ofs = np.zeros(self.V + 1, np.uint64)
+        # in- and out-directions hold the same number of live edges
+        adj = np.zeros(max(int(info.live_edges), 1), np.uint32)
+        self.ctx.check(self.lib.ll_gpu_snapshot_adjacency(
+            self._h, level, direction, ofs.ctypes.data_as(C.POINTER(C.c_uint64)),
+            adj.ctypes.data_as(C.POINTER(C.c_uint32))))
+        return ofs, adj[: int(ofs[-1])]
+
+    def degrees(self, direction: int, level: int = -1) -> np.ndarray:
+        d = np.zeros(self.V, np.uint32)
+        self.ctx.check(self.lib.ll_gpu_snapshot_degrees(self._h, level, direction,
+                                                        d.ctypes.data_as(C.POINTER(C.c_uint32))))
+        return d
+
+    # ---- algorithms --------------------------------------------------------------
+    def pagerank(self, iters: int = 10, damping: float = 0.85, level: int = -1,
+                 cap: int = 0, want_ranks: bool = True) -> tuple[np.ndarray | None, float]:
+        out = np.zeros(self.V, np.float64) if want_ranks else None
+        diff = C.c_double()
+        self.ctx.check(self.lib.ll_gpu_pagerank_ex(
+            self._h, level, iters, damping, cap,
+            out.ctypes.data_as(C.POINTER(C.c_double)) if out is not None else None,
+            C.byref(diff)))
+        return out, float(diff.value)
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) and self._h.value:
+            self.lib.ll_gpu_graph_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        self.close()

@@ -1,0 +1,131 @@
+"""HIP path vs the oracle: snapshot materialisation (bit-exact) and PageRank
+(L1 relative error <= 1e-6 against FP64 pagerank_double)."""
+import numpy as np
+import pytest
+
+import oracle
+from conftest import gpu_case
+from llama_amd.graph import DIR_IN, DIR_OUT, DeviceGraph, HostGraph, build_multilevel
+
+pytestmark = pytest.mark.gpu
+
+PR_TOL = 1e-6  # L1 relative, FP32 + compensated sums vs FP64
+
+
+def l1rel(a, b):
+    return float(np.abs(a - b).sum() / np.abs(b).sum())
+
+
+@pytest.fixture(scope="module")
+def ml():
+    g = build_multilevel(12, levels=6, ins_frac=0.03, del_frac=0.01, seed=7)
+    return g, oracle.OracleGraph.from_host(g)
+
+
+def test_snapshot_adjacency_bit_exact_every_level(ctx, ml):
+    g, o = ml
+    gpu_case("snapshot adjacency s12 x6 levels")
+    dg = DeviceGraph(ctx, g)
+    for L in range(g.num_levels):
+        for d in (DIR_OUT, DIR_IN):
+            oo, oa = o.flatten(d, L)
+            go, ga = dg.adjacency(d, L)
+            assert np.array_equal(go, oo), (L, d)
+            assert np.array_equal(ga, oa), (L, d)
+            deg = g.level_arrays(d, L).vt["degree"][:-1]
+            assert np.array_equal(dg.degrees(d, L), deg)
+    info = dg.snapshot_info()
+    assert info.live_edges == oa.size and info.stored_words >= info.live_edges
+
+
+def test_incremental_push_matches_full_push(ctx):
+    gpu_case("incremental level push")
+    g = build_multilevel(10, levels=2, ins_frac=0.05, del_frac=0.02, seed=3)
+    dg = DeviceGraph(ctx, g)
+    for k in range(3):
+        g.add_random_level(0.04, 0.02, 50 + k)
+        dg.sync_levels()  # push the new level + replay its deletion patches
+        o = oracle.OracleGraph.from_host(g)
+        for d in (DIR_OUT, DIR_IN):
+            oo, oa = o.flatten(d)
+            go, ga = dg.adjacency(d)
+            assert np.array_equal(ga, oa) and np.array_equal(go, oo)
+
+
+@pytest.mark.parametrize("cap", [512, 1024, 2048, 4096])
+def test_pagerank_multilevel(ctx, ml, cap):
+    g, o = ml
+    gpu_case(f"pagerank s12 x6 cap={cap}")
+    dg = DeviceGraph(ctx, g)
+    for L in (0, g.num_levels - 1):
+        ref, ref_diff = o.pagerank(20, level=L)
+        r, diff = dg.pagerank(20, level=L, cap=cap)
+        err = l1rel(r, ref)
+        print(f"level {L} cap {cap} L1rel {err:.3e}")
+        assert err <= PR_TOL
+        assert abs(diff - ref_diff) <= 1e-3 * ref_diff + 1e-9
+
+
+def test_pagerank_single_level_s16(ctx):
+    gpu_case("pagerank single level s16")
+    g = build_multilevel(16, levels=1, seed=1, deletions=False)
+    o = oracle.OracleGraph.from_host(g)
+    ref, _ = o.pagerank(10)
+    dg = DeviceGraph(ctx, g)
+    r, _ = dg.pagerank(10)
+    err = l1rel(r, ref)
+    print(f"s16 L1rel {err:.3e}")
+    assert err <= PR_TOL
+
+
+def test_pagerank_hub_split_over_pieces(ctx):
+    """In-degree 20000 > one piece: the partials are combined in order."""
+    gpu_case("pagerank hub star 20000")
+    k = 20000
+    n = k + 3
+    src = np.concatenate([np.arange(1, k + 1), [0, n - 1]])
+    dst = np.concatenate([np.zeros(k, int), [n - 2, 0]])
+    g = HostGraph(n)
+    g.add_level(src, dst)
+    o = oracle.OracleGraph.from_host(g)
+    ref, _ = o.pagerank(5)
+    dg = DeviceGraph(ctx, g)
+    for cap in (512, 4096):
+        r, _ = dg.pagerank(5, cap=cap)
+        assert l1rel(r, ref) <= PR_TOL
+
+
+def test_pagerank_empty_and_tiny(ctx):
+    gpu_case("pagerank empty graph")
+    g = HostGraph(5)
+    g.add_level([], [])
+    dg = DeviceGraph(ctx, g)
+    r, _ = dg.pagerank(1)
+    assert np.allclose(r, 0.15 / 5, rtol=1e-6)
+    g2 = HostGraph(1)
+    g2.add_level([], [])
+    r2, _ = DeviceGraph(ctx, g2).pagerank(3)
+    assert np.allclose(r2, [0.15], rtol=1e-6)
+
+
+def test_pagerank_deterministic(ctx, ml):
+    g, _ = ml
+    gpu_case("pagerank determinism")
+    dg = DeviceGraph(ctx, g)
+    a, _ = dg.pagerank(10)
+    dg.drop_snapshots()
+    b, _ = dg.pagerank(10)
+    assert np.array_equal(a, b)
+
+
+def test_mirror_errors(ctx):
+    gpu_case("mirror error paths")
+    from llama_amd.graph import GpuError
+    g = HostGraph(4)
+    g.add_level([0], [1])
+    dg = DeviceGraph(ctx, g)
+    v = g.level_view(DIR_OUT, 0)
+    with pytest.raises(GpuError):  # level pushed twice
+        ctx.check(ctx.lib.ll_gpu_graph_push_level(dg._h, v))
+    with pytest.raises(GpuError):
+        dg.pagerank(0)
