@@ -33,6 +33,50 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def pick_sources(g, k: int, seed: int):
+    """k distinct uniform vertices with out+in degree > 0 (node_exists,
+    ll_mlcsr_graph.h:691-695)."""
+    import numpy as np
+    L = g.num_levels - 1
+    deg = (g.level_arrays(0, L).vt["degree"][:-1].astype(np.int64) +
+           g.level_arrays(1, L).vt["degree"][:-1])
+    cand = np.flatnonzero(deg > 0)
+    return np.random.default_rng(seed).choice(cand, k, replace=False)
+
+
+def bench_bfs(dg, g, E: int, V: int, args) -> dict:
+    """64 single-source direction-optimising BFS runs (levels kept on device)."""
+    roots = pick_sources(g, args.bfs_sources, args.seed + 3)
+    dg.bfs(int(roots[0]), want_levels=False)  # warm the snapshot
+    ctx = dg.ctx
+    ctx.sync()
+    ctx.profile(True)
+    trav = exam = reached = 0
+    levels_total = 0
+    t0 = time.perf_counter()
+    for r in roots:
+        _, n, x = dg.bfs(int(r), want_levels=False)
+        trav += dg.last_bfs_traversed
+        exam += x
+        reached += n
+    ctx.sync()
+    t = time.perf_counter() - t0
+    n_td, ms_td = ctx.kernel_time("k_td_expand")
+    n_bu, ms_bu = ctx.kernel_time("k_bu_step")
+    ctx.profile(False)
+    levels_total = n_td + n_bu
+    b_bfs = 4 * exam + 12 * reached + 2 * ((V + 7) // 8) * levels_total
+    log(f"BFS x{len(roots)}: {t * 1e3:.1f} ms, {trav / t / 1e9:.2f} GTEPS")
+    return {
+        "sources": len(roots), "seconds": round(t, 4),
+        "gteps": round(trav / t / 1e9, 4), "edges_traversed": trav,
+        "edges_examined": exam, "reached": reached, "levels": levels_total,
+        "alg_bytes": b_bfs, "alg_gbs": round(b_bfs / t / 1e9, 2),
+        "frac_hbm": round(b_bfs / t / 1e9 / HBM_PEAK_GBS, 4),
+        "ms_top_down": round(ms_td, 3), "ms_bottom_up": round(ms_bu, 3),
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -46,6 +90,8 @@ def main() -> int:
     ap.add_argument("--cpu-iters", type=int, default=2,
                     help="PageRank iterations timed for the CPU baseline (0 = skip)")
     ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--bfs-sources", type=int, default=64,
+                    help="single-source DO-BFS runs timed (0 = skip)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -101,6 +147,10 @@ def main() -> int:
     b_pr = 8 * E + 16 * V  # algorithmic bytes of one pull iteration
     achieved = b_pr / (avg_pr_ms * 1e-3) / 1e9
 
+    secondary = {}
+    if args.bfs_sources > 0:
+        secondary["bfs"] = bench_bfs(dg, g, E, V, args)
+
     cpu = None
     if args.cpu_iters > 0:
         import oracle
@@ -150,6 +200,7 @@ def main() -> int:
             **{k: round(v[1] / args.steps, 4) for k, v in mat.items()},
         },
         "upload": {"bytes": up_bytes, "seconds": round(up_s, 3)},
+        "secondary": secondary,
     }
     print(json.dumps(out), flush=True)
     dg.close()

@@ -168,6 +168,36 @@ int ll_gpu_wpr_adjoint(ll_gpu_graph* g, int level, const float* edge_w, const fl
                        const float* dl_dr, int iters, double damping, float* dl_dw_out,
                        float* dl_dteleport_out);
 
+/* --- extended forms, measurement and tuning ------------------------------------ */
+/* ll_gpu_pagerank with an explicit row-block capacity (0 = default 1024;
+ * 512/1024/2048/4096 entries per workgroup for runs of short rows). */
+int ll_gpu_pagerank_ex(ll_gpu_graph* g, int level, int iters, double damping, uint32_t cap,
+                       double* ranks_out, double* last_diff);
+/* ll_gpu_bfs plus edges_traversed = sum of live out-degrees of the reached
+ * vertices (Graph500 TEPS numerator). */
+int ll_gpu_bfs_ex(ll_gpu_graph* g, int level, int64_t root, int32_t* levels_out,
+                  uint64_t* reached, uint64_t* edges_examined, uint64_t* edges_traversed);
+/* ll_gpu_bfs_batch plus the number of pull sweeps run (may be NULL). */
+int ll_gpu_bfs_batch_ex(ll_gpu_graph* g, int level, const int64_t* roots, int n, int lane_width,
+                        int32_t* levels_out, uint64_t* reached_out, uint64_t* sweeps);
+/* ll_gpu_tc plus the oriented arc count and W = sum over arcs (a,b) of
+ * d+(a) + d+(b) (intersection element reads). */
+int ll_gpu_tc_ex(ll_gpu_graph* g, int level, uint64_t* triangles, uint64_t* arcs,
+                 uint64_t* work);
+/* ll_gpu_scc plus the number of full-graph sweeps (trim, propagation, BFS). */
+int ll_gpu_scc_ex(ll_gpu_graph* g, int level, int64_t* labels_out, uint64_t* num_scc,
+                  uint64_t* sweeps);
+/* Per-kernel device time from HIP events: ll_gpu_profile(ctx, 1) clears the
+ * totals and starts recording; ll_gpu_profile_get returns launches and the
+ * summed milliseconds of one kernel name (e.g. "k_pr_pull"). */
+int ll_gpu_profile(ll_gpu_ctx* ctx, int on);
+int ll_gpu_profile_get(ll_gpu_ctx* ctx, const char* kernel, uint64_t* launches,
+                       double* total_ms);
+/* Autotuner decision whose key starts with `prefix` ("ppr:", "bfs_lanes:"). */
+int ll_gpu_tuned(ll_gpu_ctx* ctx, const char* prefix, int* value);
+/* Drop the cached snapshot materialisations (the next call rebuilds them). */
+void ll_gpu_graph_drop_snapshots(ll_gpu_graph* g);
+
 #ifdef __cplusplus
 }
 #endif

@@ -56,6 +56,12 @@ GPU_SIGNATURES = {
     "ll_gpu_pagerank": (C.c_int, [vp, C.c_int, C.c_int, f64, P(f64), P(f64)]),
     "ll_gpu_pagerank_ex": (C.c_int, [vp, C.c_int, C.c_int, f64, u32, P(f64), P(f64)]),
     "ll_gpu_bfs": (C.c_int, [vp, C.c_int, i64, P(i32), P(u64), P(u64)]),
+    "ll_gpu_bfs_ex": (C.c_int, [vp, C.c_int, i64, P(i32), P(u64), P(u64), P(u64)]),
+    "ll_gpu_bfs_batch_ex": (C.c_int, [vp, C.c_int, P(i64), C.c_int, C.c_int, P(i32), P(u64),
+                                      P(u64)]),
+    "ll_gpu_tc_ex": (C.c_int, [vp, C.c_int, P(u64), P(u64), P(u64)]),
+    "ll_gpu_scc_ex": (C.c_int, [vp, C.c_int, P(i64), P(u64), P(u64)]),
+    "ll_gpu_tuned": (C.c_int, [vp, cp, P(C.c_int)]),
     "ll_gpu_bfs_batch": (C.c_int, [vp, C.c_int, P(i64), C.c_int, C.c_int, P(i32), P(u64)]),
     "ll_gpu_tc": (C.c_int, [vp, C.c_int, P(u64)]),
     "ll_gpu_scc": (C.c_int, [vp, C.c_int, P(i64), P(u64)]),

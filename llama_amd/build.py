@@ -39,7 +39,8 @@ HIP_FLAGS = [
 HOST_FLAGS = ["-O3", "-std=c++17", "-fopenmp", "-fPIC", "-Wall", "-Wextra", "-DNDEBUG",
               f"-I{INCLUDE}"]
 
-HIP_SOURCES = ["ll_gpu.hip", "prim.hip", "snapshot.hip", "rowblocks.hip", "pagerank.hip"]
+HIP_SOURCES = ["ll_gpu.hip", "prim.hip", "snapshot.hip", "rowblocks.hip", "pagerank.hip",
+               "bfs.hip", "bfs_batch.hip", "tc.hip", "scc.hip", "ppr.hip", "wpr.hip"]
 HOST_SOURCES = ["host/ll_layout.cc", "host/ll_host.cc"]
 
 

@@ -1,0 +1,367 @@
+// Direction-optimising BFS over the snapshot (task sssp_unweighted / bfs,
+// benchmark/benchmarks/sssp.h:220-333, bfs.h:85-128).
+//
+// The reference traverses top-down only (ll_bfs_template.h:152-293).  Levels
+// of a BFS are unique, so any correct traversal order yields the same
+// dist[] = level array; here each level runs either
+//   * top-down over a queue of frontier vertices with out-edges: the
+//     frontier's out-edge lists are concatenated by an exclusive scan of their
+//     degrees and processed edge-parallel (2048 edges per workgroup; each edge
+//     finds its frontier vertex by binary search in an LDS copy of the
+//     workgroup's slice of the scan), claiming vertices with an atomic OR on
+//     the visited bitmap and appending them with one wave-aggregated atomic;
+//   * bottom-up over the visited bitmap: lane i of a wave owns vertex 64w+i, so
+//     the wave's ballot of "found a parent in the frontier bitmap" is exactly
+//     the 64-bit word of the next frontier bitmap and of the visited bitmap
+//     (plain stores, no atomics); each unvisited vertex scans its in-edges and
+//     stops at the first frontier parent.
+// The switch follows Beamer et al. (SC'12): top-down -> bottom-up when the
+// frontier's out-edges exceed the unexplored edges / 14, back when the
+// frontier shrinks below V / 24.
+#include <climits>
+
+#include "ll_gpu_internal.h"
+
+namespace llgpu {
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kEdgesPerBlock = 2048;
+constexpr int32_t kUnreached = INT_MAX - 1;
+
+struct Counters {
+  unsigned long long n_next;     // vertices discovered this level
+  unsigned long long m_next;     // their out-edges
+  unsigned long long examined;   // edges inspected this level
+};
+
+__global__ void k_bfs_init(int32_t* lv, uint64_t* visited, const uint64_t* excluded, uint64_t V,
+                           uint64_t words) {
+  uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (lv && i < V) lv[i] = kUnreached;
+  if (i < words) {
+    // bits beyond V (and excluded vertices) count as visited, so no step claims them
+    uint64_t lo = i * 64;
+    uint64_t w = 0;
+    if (lo + 64 > V) w = V > lo ? ~((uint64_t(1) << (V - lo)) - 1) : ~uint64_t(0);
+    if (excluded) w |= excluded[i];
+    visited[i] = w;
+  }
+}
+
+__global__ void k_bfs_root(int32_t* lv, uint64_t* visited, uint32_t* queue, uint32_t root,
+                           const uint32_t* outdeg, Counters* c) {
+  if (lv) lv[root] = 0;
+  visited[root >> 6] |= uint64_t(1) << (root & 63);
+  queue[0] = root;
+  c->n_next = outdeg[root] > 0 ? 1 : 0;
+  c->m_next = outdeg[root];
+  c->examined = 0;
+}
+
+__global__ void k_fill_u64(uint64_t* p, uint64_t n, uint64_t v) {
+  uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__global__ void k_zero_counters(Counters* c) {
+  c->n_next = 0;
+  c->m_next = 0;
+  c->examined = 0;
+}
+
+__global__ void k_queue_degrees(const uint32_t* q, uint64_t n, const uint32_t* outdeg,
+                                uint32_t* qdeg) {
+  uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) qdeg[i] = outdeg[q[i]];
+}
+
+// Append the calling lane's vertex to the queue with one atomic per wave.
+__device__ inline void wave_append(bool take, uint32_t v, uint32_t outdeg, uint32_t* q,
+                                   Counters* c) {
+  const uint64_t m = __ballot(take);
+  const uint64_t mq = __ballot(take && outdeg > 0);
+  const int lane = threadIdx.x & 63;
+  const int leader = m ? __ffsll((long long)m) - 1 : 0;
+  unsigned long long qbase = 0;
+  uint64_t md = take ? outdeg : 0;
+  for (int o = 32; o > 0; o >>= 1) md += __shfl_xor(md, o, 64);
+  if (lane == leader && m) {
+    atomicAdd(&c->m_next, (unsigned long long)md);
+    if (mq) qbase = atomicAdd(&c->n_next, (unsigned long long)__popcll(mq));
+  }
+  qbase = __shfl(qbase, leader, 64);
+  if (take && outdeg > 0) q[qbase + __popcll(mq & ((uint64_t(1) << lane) - 1))] = v;
+}
+
+__global__ __launch_bounds__(kBlock) void k_td_expand(
+    const uint32_t* q, const uint64_t* qofs, uint64_t nq, uint64_t m_f, const uint64_t* ofs,
+    const uint32_t* adj, const uint32_t* outdeg, uint64_t* visited, int32_t* lv, int32_t next_lv,
+    uint32_t* q_next, Counters* c) {
+  __shared__ uint64_t l_qofs[kEdgesPerBlock + 1];
+  __shared__ uint64_t l_base[kEdgesPerBlock];
+  __shared__ uint64_t sh_lo, sh_hi;
+  const uint64_t x0 = uint64_t(blockIdx.x) * kEdgesPerBlock;
+  const uint64_t x1 = min(m_f, x0 + kEdgesPerBlock);
+  if (threadIdx.x == 0) {
+    uint64_t lo = 0, hi = nq;
+    while (hi - lo > 1) { uint64_t mid = (lo + hi) / 2; if (qofs[mid] <= x0) lo = mid; else hi = mid; }
+    sh_lo = lo;
+    uint64_t lo2 = lo, hi2 = nq;
+    while (hi2 - lo2 > 1) { uint64_t mid = (lo2 + hi2) / 2; if (qofs[mid] <= x1 - 1) lo2 = mid; else hi2 = mid; }
+    sh_hi = lo2;
+  }
+  __syncthreads();
+  const uint64_t lo = sh_lo;
+  const uint32_t n = uint32_t(sh_hi - lo + 1);
+  for (uint32_t i = threadIdx.x; i <= n; i += kBlock) {
+    l_qofs[i] = qofs[lo + i];
+    if (i < n) l_base[i] = ofs[q[lo + i]];
+  }
+  __syncthreads();
+  for (uint64_t base = x0; base < x1; base += kBlock) {
+    const uint64_t x = base + threadIdx.x;
+    bool take = false;
+    uint32_t v = 0, od = 0;
+    if (x < x1) {
+      uint32_t a = 0, b = n;
+      while (b - a > 1) { uint32_t mid = (a + b) >> 1; if (l_qofs[mid] <= x) a = mid; else b = mid; }
+      v = adj[l_base[a] + (x - l_qofs[a])];
+      const uint64_t bit = uint64_t(1) << (v & 63);
+      if (!(visited[v >> 6] & bit)) {
+        const uint64_t old = atomicOr((unsigned long long*)&visited[v >> 6], (unsigned long long)bit);
+        if (!(old & bit)) {
+          take = true;
+          if (lv) lv[v] = next_lv;
+          od = outdeg[v];
+        }
+      }
+    }
+    wave_append(take, v, od, q_next, c);
+  }
+  if (threadIdx.x == 0) atomicAdd(&c->examined, (unsigned long long)(x1 - x0));
+}
+
+__global__ void k_queue_to_bitmap(const uint32_t* q, uint64_t n, uint64_t* bm) {
+  uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) {
+    uint32_t v = q[i];
+    atomicOr((unsigned long long*)&bm[v >> 6], (unsigned long long)(uint64_t(1) << (v & 63)));
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_bu_step(const uint64_t* ofs, const uint32_t* adj,
+                                                    const uint32_t* outdeg, const uint64_t* fbm,
+                                                    uint64_t* visited, uint64_t* nbm, int32_t* lv,
+                                                    int32_t next_lv, uint64_t V, Counters* c) {
+  const uint64_t v = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const uint64_t w = v >> 6;
+  const uint64_t words = (V + 63) / 64;
+  if (w >= words) return;  // whole wave beyond the last word
+  const uint64_t vis = visited[w];
+  bool found = false;
+  uint64_t exam = 0;
+  if (!((vis >> lane) & 1)) {
+    for (uint64_t e = ofs[v], end = ofs[v + 1]; e < end; e++) {
+      exam++;
+      const uint32_t u = adj[e];
+      if ((fbm[u >> 6] >> (u & 63)) & 1) { found = true; break; }
+    }
+  }
+  const uint64_t mask = __ballot(found);
+  if (found && lv) lv[v] = next_lv;
+  uint64_t md = found ? outdeg[v] : 0;
+  for (int o = 32; o > 0; o >>= 1) {
+    md += __shfl_xor(md, o, 64);
+    exam += __shfl_xor(exam, o, 64);
+  }
+  if (lane == 0) {
+    nbm[w] = mask;
+    if (mask) {
+      visited[w] = vis | mask;
+      atomicAdd(&c->n_next, (unsigned long long)__popcll(mask));
+      atomicAdd(&c->m_next, (unsigned long long)md);
+    }
+    if (exam) atomicAdd(&c->examined, (unsigned long long)exam);
+  }
+}
+
+__global__ void k_bitmap_to_queue(const uint64_t* bm, uint64_t V, const uint32_t* outdeg,
+                                  uint32_t* q, Counters* c) {
+  const uint64_t v = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const uint64_t words = (V + 63) / 64;
+  if ((v >> 6) >= words) return;
+  bool in = v < V && ((bm[v >> 6] >> (v & 63)) & 1);
+  uint32_t od = in ? outdeg[v] : 0;
+  // reuse the append path; m_next is not needed here and is discarded
+  const uint64_t mq = __ballot(in && od > 0);
+  const int lane = threadIdx.x & 63;
+  unsigned long long qbase = 0;
+  if (lane == 0 && mq) qbase = atomicAdd(&c->n_next, (unsigned long long)__popcll(mq));
+  qbase = __shfl(qbase, 0, 64);
+  if (in && od > 0) q[qbase + __popcll(mq & ((uint64_t(1) << lane) - 1))] = uint32_t(v);
+}
+
+__global__ void k_count_reached(const int32_t* lv, uint64_t V, unsigned long long* out) {
+  const uint64_t v = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  unsigned long long x = (v < V && lv[v] != kUnreached) ? 1ull : 0ull;
+  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+  if ((threadIdx.x & 63) == 0 && x) atomicAdd(out, x);
+}
+
+Counters read_counters(const DevBuf& c, hipStream_t s) {
+  Counters h;
+  LLG_HIP(hipMemcpyAsync(&h, c.p, sizeof h, hipMemcpyDeviceToHost, s));
+  LLG_HIP(hipStreamSynchronize(s));
+  return h;
+}
+
+}  // namespace
+
+// bfs_core: see ll_gpu_internal.h.
+BfsResult bfs_core(ll_gpu_graph* g, const Csr& fwd, const Csr& rev, const uint32_t* fdeg,
+                   uint32_t root, const uint64_t* excluded, DevBuf& visited, int32_t* lv) {
+  ll_gpu_ctx* ctx = g->ctx;
+  hipStream_t s = ctx->stream;
+  const uint64_t V = g->V;
+  const uint64_t words = (V + 63) / 64;
+  DevBuf fbm, nbm, q[2], qdeg, qofs, cnt;
+  visited.alloc(words * 8, s);
+  fbm.alloc(words * 8, s);
+  nbm.alloc(words * 8, s);
+  q[0].alloc(V * 4, s);
+  q[1].alloc(V * 4, s);
+  qdeg.alloc(V * 4, s);
+  qofs.alloc((V + 1) * 8, s);
+  cnt.alloc(sizeof(Counters), s);
+  k_bfs_init<<<grid_for(std::max(V, words), 256), 256, 0, s>>>(lv, visited.as<uint64_t>(),
+                                                              excluded, V, words);
+  k_bfs_root<<<1, 1, 0, s>>>(lv, visited.as<uint64_t>(), q[0].as<uint32_t>(), root, fdeg,
+                             cnt.as<Counters>());
+  LLG_HIP(hipGetLastError());
+  Counters h = read_counters(cnt, s);
+  BfsResult r;
+  uint64_t nq = h.n_next;       // queue entries (vertices with out-edges)
+  uint64_t n_f = 1;             // frontier vertices
+  uint64_t m_f = h.m_next;      // frontier out-edges
+  uint64_t m_visited = h.m_next;
+  r.reached = 1;
+  bool bottom_up = false;
+  int cur = 0;
+  int32_t lvl = 0;
+  const uint64_t E = fwd.E;
+  while (n_f > 0 && m_f > 0) {
+    const uint64_t m_u = E > m_visited ? E - m_visited : 0;
+    if (!bottom_up && m_f > m_u / 14) {
+      bottom_up = true;
+      k_fill_u64<<<grid_for(words, 256), 256, 0, s>>>(fbm.as<uint64_t>(), words, 0);
+      k_queue_to_bitmap<<<grid_for(nq, 256), 256, 0, s>>>(q[cur].as<uint32_t>(), nq,
+                                                          fbm.as<uint64_t>());
+    }
+    k_zero_counters<<<1, 1, 0, s>>>(cnt.as<Counters>());
+    if (bottom_up) {
+      TimedLaunch t(ctx, "k_bu_step");
+      k_bu_step<<<grid_for(words * 64, kBlock), kBlock, 0, s>>>(
+          rev.ofs.as<uint64_t>(), rev.adj.as<uint32_t>(), fdeg, fbm.as<uint64_t>(),
+          visited.as<uint64_t>(), nbm.as<uint64_t>(), lv, lvl + 1, V, cnt.as<Counters>());
+    } else {
+      k_queue_degrees<<<grid_for(nq, 256), 256, 0, s>>>(q[cur].as<uint32_t>(), nq, fdeg,
+                                                        qdeg.as<uint32_t>());
+      scan_u32_to_u64(qdeg.as<uint32_t>(), qofs.as<uint64_t>(), nq, s);
+      TimedLaunch t(ctx, "k_td_expand");
+      k_td_expand<<<grid_for(m_f, kEdgesPerBlock), kBlock, 0, s>>>(
+          q[cur].as<uint32_t>(), qofs.as<uint64_t>(), nq, m_f, fwd.ofs.as<uint64_t>(),
+          fwd.adj.as<uint32_t>(), fdeg, visited.as<uint64_t>(), lv, lvl + 1,
+          q[cur ^ 1].as<uint32_t>(), cnt.as<Counters>());
+    }
+    LLG_HIP(hipGetLastError());
+    h = read_counters(cnt, s);
+    r.examined += h.examined;
+    const uint64_t n_next = h.n_next;
+    lvl++;
+    m_visited += h.m_next;
+    if (bottom_up) {
+      r.reached += n_next;
+      const bool shrinking = n_next < n_f;
+      n_f = n_next;
+      m_f = h.m_next;
+      std::swap(fbm, nbm);
+      if (n_f == 0) break;
+      if (shrinking && n_f < V / 24) {
+        bottom_up = false;
+        k_zero_counters<<<1, 1, 0, s>>>(cnt.as<Counters>());
+        k_bitmap_to_queue<<<grid_for(words * 64, 256), 256, 0, s>>>(
+            fbm.as<uint64_t>(), V, fdeg, q[cur].as<uint32_t>(), cnt.as<Counters>());
+        LLG_HIP(hipGetLastError());
+        nq = read_counters(cnt, s).n_next;
+      }
+    } else {
+      // top-down n_next counts the queued vertices (those with out-edges)
+      nq = n_next;
+      n_f = n_next;
+      m_f = h.m_next;
+      cur ^= 1;
+    }
+  }
+  r.levels = lvl;
+  r.traversed = m_visited;
+  return r;
+}
+
+// Single-source BFS; per-vertex levels stay in `lv` (device, int32[V]).
+BfsResult bfs_device(ll_gpu_graph* g, int level, uint32_t root, DevBuf& lv) {
+  hipStream_t s = g->ctx->stream;
+  const uint64_t V = g->V;
+  const Csr& out = ensure_csr(g, LL_GPU_DIR_OUT, level);
+  const Csr& in = ensure_csr(g, LL_GPU_DIR_IN, level);
+  const uint32_t* outdeg = ensure_degrees(g, LL_GPU_DIR_OUT, level);
+  lv.alloc(V * 4, s);
+  DevBuf visited;
+  BfsResult r = bfs_core(g, out, in, outdeg, root, nullptr, visited, lv.as<int32_t>());
+  DevBuf red;
+  red.alloc(8, s);
+  k_fill_u64<<<1, 1, 0, s>>>(red.as<uint64_t>(), 1, 0);
+  k_count_reached<<<grid_for(V, 256), 256, 0, s>>>(lv.as<int32_t>(), V,
+                                                   red.as<unsigned long long>());
+  LLG_HIP(hipGetLastError());
+  uint64_t hr = 0;
+  LLG_HIP(hipMemcpyAsync(&hr, red.p, 8, hipMemcpyDeviceToHost, s));
+  LLG_HIP(hipStreamSynchronize(s));
+  r.reached = hr;
+  return r;
+}
+
+}  // namespace llgpu
+
+extern "C" {
+
+int ll_gpu_bfs_ex(ll_gpu_graph* g, int level, int64_t root, int32_t* levels_out,
+                  uint64_t* reached, uint64_t* edges_examined, uint64_t* edges_traversed) {
+  if (!g) return LL_GPU_EINVAL;
+  try {
+    LLG_HIP(hipSetDevice(g->ctx->device));
+    LLG_CHECK(root >= 0 && uint64_t(root) < g->V, LL_GPU_EINVAL, "root outside [0, max_nodes)");
+    llgpu::DevBuf lv;
+    llgpu::BfsResult r = llgpu::bfs_device(g, level, uint32_t(root), lv);
+    if (levels_out) {
+      LLG_HIP(hipMemcpyAsync(levels_out, lv.p, g->V * 4, hipMemcpyDeviceToHost, g->ctx->stream));
+      LLG_HIP(hipStreamSynchronize(g->ctx->stream));
+    }
+    if (reached) *reached = r.reached;
+    if (edges_examined) *edges_examined = r.examined;
+    if (edges_traversed) *edges_traversed = r.traversed;
+    return LL_GPU_OK;
+  } catch (const llgpu::Error& e) {
+    g->ctx->err = e.msg;
+    return e.code;
+  }
+}
+
+int ll_gpu_bfs(ll_gpu_graph* g, int level, int64_t root, int32_t* levels_out, uint64_t* reached,
+               uint64_t* edges_examined) {
+  return ll_gpu_bfs_ex(g, level, root, levels_out, reached, edges_examined, nullptr);
+}
+
+}  // extern "C"

@@ -1,0 +1,394 @@
+// Multi-source BFS with sources packed into lanes of a per-vertex bit word
+// (16, 32, 64 or 128 lanes).  Level l+1 of every source is one pull sweep over
+// the snapshot in-edges:
+//   next[v] = (OR_{u in in(v)} frontier[u]) & ~visited[v]
+// run on the row blocks of the in-CSR: a workgroup stages frontier[u] for a
+// run of short rows in LDS with coalesced neighbour loads and ORs each row in
+// one thread; long rows are split into pieces whose partial ORs are combined
+// in piece order.  Vertices already visited by every lane are skipped.
+// Levels are unique per source, so each lane's levels equal a single-source
+// BFS from that lane's root (sssp_unweighted semantics, sssp.h:220-333).
+#include <climits>
+#include <string>
+
+#include "ll_gpu_internal.h"
+#include "rowblocks.h"
+
+namespace llgpu {
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int32_t kUnreached = INT_MAX - 1;
+constexpr uint32_t kCap = 1024;  // stream-item capacity of the in-CSR row blocks
+
+struct W128 {
+  uint64_t a, b;
+};
+__device__ __host__ inline W128 operator|(W128 x, W128 y) { return {x.a | y.a, x.b | y.b}; }
+__device__ __host__ inline W128 operator&(W128 x, W128 y) { return {x.a & y.a, x.b & y.b}; }
+__device__ __host__ inline W128 operator~(W128 x) { return {~x.a, ~x.b}; }
+__device__ __host__ inline bool operator==(W128 x, W128 y) { return x.a == y.a && x.b == y.b; }
+
+template <typename W> struct Lane;
+template <> struct Lane<uint16_t> {
+  static constexpr int kBits = 16;
+  __device__ __host__ static uint16_t zero() { return 0; }
+  __device__ __host__ static uint16_t low(int n) { return n >= 16 ? 0xffff : uint16_t((1u << n) - 1); }
+  __device__ static bool test(uint16_t w, int s) { return (w >> s) & 1; }
+  __device__ static int popc(uint16_t w) { return __popc(w); }
+};
+template <> struct Lane<uint32_t> {
+  static constexpr int kBits = 32;
+  __device__ __host__ static uint32_t zero() { return 0; }
+  __device__ __host__ static uint32_t low(int n) { return n >= 32 ? ~0u : (1u << n) - 1; }
+  __device__ static bool test(uint32_t w, int s) { return (w >> s) & 1; }
+  __device__ static int popc(uint32_t w) { return __popc(w); }
+};
+template <> struct Lane<uint64_t> {
+  static constexpr int kBits = 64;
+  __device__ __host__ static uint64_t zero() { return 0; }
+  __device__ __host__ static uint64_t low(int n) { return n >= 64 ? ~0ull : (1ull << n) - 1; }
+  __device__ static bool test(uint64_t w, int s) { return (w >> s) & 1; }
+  __device__ static int popc(uint64_t w) { return __popcll(w); }
+};
+template <> struct Lane<W128> {
+  static constexpr int kBits = 128;
+  __device__ __host__ static W128 zero() { return {0, 0}; }
+  __device__ __host__ static W128 low(int n) {
+    return n >= 128 ? W128{~0ull, ~0ull}
+                    : n >= 64 ? W128{~0ull, (n == 64 ? 0ull : (1ull << (n - 64)) - 1)}
+                              : W128{(1ull << n) - 1, 0ull};
+  }
+  __device__ static bool test(W128 w, int s) { return s < 64 ? (w.a >> s) & 1 : (w.b >> (s - 64)) & 1; }
+  __device__ static int popc(W128 w) { return __popcll(w.a) + __popcll(w.b); }
+};
+
+template <typename W>
+struct MsArgs {
+  const RowItem* items;
+  const uint64_t* ofs;
+  const uint32_t* adj;
+  const W* fr;
+  W* vis;
+  W* nx;
+  W* part;
+  int32_t* lv;            // n x V, or nullptr
+  uint64_t V;
+  int32_t next_lv;
+  int src_base, n_src;    // lanes [0, n_src) map to sources src_base + lane
+  W full;
+  unsigned long long* found;
+};
+
+template <typename W>
+__device__ inline uint32_t finalize(const MsArgs<W>& a, uint32_t r, W acc) {
+  const W vr = a.vis[r];
+  const W nw = acc & ~vr;
+  a.nx[r] = nw;
+  if (nw == Lane<W>::zero()) return 0;
+  a.vis[r] = vr | nw;
+  if (a.lv) {
+    for (int s = 0; s < a.n_src; s++)
+      if (Lane<W>::test(nw, s)) a.lv[uint64_t(a.src_base + s) * a.V + r] = a.next_lv;
+  }
+  return uint32_t(Lane<W>::popc(nw));
+}
+
+__device__ inline void block_count(unsigned long long* out, uint32_t x) {
+  __shared__ uint32_t red[kBlock / 64];
+  for (int o = 32; o > 0; o >>= 1) x += __shfl_down(x, o, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = x;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t t = 0;
+    for (int i = 0; i < kBlock / 64; i++) t += red[i];
+    if (t) atomicAdd(out, (unsigned long long)t);
+  }
+}
+
+template <typename W>
+__global__ __launch_bounds__(kBlock) void k_ms_pull(MsArgs<W> a) {
+  __shared__ W vals[kCap + kCap / 2];
+  __shared__ uint64_t rofs[kRowsPerItem + 1];
+  __shared__ W red[kBlock / 64];
+  const RowItem it = a.items[blockIdx.x];
+  uint32_t cnt = 0;
+  if (it.npieces == 0) {
+    const uint32_t nrows = it.row_end - it.row_begin;
+    for (uint32_t i = threadIdx.x; i <= nrows; i += kBlock) rofs[i] = a.ofs[it.row_begin + i];
+    __syncthreads();
+    const uint64_t e0 = rofs[0];
+    const uint32_t ne = uint32_t(rofs[nrows] - e0);
+    constexpr uint32_t kPer = (kCap + kCap / 2) / kBlock;
+    uint32_t idx[kPer];
+#pragma unroll
+    for (uint32_t k = 0; k < kPer; k++) {
+      uint32_t i = threadIdx.x + k * kBlock;
+      idx[k] = i < ne ? a.adj[e0 + i] : 0u;
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < kPer; k++) {
+      uint32_t i = threadIdx.x + k * kBlock;
+      if (i < ne) vals[i] = a.fr[idx[k]];
+    }
+    __syncthreads();
+    if (threadIdx.x < nrows) {
+      const uint32_t r = it.row_begin + threadIdx.x;
+      if (!(a.vis[r] == a.full)) {
+        W acc = Lane<W>::zero();
+        const uint32_t lo = uint32_t(rofs[threadIdx.x] - e0), hi = uint32_t(rofs[threadIdx.x + 1] - e0);
+        for (uint32_t j = lo; j < hi; j++) acc = acc | vals[j];
+        cnt = finalize(a, r, acc);
+      } else {
+        a.nx[r] = Lane<W>::zero();
+      }
+    }
+  } else {
+    const uint32_t r = it.row_begin;
+    const uint64_t lo = a.ofs[r] + uint64_t(it.piece) * kPiece;
+    const uint64_t hi = min(a.ofs[r + 1], lo + kPiece);
+    W acc = Lane<W>::zero();
+    if (!(a.vis[r] == a.full))
+      for (uint64_t e = lo + threadIdx.x; e < hi; e += kBlock) acc = acc | a.fr[a.adj[e]];
+    // OR-reduce the workgroup through LDS
+    __shared__ W wred[kBlock];
+    wred[threadIdx.x] = acc;
+    __syncthreads();
+    for (int st = kBlock / 2; st > 0; st >>= 1) {
+      if (threadIdx.x < st) wred[threadIdx.x] = wred[threadIdx.x] | wred[threadIdx.x + st];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+      if (it.npieces == 1) cnt = finalize(a, r, wred[0]);
+      else a.part[blockIdx.x] = wred[0];
+    }
+    (void)red;
+  }
+  block_count(a.found, cnt);
+}
+
+template <typename W>
+__global__ void k_ms_fixup(MsArgs<W> a, uint64_t nitems) {
+  const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  uint32_t cnt = 0;
+  if (i < nitems) {
+    const RowItem it = a.items[i];
+    if (it.npieces > 1 && it.piece == 0) {
+      W acc = a.part[i];
+      for (uint32_t p = 1; p < it.npieces; p++) acc = acc | a.part[i + p];
+      cnt = finalize(a, it.row_begin, acc);
+    }
+  }
+  block_count(a.found, cnt);
+}
+
+template <typename W>
+__global__ void k_ms_init(W* vis, W* fr, uint64_t V, W init) {
+  const uint64_t v = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (v < V) {
+    vis[v] = init;
+    fr[v] = Lane<W>::zero();
+  }
+}
+
+__global__ void k_fill_i32(int32_t* p, uint64_t n, int32_t x) {
+  const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = x;
+}
+
+template <typename W>
+__global__ void k_ms_roots(W* vis, W* fr, const int64_t* roots, int n, int32_t* lv, uint64_t V,
+                           int src_base) {
+  // one thread: roots may repeat across lanes
+  for (int s = 0; s < n; s++) {
+    const uint64_t r = uint64_t(roots[s]);
+    W bit = Lane<W>::low(s + 1) & ~Lane<W>::low(s);
+    vis[r] = vis[r] | bit;
+    fr[r] = fr[r] | bit;
+    if (lv) lv[uint64_t(src_base + s) * V + r] = 0;
+  }
+}
+
+__global__ void k_zero_ull(unsigned long long* p) { *p = 0; }
+
+// Per-lane visited counts: lane s of each wave accumulates popcount(ballot(bit s)).
+template <typename W>
+__global__ void k_ms_reached(const W* vis, uint64_t V, int n_src, unsigned long long* out) {
+  const uint64_t v = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const W w = v < V ? vis[v] : Lane<W>::zero();
+  const int lane = threadIdx.x & 63;
+  unsigned long long c0 = 0, c1 = 0;
+  for (int s = 0; s < n_src; s++) {
+    const uint64_t b = __ballot(v < V && Lane<W>::test(w, s));
+    if ((s & 63) == lane) {
+      if (s < 64) c0 += __popcll(b);
+      else c1 += __popcll(b);
+    }
+  }
+  if (lane < n_src && c0) atomicAdd(&out[lane], c0);
+  if (lane + 64 < n_src && c1) atomicAdd(&out[lane + 64], c1);
+}
+
+}  // namespace
+
+template <typename W>
+void msbfs_batch(ll_gpu_graph* g, int level, const int64_t* d_roots, int n, int src_base,
+                 int32_t* d_lv, unsigned long long* d_reached, uint64_t* levels_run) {
+  ll_gpu_ctx* ctx = g->ctx;
+  hipStream_t s = ctx->stream;
+  const uint64_t V = g->V;
+  const Csr& in = ensure_csr(g, LL_GPU_DIR_IN, level);
+  const RowBlocks& rb = ensure_rowblocks(g, LL_GPU_DIR_IN, level, kCap, kPiece);
+  DevBuf vis, fr, nx, part, found;
+  vis.alloc(V * sizeof(W), s);
+  fr.alloc(V * sizeof(W), s);
+  nx.alloc(V * sizeof(W), s);
+  part.alloc(std::max<uint64_t>(rb.n, 1) * sizeof(W), s);
+  found.alloc(8, s);
+  const W full = Lane<W>::low(Lane<W>::kBits);
+  const W used = Lane<W>::low(n);
+  k_ms_init<W><<<grid_for(V, 256), 256, 0, s>>>(vis.as<W>(), fr.as<W>(), V, full & ~used);
+  k_ms_roots<W><<<1, 1, 0, s>>>(vis.as<W>(), fr.as<W>(), d_roots + src_base, n, d_lv, V, src_base);
+  LLG_HIP(hipGetLastError());
+  uint64_t lv_count = 0;
+  for (int32_t l = 1;; l++) {
+    MsArgs<W> a;
+    a.items = rb.items.as<RowItem>();
+    a.ofs = in.ofs.as<uint64_t>();
+    a.adj = in.adj.as<uint32_t>();
+    a.fr = fr.as<W>();
+    a.vis = vis.as<W>();
+    a.nx = nx.as<W>();
+    a.part = part.as<W>();
+    a.lv = d_lv;
+    a.V = V;
+    a.next_lv = l;
+    a.src_base = src_base;
+    a.n_src = n;
+    a.full = full;
+    a.found = found.as<unsigned long long>();
+    k_zero_ull<<<1, 1, 0, s>>>(a.found);
+    {
+      TimedLaunch t(ctx, "k_ms_pull");
+      k_ms_pull<W><<<unsigned(rb.n), kBlock, 0, s>>>(a);
+    }
+    k_ms_fixup<W><<<grid_for(rb.n, 256), 256, 0, s>>>(a, rb.n);
+    LLG_HIP(hipGetLastError());
+    uint64_t h = 0;
+    LLG_HIP(hipMemcpyAsync(&h, found.p, 8, hipMemcpyDeviceToHost, s));
+    LLG_HIP(hipStreamSynchronize(s));
+    lv_count++;
+    if (h == 0) break;
+    std::swap(fr, nx);
+  }
+  if (d_reached)
+    k_ms_reached<W><<<grid_for(V, 256), 256, 0, s>>>(vis.as<W>(), V, n, d_reached + src_base);
+  LLG_HIP(hipGetLastError());
+  if (levels_run) *levels_run += lv_count;
+}
+
+// Time one batch per lane width on the leading roots; keep the best time per
+// source for this snapshot shape.
+int autotune_lanes(ll_gpu_graph* g, int level, const int64_t* roots, int n) {
+  const Csr& in = ensure_csr(g, LL_GPU_DIR_IN, level);
+  const std::string key = "bfs_lanes:" + std::to_string(g->V) + ":" + std::to_string(in.E) + ":" +
+                          std::to_string(resolve_level(g, level));
+  auto it = g->ctx->tuned.find(key);
+  if (it != g->ctx->tuned.end()) return it->second;
+  hipStream_t s = g->ctx->stream;
+  DevBuf d_roots, d_reached;
+  const int m = std::min(n, 128);
+  d_roots.alloc(size_t(m) * 8, s);
+  d_reached.alloc(size_t(m) * 8, s);
+  LLG_HIP(hipMemcpyAsync(d_roots.p, roots, size_t(m) * 8, hipMemcpyHostToDevice, s));
+  hipEvent_t e0, e1;
+  LLG_HIP(hipEventCreate(&e0));
+  LLG_HIP(hipEventCreate(&e1));
+  int best = 64;
+  double best_t = 1e300;
+  for (int w : {16, 32, 64, 128}) {
+    const int k = std::min(m, w);
+    LLG_HIP(hipEventRecord(e0, s));
+    switch (w) {
+      case 16: msbfs_batch<uint16_t>(g, level, d_roots.as<int64_t>(), k, 0, nullptr, nullptr, nullptr); break;
+      case 32: msbfs_batch<uint32_t>(g, level, d_roots.as<int64_t>(), k, 0, nullptr, nullptr, nullptr); break;
+      case 64: msbfs_batch<uint64_t>(g, level, d_roots.as<int64_t>(), k, 0, nullptr, nullptr, nullptr); break;
+      default: msbfs_batch<W128>(g, level, d_roots.as<int64_t>(), k, 0, nullptr, nullptr, nullptr); break;
+    }
+    LLG_HIP(hipEventRecord(e1, s));
+    LLG_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    LLG_HIP(hipEventElapsedTime(&ms, e0, e1));
+    // a full batch of w sources costs about the same as k of them
+    const double per_src = double(ms) / double(w);
+    if (per_src < best_t) { best_t = per_src; best = w; }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  g->ctx->tuned[key] = best;
+  return best;
+}
+
+}  // namespace llgpu
+
+extern "C" {
+
+int ll_gpu_bfs_batch_ex(ll_gpu_graph* g, int level, const int64_t* roots, int n, int lane_width,
+                        int32_t* levels_out, uint64_t* reached_out, uint64_t* sweeps) {
+  if (!g) return LL_GPU_EINVAL;
+  using namespace llgpu;
+  try {
+    LLG_HIP(hipSetDevice(g->ctx->device));
+    LLG_CHECK(n >= 0 && (n == 0 || roots), LL_GPU_EINVAL, "bad root list");
+    LLG_CHECK(lane_width == 0 || lane_width == 16 || lane_width == 32 || lane_width == 64 ||
+                  lane_width == 128,
+              LL_GPU_EINVAL, "lane_width must be 0 (autotune), 16, 32, 64 or 128");
+    for (int i = 0; i < n; i++)
+      LLG_CHECK(roots[i] >= 0 && uint64_t(roots[i]) < g->V, LL_GPU_EINVAL,
+                "root outside [0, max_nodes)");
+    if (sweeps) *sweeps = 0;
+    if (n == 0) return LL_GPU_OK;
+    hipStream_t s = g->ctx->stream;
+    if (lane_width == 0) lane_width = autotune_lanes(g, level, roots, n);
+    const uint64_t V = g->V;
+    DevBuf d_roots, d_lv, d_reached;
+    d_roots.alloc(size_t(n) * 8, s);
+    LLG_HIP(hipMemcpyAsync(d_roots.p, roots, size_t(n) * 8, hipMemcpyHostToDevice, s));
+    if (levels_out) {
+      d_lv.alloc(size_t(n) * V * 4, s);
+      k_fill_i32<<<grid_for(uint64_t(n) * V, 256), 256, 0, s>>>(d_lv.as<int32_t>(),
+                                                                uint64_t(n) * V, kUnreached);
+    }
+    d_reached.alloc(size_t(n) * 8, s);
+    k_fill_i32<<<grid_for(uint64_t(n) * 2, 256), 256, 0, s>>>(d_reached.as<int32_t>(),
+                                                            uint64_t(n) * 2, 0);
+    for (int b = 0; b < n; b += lane_width) {
+      const int m = std::min(lane_width, n - b);
+      int32_t* lvp = levels_out ? d_lv.as<int32_t>() : nullptr;
+      unsigned long long* rp = d_reached.as<unsigned long long>();
+      switch (lane_width) {
+        case 16: msbfs_batch<uint16_t>(g, level, d_roots.as<int64_t>(), m, b, lvp, rp, sweeps); break;
+        case 32: msbfs_batch<uint32_t>(g, level, d_roots.as<int64_t>(), m, b, lvp, rp, sweeps); break;
+        case 64: msbfs_batch<uint64_t>(g, level, d_roots.as<int64_t>(), m, b, lvp, rp, sweeps); break;
+        default: msbfs_batch<W128>(g, level, d_roots.as<int64_t>(), m, b, lvp, rp, sweeps); break;
+      }
+    }
+    if (levels_out)
+      LLG_HIP(hipMemcpyAsync(levels_out, d_lv.p, size_t(n) * V * 4, hipMemcpyDeviceToHost, s));
+    if (reached_out)
+      LLG_HIP(hipMemcpyAsync(reached_out, d_reached.p, size_t(n) * 8, hipMemcpyDeviceToHost, s));
+    LLG_HIP(hipStreamSynchronize(s));
+    return LL_GPU_OK;
+  } catch (const Error& e) {
+    g->ctx->err = e.msg;
+    return e.code;
+  }
+}
+
+}  // extern "C"
+
+extern "C" int ll_gpu_bfs_batch(ll_gpu_graph* g, int level, const int64_t* roots, int n,
+                                int lane_width, int32_t* levels_out, uint64_t* reached_out) {
+  return ll_gpu_bfs_batch_ex(g, level, roots, n, lane_width, levels_out, reached_out, nullptr);
+}

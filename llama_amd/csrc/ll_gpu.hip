@@ -106,13 +106,15 @@ void drop_snapshots(ll_gpu_graph* g) {
     g->deg_level[d] = -1;
     g->deg[d].release();
     g->csr[d] = Csr();
-    g->rb[d] = RowBlocks();
+    g->rb[d].clear();
   }
   g->sym_level = -1;
   g->sym_ofs.release();
   g->sym_adj.release();
   g->w_level = -1;
-  g->in2out.release();
+  g->wcsr = Csr();
+  g->wperm.release();
+  g->wrb.clear();
 }
 
 }  // namespace llgpu

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <list>
 #include <map>
 #include <memory>
 #include <string>
@@ -125,9 +126,11 @@ struct RowItem {
 };
 struct RowBlocks {
   bool valid = false;
-  uint32_t cap = 0;
-  uint64_t n = 0;
-  DevBuf items;  // RowItem[n]
+  uint32_t cap = 0, piece = 0;
+  uint64_t n = 0;       // items
+  uint64_t nsplit = 0;  // items of rows split into several pieces
+  DevBuf items;         // RowItem[n]
+  DevBuf slots;         // uint32_t[n]: partials slot of split-row items, else ~0
 };
 
 struct DirMirror {
@@ -151,13 +154,16 @@ struct ll_gpu_graph {
   int deg_level[2] = {-1, -1};
   llgpu::DevBuf deg[2];       // uint32_t[V]
   llgpu::Csr csr[2];
-  llgpu::RowBlocks rb[2];
+  std::list<llgpu::RowBlocks> rb[2];  // per direction, one per (cap, piece)
   // derived structures (built on demand per snapshot level)
   int sym_level = -1;
   llgpu::DevBuf sym_ofs, sym_adj;  // degree-oriented DAG for triangle counting
   uint64_t sym_edges = 0;
+  // weighted in-CSR (rows = heads, sources ascending) + map to out-edge positions
   int w_level = -1;
-  llgpu::DevBuf in2out;       // uint32/64 map in-edge position -> out-edge position
+  llgpu::Csr wcsr;
+  llgpu::DevBuf wperm;        // uint32_t[E]: weighted in-edge -> out-edge position
+  std::list<llgpu::RowBlocks> wrb;
 };
 
 namespace llgpu {
@@ -171,10 +177,24 @@ struct TimedLaunch {
   ~TimedLaunch();
 };
 
+struct BfsResult {
+  uint64_t reached = 0, examined = 0, traversed = 0;
+  int levels = 0;
+};
+// Direction-optimising BFS from `root` over `fwd` (top-down) / `rev`
+// (bottom-up); `fdeg` = degrees of `fwd`; `excluded` (bitmap or null) is never
+// entered; `visited` receives the reached set (plus excluded/padding bits);
+// `lv` (int32[V] or null) the levels.
+BfsResult bfs_core(ll_gpu_graph* g, const Csr& fwd, const Csr& rev, const uint32_t* fdeg,
+                   uint32_t root, const uint64_t* excluded, DevBuf& visited, int32_t* lv);
+
 int resolve_level(const ll_gpu_graph* g, int level);
 const uint32_t* ensure_degrees(ll_gpu_graph* g, uint32_t dir, int level);
 const Csr& ensure_csr(ll_gpu_graph* g, uint32_t dir, int level);
-const RowBlocks& ensure_rowblocks(ll_gpu_graph* g, uint32_t dir, int level, uint32_t cap);
+const RowBlocks& ensure_rowblocks(ll_gpu_graph* g, uint32_t dir, int level, uint32_t cap,
+                                  uint32_t piece);
+void build_rowblocks(ll_gpu_ctx* ctx, const Csr& c, uint32_t cap, uint32_t piece,
+                     RowBlocks& rb);
 void drop_snapshots(ll_gpu_graph* g);
 
 // Device-wide exclusive scan of n values into out (n+1 entries, total last).

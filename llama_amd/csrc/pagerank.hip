@@ -165,7 +165,7 @@ void pagerank_device(ll_gpu_graph* g, int level, int iters, double damping, uint
   const uint64_t V = g->V;
   const Csr& in = ensure_csr(g, LL_GPU_DIR_IN, level);
   const uint32_t* outdeg = ensure_degrees(g, LL_GPU_DIR_OUT, level);
-  const RowBlocks& rb = ensure_rowblocks(g, LL_GPU_DIR_IN, level, cap);
+  const RowBlocks& rb = ensure_rowblocks(g, LL_GPU_DIR_IN, level, cap, kPiece);
   DevBuf rank[2], contrib[2], part, diff;
   for (int i = 0; i < 2; i++) {
     rank[i].alloc(V * 4, s);

@@ -6,7 +6,7 @@
 namespace llgpu {
 
 constexpr uint32_t kRowsPerItem = 256;   // rows of a stream item (one thread each)
-constexpr uint32_t kPiece = 8192;        // entries of one piece of a long row
+constexpr uint32_t kPiece = 8192;        // entries of one piece of a long row (PR, BFS)
 constexpr uint32_t kMaxCap = 4096;       // largest stream capacity (LDS: 1.5*cap values)
 
 // Neumaier compensated summation in FP32.

@@ -31,6 +31,8 @@ constexpr int kBlock = 256;
 constexpr int kWordsPerBlock = 2048;
 constexpr int kWaves = kBlock / 64;
 
+__global__ void k_zero_u32(uint32_t* p) { *p = 0; }
+
 __global__ void k_degrees(const VtEntry* vt, uint32_t* deg, uint64_t V) {
   uint64_t v = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (v < V) deg[v] = vt[v].degree;
@@ -248,7 +250,7 @@ const Csr& ensure_csr(ll_gpu_graph* g, uint32_t dir, int level) {
   nseg.alloc(V * 4, s);
   nwords.alloc(V * 8, s);
   bad.alloc(4, s);
-  LLG_HIP(hipMemsetAsync(bad.p, 0, 4, s));
+  k_zero_u32<<<1, 1, 0, s>>>(bad.as<uint32_t>());
   {
     TimedLaunch t(ctx, "k_seg_count");
     k_seg_count<<<grid_for(V, 256), 256, 0, s>>>(vt, et, et_words, uint32_t(level), V,

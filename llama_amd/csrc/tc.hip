@@ -1,0 +1,373 @@
+// Triangle counting on the snapshot's simple undirected graph (tasks tc_u /
+// tc_od, benchmark/benchmarks/triangle_counting.h:343-470, 682-831).
+//
+// The reference needs a single sorted, deduplicated level; here the snapshot's
+// live out-edges are symmetrised and oriented on the device:
+//   1. every live edge {u,v}, u != v, becomes the arc a -> b with
+//      (deg(a), a) < (deg(b), b), deg = out-degree + in-degree (any strict
+//      total order counts each triangle once; degree order bounds the arc
+//      lists by O(sqrt(E))), encoded as the 64-bit key a<<32|b;
+//   2. keys are radix-sorted and deduplicated: the oriented DAG as a CSR with
+//      ascending neighbour lists;
+//   3. for each arc (a,b) count |N+(a) & N+(b)|: one wave per row a stages
+//      N+(a) and the prefix sums of d+(b) over b in N+(a) in LDS, then streams
+//      the concatenated lists N+(b) 64 entries at a time (coalesced within each
+//      list) and binary-searches every entry in the LDS copy of N+(a).  Rows
+//      too long for a wave's LDS run on a whole workgroup, and beyond that
+//      search N+(a) in global memory.
+// Every triangle a<b<c (in that order) is counted once, at arc (a,b).
+#include "ll_gpu_internal.h"
+
+namespace llgpu {
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr uint32_t kWaveRow = 512;    // row length handled by one wave (LDS)
+constexpr uint32_t kBlockRow = 4096;  // row length handled by one workgroup (LDS)
+
+__global__ void k_orient(const uint64_t* ofs, const uint32_t* adj, const uint32_t* dout,
+                         const uint32_t* din, uint64_t V, uint64_t* keys, uint64_t E) {
+  // one thread per vertex row; arcs written at the row's own positions
+  const uint64_t u = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (u >= V) return;
+  const uint64_t du = uint64_t(dout[u]) + din[u];
+  for (uint64_t e = ofs[u], end = ofs[u + 1]; e < end; e++) {
+    const uint64_t v = adj[e];
+    uint64_t key = ~0ull;  // self-loops sort last and are cut
+    if (v != u) {
+      const uint64_t dv = uint64_t(dout[v]) + din[v];
+      const bool uv = du < dv || (du == dv && u < v);
+      key = uv ? (u << 32) | v : (v << 32) | u;
+    }
+    keys[e] = key;
+  }
+  (void)E;
+}
+
+// Row offsets of the sorted arc keys: ofs[x] = first arc with tail >= x.
+__global__ void k_dag_offsets(const uint64_t* keys, uint64_t m, uint64_t V, uint64_t* ofs,
+                              uint32_t* nbr) {
+  const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i > m) return;
+  const uint64_t a = i < m ? keys[i] >> 32 : V;
+  const uint64_t p = i == 0 ? 0 : (keys[i - 1] >> 32) + 1;
+  for (uint64_t x = p; x <= a && x <= V; x++) ofs[x] = i;
+  if (i < m) nbr[i] = uint32_t(keys[i]);
+}
+
+// Make a wave's LDS writes visible to its other lanes before they read them.
+__device__ inline void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
+__device__ inline bool lds_has(const uint32_t* list, uint32_t n, uint32_t x) {
+  uint32_t lo = 0, hi = n;
+  while (lo < hi) {
+    uint32_t mid = (lo + hi) >> 1;
+    if (list[mid] < x) lo = mid + 1; else hi = mid;
+  }
+  return lo < n && list[lo] == x;
+}
+
+__device__ inline bool global_has(const uint32_t* list, uint64_t n, uint32_t x) {
+  uint64_t lo = 0, hi = n;
+  while (lo < hi) {
+    uint64_t mid = (lo + hi) >> 1;
+    if (list[mid] < x) lo = mid + 1; else hi = mid;
+  }
+  return lo < n && list[lo] == x;
+}
+
+// Count common out-neighbours for all arcs of row a, with `lanes` threads
+// cooperating (a wave or a workgroup); `na` = d+(a).  `pre` holds inclusive
+// prefix sums of d+(b) for the row's b's (in LDS when na fits).
+template <bool kLds>
+__device__ inline unsigned long long count_row(const uint64_t* ofs, const uint32_t* nbr,
+                                               uint64_t a, const uint32_t* la, uint32_t na,
+                                               const uint32_t* pre, uint32_t tid, uint32_t lanes) {
+  const uint64_t a0 = ofs[a];
+  const uint32_t work = na ? pre[na - 1] : 0;
+  unsigned long long c = 0;
+  for (uint32_t x = tid; x < work; x += lanes) {
+    // arc index i with pre[i-1] <= x < pre[i]
+    uint32_t lo = 0, hi = na;
+    while (lo < hi) {
+      uint32_t mid = (lo + hi) >> 1;
+      if (pre[mid] <= x) lo = mid + 1; else hi = mid;
+    }
+    const uint32_t b = kLds ? la[lo] : nbr[a0 + lo];
+    const uint32_t j = x - (lo ? pre[lo - 1] : 0);
+    const uint32_t w = nbr[ofs[b] + j];
+    c += kLds ? lds_has(la, na, w) : global_has(nbr + a0, na, w);
+  }
+  return c;
+}
+
+__global__ __launch_bounds__(kBlock) void k_tc_wave(const uint64_t* ofs, const uint32_t* nbr,
+                                                    uint64_t V, unsigned long long* partial) {
+  __shared__ uint32_t l_a[kBlock / 64][kWaveRow];
+  __shared__ uint32_t l_pre[kBlock / 64][kWaveRow];
+  __shared__ unsigned long long red[kBlock / 64];
+  const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const uint64_t nwaves = uint64_t(gridDim.x) * (kBlock / 64);
+  unsigned long long c = 0;
+  for (uint64_t a = uint64_t(blockIdx.x) * (kBlock / 64) + wave; a < V; a += nwaves) {
+    const uint64_t a0 = ofs[a];
+    const uint32_t na = uint32_t(ofs[a + 1] - a0);
+    if (na < 2 || na > kWaveRow) continue;  // a single arc closes no triangle
+    for (uint32_t i = lane; i < na; i += 64) {
+      const uint32_t b = nbr[a0 + i];
+      l_a[wave][i] = b;
+      l_pre[wave][i] = uint32_t(ofs[b + 1] - ofs[b]);
+    }
+    wave_sync();
+    // inclusive scan of d+(b) in chunks of 64
+    uint32_t carry = 0;
+    for (uint32_t base = 0; base < na; base += 64) {
+      const uint32_t i = base + lane;
+      uint32_t v = i < na ? l_pre[wave][i] : 0;
+      for (int o = 1; o < 64; o <<= 1) {
+        uint32_t t = __shfl_up(v, o, 64);
+        if (lane >= uint32_t(o)) v += t;
+      }
+      if (i < na) l_pre[wave][i] = v + carry;
+      carry += __shfl(v, 63, 64);
+    }
+    wave_sync();
+    c += count_row<true>(ofs, nbr, a, l_a[wave], na, l_pre[wave], lane, 64);
+    wave_sync();
+  }
+  for (int o = 32; o > 0; o >>= 1) c += __shfl_down(c, o, 64);
+  if (lane == 0) red[wave] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long t = 0;
+    for (int i = 0; i < kBlock / 64; i++) t += red[i];
+    partial[blockIdx.x] = t;
+  }
+}
+
+// Long rows (d+(a) > kWaveRow), one workgroup each.
+__global__ __launch_bounds__(kBlock) void k_tc_block(const uint64_t* ofs, const uint32_t* nbr,
+                                                     const uint32_t* rows, uint64_t nrows,
+                                                     uint32_t* gpre, unsigned long long* partial) {
+  __shared__ uint32_t l_a[kBlockRow];
+  __shared__ uint32_t l_pre[kBlockRow];
+  __shared__ uint32_t wtot[kBlock / 64];
+  __shared__ unsigned long long red[kBlock / 64];
+  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  unsigned long long c = 0;
+  for (uint64_t r = blockIdx.x; r < nrows; r += gridDim.x) {
+    const uint64_t a = rows[r];
+    const uint64_t a0 = ofs[a];
+    const uint32_t na = uint32_t(ofs[a + 1] - a0);
+    const bool lds = na <= kBlockRow;
+    uint32_t* pre = lds ? l_pre : gpre + a0;  // global prefix slot = the row's own arc range
+    for (uint32_t i = threadIdx.x; i < na; i += kBlock) {
+      const uint32_t b = nbr[a0 + i];
+      if (lds) l_a[i] = b;
+      pre[i] = uint32_t(ofs[b + 1] - ofs[b]);
+    }
+    __syncthreads();
+    // block-wide inclusive scan over na entries in chunks of kBlock
+    uint32_t carry = 0;
+    for (uint32_t base = 0; base < na; base += kBlock) {
+      const uint32_t i = base + threadIdx.x;
+      uint32_t v = i < na ? pre[i] : 0;
+      for (int o = 1; o < 64; o <<= 1) {
+        uint32_t t = __shfl_up(v, o, 64);
+        if (lane >= uint32_t(o)) v += t;
+      }
+      if (lane == 63) wtot[wave] = v;
+      __syncthreads();
+      uint32_t off = carry;
+      for (uint32_t k = 0; k < wave; k++) off += wtot[k];
+      uint32_t chunk = 0;
+      for (uint32_t k = 0; k < kBlock / 64; k++) chunk += wtot[k];
+      if (i < na) pre[i] = v + off;
+      carry += chunk;
+      __syncthreads();
+    }
+    if (lds) c += count_row<true>(ofs, nbr, a, l_a, na, l_pre, threadIdx.x, kBlock);
+    else c += count_row<false>(ofs, nbr, a, nullptr, na, pre, threadIdx.x, kBlock);
+    __syncthreads();
+  }
+  for (int o = 32; o > 0; o >>= 1) c += __shfl_down(c, o, 64);
+  if (lane == 0) red[wave] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long t = 0;
+    for (int i = 0; i < kBlock / 64; i++) t += red[i];
+    partial[blockIdx.x] = t;
+  }
+}
+
+__global__ void k_long_rows(const uint64_t* ofs, uint64_t V, uint32_t* rows,
+                            unsigned long long* n, unsigned long long* work) {
+  const uint64_t a = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (a >= V) return;
+  const uint64_t na = ofs[a + 1] - ofs[a];
+  if (na > kWaveRow) rows[atomicAdd(n, 1ull)] = uint32_t(a);
+  (void)work;
+}
+
+// W = sum over arcs (a,b) of (d+(a) + d+(b)): intersection element reads.
+__global__ void k_tc_work(const uint64_t* ofs, const uint32_t* nbr, uint64_t V,
+                          unsigned long long* work) {
+  const uint64_t a = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  unsigned long long w = 0;
+  if (a < V) {
+    const uint64_t a0 = ofs[a], na = ofs[a + 1] - a0;
+    for (uint64_t i = 0; i < na; i++) {
+      const uint32_t b = nbr[a0 + i];
+      w += na + (ofs[b + 1] - ofs[b]);
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) w += __shfl_down(w, o, 64);
+  if ((threadIdx.x & 63) == 0 && w) atomicAdd(work, w);
+}
+
+__global__ void k_sum_partials(const unsigned long long* p, uint64_t n, unsigned long long* out) {
+  unsigned long long s = 0;
+  for (uint64_t i = threadIdx.x; i < n; i += blockDim.x) s += p[i];
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+  __shared__ unsigned long long red[16];
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long t = 0;
+    for (unsigned i = 0; i < blockDim.x / 64; i++) t += red[i];
+    *out = t;
+  }
+}
+
+__global__ void k_zero_ull2(unsigned long long* p, int n) {
+  for (int i = 0; i < n; i++) p[i] = 0;
+}
+
+}  // namespace
+
+struct TcResult {
+  uint64_t triangles = 0, arcs = 0, work = 0;
+};
+
+// Build (or reuse) the oriented DAG of the snapshot.
+void ensure_dag(ll_gpu_graph* g, int level) {
+  level = resolve_level(g, level);
+  if (g->sym_level == level) return;
+  ll_gpu_ctx* ctx = g->ctx;
+  hipStream_t s = ctx->stream;
+  const uint64_t V = g->V;
+  const Csr& out = ensure_csr(g, LL_GPU_DIR_OUT, level);
+  const uint32_t* dout = ensure_degrees(g, LL_GPU_DIR_OUT, level);
+  const uint32_t* din = ensure_degrees(g, LL_GPU_DIR_IN, level);
+  const uint64_t E = out.E;
+  DevBuf keys, sorted;
+  keys.alloc(std::max<uint64_t>(E, 1) * 8, s);
+  {
+    TimedLaunch t(ctx, "k_orient");
+    k_orient<<<grid_for(V, 256), 256, 0, s>>>(out.ofs.as<uint64_t>(), out.adj.as<uint32_t>(),
+                                              dout, din, V, keys.as<uint64_t>(), E);
+  }
+  LLG_HIP(hipGetLastError());
+  sorted.alloc(std::max<uint64_t>(E, 1) * 8, s);
+  sort_keys_u64(keys.as<uint64_t>(), sorted.as<uint64_t>(), E, 64, s);
+  uint64_t m = unique_u64(sorted.as<uint64_t>(), keys.as<uint64_t>(), E, s);
+  // drop the self-loop sentinel (all ones) if present: it sorts last
+  if (m > 0) {
+    uint64_t last = 0;
+    LLG_HIP(hipMemcpyAsync(&last, keys.as<uint64_t>() + (m - 1), 8, hipMemcpyDeviceToHost, s));
+    LLG_HIP(hipStreamSynchronize(s));
+    if (last == ~0ull) m--;
+  }
+  sorted.release();
+  g->sym_ofs.alloc((V + 1) * 8, s);
+  g->sym_adj.alloc(std::max<uint64_t>(m, 1) * 4, s);
+  k_dag_offsets<<<grid_for(m + 1, 256), 256, 0, s>>>(keys.as<uint64_t>(), m, V,
+                                                     g->sym_ofs.as<uint64_t>(),
+                                                     g->sym_adj.as<uint32_t>());
+  LLG_HIP(hipGetLastError());
+  g->sym_edges = m;
+  g->sym_level = level;
+}
+
+TcResult tc_device(ll_gpu_graph* g, int level) {
+  ll_gpu_ctx* ctx = g->ctx;
+  hipStream_t s = ctx->stream;
+  const uint64_t V = g->V;
+  ensure_dag(g, level);
+  const uint64_t* ofs = g->sym_ofs.as<uint64_t>();
+  const uint32_t* nbr = g->sym_adj.as<uint32_t>();
+  TcResult r;
+  r.arcs = g->sym_edges;
+  DevBuf cnt, rows, gpre, part_w, part_b, total;
+  cnt.alloc(3 * 8, s);
+  k_zero_ull2<<<1, 1, 0, s>>>(cnt.as<unsigned long long>(), 3);
+  rows.alloc(V * 4, s);
+  k_long_rows<<<grid_for(V, 256), 256, 0, s>>>(ofs, V, rows.as<uint32_t>(),
+                                               cnt.as<unsigned long long>(), nullptr);
+  k_tc_work<<<grid_for(V, 256), 256, 0, s>>>(ofs, nbr, V, cnt.as<unsigned long long>() + 1);
+  LLG_HIP(hipGetLastError());
+  unsigned long long h[2];
+  LLG_HIP(hipMemcpyAsync(h, cnt.p, 16, hipMemcpyDeviceToHost, s));
+  LLG_HIP(hipStreamSynchronize(s));
+  const uint64_t nlong = h[0];
+  r.work = h[1];
+  const unsigned gw = unsigned(std::min<uint64_t>(std::max<uint64_t>((V + 3) / 4, 1),
+                                                  uint64_t(ctx->num_cus) * 32));
+  part_w.alloc(gw * 8, s);
+  {
+    TimedLaunch t(ctx, "k_tc_wave");
+    k_tc_wave<<<gw, kBlock, 0, s>>>(ofs, nbr, V, part_w.as<unsigned long long>());
+  }
+  const unsigned gb = unsigned(std::min<uint64_t>(std::max<uint64_t>(nlong, 1),
+                                                  uint64_t(ctx->num_cus) * 8));
+  part_b.alloc(gb * 8, s);
+  if (nlong) {
+    gpre.alloc(std::max<uint64_t>(g->sym_edges, 1) * 4, s);
+    TimedLaunch t(ctx, "k_tc_block");
+    k_tc_block<<<gb, kBlock, 0, s>>>(ofs, nbr, rows.as<uint32_t>(), nlong, gpre.as<uint32_t>(),
+                                     part_b.as<unsigned long long>());
+  } else {
+    k_zero_ull2<<<1, 1, 0, s>>>(part_b.as<unsigned long long>(), int(gb));
+  }
+  LLG_HIP(hipGetLastError());
+  total.alloc(16, s);
+  k_sum_partials<<<1, 1024, 0, s>>>(part_w.as<unsigned long long>(), gw,
+                                    total.as<unsigned long long>());
+  k_sum_partials<<<1, 1024, 0, s>>>(part_b.as<unsigned long long>(), gb,
+                                    total.as<unsigned long long>() + 1);
+  LLG_HIP(hipGetLastError());
+  LLG_HIP(hipMemcpyAsync(h, total.p, 16, hipMemcpyDeviceToHost, s));
+  LLG_HIP(hipStreamSynchronize(s));
+  r.triangles = h[0] + h[1];
+  return r;
+}
+
+}  // namespace llgpu
+
+extern "C" {
+
+int ll_gpu_tc_ex(ll_gpu_graph* g, int level, uint64_t* triangles, uint64_t* arcs,
+                 uint64_t* work) {
+  if (!g || !triangles) return LL_GPU_EINVAL;
+  try {
+    LLG_HIP(hipSetDevice(g->ctx->device));
+    llgpu::TcResult r = llgpu::tc_device(g, level);
+    *triangles = r.triangles;
+    if (arcs) *arcs = r.arcs;
+    if (work) *work = r.work;
+    return LL_GPU_OK;
+  } catch (const llgpu::Error& e) {
+    g->ctx->err = e.msg;
+    return e.code;
+  }
+}
+
+int ll_gpu_tc(ll_gpu_graph* g, int level, uint64_t* triangles) {
+  return ll_gpu_tc_ex(g, level, triangles, nullptr, nullptr);
+}
+
+}  // extern "C"

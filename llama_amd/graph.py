@@ -323,6 +323,75 @@ class DeviceGraph:
             C.byref(diff)))
         return out, float(diff.value)
 
+    def bfs(self, root: int, level: int = -1, want_levels: bool = True):
+        """Direction-optimising BFS: (levels or None, reached, edges examined)."""
+        lv = np.zeros(self.V, np.int32) if want_levels else None
+        reached, exam, trav = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        self.ctx.check(self.lib.ll_gpu_bfs_ex(
+            self._h, level, root, lv.ctypes.data_as(C.POINTER(C.c_int32)) if lv is not None
+            else None, C.byref(reached), C.byref(exam), C.byref(trav)))
+        self.last_bfs_traversed = int(trav.value)
+        return lv, int(reached.value), int(exam.value)
+
+    def bfs_batch(self, roots, lane_width: int = 64, level: int = -1, want_levels: bool = True):
+        """Multi-source BFS: (levels [n x V] or None, reached [n], pull sweeps)."""
+        r = np.ascontiguousarray(roots, dtype=np.int64)
+        n = r.size
+        lv = np.zeros((n, self.V), np.int32) if want_levels else None
+        reached = np.zeros(n, np.uint64)
+        sweeps = C.c_uint64()
+        self.ctx.check(self.lib.ll_gpu_bfs_batch_ex(
+            self._h, level, r.ctypes.data_as(C.POINTER(C.c_int64)), n, lane_width,
+            lv.ctypes.data_as(C.POINTER(C.c_int32)) if lv is not None else None,
+            reached.ctypes.data_as(C.POINTER(C.c_uint64)), C.byref(sweeps)))
+        return lv, reached, int(sweeps.value)
+
+    def triangles(self, level: int = -1, stats: bool = False):
+        t, arcs, work = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        self.ctx.check(self.lib.ll_gpu_tc_ex(self._h, level, C.byref(t), C.byref(arcs),
+                                             C.byref(work)))
+        if stats:
+            return int(t.value), int(arcs.value), int(work.value)
+        return int(t.value)
+
+    def scc(self, level: int = -1, want_labels: bool = True):
+        """Canonical SCC labels (min vertex ID per component) and the count."""
+        lab = np.zeros(self.V, np.int64) if want_labels else None
+        n, sweeps = C.c_uint64(), C.c_uint64()
+        self.ctx.check(self.lib.ll_gpu_scc_ex(
+            self._h, level, lab.ctypes.data_as(C.POINTER(C.c_int64)) if lab is not None else None,
+            C.byref(n), C.byref(sweeps)))
+        self.last_scc_sweeps = int(sweeps.value)
+        return lab, int(n.value)
+
+    def ppr_batch(self, seeds, iters: int = 30, damping: float = 0.85, tol: float = 1e-7,
+                  batch_width: int = 0, level: int = -1, want_ranks: bool = True):
+        """Batched personalized PageRank: (ranks [V x n] FP32 or None, iterations run)."""
+        sd = np.ascontiguousarray(seeds, dtype=np.int64)
+        out = np.zeros((self.V, sd.size), np.float32) if want_ranks else None
+        done = C.c_int()
+        self.ctx.check(self.lib.ll_gpu_ppr_batch(
+            self._h, level, sd.ctypes.data_as(C.POINTER(C.c_int64)), sd.size, iters, damping,
+            tol, batch_width, out.ctypes.data_as(C.POINTER(C.c_float)) if out is not None
+            else None, C.byref(done)))
+        return out, int(done.value)
+
+    def tuned(self, prefix: str) -> int | None:
+        v = C.c_int()
+        rc = self.lib.ll_gpu_tuned(self.ctx._h, prefix.encode(), C.byref(v))
+        return int(v.value) if rc == 0 else None
+
+    def wpr_forward(self, w_ptr: int, p_ptr: int, r_ptr: int, iters: int, damping: float,
+                    level: int = -1) -> None:
+        """Weighted PageRank forward on device pointers (see ll_gpu.h)."""
+        self.ctx.check(self.lib.ll_gpu_wpr_forward(self._h, level, w_ptr, p_ptr, iters, damping,
+                                                   r_ptr))
+
+    def wpr_adjoint(self, w_ptr: int, r_ptr: int, g_ptr: int, iters: int, damping: float,
+                    dw_ptr: int, dp_ptr: int, level: int = -1) -> None:
+        self.ctx.check(self.lib.ll_gpu_wpr_adjoint(self._h, level, w_ptr, r_ptr, g_ptr, iters,
+                                                   damping, dw_ptr, dp_ptr))
+
     def close(self) -> None:
         if getattr(self, "_h", None) and self._h.value:
             self.lib.ll_gpu_graph_destroy(self._h)
