@@ -82,9 +82,21 @@ def build_gpu(jobs: int = 8) -> Path:
     return out
 
 
+def build_cli() -> Path:
+    """ll_gpu_bench: the C++ task driver (links both libraries, rpath $ORIGIN)."""
+    out = PKG / "ll_gpu_bench"
+    tmp = BUILD / "ll_gpu_bench.tmp"
+    _run(["g++", *HOST_FLAGS, f"-I{CSRC / 'host'}", str(CSRC / "host" / "ll_gpu_bench.cc"),
+          "-o", str(tmp), f"-L{PKG}", "-lll_host", "-lll_gpu", "-Wl,-rpath,$ORIGIN",
+          "-Wl,--allow-shlib-undefined"])
+    os.replace(tmp, out)
+    return out
+
+
 def build_all(jobs: int = 8) -> None:
     build_host()
     build_gpu(jobs)
+    build_cli()
 
 
 def clean() -> None:

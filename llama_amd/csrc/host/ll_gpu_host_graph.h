@@ -1,0 +1,39 @@
+// Graph-access overloads that let the ll_gpu tasks (ll_gpu_tasks.h) run on the
+// host layout of this package (llhost::Graph, ll_layout.h).
+#pragma once
+
+#include <cstring>
+#include <vector>
+
+#include "ll_gpu.h"
+#include "ll_host.h"
+#include "ll_layout.h"
+
+namespace llhost {
+// The C++ graph behind a C-ABI handle (defined in ll_host.cc).
+const Graph& graph_of(const ll_host_graph* h);
+}  // namespace llhost
+
+inline uint64_t ll_gpu_max_nodes(const llhost::Graph& G) { return G.num_nodes(); }
+inline int ll_gpu_num_levels(const llhost::Graph& G) { return G.num_levels(); }
+inline uint32_t ll_gpu_flags(const llhost::Graph& G) {
+  return LL_GPU_HAS_CONT | (G.deletions() ? LL_GPU_HAS_DELETIONS : 0u);
+}
+inline void ll_gpu_view(const llhost::Graph& G, uint32_t dir, int level, ll_gpu_level_view* v) {
+  const llhost::Level& lv = G.level(int(dir), level);
+  std::memset(v, 0, sizeof(*v));
+  v->dir = dir;
+  v->level = uint32_t(level);
+  v->num_nodes = G.num_nodes();
+  v->num_vt_entries = lv.vt.size();
+  v->num_et_words = lv.et.size();
+  v->vt = lv.vt.data();
+  v->et = lv.et.data();
+  v->flags = ll_gpu_flags(G);
+}
+inline void ll_gpu_patches(const llhost::Graph& G, uint32_t dir, int level,
+                           std::vector<uint64_t>& ids, std::vector<uint32_t>& maxlev) {
+  const llhost::PatchList& p = G.patches(int(dir), level);
+  ids.assign(p.edge_ids.begin(), p.edge_ids.end());
+  maxlev.assign(p.maxlev.begin(), p.maxlev.end());
+}

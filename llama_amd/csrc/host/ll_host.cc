@@ -24,6 +24,10 @@ struct ll_host_graph {
   llhost::vec<uint32_t> flat_adj;
 };
 
+namespace llhost {
+const Graph& graph_of(const ll_host_graph* h) { return *h->g; }
+}  // namespace llhost
+
 namespace {
 thread_local std::string g_err;
 int fail(int code, const std::string& msg) {

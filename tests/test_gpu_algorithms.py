@@ -205,3 +205,43 @@ def test_wpr_forward_and_adjoint_vs_fp64(ctx):
     dp = pt.grad.cpu().numpy().astype(np.float64)
     assert np.abs(dw - dw_ref).sum() / np.abs(dw_ref).sum() <= 1e-4
     assert np.abs(dp - dp_ref).sum() / np.abs(dp_ref).sum() <= 1e-4
+
+
+# ---- C++ task driver (ll_gpu_bench) ----------------------------------------------------
+def test_cpp_task_driver_matches_oracle(tmp_path):
+    """The C++ task classes (reference task interface) through the CLI driver:
+    one level per .xs1 file, results printed in the reference format."""
+    import subprocess
+    from pathlib import Path
+    from llama_amd.graph import Edges
+    gpu_case("ll_gpu_bench cli")
+    root = Path(__file__).resolve().parent.parent
+    exe = root / "llama_amd" / "ll_gpu_bench"
+    e0 = Edges.rmat(8, 8, seed=3, perm_seed=4)
+    s, d = e0.arrays()
+    f0 = str(tmp_path / "l0.xs1")
+    e0.write_xs1(f0, 256)
+    e1 = Edges.from_arrays([1, 2, 3], [200, 100, 50])
+    f1 = str(tmp_path / "l1.xs1")
+    e1.write_xs1(f1, 256)
+    out = str(tmp_path / "pr.txt")
+    r = subprocess.run([str(exe), "-r", "gpu_pagerank", "-n", "10", "-c", "2", "-o", out, f0, f1],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    g = HostGraph(256)
+    g.add_level(s, d)
+    g.add_level([1, 2, 3], [200, 100, 50])
+    ref, _ = oracle.OracleGraph.from_host(g).pagerank(10)
+    vals = []
+    for line in open(out):
+        if ":" in line:
+            vals += [float(x) for x in line.split(":")[1].split()]
+    assert np.allclose(vals, ref[:50], atol=1e-8)
+    ret = [l for l in r.stdout.splitlines() if l.startswith("Return value:")][0]
+    assert abs(float(ret.split(":")[1]) - ref.sum()) < 1e-5
+    r2 = subprocess.run([str(exe), "-r", "gpu_sssp_unweighted", "-R", "3", f0, f1],
+                        capture_output=True, text=True, timeout=120)
+    assert r2.returncode == 0, r2.stderr
+    lv, _ = oracle.OracleGraph.from_host(g).bfs(3)
+    ret2 = [l for l in r2.stdout.splitlines() if l.startswith("Return value:")][0]
+    assert int(float(ret2.split(":")[1])) == int(lv[lv < 2**31 - 2].max())
