@@ -77,6 +77,101 @@ def bench_bfs(dg, g, E: int, V: int, args) -> dict:
     }
 
 
+def bench_bfs_batch(dg, g, V: int, args, single: dict | None) -> dict:
+    """Multi-source BFS over the batch sources in 64-bit lanes (and the
+    autotuned width), against the per-source cost of single-source runs."""
+    import numpy as np
+    roots = pick_sources(g, args.batch_sources, args.seed + 4)
+    ctx = dg.ctx
+    out = {"sources": len(roots)}
+    for lanes in (64, 0):
+        dg.bfs_batch(roots[:64], lane_width=lanes, want_levels=False)  # warm + tune
+        ctx.sync()
+        ctx.profile(True)
+        t0 = time.perf_counter()
+        _, reached, sweeps = dg.bfs_batch(roots, lane_width=lanes, want_levels=False)
+        ctx.sync()
+        t = time.perf_counter() - t0
+        n_k, ms_k = ctx.kernel_time("k_ms_pull")
+        ctx.profile(False)
+        key = "lanes64" if lanes == 64 else f"autotuned_lanes{dg.tuned('bfs_lanes:')}"
+        out[key] = {"seconds": round(t, 4), "sweeps": sweeps, "kernel_ms": round(ms_k, 3),
+                    "reached_total": int(reached.sum())}
+    if single:
+        per_src = single["seconds"] / single["sources"]
+        seq = per_src * len(roots)
+        out["sequential_estimate_seconds"] = round(seq, 4)
+        out["speedup_vs_sequential"] = round(seq / out["lanes64"]["seconds"], 2)
+    log(f"batched BFS x{len(roots)}: {out}")
+    return out
+
+
+def bench_ppr(dg, g, E: int, V: int, args) -> dict:
+    """Batched personalized PageRank, 30 iterations, tol 1e-7, autotuned width."""
+    seeds = pick_sources(g, args.ppr_seeds, args.seed + 5)
+    ctx = dg.ctx
+    dg.ppr_batch(seeds[:16], iters=2, tol=0.0, batch_width=0, want_ranks=False)  # autotune
+    width = dg.tuned("ppr:")
+    ctx.sync()
+    t0 = time.perf_counter()
+    _, it = dg.ppr_batch(seeds, iters=30, tol=1e-7, batch_width=width, want_ranks=False)
+    ctx.sync()
+    t = time.perf_counter() - t0
+    b = len(seeds)
+    alg = (4 * E + 4 * b * E + 8 * b * V + 8 * V) * it
+    out = {"seeds": b, "iterations": it, "batch_width": width, "seconds": round(t, 4),
+           "gteps": round(E * b * it / t / 1e9, 3), "alg_gbs": round(alg / t / 1e9, 1),
+           "frac_hbm": round(alg / t / 1e9 / HBM_PEAK_GBS, 4)}
+    log(f"PPR: {out}")
+    return out
+
+
+def bench_wpr(ctx, args) -> dict:
+    """Weighted PageRank forward vs adjoint on R-MAT (single level), weights
+    U(0.1, 1) row-normalised, uniform teleport, 50 iterations each."""
+    import numpy as np
+    import torch
+    from llama_amd.graph import DIR_OUT, DeviceGraph, build_multilevel
+    g = build_multilevel(args.wpr_scale, levels=1, seed=args.seed + 5, deletions=False)
+    dg = DeviceGraph(ctx, g)
+    ofs, adj = dg.adjacency(DIR_OUT)
+    V, E = dg.V, adj.size
+    rng = np.random.default_rng(args.seed + 6)
+    w = rng.uniform(0.1, 1.0, E).astype(np.float32)
+    src = np.repeat(np.arange(V), np.diff(ofs).astype(np.int64))
+    rs = np.bincount(src, weights=w, minlength=V)
+    w = (w / rs[src]).astype(np.float32)
+    dev = torch.device("cuda:0")
+    wt = torch.from_numpy(w).to(dev)
+    pt = torch.full((V,), 1.0 / V, dtype=torch.float32, device=dev)
+    r = torch.empty_like(pt)
+    gr = torch.zeros_like(pt)
+    tgt = torch.from_numpy(rng.choice(V, 1000, replace=False)).to(dev)
+    dw = torch.empty_like(wt)
+    dp = torch.empty_like(pt)
+    K = 50
+    torch.cuda.synchronize()
+    dg.wpr_forward(wt.data_ptr(), pt.data_ptr(), r.data_ptr(), K, 0.85)  # warm
+    t0 = time.perf_counter()
+    dg.wpr_forward(wt.data_ptr(), pt.data_ptr(), r.data_ptr(), K, 0.85)
+    t_fwd = time.perf_counter() - t0
+    gr[tgt] = 2 * r[tgt]
+    torch.cuda.synchronize()
+    dg.wpr_adjoint(wt.data_ptr(), r.data_ptr(), gr.data_ptr(), K, 0.85, dw.data_ptr(),
+                   dp.data_ptr())
+    t0 = time.perf_counter()
+    dg.wpr_adjoint(wt.data_ptr(), r.data_ptr(), gr.data_ptr(), K, 0.85, dw.data_ptr(),
+                   dp.data_ptr())
+    t_adj = time.perf_counter() - t0
+    out = {"graph": f"R-MAT s{args.wpr_scale} EF16, 1 level", "edges": int(E),
+           "iterations": K, "forward_s": round(t_fwd, 4), "adjoint_s": round(t_adj, 4),
+           "adjoint_over_forward": round(t_adj / t_fwd, 3),
+           "forward_alg_gbs": round((12 * E + 16 * V) * K / t_fwd / 1e9, 1)}
+    log(f"WPR: {out}")
+    dg.close()
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -87,11 +182,21 @@ def main() -> int:
     ap.add_argument("--edge-factor", type=float, default=16.0)
     ap.add_argument("--iters", type=int, default=20)
     ap.add_argument("--cap", type=int, default=0, help="row-block capacity (0 = default)")
-    ap.add_argument("--cpu-iters", type=int, default=2,
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="run a plain device copy of this size first (calibrates rocprof "
+                         "FETCH_SIZE/WRITE_SIZE against known bytes)")
+    ap.add_argument("--reorder", type=int, default=0,
+                    help="1 = degree-ordered contribution slots in PageRank")
+    ap.add_argument("--cpu-iters", type=int, default=6,
                     help="PageRank iterations timed for the CPU baseline (0 = skip)")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--bfs-sources", type=int, default=64,
                     help="single-source DO-BFS runs timed (0 = skip)")
+    ap.add_argument("--batch-sources", type=int, default=512,
+                    help="multi-source BFS sources (0 = skip)")
+    ap.add_argument("--ppr-seeds", type=int, default=128, help="batched PPR seeds (0 = skip)")
+    ap.add_argument("--wpr-scale", type=int, default=23,
+                    help="weighted PageRank + adjoint graph scale, single level (0 = skip)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -110,6 +215,8 @@ def main() -> int:
     log(f"host graph built in {time.time() - t0:.1f}s")
 
     ctx = GpuContext(0)
+    if args.reorder:
+        ctx.set_option("pr_reorder", 1)
     t1 = time.time()
     dg = DeviceGraph(ctx, g)
     ctx.sync()
@@ -120,6 +227,17 @@ def main() -> int:
     E, V = int(info.live_edges), dg.V
     log(f"snapshot: V={V} E_live={E} stored={info.stored_words} segments={info.segments} "
         f"continuations={info.continuations} materialize={info.materialize_ms:.2f}ms")
+
+    if args.copy_probe_mib:
+        import torch
+        n = args.copy_probe_mib << 20
+        src = torch.ones(n, dtype=torch.uint8, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(2):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copy probe: 2 x {args.copy_probe_mib} MiB device copy")
+        del src, dst
 
     def step():
         dg.drop_snapshots()
@@ -138,7 +256,7 @@ def main() -> int:
     n_pr, ms_pr = ctx.kernel_time("k_pr_pull")
     n_fx, ms_fx = ctx.kernel_time("k_pr_fixup")
     mat = {k: ctx.kernel_time(k) for k in ("k_seg_count", "k_seg_fill", "k_live_count",
-                                           "k_compact", "k_degrees")}
+                                           "k_compact", "k_degrees", "pr_reorder")}
     ctx.profile(False)
     log(f"timed {args.steps} steps done")
     step_ms = (te - ts) * 1e3 / args.steps
@@ -150,6 +268,14 @@ def main() -> int:
     secondary = {}
     if args.bfs_sources > 0:
         secondary["bfs"] = bench_bfs(dg, g, E, V, args)
+    if args.batch_sources > 0:
+        secondary["bfs_batch"] = bench_bfs_batch(dg, g, V, args, secondary.get("bfs"))
+    if args.ppr_seeds > 0:
+        secondary["ppr_batch"] = bench_ppr(dg, g, E, V, args)
+
+    if args.wpr_scale > 0:
+        dg.close()
+        secondary["wpr_adjoint"] = bench_wpr(ctx, args)
 
     cpu = None
     if args.cpu_iters > 0:

@@ -195,6 +195,9 @@ int ll_gpu_profile_get(ll_gpu_ctx* ctx, const char* kernel, uint64_t* launches,
                        double* total_ms);
 /* Autotuner decision whose key starts with `prefix` ("ppr:", "bfs_lanes:"). */
 int ll_gpu_tuned(ll_gpu_ctx* ctx, const char* prefix, int* value);
+/* Tuning knobs: "pr_reorder" (1 = gather PageRank contributions in
+ * descending-out-degree slot order), "pr_cap" (default row-block capacity). */
+int ll_gpu_set_option(ll_gpu_ctx* ctx, const char* name, long value);
 /* Drop the cached snapshot materialisations (the next call rebuilds them). */
 void ll_gpu_graph_drop_snapshots(ll_gpu_graph* g);
 

@@ -62,6 +62,7 @@ GPU_SIGNATURES = {
     "ll_gpu_tc_ex": (C.c_int, [vp, C.c_int, P(u64), P(u64), P(u64)]),
     "ll_gpu_scc_ex": (C.c_int, [vp, C.c_int, P(i64), P(u64), P(u64)]),
     "ll_gpu_tuned": (C.c_int, [vp, cp, P(C.c_int)]),
+    "ll_gpu_set_option": (C.c_int, [vp, cp, C.c_long]),
     "ll_gpu_bfs_batch": (C.c_int, [vp, C.c_int, P(i64), C.c_int, C.c_int, P(i32), P(u64)]),
     "ll_gpu_tc": (C.c_int, [vp, C.c_int, P(u64)]),
     "ll_gpu_scc": (C.c_int, [vp, C.c_int, P(i64), P(u64)]),
@@ -120,10 +121,21 @@ _host = None
 
 
 def gpu() -> C.CDLL:
-    """libll_gpu.so (loads HIP; symbols not yet implemented are skipped)."""
+    """libll_gpu.so.
+
+    When PyTorch is installed it is imported first: torch bundles its own HIP
+    and HSA runtimes with the same SONAMEs (libamdhip64.so.7,
+    libhsa-runtime64.so.1) as /opt/rocm, so loading torch first makes this
+    library bind to the runtime torch uses.  Loading them the other way round
+    puts two HSA runtimes in one process and torch then finds no GPU.
+    """
     global _gpu
     if _gpu is None:
-        _gpu = _load("libll_gpu.so", GPU_SIGNATURES, optional_ok=True)
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
+        _gpu = _load("libll_gpu.so", GPU_SIGNATURES)
     return _gpu
 
 

@@ -111,6 +111,9 @@ void drop_snapshots(ll_gpu_graph* g) {
   g->sym_level = -1;
   g->sym_ofs.release();
   g->sym_adj.release();
+  g->pr_level = -1;
+  g->pr_inv.release();
+  g->pr_adj.release();
   g->w_level = -1;
   g->wcsr = Csr();
   g->wperm.release();
@@ -176,6 +179,18 @@ int ll_gpu_sync(ll_gpu_ctx* c) {
 }
 
 void* ll_gpu_stream(ll_gpu_ctx* c) { return c ? static_cast<void*>(c->stream) : nullptr; }
+
+int ll_gpu_set_option(ll_gpu_ctx* c, const char* name, long value) {
+  if (!c || !name) return LL_GPU_EINVAL;
+  static const char* known[] = {"pr_reorder", "pr_cap"};
+  for (const char* k : known)
+    if (std::strcmp(k, name) == 0) {
+      c->options[name] = value;
+      return LL_GPU_OK;
+    }
+  c->err = std::string("unknown option ") + name;
+  return LL_GPU_EINVAL;
+}
 
 int ll_gpu_profile(ll_gpu_ctx* c, int on) {
   if (!c) return LL_GPU_EINVAL;

@@ -105,7 +105,8 @@ struct ll_gpu_ctx {
   void* pinned = nullptr;      // staging for uploads
   size_t pinned_bytes = 0;
   hipEvent_t pinned_ev[2] = {nullptr, nullptr};
-  std::map<std::string, int> tuned;  // autotuner decisions
+  std::map<std::string, int> tuned;    // autotuner decisions
+  std::map<std::string, long> options; // ll_gpu_set_option knobs
 };
 
 namespace llgpu {
@@ -159,6 +160,9 @@ struct ll_gpu_graph {
   int sym_level = -1;
   llgpu::DevBuf sym_ofs, sym_adj;  // degree-oriented DAG for triangle counting
   uint64_t sym_edges = 0;
+  // PageRank locality order of the in-CSR (option pr_reorder)
+  int pr_level = -1;
+  llgpu::DevBuf pr_inv, pr_adj;
   // weighted in-CSR (rows = heads, sources ascending) + map to out-edge positions
   int w_level = -1;
   llgpu::Csr wcsr;

@@ -29,6 +29,7 @@ struct PrArgs {
   const uint32_t* outdeg;
   float* rank_out;
   float* contrib_out;
+  const uint32_t* inv;    // optional: contribution slot of each vertex (locality order)
   Neu* part;              // per item partials of split rows
   double* diff;           // L1 change accumulator (last iteration only)
   float base, d;
@@ -38,7 +39,7 @@ __device__ inline float finalize(const PrArgs& a, uint32_t r, float S) {
   float v = a.base + a.d * S;
   a.rank_out[r] = v;
   uint32_t od = a.outdeg[r];
-  a.contrib_out[r] = od ? v / float(od) : 0.f;
+  a.contrib_out[a.inv ? a.inv[r] : r] = od ? v / float(od) : 0.f;
   return a.diff ? fabsf(v - a.rank_in[r]) : 0.f;
 }
 
@@ -129,13 +130,29 @@ __global__ void k_pr_fixup(PrArgs a, uint64_t nitems) {
   block_add_diff(a, dsum);
 }
 
-__global__ void k_pr_init(const uint32_t* outdeg, float* rank, float* contrib, uint64_t V,
-                          float r0) {
+__global__ void k_pr_init(const uint32_t* outdeg, const uint32_t* inv, float* rank,
+                          float* contrib, uint64_t V, float r0) {
   uint64_t v = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (v >= V) return;
   rank[v] = r0;
   uint32_t od = outdeg[v];
-  contrib[v] = od ? r0 / float(od) : 0.f;
+  contrib[inv ? inv[v] : v] = od ? r0 / float(od) : 0.f;
+}
+
+// Locality order for the contribution vector: slot = rank of the vertex by
+// descending out-degree, so the entries most edges gather share few cache
+// lines (the hottest ones stay in each XCD's L2).
+__global__ void k_deg_keys(const uint32_t* outdeg, uint64_t* keys, uint64_t V) {
+  uint64_t v = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (v < V) keys[v] = (uint64_t(0xffffffffu - outdeg[v]) << 32) | v;
+}
+__global__ void k_inv_from_sorted(const uint64_t* sorted, uint32_t* inv, uint64_t V) {
+  uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < V) inv[uint32_t(sorted[i])] = uint32_t(i);
+}
+__global__ void k_relabel(const uint32_t* adj, const uint32_t* inv, uint32_t* out, uint64_t E) {
+  uint64_t e = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e < E) out[e] = inv[adj[e]];
 }
 
 __global__ void k_widen(const float* in, double* out, uint64_t n) {
@@ -166,6 +183,29 @@ void pagerank_device(ll_gpu_graph* g, int level, int iters, double damping, uint
   const Csr& in = ensure_csr(g, LL_GPU_DIR_IN, level);
   const uint32_t* outdeg = ensure_degrees(g, LL_GPU_DIR_OUT, level);
   const RowBlocks& rb = ensure_rowblocks(g, LL_GPU_DIR_IN, level, cap, kPiece);
+  const bool reorder = ctx->options.count("pr_reorder") && ctx->options["pr_reorder"] != 0;
+  const uint32_t* adj = in.adj.as<uint32_t>();
+  const uint32_t* inv = nullptr;
+  if (reorder) {
+    if (g->pr_level != in.level) {
+      TimedLaunch t(ctx, "pr_reorder");
+      DevBuf keys, sorted;
+      keys.alloc(V * 8, s);
+      sorted.alloc(V * 8, s);
+      k_deg_keys<<<grid_for(V, 256), 256, 0, s>>>(outdeg, keys.as<uint64_t>(), V);
+      sort_keys_u64(keys.as<uint64_t>(), sorted.as<uint64_t>(), V, 64, s);
+      g->pr_inv.alloc(V * 4, s);
+      k_inv_from_sorted<<<grid_for(V, 256), 256, 0, s>>>(sorted.as<uint64_t>(),
+                                                         g->pr_inv.as<uint32_t>(), V);
+      g->pr_adj.alloc(std::max<uint64_t>(in.E, 1) * 4, s);
+      k_relabel<<<grid_for(in.E, 256), 256, 0, s>>>(in.adj.as<uint32_t>(), g->pr_inv.as<uint32_t>(),
+                                                    g->pr_adj.as<uint32_t>(), in.E);
+      LLG_HIP(hipGetLastError());
+      g->pr_level = in.level;
+    }
+    adj = g->pr_adj.as<uint32_t>();
+    inv = g->pr_inv.as<uint32_t>();
+  }
   DevBuf rank[2], contrib[2], part, diff;
   for (int i = 0; i < 2; i++) {
     rank[i].alloc(V * 4, s);
@@ -174,8 +214,8 @@ void pagerank_device(ll_gpu_graph* g, int level, int iters, double damping, uint
   part.alloc(std::max<uint64_t>(rb.n, 1) * sizeof(Neu), s);
   diff.alloc(8, s);
   const float N = float(V);
-  k_pr_init<<<grid_for(V, 256), 256, 0, s>>>(outdeg, rank[0].as<float>(), contrib[0].as<float>(),
-                                             V, float(1.0 / double(V)));
+  k_pr_init<<<grid_for(V, 256), 256, 0, s>>>(outdeg, inv, rank[0].as<float>(),
+                                             contrib[0].as<float>(), V, float(1.0 / double(V)));
   k_zero_f64<<<1, 1, 0, s>>>(diff.as<double>());
   LLG_HIP(hipGetLastError());
   int cur = 0;
@@ -183,7 +223,8 @@ void pagerank_device(ll_gpu_graph* g, int level, int iters, double damping, uint
     PrArgs a;
     a.items = rb.items.as<RowItem>();
     a.ofs = in.ofs.as<uint64_t>();
-    a.adj = in.adj.as<uint32_t>();
+    a.adj = adj;
+    a.inv = inv;
     a.contrib_in = contrib[cur].as<float>();
     a.rank_in = rank[cur].as<float>();
     a.outdeg = outdeg;
@@ -234,7 +275,10 @@ int ll_gpu_pagerank_ex(ll_gpu_graph* g, int level, int iters, double damping, ui
     LLG_HIP(hipSetDevice(g->ctx->device));
     LLG_CHECK(iters >= 1, LL_GPU_EINVAL, "iters must be >= 1 (the reference runs do/while)");
     LLG_CHECK(damping >= 0 && damping <= 1, LL_GPU_EINVAL, "damping must be in [0, 1]");
-    if (cap == 0) cap = 1024;
+    if (cap == 0) {
+      auto it = g->ctx->options.find("pr_cap");
+      cap = it != g->ctx->options.end() ? uint32_t(it->second) : 1024u;
+    }
     llgpu::DevBuf r;
     llgpu::pagerank_device(g, level, iters, damping, cap, r, last_diff);
     if (ranks_out) llgpu::widen_to_host(g, r.as<float>(), ranks_out, g->V);

@@ -232,6 +232,9 @@ class GpuContext:
     def stream(self) -> int:
         return int(self.lib.ll_gpu_stream(self._h) or 0)
 
+    def set_option(self, name: str, value: int) -> None:
+        self.check(self.lib.ll_gpu_set_option(self._h, name.encode(), int(value)))
+
     def profile(self, on: bool) -> None:
         self.check(self.lib.ll_gpu_profile(self._h, int(on)))
 

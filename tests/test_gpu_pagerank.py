@@ -129,3 +129,21 @@ def test_mirror_errors(ctx):
         ctx.check(ctx.lib.ll_gpu_graph_push_level(dg._h, v))
     with pytest.raises(GpuError):
         dg.pagerank(0)
+
+
+def test_pagerank_degree_ordered_contributions(ctx, ml):
+    """Option pr_reorder gathers contributions in descending-out-degree slot
+    order; results stay within the bar and match the default layout."""
+    g, o = ml
+    gpu_case("pagerank pr_reorder")
+    dg = DeviceGraph(ctx, g)
+    ref, _ = o.pagerank(20)
+    base, _ = dg.pagerank(20)
+    ctx.set_option("pr_reorder", 1)
+    try:
+        dg.drop_snapshots()
+        r, _ = dg.pagerank(20)
+    finally:
+        ctx.set_option("pr_reorder", 0)
+    assert l1rel(r, ref) <= PR_TOL
+    assert np.abs(r - base).max() <= 1e-6 * base.max()
