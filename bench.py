@@ -185,6 +185,9 @@ def main() -> int:
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="run a plain device copy of this size first (calibrates rocprof "
                          "FETCH_SIZE/WRITE_SIZE against known bytes)")
+    ap.add_argument("--sweep-pr", action="store_true",
+                    help="time k_pr_pull for every row-block capacity x contribution order "
+                         "(report on stderr) before the timed steps")
     ap.add_argument("--reorder", type=int, default=0,
                     help="1 = degree-ordered contribution slots in PageRank")
     ap.add_argument("--cpu-iters", type=int, default=6,
@@ -238,6 +241,25 @@ def main() -> int:
         torch.cuda.synchronize()
         log(f"copy probe: 2 x {args.copy_probe_mib} MiB device copy")
         del src, dst
+
+    if args.sweep_pr:
+        for reorder in (0, 1):
+            ctx.set_option("pr_reorder", reorder)
+            for cap in (512, 1024, 2048, 4096):
+                dg.drop_snapshots()
+                dg.pagerank(2, 0.85, cap=cap, want_ranks=False)
+                ctx.sync()
+                ctx.profile(True)
+                t0 = time.perf_counter()
+                dg.pagerank(args.iters, 0.85, cap=cap, want_ranks=False)
+                ctx.sync()
+                t = time.perf_counter() - t0
+                n, ms = ctx.kernel_time("k_pr_pull")
+                ctx.profile(False)
+                avg = ms / max(n, 1)
+                log(f"sweep reorder={reorder} cap={cap}: k_pr_pull {avg:.4f} ms "
+                    f"({(8 * E + 16 * V) / avg / 1e6:.1f} GB/s), run {t * 1e3:.2f} ms")
+        ctx.set_option("pr_reorder", args.reorder)
 
     def step():
         dg.drop_snapshots()

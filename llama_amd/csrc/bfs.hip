@@ -10,17 +10,19 @@
 //     finds its frontier vertex by binary search in an LDS copy of the
 //     workgroup's slice of the scan), claiming vertices with an atomic OR on
 //     the visited bitmap and appending them with one wave-aggregated atomic;
-//   * bottom-up over the visited bitmap: lane i of a wave owns vertex 64w+i, so
-//     the wave's ballot of "found a parent in the frontier bitmap" is exactly
-//     the 64-bit word of the next frontier bitmap and of the visited bitmap
-//     (plain stores, no atomics); each unvisited vertex scans its in-edges and
-//     stops at the first frontier parent.
+//   * bottom-up over row blocks of the in-edges: workgroups whose rows are all
+//     visited return at once; the others read their neighbour IDs coalesced and
+//     pack "neighbour in the frontier bitmap" into 64-bit words with one wave
+//     ballot per 64 edges (the V/8-byte bitmap stays in L2), and every
+//     unvisited row tests its bit range; new vertices are set in the next
+//     frontier bitmap with an atomic OR whose old value keeps counts exact.
 // The switch follows Beamer et al. (SC'12): top-down -> bottom-up when the
 // frontier's out-edges exceed the unexplored edges / 14, back when the
 // frontier shrinks below V / 24.
 #include <climits>
 
 #include "ll_gpu_internal.h"
+#include "rowblocks.h"
 
 namespace llgpu {
 
@@ -29,6 +31,7 @@ namespace {
 constexpr int kBlock = 256;
 constexpr int kEdgesPerBlock = 2048;
 constexpr int32_t kUnreached = INT_MAX - 1;
+constexpr uint32_t kBuCap = kBfsRowCap;
 
 struct Counters {
   unsigned long long n_next;     // vertices discovered this level
@@ -151,41 +154,114 @@ __global__ void k_queue_to_bitmap(const uint32_t* q, uint64_t n, uint64_t* bm) {
   }
 }
 
-__global__ __launch_bounds__(kBlock) void k_bu_step(const uint64_t* ofs, const uint32_t* adj,
-                                                    const uint32_t* outdeg, const uint64_t* fbm,
-                                                    uint64_t* visited, uint64_t* nbm, int32_t* lv,
-                                                    int32_t next_lv, uint64_t V, Counters* c) {
-  const uint64_t v = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
-  const int lane = threadIdx.x & 63;
-  const uint64_t w = v >> 6;
-  const uint64_t words = (V + 63) / 64;
-  if (w >= words) return;  // whole wave beyond the last word
-  const uint64_t vis = visited[w];
-  bool found = false;
-  uint64_t exam = 0;
-  if (!((vis >> lane) & 1)) {
-    for (uint64_t e = ofs[v], end = ofs[v + 1]; e < end; e++) {
-      exam++;
+// Bottom-up level over row blocks of the reverse CSR: a workgroup first
+// checks whether any of its rows is unvisited (else it returns at once), then
+// loads its run of neighbour IDs with coalesced reads and packs "neighbour is
+// in the frontier" into bits with one ballot per 64 edges (the frontier bitmap
+// is V/8 bytes and stays in L2); each unvisited row then tests its bit range.
+// No early exit per vertex, but no divergent per-lane edge walks either.
+__global__ __launch_bounds__(kBlock) void k_bu_rows(const RowItem* items, const uint64_t* ofs,
+                                                    const uint32_t* adj, const uint32_t* outdeg,
+                                                    const uint64_t* fbm, const uint64_t* visited,
+                                                    uint64_t* nbm, int32_t* lv, int32_t next_lv,
+                                                    Counters* c) {
+  constexpr uint32_t kMaxEdges = kBuCap + kBuCap / 2;
+  __shared__ uint64_t rofs[kRowsPerItem + 1];
+  __shared__ uint64_t bits[kMaxEdges / 64];
+  __shared__ unsigned long long red[3][kBlock / 64];
+  const RowItem it = items[blockIdx.x];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  unsigned long long n_new = 0, m_new = 0, exam = 0;
+  if (it.npieces == 0) {
+    const uint32_t nrows = it.row_end - it.row_begin;
+    const uint32_t r = it.row_begin + threadIdx.x;
+    const bool unv = threadIdx.x < nrows && !((visited[r >> 6] >> (r & 63)) & 1);
+    if (!__syncthreads_or(unv)) return;
+    for (uint32_t i = threadIdx.x; i <= nrows; i += kBlock) rofs[i] = ofs[it.row_begin + i];
+    __syncthreads();
+    const uint64_t e0 = rofs[0];
+    const uint32_t ne = uint32_t(rofs[nrows] - e0);
+    for (uint32_t base = 0; base < ne; base += kBlock) {
+      const uint32_t i = base + threadIdx.x;
+      bool in_f = false;
+      if (i < ne) {
+        const uint32_t u = adj[e0 + i];
+        in_f = (fbm[u >> 6] >> (u & 63)) & 1;
+      }
+      const uint64_t m = __ballot(in_f);
+      if (lane == 0) bits[(base >> 6) + wave] = m;
+    }
+    __syncthreads();
+    if (unv) {
+      const uint32_t lo = uint32_t(rofs[threadIdx.x] - e0), hi = uint32_t(rofs[threadIdx.x + 1] - e0);
+      exam = hi - lo;
+      bool found = false;
+      for (uint32_t j = lo; j < hi && !found;) {
+        const uint32_t w = j >> 6, b0 = j & 63;
+        const uint32_t nb = min(64u - b0, hi - j);
+        const uint64_t mask = (nb == 64 ? ~0ull : ((1ull << nb) - 1)) << b0;
+        found = (bits[w] & mask) != 0;
+        j += nb;
+      }
+      if (found) {
+        lv ? (void)(lv[r] = next_lv) : (void)0;
+        const uint64_t bit = uint64_t(1) << (r & 63);
+        const unsigned long long old = atomicOr((unsigned long long*)&nbm[r >> 6], (unsigned long long)bit);
+        if (!(old & bit)) {
+          n_new = 1;
+          m_new = outdeg[r];
+        }
+      }
+    }
+  } else {
+    const uint32_t r = it.row_begin;
+    if ((visited[r >> 6] >> (r & 63)) & 1) return;
+    const uint64_t lo = ofs[r] + uint64_t(it.piece) * kPiece;
+    const uint64_t hi = min(ofs[r + 1], lo + kPiece);
+    bool f = false;
+    for (uint64_t e = lo + threadIdx.x; e < hi && !f; e += kBlock) {
       const uint32_t u = adj[e];
-      if ((fbm[u >> 6] >> (u & 63)) & 1) { found = true; break; }
+      f = (fbm[u >> 6] >> (u & 63)) & 1;
+    }
+    const bool found = __syncthreads_or(f);
+    if (threadIdx.x == 0) {
+      exam = hi - lo;
+      if (found) {
+        if (lv) lv[r] = next_lv;
+        const uint64_t bit = uint64_t(1) << (r & 63);
+        const unsigned long long old = atomicOr((unsigned long long*)&nbm[r >> 6], (unsigned long long)bit);
+        if (!(old & bit)) {
+          n_new = 1;
+          m_new = outdeg[r];
+        }
+      }
     }
   }
-  const uint64_t mask = __ballot(found);
-  if (found && lv) lv[v] = next_lv;
-  uint64_t md = found ? outdeg[v] : 0;
   for (int o = 32; o > 0; o >>= 1) {
-    md += __shfl_xor(md, o, 64);
-    exam += __shfl_xor(exam, o, 64);
+    n_new += __shfl_down(n_new, o, 64);
+    m_new += __shfl_down(m_new, o, 64);
+    exam += __shfl_down(exam, o, 64);
   }
   if (lane == 0) {
-    nbm[w] = mask;
-    if (mask) {
-      visited[w] = vis | mask;
-      atomicAdd(&c->n_next, (unsigned long long)__popcll(mask));
-      atomicAdd(&c->m_next, (unsigned long long)md);
-    }
-    if (exam) atomicAdd(&c->examined, (unsigned long long)exam);
+    red[0][wave] = n_new;
+    red[1][wave] = m_new;
+    red[2][wave] = exam;
   }
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    unsigned long long t = 0;
+    for (int i = 0; i < kBlock / 64; i++) t += red[threadIdx.x][i];
+    if (t) {
+      unsigned long long* dst = threadIdx.x == 0 ? &c->n_next : threadIdx.x == 1 ? &c->m_next
+                                                                                : &c->examined;
+      atomicAdd(dst, t);
+    }
+  }
+}
+
+__global__ void k_visited_or(uint64_t* visited, const uint64_t* nbm, uint64_t words) {
+  const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < words) visited[i] |= nbm[i];
 }
 
 __global__ void k_bitmap_to_queue(const uint64_t* bm, uint64_t V, const uint32_t* outdeg,
@@ -221,8 +297,9 @@ Counters read_counters(const DevBuf& c, hipStream_t s) {
 }  // namespace
 
 // bfs_core: see ll_gpu_internal.h.
-BfsResult bfs_core(ll_gpu_graph* g, const Csr& fwd, const Csr& rev, const uint32_t* fdeg,
-                   uint32_t root, const uint64_t* excluded, DevBuf& visited, int32_t* lv) {
+BfsResult bfs_core(ll_gpu_graph* g, const Csr& fwd, const Csr& rev, const RowBlocks& rev_rb,
+                   const uint32_t* fdeg, uint32_t root, const uint64_t* excluded,
+                   DevBuf& visited, int32_t* lv) {
   ll_gpu_ctx* ctx = g->ctx;
   hipStream_t s = ctx->stream;
   const uint64_t V = g->V;
@@ -262,10 +339,16 @@ BfsResult bfs_core(ll_gpu_graph* g, const Csr& fwd, const Csr& rev, const uint32
     }
     k_zero_counters<<<1, 1, 0, s>>>(cnt.as<Counters>());
     if (bottom_up) {
-      TimedLaunch t(ctx, "k_bu_step");
-      k_bu_step<<<grid_for(words * 64, kBlock), kBlock, 0, s>>>(
-          rev.ofs.as<uint64_t>(), rev.adj.as<uint32_t>(), fdeg, fbm.as<uint64_t>(),
-          visited.as<uint64_t>(), nbm.as<uint64_t>(), lv, lvl + 1, V, cnt.as<Counters>());
+      k_fill_u64<<<grid_for(words, 256), 256, 0, s>>>(nbm.as<uint64_t>(), words, 0);
+      {
+        TimedLaunch t(ctx, "k_bu_step");
+        k_bu_rows<<<unsigned(rev_rb.n), kBlock, 0, s>>>(
+            rev_rb.items.as<RowItem>(), rev.ofs.as<uint64_t>(), rev.adj.as<uint32_t>(), fdeg,
+            fbm.as<uint64_t>(), visited.as<uint64_t>(), nbm.as<uint64_t>(), lv, lvl + 1,
+            cnt.as<Counters>());
+      }
+      k_visited_or<<<grid_for(words, 256), 256, 0, s>>>(visited.as<uint64_t>(),
+                                                        nbm.as<uint64_t>(), words);
     } else {
       k_queue_degrees<<<grid_for(nq, 256), 256, 0, s>>>(q[cur].as<uint32_t>(), nq, fdeg,
                                                         qdeg.as<uint32_t>());
@@ -317,9 +400,10 @@ BfsResult bfs_device(ll_gpu_graph* g, int level, uint32_t root, DevBuf& lv) {
   const Csr& out = ensure_csr(g, LL_GPU_DIR_OUT, level);
   const Csr& in = ensure_csr(g, LL_GPU_DIR_IN, level);
   const uint32_t* outdeg = ensure_degrees(g, LL_GPU_DIR_OUT, level);
+  const RowBlocks& in_rb = ensure_rowblocks(g, LL_GPU_DIR_IN, level, kBuCap, kPiece);
   lv.alloc(V * 4, s);
   DevBuf visited;
-  BfsResult r = bfs_core(g, out, in, outdeg, root, nullptr, visited, lv.as<int32_t>());
+  BfsResult r = bfs_core(g, out, in, in_rb, outdeg, root, nullptr, visited, lv.as<int32_t>());
   DevBuf red;
   red.alloc(8, s);
   k_fill_u64<<<1, 1, 0, s>>>(red.as<uint64_t>(), 1, 0);

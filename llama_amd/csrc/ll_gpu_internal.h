@@ -186,11 +186,13 @@ struct BfsResult {
   int levels = 0;
 };
 // Direction-optimising BFS from `root` over `fwd` (top-down) / `rev`
-// (bottom-up); `fdeg` = degrees of `fwd`; `excluded` (bitmap or null) is never
+// (bottom-up, on its row blocks `rev_rb` of capacity kBfsRowCap); `fdeg` = degrees of `fwd`; `excluded` (bitmap or null) is never
 // entered; `visited` receives the reached set (plus excluded/padding bits);
 // `lv` (int32[V] or null) the levels.
-BfsResult bfs_core(ll_gpu_graph* g, const Csr& fwd, const Csr& rev, const uint32_t* fdeg,
-                   uint32_t root, const uint64_t* excluded, DevBuf& visited, int32_t* lv);
+BfsResult bfs_core(ll_gpu_graph* g, const Csr& fwd, const Csr& rev, const RowBlocks& rev_rb,
+                   const uint32_t* fdeg, uint32_t root, const uint64_t* excluded,
+                   DevBuf& visited, int32_t* lv);
+constexpr uint32_t kBfsRowCap = 1024;  // row-block capacity used by bottom-up steps
 
 int resolve_level(const ll_gpu_graph* g, int level);
 const uint32_t* ensure_degrees(ll_gpu_graph* g, uint32_t dir, int level);
