@@ -188,8 +188,8 @@ def main() -> int:
     ap.add_argument("--sweep-pr", action="store_true",
                     help="time k_pr_pull for every row-block capacity x contribution order "
                          "(report on stderr) before the timed steps")
-    ap.add_argument("--reorder", type=int, default=0,
-                    help="1 = degree-ordered contribution slots in PageRank")
+    ap.add_argument("--reorder", type=int, default=1,
+                    help="1 = degree-ordered contribution slots in PageRank (default)")
     ap.add_argument("--cpu-iters", type=int, default=6,
                     help="PageRank iterations timed for the CPU baseline (0 = skip)")
     ap.add_argument("--seed", type=int, default=1)
@@ -218,8 +218,7 @@ def main() -> int:
     log(f"host graph built in {time.time() - t0:.1f}s")
 
     ctx = GpuContext(0)
-    if args.reorder:
-        ctx.set_option("pr_reorder", 1)
+    ctx.set_option("pr_reorder", args.reorder)
     t1 = time.time()
     dg = DeviceGraph(ctx, g)
     ctx.sync()

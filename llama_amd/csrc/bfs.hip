@@ -181,15 +181,25 @@ __global__ __launch_bounds__(kBlock) void k_bu_rows(const RowItem* items, const 
     __syncthreads();
     const uint64_t e0 = rofs[0];
     const uint32_t ne = uint32_t(rofs[nrows] - e0);
-    for (uint32_t base = 0; base < ne; base += kBlock) {
-      const uint32_t i = base + threadIdx.x;
-      bool in_f = false;
-      if (i < ne) {
-        const uint32_t u = adj[e0 + i];
-        in_f = (fbm[u >> 6] >> (u & 63)) & 1;
-      }
-      const uint64_t m = __ballot(in_f);
-      if (lane == 0) bits[(base >> 6) + wave] = m;
+    // all neighbour IDs first, then all bitmap words, so loads overlap
+    constexpr uint32_t kPer = kMaxEdges / kBlock;
+    uint32_t u[kPer];
+    uint64_t fw[kPer];
+#pragma unroll
+    for (uint32_t k = 0; k < kPer; k++) {
+      const uint32_t i = k * kBlock + threadIdx.x;
+      u[k] = i < ne ? adj[e0 + i] : 0u;
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < kPer; k++) {
+      const uint32_t i = k * kBlock + threadIdx.x;
+      fw[k] = i < ne ? fbm[u[k] >> 6] : 0ull;
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < kPer; k++) {
+      const uint32_t i = k * kBlock + threadIdx.x;
+      const uint64_t m = __ballot(i < ne && ((fw[k] >> (u[k] & 63)) & 1));
+      if (lane == 0) bits[k * (kBlock / 64) + wave] = m;
     }
     __syncthreads();
     if (unv) {
@@ -266,25 +276,29 @@ __global__ void k_visited_or(uint64_t* visited, const uint64_t* nbm, uint64_t wo
 
 __global__ void k_bitmap_to_queue(const uint64_t* bm, uint64_t V, const uint32_t* outdeg,
                                   uint32_t* q, Counters* c) {
+  // one atomic per workgroup: waves take consecutive slices of its range
+  __shared__ unsigned wcnt[4];
+  __shared__ unsigned long long base;
   const uint64_t v = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  const uint64_t words = (V + 63) / 64;
-  if ((v >> 6) >= words) return;
-  bool in = v < V && ((bm[v >> 6] >> (v & 63)) & 1);
-  uint32_t od = in ? outdeg[v] : 0;
-  // reuse the append path; m_next is not needed here and is discarded
-  const uint64_t mq = __ballot(in && od > 0);
-  const int lane = threadIdx.x & 63;
-  unsigned long long qbase = 0;
-  if (lane == 0 && mq) qbase = atomicAdd(&c->n_next, (unsigned long long)__popcll(mq));
-  qbase = __shfl(qbase, 0, 64);
-  if (in && od > 0) q[qbase + __popcll(mq & ((uint64_t(1) << lane) - 1))] = uint32_t(v);
+  const bool in = v < V && ((bm[v >> 6] >> (v & 63)) & 1) && outdeg[v] > 0;
+  const uint64_t mq = __ballot(in);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) wcnt[wave] = __popcll(mq);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned t = 0;
+    for (int i = 0; i < 4; i++) t += wcnt[i];
+    base = t ? atomicAdd(&c->n_next, (unsigned long long)t) : 0;
+  }
+  __syncthreads();
+  unsigned long long off = base;
+  for (int i = 0; i < wave; i++) off += wcnt[i];
+  if (in) q[off + __popcll(mq & ((uint64_t(1) << lane) - 1))] = uint32_t(v);
 }
 
 __global__ void k_count_reached(const int32_t* lv, uint64_t V, unsigned long long* out) {
   const uint64_t v = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  unsigned long long x = (v < V && lv[v] != kUnreached) ? 1ull : 0ull;
-  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
-  if ((threadIdx.x & 63) == 0 && x) atomicAdd(out, x);
+  block_atomic_add(out, (v < V && lv[v] != kUnreached) ? 1ull : 0ull);
 }
 
 Counters read_counters(const DevBuf& c, hipStream_t s) {

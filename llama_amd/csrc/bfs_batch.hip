@@ -116,6 +116,12 @@ __global__ __launch_bounds__(kBlock) void k_ms_pull(MsArgs<W> a) {
   uint32_t cnt = 0;
   if (it.npieces == 0) {
     const uint32_t nrows = it.row_end - it.row_begin;
+    // rows already reached from every lane need no gathers at all
+    const bool open = threadIdx.x < nrows && !(a.vis[it.row_begin + threadIdx.x] == a.full);
+    if (!__syncthreads_or(open)) {
+      if (threadIdx.x < nrows) a.nx[it.row_begin + threadIdx.x] = Lane<W>::zero();
+      return;
+    }
     for (uint32_t i = threadIdx.x; i <= nrows; i += kBlock) rofs[i] = a.ofs[it.row_begin + i];
     __syncthreads();
     const uint64_t e0 = rofs[0];
@@ -212,12 +218,16 @@ __global__ void k_ms_roots(W* vis, W* fr, const int64_t* roots, int n, int32_t* 
 
 __global__ void k_zero_ull(unsigned long long* p) { *p = 0; }
 
-// Per-lane visited counts: lane s of each wave accumulates popcount(ballot(bit s)).
+// Per-lane visited counts: lane s of each wave accumulates popcount(ballot(bit s)),
+// the workgroup sums its waves in LDS and adds once per lane.
 template <typename W>
 __global__ void k_ms_reached(const W* vis, uint64_t V, int n_src, unsigned long long* out) {
+  __shared__ unsigned long long acc[128];
   const uint64_t v = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   const W w = v < V ? vis[v] : Lane<W>::zero();
   const int lane = threadIdx.x & 63;
+  for (int i = threadIdx.x; i < 128; i += blockDim.x) acc[i] = 0;
+  __syncthreads();
   unsigned long long c0 = 0, c1 = 0;
   for (int s = 0; s < n_src; s++) {
     const uint64_t b = __ballot(v < V && Lane<W>::test(w, s));
@@ -226,8 +236,11 @@ __global__ void k_ms_reached(const W* vis, uint64_t V, int n_src, unsigned long 
       else c1 += __popcll(b);
     }
   }
-  if (lane < n_src && c0) atomicAdd(&out[lane], c0);
-  if (lane + 64 < n_src && c1) atomicAdd(&out[lane + 64], c1);
+  if (lane < n_src && c0) atomicAdd(&acc[lane], c0);
+  if (lane + 64 < n_src && c1) atomicAdd(&acc[lane + 64], c1);
+  __syncthreads();
+  for (int i = threadIdx.x; i < n_src; i += blockDim.x)
+    if (acc[i]) atomicAdd(&out[i], acc[i]);
 }
 
 }  // namespace

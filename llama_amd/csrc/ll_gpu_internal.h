@@ -212,6 +212,20 @@ void sort_keys_u64(const uint64_t* keys_in, uint64_t* keys_out, uint64_t n, int 
                    hipStream_t s);
 uint64_t unique_u64(const uint64_t* in, uint64_t* out, uint64_t n, hipStream_t s);
 
+// Sum x over the workgroup and add it to *p with a single atomic.  Every
+// thread of the workgroup must call it (it synchronises the workgroup).
+__device__ inline void block_atomic_add(unsigned long long* p, unsigned long long x) {
+  __shared__ unsigned long long red[16];
+  for (int o = 32; o > 0; o >>= 1) x += __shfl_down(x, o, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = x;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long t = 0;
+    for (unsigned i = 0; i < (blockDim.x + 63) / 64; i++) t += red[i];
+    if (t) atomicAdd(p, t);
+  }
+}
+
 inline unsigned grid_for(uint64_t n, unsigned block) {
   uint64_t g = (n + block - 1) / block;
   return unsigned(g == 0 ? 1 : g);

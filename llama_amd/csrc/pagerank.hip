@@ -183,7 +183,8 @@ void pagerank_device(ll_gpu_graph* g, int level, int iters, double damping, uint
   const Csr& in = ensure_csr(g, LL_GPU_DIR_IN, level);
   const uint32_t* outdeg = ensure_degrees(g, LL_GPU_DIR_OUT, level);
   const RowBlocks& rb = ensure_rowblocks(g, LL_GPU_DIR_IN, level, cap, kPiece);
-  const bool reorder = ctx->options.count("pr_reorder") && ctx->options["pr_reorder"] != 0;
+  // contributions in descending-out-degree slot order unless turned off
+  const bool reorder = !ctx->options.count("pr_reorder") || ctx->options["pr_reorder"] != 0;
   const uint32_t* adj = in.adj.as<uint32_t>();
   const uint32_t* inv = nullptr;
   if (reorder) {
@@ -277,7 +278,7 @@ int ll_gpu_pagerank_ex(ll_gpu_graph* g, int level, int iters, double damping, ui
     LLG_CHECK(damping >= 0 && damping <= 1, LL_GPU_EINVAL, "damping must be in [0, 1]");
     if (cap == 0) {
       auto it = g->ctx->options.find("pr_cap");
-      cap = it != g->ctx->options.end() ? uint32_t(it->second) : 1024u;
+      cap = it != g->ctx->options.end() ? uint32_t(it->second) : 2048u;
     }
     llgpu::DevBuf r;
     llgpu::pagerank_device(g, level, iters, damping, cap, r, last_diff);

@@ -73,6 +73,41 @@ __device__ inline void finalize_row(const PprArgs& a, uint32_t t, int li, const 
   }
 }
 
+// acc += sum over edges e in [lo, hi) step `stride` of rin[adj[e]][cols] / outdeg:
+// four rows are loaded before any is accumulated, so each lane group keeps
+// four independent gathers in flight.
+template <int B>
+__device__ inline void accumulate_rows(const PprArgs& a, uint64_t lo, uint64_t hi,
+                                       uint64_t stride, int li, Neu* acc) {
+  constexpr int EPL = Geo<B>::EPL;
+  constexpr int U = 4;
+  uint64_t e = lo;
+  for (; e + (U - 1) * stride < hi; e += U * stride) {
+    uint32_t w[U];
+    float f[U], x[U][EPL];
+#pragma unroll
+    for (int u = 0; u < U; u++) w[u] = a.adj[e + u * stride];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      f[u] = a.invdeg[w[u]];
+      const float* row = a.rin + uint64_t(w[u]) * B + li * EPL;
+#pragma unroll
+      for (int k = 0; k < EPL; k++) x[u][k] = row[k];
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++)
+#pragma unroll
+      for (int k = 0; k < EPL; k++) acc[k].add(x[u][k] * f[u]);
+  }
+  for (; e < hi; e += stride) {
+    const uint32_t w = a.adj[e];
+    const float f = a.invdeg[w];
+    const float* row = a.rin + uint64_t(w) * B + li * EPL;
+#pragma unroll
+    for (int k = 0; k < EPL; k++) acc[k].add(row[k] * f);
+  }
+}
+
 template <int B>
 __global__ __launch_bounds__(kBlock) void k_ppr_iter(PprArgs a) {
   if (a.st->converged) return;
@@ -93,13 +128,7 @@ __global__ __launch_bounds__(kBlock) void k_ppr_iter(PprArgs a) {
         Neu acc[EPL];
 #pragma unroll
         for (int k = 0; k < EPL; k++) acc[k] = Neu::zero();
-        for (uint64_t e = a.ofs[t], end = a.ofs[t + 1]; e < end; e++) {
-          const uint32_t w = a.adj[e];
-          const float f = a.invdeg[w];
-          const float* row = a.rin + uint64_t(w) * B + li * EPL;
-#pragma unroll
-          for (int k = 0; k < EPL; k++) acc[k].add(row[k] * f);
-        }
+        accumulate_rows<B>(a, a.ofs[t], a.ofs[t + 1], 1, li, acc);
         finalize_row<B>(a, t, li, acc, dsum);
       }
     } else {
@@ -109,13 +138,7 @@ __global__ __launch_bounds__(kBlock) void k_ppr_iter(PprArgs a) {
       Neu acc[EPL];
 #pragma unroll
       for (int k = 0; k < EPL; k++) acc[k] = Neu::zero();
-      for (uint64_t e = lo + slot; e < hi; e += SLOTS) {
-        const uint32_t w = a.adj[e];
-        const float f = a.invdeg[w];
-        const float* row = a.rin + uint64_t(w) * B + li * EPL;
-#pragma unroll
-        for (int k = 0; k < EPL; k++) acc[k].add(row[k] * f);
-      }
+      accumulate_rows<B>(a, lo + slot, hi, SLOTS, li, acc);
 #pragma unroll
       for (int k = 0; k < EPL; k++) {
         ls[slot][li * EPL + k] = acc[k].s;

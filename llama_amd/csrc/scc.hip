@@ -49,8 +49,7 @@ __global__ void k_trim(const uint64_t* iofs, const uint32_t* iadj, const uint64_
   if (v < V && label[v] == kNone)
     t = !has_active(iofs, iadj, v, label) || !has_active(oofs, oadj, v, label);
   if (t) label[v] = uint32_t(v);
-  const uint64_t m = __ballot(t);
-  if ((threadIdx.x & 63) == 0 && m) atomicAdd(changed, (unsigned long long)__popcll(m));
+  block_atomic_add(changed, t ? 1ull : 0ull);
 }
 
 __global__ void k_pivot(const uint32_t* label, const uint32_t* dout, const uint32_t* din,
@@ -104,8 +103,7 @@ __global__ void k_color_init(const uint32_t* label, uint32_t* color, uint32_t* m
     mark[v] = 0;
     minlab[v] = kNone;
   }
-  const uint64_t m = __ballot(a);
-  if ((threadIdx.x & 63) == 0 && m) atomicAdd(active, (unsigned long long)__popcll(m));
+  block_atomic_add(active, a ? 1ull : 0ull);
 }
 
 // colour[v] = max(colour[v], colour[u]) over active in-neighbours u
@@ -127,8 +125,7 @@ __global__ void k_color_prop(const uint64_t* iofs, const uint32_t* iadj, const u
       ch = true;
     }
   }
-  const uint64_t m = __ballot(ch);
-  if ((threadIdx.x & 63) == 0 && m) atomicAdd(changed, 1ull);
+  block_atomic_add(changed, ch ? 1ull : 0ull);
 }
 
 __global__ void k_root_mark(const uint32_t* label, const uint32_t* color, uint32_t* mark,
@@ -154,8 +151,7 @@ __global__ void k_bw_prop(const uint64_t* oofs, const uint32_t* oadj, const uint
       }
     }
   }
-  const uint64_t m = __ballot(ch);
-  if ((threadIdx.x & 63) == 0 && m) atomicAdd(changed, 1ull);
+  block_atomic_add(changed, ch ? 1ull : 0ull);
 }
 
 __global__ void k_comp_min(const uint32_t* label, const uint32_t* color, const uint32_t* mark,
@@ -178,8 +174,7 @@ __global__ void k_labels_out(const uint32_t* label, int64_t* out, uint64_t V,
     out[v] = int64_t(label[v]);
     root = label[v] == uint32_t(v);
   }
-  const uint64_t m = __ballot(root);
-  if ((threadIdx.x & 63) == 0 && m) atomicAdd(nscc, (unsigned long long)__popcll(m));
+  block_atomic_add(nscc, root ? 1ull : 0ull);
 }
 
 unsigned long long read_ull(const DevBuf& b, hipStream_t s) {
