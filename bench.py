@@ -242,9 +242,10 @@ def main() -> int:
         del src, dst
 
     if args.sweep_pr:
-        for reorder in (0, 1):
+        for reorder, nt_hot in ((0, 0), (1, 0), (1, 1 << 18), (1, 1 << 19), (1, 1 << 20)):
             ctx.set_option("pr_reorder", reorder)
-            for cap in (512, 1024, 2048, 4096):
+            ctx.set_option("pr_nt_hot", nt_hot)
+            for cap in (1024, 2048, 4096):
                 dg.drop_snapshots()
                 dg.pagerank(2, 0.85, cap=cap, want_ranks=False)
                 ctx.sync()
@@ -256,9 +257,10 @@ def main() -> int:
                 n, ms = ctx.kernel_time("k_pr_pull")
                 ctx.profile(False)
                 avg = ms / max(n, 1)
-                log(f"sweep reorder={reorder} cap={cap}: k_pr_pull {avg:.4f} ms "
+                log(f"sweep reorder={reorder} nt_hot={nt_hot} cap={cap}: k_pr_pull {avg:.4f} ms "
                     f"({(8 * E + 16 * V) / avg / 1e6:.1f} GB/s), run {t * 1e3:.2f} ms")
         ctx.set_option("pr_reorder", args.reorder)
+        ctx.set_option("pr_nt_hot", 0)
 
     def step():
         dg.drop_snapshots()

@@ -32,6 +32,7 @@ struct PrArgs {
   const uint32_t* inv;    // optional: contribution slot of each vertex (locality order)
   Neu* part;              // per item partials of split rows
   double* diff;           // L1 change accumulator (last iteration only)
+  uint32_t hot_slots;     // contribution slots gathered with normal loads (kNT)
   float base, d;
 };
 
@@ -56,7 +57,10 @@ __device__ inline void block_add_diff(const PrArgs& a, float x) {
   }
 }
 
-template <uint32_t CAP>
+// kNT: stream the neighbour IDs with non-temporal loads and gather
+// contributions of slots >= a.hot_slots non-temporally too, so neither
+// evicts the hot (high out-degree) contribution lines from L2.
+template <uint32_t CAP, bool kNT>
 __global__ __launch_bounds__(kBlock) void k_pr_pull(PrArgs a) {
   __shared__ float vals[CAP + CAP / 2];
   __shared__ uint64_t rofs[kRowsPerItem + 1];
@@ -76,12 +80,14 @@ __global__ __launch_bounds__(kBlock) void k_pr_pull(PrArgs a) {
 #pragma unroll
     for (uint32_t k = 0; k < kPer; k++) {
       uint32_t i = threadIdx.x + k * kBlock;
-      idx[k] = i < ne ? a.adj[e0 + i] : 0u;
+      idx[k] = i < ne ? (kNT ? __builtin_nontemporal_load(a.adj + e0 + i) : a.adj[e0 + i]) : 0u;
     }
 #pragma unroll
     for (uint32_t k = 0; k < kPer; k++) {
       uint32_t i = threadIdx.x + k * kBlock;
-      v[k] = i < ne ? a.contrib_in[idx[k]] : 0.f;
+      v[k] = i >= ne ? 0.f
+             : (kNT && idx[k] >= a.hot_slots) ? __builtin_nontemporal_load(a.contrib_in + idx[k])
+                                               : a.contrib_in[idx[k]];
     }
 #pragma unroll
     for (uint32_t k = 0; k < kPer; k++) {
@@ -164,12 +170,16 @@ __global__ void k_zero_f64(double* p) { *p = 0.0; }
 
 }  // namespace
 
-void launch_pr_pull(uint32_t cap, unsigned nitems, const PrArgs& a, hipStream_t s) {
+void launch_pr_pull(uint32_t cap, bool nt, unsigned nitems, const PrArgs& a, hipStream_t s) {
   switch (cap) {
-    case 512: k_pr_pull<512><<<nitems, kBlock, 0, s>>>(a); break;
-    case 1024: k_pr_pull<1024><<<nitems, kBlock, 0, s>>>(a); break;
-    case 2048: k_pr_pull<2048><<<nitems, kBlock, 0, s>>>(a); break;
-    case 4096: k_pr_pull<4096><<<nitems, kBlock, 0, s>>>(a); break;
+    case 512: nt ? k_pr_pull<512, true><<<nitems, kBlock, 0, s>>>(a)
+                 : k_pr_pull<512, false><<<nitems, kBlock, 0, s>>>(a); break;
+    case 1024: nt ? k_pr_pull<1024, true><<<nitems, kBlock, 0, s>>>(a)
+                  : k_pr_pull<1024, false><<<nitems, kBlock, 0, s>>>(a); break;
+    case 2048: nt ? k_pr_pull<2048, true><<<nitems, kBlock, 0, s>>>(a)
+                  : k_pr_pull<2048, false><<<nitems, kBlock, 0, s>>>(a); break;
+    case 4096: nt ? k_pr_pull<4096, true><<<nitems, kBlock, 0, s>>>(a)
+                  : k_pr_pull<4096, false><<<nitems, kBlock, 0, s>>>(a); break;
     default: throw Error{LL_GPU_EINVAL, "unsupported PageRank row-block capacity"};
   }
 }
@@ -187,6 +197,9 @@ void pagerank_device(ll_gpu_graph* g, int level, int iters, double damping, uint
   const bool reorder = !ctx->options.count("pr_reorder") || ctx->options["pr_reorder"] != 0;
   const uint32_t* adj = in.adj.as<uint32_t>();
   const uint32_t* inv = nullptr;
+  // option pr_nt_hot = N > 0: non-temporal streams, slots >= N gathered non-temporally
+  const long nt_hot = ctx->options.count("pr_nt_hot") ? ctx->options["pr_nt_hot"] : 0;
+  const bool nt = nt_hot > 0;
   if (reorder) {
     if (g->pr_level != in.level) {
       TimedLaunch t(ctx, "pr_reorder");
@@ -226,6 +239,7 @@ void pagerank_device(ll_gpu_graph* g, int level, int iters, double damping, uint
     a.ofs = in.ofs.as<uint64_t>();
     a.adj = adj;
     a.inv = inv;
+    a.hot_slots = uint32_t(nt_hot);
     a.contrib_in = contrib[cur].as<float>();
     a.rank_in = rank[cur].as<float>();
     a.outdeg = outdeg;
@@ -237,7 +251,7 @@ void pagerank_device(ll_gpu_graph* g, int level, int iters, double damping, uint
     a.d = float(damping);
     {
       TimedLaunch t(ctx, "k_pr_pull");
-      launch_pr_pull(cap, unsigned(rb.n), a, s);
+      launch_pr_pull(cap, nt, unsigned(rb.n), a, s);
     }
     {
       TimedLaunch t(ctx, "k_pr_fixup");
