@@ -172,6 +172,17 @@ def bench_wpr(ctx, args) -> dict:
     return out
 
 
+# one PageRank pull iteration: pull (per vertex, or per (row, segment) pair),
+# split-row fix-up, and the segment merge when the pull is segmented
+PR_ITERATION_KERNELS = ("k_pr_pull", "k_pr_fixup", "k_pr_merge")
+
+
+def iteration_ms(ctx, iterations):
+    """Device time of one PageRank iteration: the iteration kernels' recorded
+    time (HIP events on the library stream) over `iterations`."""
+    return sum(ctx.kernel_time(k)[1] for k in PR_ITERATION_KERNELS) / max(iterations, 1)
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -186,10 +197,13 @@ def main() -> int:
                     help="run a plain device copy of this size first (calibrates rocprof "
                          "FETCH_SIZE/WRITE_SIZE against known bytes)")
     ap.add_argument("--sweep-pr", action="store_true",
-                    help="time k_pr_pull for every row-block capacity x contribution order "
-                         "(report on stderr) before the timed steps")
+                    help="time one PageRank iteration for every row-block capacity x "
+                         "contribution order x column segmentation (report on stderr) "
+                         "before the timed steps")
     ap.add_argument("--reorder", type=int, default=1,
                     help="1 = degree-ordered contribution slots in PageRank (default)")
+    ap.add_argument("--seg-bits", type=int, default=-1,
+                    help="PageRank column segments of 2^N slots (0 = off, -1 = automatic)")
     ap.add_argument("--cpu-iters", type=int, default=6,
                     help="PageRank iterations timed for the CPU baseline (0 = skip)")
     ap.add_argument("--seed", type=int, default=1)
@@ -242,25 +256,28 @@ def main() -> int:
         del src, dst
 
     if args.sweep_pr:
-        for reorder, nt_hot in ((0, 0), (1, 0), (1, 1 << 18), (1, 1 << 19), (1, 1 << 20)):
+        for reorder, seg in ((0, 0), (1, 0), (1, 17), (1, 18), (1, 19), (1, 20)):
             ctx.set_option("pr_reorder", reorder)
-            ctx.set_option("pr_nt_hot", nt_hot)
+            ctx.set_option("pr_seg_bits", seg)
             for cap in (1024, 2048, 4096):
                 dg.drop_snapshots()
+                ctx.profile(True)
                 dg.pagerank(2, 0.85, cap=cap, want_ranks=False)
                 ctx.sync()
+                build_ms = sum(ctx.kernel_time(k)[1] for k in ("pr_reorder", "pr_segments"))
+                ctx.profile(False)
                 ctx.profile(True)
                 t0 = time.perf_counter()
                 dg.pagerank(args.iters, 0.85, cap=cap, want_ranks=False)
                 ctx.sync()
                 t = time.perf_counter() - t0
-                n, ms = ctx.kernel_time("k_pr_pull")
+                it_ms = iteration_ms(ctx, args.iters)
                 ctx.profile(False)
-                avg = ms / max(n, 1)
-                log(f"sweep reorder={reorder} nt_hot={nt_hot} cap={cap}: k_pr_pull {avg:.4f} ms "
-                    f"({(8 * E + 16 * V) / avg / 1e6:.1f} GB/s), run {t * 1e3:.2f} ms")
+                log(f"sweep reorder={reorder} seg_bits={seg} cap={cap}: iteration {it_ms:.4f} ms "
+                    f"({(8 * E + 16 * V) / it_ms / 1e6:.1f} GB/s), run {t * 1e3:.2f} ms, "
+                    f"layout build {build_ms:.2f} ms")
         ctx.set_option("pr_reorder", args.reorder)
-        ctx.set_option("pr_nt_hot", 0)
+        ctx.set_option("pr_seg_bits", args.seg_bits)
 
     def step():
         dg.drop_snapshots()
@@ -277,16 +294,17 @@ def main() -> int:
     ctx.sync()
     te = time.perf_counter()
     n_pr, ms_pr = ctx.kernel_time("k_pr_pull")
-    n_fx, ms_fx = ctx.kernel_time("k_pr_fixup")
+    it_kernels = {k: ctx.kernel_time(k) for k in PR_ITERATION_KERNELS}
     mat = {k: ctx.kernel_time(k) for k in ("k_seg_count", "k_seg_fill", "k_live_count",
-                                           "k_compact", "k_degrees", "pr_reorder")}
+                                           "k_compact", "k_degrees", "pr_reorder",
+                                           "pr_segments")}
+    avg_it_ms = iteration_ms(ctx, args.iters * args.steps)
     ctx.profile(False)
     log(f"timed {args.steps} steps done")
     step_ms = (te - ts) * 1e3 / args.steps
     gteps = E * args.iters / (step_ms * 1e-3) / 1e9
-    avg_pr_ms = ms_pr / max(n_pr, 1)
     b_pr = 8 * E + 16 * V  # algorithmic bytes of one pull iteration
-    achieved = b_pr / (avg_pr_ms * 1e-3) / 1e9
+    achieved = b_pr / (avg_it_ms * 1e-3) / 1e9
 
     secondary = {}
     if args.bfs_sources > 0:
@@ -338,14 +356,17 @@ def main() -> int:
             "continuations": int(info.continuations), "seed": args.seed,
         },
         "roofline": {
-            "bound": "hbm", "kernel": "k_pr_pull", "achieved": round(achieved, 2),
+            "bound": "hbm", "kernel": "+".join(PR_ITERATION_KERNELS),
+            "achieved": round(achieved, 2),
             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None, "bytes_per_launch": b_pr, "avg_launch_ms": round(avg_pr_ms, 5),
+            "traffic": None, "bytes_per_launch": b_pr, "avg_launch_ms": round(avg_it_ms, 5),
             "launches": n_pr,
+            "per_kernel_avg_ms": {k: round(v[1] / max(n_pr, 1), 5)
+                                  for k, v in it_kernels.items()},
         },
         "cpu_baseline": cpu,
         "breakdown_ms_per_step": {
-            "pr_pull": round(ms_pr / args.steps, 4), "pr_fixup": round(ms_fx / args.steps, 4),
+            **{k: round(v[1] / args.steps, 4) for k, v in it_kernels.items()},
             **{k: round(v[1] / args.steps, 4) for k, v in mat.items()},
         },
         "upload": {"bytes": up_bytes, "seconds": round(up_s, 3)},

@@ -197,8 +197,8 @@ int ll_gpu_profile_get(ll_gpu_ctx* ctx, const char* kernel, uint64_t* launches,
 int ll_gpu_tuned(ll_gpu_ctx* ctx, const char* prefix, int* value);
 /* Tuning knobs: "pr_reorder" (1 = gather PageRank contributions in
  * descending-out-degree slot order; default), "pr_cap" (default row-block
- * capacity, 2048), "pr_nt_hot" (N > 0: stream neighbour IDs and gather slots
- * >= N with non-temporal loads; 0 = off). */
+ * capacity, 2048), "pr_seg_bits" (pull over column segments of 2^N
+ * contribution slots; 0 = off, -1 = automatic: 19 above 2^20 vertices). */
 int ll_gpu_set_option(ll_gpu_ctx* ctx, const char* name, long value);
 /* Drop the cached snapshot materialisations (the next call rebuilds them). */
 void ll_gpu_graph_drop_snapshots(ll_gpu_graph* g);

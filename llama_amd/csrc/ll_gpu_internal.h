@@ -134,6 +134,25 @@ struct RowBlocks {
   DevBuf slots;         // uint32_t[n]: partials slot of split-row items, else ~0
 };
 
+// Column-segmented in-CSR for the PageRank pull (built by build_pr_segments):
+// contribution slots cut into S segments of 2^zb; rows of `pairs` are the
+// (row, segment) cells with edges, segment-major, rows ascending per segment.
+constexpr uint32_t kMaxSegments = 64;
+struct PrSegments {
+  bool valid = false;
+  int level = -1;
+  uint32_t zb = 0, S = 0;
+  uint32_t req_zb = 0; // segment bits asked for (zb may be larger)
+  bool slots = false;  // sources are degree-ordered slots (else vertex IDs)
+  uint64_t P = 0;      // pairs
+  uint64_t nb = 0;     // merge row blocks of 256 rows
+  Csr pairs;           // ofs: uint64_t[P+1] into adj = source slots (uint32_t[E])
+  DevBuf pair_row;     // uint32_t[P]
+  DevBuf pseg;         // uint32_t[S+1]: first pair of each segment
+  DevBuf mt;           // uint32_t[S*(nb+1)]: first pair of segment s at row block b
+  RowBlocks rb;        // row blocks of `pairs`
+};
+
 struct DirMirror {
   std::vector<DevBuf> vt;   // VtEntry[V+1] per level
   std::vector<DevBuf> et;   // uint64_t[words] per level
@@ -163,6 +182,7 @@ struct ll_gpu_graph {
   // PageRank locality order of the in-CSR (option pr_reorder)
   int pr_level = -1;
   llgpu::DevBuf pr_inv, pr_adj;
+  llgpu::PrSegments pr_seg;   // column segments of the in-CSR in slot order
   // weighted in-CSR (rows = heads, sources ascending) + map to out-edge positions
   int w_level = -1;
   llgpu::Csr wcsr;
@@ -202,6 +222,11 @@ const RowBlocks& ensure_rowblocks(ll_gpu_graph* g, uint32_t dir, int level, uint
 void build_rowblocks(ll_gpu_ctx* ctx, const Csr& c, uint32_t cap, uint32_t piece,
                      RowBlocks& rb);
 void drop_snapshots(ll_gpu_graph* g);
+// Column segments of `in` (source IDs mapped through `inv` when given) with
+// segments of 2^zb slots (zb raised until at most kMaxSegments segments).
+void build_pr_segments(ll_gpu_graph* g, const Csr& in, const uint32_t* inv, uint32_t zb,
+                       PrSegments& ps);
+const RowBlocks& pr_segment_rowblocks(ll_gpu_ctx* ctx, PrSegments& ps, uint32_t cap);
 
 // Device-wide exclusive scan of n values into out (n+1 entries, total last).
 void scan_u32_to_u64(const uint32_t* in, uint64_t* out, uint64_t n, hipStream_t s);

@@ -11,6 +11,11 @@
 // by k_pr_fixup.  Bytes per iteration: 4 B in-neighbour ID + 4 B contribution
 // gather per edge, 8 B offset + 4 B out-degree + 8 B rank/contrib writes per
 // vertex.
+//
+// On large graphs the pull runs over column segments of the in-CSR (built by
+// pr_segments.hip): the same kernel reduces every (row, segment) pair to a
+// compensated partial, and k_pr_merge adds each row's partials in segment
+// order before the finalisation above.
 #include "ll_gpu_internal.h"
 #include "rowblocks.h"
 
@@ -32,7 +37,7 @@ struct PrArgs {
   const uint32_t* inv;    // optional: contribution slot of each vertex (locality order)
   Neu* part;              // per item partials of split rows
   double* diff;           // L1 change accumulator (last iteration only)
-  uint32_t hot_slots;     // contribution slots gathered with normal loads (kNT)
+  Neu* pair_part;         // segmented pull: compensated partial per pair (row of the CSR)
   float base, d;
 };
 
@@ -42,6 +47,16 @@ __device__ inline float finalize(const PrArgs& a, uint32_t r, float S) {
   uint32_t od = a.outdeg[r];
   a.contrib_out[a.inv ? a.inv[r] : r] = od ? v / float(od) : 0.f;
   return a.diff ? fabsf(v - a.rank_in[r]) : 0.f;
+}
+
+// Row r of the CSR is done: a vertex (finalise) or a pair (store the partial).
+template <bool kPartial>
+__device__ inline float finish(const PrArgs& a, uint32_t r, const Neu& s) {
+  if (kPartial) {
+    a.pair_part[r] = s;
+    return 0.f;
+  }
+  return finalize(a, r, s.value());
 }
 
 __device__ inline void block_add_diff(const PrArgs& a, float x) {
@@ -57,10 +72,7 @@ __device__ inline void block_add_diff(const PrArgs& a, float x) {
   }
 }
 
-// kNT: stream the neighbour IDs with non-temporal loads and gather
-// contributions of slots >= a.hot_slots non-temporally too, so neither
-// evicts the hot (high out-degree) contribution lines from L2.
-template <uint32_t CAP, bool kNT>
+template <uint32_t CAP, bool kPartial>
 __global__ __launch_bounds__(kBlock) void k_pr_pull(PrArgs a) {
   __shared__ float vals[CAP + CAP / 2];
   __shared__ uint64_t rofs[kRowsPerItem + 1];
@@ -80,14 +92,12 @@ __global__ __launch_bounds__(kBlock) void k_pr_pull(PrArgs a) {
 #pragma unroll
     for (uint32_t k = 0; k < kPer; k++) {
       uint32_t i = threadIdx.x + k * kBlock;
-      idx[k] = i < ne ? (kNT ? __builtin_nontemporal_load(a.adj + e0 + i) : a.adj[e0 + i]) : 0u;
+      idx[k] = i < ne ? a.adj[e0 + i] : 0u;
     }
 #pragma unroll
     for (uint32_t k = 0; k < kPer; k++) {
       uint32_t i = threadIdx.x + k * kBlock;
-      v[k] = i >= ne ? 0.f
-             : (kNT && idx[k] >= a.hot_slots) ? __builtin_nontemporal_load(a.contrib_in + idx[k])
-                                               : a.contrib_in[idx[k]];
+      v[k] = i < ne ? a.contrib_in[idx[k]] : 0.f;
     }
 #pragma unroll
     for (uint32_t k = 0; k < kPer; k++) {
@@ -99,7 +109,7 @@ __global__ __launch_bounds__(kBlock) void k_pr_pull(PrArgs a) {
       const uint32_t lo = uint32_t(rofs[threadIdx.x] - e0), hi = uint32_t(rofs[threadIdx.x + 1] - e0);
       Neu s = Neu::zero();
       for (uint32_t j = lo; j < hi; j++) s.add(vals[j]);
-      dsum = finalize(a, it.row_begin + threadIdx.x, s.value());
+      dsum = finish<kPartial>(a, it.row_begin + threadIdx.x, s);
     }
   } else {
     const uint32_t r = it.row_begin;
@@ -114,7 +124,7 @@ __global__ __launch_bounds__(kBlock) void k_pr_pull(PrArgs a) {
     if (threadIdx.x == 0) {
       Neu t = red[0];
       for (int i = 1; i < kBlock / 64; i++) t.merge(red[i]);
-      if (it.npieces == 1) dsum = finalize(a, r, t.value());
+      if (it.npieces == 1) dsum = finish<kPartial>(a, r, t);
       else a.part[blockIdx.x] = t;
     }
   }
@@ -122,6 +132,7 @@ __global__ __launch_bounds__(kBlock) void k_pr_pull(PrArgs a) {
 }
 
 // Rows split into several pieces: combine the partials in piece order.
+template <bool kPartial>
 __global__ void k_pr_fixup(PrArgs a, uint64_t nitems) {
   uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   float dsum = 0.f;
@@ -130,9 +141,34 @@ __global__ void k_pr_fixup(PrArgs a, uint64_t nitems) {
     if (it.npieces > 1 && it.piece == 0) {
       Neu t = a.part[i];
       for (uint32_t p = 1; p < it.npieces; p++) t.merge(a.part[i + p]);
-      dsum = finalize(a, it.row_begin, t.value());
+      dsum = finish<kPartial>(a, it.row_begin, t);
     }
   }
+  block_add_diff(a, dsum);
+}
+
+// Segmented pull: block b finalises rows [256 b, 256 b + 256), adding their
+// pair partials segment by segment (a row has at most one pair per segment,
+// so the threads of one segment never share an accumulator).
+__global__ __launch_bounds__(kBlock) void k_pr_merge(PrArgs a, const uint32_t* pair_row,
+                                                     const uint32_t* mt, uint32_t S, uint64_t nb,
+                                                     uint64_t V) {
+  static_assert(kBlock == kRowsPerItem, "one thread per row of the block");
+  __shared__ Neu acc[kBlock];
+  const uint64_t b = blockIdx.x;
+  const uint32_t r0 = uint32_t(b * kRowsPerItem);
+  acc[threadIdx.x] = Neu::zero();
+  __syncthreads();
+  for (uint32_t s = 0; s < S; s++) {
+    const uint32_t lo = mt[uint64_t(s) * (nb + 1) + b], hi = mt[uint64_t(s) * (nb + 1) + b + 1];
+    if (lo == hi) continue;  // uniform over the block
+    const uint32_t p = lo + threadIdx.x;
+    if (p < hi) acc[pair_row[p] - r0].merge(a.pair_part[p]);
+    __syncthreads();
+  }
+  float dsum = 0.f;
+  const uint64_t r = uint64_t(r0) + threadIdx.x;
+  if (r < V) dsum = finalize(a, uint32_t(r), acc[threadIdx.x].value());
   block_add_diff(a, dsum);
 }
 
@@ -170,16 +206,13 @@ __global__ void k_zero_f64(double* p) { *p = 0.0; }
 
 }  // namespace
 
-void launch_pr_pull(uint32_t cap, bool nt, unsigned nitems, const PrArgs& a, hipStream_t s) {
+template <bool kPartial>
+void launch_pr_pull(uint32_t cap, unsigned nitems, const PrArgs& a, hipStream_t s) {
   switch (cap) {
-    case 512: nt ? k_pr_pull<512, true><<<nitems, kBlock, 0, s>>>(a)
-                 : k_pr_pull<512, false><<<nitems, kBlock, 0, s>>>(a); break;
-    case 1024: nt ? k_pr_pull<1024, true><<<nitems, kBlock, 0, s>>>(a)
-                  : k_pr_pull<1024, false><<<nitems, kBlock, 0, s>>>(a); break;
-    case 2048: nt ? k_pr_pull<2048, true><<<nitems, kBlock, 0, s>>>(a)
-                  : k_pr_pull<2048, false><<<nitems, kBlock, 0, s>>>(a); break;
-    case 4096: nt ? k_pr_pull<4096, true><<<nitems, kBlock, 0, s>>>(a)
-                  : k_pr_pull<4096, false><<<nitems, kBlock, 0, s>>>(a); break;
+    case 512: k_pr_pull<512, kPartial><<<nitems, kBlock, 0, s>>>(a); break;
+    case 1024: k_pr_pull<1024, kPartial><<<nitems, kBlock, 0, s>>>(a); break;
+    case 2048: k_pr_pull<2048, kPartial><<<nitems, kBlock, 0, s>>>(a); break;
+    case 4096: k_pr_pull<4096, kPartial><<<nitems, kBlock, 0, s>>>(a); break;
     default: throw Error{LL_GPU_EINVAL, "unsupported PageRank row-block capacity"};
   }
 }
@@ -192,14 +225,16 @@ void pagerank_device(ll_gpu_graph* g, int level, int iters, double damping, uint
   const uint64_t V = g->V;
   const Csr& in = ensure_csr(g, LL_GPU_DIR_IN, level);
   const uint32_t* outdeg = ensure_degrees(g, LL_GPU_DIR_OUT, level);
-  const RowBlocks& rb = ensure_rowblocks(g, LL_GPU_DIR_IN, level, cap, kPiece);
   // contributions in descending-out-degree slot order unless turned off
   const bool reorder = !ctx->options.count("pr_reorder") || ctx->options["pr_reorder"] != 0;
+  // column segments of 2^pr_seg_bits slots (0 = off); on by default once the
+  // contribution vector outgrows an XCD's L2
+  long seg_bits = ctx->options.count("pr_seg_bits") ? ctx->options["pr_seg_bits"] : -1;
+  if (seg_bits < 0) seg_bits = V > (uint64_t(1) << 20) ? 19 : 0;
+  LLG_CHECK(seg_bits <= 31, LL_GPU_EINVAL, "pr_seg_bits must be <= 31");
+  const bool segmented = seg_bits > 0;
   const uint32_t* adj = in.adj.as<uint32_t>();
   const uint32_t* inv = nullptr;
-  // option pr_nt_hot = N > 0: non-temporal streams, slots >= N gathered non-temporally
-  const long nt_hot = ctx->options.count("pr_nt_hot") ? ctx->options["pr_nt_hot"] : 0;
-  const bool nt = nt_hot > 0;
   if (reorder) {
     if (g->pr_level != in.level) {
       TimedLaunch t(ctx, "pr_reorder");
@@ -211,21 +246,39 @@ void pagerank_device(ll_gpu_graph* g, int level, int iters, double damping, uint
       g->pr_inv.alloc(V * 4, s);
       k_inv_from_sorted<<<grid_for(V, 256), 256, 0, s>>>(sorted.as<uint64_t>(),
                                                          g->pr_inv.as<uint32_t>(), V);
-      g->pr_adj.alloc(std::max<uint64_t>(in.E, 1) * 4, s);
-      k_relabel<<<grid_for(in.E, 256), 256, 0, s>>>(in.adj.as<uint32_t>(), g->pr_inv.as<uint32_t>(),
-                                                    g->pr_adj.as<uint32_t>(), in.E);
       LLG_HIP(hipGetLastError());
+      g->pr_adj.release();
       g->pr_level = in.level;
     }
-    adj = g->pr_adj.as<uint32_t>();
     inv = g->pr_inv.as<uint32_t>();
+    if (!segmented) {
+      if (!g->pr_adj.p) {
+        TimedLaunch t(ctx, "pr_reorder");
+        g->pr_adj.alloc(std::max<uint64_t>(in.E, 1) * 4, s);
+        k_relabel<<<grid_for(in.E, 256), 256, 0, s>>>(in.adj.as<uint32_t>(), inv,
+                                                      g->pr_adj.as<uint32_t>(), in.E);
+        LLG_HIP(hipGetLastError());
+      }
+      adj = g->pr_adj.as<uint32_t>();
+    }
   }
-  DevBuf rank[2], contrib[2], part, diff;
+  PrSegments& ps = g->pr_seg;
+  if (segmented && !(ps.valid && ps.level == in.level && ps.req_zb == uint32_t(seg_bits) &&
+                     ps.slots == reorder)) {
+    TimedLaunch t(ctx, "pr_segments");
+    build_pr_segments(g, in, inv, uint32_t(seg_bits), ps);
+    ps.req_zb = uint32_t(seg_bits);
+    ps.slots = reorder;
+  }
+  const RowBlocks& rb = segmented ? pr_segment_rowblocks(ctx, ps, cap)
+                                  : ensure_rowblocks(g, LL_GPU_DIR_IN, level, cap, kPiece);
+  DevBuf rank[2], contrib[2], part, diff, pair_part;
   for (int i = 0; i < 2; i++) {
     rank[i].alloc(V * 4, s);
     contrib[i].alloc(V * 4, s);
   }
   part.alloc(std::max<uint64_t>(rb.n, 1) * sizeof(Neu), s);
+  if (segmented) pair_part.alloc(std::max<uint64_t>(ps.P, 1) * sizeof(Neu), s);
   diff.alloc(8, s);
   const float N = float(V);
   k_pr_init<<<grid_for(V, 256), 256, 0, s>>>(outdeg, inv, rank[0].as<float>(),
@@ -236,26 +289,40 @@ void pagerank_device(ll_gpu_graph* g, int level, int iters, double damping, uint
   for (int it = 0; it < iters; it++) {
     PrArgs a;
     a.items = rb.items.as<RowItem>();
-    a.ofs = in.ofs.as<uint64_t>();
-    a.adj = adj;
+    a.ofs = segmented ? ps.pairs.ofs.as<uint64_t>() : in.ofs.as<uint64_t>();
+    a.adj = segmented ? ps.pairs.adj.as<uint32_t>() : adj;
     a.inv = inv;
-    a.hot_slots = uint32_t(nt_hot);
     a.contrib_in = contrib[cur].as<float>();
     a.rank_in = rank[cur].as<float>();
     a.outdeg = outdeg;
     a.rank_out = rank[cur ^ 1].as<float>();
     a.contrib_out = contrib[cur ^ 1].as<float>();
     a.part = part.as<Neu>();
+    a.pair_part = pair_part.as<Neu>();
     a.diff = (it == iters - 1 && last_diff) ? diff.as<double>() : nullptr;
     a.base = float((1.0 - damping) / double(N));
     a.d = float(damping);
-    {
-      TimedLaunch t(ctx, "k_pr_pull");
-      launch_pr_pull(cap, nt, unsigned(rb.n), a, s);
-    }
-    {
+    if (segmented) {
+      PrArgs pa = a;
+      pa.diff = nullptr;
+      if (rb.n) {
+        {
+          TimedLaunch t(ctx, "k_pr_pull");
+          launch_pr_pull<true>(cap, unsigned(rb.n), pa, s);
+        }
+        TimedLaunch t(ctx, "k_pr_fixup");
+        k_pr_fixup<true><<<grid_for(rb.n, 256), 256, 0, s>>>(pa, rb.n);
+      }
+      TimedLaunch t(ctx, "k_pr_merge");
+      k_pr_merge<<<unsigned(ps.nb), kBlock, 0, s>>>(a, ps.pair_row.as<uint32_t>(),
+                                                    ps.mt.as<uint32_t>(), ps.S, ps.nb, V);
+    } else {
+      {
+        TimedLaunch t(ctx, "k_pr_pull");
+        launch_pr_pull<false>(cap, unsigned(rb.n), a, s);
+      }
       TimedLaunch t(ctx, "k_pr_fixup");
-      k_pr_fixup<<<grid_for(rb.n, 256), 256, 0, s>>>(a, rb.n);
+      k_pr_fixup<false><<<grid_for(rb.n, 256), 256, 0, s>>>(a, rb.n);
     }
     LLG_HIP(hipGetLastError());
     cur ^= 1;

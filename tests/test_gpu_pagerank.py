@@ -16,6 +16,16 @@ def l1rel(a, b):
     return float(np.abs(a - b).sum() / np.abs(b).sum())
 
 
+@pytest.fixture
+def seg_bits(ctx):
+    """Set the PageRank column-segment option for one test, then restore the
+    automatic default (-1)."""
+    def set_bits(n):
+        ctx.set_option("pr_seg_bits", n)
+    yield set_bits
+    ctx.set_option("pr_seg_bits", -1)
+
+
 @pytest.fixture(scope="module")
 def ml():
     g = build_multilevel(12, levels=6, ins_frac=0.03, del_frac=0.01, seed=7)
@@ -52,10 +62,14 @@ def test_incremental_push_matches_full_push(ctx):
             assert np.array_equal(ga, oa) and np.array_equal(go, oo)
 
 
+@pytest.mark.parametrize("seg", [0, 6, 9])
 @pytest.mark.parametrize("cap", [512, 1024, 2048, 4096])
-def test_pagerank_multilevel(ctx, ml, cap):
+def test_pagerank_multilevel(ctx, ml, cap, seg, seg_bits):
+    """seg = column segments of 2^seg slots (6: 64 segments, the maximum;
+    9: 8 segments; 0: plain pull)."""
     g, o = ml
-    gpu_case(f"pagerank s12 x6 cap={cap}")
+    gpu_case(f"pagerank s12 x6 cap={cap} seg_bits={seg}")
+    seg_bits(seg)
     dg = DeviceGraph(ctx, g)
     for L in (0, g.num_levels - 1):
         ref, ref_diff = o.pagerank(20, level=L)
@@ -78,8 +92,10 @@ def test_pagerank_single_level_s16(ctx):
     assert err <= PR_TOL
 
 
-def test_pagerank_hub_split_over_pieces(ctx):
-    """In-degree 20000 > one piece: the partials are combined in order."""
+def test_pagerank_hub_split_over_pieces(ctx, seg_bits):
+    """In-degree 20000 > one piece: the partials are combined in order (plain
+    pull; segmented with one segment, where the hub's single pair is split;
+    and with 40 segments)."""
     gpu_case("pagerank hub star 20000")
     k = 20000
     n = k + 3
@@ -90,13 +106,17 @@ def test_pagerank_hub_split_over_pieces(ctx):
     o = oracle.OracleGraph.from_host(g)
     ref, _ = o.pagerank(5)
     dg = DeviceGraph(ctx, g)
-    for cap in (512, 4096):
-        r, _ = dg.pagerank(5, cap=cap)
-        assert l1rel(r, ref) <= PR_TOL
+    for seg in (0, 15, 9):
+        seg_bits(seg)
+        for cap in (512, 4096):
+            r, _ = dg.pagerank(5, cap=cap)
+            assert l1rel(r, ref) <= PR_TOL, (seg, cap)
 
 
-def test_pagerank_empty_and_tiny(ctx):
-    gpu_case("pagerank empty graph")
+@pytest.mark.parametrize("seg", [0, 2])
+def test_pagerank_empty_and_tiny(ctx, seg, seg_bits):
+    gpu_case(f"pagerank empty graph seg_bits={seg}")
+    seg_bits(seg)
     g = HostGraph(5)
     g.add_level([], [])
     dg = DeviceGraph(ctx, g)
@@ -106,11 +126,18 @@ def test_pagerank_empty_and_tiny(ctx):
     g2.add_level([], [])
     r2, _ = DeviceGraph(ctx, g2).pagerank(3)
     assert np.allclose(r2, [0.15], rtol=1e-6)
+    g3 = HostGraph(300)  # one edge; rows beyond the first merge block
+    g3.add_level([7], [290])
+    r3, _ = DeviceGraph(ctx, g3).pagerank(2)
+    ref3, _ = oracle.OracleGraph.from_host(g3).pagerank(2)
+    assert l1rel(r3, ref3) <= PR_TOL
 
 
-def test_pagerank_deterministic(ctx, ml):
+@pytest.mark.parametrize("seg", [0, 7])
+def test_pagerank_deterministic(ctx, ml, seg, seg_bits):
     g, _ = ml
-    gpu_case("pagerank determinism")
+    gpu_case(f"pagerank determinism seg_bits={seg}")
+    seg_bits(seg)
     dg = DeviceGraph(ctx, g)
     a, _ = dg.pagerank(10)
     dg.drop_snapshots()
@@ -131,11 +158,12 @@ def test_mirror_errors(ctx):
         dg.pagerank(0)
 
 
-def test_pagerank_degree_ordered_contributions(ctx, ml):
+def test_pagerank_degree_ordered_contributions(ctx, ml, seg_bits):
     """Option pr_reorder gathers contributions in descending-out-degree slot
     order; results stay within the bar and match the default layout."""
     g, o = ml
     gpu_case("pagerank pr_reorder")
+    seg_bits(0)
     dg = DeviceGraph(ctx, g)
     ref, _ = o.pagerank(20)
     try:
@@ -149,3 +177,25 @@ def test_pagerank_degree_ordered_contributions(ctx, ml):
     assert l1rel(r, ref) <= PR_TOL and l1rel(base, ref) <= PR_TOL
     # same terms in the same order: only the slots of the contributions differ
     assert np.array_equal(r, base)
+
+
+def test_pagerank_segments_by_vertex_id(ctx, ml, seg_bits):
+    """Column segments over vertex IDs (no degree order) and over slots agree
+    with the oracle; switching the option rebuilds the cached layout."""
+    g, o = ml
+    gpu_case("pagerank segments without reorder")
+    ref, _ = o.pagerank(20)
+    dg = DeviceGraph(ctx, g)
+    seg_bits(8)
+    try:
+        ctx.set_option("pr_reorder", 0)
+        a, _ = dg.pagerank(20)
+        ctx.set_option("pr_reorder", 1)
+        b, _ = dg.pagerank(20)
+    finally:
+        ctx.set_option("pr_reorder", 1)
+    assert l1rel(a, ref) <= PR_TOL and l1rel(b, ref) <= PR_TOL
+    from llama_amd.graph import GpuError
+    seg_bits(40)
+    with pytest.raises(GpuError):
+        dg.pagerank(1)
