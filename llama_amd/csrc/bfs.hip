@@ -33,11 +33,20 @@ constexpr int kEdgesPerBlock = 2048;
 constexpr int32_t kUnreached = INT_MAX - 1;
 constexpr uint32_t kBuCap = kBfsRowCap;
 
-struct Counters {
+// Level counters, striped over kStripes cache lines: workgroups add their
+// totals to stripe blockIdx % kStripes, so hundreds of thousands of
+// workgroups do not serialise on one L2 address.  Queue slots are still
+// allocated from stripe 0's n_next (fetch-add needs one counter); the host
+// sums the stripes.
+constexpr int kStripes = 64;
+struct alignas(128) Counters {
   unsigned long long n_next;     // vertices discovered this level
   unsigned long long m_next;     // their out-edges
   unsigned long long examined;   // edges inspected this level
 };
+static_assert(sizeof(Counters) == 128, "one stripe per 128-byte line");
+
+__device__ inline Counters* stripe(Counters* c) { return c + (blockIdx.x & (kStripes - 1)); }
 
 __global__ void k_bfs_init(int32_t* lv, uint64_t* visited, const uint64_t* excluded, uint64_t V,
                            uint64_t words) {
@@ -68,7 +77,8 @@ __global__ void k_fill_u64(uint64_t* p, uint64_t n, uint64_t v) {
   if (i < n) p[i] = v;
 }
 
-__global__ void k_zero_counters(Counters* c) {
+__global__ void k_zero_counters(Counters* c) {  // <<<1, kStripes>>>
+  c += threadIdx.x;
   c->n_next = 0;
   c->m_next = 0;
   c->examined = 0;
@@ -91,7 +101,7 @@ __device__ inline void wave_append(bool take, uint32_t v, uint32_t outdeg, uint3
   uint64_t md = take ? outdeg : 0;
   for (int o = 32; o > 0; o >>= 1) md += __shfl_xor(md, o, 64);
   if (lane == leader && m) {
-    atomicAdd(&c->m_next, (unsigned long long)md);
+    atomicAdd(&stripe(c)->m_next, (unsigned long long)md);
     if (mq) qbase = atomicAdd(&c->n_next, (unsigned long long)__popcll(mq));
   }
   qbase = __shfl(qbase, leader, 64);
@@ -143,7 +153,7 @@ __global__ __launch_bounds__(kBlock) void k_td_expand(
     }
     wave_append(take, v, od, q_next, c);
   }
-  if (threadIdx.x == 0) atomicAdd(&c->examined, (unsigned long long)(x1 - x0));
+  if (threadIdx.x == 0) atomicAdd(&stripe(c)->examined, (unsigned long long)(x1 - x0));
 }
 
 __global__ void k_queue_to_bitmap(const uint32_t* q, uint64_t n, uint64_t* bm) {
@@ -262,8 +272,9 @@ __global__ __launch_bounds__(kBlock) void k_bu_rows(const RowItem* items, const 
     unsigned long long t = 0;
     for (int i = 0; i < kBlock / 64; i++) t += red[threadIdx.x][i];
     if (t) {
-      unsigned long long* dst = threadIdx.x == 0 ? &c->n_next : threadIdx.x == 1 ? &c->m_next
-                                                                                : &c->examined;
+      Counters* cs = stripe(c);
+      unsigned long long* dst = threadIdx.x == 0 ? &cs->n_next : threadIdx.x == 1 ? &cs->m_next
+                                                                                  : &cs->examined;
       atomicAdd(dst, t);
     }
   }
@@ -302,9 +313,15 @@ __global__ void k_count_reached(const int32_t* lv, uint64_t V, unsigned long lon
 }
 
 Counters read_counters(const DevBuf& c, hipStream_t s) {
-  Counters h;
-  LLG_HIP(hipMemcpyAsync(&h, c.p, sizeof h, hipMemcpyDeviceToHost, s));
+  Counters st[kStripes];
+  LLG_HIP(hipMemcpyAsync(st, c.p, sizeof st, hipMemcpyDeviceToHost, s));
   LLG_HIP(hipStreamSynchronize(s));
+  Counters h = st[0];
+  for (int i = 1; i < kStripes; i++) {
+    h.n_next += st[i].n_next;
+    h.m_next += st[i].m_next;
+    h.examined += st[i].examined;
+  }
   return h;
 }
 
@@ -326,9 +343,10 @@ BfsResult bfs_core(ll_gpu_graph* g, const Csr& fwd, const Csr& rev, const RowBlo
   q[1].alloc(V * 4, s);
   qdeg.alloc(V * 4, s);
   qofs.alloc((V + 1) * 8, s);
-  cnt.alloc(sizeof(Counters), s);
+  cnt.alloc(kStripes * sizeof(Counters), s);
   k_bfs_init<<<grid_for(std::max(V, words), 256), 256, 0, s>>>(lv, visited.as<uint64_t>(),
                                                               excluded, V, words);
+  k_zero_counters<<<1, kStripes, 0, s>>>(cnt.as<Counters>());
   k_bfs_root<<<1, 1, 0, s>>>(lv, visited.as<uint64_t>(), q[0].as<uint32_t>(), root, fdeg,
                              cnt.as<Counters>());
   LLG_HIP(hipGetLastError());
@@ -351,7 +369,7 @@ BfsResult bfs_core(ll_gpu_graph* g, const Csr& fwd, const Csr& rev, const RowBlo
       k_queue_to_bitmap<<<grid_for(nq, 256), 256, 0, s>>>(q[cur].as<uint32_t>(), nq,
                                                           fbm.as<uint64_t>());
     }
-    k_zero_counters<<<1, 1, 0, s>>>(cnt.as<Counters>());
+    k_zero_counters<<<1, kStripes, 0, s>>>(cnt.as<Counters>());
     if (bottom_up) {
       k_fill_u64<<<grid_for(words, 256), 256, 0, s>>>(nbm.as<uint64_t>(), words, 0);
       {
@@ -388,7 +406,7 @@ BfsResult bfs_core(ll_gpu_graph* g, const Csr& fwd, const Csr& rev, const RowBlo
       if (n_f == 0) break;
       if (shrinking && n_f < V / 24) {
         bottom_up = false;
-        k_zero_counters<<<1, 1, 0, s>>>(cnt.as<Counters>());
+        k_zero_counters<<<1, kStripes, 0, s>>>(cnt.as<Counters>());
         k_bitmap_to_queue<<<grid_for(words * 64, 256), 256, 0, s>>>(
             fbm.as<uint64_t>(), V, fdeg, q[cur].as<uint32_t>(), cnt.as<Counters>());
         LLG_HIP(hipGetLastError());

@@ -95,16 +95,20 @@ __device__ inline uint32_t finalize(const MsArgs<W>& a, uint32_t r, W acc) {
   return uint32_t(Lane<W>::popc(nw));
 }
 
-__device__ inline void block_count(unsigned long long* out, uint32_t x) {
-  __shared__ uint32_t red[kBlock / 64];
-  for (int o = 32; o > 0; o >>= 1) x += __shfl_down(x, o, 64);
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = x;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    uint32_t t = 0;
-    for (int i = 0; i < kBlock / 64; i++) t += red[i];
-    if (t) atomicAdd(out, (unsigned long long)t);
-  }
+// "Something was discovered" flag: a workgroup that discovered vertices
+// stores 1 into its stripe (blockIdx % kFlagStripes, one 128-byte line each),
+// so the stores of hundreds of thousands of workgroups spread over many L2
+// lines; the host ORs the stripes.
+constexpr int kFlagStripes = 64;
+constexpr int kFlagStride = 16;  // u64 per 128-byte line
+
+__device__ inline void block_flag(unsigned long long* flags, uint32_t x) {
+  if (__syncthreads_or(x != 0) && threadIdx.x == 0)
+    flags[(blockIdx.x % kFlagStripes) * kFlagStride] = 1ull;
+}
+
+__global__ void k_zero_flags(unsigned long long* flags) {  // <<<1, kFlagStripes>>>
+  flags[threadIdx.x * kFlagStride] = 0ull;
 }
 
 template <typename W>
@@ -171,7 +175,7 @@ __global__ __launch_bounds__(kBlock) void k_ms_pull(MsArgs<W> a) {
     }
     (void)red;
   }
-  block_count(a.found, cnt);
+  block_flag(a.found, cnt);
 }
 
 template <typename W>
@@ -186,7 +190,7 @@ __global__ void k_ms_fixup(MsArgs<W> a, uint64_t nitems) {
       cnt = finalize(a, it.row_begin, acc);
     }
   }
-  block_count(a.found, cnt);
+  block_flag(a.found, cnt);
 }
 
 template <typename W>
@@ -258,7 +262,7 @@ void msbfs_batch(ll_gpu_graph* g, int level, const int64_t* d_roots, int n, int 
   fr.alloc(V * sizeof(W), s);
   nx.alloc(V * sizeof(W), s);
   part.alloc(std::max<uint64_t>(rb.n, 1) * sizeof(W), s);
-  found.alloc(8, s);
+  found.alloc(kFlagStripes * kFlagStride * 8, s);
   const W full = Lane<W>::low(Lane<W>::kBits);
   const W used = Lane<W>::low(n);
   k_ms_init<W><<<grid_for(V, 256), 256, 0, s>>>(vis.as<W>(), fr.as<W>(), V, full & ~used);
@@ -281,17 +285,19 @@ void msbfs_batch(ll_gpu_graph* g, int level, const int64_t* d_roots, int n, int 
     a.n_src = n;
     a.full = full;
     a.found = found.as<unsigned long long>();
-    k_zero_ull<<<1, 1, 0, s>>>(a.found);
+    k_zero_flags<<<1, kFlagStripes, 0, s>>>(a.found);
     {
       TimedLaunch t(ctx, "k_ms_pull");
       k_ms_pull<W><<<unsigned(rb.n), kBlock, 0, s>>>(a);
     }
     k_ms_fixup<W><<<grid_for(rb.n, 256), 256, 0, s>>>(a, rb.n);
     LLG_HIP(hipGetLastError());
-    uint64_t h = 0;
-    LLG_HIP(hipMemcpyAsync(&h, found.p, 8, hipMemcpyDeviceToHost, s));
+    uint64_t fl[kFlagStripes * kFlagStride];
+    LLG_HIP(hipMemcpyAsync(fl, found.p, sizeof fl, hipMemcpyDeviceToHost, s));
     LLG_HIP(hipStreamSynchronize(s));
     lv_count++;
+    uint64_t h = 0;
+    for (int i = 0; i < kFlagStripes; i++) h |= fl[i * kFlagStride];
     if (h == 0) break;
     std::swap(fr, nx);
   }
