@@ -212,10 +212,10 @@ __global__ __launch_bounds__(kBlock) void k_bu_rows(const RowItem* items, const 
       if (lane == 0) bits[k * (kBlock / 64) + wave] = m;
     }
     __syncthreads();
+    bool found = false;
     if (unv) {
       const uint32_t lo = uint32_t(rofs[threadIdx.x] - e0), hi = uint32_t(rofs[threadIdx.x + 1] - e0);
       exam = hi - lo;
-      bool found = false;
       for (uint32_t j = lo; j < hi && !found;) {
         const uint32_t w = j >> 6, b0 = j & 63;
         const uint32_t nb = min(64u - b0, hi - j);
@@ -224,14 +224,20 @@ __global__ __launch_bounds__(kBlock) void k_bu_rows(const RowItem* items, const 
         j += nb;
       }
       if (found) {
-        lv ? (void)(lv[r] = next_lv) : (void)0;
-        const uint64_t bit = uint64_t(1) << (r & 63);
-        const unsigned long long old = atomicOr((unsigned long long*)&nbm[r >> 6], (unsigned long long)bit);
-        if (!(old & bit)) {
-          n_new = 1;
-          m_new = outdeg[r];
-        }
+        if (lv) lv[r] = next_lv;
+        n_new = 1;  // a stream row belongs to this item only: no other writer sets its bit
+        m_new = outdeg[r];
       }
+    }
+    // the wave's rows are consecutive: its found bits land in at most two
+    // bitmap words, set with one atomic each
+    const uint64_t fm = __ballot(found);
+    if (lane == 0 && fm) {
+      const uint32_t r0 = it.row_begin + (threadIdx.x & ~63u);
+      const uint32_t sh = r0 & 63;
+      atomicOr((unsigned long long*)&nbm[r0 >> 6], (unsigned long long)(fm << sh));
+      if (sh && (fm >> (64 - sh)))
+        atomicOr((unsigned long long*)&nbm[(r0 >> 6) + 1], (unsigned long long)(fm >> (64 - sh)));
     }
   } else {
     const uint32_t r = it.row_begin;
