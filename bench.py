@@ -256,7 +256,8 @@ def main() -> int:
         del src, dst
 
     if args.sweep_pr:
-        for reorder, seg in ((0, 0), (1, 0), (1, 17), (1, 18), (1, 19), (1, 20)):
+        for reorder, seg in ((0, 0), (1, 0), (0, 18), (0, 19), (1, 17), (1, 18), (1, 19),
+                             (1, 20)):
             ctx.set_option("pr_reorder", reorder)
             ctx.set_option("pr_seg_bits", seg)
             for cap in (1024, 2048, 4096):
