@@ -101,7 +101,9 @@ def bench_bfs_batch(dg, g, V: int, args, single: dict | None) -> dict:
         per_src = single["seconds"] / single["sources"]
         seq = per_src * len(roots)
         out["sequential_estimate_seconds"] = round(seq, 4)
-        out["speedup_vs_sequential"] = round(seq / out["lanes64"]["seconds"], 2)
+        out["speedup_vs_sequential_lanes64"] = round(seq / out["lanes64"]["seconds"], 2)
+        best = min(v["seconds"] for k, v in out.items() if isinstance(v, dict))
+        out["speedup_vs_sequential_best"] = round(seq / best, 2)
     log(f"batched BFS x{len(roots)}: {out}")
     return out
 
@@ -200,6 +202,9 @@ def main() -> int:
                     help="time one PageRank iteration for every row-block capacity x "
                          "contribution order x column segmentation (report on stderr) "
                          "before the timed steps")
+    ap.add_argument("--sweep-configs", default="",
+                    help="with --sweep-pr: comma-separated reorder:seg_bits:cap triples "
+                         "(default: the full grid)")
     ap.add_argument("--reorder", type=int, default=1,
                     help="1 = degree-ordered contribution slots in PageRank (default)")
     ap.add_argument("--seg-bits", type=int, default=-1,
@@ -256,11 +261,16 @@ def main() -> int:
         del src, dst
 
     if args.sweep_pr:
-        for reorder, seg in ((0, 0), (1, 0), (0, 18), (0, 19), (1, 17), (1, 18), (1, 19),
-                             (1, 20)):
+        if args.sweep_configs:
+            configs = [tuple(int(x) for x in c.split(":")) for c in args.sweep_configs.split(",")]
+        else:
+            configs = [(r, sb, c) for r, sb in ((0, 0), (1, 0), (0, 18), (0, 19), (1, 17),
+                                                (1, 18), (1, 19), (1, 20))
+                       for c in (1024, 2048, 4096)]
+        for reorder, seg, cap in configs:
             ctx.set_option("pr_reorder", reorder)
             ctx.set_option("pr_seg_bits", seg)
-            for cap in (1024, 2048, 4096):
+            if True:
                 dg.drop_snapshots()
                 ctx.profile(True)
                 dg.pagerank(2, 0.85, cap=cap, want_ranks=False)

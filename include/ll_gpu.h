@@ -198,7 +198,7 @@ int ll_gpu_tuned(ll_gpu_ctx* ctx, const char* prefix, int* value);
 /* Tuning knobs: "pr_reorder" (1 = gather PageRank contributions in
  * descending-out-degree slot order; default), "pr_cap" (default row-block
  * capacity, 2048), "pr_seg_bits" (pull over column segments of 2^N
- * contribution slots; 0 = off, -1 = automatic: 19 above 2^20 vertices). */
+ * contribution slots; 0 = off, -1 = automatic = off). */
 int ll_gpu_set_option(ll_gpu_ctx* ctx, const char* name, long value);
 /* Drop the cached snapshot materialisations (the next call rebuilds them). */
 void ll_gpu_graph_drop_snapshots(ll_gpu_graph* g);

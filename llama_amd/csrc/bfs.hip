@@ -169,7 +169,21 @@ __global__ void k_queue_to_bitmap(const uint32_t* q, uint64_t n, uint64_t* bm) {
 // loads its run of neighbour IDs with coalesced reads and packs "neighbour is
 // in the frontier" into bits with one ballot per 64 edges (the frontier bitmap
 // is V/8 bytes and stays in L2); each unvisited row then tests its bit range.
-// No early exit per vertex, but no divergent per-lane edge walks either.
+// No early exit per vertex, but no divergent per-lane edge walks either --
+// except in items whose open rows hold under a quarter of the edges, where
+// each open row walks its own edges and stops at the first parent.
+// Sum over the workgroup, returned to every thread.
+__device__ inline uint32_t block_sum_u32(uint32_t x) {
+  __shared__ uint32_t part[kBlock / 64];
+  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = x;
+  __syncthreads();
+  uint32_t t = 0;
+  for (int i = 0; i < kBlock / 64; i++) t += part[i];
+  __syncthreads();  // part may be reused by the next call
+  return t;
+}
+
 __global__ __launch_bounds__(kBlock) void k_bu_rows(const RowItem* items, const uint64_t* ofs,
                                                     const uint32_t* adj, const uint32_t* outdeg,
                                                     const uint64_t* fbm, const uint64_t* visited,
@@ -191,43 +205,61 @@ __global__ __launch_bounds__(kBlock) void k_bu_rows(const RowItem* items, const 
     __syncthreads();
     const uint64_t e0 = rofs[0];
     const uint32_t ne = uint32_t(rofs[nrows] - e0);
-    // all neighbour IDs first, then all bitmap words, so loads overlap
-    constexpr uint32_t kPer = kMaxEdges / kBlock;
-    uint32_t u[kPer];
-    uint64_t fw[kPer];
-#pragma unroll
-    for (uint32_t k = 0; k < kPer; k++) {
-      const uint32_t i = k * kBlock + threadIdx.x;
-      u[k] = i < ne ? adj[e0 + i] : 0u;
-    }
-#pragma unroll
-    for (uint32_t k = 0; k < kPer; k++) {
-      const uint32_t i = k * kBlock + threadIdx.x;
-      fw[k] = i < ne ? fbm[u[k] >> 6] : 0ull;
-    }
-#pragma unroll
-    for (uint32_t k = 0; k < kPer; k++) {
-      const uint32_t i = k * kBlock + threadIdx.x;
-      const uint64_t m = __ballot(i < ne && ((fw[k] >> (u[k] & 63)) & 1));
-      if (lane == 0) bits[k * (kBlock / 64) + wave] = m;
-    }
-    __syncthreads();
+    const uint32_t lo = threadIdx.x < nrows ? uint32_t(rofs[threadIdx.x] - e0) : 0u;
+    const uint32_t hi = threadIdx.x < nrows ? uint32_t(rofs[threadIdx.x + 1] - e0) : 0u;
+    // Late levels leave a few open rows per item (mostly vertices the search
+    // never reaches): when their edges are under a quarter of the item's, each
+    // open row walks its own edges and stops at the first frontier parent.
+    const uint32_t open_edges = block_sum_u32(unv ? hi - lo : 0u);
     bool found = false;
-    if (unv) {
-      const uint32_t lo = uint32_t(rofs[threadIdx.x] - e0), hi = uint32_t(rofs[threadIdx.x + 1] - e0);
-      exam = hi - lo;
-      for (uint32_t j = lo; j < hi && !found;) {
-        const uint32_t w = j >> 6, b0 = j & 63;
-        const uint32_t nb = min(64u - b0, hi - j);
-        const uint64_t mask = (nb == 64 ? ~0ull : ((1ull << nb) - 1)) << b0;
-        found = (bits[w] & mask) != 0;
-        j += nb;
+    if (open_edges * 4 < ne) {
+      if (unv) {
+        for (uint32_t j = lo; j < hi; j++) {
+          const uint32_t u = adj[e0 + j];
+          exam++;
+          if ((fbm[u >> 6] >> (u & 63)) & 1) {
+            found = true;
+            break;
+          }
+        }
       }
-      if (found) {
-        if (lv) lv[r] = next_lv;
-        n_new = 1;  // a stream row belongs to this item only: no other writer sets its bit
-        m_new = outdeg[r];
+    } else {
+      // all neighbour IDs first, then all bitmap words, so loads overlap
+      constexpr uint32_t kPer = kMaxEdges / kBlock;
+      uint32_t u[kPer];
+      uint64_t fw[kPer];
+#pragma unroll
+      for (uint32_t k = 0; k < kPer; k++) {
+        const uint32_t i = k * kBlock + threadIdx.x;
+        u[k] = i < ne ? adj[e0 + i] : 0u;
       }
+#pragma unroll
+      for (uint32_t k = 0; k < kPer; k++) {
+        const uint32_t i = k * kBlock + threadIdx.x;
+        fw[k] = i < ne ? fbm[u[k] >> 6] : 0ull;
+      }
+#pragma unroll
+      for (uint32_t k = 0; k < kPer; k++) {
+        const uint32_t i = k * kBlock + threadIdx.x;
+        const uint64_t m = __ballot(i < ne && ((fw[k] >> (u[k] & 63)) & 1));
+        if (lane == 0) bits[k * (kBlock / 64) + wave] = m;
+      }
+      __syncthreads();
+      if (unv) {
+        exam = hi - lo;
+        for (uint32_t j = lo; j < hi && !found;) {
+          const uint32_t w = j >> 6, b0 = j & 63;
+          const uint32_t nb = min(64u - b0, hi - j);
+          const uint64_t mask = (nb == 64 ? ~0ull : ((1ull << nb) - 1)) << b0;
+          found = (bits[w] & mask) != 0;
+          j += nb;
+        }
+      }
+    }
+    if (found) {
+      if (lv) lv[r] = next_lv;
+      n_new = 1;  // a stream row belongs to this item only: no other writer sets its bit
+      m_new = outdeg[r];
     }
     // the wave's rows are consecutive: its found bits land in at most two
     // bitmap words, set with one atomic each
@@ -318,10 +350,12 @@ __global__ void k_count_reached(const int32_t* lv, uint64_t V, unsigned long lon
   block_atomic_add(out, (v < V && lv[v] != kUnreached) ? 1ull : 0ull);
 }
 
-Counters read_counters(const DevBuf& c, hipStream_t s) {
-  Counters st[kStripes];
-  LLG_HIP(hipMemcpyAsync(st, c.p, sizeof st, hipMemcpyDeviceToHost, s));
-  LLG_HIP(hipStreamSynchronize(s));
+Counters read_counters(ll_gpu_ctx* ctx, const DevBuf& c) {
+  static_assert(kStripes * sizeof(Counters) <= kReadbackBytes, "stripes fit the read-back buffer");
+  const Counters* st = static_cast<const Counters*>(ctx->readback);
+  LLG_HIP(hipMemcpyAsync(ctx->readback, c.p, kStripes * sizeof(Counters), hipMemcpyDeviceToHost,
+                         ctx->stream));
+  LLG_HIP(hipStreamSynchronize(ctx->stream));
   Counters h = st[0];
   for (int i = 1; i < kStripes; i++) {
     h.n_next += st[i].n_next;
@@ -356,7 +390,7 @@ BfsResult bfs_core(ll_gpu_graph* g, const Csr& fwd, const Csr& rev, const RowBlo
   k_bfs_root<<<1, 1, 0, s>>>(lv, visited.as<uint64_t>(), q[0].as<uint32_t>(), root, fdeg,
                              cnt.as<Counters>());
   LLG_HIP(hipGetLastError());
-  Counters h = read_counters(cnt, s);
+  Counters h = read_counters(ctx, cnt);
   BfsResult r;
   uint64_t nq = h.n_next;       // queue entries (vertices with out-edges)
   uint64_t n_f = 1;             // frontier vertices
@@ -398,7 +432,7 @@ BfsResult bfs_core(ll_gpu_graph* g, const Csr& fwd, const Csr& rev, const RowBlo
           q[cur ^ 1].as<uint32_t>(), cnt.as<Counters>());
     }
     LLG_HIP(hipGetLastError());
-    h = read_counters(cnt, s);
+    h = read_counters(ctx, cnt);
     r.examined += h.examined;
     const uint64_t n_next = h.n_next;
     lvl++;
@@ -416,7 +450,7 @@ BfsResult bfs_core(ll_gpu_graph* g, const Csr& fwd, const Csr& rev, const RowBlo
         k_bitmap_to_queue<<<grid_for(words * 64, 256), 256, 0, s>>>(
             fbm.as<uint64_t>(), V, fdeg, q[cur].as<uint32_t>(), cnt.as<Counters>());
         LLG_HIP(hipGetLastError());
-        nq = read_counters(cnt, s).n_next;
+        nq = read_counters(ctx, cnt).n_next;
       }
     } else {
       // top-down n_next counts the queued vertices (those with out-edges)

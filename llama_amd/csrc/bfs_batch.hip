@@ -95,6 +95,18 @@ __device__ inline uint32_t finalize(const MsArgs<W>& a, uint32_t r, W acc) {
   return uint32_t(Lane<W>::popc(nw));
 }
 
+// Sum over the workgroup, returned to every thread.
+__device__ inline uint32_t block_sum_u32(uint32_t x) {
+  __shared__ uint32_t part[kBlock / 64];
+  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = x;
+  __syncthreads();
+  uint32_t t = 0;
+  for (int i = 0; i < kBlock / 64; i++) t += part[i];
+  __syncthreads();  // part may be reused by the next call
+  return t;
+}
+
 // "Something was discovered" flag: a workgroup that discovered vertices
 // stores 1 into its stripe (blockIdx % kFlagStripes, one 128-byte line each),
 // so the stores of hundreds of thousands of workgroups spread over many L2
@@ -130,6 +142,29 @@ __global__ __launch_bounds__(kBlock) void k_ms_pull(MsArgs<W> a) {
     __syncthreads();
     const uint64_t e0 = rofs[0];
     const uint32_t ne = uint32_t(rofs[nrows] - e0);
+    // few open edges (late sweeps): each open row walks its own edges and stops
+    // once every lane has reached it
+    const uint32_t lo0 = threadIdx.x < nrows ? uint32_t(rofs[threadIdx.x] - e0) : 0u;
+    const uint32_t hi0 = threadIdx.x < nrows ? uint32_t(rofs[threadIdx.x + 1] - e0) : 0u;
+    const uint32_t open_edges = block_sum_u32(open ? hi0 - lo0 : 0u);
+    if (open_edges * 4 < ne) {
+      if (threadIdx.x < nrows) {
+        const uint32_t r = it.row_begin + threadIdx.x;
+        if (open) {
+          const W vr = a.vis[r];
+          W acc = Lane<W>::zero();
+          for (uint32_t j = lo0; j < hi0; j++) {
+            acc = acc | a.fr[a.adj[e0 + j]];
+            if ((acc | vr) == a.full) break;
+          }
+          cnt = finalize(a, r, acc);
+        } else {
+          a.nx[r] = Lane<W>::zero();
+        }
+      }
+      block_flag(a.found, cnt);
+      return;
+    }
     constexpr uint32_t kPer = (kCap + kCap / 2) / kBlock;
     uint32_t idx[kPer];
 #pragma unroll
@@ -292,8 +327,10 @@ void msbfs_batch(ll_gpu_graph* g, int level, const int64_t* d_roots, int n, int 
     }
     k_ms_fixup<W><<<grid_for(rb.n, 256), 256, 0, s>>>(a, rb.n);
     LLG_HIP(hipGetLastError());
-    uint64_t fl[kFlagStripes * kFlagStride];
-    LLG_HIP(hipMemcpyAsync(fl, found.p, sizeof fl, hipMemcpyDeviceToHost, s));
+    static_assert(kFlagStripes * kFlagStride * 8 <= kReadbackBytes, "flags fit the read-back buffer");
+    const uint64_t* fl = static_cast<const uint64_t*>(ctx->readback);
+    LLG_HIP(hipMemcpyAsync(ctx->readback, found.p, kFlagStripes * kFlagStride * 8,
+                           hipMemcpyDeviceToHost, s));
     LLG_HIP(hipStreamSynchronize(s));
     lv_count++;
     uint64_t h = 0;

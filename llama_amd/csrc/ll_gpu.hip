@@ -143,6 +143,8 @@ int ll_gpu_create(int device, ll_gpu_ctx** out) {
     hipError_t e = hipHostMalloc(&c->pinned, kStageBytes, hipHostMallocDefault);
     LLG_CHECK(e == hipSuccess, LL_GPU_ENOMEM, "pinned staging allocation failed");
     c->pinned_bytes = kStageBytes;
+    e = hipHostMalloc(&c->readback, kReadbackBytes, hipHostMallocDefault);
+    LLG_CHECK(e == hipSuccess, LL_GPU_ENOMEM, "pinned read-back allocation failed");
     for (auto& ev : c->pinned_ev) {
       LLG_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
       LLG_HIP(hipEventRecord(ev, c->stream));
@@ -168,6 +170,7 @@ void ll_gpu_destroy(ll_gpu_ctx* c) {
   for (auto& ev : c->pinned_ev)
     if (ev) (void)hipEventDestroy(ev);
   if (c->pinned) (void)hipHostFree(c->pinned);
+  if (c->readback) (void)hipHostFree(c->readback);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }

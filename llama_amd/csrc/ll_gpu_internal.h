@@ -96,6 +96,8 @@ struct KernelTimer {
 
 }  // namespace llgpu
 
+constexpr size_t kReadbackBytes = size_t(64) << 10;
+
 struct ll_gpu_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -105,6 +107,7 @@ struct ll_gpu_ctx {
   void* pinned = nullptr;      // staging for uploads
   size_t pinned_bytes = 0;
   hipEvent_t pinned_ev[2] = {nullptr, nullptr};
+  void* readback = nullptr;    // small pinned buffer for per-level counter reads
   std::map<std::string, int> tuned;    // autotuner decisions
   std::map<std::string, long> options; // ll_gpu_set_option knobs
 };

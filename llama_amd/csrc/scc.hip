@@ -177,11 +177,11 @@ __global__ void k_labels_out(const uint32_t* label, int64_t* out, uint64_t V,
   block_atomic_add(nscc, root ? 1ull : 0ull);
 }
 
-unsigned long long read_ull(const DevBuf& b, hipStream_t s) {
-  unsigned long long h = 0;
-  LLG_HIP(hipMemcpyAsync(&h, b.p, 8, hipMemcpyDeviceToHost, s));
-  LLG_HIP(hipStreamSynchronize(s));
-  return h;
+// One counter through the context's pinned read-back buffer.
+unsigned long long read_ull(ll_gpu_ctx* ctx, const DevBuf& b) {
+  LLG_HIP(hipMemcpyAsync(ctx->readback, b.p, 8, hipMemcpyDeviceToHost, ctx->stream));
+  LLG_HIP(hipStreamSynchronize(ctx->stream));
+  return *static_cast<const unsigned long long*>(ctx->readback);
 }
 
 }  // namespace
@@ -217,7 +217,7 @@ SccResult scc_device(ll_gpu_graph* g, int level, DevBuf& labels64) {
                                 cnt.as<unsigned long long>());
       LLG_HIP(hipGetLastError());
       r.sweeps++;
-      if (read_ull(cnt, s) == 0) break;
+      if (read_ull(g->ctx, cnt) == 0) break;
     }
   };
   trim(4);
@@ -225,7 +225,7 @@ SccResult scc_device(ll_gpu_graph* g, int level, DevBuf& labels64) {
   k_zero<<<1, 1, 0, s>>>(cnt.as<unsigned long long>());
   k_pivot<<<gV, 256, 0, s>>>(label.as<uint32_t>(), dout, din, V, cnt.as<unsigned long long>());
   LLG_HIP(hipGetLastError());
-  const unsigned long long best = read_ull(cnt, s);
+  const unsigned long long best = read_ull(g->ctx, cnt);
   if (best) {
     const uint32_t pivot = uint32_t(best & 0xffffffffull);
     excl.alloc(words * 8, s);
@@ -261,14 +261,14 @@ SccResult scc_device(ll_gpu_graph* g, int level, DevBuf& labels64) {
                                     mark.as<uint32_t>(), minlab.as<uint32_t>(), V,
                                     cnt.as<unsigned long long>());
     LLG_HIP(hipGetLastError());
-    if (read_ull(cnt, s) == 0) break;
+    if (read_ull(g->ctx, cnt) == 0) break;
     for (;;) {
       k_zero<<<1, 1, 0, s>>>(cnt.as<unsigned long long>());
       k_color_prop<<<gV, 256, 0, s>>>(io, ia, label.as<uint32_t>(), color.as<uint32_t>(), V,
                                       cnt.as<unsigned long long>());
       LLG_HIP(hipGetLastError());
       r.sweeps++;
-      if (read_ull(cnt, s) == 0) break;
+      if (read_ull(g->ctx, cnt) == 0) break;
     }
     k_root_mark<<<gV, 256, 0, s>>>(label.as<uint32_t>(), color.as<uint32_t>(),
                                    mark.as<uint32_t>(), V);
@@ -278,7 +278,7 @@ SccResult scc_device(ll_gpu_graph* g, int level, DevBuf& labels64) {
                                    mark.as<uint32_t>(), V, cnt.as<unsigned long long>());
       LLG_HIP(hipGetLastError());
       r.sweeps++;
-      if (read_ull(cnt, s) == 0) break;
+      if (read_ull(g->ctx, cnt) == 0) break;
     }
     k_comp_min<<<gV, 256, 0, s>>>(label.as<uint32_t>(), color.as<uint32_t>(),
                                   mark.as<uint32_t>(), minlab.as<uint32_t>(), V);
@@ -292,7 +292,7 @@ SccResult scc_device(ll_gpu_graph* g, int level, DevBuf& labels64) {
   k_labels_out<<<gV, 256, 0, s>>>(label.as<uint32_t>(), labels64.as<int64_t>(), V,
                                   cnt.as<unsigned long long>());
   LLG_HIP(hipGetLastError());
-  r.num_scc = read_ull(cnt, s);
+  r.num_scc = read_ull(g->ctx, cnt);
   return r;
 }
 
