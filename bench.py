@@ -209,6 +209,8 @@ def main() -> int:
                     help="1 = degree-ordered contribution slots in PageRank (default)")
     ap.add_argument("--seg-bits", type=int, default=-1,
                     help="PageRank column segments of 2^N slots (0 = off, -1 = automatic)")
+    ap.add_argument("--item-sort", type=int, default=-1,
+                    help="PageRank items sorted by slot (1), not (0), automatic (-1)")
     ap.add_argument("--cpu-iters", type=int, default=6,
                     help="PageRank iterations timed for the CPU baseline (0 = skip)")
     ap.add_argument("--seed", type=int, default=1)
@@ -238,6 +240,8 @@ def main() -> int:
 
     ctx = GpuContext(0)
     ctx.set_option("pr_reorder", args.reorder)
+    ctx.set_option("pr_seg_bits", args.seg_bits)
+    ctx.set_option("pr_item_sort", args.item_sort)
     t1 = time.time()
     dg = DeviceGraph(ctx, g)
     ctx.sync()
@@ -262,20 +266,23 @@ def main() -> int:
 
     if args.sweep_pr:
         if args.sweep_configs:
-            configs = [tuple(int(x) for x in c.split(":")) for c in args.sweep_configs.split(",")]
+            configs = [tuple(int(x) for x in (c + ":0").split(":")[:4])
+                       for c in args.sweep_configs.split(",")]
         else:
-            configs = [(r, sb, c) for r, sb in ((0, 0), (1, 0), (0, 18), (0, 19), (1, 17),
-                                                (1, 18), (1, 19), (1, 20))
+            configs = [(r, sb, c, so) for r, sb, so in ((0, 0, 0), (1, 0, 0), (1, 0, 1),
+                                                        (0, 19, 0), (1, 19, 0), (1, 20, 0))
                        for c in (1024, 2048, 4096)]
-        for reorder, seg, cap in configs:
+        for reorder, seg, cap, srt in configs:
             ctx.set_option("pr_reorder", reorder)
             ctx.set_option("pr_seg_bits", seg)
+            ctx.set_option("pr_item_sort", srt)
             if True:
                 dg.drop_snapshots()
                 ctx.profile(True)
                 dg.pagerank(2, 0.85, cap=cap, want_ranks=False)
                 ctx.sync()
-                build_ms = sum(ctx.kernel_time(k)[1] for k in ("pr_reorder", "pr_segments"))
+                build_ms = sum(ctx.kernel_time(k)[1]
+                               for k in ("pr_reorder", "pr_segments", "pr_item_sort"))
                 ctx.profile(False)
                 ctx.profile(True)
                 t0 = time.perf_counter()
@@ -284,11 +291,13 @@ def main() -> int:
                 t = time.perf_counter() - t0
                 it_ms = iteration_ms(ctx, args.iters)
                 ctx.profile(False)
-                log(f"sweep reorder={reorder} seg_bits={seg} cap={cap}: iteration {it_ms:.4f} ms "
+                log(f"sweep reorder={reorder} seg_bits={seg} cap={cap} item_sort={srt}: "
+                    f"iteration {it_ms:.4f} ms "
                     f"({(8 * E + 16 * V) / it_ms / 1e6:.1f} GB/s), run {t * 1e3:.2f} ms, "
                     f"layout build {build_ms:.2f} ms")
         ctx.set_option("pr_reorder", args.reorder)
         ctx.set_option("pr_seg_bits", args.seg_bits)
+        ctx.set_option("pr_item_sort", args.item_sort)
 
     def step():
         dg.drop_snapshots()
@@ -308,7 +317,7 @@ def main() -> int:
     it_kernels = {k: ctx.kernel_time(k) for k in PR_ITERATION_KERNELS}
     mat = {k: ctx.kernel_time(k) for k in ("k_seg_count", "k_seg_fill", "k_live_count",
                                            "k_compact", "k_degrees", "pr_reorder",
-                                           "pr_segments")}
+                                           "pr_segments", "pr_item_sort")}
     avg_it_ms = iteration_ms(ctx, args.iters * args.steps)
     ctx.profile(False)
     log(f"timed {args.steps} steps done")

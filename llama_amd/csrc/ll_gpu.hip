@@ -114,6 +114,9 @@ void drop_snapshots(ll_gpu_graph* g) {
   g->pr_level = -1;
   g->pr_inv.release();
   g->pr_adj.release();
+  g->pr_sadj.release();
+  g->pr_sperm.release();
+  g->pr_sort_cap = 0;
   g->pr_seg = llgpu::PrSegments();
   g->w_level = -1;
   g->wcsr = Csr();
@@ -186,7 +189,7 @@ void* ll_gpu_stream(ll_gpu_ctx* c) { return c ? static_cast<void*>(c->stream) : 
 
 int ll_gpu_set_option(ll_gpu_ctx* c, const char* name, long value) {
   if (!c || !name) return LL_GPU_EINVAL;
-  static const char* known[] = {"pr_reorder", "pr_cap", "pr_seg_bits"};
+  static const char* known[] = {"pr_reorder", "pr_cap", "pr_seg_bits", "pr_item_sort"};
   for (const char* k : known)
     if (std::strcmp(k, name) == 0) {
       c->options[name] = value;

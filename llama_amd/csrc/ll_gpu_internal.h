@@ -185,6 +185,8 @@ struct ll_gpu_graph {
   // PageRank locality order of the in-CSR (option pr_reorder)
   int pr_level = -1;
   llgpu::DevBuf pr_inv, pr_adj;
+  llgpu::DevBuf pr_sadj, pr_sperm;  // pr_adj with every row-block item sorted by slot
+  uint32_t pr_sort_cap = 0;         // row-block capacity the sorted copy was built for
   llgpu::PrSegments pr_seg;   // column segments of the in-CSR in slot order
   // weighted in-CSR (rows = heads, sources ascending) + map to out-edge positions
   int w_level = -1;

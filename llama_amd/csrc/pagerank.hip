@@ -38,6 +38,7 @@ struct PrArgs {
   Neu* part;              // per item partials of split rows
   double* diff;           // L1 change accumulator (last iteration only)
   Neu* pair_part;         // segmented pull: compensated partial per pair (row of the CSR)
+  const uint16_t* perm;   // sorted items: original in-item position of each entry
   float base, d;
 };
 
@@ -72,7 +73,11 @@ __device__ inline void block_add_diff(const PrArgs& a, float x) {
   }
 }
 
-template <uint32_t CAP, bool kPartial>
+// kSorted: every item's entries are sorted by slot (a wave's gathers then
+// share cache lines) and a.perm restores the row order in LDS, so the row
+// sums are the same as without sorting; pieces of long rows are summed in
+// slot order.
+template <uint32_t CAP, bool kPartial, bool kSorted>
 __global__ __launch_bounds__(kBlock) void k_pr_pull(PrArgs a) {
   __shared__ float vals[CAP + CAP / 2];
   __shared__ uint64_t rofs[kRowsPerItem + 1];
@@ -89,10 +94,13 @@ __global__ __launch_bounds__(kBlock) void k_pr_pull(PrArgs a) {
     constexpr uint32_t kPer = (CAP + CAP / 2) / kBlock;
     uint32_t idx[kPer];
     float v[kPer];
+    uint32_t pos[kPer];
 #pragma unroll
     for (uint32_t k = 0; k < kPer; k++) {
       uint32_t i = threadIdx.x + k * kBlock;
       idx[k] = i < ne ? a.adj[e0 + i] : 0u;
+      pos[k] = i;
+      if (kSorted && i < ne) pos[k] = a.perm[e0 + i];
     }
 #pragma unroll
     for (uint32_t k = 0; k < kPer; k++) {
@@ -102,7 +110,7 @@ __global__ __launch_bounds__(kBlock) void k_pr_pull(PrArgs a) {
 #pragma unroll
     for (uint32_t k = 0; k < kPer; k++) {
       uint32_t i = threadIdx.x + k * kBlock;
-      if (i < ne) vals[i] = v[k];
+      if (i < ne) vals[pos[k]] = v[k];
     }
     __syncthreads();
     if (threadIdx.x < nrows) {
@@ -197,6 +205,51 @@ __global__ void k_relabel(const uint32_t* adj, const uint32_t* inv, uint32_t* ou
   if (e < E) out[e] = inv[adj[e]];
 }
 
+// Per-item slot order (option pr_item_sort): the entries of every row-block
+// item (a run of short rows, or one piece of a long row; at most 8192) sorted
+// ascending by contribution slot with a bitonic network in LDS, keeping each
+// entry's original in-item position.
+constexpr uint32_t kSortMax = 8192;
+static_assert(kSortMax >= kPiece && kSortMax >= kMaxCap + kMaxCap / 2, "items fit the sort");
+__global__ __launch_bounds__(kBlock) void k_item_sort(const RowItem* items, const uint64_t* ofs,
+                                                      const uint32_t* adj, uint32_t* sadj,
+                                                      uint16_t* sperm) {
+  __shared__ uint64_t key[kSortMax];
+  const RowItem it = items[blockIdx.x];
+  uint64_t lo, hi;
+  if (it.npieces == 0) {
+    lo = ofs[it.row_begin];
+    hi = ofs[it.row_end];
+  } else {
+    lo = ofs[it.row_begin] + uint64_t(it.piece) * kPiece;
+    hi = min(ofs[it.row_begin + 1], lo + kPiece);
+  }
+  const uint32_t n = uint32_t(hi - lo);
+  uint32_t P = 1;
+  while (P < n) P <<= 1;
+  for (uint32_t i = threadIdx.x; i < P; i += kBlock)
+    key[i] = i < n ? (uint64_t(adj[lo + i]) << 16) | i : ~uint64_t(0);
+  __syncthreads();
+  for (uint32_t k = 2; k <= P; k <<= 1)
+    for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+      for (uint32_t i = threadIdx.x; i < P; i += kBlock) {
+        const uint32_t l = i ^ j;
+        if (l > i) {
+          const uint64_t x = key[i], y = key[l];
+          if ((x > y) == ((i & k) == 0)) {
+            key[i] = y;
+            key[l] = x;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  for (uint32_t i = threadIdx.x; i < n; i += kBlock) {
+    sadj[lo + i] = uint32_t(key[i] >> 16);
+    sperm[lo + i] = uint16_t(key[i] & 0xffffu);
+  }
+}
+
 __global__ void k_widen(const float* in, double* out, uint64_t n) {
   uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i < n) out[i] = double(in[i]);
@@ -206,15 +259,21 @@ __global__ void k_zero_f64(double* p) { *p = 0.0; }
 
 }  // namespace
 
-template <bool kPartial>
-void launch_pr_pull(uint32_t cap, unsigned nitems, const PrArgs& a, hipStream_t s) {
+template <bool kPartial, bool kSorted>
+void launch_pr_pull_t(uint32_t cap, unsigned nitems, const PrArgs& a, hipStream_t s) {
   switch (cap) {
-    case 512: k_pr_pull<512, kPartial><<<nitems, kBlock, 0, s>>>(a); break;
-    case 1024: k_pr_pull<1024, kPartial><<<nitems, kBlock, 0, s>>>(a); break;
-    case 2048: k_pr_pull<2048, kPartial><<<nitems, kBlock, 0, s>>>(a); break;
-    case 4096: k_pr_pull<4096, kPartial><<<nitems, kBlock, 0, s>>>(a); break;
+    case 512: k_pr_pull<512, kPartial, kSorted><<<nitems, kBlock, 0, s>>>(a); break;
+    case 1024: k_pr_pull<1024, kPartial, kSorted><<<nitems, kBlock, 0, s>>>(a); break;
+    case 2048: k_pr_pull<2048, kPartial, kSorted><<<nitems, kBlock, 0, s>>>(a); break;
+    case 4096: k_pr_pull<4096, kPartial, kSorted><<<nitems, kBlock, 0, s>>>(a); break;
     default: throw Error{LL_GPU_EINVAL, "unsupported PageRank row-block capacity"};
   }
+}
+
+template <bool kPartial>
+void launch_pr_pull(uint32_t cap, bool sorted, unsigned nitems, const PrArgs& a, hipStream_t s) {
+  if (sorted && !kPartial) launch_pr_pull_t<false, true>(cap, nitems, a, s);
+  else launch_pr_pull_t<kPartial, false>(cap, nitems, a, s);
 }
 
 // Run `iters` pull iterations; leaves FP32 ranks in rank_final (device).
@@ -249,6 +308,8 @@ void pagerank_device(ll_gpu_graph* g, int level, int iters, double damping, uint
                                                          g->pr_inv.as<uint32_t>(), V);
       LLG_HIP(hipGetLastError());
       g->pr_adj.release();
+      g->pr_sadj.release();
+      g->pr_sperm.release();
       g->pr_level = in.level;
     }
     inv = g->pr_inv.as<uint32_t>();
@@ -273,6 +334,24 @@ void pagerank_device(ll_gpu_graph* g, int level, int iters, double damping, uint
   }
   const RowBlocks& rb = segmented ? pr_segment_rowblocks(ctx, ps, cap)
                                   : ensure_rowblocks(g, LL_GPU_DIR_IN, level, cap, kPiece);
+  // entries of every item sorted by slot (plain pull over slots only)
+  const long sort_opt = ctx->options.count("pr_item_sort") ? ctx->options["pr_item_sort"] : -1;
+  const bool item_sort = reorder && !segmented && sort_opt > 0;
+  const uint16_t* perm = nullptr;
+  if (item_sort) {
+    if (!g->pr_sadj.p || g->pr_sort_cap != cap) {
+      TimedLaunch t(ctx, "pr_item_sort");
+      g->pr_sadj.alloc(std::max<uint64_t>(in.E, 1) * 4, s);
+      g->pr_sperm.alloc(std::max<uint64_t>(in.E, 1) * 2, s);
+      k_item_sort<<<unsigned(rb.n), kBlock, 0, s>>>(rb.items.as<RowItem>(), in.ofs.as<uint64_t>(),
+                                                    adj, g->pr_sadj.as<uint32_t>(),
+                                                    g->pr_sperm.as<uint16_t>());
+      LLG_HIP(hipGetLastError());
+      g->pr_sort_cap = cap;
+    }
+    adj = g->pr_sadj.as<uint32_t>();
+    perm = g->pr_sperm.as<uint16_t>();
+  }
   DevBuf rank[2], contrib[2], part, diff, pair_part;
   for (int i = 0; i < 2; i++) {
     rank[i].alloc(V * 4, s);
@@ -300,6 +379,7 @@ void pagerank_device(ll_gpu_graph* g, int level, int iters, double damping, uint
     a.contrib_out = contrib[cur ^ 1].as<float>();
     a.part = part.as<Neu>();
     a.pair_part = pair_part.as<Neu>();
+    a.perm = perm;
     a.diff = (it == iters - 1 && last_diff) ? diff.as<double>() : nullptr;
     a.base = float((1.0 - damping) / double(N));
     a.d = float(damping);
@@ -309,7 +389,7 @@ void pagerank_device(ll_gpu_graph* g, int level, int iters, double damping, uint
       if (rb.n) {
         {
           TimedLaunch t(ctx, "k_pr_pull");
-          launch_pr_pull<true>(cap, unsigned(rb.n), pa, s);
+          launch_pr_pull<true>(cap, false, unsigned(rb.n), pa, s);
         }
         TimedLaunch t(ctx, "k_pr_fixup");
         k_pr_fixup<true><<<grid_for(rb.n, 256), 256, 0, s>>>(pa, rb.n);
@@ -320,7 +400,7 @@ void pagerank_device(ll_gpu_graph* g, int level, int iters, double damping, uint
     } else {
       {
         TimedLaunch t(ctx, "k_pr_pull");
-        launch_pr_pull<false>(cap, unsigned(rb.n), a, s);
+        launch_pr_pull<false>(cap, item_sort, unsigned(rb.n), a, s);
       }
       TimedLaunch t(ctx, "k_pr_fixup");
       k_pr_fixup<false><<<grid_for(rb.n, 256), 256, 0, s>>>(a, rb.n);

@@ -199,3 +199,41 @@ def test_pagerank_segments_by_vertex_id(ctx, ml, seg_bits):
     seg_bits(40)
     with pytest.raises(GpuError):
         dg.pagerank(1)
+
+
+@pytest.mark.parametrize("cap", [512, 4096])
+def test_pagerank_item_sorted_gathers(ctx, ml, cap):
+    """Option pr_item_sort: every row-block item's entries sorted by slot
+    (LDS restores row order; pieces of long rows summed in slot order)."""
+    g, o = ml
+    gpu_case(f"pagerank pr_item_sort cap={cap}")
+    dg = DeviceGraph(ctx, g)
+    ref, _ = o.pagerank(20)
+    try:
+        ctx.set_option("pr_item_sort", 1)
+        a, _ = dg.pagerank(20, cap=cap)
+        dg.drop_snapshots()
+        b, _ = dg.pagerank(20, cap=cap)
+    finally:
+        ctx.set_option("pr_item_sort", -1)
+    assert l1rel(a, ref) <= PR_TOL
+    assert np.array_equal(a, b)
+
+
+def test_pagerank_item_sorted_hub(ctx):
+    gpu_case("pagerank pr_item_sort hub 20000")
+    k = 20000
+    n = k + 3
+    rng = np.random.default_rng(2)
+    src = np.concatenate([rng.permutation(np.arange(1, k + 1)), [0, n - 1]])
+    dst = np.concatenate([np.zeros(k, int), [n - 2, 0]])
+    g = HostGraph(n)
+    g.add_level(src, dst)
+    ref, _ = oracle.OracleGraph.from_host(g).pagerank(5)
+    dg = DeviceGraph(ctx, g)
+    try:
+        ctx.set_option("pr_item_sort", 1)
+        r, _ = dg.pagerank(5, cap=1024)
+    finally:
+        ctx.set_option("pr_item_sort", -1)
+    assert l1rel(r, ref) <= PR_TOL
