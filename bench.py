@@ -174,9 +174,10 @@ def bench_wpr(ctx, args) -> dict:
     return out
 
 
-# one PageRank pull iteration: pull (per vertex, or per (row, segment) pair),
-# split-row fix-up, and the segment merge when the pull is segmented
-PR_ITERATION_KERNELS = ("k_pr_pull", "k_pr_fixup", "k_pr_merge")
+# one PageRank pull iteration: pull (per vertex, per (row, segment) pair, or
+# the hot pass of a hot/cold split), the cold pass of a split, split-row
+# fix-ups, and the segment merge when the pull is segmented
+PR_ITERATION_KERNELS = ("k_pr_pull", "k_pr_pull_cold", "k_pr_fixup", "k_pr_merge")
 
 
 def iteration_ms(ctx, iterations):
@@ -203,7 +204,7 @@ def main() -> int:
                          "contribution order x column segmentation (report on stderr) "
                          "before the timed steps")
     ap.add_argument("--sweep-configs", default="",
-                    help="with --sweep-pr: comma-separated reorder:seg_bits:cap triples "
+                    help="with --sweep-pr: comma-separated reorder:seg_bits:cap[:item_sort[:hot_bits]] "
                          "(default: the full grid)")
     ap.add_argument("--reorder", type=int, default=1,
                     help="1 = degree-ordered contribution slots in PageRank (default)")
@@ -211,6 +212,8 @@ def main() -> int:
                     help="PageRank column segments of 2^N slots (0 = off, -1 = automatic)")
     ap.add_argument("--item-sort", type=int, default=-1,
                     help="PageRank items sorted by slot (1), not (0), automatic (-1)")
+    ap.add_argument("--hot-bits", type=int, default=-1,
+                    help="PageRank hot/cold split at slot 2^N (0 = off, -1 = automatic)")
     ap.add_argument("--cpu-iters", type=int, default=6,
                     help="PageRank iterations timed for the CPU baseline (0 = skip)")
     ap.add_argument("--seed", type=int, default=1)
@@ -242,6 +245,7 @@ def main() -> int:
     ctx.set_option("pr_reorder", args.reorder)
     ctx.set_option("pr_seg_bits", args.seg_bits)
     ctx.set_option("pr_item_sort", args.item_sort)
+    ctx.set_option("pr_hot_bits", args.hot_bits)
     t1 = time.time()
     dg = DeviceGraph(ctx, g)
     ctx.sync()
@@ -266,23 +270,27 @@ def main() -> int:
 
     if args.sweep_pr:
         if args.sweep_configs:
-            configs = [tuple(int(x) for x in (c + ":0").split(":")[:4])
+            configs = [tuple(int(x) for x in (c + ":0:0").split(":")[:5])
                        for c in args.sweep_configs.split(",")]
         else:
-            configs = [(r, sb, c, so) for r, sb, so in ((0, 0, 0), (1, 0, 0), (1, 0, 1),
-                                                        (0, 19, 0), (1, 19, 0), (1, 20, 0))
+            configs = [(r, sb, c, so, hb)
+                       for r, sb, so, hb in ((0, 0, 0, 0), (1, 0, 0, 0), (1, 0, 1, 0),
+                                             (1, 19, 0, 0), (1, 0, 0, 18), (1, 0, 0, 19),
+                                             (1, 0, 0, 20), (1, 0, 0, 21))
                        for c in (1024, 2048, 4096)]
-        for reorder, seg, cap, srt in configs:
+        for reorder, seg, cap, srt, hb in configs:
             ctx.set_option("pr_reorder", reorder)
             ctx.set_option("pr_seg_bits", seg)
             ctx.set_option("pr_item_sort", srt)
+            ctx.set_option("pr_hot_bits", hb)
             if True:
                 dg.drop_snapshots()
                 ctx.profile(True)
                 dg.pagerank(2, 0.85, cap=cap, want_ranks=False)
                 ctx.sync()
                 build_ms = sum(ctx.kernel_time(k)[1]
-                               for k in ("pr_reorder", "pr_segments", "pr_item_sort"))
+                               for k in ("pr_reorder", "pr_segments", "pr_item_sort",
+                                         "pr_hotcold"))
                 ctx.profile(False)
                 ctx.profile(True)
                 t0 = time.perf_counter()
@@ -291,13 +299,15 @@ def main() -> int:
                 t = time.perf_counter() - t0
                 it_ms = iteration_ms(ctx, args.iters)
                 ctx.profile(False)
-                log(f"sweep reorder={reorder} seg_bits={seg} cap={cap} item_sort={srt}: "
+                log(f"sweep reorder={reorder} seg_bits={seg} cap={cap} item_sort={srt} "
+                    f"hot_bits={hb}: "
                     f"iteration {it_ms:.4f} ms "
                     f"({(8 * E + 16 * V) / it_ms / 1e6:.1f} GB/s), run {t * 1e3:.2f} ms, "
                     f"layout build {build_ms:.2f} ms")
         ctx.set_option("pr_reorder", args.reorder)
         ctx.set_option("pr_seg_bits", args.seg_bits)
         ctx.set_option("pr_item_sort", args.item_sort)
+        ctx.set_option("pr_hot_bits", args.hot_bits)
 
     def step():
         dg.drop_snapshots()
@@ -317,7 +327,7 @@ def main() -> int:
     it_kernels = {k: ctx.kernel_time(k) for k in PR_ITERATION_KERNELS}
     mat = {k: ctx.kernel_time(k) for k in ("k_seg_count", "k_seg_fill", "k_live_count",
                                            "k_compact", "k_degrees", "pr_reorder",
-                                           "pr_segments", "pr_item_sort")}
+                                           "pr_segments", "pr_item_sort", "pr_hotcold")}
     avg_it_ms = iteration_ms(ctx, args.iters * args.steps)
     ctx.profile(False)
     log(f"timed {args.steps} steps done")

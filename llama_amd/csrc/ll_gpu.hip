@@ -118,6 +118,7 @@ void drop_snapshots(ll_gpu_graph* g) {
   g->pr_sperm.release();
   g->pr_sort_cap = 0;
   g->pr_seg = llgpu::PrSegments();
+  g->pr_hc = llgpu::PrHotCold();
   g->w_level = -1;
   g->wcsr = Csr();
   g->wperm.release();
@@ -189,7 +190,8 @@ void* ll_gpu_stream(ll_gpu_ctx* c) { return c ? static_cast<void*>(c->stream) : 
 
 int ll_gpu_set_option(ll_gpu_ctx* c, const char* name, long value) {
   if (!c || !name) return LL_GPU_EINVAL;
-  static const char* known[] = {"pr_reorder", "pr_cap", "pr_seg_bits", "pr_item_sort"};
+  static const char* known[] = {"pr_reorder", "pr_cap", "pr_seg_bits", "pr_item_sort",
+                                "pr_hot_bits"};
   for (const char* k : known)
     if (std::strcmp(k, name) == 0) {
       c->options[name] = value;

@@ -156,6 +156,17 @@ struct PrSegments {
   RowBlocks rb;        // row blocks of `pairs`
 };
 
+// Hot/cold split of the PageRank in-CSR (built by build_pr_hotcold): per row,
+// the in-edges whose source slot is below 2^hb (hot) and the others (cold),
+// each in snapshot order, as two CSRs over all V rows.
+struct PrHotCold {
+  bool valid = false;
+  int level = -1;
+  uint32_t hb = 0;
+  Csr hot, cold;
+  RowBlocks hot_rb, cold_rb;
+};
+
 struct DirMirror {
   std::vector<DevBuf> vt;   // VtEntry[V+1] per level
   std::vector<DevBuf> et;   // uint64_t[words] per level
@@ -188,6 +199,7 @@ struct ll_gpu_graph {
   llgpu::DevBuf pr_sadj, pr_sperm;  // pr_adj with every row-block item sorted by slot
   uint32_t pr_sort_cap = 0;         // row-block capacity the sorted copy was built for
   llgpu::PrSegments pr_seg;   // column segments of the in-CSR in slot order
+  llgpu::PrHotCold pr_hc;     // hot/cold split of the in-CSR in slot order
   // weighted in-CSR (rows = heads, sources ascending) + map to out-edge positions
   int w_level = -1;
   llgpu::Csr wcsr;
@@ -232,6 +244,9 @@ void drop_snapshots(ll_gpu_graph* g);
 void build_pr_segments(ll_gpu_graph* g, const Csr& in, const uint32_t* inv, uint32_t zb,
                        PrSegments& ps);
 const RowBlocks& pr_segment_rowblocks(ll_gpu_ctx* ctx, PrSegments& ps, uint32_t cap);
+// Hot/cold split of `in` whose adjacency holds slots (`slot_adj`): hot = slot < 2^hb.
+void build_pr_hotcold(ll_gpu_graph* g, const Csr& in, const uint32_t* slot_adj, uint32_t hb,
+                      PrHotCold& hc);
 
 // Device-wide exclusive scan of n values into out (n+1 entries, total last).
 void scan_u32_to_u64(const uint32_t* in, uint64_t* out, uint64_t n, hipStream_t s);

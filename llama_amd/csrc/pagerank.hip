@@ -50,12 +50,23 @@ __device__ inline float finalize(const PrArgs& a, uint32_t r, float S) {
   return a.diff ? fabsf(v - a.rank_in[r]) : 0.f;
 }
 
-// Row r of the CSR is done: a vertex (finalise) or a pair (store the partial).
-template <bool kPartial>
-__device__ inline float finish(const PrArgs& a, uint32_t r, const Neu& s) {
-  if (kPartial) {
+// What a pull does with a finished row sum:
+//   kFinal    -- finalise the vertex (rank, contribution);
+//   kPartOut  -- store the compensated partial in pair_part[row] (a (row,
+//                segment) pair, or a vertex in the hot pass of a hot/cold split);
+//   kPartIn   -- start from pair_part[row] (the hot partial), then finalise.
+enum PullMode : int { kFinal = 0, kPartOut = 1, kPartIn = 2 };
+
+template <int kMode>
+__device__ inline float finish(const PrArgs& a, uint32_t r, Neu s) {
+  if (kMode == kPartOut) {
     a.pair_part[r] = s;
     return 0.f;
+  }
+  if (kMode == kPartIn) {
+    Neu t = a.pair_part[r];
+    t.merge(s);
+    s = t;
   }
   return finalize(a, r, s.value());
 }
@@ -77,7 +88,7 @@ __device__ inline void block_add_diff(const PrArgs& a, float x) {
 // share cache lines) and a.perm restores the row order in LDS, so the row
 // sums are the same as without sorting; pieces of long rows are summed in
 // slot order.
-template <uint32_t CAP, bool kPartial, bool kSorted>
+template <uint32_t CAP, int kMode, bool kSorted>
 __global__ __launch_bounds__(kBlock) void k_pr_pull(PrArgs a) {
   __shared__ float vals[CAP + CAP / 2];
   __shared__ uint64_t rofs[kRowsPerItem + 1];
@@ -117,7 +128,7 @@ __global__ __launch_bounds__(kBlock) void k_pr_pull(PrArgs a) {
       const uint32_t lo = uint32_t(rofs[threadIdx.x] - e0), hi = uint32_t(rofs[threadIdx.x + 1] - e0);
       Neu s = Neu::zero();
       for (uint32_t j = lo; j < hi; j++) s.add(vals[j]);
-      dsum = finish<kPartial>(a, it.row_begin + threadIdx.x, s);
+      dsum = finish<kMode>(a, it.row_begin + threadIdx.x, s);
     }
   } else {
     const uint32_t r = it.row_begin;
@@ -132,7 +143,7 @@ __global__ __launch_bounds__(kBlock) void k_pr_pull(PrArgs a) {
     if (threadIdx.x == 0) {
       Neu t = red[0];
       for (int i = 1; i < kBlock / 64; i++) t.merge(red[i]);
-      if (it.npieces == 1) dsum = finish<kPartial>(a, r, t);
+      if (it.npieces == 1) dsum = finish<kMode>(a, r, t);
       else a.part[blockIdx.x] = t;
     }
   }
@@ -140,7 +151,7 @@ __global__ __launch_bounds__(kBlock) void k_pr_pull(PrArgs a) {
 }
 
 // Rows split into several pieces: combine the partials in piece order.
-template <bool kPartial>
+template <int kMode>
 __global__ void k_pr_fixup(PrArgs a, uint64_t nitems) {
   uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   float dsum = 0.f;
@@ -149,7 +160,7 @@ __global__ void k_pr_fixup(PrArgs a, uint64_t nitems) {
     if (it.npieces > 1 && it.piece == 0) {
       Neu t = a.part[i];
       for (uint32_t p = 1; p < it.npieces; p++) t.merge(a.part[i + p]);
-      dsum = finish<kPartial>(a, it.row_begin, t);
+      dsum = finish<kMode>(a, it.row_begin, t);
     }
   }
   block_add_diff(a, dsum);
@@ -259,21 +270,21 @@ __global__ void k_zero_f64(double* p) { *p = 0.0; }
 
 }  // namespace
 
-template <bool kPartial, bool kSorted>
+template <int kMode, bool kSorted>
 void launch_pr_pull_t(uint32_t cap, unsigned nitems, const PrArgs& a, hipStream_t s) {
   switch (cap) {
-    case 512: k_pr_pull<512, kPartial, kSorted><<<nitems, kBlock, 0, s>>>(a); break;
-    case 1024: k_pr_pull<1024, kPartial, kSorted><<<nitems, kBlock, 0, s>>>(a); break;
-    case 2048: k_pr_pull<2048, kPartial, kSorted><<<nitems, kBlock, 0, s>>>(a); break;
-    case 4096: k_pr_pull<4096, kPartial, kSorted><<<nitems, kBlock, 0, s>>>(a); break;
+    case 512: k_pr_pull<512, kMode, kSorted><<<nitems, kBlock, 0, s>>>(a); break;
+    case 1024: k_pr_pull<1024, kMode, kSorted><<<nitems, kBlock, 0, s>>>(a); break;
+    case 2048: k_pr_pull<2048, kMode, kSorted><<<nitems, kBlock, 0, s>>>(a); break;
+    case 4096: k_pr_pull<4096, kMode, kSorted><<<nitems, kBlock, 0, s>>>(a); break;
     default: throw Error{LL_GPU_EINVAL, "unsupported PageRank row-block capacity"};
   }
 }
 
-template <bool kPartial>
+template <int kMode>
 void launch_pr_pull(uint32_t cap, bool sorted, unsigned nitems, const PrArgs& a, hipStream_t s) {
-  if (sorted && !kPartial) launch_pr_pull_t<false, true>(cap, nitems, a, s);
-  else launch_pr_pull_t<kPartial, false>(cap, nitems, a, s);
+  if (sorted && kMode == kFinal) launch_pr_pull_t<kFinal, true>(cap, nitems, a, s);
+  else launch_pr_pull_t<kMode, false>(cap, nitems, a, s);
 }
 
 // Run `iters` pull iterations; leaves FP32 ranks in rank_final (device).
@@ -334,9 +345,24 @@ void pagerank_device(ll_gpu_graph* g, int level, int iters, double damping, uint
   }
   const RowBlocks& rb = segmented ? pr_segment_rowblocks(ctx, ps, cap)
                                   : ensure_rowblocks(g, LL_GPU_DIR_IN, level, cap, kPiece);
+  // hot/cold split at slot 2^pr_hot_bits (0 = off, -1 = automatic = off)
+  long hot_bits = ctx->options.count("pr_hot_bits") ? ctx->options["pr_hot_bits"] : -1;
+  if (hot_bits < 0) hot_bits = 0;
+  LLG_CHECK(hot_bits <= 31, LL_GPU_EINVAL, "pr_hot_bits must be <= 31");
+  const bool hotcold = reorder && !segmented && hot_bits > 0;
+  PrHotCold& hc = g->pr_hc;
+  if (hotcold) {
+    if (!(hc.valid && hc.level == in.level && hc.hb == uint32_t(hot_bits))) {
+      TimedLaunch t(ctx, "pr_hotcold");
+      build_pr_hotcold(g, in, adj, uint32_t(hot_bits), hc);
+    }
+    if (!hc.hot_rb.valid || hc.hot_rb.cap != cap) build_rowblocks(ctx, hc.hot, cap, kPiece, hc.hot_rb);
+    if (!hc.cold_rb.valid || hc.cold_rb.cap != cap)
+      build_rowblocks(ctx, hc.cold, cap, kPiece, hc.cold_rb);
+  }
   // entries of every item sorted by slot (plain pull over slots only)
   const long sort_opt = ctx->options.count("pr_item_sort") ? ctx->options["pr_item_sort"] : -1;
-  const bool item_sort = reorder && !segmented && sort_opt > 0;
+  const bool item_sort = reorder && !segmented && !hotcold && sort_opt > 0;
   const uint16_t* perm = nullptr;
   if (item_sort) {
     if (!g->pr_sadj.p || g->pr_sort_cap != cap) {
@@ -357,8 +383,11 @@ void pagerank_device(ll_gpu_graph* g, int level, int iters, double damping, uint
     rank[i].alloc(V * 4, s);
     contrib[i].alloc(V * 4, s);
   }
-  part.alloc(std::max<uint64_t>(rb.n, 1) * sizeof(Neu), s);
+  uint64_t nparts = std::max<uint64_t>(rb.n, 1);
+  if (hotcold) nparts = std::max(nparts, std::max(hc.hot_rb.n, hc.cold_rb.n));
+  part.alloc(nparts * sizeof(Neu), s);
   if (segmented) pair_part.alloc(std::max<uint64_t>(ps.P, 1) * sizeof(Neu), s);
+  if (hotcold) pair_part.alloc(V * sizeof(Neu), s);  // hot partial per vertex
   diff.alloc(8, s);
   const float N = float(V);
   k_pr_init<<<grid_for(V, 256), 256, 0, s>>>(outdeg, inv, rank[0].as<float>(),
@@ -389,21 +418,47 @@ void pagerank_device(ll_gpu_graph* g, int level, int iters, double damping, uint
       if (rb.n) {
         {
           TimedLaunch t(ctx, "k_pr_pull");
-          launch_pr_pull<true>(cap, false, unsigned(rb.n), pa, s);
+          launch_pr_pull<kPartOut>(cap, false, unsigned(rb.n), pa, s);
         }
         TimedLaunch t(ctx, "k_pr_fixup");
-        k_pr_fixup<true><<<grid_for(rb.n, 256), 256, 0, s>>>(pa, rb.n);
+        k_pr_fixup<kPartOut><<<grid_for(rb.n, 256), 256, 0, s>>>(pa, rb.n);
       }
       TimedLaunch t(ctx, "k_pr_merge");
       k_pr_merge<<<unsigned(ps.nb), kBlock, 0, s>>>(a, ps.pair_row.as<uint32_t>(),
                                                     ps.mt.as<uint32_t>(), ps.S, ps.nb, V);
+    } else if (hotcold) {
+      // hot pass: partial of every vertex over its hot in-edges (L2-resident
+      // gathers); cold pass: continue from it over the cold in-edges, finalise
+      PrArgs h = a;
+      h.items = hc.hot_rb.items.as<RowItem>();
+      h.ofs = hc.hot.ofs.as<uint64_t>();
+      h.adj = hc.hot.adj.as<uint32_t>();
+      h.diff = nullptr;
+      {
+        TimedLaunch t(ctx, "k_pr_pull");
+        launch_pr_pull<kPartOut>(cap, false, unsigned(hc.hot_rb.n), h, s);
+      }
+      {
+        TimedLaunch t(ctx, "k_pr_fixup");
+        k_pr_fixup<kPartOut><<<grid_for(hc.hot_rb.n, 256), 256, 0, s>>>(h, hc.hot_rb.n);
+      }
+      PrArgs c = a;
+      c.items = hc.cold_rb.items.as<RowItem>();
+      c.ofs = hc.cold.ofs.as<uint64_t>();
+      c.adj = hc.cold.adj.as<uint32_t>();
+      {
+        TimedLaunch t(ctx, "k_pr_pull_cold");
+        launch_pr_pull<kPartIn>(cap, false, unsigned(hc.cold_rb.n), c, s);
+      }
+      TimedLaunch t(ctx, "k_pr_fixup");
+      k_pr_fixup<kPartIn><<<grid_for(hc.cold_rb.n, 256), 256, 0, s>>>(c, hc.cold_rb.n);
     } else {
       {
         TimedLaunch t(ctx, "k_pr_pull");
-        launch_pr_pull<false>(cap, item_sort, unsigned(rb.n), a, s);
+        launch_pr_pull<kFinal>(cap, item_sort, unsigned(rb.n), a, s);
       }
       TimedLaunch t(ctx, "k_pr_fixup");
-      k_pr_fixup<false><<<grid_for(rb.n, 256), 256, 0, s>>>(a, rb.n);
+      k_pr_fixup<kFinal><<<grid_for(rb.n, 256), 256, 0, s>>>(a, rb.n);
     }
     LLG_HIP(hipGetLastError());
     cur ^= 1;

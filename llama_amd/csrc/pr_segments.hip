@@ -72,7 +72,7 @@ __global__ __launch_bounds__(kB) void k_seg_hist(const RowItem* items, uint64_t 
   for (uint64_t r0 = lo; r0 < hi; r0 += kB) {
     const uint64_t e = r0 + threadIdx.x;
     const bool valid = e < hi;
-    const uint32_t sg = valid ? slot[e] >> zb : 0u;
+    const uint32_t sg = valid ? min(slot[e] >> zb, S - 1) : 0u;
     const uint64_t m = wave_match(sg, valid, nbits);
     // the lowest lane of each group adds the group's size
     if (valid && (m & ((uint64_t(1) << lane) - 1)) == 0) atomicAdd(&h[sg], uint32_t(__popcll(m)));
@@ -83,11 +83,14 @@ __global__ __launch_bounds__(kB) void k_seg_hist(const RowItem* items, uint64_t 
 
 // Stable scatter of every item's edges to their segment-major positions:
 // base[s * n + item] (exclusive scan of cnt) + rank among the item's earlier
-// edges of the same segment.
+// edges of the same segment.  Positions from `split` on go to out_hi (at
+// pos - split), the others to out_lo.  The segment of a slot is
+// min(slot >> zb, S - 1), so the last segment may be wider.
 __global__ __launch_bounds__(kB) void k_seg_scatter(const RowItem* items, uint64_t n,
                                                     const uint64_t* ofs, const uint32_t* slot,
                                                     uint32_t zb, uint32_t S, uint32_t nbits,
-                                                    const uint64_t* base, uint32_t* out_slot,
+                                                    const uint64_t* base, uint64_t split,
+                                                    uint32_t* out_lo, uint32_t* out_hi,
                                                     uint32_t* out_row) {
   __shared__ uint64_t rofs[kRowsPerItem + 1];
   __shared__ uint64_t sbase[kMaxSegments];
@@ -113,7 +116,7 @@ __global__ __launch_bounds__(kB) void k_seg_scatter(const RowItem* items, uint64
     const uint64_t e = r0 + threadIdx.x;
     const bool valid = e < hi;
     const uint32_t sl = valid ? slot[e] : 0u;
-    const uint32_t sg = sl >> zb;
+    const uint32_t sg = min(sl >> zb, S - 1);
     const uint64_t m = wave_match(sg, valid, nbits);
     const uint32_t rank = __popcll(m & ((uint64_t(1) << lane) - 1));
     if (valid && (m >> lane) == 1u) wc[w][sg] = rank + 1;  // highest lane of its group
@@ -131,7 +134,8 @@ __global__ __launch_bounds__(kB) void k_seg_scatter(const RowItem* items, uint64
         }
         row += a;
       }
-      out_slot[pos] = sl;
+      if (pos < split) out_lo[pos] = sl;
+      else out_hi[pos - split] = sl;
       out_row[pos] = row;
     }
     __syncthreads();
@@ -276,8 +280,9 @@ void build_pr_segments(ll_gpu_graph* g, const Csr& in, const uint32_t* inv, uint
   rowtmp.alloc(std::max<uint64_t>(E, 1) * 4, s);
   if (n) {
     k_seg_scatter<<<unsigned(n), kB, 0, s>>>(irb.items.as<RowItem>(), n, in.ofs.as<uint64_t>(),
-                                             sl, zb, S, nbits, base.as<uint64_t>(),
-                                             ps.pairs.adj.as<uint32_t>(), rowtmp.as<uint32_t>());
+                                             sl, zb, S, nbits, base.as<uint64_t>(), E,
+                                             ps.pairs.adj.as<uint32_t>(), nullptr,
+                                             rowtmp.as<uint32_t>());
     LLG_HIP(hipGetLastError());
   }
   slot.release();
@@ -322,6 +327,73 @@ void build_pr_segments(ll_gpu_graph* g, const Csr& in, const uint32_t* inv, uint
   ps.pairs.V = P;
   ps.pairs.E = E;
   ps.valid = true;
+}
+
+namespace {
+
+// hot_ofs[r] = first hot position of row r: rows ascend over the hot
+// positions, so each run start fills the offsets of the rows up to its own.
+__global__ void k_hot_ofs(const uint32_t* row, uint64_t e_hot, uint64_t V, uint64_t* hot_ofs) {
+  const uint64_t p = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (p > e_hot) return;
+  const uint64_t prev = p == 0 ? 0 : uint64_t(row[p - 1]) + 1;  // first row not yet covered
+  const uint64_t cur = p == e_hot ? V : uint64_t(row[p]);
+  for (uint64_t r = prev; r <= cur && r <= V; r++) hot_ofs[r] = p;
+}
+
+__global__ void k_cold_ofs(const uint64_t* ofs, const uint64_t* hot_ofs, uint64_t V,
+                           uint64_t* cold_ofs) {
+  const uint64_t r = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (r <= V) cold_ofs[r] = ofs[r] - hot_ofs[r];
+}
+
+}  // namespace
+
+void build_pr_hotcold(ll_gpu_graph* g, const Csr& in, const uint32_t* slot_adj, uint32_t hb,
+                      PrHotCold& hc) {
+  ll_gpu_ctx* ctx = g->ctx;
+  hipStream_t s = ctx->stream;
+  const uint64_t V = in.V, E = in.E;
+  hc = PrHotCold();
+  hc.level = in.level;
+  hc.hb = hb;
+  const RowBlocks& irb = ensure_rowblocks(g, LL_GPU_DIR_IN, in.level, 2048, kPiece);
+  const uint64_t n = irb.n;
+  constexpr uint32_t S = 2, nbits = 1;
+  DevBuf cnt, base, rowtmp;
+  cnt.alloc(S * n * 4, s);
+  base.alloc((S * n + 1) * 8, s);
+  k_seg_hist<<<unsigned(n), kB, 0, s>>>(irb.items.as<RowItem>(), n, in.ofs.as<uint64_t>(),
+                                        slot_adj, hb, S, nbits, cnt.as<uint32_t>());
+  LLG_HIP(hipGetLastError());
+  scan_u32_to_u64(cnt.as<uint32_t>(), base.as<uint64_t>(), S * n, s);
+  uint64_t e_hot = 0;
+  LLG_HIP(hipMemcpyAsync(&e_hot, base.as<uint64_t>() + n, 8, hipMemcpyDeviceToHost, s));
+  LLG_HIP(hipStreamSynchronize(s));
+  LLG_CHECK(e_hot <= E, LL_GPU_ESTATE, "hot/cold split: inconsistent counts");
+  Csr& hot = hc.hot;
+  Csr& cold = hc.cold;
+  hot.V = cold.V = V;
+  hot.E = e_hot;
+  cold.E = E - e_hot;
+  hot.level = cold.level = in.level;
+  hot.adj.alloc(std::max<uint64_t>(hot.E, 1) * 4, s);
+  cold.adj.alloc(std::max<uint64_t>(cold.E, 1) * 4, s);
+  rowtmp.alloc(std::max<uint64_t>(E, 1) * 4, s);
+  k_seg_scatter<<<unsigned(n), kB, 0, s>>>(irb.items.as<RowItem>(), n, in.ofs.as<uint64_t>(),
+                                           slot_adj, hb, S, nbits, base.as<uint64_t>(), e_hot,
+                                           hot.adj.as<uint32_t>(), cold.adj.as<uint32_t>(),
+                                           rowtmp.as<uint32_t>());
+  LLG_HIP(hipGetLastError());
+  hot.ofs.alloc((V + 1) * 8, s);
+  cold.ofs.alloc((V + 1) * 8, s);
+  k_hot_ofs<<<grid_for(e_hot + 1, 256), 256, 0, s>>>(rowtmp.as<uint32_t>(), e_hot, V,
+                                                     hot.ofs.as<uint64_t>());
+  k_cold_ofs<<<grid_for(V + 1, 256), 256, 0, s>>>(in.ofs.as<uint64_t>(), hot.ofs.as<uint64_t>(), V,
+                                                  cold.ofs.as<uint64_t>());
+  LLG_HIP(hipGetLastError());
+  hot.valid = cold.valid = true;
+  hc.valid = true;
 }
 
 const RowBlocks& pr_segment_rowblocks(ll_gpu_ctx* ctx, PrSegments& ps, uint32_t cap) {

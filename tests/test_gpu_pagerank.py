@@ -237,3 +237,49 @@ def test_pagerank_item_sorted_hub(ctx):
     finally:
         ctx.set_option("pr_item_sort", -1)
     assert l1rel(r, ref) <= PR_TOL
+
+
+@pytest.mark.parametrize("cap", [512, 4096])
+@pytest.mark.parametrize("hot", [6, 9])
+def test_pagerank_hot_cold_split(ctx, ml, cap, hot):
+    """Option pr_hot_bits: a hot pass over in-edges from slots < 2^hot writes
+    per-vertex partials, the cold pass continues from them and finalises."""
+    g, o = ml
+    gpu_case(f"pagerank hot/cold hot_bits={hot} cap={cap}")
+    dg = DeviceGraph(ctx, g)
+    try:
+        ctx.set_option("pr_hot_bits", hot)
+        for L in (0, g.num_levels - 1):
+            ref, ref_diff = o.pagerank(20, level=L)
+            r, diff = dg.pagerank(20, level=L, cap=cap)
+            assert l1rel(r, ref) <= PR_TOL
+            assert abs(diff - ref_diff) <= 1e-3 * ref_diff + 1e-9
+        a, _ = dg.pagerank(10, cap=cap)
+        dg.drop_snapshots()
+        b, _ = dg.pagerank(10, cap=cap)
+        assert np.array_equal(a, b)
+    finally:
+        ctx.set_option("pr_hot_bits", -1)
+
+
+def test_pagerank_hot_cold_hub_and_empty(ctx):
+    gpu_case("pagerank hot/cold hub 20000 + empty")
+    k = 20000
+    n = k + 3
+    rng = np.random.default_rng(3)
+    src = np.concatenate([rng.permutation(np.arange(1, k + 1)), [0, n - 1]])
+    dst = np.concatenate([np.zeros(k, int), [n - 2, 0]])
+    g = HostGraph(n)
+    g.add_level(src, dst)
+    ref, _ = oracle.OracleGraph.from_host(g).pagerank(5)
+    e = HostGraph(5)
+    e.add_level([], [])
+    try:
+        for hot in (4, 14, 20):  # hub split across both passes; all hot
+            ctx.set_option("pr_hot_bits", hot)
+            r, _ = DeviceGraph(ctx, g).pagerank(5, cap=1024)
+            assert l1rel(r, ref) <= PR_TOL, hot
+        r0, _ = DeviceGraph(ctx, e).pagerank(1)
+        assert np.allclose(r0, 0.15 / 5, rtol=1e-6)
+    finally:
+        ctx.set_option("pr_hot_bits", -1)
