@@ -224,24 +224,38 @@ __global__ __launch_bounds__(kBlock) void k_bu_rows(const RowItem* items, const 
         }
       }
     } else {
+      // entries of visited rows are neither loaded nor tested: rowof maps an
+      // entry to its row in the item (written for unvisited rows only)
+      __shared__ uint8_t rowof[kMaxEdges];
+      __shared__ uint8_t openf[kRowsPerItem];
+      if (threadIdx.x < nrows) {
+        openf[threadIdx.x] = unv ? 1 : 0;
+        if (unv)
+          for (uint32_t j = lo; j < hi; j++) rowof[j] = uint8_t(threadIdx.x);
+      }
+      __syncthreads();
       // all neighbour IDs first, then all bitmap words, so loads overlap
       constexpr uint32_t kPer = kMaxEdges / kBlock;
       uint32_t u[kPer];
       uint64_t fw[kPer];
+      bool need[kPer];
 #pragma unroll
       for (uint32_t k = 0; k < kPer; k++) {
         const uint32_t i = k * kBlock + threadIdx.x;
-        u[k] = i < ne ? adj[e0 + i] : 0u;
+        need[k] = false;
+        if (i < ne) {
+          const uint32_t rr = rowof[i];
+          need[k] = rr < nrows && openf[rr] && i >= uint32_t(rofs[rr] - e0) &&
+                    i < uint32_t(rofs[rr + 1] - e0);
+        }
+        u[k] = need[k] ? adj[e0 + i] : 0u;
       }
 #pragma unroll
-      for (uint32_t k = 0; k < kPer; k++) {
-        const uint32_t i = k * kBlock + threadIdx.x;
-        fw[k] = i < ne ? fbm[u[k] >> 6] : 0ull;
-      }
+      for (uint32_t k = 0; k < kPer; k++) fw[k] = need[k] ? fbm[u[k] >> 6] : 0ull;
 #pragma unroll
       for (uint32_t k = 0; k < kPer; k++) {
         const uint32_t i = k * kBlock + threadIdx.x;
-        const uint64_t m = __ballot(i < ne && ((fw[k] >> (u[k] & 63)) & 1));
+        const uint64_t m = __ballot(need[k] && ((fw[k] >> (u[k] & 63)) & 1));
         if (lane == 0) bits[k * (kBlock / 64) + wave] = m;
       }
       __syncthreads();
@@ -323,31 +337,51 @@ __global__ void k_visited_or(uint64_t* visited, const uint64_t* nbm, uint64_t wo
   if (i < words) visited[i] |= nbm[i];
 }
 
-__global__ void k_bitmap_to_queue(const uint64_t* bm, uint64_t V, const uint32_t* outdeg,
-                                  uint32_t* q, Counters* c) {
-  // one atomic per workgroup: waves take consecutive slices of its range
-  __shared__ unsigned wcnt[4];
+// Frontier bitmap -> queue of its vertices with out-edges.  One thread per
+// 64-bit bitmap word, a workgroup scan of the per-word counts and one queue
+// allocation (atomic) per workgroup of 256 words.
+__global__ __launch_bounds__(kBlock) void k_bitmap_to_queue(const uint64_t* bm, uint64_t V,
+                                                            const uint32_t* outdeg, uint32_t* q,
+                                                            Counters* c) {
+  __shared__ uint32_t wsum[kBlock / 64];
   __shared__ unsigned long long base;
-  const uint64_t v = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  const bool in = v < V && ((bm[v >> 6] >> (v & 63)) & 1) && outdeg[v] > 0;
-  const uint64_t mq = __ballot(in);
+  const uint64_t words = (V + 63) / 64;
+  const uint64_t w = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
+  uint64_t bits = w < words ? bm[w] : 0ull;
+  if (w == words - 1 && (V & 63)) bits &= (uint64_t(1) << (V & 63)) - 1;  // padding bits
+  uint64_t keep = 0;  // frontier vertices of this word with out-edges
+  for (uint64_t b = bits; b; b &= b - 1) {
+    const int i = __ffsll((long long)b) - 1;
+    if (outdeg[w * 64 + i] > 0) keep |= uint64_t(1) << i;
+  }
+  const uint32_t cnt = __popcll(keep);
+  // inclusive scan over the wave, then wave offsets through LDS
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  if (lane == 0) wcnt[wave] = __popcll(mq);
+  uint32_t x = cnt;
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) wsum[wave] = x;
   __syncthreads();
   if (threadIdx.x == 0) {
-    unsigned t = 0;
-    for (int i = 0; i < 4; i++) t += wcnt[i];
-    base = t ? atomicAdd(&c->n_next, (unsigned long long)t) : 0;
+    uint32_t t = 0;
+    for (int i = 0; i < kBlock / 64; i++) t += wsum[i];
+    base = t ? atomicAdd(&c->n_next, (unsigned long long)t) : 0ull;
   }
   __syncthreads();
-  unsigned long long off = base;
-  for (int i = 0; i < wave; i++) off += wcnt[i];
-  if (in) q[off + __popcll(mq & ((uint64_t(1) << lane) - 1))] = uint32_t(v);
+  unsigned long long off = base + x - cnt;
+  for (int i = 0; i < wave; i++) off += wsum[i];
+  for (uint64_t b = keep; b; b &= b - 1) q[off++] = uint32_t(w * 64 + (__ffsll((long long)b) - 1));
 }
 
+// Reached vertices (grid-stride over a fixed grid: one atomic per workgroup).
 __global__ void k_count_reached(const int32_t* lv, uint64_t V, unsigned long long* out) {
-  const uint64_t v = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  block_atomic_add(out, (v < V && lv[v] != kUnreached) ? 1ull : 0ull);
+  unsigned long long n = 0;
+  for (uint64_t v = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; v < V;
+       v += uint64_t(gridDim.x) * blockDim.x)
+    n += lv[v] != kUnreached ? 1ull : 0ull;
+  block_atomic_add(out, n);
 }
 
 Counters read_counters(ll_gpu_ctx* ctx, const DevBuf& c) {
@@ -447,7 +481,7 @@ BfsResult bfs_core(ll_gpu_graph* g, const Csr& fwd, const Csr& rev, const RowBlo
       if (shrinking && n_f < V / 24) {
         bottom_up = false;
         k_zero_counters<<<1, kStripes, 0, s>>>(cnt.as<Counters>());
-        k_bitmap_to_queue<<<grid_for(words * 64, 256), 256, 0, s>>>(
+        k_bitmap_to_queue<<<grid_for(words, kBlock), kBlock, 0, s>>>(
             fbm.as<uint64_t>(), V, fdeg, q[cur].as<uint32_t>(), cnt.as<Counters>());
         LLG_HIP(hipGetLastError());
         nq = read_counters(ctx, cnt).n_next;
@@ -479,8 +513,8 @@ BfsResult bfs_device(ll_gpu_graph* g, int level, uint32_t root, DevBuf& lv) {
   DevBuf red;
   red.alloc(8, s);
   k_fill_u64<<<1, 1, 0, s>>>(red.as<uint64_t>(), 1, 0);
-  k_count_reached<<<grid_for(V, 256), 256, 0, s>>>(lv.as<int32_t>(), V,
-                                                   red.as<unsigned long long>());
+  k_count_reached<<<std::min(grid_for(V, 256), 4u * unsigned(g->ctx->num_cus)), 256, 0, s>>>(
+      lv.as<int32_t>(), V, red.as<unsigned long long>());
   LLG_HIP(hipGetLastError());
   uint64_t hr = 0;
   LLG_HIP(hipMemcpyAsync(&hr, red.p, 8, hipMemcpyDeviceToHost, s));

@@ -165,17 +165,35 @@ __global__ __launch_bounds__(kBlock) void k_ms_pull(MsArgs<W> a) {
       block_flag(a.found, cnt);
       return;
     }
+    // entries of rows every lane has reached are neither loaded nor gathered:
+    // rowof maps an entry to its row in the item
+    __shared__ uint8_t rowof[kCap + kCap / 2];
+    __shared__ uint8_t openf[kRowsPerItem];
+    if (threadIdx.x < nrows) {
+      openf[threadIdx.x] = open ? 1 : 0;
+      if (open)
+        for (uint32_t j = lo0; j < hi0; j++) rowof[j] = uint8_t(threadIdx.x);
+    }
+    __syncthreads();
     constexpr uint32_t kPer = (kCap + kCap / 2) / kBlock;
     uint32_t idx[kPer];
+    bool need[kPer];
 #pragma unroll
     for (uint32_t k = 0; k < kPer; k++) {
       uint32_t i = threadIdx.x + k * kBlock;
-      idx[k] = i < ne ? a.adj[e0 + i] : 0u;
+      // rowof is only written for open rows: test the row range, not just the byte
+      need[k] = false;
+      if (i < ne) {
+        const uint32_t rr = rowof[i];
+        need[k] = rr < nrows && openf[rr] && i >= uint32_t(rofs[rr] - e0) &&
+                  i < uint32_t(rofs[rr + 1] - e0);
+      }
+      idx[k] = need[k] ? a.adj[e0 + i] : 0u;
     }
 #pragma unroll
     for (uint32_t k = 0; k < kPer; k++) {
       uint32_t i = threadIdx.x + k * kBlock;
-      if (i < ne) vals[i] = a.fr[idx[k]];
+      if (need[k]) vals[i] = a.fr[idx[k]];
     }
     __syncthreads();
     if (threadIdx.x < nrows) {
@@ -348,8 +366,11 @@ void msbfs_batch(ll_gpu_graph* g, int level, const int64_t* d_roots, int n, int 
 // source for this snapshot shape.
 int autotune_lanes(ll_gpu_graph* g, int level, const int64_t* roots, int n) {
   const Csr& in = ensure_csr(g, LL_GPU_DIR_IN, level);
+  // decisions per request size: the smallest power of two >= min(n, 128)
+  int bucket = 16;
+  while (bucket < std::min(n, 128)) bucket <<= 1;
   const std::string key = "bfs_lanes:" + std::to_string(g->V) + ":" + std::to_string(in.E) + ":" +
-                          std::to_string(resolve_level(g, level));
+                          std::to_string(resolve_level(g, level)) + ":" + std::to_string(bucket);
   auto it = g->ctx->tuned.find(key);
   if (it != g->ctx->tuned.end()) return it->second;
   hipStream_t s = g->ctx->stream;
@@ -376,9 +397,10 @@ int autotune_lanes(ll_gpu_graph* g, int level, const int64_t* roots, int n) {
     LLG_HIP(hipEventSynchronize(e1));
     float ms = 0;
     LLG_HIP(hipEventElapsedTime(&ms, e0, e1));
-    // a full batch of w sources costs about the same as k of them
-    const double per_src = double(ms) / double(w);
-    if (per_src < best_t) { best_t = per_src; best = w; }
+    // a sweep costs about the same for k or w lanes: the request of n
+    // sources takes ceil(n / w) batches of this width
+    const double est = double(ms) * double((n + w - 1) / w);
+    if (est < best_t) { best_t = est; best = w; }
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);

@@ -392,10 +392,14 @@ int ll_gpu_ppr_batch(ll_gpu_graph* g, int level, const int64_t* seeds, int n, in
     if (iters_done) *iters_done = 0;
     if (n == 0) return LL_GPU_OK;
     if (batch_width == 0) {
-      // autotune: one batch per width on the leading seeds, best time per seed
+      // autotune: one batch per width on the leading seeds; keep the width
+      // with the least estimated time for the whole request
       const Csr& in = ensure_csr(g, LL_GPU_DIR_IN, level);
+      int bucket = 16;  // decisions per request size: smallest power of two >= min(n, 128)
+      while (bucket < std::min(n, 128)) bucket <<= 1;
       const std::string key = "ppr:" + std::to_string(g->V) + ":" + std::to_string(in.E) +
-                              ":" + std::to_string(resolve_level(g, level));
+                              ":" + std::to_string(resolve_level(g, level)) + ":" +
+                              std::to_string(bucket);
       auto it = g->ctx->tuned.find(key);
       if (it == g->ctx->tuned.end()) {
         int best = 64;
@@ -404,8 +408,9 @@ int ll_gpu_ppr_batch(ll_gpu_graph* g, int level, const int64_t* seeds, int n, in
         for (int w : {16, 32, 64, 128}) {
           const double ms = ppr_dispatch(w, g, level, seeds, std::min(n, w), probe_iters,
                                          damping, 0.0, nullptr, nullptr, 1);
-          const double per_seed = ms / double(w);
-          if (per_seed < best_t) { best_t = per_seed; best = w; }
+          // the request of n seeds takes ceil(n / w) batches of this width
+          const double est = ms * double((n + w - 1) / w);
+          if (est < best_t) { best_t = est; best = w; }
         }
         g->ctx->tuned[key] = best;
         batch_width = best;
