@@ -77,6 +77,18 @@ def bench_bfs(dg, g, E: int, V: int, args) -> dict:
     }
 
 
+def tuned_key(kind: str, dg, E: int | None, n: int) -> str:
+    """Key of the autotuner decision for a request of n sources/seeds on the
+    latest snapshot: kind:V:E:level:bucket, bucket = smallest power of two
+    >= min(n, 128) (at least 16)."""
+    if E is None:
+        E = dg.snapshot_info().live_edges
+    b = 16
+    while b < min(n, 128):
+        b <<= 1
+    return f"{kind}:{dg.V}:{E}:{dg.num_levels - 1}:{b}"
+
+
 def bench_bfs_batch(dg, g, V: int, args, single: dict | None) -> dict:
     """Multi-source BFS over the batch sources in 64-bit lanes (and the
     autotuned width), against the per-source cost of single-source runs."""
@@ -85,7 +97,7 @@ def bench_bfs_batch(dg, g, V: int, args, single: dict | None) -> dict:
     ctx = dg.ctx
     out = {"sources": len(roots)}
     for lanes in (64, 0):
-        dg.bfs_batch(roots[:64], lane_width=lanes, want_levels=False)  # warm + tune
+        dg.bfs_batch(roots[:128], lane_width=lanes, want_levels=False)  # warm + tune
         ctx.sync()
         ctx.profile(True)
         t0 = time.perf_counter()
@@ -94,7 +106,8 @@ def bench_bfs_batch(dg, g, V: int, args, single: dict | None) -> dict:
         t = time.perf_counter() - t0
         n_k, ms_k = ctx.kernel_time("k_ms_pull")
         ctx.profile(False)
-        key = "lanes64" if lanes == 64 else f"autotuned_lanes{dg.tuned('bfs_lanes:')}"
+        key = "lanes64" if lanes == 64 else \
+            f"autotuned_lanes{dg.tuned(tuned_key('bfs_lanes', dg, None, len(roots)))}"
         out[key] = {"seconds": round(t, 4), "sweeps": sweeps, "kernel_ms": round(ms_k, 3),
                     "reached_total": int(reached.sum())}
     if single:
@@ -112,11 +125,13 @@ def bench_ppr(dg, g, E: int, V: int, args) -> dict:
     """Batched personalized PageRank, 30 iterations, tol 1e-7, autotuned width."""
     seeds = pick_sources(g, args.ppr_seeds, args.seed + 5)
     ctx = dg.ctx
-    dg.ppr_batch(seeds[:16], iters=2, tol=0.0, batch_width=0, want_ranks=False)  # autotune
-    width = dg.tuned("ppr:")
+    # autotune on a request of the same size bucket (min(n, 128) seeds), then
+    # time the product path (batch_width=0 reuses the decision)
+    dg.ppr_batch(seeds[:128], iters=2, tol=0.0, batch_width=0, want_ranks=False)
+    width = dg.tuned(tuned_key("ppr", dg, E, len(seeds)))
     ctx.sync()
     t0 = time.perf_counter()
-    _, it = dg.ppr_batch(seeds, iters=30, tol=1e-7, batch_width=width, want_ranks=False)
+    _, it = dg.ppr_batch(seeds, iters=30, tol=1e-7, batch_width=0, want_ranks=False)
     ctx.sync()
     t = time.perf_counter() - t0
     b = len(seeds)

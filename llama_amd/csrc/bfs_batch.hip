@@ -37,6 +37,9 @@ template <> struct Lane<uint16_t> {
   __device__ __host__ static uint16_t low(int n) { return n >= 16 ? 0xffff : uint16_t((1u << n) - 1); }
   __device__ static bool test(uint16_t w, int s) { return (w >> s) & 1; }
   __device__ static int popc(uint16_t w) { return __popc(w); }
+  __device__ __host__ static uint64_t lo64(uint16_t w) { return uint64_t(w); }
+  __device__ __host__ static uint64_t hi64(uint16_t) { return 0; }
+  __device__ __host__ static uint16_t from64(uint64_t lo, uint64_t) { return uint16_t(lo); }
 };
 template <> struct Lane<uint32_t> {
   static constexpr int kBits = 32;
@@ -44,6 +47,9 @@ template <> struct Lane<uint32_t> {
   __device__ __host__ static uint32_t low(int n) { return n >= 32 ? ~0u : (1u << n) - 1; }
   __device__ static bool test(uint32_t w, int s) { return (w >> s) & 1; }
   __device__ static int popc(uint32_t w) { return __popc(w); }
+  __device__ __host__ static uint64_t lo64(uint32_t w) { return uint64_t(w); }
+  __device__ __host__ static uint64_t hi64(uint32_t) { return 0; }
+  __device__ __host__ static uint32_t from64(uint64_t lo, uint64_t) { return uint32_t(lo); }
 };
 template <> struct Lane<uint64_t> {
   static constexpr int kBits = 64;
@@ -51,6 +57,9 @@ template <> struct Lane<uint64_t> {
   __device__ __host__ static uint64_t low(int n) { return n >= 64 ? ~0ull : (1ull << n) - 1; }
   __device__ static bool test(uint64_t w, int s) { return (w >> s) & 1; }
   __device__ static int popc(uint64_t w) { return __popcll(w); }
+  __device__ __host__ static uint64_t lo64(uint64_t w) { return w; }
+  __device__ __host__ static uint64_t hi64(uint64_t) { return 0; }
+  __device__ __host__ static uint64_t from64(uint64_t lo, uint64_t) { return lo; }
 };
 template <> struct Lane<W128> {
   static constexpr int kBits = 128;
@@ -62,6 +71,9 @@ template <> struct Lane<W128> {
   }
   __device__ static bool test(W128 w, int s) { return s < 64 ? (w.a >> s) & 1 : (w.b >> (s - 64)) & 1; }
   __device__ static int popc(W128 w) { return __popcll(w.a) + __popcll(w.b); }
+  __device__ __host__ static uint64_t lo64(W128 w) { return w.a; }
+  __device__ __host__ static uint64_t hi64(W128 w) { return w.b; }
+  __device__ __host__ static W128 from64(uint64_t lo, uint64_t hi) { return {lo, hi}; }
 };
 
 template <typename W>
@@ -78,21 +90,22 @@ struct MsArgs {
   int32_t next_lv;
   int src_base, n_src;    // lanes [0, n_src) map to sources src_base + lane
   W full;
-  unsigned long long* found;
+  W done;                 // lanes whose search has ended (empty frontier): count as reached
+  unsigned long long* found;  // striped OR of the lanes that discovered vertices
 };
 
 template <typename W>
-__device__ inline uint32_t finalize(const MsArgs<W>& a, uint32_t r, W acc) {
+__device__ inline W finalize(const MsArgs<W>& a, uint32_t r, W acc) {
   const W vr = a.vis[r];
   const W nw = acc & ~vr;
   a.nx[r] = nw;
-  if (nw == Lane<W>::zero()) return 0;
+  if (nw == Lane<W>::zero()) return nw;
   a.vis[r] = vr | nw;
   if (a.lv) {
     for (int s = 0; s < a.n_src; s++)
       if (Lane<W>::test(nw, s)) a.lv[uint64_t(a.src_base + s) * a.V + r] = a.next_lv;
   }
-  return uint32_t(Lane<W>::popc(nw));
+  return nw;
 }
 
 // Sum over the workgroup, returned to every thread.
@@ -107,20 +120,36 @@ __device__ inline uint32_t block_sum_u32(uint32_t x) {
   return t;
 }
 
-// "Something was discovered" flag: a workgroup that discovered vertices
-// stores 1 into its stripe (blockIdx % kFlagStripes, one 128-byte line each),
-// so the stores of hundreds of thousands of workgroups spread over many L2
-// lines; the host ORs the stripes.
+// Lanes that discovered vertices: a workgroup ORs its threads' words and
+// atomically ORs them into its stripe (blockIdx % kFlagStripes, one 128-byte
+// line each: words 0 and 1 = lanes 0-63 and 64-127), so hundreds of thousands
+// of workgroups spread over many L2 lines; the host ORs the stripes.
 constexpr int kFlagStripes = 64;
 constexpr int kFlagStride = 16;  // u64 per 128-byte line
 
-__device__ inline void block_flag(unsigned long long* flags, uint32_t x) {
-  if (__syncthreads_or(x != 0) && threadIdx.x == 0)
-    flags[(blockIdx.x % kFlagStripes) * kFlagStride] = 1ull;
+template <typename W>
+__device__ inline void block_or(unsigned long long* flags, W x) {
+  __shared__ unsigned long long part[2][kBlock / 64];
+  unsigned long long lo = Lane<W>::lo64(x), hi = Lane<W>::hi64(x);
+  for (int o = 32; o > 0; o >>= 1) {
+    lo |= __shfl_xor(lo, o, 64);
+    hi |= __shfl_xor(hi, o, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    part[0][threadIdx.x >> 6] = lo;
+    part[1][threadIdx.x >> 6] = hi;
+  }
+  __syncthreads();
+  if (threadIdx.x < 2) {
+    unsigned long long t = 0;
+    for (int i = 0; i < kBlock / 64; i++) t |= part[threadIdx.x][i];
+    if (t) atomicOr(&flags[(blockIdx.x % kFlagStripes) * kFlagStride + threadIdx.x], t);
+  }
 }
 
 __global__ void k_zero_flags(unsigned long long* flags) {  // <<<1, kFlagStripes>>>
   flags[threadIdx.x * kFlagStride] = 0ull;
+  flags[threadIdx.x * kFlagStride + 1] = 0ull;
 }
 
 template <typename W>
@@ -129,11 +158,12 @@ __global__ __launch_bounds__(kBlock) void k_ms_pull(MsArgs<W> a) {
   __shared__ uint64_t rofs[kRowsPerItem + 1];
   __shared__ W red[kBlock / 64];
   const RowItem it = a.items[blockIdx.x];
-  uint32_t cnt = 0;
+  W cnt = Lane<W>::zero();  // lanes this thread's row was discovered by
   if (it.npieces == 0) {
     const uint32_t nrows = it.row_end - it.row_begin;
-    // rows already reached from every lane need no gathers at all
-    const bool open = threadIdx.x < nrows && !(a.vis[it.row_begin + threadIdx.x] == a.full);
+    // rows already reached from every lane still searching need no gathers
+    const bool open =
+        threadIdx.x < nrows && !((a.vis[it.row_begin + threadIdx.x] | a.done) == a.full);
     if (!__syncthreads_or(open)) {
       if (threadIdx.x < nrows) a.nx[it.row_begin + threadIdx.x] = Lane<W>::zero();
       return;
@@ -155,14 +185,14 @@ __global__ __launch_bounds__(kBlock) void k_ms_pull(MsArgs<W> a) {
           W acc = Lane<W>::zero();
           for (uint32_t j = lo0; j < hi0; j++) {
             acc = acc | a.fr[a.adj[e0 + j]];
-            if ((acc | vr) == a.full) break;
+            if ((acc | vr | a.done) == a.full) break;
           }
           cnt = finalize(a, r, acc);
         } else {
           a.nx[r] = Lane<W>::zero();
         }
       }
-      block_flag(a.found, cnt);
+      block_or(a.found, cnt);
       return;
     }
     // entries of rows every lane has reached are neither loaded nor gathered:
@@ -198,7 +228,7 @@ __global__ __launch_bounds__(kBlock) void k_ms_pull(MsArgs<W> a) {
     __syncthreads();
     if (threadIdx.x < nrows) {
       const uint32_t r = it.row_begin + threadIdx.x;
-      if (!(a.vis[r] == a.full)) {
+      if (!((a.vis[r] | a.done) == a.full)) {
         W acc = Lane<W>::zero();
         const uint32_t lo = uint32_t(rofs[threadIdx.x] - e0), hi = uint32_t(rofs[threadIdx.x + 1] - e0);
         for (uint32_t j = lo; j < hi; j++) acc = acc | vals[j];
@@ -212,7 +242,7 @@ __global__ __launch_bounds__(kBlock) void k_ms_pull(MsArgs<W> a) {
     const uint64_t lo = a.ofs[r] + uint64_t(it.piece) * kPiece;
     const uint64_t hi = min(a.ofs[r + 1], lo + kPiece);
     W acc = Lane<W>::zero();
-    if (!(a.vis[r] == a.full))
+    if (!((a.vis[r] | a.done) == a.full))
       for (uint64_t e = lo + threadIdx.x; e < hi; e += kBlock) acc = acc | a.fr[a.adj[e]];
     // OR-reduce the workgroup through LDS
     __shared__ W wred[kBlock];
@@ -228,13 +258,13 @@ __global__ __launch_bounds__(kBlock) void k_ms_pull(MsArgs<W> a) {
     }
     (void)red;
   }
-  block_flag(a.found, cnt);
+  block_or(a.found, cnt);
 }
 
 template <typename W>
 __global__ void k_ms_fixup(MsArgs<W> a, uint64_t nitems) {
   const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  uint32_t cnt = 0;
+  W cnt = Lane<W>::zero();
   if (i < nitems) {
     const RowItem it = a.items[i];
     if (it.npieces > 1 && it.piece == 0) {
@@ -243,7 +273,7 @@ __global__ void k_ms_fixup(MsArgs<W> a, uint64_t nitems) {
       cnt = finalize(a, it.row_begin, acc);
     }
   }
-  block_flag(a.found, cnt);
+  block_or(a.found, cnt);
 }
 
 template <typename W>
@@ -322,6 +352,7 @@ void msbfs_batch(ll_gpu_graph* g, int level, const int64_t* d_roots, int n, int 
   k_ms_roots<W><<<1, 1, 0, s>>>(vis.as<W>(), fr.as<W>(), d_roots + src_base, n, d_lv, V, src_base);
   LLG_HIP(hipGetLastError());
   uint64_t lv_count = 0;
+  W done = full & ~used;  // lanes not searching (unused); later also finished ones
   for (int32_t l = 1;; l++) {
     MsArgs<W> a;
     a.items = rb.items.as<RowItem>();
@@ -337,6 +368,7 @@ void msbfs_batch(ll_gpu_graph* g, int level, const int64_t* d_roots, int n, int 
     a.src_base = src_base;
     a.n_src = n;
     a.full = full;
+    a.done = done;
     a.found = found.as<unsigned long long>();
     k_zero_flags<<<1, kFlagStripes, 0, s>>>(a.found);
     {
@@ -351,9 +383,13 @@ void msbfs_batch(ll_gpu_graph* g, int level, const int64_t* d_roots, int n, int 
                            hipMemcpyDeviceToHost, s));
     LLG_HIP(hipStreamSynchronize(s));
     lv_count++;
-    uint64_t h = 0;
-    for (int i = 0; i < kFlagStripes; i++) h |= fl[i * kFlagStride];
-    if (h == 0) break;
+    uint64_t lo = 0, hi = 0;  // lanes that discovered vertices: the next frontier's lanes
+    for (int i = 0; i < kFlagStripes; i++) {
+      lo |= fl[i * kFlagStride];
+      hi |= fl[i * kFlagStride + 1];
+    }
+    if ((lo | hi) == 0) break;
+    done = full & ~Lane<W>::from64(lo, hi);
     std::swap(fr, nx);
   }
   if (d_reached)
