@@ -195,6 +195,30 @@ def bench_wpr(ctx, args) -> dict:
 PR_ITERATION_KERNELS = ("k_pr_pull", "k_pr_pull_cold", "k_pr_fixup", "k_pr_merge")
 
 
+TRAFFIC_PROFILE = ROOT / "profiles" / "k_pr_pull_traffic_rmat_s24_l11.json"
+
+
+def measured_traffic(args, V: int, E: int, used: list) -> dict | None:
+    """HBM bytes per k_pr_pull dispatch from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE passes (each scaled by what the counter reads for a 1 GiB device
+    copy in the same run), when this run is the workload and PageRank variant
+    they were measured on; else None."""
+    try:
+        prof = json.loads(TRAFFIC_PROFILE.read_text())
+    except (OSError, ValueError):
+        return None
+    w = prof["workload"]
+    plain = (args.reorder == 1 and args.seg_bits in (-1, 0) and args.hot_bits in (-1, 0)
+             and args.item_sort in (-1, 0) and args.cap in (0, 2048))
+    same = (w["num_nodes"] == V and w["live_edges"] == E and w["seed"] == args.seed and plain
+            and used == ["k_pr_pull", "k_pr_fixup"])
+    if not same:
+        return None
+    return {"bytes": int(prof["traffic_bytes_per_dispatch"]),
+            "source": f"profiles/{TRAFFIC_PROFILE.name}: FETCH_SIZE x{prof['fetch_scale']:.2f} + "
+                      f"WRITE_SIZE x{prof['write_scale']:.2f} KiB per k_pr_pull dispatch"}
+
+
 def iteration_ms(ctx, iterations):
     """Device time of one PageRank iteration: the iteration kernels' recorded
     time (HIP events on the library stream) over `iterations`."""
@@ -350,6 +374,8 @@ def main() -> int:
     gteps = E * args.iters / (step_ms * 1e-3) / 1e9
     b_pr = 8 * E + 16 * V  # algorithmic bytes of one pull iteration
     achieved = b_pr / (avg_it_ms * 1e-3) / 1e9
+    used = [k for k, v in it_kernels.items() if v[0] > 0]
+    traffic = measured_traffic(args, V, E, used)
 
     secondary = {}
     if args.bfs_sources > 0:
@@ -401,10 +427,12 @@ def main() -> int:
             "continuations": int(info.continuations), "seed": args.seed,
         },
         "roofline": {
-            "bound": "hbm", "kernel": "+".join(PR_ITERATION_KERNELS),
+            "bound": "hbm", "kernel": "+".join(used),
             "achieved": round(achieved, 2),
             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None, "bytes_per_launch": b_pr, "avg_launch_ms": round(avg_it_ms, 5),
+            "traffic": traffic["bytes"] if traffic else None,
+            "traffic_source": traffic["source"] if traffic else None,
+            "bytes_per_launch": b_pr, "avg_launch_ms": round(avg_it_ms, 5),
             "launches": n_pr,
             "per_kernel_avg_ms": {k: round(v[1] / max(n_pr, 1), 5)
                                   for k, v in it_kernels.items()},

@@ -255,6 +255,8 @@ void sort_pairs_u64_u32(const uint64_t* keys_in, uint64_t* keys_out, const uint3
                         uint32_t* vals_out, uint64_t n, int end_bit, hipStream_t s);
 void sort_keys_u64(const uint64_t* keys_in, uint64_t* keys_out, uint64_t n, int end_bit,
                    hipStream_t s);
+void sort_pairs_u32_u32(const uint32_t* keys_in, uint32_t* keys_out, const uint32_t* vals_in,
+                        uint32_t* vals_out, uint64_t n, int end_bit, hipStream_t s);
 uint64_t unique_u64(const uint64_t* in, uint64_t* out, uint64_t n, hipStream_t s);
 
 // Sum x over the workgroup and add it to *p with a single atomic.  Every

@@ -201,15 +201,35 @@ __global__ void k_pr_init(const uint32_t* outdeg, const uint32_t* inv, float* ra
 }
 
 // Locality order for the contribution vector: slot = rank of the vertex by
-// descending out-degree, so the entries most edges gather share few cache
-// lines (the hottest ones stay in each XCD's L2).
-__global__ void k_deg_keys(const uint32_t* outdeg, uint64_t* keys, uint64_t V) {
-  uint64_t v = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (v < V) keys[v] = (uint64_t(0xffffffffu - outdeg[v]) << 32) | v;
+// descending out-degree (ties by vertex ID), so the entries most edges gather
+// share few cache lines (the hottest ones stay in each XCD's L2).  A stable
+// radix sort of (maxdeg - outdeg) with the vertex IDs as values needs only the
+// bits of maxdeg.
+__global__ void k_max_u32(const uint32_t* x, uint64_t n, uint32_t* out) {
+  uint32_t m = 0;
+  for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += uint64_t(gridDim.x) * blockDim.x)
+    m = max(m, x[i]);
+  for (int o = 32; o > 0; o >>= 1) m = max(m, __shfl_xor(m, o, 64));
+  __shared__ uint32_t part[4];
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int i = 1; i < int(blockDim.x / 64); i++) m = max(m, part[i]);
+    atomicMax(out, m);
+  }
 }
-__global__ void k_inv_from_sorted(const uint64_t* sorted, uint32_t* inv, uint64_t V) {
+__global__ void k_deg_keys(const uint32_t* outdeg, uint32_t maxdeg, uint32_t* keys, uint32_t* ids,
+                           uint64_t V) {
+  uint64_t v = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (v < V) {
+    keys[v] = maxdeg - outdeg[v];
+    ids[v] = uint32_t(v);
+  }
+}
+__global__ void k_inv_from_sorted(const uint32_t* sorted_ids, uint32_t* inv, uint64_t V) {
   uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i < V) inv[uint32_t(sorted[i])] = uint32_t(i);
+  if (i < V) inv[sorted_ids[i]] = uint32_t(i);
 }
 __global__ void k_relabel(const uint32_t* adj, const uint32_t* inv, uint32_t* out, uint64_t E) {
   uint64_t e = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -267,6 +287,7 @@ __global__ void k_widen(const float* in, double* out, uint64_t n) {
 }
 
 __global__ void k_zero_f64(double* p) { *p = 0.0; }
+__global__ void k_fill_u32(uint32_t* p, uint32_t v) { *p = v; }
 
 }  // namespace
 
@@ -309,13 +330,26 @@ void pagerank_device(ll_gpu_graph* g, int level, int iters, double damping, uint
   if (reorder) {
     if (g->pr_level != in.level) {
       TimedLaunch t(ctx, "pr_reorder");
-      DevBuf keys, sorted;
-      keys.alloc(V * 8, s);
-      sorted.alloc(V * 8, s);
-      k_deg_keys<<<grid_for(V, 256), 256, 0, s>>>(outdeg, keys.as<uint64_t>(), V);
-      sort_keys_u64(keys.as<uint64_t>(), sorted.as<uint64_t>(), V, 64, s);
+      DevBuf mx, keys, ids, skeys, sids;
+      mx.alloc(4, s);
+      k_fill_u32<<<1, 1, 0, s>>>(mx.as<uint32_t>(), 0u);
+      k_max_u32<<<std::min(grid_for(V, 256), 4u * unsigned(ctx->num_cus)), 256, 0, s>>>(
+          outdeg, V, mx.as<uint32_t>());
+      LLG_HIP(hipMemcpyAsync(ctx->readback, mx.p, 4, hipMemcpyDeviceToHost, s));
+      LLG_HIP(hipStreamSynchronize(s));
+      const uint32_t maxdeg = *static_cast<const uint32_t*>(ctx->readback);
+      int bits = 1;
+      while (bits < 32 && (maxdeg >> bits) != 0) bits++;
+      keys.alloc(V * 4, s);
+      ids.alloc(V * 4, s);
+      skeys.alloc(V * 4, s);
+      sids.alloc(V * 4, s);
+      k_deg_keys<<<grid_for(V, 256), 256, 0, s>>>(outdeg, maxdeg, keys.as<uint32_t>(),
+                                                  ids.as<uint32_t>(), V);
+      sort_pairs_u32_u32(keys.as<uint32_t>(), skeys.as<uint32_t>(), ids.as<uint32_t>(),
+                         sids.as<uint32_t>(), V, bits, s);
       g->pr_inv.alloc(V * 4, s);
-      k_inv_from_sorted<<<grid_for(V, 256), 256, 0, s>>>(sorted.as<uint64_t>(),
+      k_inv_from_sorted<<<grid_for(V, 256), 256, 0, s>>>(sids.as<uint32_t>(),
                                                          g->pr_inv.as<uint32_t>(), V);
       LLG_HIP(hipGetLastError());
       g->pr_adj.release();

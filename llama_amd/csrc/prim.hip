@@ -59,6 +59,18 @@ void sort_pairs_u64_u32(const uint64_t* keys_in, uint64_t* keys_out, const uint3
                                              0, end_bit, s));
 }
 
+void sort_pairs_u32_u32(const uint32_t* keys_in, uint32_t* keys_out, const uint32_t* vals_in,
+                        uint32_t* vals_out, uint64_t n, int end_bit, hipStream_t s) {
+  if (n == 0) return;
+  size_t tmp = 0;
+  LLG_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, keys_in, keys_out, vals_in, vals_out,
+                                             n, 0, end_bit, s));
+  DevBuf t;
+  t.alloc(tmp, s);
+  LLG_HIP(hipcub::DeviceRadixSort::SortPairs(t.p, tmp, keys_in, keys_out, vals_in, vals_out, n,
+                                             0, end_bit, s));
+}
+
 void sort_keys_u64(const uint64_t* keys_in, uint64_t* keys_out, uint64_t n, int end_bit,
                    hipStream_t s) {
   if (n == 0) return;

@@ -115,18 +115,26 @@ struct BlockSegs {
   uint32_t n;  // segments overlapping the block
 };
 
-// Load the block's segment range into LDS; returns it.
+// First and last segment of every block of kWordsPerBlock virtual words
+// (one thread per block: the binary searches run in parallel instead of
+// serially at the head of each compaction workgroup).
+__global__ void k_block_segs(const uint64_t* seg_vofs, uint64_t nseg, uint64_t total,
+                             uint64_t nblk, uint64_t* blk_seg) {
+  const uint64_t b = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (b >= nblk) return;
+  const uint64_t x0 = b * kWordsPerBlock;
+  const uint64_t x1 = min(total, x0 + kWordsPerBlock);
+  const uint64_t lo = seg_search_global(seg_vofs, 0, nseg, x0);
+  blk_seg[2 * b] = lo;
+  blk_seg[2 * b + 1] = seg_search_global(seg_vofs, lo, nseg, x1 - 1);
+}
+
+// Load the block's segment range (from k_block_segs) into LDS; returns it.
 __device__ inline BlockSegs load_block_segs(const uint64_t* seg_vofs, const uint64_t* seg_addr,
-                                            uint64_t nseg, uint64_t x0, uint64_t x1,
-                                            uint64_t* l_vofs, uint64_t* l_addr) {
-  __shared__ uint64_t sh_lo, sh_hi;
-  if (threadIdx.x == 0) {
-    sh_lo = seg_search_global(seg_vofs, 0, nseg, x0);
-    sh_hi = seg_search_global(seg_vofs, 0, nseg, x1 - 1);
-  }
-  __syncthreads();
-  uint64_t lo = sh_lo;
-  uint32_t n = uint32_t(sh_hi - lo + 1);
+                                            const uint64_t* blk_seg, uint64_t* l_vofs,
+                                            uint64_t* l_addr) {
+  const uint64_t lo = blk_seg[2 * blockIdx.x];
+  const uint32_t n = uint32_t(blk_seg[2 * blockIdx.x + 1] - lo + 1);
   for (uint32_t i = threadIdx.x; i <= n; i += blockDim.x) {
     l_vofs[i] = seg_vofs[lo + i];
     if (i < n) l_addr[i] = seg_addr[lo + i];
@@ -153,7 +161,8 @@ __device__ inline uint64_t load_word(const uint64_t* const* et, const uint64_t* 
 
 __global__ __launch_bounds__(kBlock) void k_live_count(const uint64_t* const* et,
                                                        const uint64_t* seg_vofs,
-                                                       const uint64_t* seg_addr, uint64_t nseg,
+                                                       const uint64_t* seg_addr,
+                                                       const uint64_t* blk_seg,
                                                        uint64_t total, uint32_t level,
                                                        uint32_t* blk_cnt) {
   __shared__ uint64_t l_vofs[kWordsPerBlock + 1];
@@ -161,7 +170,7 @@ __global__ __launch_bounds__(kBlock) void k_live_count(const uint64_t* const* et
   __shared__ uint32_t wsum[kWaves];
   const uint64_t x0 = uint64_t(blockIdx.x) * kWordsPerBlock;
   const uint64_t x1 = min(total, x0 + kWordsPerBlock);
-  BlockSegs bs = load_block_segs(seg_vofs, seg_addr, nseg, x0, x1, l_vofs, l_addr);
+  BlockSegs bs = load_block_segs(seg_vofs, seg_addr, blk_seg, l_vofs, l_addr);
   uint32_t cnt = 0;
   for (uint64_t x = x0 + threadIdx.x; x < x1; x += kBlock) {
     uint64_t w = load_word(et, l_vofs, l_addr, bs.n, x);
@@ -180,7 +189,8 @@ __global__ __launch_bounds__(kBlock) void k_live_count(const uint64_t* const* et
 template <bool kDeletions>
 __global__ __launch_bounds__(kBlock) void k_compact(const uint64_t* const* et,
                                                     const uint64_t* seg_vofs,
-                                                    const uint64_t* seg_addr, uint64_t nseg,
+                                                    const uint64_t* seg_addr,
+                                                    const uint64_t* blk_seg,
                                                     uint64_t total, uint32_t level,
                                                     const uint64_t* blk_ofs, uint32_t* adj) {
   __shared__ uint64_t l_vofs[kWordsPerBlock + 1];
@@ -188,7 +198,7 @@ __global__ __launch_bounds__(kBlock) void k_compact(const uint64_t* const* et,
   __shared__ uint32_t wcnt[kWaves];
   const uint64_t x0 = uint64_t(blockIdx.x) * kWordsPerBlock;
   const uint64_t x1 = min(total, x0 + kWordsPerBlock);
-  BlockSegs bs = load_block_segs(seg_vofs, seg_addr, nseg, x0, x1, l_vofs, l_addr);
+  BlockSegs bs = load_block_segs(seg_vofs, seg_addr, blk_seg, l_vofs, l_addr);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   uint64_t out = kDeletions ? blk_ofs[blockIdx.x] : x0;
   // Rounds of 256 consecutive words keep the output in virtual order.
@@ -302,14 +312,18 @@ const Csr& ensure_csr(ll_gpu_graph* g, uint32_t dir, int level) {
   c.adj.alloc(std::max<uint64_t>(E, 1) * 4, s);
   uint64_t live = W;
   if (nblk > 0) {
-    DevBuf blk_cnt, blk_ofs;
+    DevBuf blk_cnt, blk_ofs, blk_seg;
+    blk_seg.alloc(nblk * 16, s);
+    k_block_segs<<<grid_for(nblk, 256), 256, 0, s>>>(seg_vofs.as<uint64_t>(), S, W, nblk,
+                                                     blk_seg.as<uint64_t>());
     if (dels) {
       blk_cnt.alloc(nblk * 4, s);
       blk_ofs.alloc((nblk + 1) * 8, s);
       {
         TimedLaunch t(ctx, "k_live_count");
         k_live_count<<<unsigned(nblk), kBlock, 0, s>>>(et, seg_vofs.as<uint64_t>(),
-                                                        seg_addr.as<uint64_t>(), S, W,
+                                                        seg_addr.as<uint64_t>(),
+                                                        blk_seg.as<uint64_t>(), W,
                                                         uint32_t(level), blk_cnt.as<uint32_t>());
       }
       LLG_HIP(hipGetLastError());
@@ -324,12 +338,14 @@ const Csr& ensure_csr(ll_gpu_graph* g, uint32_t dir, int level) {
       TimedLaunch t(ctx, "k_compact");
       if (dels)
         k_compact<true><<<unsigned(nblk), kBlock, 0, s>>>(et, seg_vofs.as<uint64_t>(),
-                                                           seg_addr.as<uint64_t>(), S, W,
+                                                           seg_addr.as<uint64_t>(),
+                                                           blk_seg.as<uint64_t>(), W,
                                                            uint32_t(level), blk_ofs.as<uint64_t>(),
                                                            c.adj.as<uint32_t>());
       else
         k_compact<false><<<unsigned(nblk), kBlock, 0, s>>>(et, seg_vofs.as<uint64_t>(),
-                                                            seg_addr.as<uint64_t>(), S, W,
+                                                            seg_addr.as<uint64_t>(),
+                                                            blk_seg.as<uint64_t>(), W,
                                                             uint32_t(level), nullptr,
                                                             c.adj.as<uint32_t>());
     }
