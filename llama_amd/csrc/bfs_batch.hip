@@ -91,16 +91,20 @@ struct MsArgs {
   int src_base, n_src;    // lanes [0, n_src) map to sources src_base + lane
   W full;
   W done;                 // lanes whose search has ended (empty frontier): count as reached
+  const uint32_t* outdeg; // out-degrees (frontier work estimate)
   unsigned long long* found;  // striped OR of the lanes that discovered vertices
 };
 
+// Row r's new lanes: visited |= new, next frontier = new, levels; adds the
+// out-degree of a newly discovered vertex to `work` (next frontier's edges).
 template <typename W>
-__device__ inline W finalize(const MsArgs<W>& a, uint32_t r, W acc) {
+__device__ inline W finalize(const MsArgs<W>& a, uint32_t r, W acc, unsigned long long& work) {
   const W vr = a.vis[r];
   const W nw = acc & ~vr;
   a.nx[r] = nw;
   if (nw == Lane<W>::zero()) return nw;
   a.vis[r] = vr | nw;
+  work += a.outdeg[r];
   if (a.lv) {
     for (int s = 0; s < a.n_src; s++)
       if (Lane<W>::test(nw, s)) a.lv[uint64_t(a.src_base + s) * a.V + r] = a.next_lv;
@@ -120,36 +124,46 @@ __device__ inline uint32_t block_sum_u32(uint32_t x) {
   return t;
 }
 
-// Lanes that discovered vertices: a workgroup ORs its threads' words and
-// atomically ORs them into its stripe (blockIdx % kFlagStripes, one 128-byte
-// line each: words 0 and 1 = lanes 0-63 and 64-127), so hundreds of thousands
-// of workgroups spread over many L2 lines; the host ORs the stripes.
+// Sweep report: the lanes that discovered vertices (OR) and the out-degree
+// sum of the discovered vertices.  A workgroup reduces its threads' values and
+// adds them to its stripe (blockIdx % kFlagStripes, one 128-byte line each:
+// word 0/1 = lanes 0-63/64-127, word 2 = out-degree sum), so hundreds of
+// thousands of workgroups spread over many L2 lines; the host combines the
+// stripes.
 constexpr int kFlagStripes = 64;
 constexpr int kFlagStride = 16;  // u64 per 128-byte line
 
 template <typename W>
-__device__ inline void block_or(unsigned long long* flags, W x) {
-  __shared__ unsigned long long part[2][kBlock / 64];
+__device__ inline void block_report(unsigned long long* flags, W x, unsigned long long work) {
+  __shared__ unsigned long long part[3][kBlock / 64];
   unsigned long long lo = Lane<W>::lo64(x), hi = Lane<W>::hi64(x);
   for (int o = 32; o > 0; o >>= 1) {
     lo |= __shfl_xor(lo, o, 64);
     hi |= __shfl_xor(hi, o, 64);
+    work += __shfl_xor(work, o, 64);
   }
   if ((threadIdx.x & 63) == 0) {
     part[0][threadIdx.x >> 6] = lo;
     part[1][threadIdx.x >> 6] = hi;
+    part[2][threadIdx.x >> 6] = work;
   }
   __syncthreads();
-  if (threadIdx.x < 2) {
+  if (threadIdx.x < 3) {
     unsigned long long t = 0;
-    for (int i = 0; i < kBlock / 64; i++) t |= part[threadIdx.x][i];
-    if (t) atomicOr(&flags[(blockIdx.x % kFlagStripes) * kFlagStride + threadIdx.x], t);
+    for (int i = 0; i < kBlock / 64; i++)
+      t = threadIdx.x < 2 ? (t | part[threadIdx.x][i]) : t + part[2][i];
+    unsigned long long* dst = &flags[(blockIdx.x % kFlagStripes) * kFlagStride + threadIdx.x];
+    if (t) {
+      if (threadIdx.x < 2) atomicOr(dst, t);
+      else atomicAdd(dst, t);
+    }
   }
 }
 
 __global__ void k_zero_flags(unsigned long long* flags) {  // <<<1, kFlagStripes>>>
   flags[threadIdx.x * kFlagStride] = 0ull;
   flags[threadIdx.x * kFlagStride + 1] = 0ull;
+  flags[threadIdx.x * kFlagStride + 2] = 0ull;
 }
 
 template <typename W>
@@ -159,6 +173,7 @@ __global__ __launch_bounds__(kBlock) void k_ms_pull(MsArgs<W> a) {
   __shared__ W red[kBlock / 64];
   const RowItem it = a.items[blockIdx.x];
   W cnt = Lane<W>::zero();  // lanes this thread's row was discovered by
+  unsigned long long work = 0;  // out-edges of the vertices it discovered
   if (it.npieces == 0) {
     const uint32_t nrows = it.row_end - it.row_begin;
     // rows already reached from every lane still searching need no gathers
@@ -187,12 +202,12 @@ __global__ __launch_bounds__(kBlock) void k_ms_pull(MsArgs<W> a) {
             acc = acc | a.fr[a.adj[e0 + j]];
             if ((acc | vr | a.done) == a.full) break;
           }
-          cnt = finalize(a, r, acc);
+          cnt = finalize(a, r, acc, work);
         } else {
           a.nx[r] = Lane<W>::zero();
         }
       }
-      block_or(a.found, cnt);
+      block_report(a.found, cnt, work);
       return;
     }
     // entries of rows every lane has reached are neither loaded nor gathered:
@@ -232,7 +247,7 @@ __global__ __launch_bounds__(kBlock) void k_ms_pull(MsArgs<W> a) {
         W acc = Lane<W>::zero();
         const uint32_t lo = uint32_t(rofs[threadIdx.x] - e0), hi = uint32_t(rofs[threadIdx.x + 1] - e0);
         for (uint32_t j = lo; j < hi; j++) acc = acc | vals[j];
-        cnt = finalize(a, r, acc);
+        cnt = finalize(a, r, acc, work);
       } else {
         a.nx[r] = Lane<W>::zero();
       }
@@ -253,27 +268,28 @@ __global__ __launch_bounds__(kBlock) void k_ms_pull(MsArgs<W> a) {
       __syncthreads();
     }
     if (threadIdx.x == 0) {
-      if (it.npieces == 1) cnt = finalize(a, r, wred[0]);
+      if (it.npieces == 1) cnt = finalize(a, r, wred[0], work);
       else a.part[blockIdx.x] = wred[0];
     }
     (void)red;
   }
-  block_or(a.found, cnt);
+  block_report(a.found, cnt, work);
 }
 
 template <typename W>
 __global__ void k_ms_fixup(MsArgs<W> a, uint64_t nitems) {
   const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   W cnt = Lane<W>::zero();
+  unsigned long long work = 0;
   if (i < nitems) {
     const RowItem it = a.items[i];
     if (it.npieces > 1 && it.piece == 0) {
       W acc = a.part[i];
       for (uint32_t p = 1; p < it.npieces; p++) acc = acc | a.part[i + p];
-      cnt = finalize(a, it.row_begin, acc);
+      cnt = finalize(a, it.row_begin, acc, work);
     }
   }
-  block_or(a.found, cnt);
+  block_report(a.found, cnt, work);
 }
 
 template <typename W>
@@ -305,22 +321,167 @@ __global__ void k_ms_roots(W* vis, W* fr, const int64_t* roots, int n, int32_t* 
 
 __global__ void k_zero_ull(unsigned long long* p) { *p = 0; }
 
+// Out-edges of the roots (first sweep's work), into stripe 0's word 2.
+__global__ void k_roots_work(const int64_t* roots, int n, const uint32_t* outdeg,
+                             unsigned long long* flags) {
+  unsigned long long w = 0;
+  for (int s = 0; s < n; s++) w += outdeg[roots[s]];
+  flags[2] = w;
+}
+
+// ---- top-down (push) sweeps for small frontiers ---------------------------
+// Frontier queue: vertices with frontier lanes and out-edges.  Fixed grid;
+// each workgroup owns a contiguous vertex range, counts it, takes one queue
+// allocation and writes its vertices in order.
+template <typename W>
+__global__ __launch_bounds__(kBlock) void k_ms_queue(const W* fr, const uint32_t* outdeg,
+                                                     uint64_t V, uint32_t* q,
+                                                     unsigned long long* qn) {
+  __shared__ unsigned long long base;
+  __shared__ uint32_t wcnt[kBlock / 64];
+  const uint64_t per = (V + gridDim.x - 1) / gridDim.x;
+  const uint64_t v0 = uint64_t(blockIdx.x) * per, v1 = min(V, v0 + per);
+  uint32_t c = 0;
+  for (uint64_t v = v0 + threadIdx.x; v < v1; v += kBlock)
+    c += (!(fr[v] == Lane<W>::zero()) && outdeg[v] > 0) ? 1u : 0u;
+  const uint32_t tot = block_sum_u32(c);
+  if (threadIdx.x == 0) base = tot ? atomicAdd(qn, (unsigned long long)tot) : 0ull;
+  __syncthreads();
+  unsigned long long off = base;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (uint64_t r0 = v0; r0 < v1; r0 += kBlock) {
+    const uint64_t v = r0 + threadIdx.x;
+    const bool in = v < v1 && !(fr[v] == Lane<W>::zero()) && outdeg[v] > 0;
+    const uint64_t m = __ballot(in);
+    if (lane == 0) wcnt[wave] = __popcll(m);
+    __syncthreads();
+    uint32_t before = 0, round = 0;
+    for (int i = 0; i < kBlock / 64; i++) {
+      if (i < wave) before += wcnt[i];
+      round += wcnt[i];
+    }
+    if (in) q[off + before + __popcll(m & ((uint64_t(1) << lane) - 1))] = uint32_t(v);
+    off += round;
+    __syncthreads();
+  }
+}
+
+__global__ void k_queue_deg(const uint32_t* q, uint64_t n, const uint32_t* outdeg, uint32_t* d) {
+  const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) d[i] = outdeg[q[i]];
+}
+
+constexpr uint32_t kPushEdges = 1024;  // queue out-edges per push workgroup
+
+// First and last queue entry of each push workgroup (parallel binary searches).
+__global__ void k_push_blocks(const uint64_t* qofs, uint64_t nq, uint64_t m_f, uint64_t nblk,
+                              uint64_t* blk) {
+  const uint64_t b = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (b >= nblk) return;
+  const uint64_t x0 = b * kPushEdges, x1 = min(m_f, x0 + kPushEdges);
+  auto search = [&](uint64_t lo, uint64_t x) {  // largest i with qofs[i] <= x
+    uint64_t hi = nq;
+    while (hi - lo > 1) {
+      const uint64_t mid = (lo + hi) / 2;
+      if (qofs[mid] <= x) lo = mid;
+      else hi = mid;
+    }
+    return lo;
+  };
+  const uint64_t lo = search(0, x0);
+  blk[2 * b] = lo;
+  blk[2 * b + 1] = search(lo, x1 - 1);
+}
+
+__device__ inline void lanes_or(uint16_t* p, uint16_t m) {
+  // 16-bit lanes: OR into the containing aligned 32-bit word
+  const uintptr_t a = reinterpret_cast<uintptr_t>(p);
+  atomicOr(reinterpret_cast<unsigned int*>(a & ~uintptr_t(3)), uint32_t(m) << ((a & 2) ? 16 : 0));
+}
+__device__ inline void lanes_or(uint32_t* p, uint32_t m) { atomicOr(p, m); }
+__device__ inline void lanes_or(uint64_t* p, uint64_t m) {
+  atomicOr(reinterpret_cast<unsigned long long*>(p), (unsigned long long)m);
+}
+__device__ inline void lanes_or(W128* p, W128 m) {
+  if (m.a) atomicOr(reinterpret_cast<unsigned long long*>(&p->a), (unsigned long long)m.a);
+  if (m.b) atomicOr(reinterpret_cast<unsigned long long*>(&p->b), (unsigned long long)m.b);
+}
+
+// Edge-parallel push over the queue's out-edges: nx[v] |= fr[u] & ~vis[v].
+template <typename W>
+__global__ __launch_bounds__(kBlock) void k_ms_push(const uint32_t* q, const uint64_t* qofs,
+                                                    const uint64_t* blk, uint64_t m_f,
+                                                    const uint64_t* ofs, const uint32_t* adj,
+                                                    const W* fr, const W* vis, W* nx) {
+  __shared__ uint64_t l_qofs[kPushEdges + 1];
+  __shared__ uint64_t l_base[kPushEdges];
+  __shared__ W l_fr[kPushEdges];
+  const uint64_t x0 = uint64_t(blockIdx.x) * kPushEdges;
+  const uint64_t x1 = min(m_f, x0 + kPushEdges);
+  const uint64_t lo = blk[2 * blockIdx.x];
+  const uint32_t n = uint32_t(blk[2 * blockIdx.x + 1] - lo + 1);
+  for (uint32_t i = threadIdx.x; i <= n; i += kBlock) {
+    l_qofs[i] = qofs[lo + i];
+    if (i < n) {
+      const uint32_t u = q[lo + i];
+      l_base[i] = ofs[u];
+      l_fr[i] = fr[u];
+    }
+  }
+  __syncthreads();
+  for (uint64_t x = x0 + threadIdx.x; x < x1; x += kBlock) {
+    uint32_t a = 0, b = n;
+    while (b - a > 1) {
+      const uint32_t mid = (a + b) >> 1;
+      if (l_qofs[mid] <= x) a = mid;
+      else b = mid;
+    }
+    const uint32_t v = adj[l_base[a] + (x - l_qofs[a])];
+    const W m = l_fr[a] & ~vis[v];
+    if (!(m == Lane<W>::zero())) lanes_or(&nx[v], m);
+  }
+}
+
+// After a push: every vertex settles its pushed lanes like a pull row would.
+template <typename W>
+__global__ __launch_bounds__(kBlock) void k_ms_settle(MsArgs<W> a) {
+  const uint64_t r = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
+  W cnt = Lane<W>::zero();
+  unsigned long long work = 0;
+  if (r < a.V) {
+    const W acc = a.nx[r];
+    if (!(acc == Lane<W>::zero())) cnt = finalize(a, uint32_t(r), acc, work);
+  }
+  block_report(a.found, cnt, work);
+}
+
+template <typename W>
+__global__ void k_zero_lanes(W* p, uint64_t n) {
+  const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = Lane<W>::zero();
+}
+
 // Per-lane visited counts: lane s of each wave accumulates popcount(ballot(bit s)),
 // the workgroup sums its waves in LDS and adds once per lane.
+// Grid-stride over a fixed grid, so the per-lane totals take few atomics.
 template <typename W>
 __global__ void k_ms_reached(const W* vis, uint64_t V, int n_src, unsigned long long* out) {
   __shared__ unsigned long long acc[128];
-  const uint64_t v = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  const W w = v < V ? vis[v] : Lane<W>::zero();
   const int lane = threadIdx.x & 63;
   for (int i = threadIdx.x; i < 128; i += blockDim.x) acc[i] = 0;
   __syncthreads();
   unsigned long long c0 = 0, c1 = 0;
-  for (int s = 0; s < n_src; s++) {
-    const uint64_t b = __ballot(v < V && Lane<W>::test(w, s));
-    if ((s & 63) == lane) {
-      if (s < 64) c0 += __popcll(b);
-      else c1 += __popcll(b);
+  const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+  const uint64_t nround = (V + stride - 1) / stride;  // uniform across the grid
+  for (uint64_t k = 0; k < nround; k++) {
+    const uint64_t v = k * stride + uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const W w = v < V ? vis[v] : Lane<W>::zero();
+    for (int s = 0; s < n_src; s++) {
+      const uint64_t b = __ballot(v < V && Lane<W>::test(w, s));
+      if ((s & 63) == lane) {
+        if (s < 64) c0 += __popcll(b);
+        else c1 += __popcll(b);
+      }
     }
   }
   if (lane < n_src && c0) atomicAdd(&acc[lane], c0);
@@ -340,17 +501,40 @@ void msbfs_batch(ll_gpu_graph* g, int level, const int64_t* d_roots, int n, int 
   const uint64_t V = g->V;
   const Csr& in = ensure_csr(g, LL_GPU_DIR_IN, level);
   const RowBlocks& rb = ensure_rowblocks(g, LL_GPU_DIR_IN, level, kCap, kPiece);
+  const uint32_t* outdeg = ensure_degrees(g, LL_GPU_DIR_OUT, level);
   DevBuf vis, fr, nx, part, found;
   vis.alloc(V * sizeof(W), s);
-  fr.alloc(V * sizeof(W), s);
-  nx.alloc(V * sizeof(W), s);
+  fr.alloc(V * sizeof(W) + 16, s);  // padded: 16-bit lanes are OR-ed through 32-bit words
+  nx.alloc(V * sizeof(W) + 16, s);
   part.alloc(std::max<uint64_t>(rb.n, 1) * sizeof(W), s);
   found.alloc(kFlagStripes * kFlagStride * 8, s);
   const W full = Lane<W>::low(Lane<W>::kBits);
   const W used = Lane<W>::low(n);
   k_ms_init<W><<<grid_for(V, 256), 256, 0, s>>>(vis.as<W>(), fr.as<W>(), V, full & ~used);
+  k_zero_flags<<<1, kFlagStripes, 0, s>>>(found.as<unsigned long long>());
   k_ms_roots<W><<<1, 1, 0, s>>>(vis.as<W>(), fr.as<W>(), d_roots + src_base, n, d_lv, V, src_base);
+  k_roots_work<<<1, 1, 0, s>>>(d_roots + src_base, n, outdeg, found.as<unsigned long long>());
   LLG_HIP(hipGetLastError());
+  static_assert(kFlagStripes * kFlagStride * 8 <= kReadbackBytes, "flags fit the read-back buffer");
+  const uint64_t* fl = static_cast<const uint64_t*>(ctx->readback);
+  auto read_report = [&](uint64_t& lo, uint64_t& hi, uint64_t& work) {
+    LLG_HIP(hipMemcpyAsync(ctx->readback, found.p, kFlagStripes * kFlagStride * 8,
+                           hipMemcpyDeviceToHost, s));
+    LLG_HIP(hipStreamSynchronize(s));
+    lo = hi = work = 0;
+    for (int i = 0; i < kFlagStripes; i++) {
+      lo |= fl[i * kFlagStride];
+      hi |= fl[i * kFlagStride + 1];
+      work += fl[i * kFlagStride + 2];
+    }
+  };
+  uint64_t lo, hi, m_f;
+  read_report(lo, hi, m_f);  // m_f: out-edges of the frontier (the roots)
+  // Push while the frontier's out-edges are under 1/kPushRatio of the edges
+  // (the first sweeps from a handful of roots); pull otherwise.
+  constexpr uint64_t kPushRatio = 16;
+  const Csr* out = nullptr;
+  DevBuf q, qdeg, qofs, qn, blk;
   uint64_t lv_count = 0;
   W done = full & ~used;  // lanes not searching (unused); later also finished ones
   for (int32_t l = 1;; l++) {
@@ -369,31 +553,61 @@ void msbfs_batch(ll_gpu_graph* g, int level, const int64_t* d_roots, int n, int 
     a.n_src = n;
     a.full = full;
     a.done = done;
+    a.outdeg = outdeg;
     a.found = found.as<unsigned long long>();
     k_zero_flags<<<1, kFlagStripes, 0, s>>>(a.found);
-    {
-      TimedLaunch t(ctx, "k_ms_pull");
-      k_ms_pull<W><<<unsigned(rb.n), kBlock, 0, s>>>(a);
+    if (m_f * kPushRatio < in.E) {
+      if (!out) {
+        out = &ensure_csr(g, LL_GPU_DIR_OUT, level);
+        q.alloc(V * 4, s);
+        qdeg.alloc(V * 4, s);
+        qofs.alloc((V + 1) * 8, s);
+        qn.alloc(8, s);
+      }
+      k_zero_ull<<<1, 1, 0, s>>>(qn.as<unsigned long long>());
+      k_ms_queue<W><<<std::min(grid_for(V, kBlock), 4u * unsigned(ctx->num_cus)), kBlock, 0, s>>>(
+          fr.as<W>(), outdeg, V, q.as<uint32_t>(), qn.as<unsigned long long>());
+      LLG_HIP(hipMemcpyAsync(ctx->readback, qn.p, 8, hipMemcpyDeviceToHost, s));
+      LLG_HIP(hipStreamSynchronize(s));
+      const uint64_t nq = *static_cast<const uint64_t*>(ctx->readback);
+      k_zero_lanes<W><<<grid_for(V, 256), 256, 0, s>>>(nx.as<W>(), V);
+      if (nq) {
+        k_queue_deg<<<grid_for(nq, 256), 256, 0, s>>>(q.as<uint32_t>(), nq, outdeg,
+                                                      qdeg.as<uint32_t>());
+        scan_u32_to_u64(qdeg.as<uint32_t>(), qofs.as<uint64_t>(), nq, s);
+        LLG_HIP(hipMemcpyAsync(ctx->readback, qofs.as<uint64_t>() + nq, 8, hipMemcpyDeviceToHost, s));
+        LLG_HIP(hipStreamSynchronize(s));
+        const uint64_t me = *static_cast<const uint64_t*>(ctx->readback);
+        const uint64_t nblk = (me + kPushEdges - 1) / kPushEdges;
+        if (nblk) {
+          blk.alloc(nblk * 16, s);
+          k_push_blocks<<<grid_for(nblk, 256), 256, 0, s>>>(qofs.as<uint64_t>(), nq, me, nblk,
+                                                            blk.as<uint64_t>());
+          TimedLaunch t(ctx, "k_ms_push");
+          k_ms_push<W><<<unsigned(nblk), kBlock, 0, s>>>(
+              q.as<uint32_t>(), qofs.as<uint64_t>(), blk.as<uint64_t>(), me,
+              out->ofs.as<uint64_t>(), out->adj.as<uint32_t>(), fr.as<W>(), vis.as<W>(),
+              nx.as<W>());
+        }
+      }
+      k_ms_settle<W><<<grid_for(V, kBlock), kBlock, 0, s>>>(a);
+    } else {
+      {
+        TimedLaunch t(ctx, "k_ms_pull");
+        k_ms_pull<W><<<unsigned(rb.n), kBlock, 0, s>>>(a);
+      }
+      k_ms_fixup<W><<<grid_for(rb.n, 256), 256, 0, s>>>(a, rb.n);
     }
-    k_ms_fixup<W><<<grid_for(rb.n, 256), 256, 0, s>>>(a, rb.n);
     LLG_HIP(hipGetLastError());
-    static_assert(kFlagStripes * kFlagStride * 8 <= kReadbackBytes, "flags fit the read-back buffer");
-    const uint64_t* fl = static_cast<const uint64_t*>(ctx->readback);
-    LLG_HIP(hipMemcpyAsync(ctx->readback, found.p, kFlagStripes * kFlagStride * 8,
-                           hipMemcpyDeviceToHost, s));
-    LLG_HIP(hipStreamSynchronize(s));
+    read_report(lo, hi, m_f);  // lanes that discovered vertices = the next frontier's lanes
     lv_count++;
-    uint64_t lo = 0, hi = 0;  // lanes that discovered vertices: the next frontier's lanes
-    for (int i = 0; i < kFlagStripes; i++) {
-      lo |= fl[i * kFlagStride];
-      hi |= fl[i * kFlagStride + 1];
-    }
     if ((lo | hi) == 0) break;
     done = full & ~Lane<W>::from64(lo, hi);
     std::swap(fr, nx);
   }
   if (d_reached)
-    k_ms_reached<W><<<grid_for(V, 256), 256, 0, s>>>(vis.as<W>(), V, n, d_reached + src_base);
+    k_ms_reached<W><<<std::min(grid_for(V, 256), 4u * unsigned(ctx->num_cus)), 256, 0, s>>>(
+        vis.as<W>(), V, n, d_reached + src_base);
   LLG_HIP(hipGetLastError());
   if (levels_run) *levels_run += lv_count;
 }
