@@ -122,6 +122,13 @@ int ll_gpu_snapshot_degrees(ll_gpu_graph* g, int level, uint32_t dir, uint32_t* 
  * the final iteration (the reference's unused `diff`). */
 int ll_gpu_pagerank(ll_gpu_graph* g, int level, int iters, double damping, double* ranks_out,
                     double* last_diff);
+/* Replaces: task `pagerank_double_push` / `pagerank_push` =
+ * ll_b_pagerank_push_ext::run (benchmark/benchmarks/pagerank.h:217-306): every
+ * vertex with out-edges adds r[t]/outdeg(t) to next[w] of its out-neighbours
+ * (the reference's ATOMIC_ADD), r' = (1-d)/N + d*next.  FP64 ranks and FP64
+ * atomic adds; ranks_out[max_nodes] and last_diff (may be NULL) as above. */
+int ll_gpu_pagerank_push(ll_gpu_graph* g, int level, int iters, double damping,
+                         double* ranks_out, double* last_diff);
 /* Replaces: task `sssp_unweighted` (benchmark/benchmarks/sssp.h:220-333) and
  * `bfs` (bfs.h:85-128): levels_out[max_nodes] (may be NULL) gets the BFS level,
  * INT_MAX-1 for unreached; reached = number of reached vertices (bfs_count);
@@ -169,8 +176,9 @@ int ll_gpu_wpr_adjoint(ll_gpu_graph* g, int level, const float* edge_w, const fl
                        float* dl_dteleport_out);
 
 /* --- extended forms, measurement and tuning ------------------------------------ */
-/* ll_gpu_pagerank with an explicit row-block capacity (0 = default 1024;
- * 512/1024/2048/4096 entries per workgroup for runs of short rows). */
+/* ll_gpu_pagerank with an explicit row-block capacity (0 = option pr_cap,
+ * default 2048; 512/1024/2048/4096 entries per workgroup for runs of short
+ * rows). */
 int ll_gpu_pagerank_ex(ll_gpu_graph* g, int level, int iters, double damping, uint32_t cap,
                        double* ranks_out, double* last_diff);
 /* ll_gpu_bfs plus edges_traversed = sum of live out-degrees of the reached

@@ -36,6 +36,7 @@ struct Entry {
 };
 const Entry kTasks[] = {
     {"gpu_pagerank", "PageRank, pull, FP32 compensated (pagerank / pagerank_double)"},
+    {"gpu_pagerank_push", "PageRank, push, FP64 atomics (pagerank_push / pagerank_double_push)"},
     {"gpu_sssp_unweighted", "BFS levels from -R (sssp_unweighted)"},
     {"gpu_bfs", "BFS reached count from -R (bfs)"},
     {"gpu_tc", "Triangle counting on the symmetrised snapshot (tc_u / tc_od)"},
@@ -161,6 +162,8 @@ int main(int argc, char** argv) {
     roots.push_back(int64_t(llhost::rng_u64(seed, 77, uint64_t(i)) % G.num_nodes()));
   std::unique_ptr<ll_gpu_task<Graph>> t;
   if (task == "gpu_pagerank") t.reset(new gpu_pagerank<Graph>(iters < 0 ? 10 : iters));
+  else if (task == "gpu_pagerank_push")
+    t.reset(new gpu_pagerank_push<Graph>(iters < 0 ? 10 : iters));
   else if (task == "gpu_sssp_unweighted") t.reset(new gpu_sssp_unweighted<Graph>(root));
   else if (task == "gpu_bfs") t.reset(new gpu_bfs<Graph>(root));
   else if (task == "gpu_tc") t.reset(new gpu_triangle_counting<Graph>());

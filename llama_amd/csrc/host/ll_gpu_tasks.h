@@ -157,6 +157,35 @@ class gpu_pagerank : public ll_gpu_task<Graph> {
   std::vector<double> rank_;
 };
 
+// pagerank_push / pagerank_double_push (pagerank.h:217-306): same ranks by
+// pushing contributions over out-edges; run() returns 0, finalize() the sum.
+template <class Graph>
+class gpu_pagerank_push : public ll_gpu_task<Graph> {
+ public:
+  explicit gpu_pagerank_push(int iters, double d = 0.85)
+      : ll_gpu_task<Graph>("GPU PageRank - Push"), iters_(iters), d_(d) {}
+  void initialize(Graph* G) override {
+    ll_gpu_task<Graph>::initialize(G);
+    rank_.assign(ll_gpu_max_nodes(*G), 0.0);
+  }
+  double run() override {
+    this->m_->check(ll_gpu_pagerank_push(this->m_->graph(), -1, iters_, d_, rank_.data(), nullptr),
+                    "ll_gpu_pagerank_push");
+    return 0;
+  }
+  double finalize() override {
+    double s = 0;
+    for (double r : rank_) s += r;
+    return s;
+  }
+  void print_results(FILE* f) override { print_part(f, rank_.data(), rank_.size()); }
+
+ private:
+  int iters_;
+  double d_;
+  std::vector<double> rank_;
+};
+
 // sssp_unweighted (sssp.h:220-333): dist = BFS level, INT_MAX-1 unreached;
 // finalize returns the largest level (LL_SSSP_RETURNS_MAX, sssp.h:53).
 template <class Graph>

@@ -326,6 +326,16 @@ class DeviceGraph:
             C.byref(diff)))
         return out, float(diff.value)
 
+    def pagerank_push(self, iters: int = 10, damping: float = 0.85,
+                      level: int = -1) -> tuple[np.ndarray, float]:
+        """Push PageRank (FP64 atomics over out-edges): (ranks, last L1 change)."""
+        out = np.zeros(self.V, np.float64)
+        diff = C.c_double()
+        self.ctx.check(self.lib.ll_gpu_pagerank_push(
+            self._h, level, iters, damping, out.ctypes.data_as(C.POINTER(C.c_double)),
+            C.byref(diff)))
+        return out, float(diff.value)
+
     def bfs(self, root: int, level: int = -1, want_levels: bool = True):
         """Direction-optimising BFS: (levels or None, reached, edges examined)."""
         lv = np.zeros(self.V, np.int32) if want_levels else None

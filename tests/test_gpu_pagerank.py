@@ -283,3 +283,25 @@ def test_pagerank_hot_cold_hub_and_empty(ctx):
         assert np.allclose(r0, 0.15 / 5, rtol=1e-6)
     finally:
         ctx.set_option("pr_hot_bits", -1)
+
+
+def test_pagerank_push_matches_fp64(ctx, ml):
+    """Push variant (FP64 atomics over out-edges) against pagerank_double."""
+    g, o = ml
+    dg = DeviceGraph(ctx, g)
+    for L in (0, g.num_levels - 1):
+        gpu_case(f"pagerank push level {L}")
+        ref, ref_diff = o.pagerank(20, level=L)
+        r, diff = dg.pagerank_push(20, level=L)
+        assert l1rel(r, ref) <= 1e-12
+        assert abs(diff - ref_diff) <= 1e-9 * ref_diff + 1e-15
+    gpu_case("pagerank push hub + empty")
+    h = HostGraph(20003)
+    h.add_level(np.arange(1, 20001), np.zeros(20000, int))
+    ref, _ = oracle.OracleGraph.from_host(h).pagerank(3)
+    r, _ = DeviceGraph(ctx, h).pagerank_push(3)
+    assert l1rel(r, ref) <= 1e-12
+    e = HostGraph(4)
+    e.add_level([], [])
+    r, _ = DeviceGraph(ctx, e).pagerank_push(2)
+    assert np.allclose(r, 0.15 / 4, rtol=1e-12)
