@@ -54,6 +54,21 @@ int ll_host_graph_has_deletions(const ll_host_graph* g);
 int ll_host_graph_add_level(ll_host_graph* g, const uint32_t* ins_src, const uint32_t* ins_dst,
                             uint64_t n_ins, const uint32_t* del_src, const uint32_t* del_dst,
                             uint64_t n_del);
+/* As ll_host_graph_add_level with a weight per inserted edge (ins_w may be
+ * NULL).  The first level decides whether the graph is weighted; weights
+ * live per level beside the out-direction edge table, one per word (0 for
+ * continuation words), like the reference's `weight` edge property
+ * (ll_mlcsr_properties.h:729-1135, filled by ll_load_utils.h:684-687). */
+int ll_host_graph_add_level_w(ll_host_graph* g, const uint32_t* ins_src, const uint32_t* ins_dst,
+                              const float* ins_w, uint64_t n_ins, const uint32_t* del_src,
+                              const uint32_t* del_dst, uint64_t n_del);
+int ll_host_graph_is_weighted(const ll_host_graph* g);
+/* Borrowed out-direction weights of one level (*w = NULL, *n = 0 when the
+ * graph is unweighted). */
+int ll_host_graph_level_weights(const ll_host_graph* g, uint32_t level, const float** w,
+                                uint64_t* n);
+/* Out-direction weights of the snapshot in ll_host_graph_flatten order. */
+int ll_host_graph_flatten_weights(ll_host_graph* g, int level, const float** w, uint64_t* n);
 /* Draw and freeze one random incremental level (uniform new edges, uniform
  * live deletions); the batches stay readable through ll_host_graph_last_batch. */
 int ll_host_graph_add_random_level(ll_host_graph* g, double ins_frac, double del_frac,

@@ -81,6 +81,11 @@ HOST_SIGNATURES = {
     "ll_host_graph_num_levels": (C.c_int, [vp]),
     "ll_host_graph_has_deletions": (C.c_int, [vp]),
     "ll_host_graph_add_level": (C.c_int, [vp, P(u32), P(u32), u64, P(u32), P(u32), u64]),
+    "ll_host_graph_add_level_w": (C.c_int, [vp, P(u32), P(u32), P(f32), u64, P(u32), P(u32),
+                                            u64]),
+    "ll_host_graph_is_weighted": (C.c_int, [vp]),
+    "ll_host_graph_level_weights": (C.c_int, [vp, u32, P(P(f32)), P(u64)]),
+    "ll_host_graph_flatten_weights": (C.c_int, [vp, C.c_int, P(P(f32)), P(u64)]),
     "ll_host_graph_add_random_level": (C.c_int, [vp, f64, f64, u64]),
     "ll_host_graph_last_batch": (C.c_int, [vp, P(vp), P(vp)]),
     "ll_host_graph_level_view": (C.c_int, [vp, u32, u32, P(LevelView)]),

@@ -22,6 +22,7 @@ struct ll_host_graph {
   ll_host_edges last_ins, last_del;
   llhost::vec<uint64_t> flat_ofs;
   llhost::vec<uint32_t> flat_adj;
+  llhost::vec<float> flat_w;
 };
 
 namespace llhost {
@@ -60,11 +61,19 @@ int ll_host_graph_has_deletions(const ll_host_graph* g) { return g ? int(g->g->d
 int ll_host_graph_add_level(ll_host_graph* g, const uint32_t* ins_src, const uint32_t* ins_dst,
                             uint64_t n_ins, const uint32_t* del_src, const uint32_t* del_dst,
                             uint64_t n_del) {
+  return ll_host_graph_add_level_w(g, ins_src, ins_dst, nullptr, n_ins, del_src, del_dst, n_del);
+}
+
+int ll_host_graph_add_level_w(ll_host_graph* g, const uint32_t* ins_src, const uint32_t* ins_dst,
+                              const float* ins_w, uint64_t n_ins, const uint32_t* del_src,
+                              const uint32_t* del_dst, uint64_t n_del) {
   if (!g) return fail(LL_GPU_EINVAL, "graph is NULL");
   if ((n_ins && (!ins_src || !ins_dst)) || (n_del && (!del_src || !del_dst)))
     return fail(LL_GPU_EINVAL, "NULL edge array");
+  if (ins_w && g->g->num_levels() > 0 && !g->g->weighted())
+    return fail(LL_GPU_EINVAL, "weights on a level of a graph whose first level had none");
   try {
-    if (!g->g->add_level(ins_src, ins_dst, n_ins, del_src, del_dst, n_del))
+    if (!g->g->add_level(ins_src, ins_dst, n_ins, del_src, del_dst, n_del, ins_w))
       return fail(LL_GPU_EINVAL, g->g->last_error());
     g->upd.reset();  // the sampling snapshot is rebuilt lazily from the graph
   } catch (const std::bad_alloc&) {
@@ -77,9 +86,11 @@ int ll_host_graph_add_level_edges(ll_host_graph* g, const ll_host_edges* ins,
                                   const ll_host_edges* del) {
   if (!g) return fail(LL_GPU_EINVAL, "graph is NULL");
   const uint64_t ni = ins ? ins->e.src.size() : 0, nd = del ? del->e.src.size() : 0;
-  return ll_host_graph_add_level(g, ni ? ins->e.src.data() : nullptr, ni ? ins->e.dst.data() : nullptr,
-                                 ni, nd ? del->e.src.data() : nullptr,
-                                 nd ? del->e.dst.data() : nullptr, nd);
+  const bool w = ni && ins->w.size() == ni;  // .xs1 records carry weights
+  return ll_host_graph_add_level_w(g, ni ? ins->e.src.data() : nullptr,
+                                   ni ? ins->e.dst.data() : nullptr, w ? ins->w.data() : nullptr,
+                                   ni, nd ? del->e.src.data() : nullptr,
+                                   nd ? del->e.dst.data() : nullptr, nd);
 }
 
 int ll_host_graph_add_random_level(ll_host_graph* g, double ins_frac, double del_frac,
@@ -133,6 +144,30 @@ int ll_host_graph_level_view(const ll_host_graph* g, uint32_t dir, uint32_t leve
   out->et = lv.et.data();
   out->flags = LL_GPU_HAS_CONT | (g->g->deletions() ? LL_GPU_HAS_DELETIONS : 0u);
   out->page_table = nullptr;
+  return LL_GPU_OK;
+}
+
+int ll_host_graph_is_weighted(const ll_host_graph* g) { return g ? int(g->g->weighted()) : 0; }
+
+int ll_host_graph_level_weights(const ll_host_graph* g, uint32_t level, const float** w,
+                                uint64_t* n) {
+  if (!g || !w || !n) return fail(LL_GPU_EINVAL, "NULL argument");
+  if (int(level) >= g->g->num_levels()) return fail(LL_GPU_EINVAL, "no such level");
+  const auto& lv = g->g->level(0, int(level));
+  *w = lv.wt.empty() ? nullptr : lv.wt.data();
+  *n = lv.wt.size();
+  return LL_GPU_OK;
+}
+
+int ll_host_graph_flatten_weights(ll_host_graph* g, int level, const float** w, uint64_t* n) {
+  if (!g || !w || !n) return fail(LL_GPU_EINVAL, "NULL argument");
+  const int L = g->g->num_levels();
+  if (L == 0) return fail(LL_GPU_ESTATE, "graph has no levels");
+  if (level < 0) level = L - 1;
+  if (level >= L) return fail(LL_GPU_EINVAL, "no such level");
+  g->g->flatten_weights(level, g->flat_w);
+  *w = g->flat_w.data();
+  *n = g->flat_w.size();
   return LL_GPU_OK;
 }
 

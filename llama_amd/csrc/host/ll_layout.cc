@@ -141,8 +141,10 @@ bool Graph::find_live_word(int dir, uint32_t u, uint32_t v, uint64_t* edge_id) c
 }
 
 bool Graph::add_level(const uint32_t* ins_src, const uint32_t* ins_dst, uint64_t n_ins,
-                      const uint32_t* del_src, const uint32_t* del_dst, uint64_t n_del) {
+                      const uint32_t* del_src, const uint32_t* del_dst, uint64_t n_del,
+                      const float* ins_w) {
   const uint32_t k = uint32_t(num_levels());
+  if (k == 0) weighted_ = ins_w != nullptr;
   if (k >= kMaxLevel) { err_ = "level IDs exhausted"; return false; }
   if (k == 0 && n_del > 0) { err_ = "level 0 cannot delete edges"; return false; }
   if (n_del > 0 && !deletions_) { err_ = "deletions need a graph created with deletions on"; return false; }
@@ -201,7 +203,14 @@ bool Graph::add_level(const uint32_t* ins_src, const uint32_t* ins_dst, uint64_t
   bool src_sorted = true;
   for (uint64_t i = 1; i < n_ins && src_sorted; i++)
     src_sorted = ins_src[i - 1] <= ins_src[i];
-  if (!build_direction(0, k, new_out, del_out, ins_src, ins_dst, n_ins, src_sorted)) return false;
+  std::vector<float> ones;
+  if (weighted_ && !ins_w) {  // an unweighted level of a weighted graph: weight 1
+    ones.assign(n_ins, 1.0f);
+    ins_w = ones.data();
+  }
+  if (!build_direction(0, k, new_out, del_out, ins_src, ins_dst, n_ins, src_sorted,
+                       weighted_ ? ins_w : nullptr))
+    return false;
   if (!build_direction(1, k, new_in, del_in, ins_dst, ins_src, n_ins, false)) return false;
   return true;
 }
@@ -210,13 +219,17 @@ bool Graph::add_level(const uint32_t* ins_src, const uint32_t* ins_dst, uint64_t
 // level k of direction `dir` exactly as init_level_from_degrees/init_node do.
 bool Graph::build_direction(int dir, uint32_t k, const vec<uint32_t>& new_cnt,
                             const vec<uint32_t>& del_cnt, const uint32_t* key,
-                            const uint32_t* val, uint64_t n, bool key_sorted) {
+                            const uint32_t* val, uint64_t n, bool key_sorted,
+                            const float* wts) {
   vec<uint64_t> gofs;
   exclusive_scan(new_cnt, gofs);
   vec<uint32_t> grouped;
+  vec<float> gw;
   const uint32_t* gval = val;
+  const float* gwt = wts;
   if (!key_sorted) {
     grouped.resize(n);
+    if (wts) gw.resize(n);
     if (n > (1u << 16)) {
       // stable group-by: sort (key << 32 | position), positions break ties
       vec<uint64_t> kp(n);
@@ -224,12 +237,20 @@ bool Graph::build_direction(int dir, uint32_t k, const vec<uint32_t>& new_cnt,
       for (uint64_t i = 0; i < n; i++) kp[i] = (uint64_t(key[i]) << 32) | i;
       radix_sort(kp, 32 + bit_width(V_));
 #pragma omp parallel for schedule(static)
-      for (uint64_t i = 0; i < n; i++) grouped[i] = val[kp[i] & 0xffffffffull];
+      for (uint64_t i = 0; i < n; i++) {
+        grouped[i] = val[kp[i] & 0xffffffffull];
+        if (wts) gw[i] = wts[kp[i] & 0xffffffffull];
+      }
     } else {
       std::vector<uint64_t> cur(gofs.begin(), gofs.end());
-      for (uint64_t i = 0; i < n; i++) grouped[cur[key[i]]++] = val[i];
+      for (uint64_t i = 0; i < n; i++) {
+        const uint64_t p = cur[key[i]]++;
+        grouped[p] = val[i];
+        if (wts) gw[p] = wts[i];
+      }
     }
     gval = grouped.data();
+    if (wts) gwt = gw.data();
   }
 
   auto& levels = dir_[dir].levels;
@@ -273,6 +294,7 @@ bool Graph::build_direction(int dir, uint32_t k, const vec<uint32_t>& new_cnt,
   vec<uint64_t> pos;
   const uint64_t total = exclusive_scan(delta, pos);
   lv.et.resize(total);
+  if (gwt) lv.wt.resize(total);
   const bool dels = deletions_;
 
   // Pass 2: write segments, continuation records and final starts.
@@ -286,6 +308,7 @@ bool Graph::build_direction(int dir, uint32_t k, const vec<uint32_t>& new_cnt,
     const uint32_t* src = gval + gofs[v];
     for (uint32_t j = 0; j < nw; j++) {
       lv.et[w + j] = dels ? word_create(src[j], kNeverDeleted) : uint64_t(src[j]);
+      if (gwt) lv.wt[w + j] = gwt[gofs[v] + j];
       if (j > 0 && src[j - 1] > src[j]) sorted = false;
     }
     if (nw > 0 && e.start != kNilEdge) e.start = int64_t(edge_create(k, w));
@@ -293,12 +316,42 @@ bool Graph::build_direction(int dir, uint32_t k, const vec<uint32_t>& new_cnt,
       const VtEntry b = prev[v];
       lv.et[w + nw] = uint64_t(b.start);
       lv.et[w + nw + 1] = uint64_t(b.len) | (uint64_t(b.degree) << 32);
+      if (gwt) lv.wt[w + nw] = lv.wt[w + nw + 1] = 0.0f;
     }
   }
   lv.vt[V_] = VtEntry{int64_t(edge_create(k, total)), 0, 0};
   lv.sorted_segments = sorted;
   levels.push_back(std::move(lv));
   return true;
+}
+
+void Graph::flatten_weights(int level, vec<float>& w) const {
+  const auto& lv = dir_[0].levels;
+  vec<uint64_t> ofs;
+  vec<uint32_t> adj;
+  flatten(0, level, ofs, adj);
+  w.resize(adj.size());
+  if (!weighted_) {
+#pragma omp parallel for schedule(static)
+    for (uint64_t i = 0; i < adj.size(); i++) w[i] = 1.0f;
+    return;
+  }
+  const bool dels = deletions_;
+#pragma omp parallel for schedule(dynamic, 4096)
+  for (uint64_t v = 0; v < V_; v++) {
+    uint64_t o = ofs[v], end = ofs[v + 1];
+    VtEntry e = lv[level].vt[v];
+    while (e.start != kNilEdge && e.len > 0 && o < end) {
+      const uint32_t k = edge_level(e.start);
+      const uint64_t* seg = lv[k].et.data() + edge_index(e.start);
+      const float* sw = lv[k].wt.data() + edge_index(e.start);
+      for (uint32_t j = 0; j < e.len; j++)
+        if (!dels || word_maxlev(seg[j]) > uint32_t(level)) w[o++] = sw[j];
+      if (k == 0) break;
+      e.start = int64_t(seg[e.len]);
+      e.len = uint32_t(seg[e.len + 1]);
+    }
+  }
 }
 
 void Graph::flatten(int dir, int level, vec<uint64_t>& ofs, vec<uint32_t>& adj) const {

@@ -68,6 +68,8 @@ static_assert(sizeof(VtEntry) == 16, "VT entry must be 16 bytes");
 struct Level {
   vec<VtEntry> vt;     // V+1 entries; VT[V] is the sentinel
   vec<uint64_t> et;    // edge-table words incl. continuation records
+  vec<float> wt;       // out direction of a weighted graph: weight per ET word (0 for
+                       // continuation words), the level's `weight` edge property
   bool sorted_segments = true; // every segment ascending by neighbour
 };
 
@@ -94,8 +96,12 @@ class Graph {
   // Freeze one new level from a batch of inserted edges (in insertion order)
   // and deleted frozen edges (each live at the latest level).  Level 0 takes
   // no deletions.  Returns false with last_error() set on bad input.
+  // ins_w (may be null): weight of each inserted edge; the first level decides
+  // whether the graph is weighted (later unweighted levels get weight 1).
   bool add_level(const uint32_t* ins_src, const uint32_t* ins_dst, uint64_t n_ins,
-                 const uint32_t* del_src, const uint32_t* del_dst, uint64_t n_del);
+                 const uint32_t* del_src, const uint32_t* del_dst, uint64_t n_del,
+                 const float* ins_w = nullptr);
+  bool weighted() const { return weighted_; }
 
   // Live degree at `level` (precomputed in the VT).
   uint32_t degree(int dir, int level, uint64_t v) const {
@@ -107,17 +113,21 @@ class Graph {
   // words deleted at <= level skipped.  This restates iter_begin/iter_next
   // (ll_mlcsr_sp.h:1994-2196) on the host for the update generator.
   void flatten(int dir, int level, vec<uint64_t>& ofs, vec<uint32_t>& adj) const;
+  // Weights of the out-direction snapshot in the order of flatten(0, level).
+  void flatten_weights(int level, vec<float>& w) const;
 
   const std::string& last_error() const { return err_; }
 
  private:
   bool build_direction(int dir, uint32_t k, const vec<uint32_t>& new_cnt,
                        const vec<uint32_t>& del_cnt, const uint32_t* key,
-                       const uint32_t* val, uint64_t n, bool key_sorted);
+                       const uint32_t* val, uint64_t n, bool key_sorted,
+                       const float* wts = nullptr);
   bool find_live_word(int dir, uint32_t u, uint32_t v, uint64_t* edge_id) const;
 
   uint64_t V_;
   bool deletions_;
+  bool weighted_ = false;
   Direction dir_[2];
   std::vector<PatchList> patches_[2];
   std::string err_;

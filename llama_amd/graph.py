@@ -101,17 +101,44 @@ class HostGraph:
         self._h = h
 
     # ---- building --------------------------------------------------------------
-    def add_level(self, ins_src=(), ins_dst=(), del_src=(), del_dst=()) -> int:
-        """Freeze one level (inserts in insertion order, deletions of live edges)."""
+    def add_level(self, ins_src=(), ins_dst=(), del_src=(), del_dst=(), weights=None) -> int:
+        """Freeze one level (inserts in insertion order, deletions of live edges);
+        `weights` (one per inserted edge) makes the graph weighted when given
+        on its first level."""
         s = np.ascontiguousarray(ins_src, dtype=np.uint32)
         d = np.ascontiguousarray(ins_dst, dtype=np.uint32)
         ds = np.ascontiguousarray(del_src, dtype=np.uint32)
         dd = np.ascontiguousarray(del_dst, dtype=np.uint32)
         if s.shape != d.shape or ds.shape != dd.shape:
             raise ValueError("edge arrays differ in length")
-        N.check_host(N.host().ll_host_graph_add_level(self._h, _u32p(s), _u32p(d), s.size,
-                                                      _u32p(ds), _u32p(dd), ds.size))
+        w = None
+        if weights is not None:
+            w = np.ascontiguousarray(weights, dtype=np.float32)
+            if w.shape != s.shape:
+                raise ValueError("one weight per inserted edge")
+        N.check_host(N.host().ll_host_graph_add_level_w(
+            self._h, _u32p(s), _u32p(d),
+            w.ctypes.data_as(C.POINTER(C.c_float)) if w is not None else None, s.size,
+            _u32p(ds), _u32p(dd), ds.size))
         return self.num_levels - 1
+
+    @property
+    def weighted(self) -> bool:
+        return bool(N.host().ll_host_graph_is_weighted(self._h))
+
+    def level_weights(self, level: int) -> np.ndarray | None:
+        """Out-direction weights of one level, one per edge-table word."""
+        p, n = C.POINTER(C.c_float)(), C.c_uint64()
+        N.check_host(N.host().ll_host_graph_level_weights(self._h, level, C.byref(p), C.byref(n)))
+        return np.ctypeslib.as_array(p, shape=(n.value,)) if n.value else None
+
+    def flatten_weights(self, level: int = -1) -> np.ndarray:
+        """Out-direction weights in flatten(DIR_OUT, level) order."""
+        p, n = C.POINTER(C.c_float)(), C.c_uint64()
+        N.check_host(N.host().ll_host_graph_flatten_weights(self._h, level, C.byref(p),
+                                                            C.byref(n)))
+        return np.ctypeslib.as_array(p, shape=(n.value,)).copy() if n.value else \
+            np.zeros(0, np.float32)
 
     def add_level_edges(self, ins: Edges | None, dels: Edges | None = None) -> int:
         N.check_host(N.host().ll_host_graph_add_level_edges(

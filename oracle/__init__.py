@@ -56,6 +56,8 @@ def lib() -> C.CDLL:
                                           P(f64)]),
             "orc_wpr_adjoint": (C.c_int, [u64, P(u64), P(u32), P(f64), P(f64), P(f64), C.c_int,
                                           f64, P(f64), P(f64)]),
+            "orc_graph_add_weights": (C.c_int, [vp, u32, P(C.c_float), u64]),
+            "orc_sssp_weighted": (i64, [vp, C.c_int, i64, P(C.c_float), P(f64), P(i64)]),
             "orc_num_threads": (C.c_int, []),
         }
         for k, (r, a) in sig.items():
@@ -120,6 +122,26 @@ class OracleGraph:
                                        C.byref(diff))
         assert rc == 0
         return r, diff.value
+
+    def add_weights(self, hg) -> None:
+        """Borrow the out-direction weights of every level of host graph hg."""
+        for k in range(hg.num_levels):
+            w = hg.level_weights(k)
+            if w is None:
+                raise ValueError("host graph is unweighted")
+            self._keep.append(w)
+            if lib().orc_graph_add_weights(self._h, k, _p(w, C.c_float), w.size) != 0:
+                raise ValueError("weights do not match the level's edge table")
+
+    def sssp_weighted(self, root: int, level: int = -1):
+        """sssp_weighted restated: (dist float32[V], reached, max distance, rounds)."""
+        dist = np.zeros(self.V, np.float32)
+        mx, rounds = C.c_double(), C.c_int64()
+        n = lib().orc_sssp_weighted(self._h, level, root, _p(dist, C.c_float), C.byref(mx),
+                                    C.byref(rounds))
+        if n < 0:
+            raise ValueError("sssp_weighted: bad level/root or unweighted graph")
+        return dist, int(n), float(mx.value), int(rounds.value)
 
     def bfs(self, root: int, level: int = -1):
         dist = np.zeros(self.V, np.int32)

@@ -19,6 +19,10 @@
 //                     symmetrised, deduplicated, self-loop-free snapshot
 //   scc               textbook Tarjan (the reference tarjan_scc.h:55-206 is not a
 //                     correct SCC algorithm), labels = min vertex ID per component
+//   sssp_weighted     ll_b_sssp_weighted::run (benchmark/benchmarks/sssp.h:59-172):
+//                     synchronous rounds, dist_nxt[s] = min(dist[n] + weight[e]) in
+//                     FP32 over the out-edges e of updated vertices, weights read
+//                     by edge ID from per-level arrays (the `weight` property)
 //   ppr, wpr, adjoint FP64 restatements of the definitions in ll_gpu.h (no
 //                     reference counterpart)
 //
@@ -59,6 +63,8 @@ struct orc_graph {
   uint64_t V;
   int deletions;
   std::vector<Lvl> lv[2];
+  std::vector<const float*> wt;  // out-direction `weight` property per level (by edge index)
+  std::vector<uint64_t> n_wt;
 };
 
 namespace {
@@ -133,6 +139,15 @@ void iter_begin(const orc_graph* g, int dir, int level, uint64_t n, Iter& it) {
     iter_next(g, dir, it);
     it.last_node = keep;
   }
+}
+
+// Visit (edge ID, neighbour) of the live out/in edges of n in iterator order.
+template <typename F>
+inline void foreach_edge(const orc_graph* g, int dir, int level, uint64_t n, F&& f) {
+  Iter it;
+  iter_begin(g, dir, level, n, it);
+  for (int64_t e = iter_next(g, dir, it); e != -1; e = iter_next(g, dir, it))
+    f(uint64_t(e), uint64_t(it.last_node));
 }
 
 // Visit the live neighbours of n at `level` in iterator order.
@@ -478,6 +493,68 @@ int orc_wpr_adjoint(uint64_t V, const uint64_t* ofs, const uint32_t* adj, const 
     for (uint64_t e = ofs[u]; e < ofs[u + 1]; e++) dw[e] = d * r[u] * lam[adj[e]];
   for (uint64_t v = 0; v < V; v++) dp[v] = (1 - d) * lam[v];
   return 0;
+}
+
+// Borrow the out-direction weights of one level (one per edge-table word).
+int orc_graph_add_weights(orc_graph* g, uint32_t level, const float* w, uint64_t n) {
+  if (level >= g->lv[0].size() || n != g->lv[0][level].n_et) return -1;
+  if (g->wt.size() <= level) {
+    g->wt.resize(level + 1, nullptr);
+    g->n_wt.resize(level + 1, 0);
+  }
+  g->wt[level] = w;
+  g->n_wt[level] = n;
+  return 0;
+}
+
+// sssp_weighted (sssp.h:59-172), restated round for round: every vertex
+// updated in the previous round relaxes its out-edges into dist_nxt (FP32
+// sums dist[n] + weight[e]), then dist = dist_nxt; stops when a round updates
+// nothing.  dist[v] = INT_MAX-1 (as float) when unreached.  Returns the number
+// of reached vertices; *max_dist (may be NULL) gets the largest finite distance
+// (the task's finalize with LL_SSSP_RETURNS_MAX, truncated to int there).
+int64_t orc_sssp_weighted(const orc_graph* g, int level, int64_t root, float* dist,
+                          double* max_dist, int64_t* rounds) {
+  level = resolve(g, level);
+  if (level < 0 || root < 0 || uint64_t(root) >= g->V) return -1;
+  for (int k = 0; k <= level; k++)
+    if (k >= int(g->wt.size()) || !g->wt[k]) return -2;  // unweighted level
+  const uint64_t V = g->V;
+  const float inf = float(INT_MAX - 1);
+  std::vector<float> nxt(V);
+  std::vector<char> upd(V, 0), upd_nxt(V, 0);
+  for (uint64_t v = 0; v < V; v++) dist[v] = nxt[v] = (int64_t(v) == root) ? 0.0f : inf;
+  upd[root] = 1;
+  int64_t r = 0;
+  for (bool any = true; any; r++) {
+    for (uint64_t n = 0; n < V; n++) {
+      if (!upd[n]) continue;
+      foreach_edge(g, 0, level, n, [&](uint64_t e, uint64_t s) {
+        const float cand = dist[n] + g->wt[e >> kBitsIndex][e & kIndexMask];
+        if (nxt[s] > cand) {
+          nxt[s] = cand;
+          upd_nxt[s] = 1;
+        }
+      });
+    }
+    any = false;
+    for (uint64_t v = 0; v < V; v++) {
+      dist[v] = nxt[v];
+      upd[v] = upd_nxt[v];
+      upd_nxt[v] = 0;
+      any = any || upd[v];
+    }
+  }
+  int64_t reached = 0;
+  double mx = 0;
+  for (uint64_t v = 0; v < V; v++)
+    if (dist[v] < inf) {
+      reached++;
+      mx = std::max(mx, double(dist[v]));
+    }
+  if (max_dist) *max_dist = mx;
+  if (rounds) *rounds = r;
+  return reached;
 }
 
 int orc_num_threads(void) { return omp_get_max_threads(); }

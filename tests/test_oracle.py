@@ -230,3 +230,60 @@ def test_oracle_iterator_matches_host_flatten_all_levels():
             assert np.array_equal(ho, oo) and np.array_equal(ha, oa)
             it, ds = o.count_check(d, L)
             assert it == ds == ha.size
+
+
+# ---- weighted SSSP (sssp.h:59-172) ---------------------------------------------------
+INF32 = np.float32(2**31 - 2)
+
+
+def test_sssp_weighted_path_closed_form():
+    """A chain 0->1->...->k: distances are the float32 running sums in path
+    order; the vertex after a gap is never reached."""
+    k = 12
+    w = np.random.default_rng(3).uniform(0.1, 3.0, k).astype(np.float32)
+    g = HostGraph(k + 2)
+    g.add_level(np.arange(k), np.arange(1, k + 1), weights=w)
+    o = oracle.OracleGraph.from_host(g)
+    o.add_weights(g)
+    dist, reached, mx, rounds = o.sssp_weighted(0)
+    exp = np.zeros(k + 1, np.float32)
+    for i in range(k):
+        exp[i + 1] = np.float32(exp[i] + w[i])
+    assert np.array_equal(dist[:k + 1], exp) and dist[k + 1] == INF32
+    assert reached == k + 1 and mx == float(exp[-1]) and rounds == k + 1
+
+
+def test_sssp_weighted_matches_scipy_dijkstra_multilevel():
+    """Random weighted levels with deletions: against scipy's FP64 Dijkstra on
+    the snapshot's weighted edges (FP32 sums: relative 1e-5)."""
+    rng = np.random.default_rng(8)
+    n = 300
+    g = HostGraph(n, deletions=True)
+    live = set()
+    for k in range(4):
+        cnt = 1500 if k == 0 else 120
+        ins = []
+        while len(ins) < cnt:
+            e = (int(rng.integers(0, n)), int(rng.integers(0, n)))
+            if e[0] != e[1] and e not in live and e not in ins:
+                ins.append(e)
+        dels = [] if k == 0 else [e for e in sorted(live) if rng.random() < 0.03]
+        w = rng.uniform(0.0, 2.0, len(ins)).astype(np.float32)
+        g.add_level([e[0] for e in ins], [e[1] for e in ins],
+                    [e[0] for e in dels], [e[1] for e in dels], weights=w)
+        live |= set(ins)
+        live -= set(dels)
+    o = oracle.OracleGraph.from_host(g)
+    o.add_weights(g)
+    for level in (0, g.num_levels - 1):
+        ofs, adj = g.flatten(DIR_OUT, level)
+        wts = g.flatten_weights(level).astype(np.float64)
+        src = np.repeat(np.arange(n), np.diff(ofs).astype(np.int64))
+        A = sp.csr_matrix((wts + 1e-300, (src, adj)), shape=(n, n))  # keep zero weights
+        for root in (0, 17, 200):
+            dist, reached, _, _ = o.sssp_weighted(root, level=level)
+            ref = csgraph.dijkstra(A, indices=root)
+            fin = np.isfinite(ref)
+            assert np.array_equal(dist[~fin], np.full((~fin).sum(), INF32))
+            assert np.allclose(dist[fin], ref[fin], rtol=1e-5, atol=1e-6)
+            assert reached == int(fin.sum())
