@@ -129,6 +129,22 @@ int ll_gpu_pagerank(ll_gpu_graph* g, int level, int iters, double damping, doubl
  * atomic adds; ranks_out[max_nodes] and last_diff (may be NULL) as above. */
 int ll_gpu_pagerank_push(ll_gpu_graph* g, int level, int iters, double damping,
                          double* ranks_out, double* last_diff);
+/* Replaces: the per-level `weight` edge property read by sssp_weighted
+ * (ll_mlcsr_edge_property<float>, ll_mlcsr_properties.h:729-1135; filled by
+ * the .xs1 loader, ll_load_utils.h:628-790): w[num_et_words] holds the weight
+ * of every out-direction edge-table word of `level` (continuation words are
+ * ignored), finite and >= 0.  Level `level` must be pushed; snapshots whose
+ * levels all carry weights get them compacted with the out-CSR. */
+int ll_gpu_graph_push_weights(ll_gpu_graph* g, uint32_t level, const float* w, uint64_t n);
+/* Replaces: task `sssp_weighted` = ll_b_sssp_weighted::run
+ * (benchmark/benchmarks/sssp.h:59-172): dist_out[max_nodes] (may be NULL) =
+ * FP32 shortest-path distances from root over the snapshot's weighted
+ * out-edges, INT_MAX-1 as float for unreached vertices (bit-exact with the
+ * reference); reached = vertices with a finite distance; max_dist = the
+ * largest one (the task's finalize under LL_SSSP_RETURNS_MAX, truncated to
+ * int there); rounds = frontier rounds run.  All may be NULL. */
+int ll_gpu_sssp(ll_gpu_graph* g, int level, int64_t root, float* dist_out, uint64_t* reached,
+                double* max_dist, uint64_t* rounds);
 /* Replaces: task `sssp_unweighted` (benchmark/benchmarks/sssp.h:220-333) and
  * `bfs` (bfs.h:85-128): levels_out[max_nodes] (may be NULL) gets the BFS level,
  * INT_MAX-1 for unreached; reached = number of reached vertices (bfs_count);

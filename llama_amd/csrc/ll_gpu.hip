@@ -325,6 +325,37 @@ int ll_gpu_graph_push_level(ll_gpu_graph* g, const ll_gpu_level_view* v) {
   });
 }
 
+int ll_gpu_graph_push_weights(ll_gpu_graph* g, uint32_t level, const float* w, uint64_t n) {
+  if (!g) return LL_GPU_EINVAL;
+  ll_gpu_ctx* c = g->ctx;
+  return guarded(c, [&]() {
+    LLG_HIP(hipSetDevice(c->device));
+    auto& d = g->dir[LL_GPU_DIR_OUT];
+    LLG_CHECK(level < d.et.size(), LL_GPU_ESTATE, "weights for a level that is not pushed");
+    LLG_CHECK(n == d.et_words[level], LL_GPU_EINVAL,
+              "one weight per out-direction edge-table word of the level");
+    LLG_CHECK(n == 0 || w != nullptr, LL_GPU_EINVAL, "NULL weights");
+    for (uint64_t i = 0; i < n; i++)
+      LLG_CHECK(w[i] >= 0.0f && w[i] <= 3.0e38f, LL_GPU_EINVAL,
+                "weights must be finite and non-negative");
+    if (d.wt.size() < d.et.size()) d.wt.resize(d.et.size());
+    DevBuf b;
+    b.alloc(std::max<size_t>(n * 4, 4), c->stream);
+    if (n) upload(c, b.p, w, n * 4);
+    d.wt[level] = std::move(b);
+    std::vector<const float*> table(d.et.size(), nullptr);
+    for (size_t k = 0; k < d.wt.size(); k++) table[k] = d.wt[k].as<const float>();
+    DevBuf t;
+    t.alloc(table.size() * sizeof(void*), c->stream);
+    LLG_HIP(hipMemcpyAsync(t.p, table.data(), table.size() * sizeof(void*), hipMemcpyHostToDevice,
+                           c->stream));
+    d.wt_table = std::move(t);
+    LLG_HIP(hipStreamSynchronize(c->stream));  // the host table is released on return
+    g->upload_bytes += n * 4;
+    llgpu::drop_snapshots(g);
+  });
+}
+
 int ll_gpu_graph_patch_deletions(ll_gpu_graph* g, uint32_t dir, const uint64_t* ids,
                                  const uint32_t* maxlev, uint64_t n) {
   if (!g) return LL_GPU_EINVAL;

@@ -121,6 +121,7 @@ struct Csr {
   uint64_t V = 0, E = 0, stored = 0, segments = 0;
   DevBuf ofs;   // uint64_t[V+1]
   DevBuf adj;   // uint32_t[E]
+  DevBuf w;     // float[E]: out-edge weights when every level up to `level` has them
   double ms = 0;
 };
 
@@ -173,6 +174,8 @@ struct DirMirror {
   std::vector<uint64_t> et_words;
   DevBuf et_table;          // const uint64_t*[levels] on device
   DevBuf et_words_dev;      // uint64_t[levels]: words per level (bounds of the walk)
+  std::vector<DevBuf> wt;   // out direction: float weight per ET word per level (may be empty)
+  DevBuf wt_table;          // const float*[levels] on device (null for levels without)
 };
 
 }  // namespace llgpu

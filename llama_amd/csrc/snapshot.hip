@@ -159,6 +159,17 @@ __device__ inline uint64_t load_word(const uint64_t* const* et, const uint64_t* 
   return et[edge_level(a)][edge_index(a) + (x - l_vofs[s])];
 }
 
+// The same word plus its level and index (for the per-level weight arrays).
+__device__ inline uint64_t load_word_at(const uint64_t* const* et, const uint64_t* l_vofs,
+                                        const uint64_t* l_addr, uint32_t n, uint64_t x,
+                                        uint32_t& lvl, uint64_t& idx) {
+  uint32_t s = seg_search_lds(l_vofs, n, x);
+  uint64_t a = l_addr[s];
+  lvl = edge_level(a);
+  idx = edge_index(a) + (x - l_vofs[s]);
+  return et[lvl][idx];
+}
+
 __global__ __launch_bounds__(kBlock) void k_live_count(const uint64_t* const* et,
                                                        const uint64_t* seg_vofs,
                                                        const uint64_t* seg_addr,
@@ -186,13 +197,16 @@ __global__ __launch_bounds__(kBlock) void k_live_count(const uint64_t* const* et
   }
 }
 
-template <bool kDeletions>
+// kWeights: also write each live word's weight (the out direction's
+// per-level weight arrays, same level and index as the word) to wout.
+template <bool kDeletions, bool kWeights>
 __global__ __launch_bounds__(kBlock) void k_compact(const uint64_t* const* et,
                                                     const uint64_t* seg_vofs,
                                                     const uint64_t* seg_addr,
                                                     const uint64_t* blk_seg,
                                                     uint64_t total, uint32_t level,
-                                                    const uint64_t* blk_ofs, uint32_t* adj) {
+                                                    const uint64_t* blk_ofs, uint32_t* adj,
+                                                    const float* const* wt, float* wout) {
   __shared__ uint64_t l_vofs[kWordsPerBlock + 1];
   __shared__ uint64_t l_addr[kWordsPerBlock];
   __shared__ uint32_t wcnt[kWaves];
@@ -205,7 +219,9 @@ __global__ __launch_bounds__(kBlock) void k_compact(const uint64_t* const* et,
   for (uint64_t base = x0; base < x1; base += kBlock) {
     uint64_t x = base + threadIdx.x;
     bool valid = x < x1;
-    uint64_t w = valid ? load_word(et, l_vofs, l_addr, bs.n, x) : 0;
+    uint32_t lvl = 0;
+    uint64_t idx = 0;
+    uint64_t w = valid ? load_word_at(et, l_vofs, l_addr, bs.n, x, lvl, idx) : 0;
     bool live = valid && (!kDeletions || word_maxlev(w) > level);
     uint64_t mask = __ballot(live);
     uint32_t rank = __popcll(mask & ((uint64_t(1) << lane) - 1));
@@ -216,7 +232,10 @@ __global__ __launch_bounds__(kBlock) void k_compact(const uint64_t* const* et,
       if (i < wave) before += wcnt[i];
       round += wcnt[i];
     }
-    if (live) adj[out + before + rank] = uint32_t(w & kIndexMask);
+    if (live) {
+      adj[out + before + rank] = uint32_t(w & kIndexMask);
+      if (kWeights) wout[out + before + rank] = wt[lvl][idx];
+    }
     out += round;
     __syncthreads();
   }
@@ -310,6 +329,10 @@ const Csr& ensure_csr(ll_gpu_graph* g, uint32_t dir, int level) {
   const uint64_t nblk = (W + kWordsPerBlock - 1) / kWordsPerBlock;
   const bool dels = (g->flags & LL_GPU_HAS_DELETIONS) != 0;
   c.adj.alloc(std::max<uint64_t>(E, 1) * 4, s);
+  // out-edge weights too when every level up to this one carries them
+  bool weighted = dir == LL_GPU_DIR_OUT && int(g->dir[dir].wt.size()) > level;
+  for (int k = 0; weighted && k <= level; k++) weighted = g->dir[dir].wt[k].p != nullptr;
+  if (weighted) c.w.alloc(std::max<uint64_t>(E, 1) * 4, s);
   uint64_t live = W;
   if (nblk > 0) {
     DevBuf blk_cnt, blk_ofs, blk_seg;
@@ -336,18 +359,22 @@ const Csr& ensure_csr(ll_gpu_graph* g, uint32_t dir, int level) {
                   std::to_string(E) + " (missing deletion patches?)");
     {
       TimedLaunch t(ctx, "k_compact");
-      if (dels)
-        k_compact<true><<<unsigned(nblk), kBlock, 0, s>>>(et, seg_vofs.as<uint64_t>(),
-                                                           seg_addr.as<uint64_t>(),
-                                                           blk_seg.as<uint64_t>(), W,
-                                                           uint32_t(level), blk_ofs.as<uint64_t>(),
-                                                           c.adj.as<uint32_t>());
+      const uint64_t* bo = dels ? blk_ofs.as<uint64_t>() : nullptr;
+      const float* const* wt = weighted ? g->dir[dir].wt_table.as<const float* const>() : nullptr;
+      float* wout = weighted ? c.w.as<float>() : nullptr;
+      const unsigned nb = unsigned(nblk);
+      const uint64_t* sv = seg_vofs.as<uint64_t>();
+      const uint64_t* sa = seg_addr.as<uint64_t>();
+      const uint64_t* bs = blk_seg.as<uint64_t>();
+      uint32_t* adj = c.adj.as<uint32_t>();
+      if (dels && weighted)
+        k_compact<true, true><<<nb, kBlock, 0, s>>>(et, sv, sa, bs, W, uint32_t(level), bo, adj, wt, wout);
+      else if (dels)
+        k_compact<true, false><<<nb, kBlock, 0, s>>>(et, sv, sa, bs, W, uint32_t(level), bo, adj, wt, wout);
+      else if (weighted)
+        k_compact<false, true><<<nb, kBlock, 0, s>>>(et, sv, sa, bs, W, uint32_t(level), bo, adj, wt, wout);
       else
-        k_compact<false><<<unsigned(nblk), kBlock, 0, s>>>(et, seg_vofs.as<uint64_t>(),
-                                                            seg_addr.as<uint64_t>(),
-                                                            blk_seg.as<uint64_t>(), W,
-                                                            uint32_t(level), nullptr,
-                                                            c.adj.as<uint32_t>());
+        k_compact<false, false><<<nb, kBlock, 0, s>>>(et, sv, sa, bs, W, uint32_t(level), bo, adj, wt, wout);
     }
     LLG_HIP(hipGetLastError());
   } else {

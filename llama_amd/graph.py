@@ -308,6 +308,10 @@ class DeviceGraph:
                         ml.ctypes.data_as(C.POINTER(C.c_uint32)), ids.size))
                 v = self.host.level_view(d, k)
                 self.ctx.check(self.lib.ll_gpu_graph_push_level(self._h, C.byref(v)))
+            w = self.host.level_weights(k)  # the level's `weight` property, if any
+            if w is not None:
+                self.ctx.check(self.lib.ll_gpu_graph_push_weights(
+                    self._h, k, w.ctypes.data_as(C.POINTER(C.c_float)), w.size))
 
     @property
     def num_levels(self) -> int:
@@ -362,6 +366,16 @@ class DeviceGraph:
             self._h, level, iters, damping, out.ctypes.data_as(C.POINTER(C.c_double)),
             C.byref(diff)))
         return out, float(diff.value)
+
+    def sssp(self, root: int, level: int = -1, want_dist: bool = True):
+        """Weighted SSSP: (dist float32[V] or None, reached, max distance, rounds)."""
+        dist = np.zeros(self.V, np.float32) if want_dist else None
+        reached, mx, rounds = C.c_uint64(), C.c_double(), C.c_uint64()
+        self.ctx.check(self.lib.ll_gpu_sssp(
+            self._h, level, root,
+            dist.ctypes.data_as(C.POINTER(C.c_float)) if dist is not None else None,
+            C.byref(reached), C.byref(mx), C.byref(rounds)))
+        return dist, int(reached.value), float(mx.value), int(rounds.value)
 
     def bfs(self, root: int, level: int = -1, want_levels: bool = True):
         """Direction-optimising BFS: (levels or None, reached, edges examined)."""
