@@ -62,7 +62,7 @@ def bench_bfs(dg, g, E: int, V: int, args) -> dict:
     ctx.sync()
     t = time.perf_counter() - t0
     n_td, ms_td = ctx.kernel_time("k_td_expand")
-    n_bu, ms_bu = ctx.kernel_time("k_bu_step")
+    n_bu, ms_bu = ctx.kernel_time("k_bu_step")  # bottom-up launches (k_bu_rows)
     ctx.profile(False)
     levels_total = n_td + n_bu
     b_bfs = 4 * exam + 12 * reached + 2 * ((V + 7) // 8) * levels_total
@@ -87,6 +87,32 @@ def tuned_key(kind: str, dg, E: int | None, n: int) -> str:
     while b < min(n, 128):
         b <<= 1
     return f"{kind}:{dg.V}:{E}:{dg.num_levels - 1}:{b}"
+
+
+def bench_sssp(dg, g, E: int, V: int, args) -> dict:
+    """Weighted SSSP (sssp_weighted) from a few sources over the levels'
+    U[0, 1) weights; distances stay on the device."""
+    roots = pick_sources(g, args.sssp_sources, args.seed + 7)
+    dg.sssp(int(roots[0]), want_dist=False)  # warm the weighted snapshot
+    ctx = dg.ctx
+    ctx.sync()
+    ctx.profile(True)
+    reached = rounds = 0
+    t0 = time.perf_counter()
+    for r in roots:
+        _, n, _, k = dg.sssp(int(r), want_dist=False)
+        reached += n
+        rounds += k
+    ctx.sync()
+    t = time.perf_counter() - t0
+    n_rx, ms_rx = ctx.kernel_time("k_relax")
+    ctx.profile(False)
+    out = {"sources": len(roots), "seconds": round(t, 4),
+           "ms_per_source": round(t * 1e3 / len(roots), 2),
+           "gteps": round(E * len(roots) / t / 1e9, 3), "reached": reached,
+           "rounds": rounds, "relax_kernel_ms": round(ms_rx, 3)}
+    log(f"SSSP: {out}")
+    return out
 
 
 def bench_bfs_batch(dg, g, V: int, args, single: dict | None) -> dict:
@@ -261,6 +287,8 @@ def main() -> int:
     ap.add_argument("--batch-sources", type=int, default=512,
                     help="multi-source BFS sources (0 = skip)")
     ap.add_argument("--ppr-seeds", type=int, default=128, help="batched PPR seeds (0 = skip)")
+    ap.add_argument("--sssp-sources", type=int, default=8,
+                    help="weighted SSSP sources on the same graph (0 = skip)")
     ap.add_argument("--wpr-scale", type=int, default=23,
                     help="weighted PageRank + adjoint graph scale, single level (0 = skip)")
     args = ap.parse_args()
@@ -277,7 +305,8 @@ def main() -> int:
     t0 = time.time()
     log(f"building R-MAT s{args.scale} x {args.levels} levels on the host")
     g = build_multilevel(args.scale, levels=args.levels, edge_factor=args.edge_factor,
-                         ins_frac=0.01, del_frac=0.001, seed=args.seed)
+                         ins_frac=0.01, del_frac=0.001, seed=args.seed,
+                         weighted=args.sssp_sources > 0)
     log(f"host graph built in {time.time() - t0:.1f}s")
 
     ctx = GpuContext(0)
@@ -382,6 +411,8 @@ def main() -> int:
         secondary["bfs"] = bench_bfs(dg, g, E, V, args)
     if args.batch_sources > 0:
         secondary["bfs_batch"] = bench_bfs_batch(dg, g, V, args, secondary.get("bfs"))
+    if args.sssp_sources > 0:
+        secondary["sssp_weighted"] = bench_sssp(dg, g, E, V, args)
     if args.ppr_seeds > 0:
         secondary["ppr_batch"] = bench_ppr(dg, g, E, V, args)
 

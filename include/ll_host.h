@@ -98,6 +98,9 @@ const uint32_t* ll_host_edges_dst(const ll_host_edges* e);
 const float* ll_host_edges_weight(const ll_host_edges* e); /* NULL when none */
 int ll_host_graph_add_level_edges(ll_host_graph* g, const ll_host_edges* ins,
                                   const ll_host_edges* del);
+/* Give every edge a seeded U[0, 1) weight (24 random bits each; random levels
+ * of a weighted graph draw theirs the same way from the level's seed). */
+int ll_host_edges_fill_weights(ll_host_edges* e, uint64_t seed);
 
 /* --- X-Stream type-1 files: struct {u32 tail; u32 head; f32 weight} -------- */
 int ll_host_xs1_write(const char* path, const ll_host_edges* e, uint64_t num_nodes);

@@ -86,6 +86,7 @@ HOST_SIGNATURES = {
     "ll_host_graph_add_level_w": (C.c_int, [vp, P(u32), P(u32), P(f32), u64, P(u32), P(u32),
                                             u64]),
     "ll_host_graph_is_weighted": (C.c_int, [vp]),
+    "ll_host_edges_fill_weights": (C.c_int, [vp, u64]),
     "ll_host_graph_level_weights": (C.c_int, [vp, u32, P(P(f32)), P(u64)]),
     "ll_host_graph_flatten_weights": (C.c_int, [vp, C.c_int, P(P(f32)), P(u64)]),
     "ll_host_graph_add_random_level": (C.c_int, [vp, f64, f64, u64]),

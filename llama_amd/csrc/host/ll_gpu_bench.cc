@@ -38,6 +38,7 @@ const Entry kTasks[] = {
     {"gpu_pagerank", "PageRank, pull, FP32 compensated (pagerank / pagerank_double)"},
     {"gpu_pagerank_push", "PageRank, push, FP64 atomics (pagerank_push / pagerank_double_push)"},
     {"gpu_sssp_unweighted", "BFS levels from -R (sssp_unweighted)"},
+    {"gpu_sssp_weighted", "Weighted SSSP from -R over the .xs1 weights (sssp_weighted)"},
     {"gpu_bfs", "BFS reached count from -R (bfs)"},
     {"gpu_tc", "Triangle counting on the symmetrised snapshot (tc_u / tc_od)"},
     {"gpu_scc", "Strongly connected components, min-ID labels (tarjan_scc)"},
@@ -112,7 +113,9 @@ int main(int argc, char** argv) {
   ll_host_graph* hg = nullptr;
   if (scale > 0) {
     ll_host_edges* base = nullptr;
+    // seeded U[0, 1) weights, so gpu_sssp_weighted runs on generated graphs too
     if (ll_host_gen_rmat(uint32_t(scale), 16.0, 0.57, 0.19, 0.19, seed, seed + 1, &base) ||
+        ll_host_edges_fill_weights(base, seed) ||
         ll_host_graph_create(uint64_t(1) << scale, 1, &hg) ||
         ll_host_graph_add_level_edges(hg, base, nullptr)) {
       std::fprintf(stderr, "Error: %s\n", ll_host_last_error());
@@ -165,6 +168,7 @@ int main(int argc, char** argv) {
   else if (task == "gpu_pagerank_push")
     t.reset(new gpu_pagerank_push<Graph>(iters < 0 ? 10 : iters));
   else if (task == "gpu_sssp_unweighted") t.reset(new gpu_sssp_unweighted<Graph>(root));
+  else if (task == "gpu_sssp_weighted") t.reset(new gpu_sssp_weighted<Graph>(root));
   else if (task == "gpu_bfs") t.reset(new gpu_bfs<Graph>(root));
   else if (task == "gpu_tc") t.reset(new gpu_triangle_counting<Graph>());
   else if (task == "gpu_scc") t.reset(new gpu_scc<Graph>());

@@ -37,3 +37,10 @@ inline void ll_gpu_patches(const llhost::Graph& G, uint32_t dir, int level,
   ids.assign(p.edge_ids.begin(), p.edge_ids.end());
   maxlev.assign(p.maxlev.begin(), p.maxlev.end());
 }
+inline bool ll_gpu_weights(const llhost::Graph& G, int level, const float** w, uint64_t* n) {
+  if (!G.weighted()) return false;
+  const llhost::Level& lv = G.level(0, level);
+  *w = lv.wt.data();
+  *n = lv.wt.size();
+  return true;
+}

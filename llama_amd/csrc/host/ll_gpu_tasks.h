@@ -11,6 +11,8 @@
 //   void      ll_gpu_view(const Graph&, dir, level, ll_gpu_level_view*)
 //   void      ll_gpu_patches(const Graph&, dir, level, std::vector<uint64_t>& ids,
 //                            std::vector<uint32_t>& maxlev)
+//   bool      ll_gpu_weights(const Graph&, level, const float** w, uint64_t* n)
+//             (the level's `weight` property by out-edge index; false if none)
 // (overloads for the host layout are in ll_gpu_host_graph.h; INTEGRATION.md
 // shows the ones for ll_mlcsr_ro_graph).  The mirror is shared by all tasks of
 // one graph and refreshed with each newly frozen level before run().
@@ -73,6 +75,12 @@ class Mirror {
         ll_gpu_view(G, d, k, &v);
         check(ll_gpu_graph_push_level(g_, &v), "ll_gpu_graph_push_level");
       }
+    for (int k = have; k < ll_gpu_num_levels(G); k++) {
+      const float* w = nullptr;
+      uint64_t n = 0;
+      if (ll_gpu_weights(G, k, &w, &n))
+        check(ll_gpu_graph_push_weights(g_, uint32_t(k), w, n), "ll_gpu_graph_push_weights");
+    }
   }
 
  private:
@@ -184,6 +192,37 @@ class gpu_pagerank_push : public ll_gpu_task<Graph> {
   int iters_;
   double d_;
   std::vector<double> rank_;
+};
+
+// sssp_weighted (sssp.h:59-172): FP32 distances over the `weight` property,
+// INT_MAX-1 unreached; finalize = largest finite distance truncated to int
+// (LL_SSSP_RETURNS_MAX, as the reference benchmark is built).
+template <class Graph>
+class gpu_sssp_weighted : public ll_gpu_task<Graph> {
+ public:
+  explicit gpu_sssp_weighted(int64_t root) : ll_gpu_task<Graph>("GPU SSSP - Weighted"), root_(root) {}
+  void initialize(Graph* G) override {
+    ll_gpu_task<Graph>::initialize(G);
+    dist_.assign(ll_gpu_max_nodes(*G), 0.0f);
+  }
+  double run() override {
+    this->m_->check(ll_gpu_sssp(this->m_->graph(), -1, root_, dist_.data(), &reached_, &max_,
+                                nullptr),
+                    "ll_gpu_sssp");
+    return 0;
+  }
+  double finalize() override { return double(int32_t(max_)); }
+  void print_results(FILE* f) override {
+    std::vector<double> d(dist_.begin(), dist_.end());
+    print_part(f, d.data(), d.size());
+  }
+  uint64_t reached() const { return reached_; }
+
+ private:
+  int64_t root_;
+  std::vector<float> dist_;
+  uint64_t reached_ = 0;
+  double max_ = 0;
 };
 
 // sssp_unweighted (sssp.h:220-333): dist = BFS level, INT_MAX-1 unreached;

@@ -25,6 +25,18 @@ struct ll_host_graph {
   llhost::vec<float> flat_w;
 };
 
+namespace {
+
+// Seeded U[0, 1) weights with 24 random bits each (exact in float).
+void uniform_weights(uint64_t n, uint64_t seed, llhost::vec<float>& w) {
+  w.resize(n);
+#pragma omp parallel for schedule(static)
+  for (uint64_t i = 0; i < n; i++)
+    w[i] = float(llhost::rng_u64(seed, 0x5757, i) >> 40) * (1.0f / 16777216.0f);
+}
+
+}  // namespace
+
 namespace llhost {
 const Graph& graph_of(const ll_host_graph* h) { return *h->g; }
 }  // namespace llhost
@@ -105,13 +117,15 @@ int ll_host_graph_add_random_level(ll_host_graph* g, double ins_frac, double del
     if (!g->upd) g->upd.reset(new llhost::Updater(*g->g));
     llhost::EdgeList ins, del;
     g->upd->next(ins_frac, del_frac, seed, ins, del);
+    llhost::vec<float> w;
+    if (g->g->weighted()) uniform_weights(ins.src.size(), seed, w);  // U[0, 1), seeded
     if (!g->g->add_level(ins.src.data(), ins.dst.data(), ins.src.size(), del.src.data(),
-                         del.dst.data(), del.src.size())) {
+                         del.dst.data(), del.src.size(), w.empty() ? nullptr : w.data())) {
       g->upd.reset();
       return fail(LL_GPU_ESTATE, g->g->last_error());
     }
     g->last_ins.e = std::move(ins);
-    g->last_ins.w.clear();
+    g->last_ins.w = std::move(w);
     g->last_del.e = std::move(del);
     g->last_del.w.clear();
   } catch (const std::bad_alloc&) {
@@ -144,6 +158,16 @@ int ll_host_graph_level_view(const ll_host_graph* g, uint32_t dir, uint32_t leve
   out->et = lv.et.data();
   out->flags = LL_GPU_HAS_CONT | (g->g->deletions() ? LL_GPU_HAS_DELETIONS : 0u);
   out->page_table = nullptr;
+  return LL_GPU_OK;
+}
+
+int ll_host_edges_fill_weights(ll_host_edges* e, uint64_t seed) {
+  if (!e) return fail(LL_GPU_EINVAL, "edges are NULL");
+  try {
+    uniform_weights(e->e.src.size(), seed, e->w);
+  } catch (const std::bad_alloc&) {
+    return fail(LL_GPU_ENOMEM, "out of host memory for weights");
+  }
   return LL_GPU_OK;
 }
 

@@ -77,6 +77,11 @@ class Edges:
         d = np.ctypeslib.as_array(N.host().ll_host_edges_dst(self._h), shape=(n,))
         return s.copy(), d.copy()
 
+    def fill_weights(self, seed: int) -> "Edges":
+        """Seeded U[0, 1) weight per edge (in place)."""
+        N.check_host(N.host().ll_host_edges_fill_weights(self._h, seed))
+        return self
+
     def write_xs1(self, path: str, num_nodes: int) -> None:
         N.check_host(N.host().ll_host_xs1_write(path.encode(), self._h, num_nodes))
 
@@ -218,10 +223,14 @@ class HostGraph:
 
 def build_multilevel(scale: int, levels: int = 1, edge_factor: float = 16.0,
                      ins_frac: float = 0.01, del_frac: float = 0.001, seed: int = 1,
-                     deletions: bool = True, rmat=(0.57, 0.19, 0.19)) -> HostGraph:
+                     deletions: bool = True, rmat=(0.57, 0.19, 0.19),
+                     weighted: bool = False) -> HostGraph:
     """R-MAT base level (seed, seed+1 permutation) plus `levels-1` random levels
-    (seed+2+k), the multi-level snapshot configuration."""
+    (seed+2+k), the multi-level snapshot configuration; `weighted` gives every
+    edge a seeded U[0, 1) weight (the levels' `weight` property)."""
     base = Edges.rmat(scale, edge_factor, *rmat, seed=seed, perm_seed=seed + 1)
+    if weighted:
+        base.fill_weights(seed)
     g = HostGraph(1 << scale, deletions=deletions)
     g.add_level_edges(base)
     del base

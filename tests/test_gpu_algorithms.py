@@ -245,3 +245,26 @@ def test_cpp_task_driver_matches_oracle(tmp_path):
     lv, _ = oracle.OracleGraph.from_host(g).bfs(3)
     ret2 = [l for l in r2.stdout.splitlines() if l.startswith("Return value:")][0]
     assert int(float(ret2.split(":")[1])) == int(lv[lv < 2**31 - 2].max())
+    # weighted SSSP over the .xs1 records' weights
+    rng = np.random.default_rng(6)
+    w0 = rng.uniform(0.1, 2.0, s.size).astype(np.float32)
+    fw0, fw1 = str(tmp_path / "w0.xs1"), str(tmp_path / "w1.xs1")
+    Edges.from_arrays(s, d, w0).write_xs1(fw0, 256)
+    Edges.from_arrays([1, 2, 3], [200, 100, 50], np.float32([0.5, 0.25, 4.0])).write_xs1(fw1, 256)
+    out3 = str(tmp_path / "sssp.txt")
+    r3 = subprocess.run([str(exe), "-r", "gpu_sssp_weighted", "-R", "3", "-o", out3, fw0, fw1],
+                        capture_output=True, text=True, timeout=120)
+    assert r3.returncode == 0, r3.stderr
+    gw = HostGraph(256)
+    gw.add_level(s, d, weights=w0)
+    gw.add_level([1, 2, 3], [200, 100, 50], weights=[0.5, 0.25, 4.0])
+    ow = oracle.OracleGraph.from_host(gw)
+    ow.add_weights(gw)
+    dist, _, mx, _ = ow.sssp_weighted(3)
+    vals = []
+    for line in open(out3):
+        if ":" in line:
+            vals += [float(x) for x in line.split(":")[1].split()]
+    assert np.array_equal(np.array(vals, np.float64).astype(np.float32), dist[:50])
+    ret3 = [l for l in r3.stdout.splitlines() if l.startswith("Return value:")][0]
+    assert int(float(ret3.split(":")[1])) == int(mx)
