@@ -106,6 +106,22 @@ int ll_host_edges_fill_weights(ll_host_edges* e, uint64_t seed);
 int ll_host_xs1_write(const char* path, const ll_host_edges* e, uint64_t num_nodes);
 int ll_host_xs1_read(const char* path, ll_host_edges** out, uint64_t* num_nodes);
 
+/* --- SNAP .net text edge lists ------------------------------------------------ */
+/* Replaces: ll_loader_net::net_loader (llama/include/llama/loaders/ll_load_net.h:
+ * 57-375): "tail head" lines, '#'/blank lines skipped, up to 99 malformed lines
+ * skipped and counted (the 100th fails), part/nparts > 0 loads the byte range
+ * of part `part` (1-based) the same way; num_nodes = largest ID + 1. */
+int ll_host_net_read(const char* path, int part, int nparts, ll_host_edges** out,
+                     uint64_t* num_nodes, uint64_t* parse_errors);
+int ll_host_net_write(const char* path, const ll_host_edges* e);
+
+/* Replaces: utils/xs1-split.cpp: records to part 0 with probability p_large
+ * (the tool's 0.8), else to one of n_small parts, with the tool's per-part
+ * quotas; `undirected` keeps (u,v),(v,u) record pairs together.  parts must
+ * hold n_small + 1 pointers; each part is owned by the caller. */
+int ll_host_edges_split(const ll_host_edges* in, int n_small, double p_large, int undirected,
+                        uint64_t seed, ll_host_edges** parts);
+
 #ifdef __cplusplus
 }
 #endif

@@ -87,6 +87,9 @@ HOST_SIGNATURES = {
                                             u64]),
     "ll_host_graph_is_weighted": (C.c_int, [vp]),
     "ll_host_edges_fill_weights": (C.c_int, [vp, u64]),
+    "ll_host_net_read": (C.c_int, [cp, C.c_int, C.c_int, P(vp), P(u64), P(u64)]),
+    "ll_host_net_write": (C.c_int, [cp, vp]),
+    "ll_host_edges_split": (C.c_int, [vp, C.c_int, f64, C.c_int, u64, P(vp)]),
     "ll_host_graph_level_weights": (C.c_int, [vp, u32, P(P(f32)), P(u64)]),
     "ll_host_graph_flatten_weights": (C.c_int, [vp, C.c_int, P(P(f32)), P(u64)]),
     "ll_host_graph_add_random_level": (C.c_int, [vp, f64, f64, u64]),

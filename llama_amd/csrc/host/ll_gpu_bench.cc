@@ -50,7 +50,7 @@ void usage() {
   std::fprintf(stderr,
                "Usage: ll_gpu_bench -r TASK [-n ITERS] [-R ROOT] [-c COUNT] [-o FILE]\n"
                "                    [--sources N] [--lanes W] [--tol T]\n"
-               "                    (FILE.xs1 ... | --rmat SCALE [--levels L] [--seed S])\n"
+               "                    (FILE.xs1|FILE.net ... | --rmat SCALE [--levels L] [--seed S])\n"
                "Tasks:\n");
   for (const Entry& e : kTasks) std::fprintf(stderr, "  %-22s %s\n", e.id, e.name);
 }
@@ -137,7 +137,15 @@ int main(int argc, char** argv) {
     for (int i = optind; i < argc; i++) {
       ll_host_edges* e = nullptr;
       uint64_t n = 0;
-      if (ll_host_xs1_read(argv[i], &e, &n)) {
+      const std::string path = argv[i];
+      const auto ends = [&](const char* x) {
+        const size_t l = std::strlen(x);
+        return path.size() >= l && path.compare(path.size() - l, l, x) == 0;
+      };
+      const int rc = ends(".net") || ends(".snap")
+                         ? ll_host_net_read(argv[i], 0, 0, &e, &n, nullptr)
+                         : ll_host_xs1_read(argv[i], &e, &n);
+      if (rc) {
         std::fprintf(stderr, "Error: %s\n", ll_host_last_error());
         return 1;
       }

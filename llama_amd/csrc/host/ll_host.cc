@@ -1,7 +1,12 @@
 // C ABI over the host level builder, generator and .xs1 files (ll_host.h).
 #include "ll_host.h"
 
+#include <sys/types.h>
+
 #include <algorithm>
+#include <cctype>
+#include <cerrno>
+#include <climits>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -332,3 +337,164 @@ int ll_host_xs1_read(const char* path, ll_host_edges** out, uint64_t* num_nodes)
 }
 
 }  // extern "C"
+
+// ---- SNAP .net text edge lists --------------------------------------------------
+// Restates the reference loader's line rules (llama/include/llama/loaders/
+// ll_load_net.h:312-375): lines starting with '#', '\r', '\n' or NUL are
+// skipped; a record is "<ws>*tail<ws>+head" followed by end of line; anything
+// else is a parse error, and the 100th error fails the load.  A partial load
+// of part k of N (1-based) covers the byte range [size*(k-1)/N, size*k/N): it
+// starts after the first newline at or past the range start (unless that is
+// byte 0) and stops after the line that reaches the range end (:180-300).
+// The reference tests the range end only after a record, so a record that
+// follows a skipped line reaching the end is loaded by two parts; here every
+// record lands in exactly one part.  IDs must fit in 32 bits (the reference
+// reads them with strtol into unsigned).
+
+namespace {
+
+bool net_parse(const char* line, uint32_t* tail, uint32_t* head) {
+  const char* s = line;
+  char* e = nullptr;
+  while (std::isspace(static_cast<unsigned char>(*s))) s++;
+  errno = 0;
+  const long t = std::strtol(s, &e, 10);
+  if (s == e || *e == '\0' || !std::isspace(static_cast<unsigned char>(*e))) return false;
+  s = e + 1;
+  while (std::isspace(static_cast<unsigned char>(*s))) s++;
+  const long h = std::strtol(s, &e, 10);
+  if (s == e || (*e != '\0' && *e != '\r' && *e != '\n')) return false;
+  if (errno || t < 0 || h < 0 || t > long(UINT32_MAX) || h > long(UINT32_MAX)) return false;
+  *tail = uint32_t(t);
+  *head = uint32_t(h);
+  return true;
+}
+
+}  // namespace
+
+int ll_host_net_read(const char* path, int part, int nparts, ll_host_edges** out,
+                     uint64_t* num_nodes, uint64_t* parse_errors) {
+  if (!path || !out) return fail(LL_GPU_EINVAL, "NULL argument");
+  if (nparts < 0 || (nparts > 0 && (part < 1 || part > nparts)))
+    return fail(LL_GPU_EINVAL, "the partial load part ID is out of bounds");
+  FILE* f = std::fopen(path, "rb");
+  if (!f) return fail(LL_GPU_EINVAL, std::string("cannot open ") + path);
+  std::fseek(f, 0, SEEK_END);
+  const long size = std::ftell(f);
+  std::fseek(f, 0, SEEK_SET);
+  long start = 0, stop = 0;  // stop 0 = to the end of the file
+  if (nparts > 0) {
+    start = long(uint64_t(size) * uint64_t(part - 1) / uint64_t(nparts));
+    stop = long(uint64_t(size) * uint64_t(part) / uint64_t(nparts));
+  }
+  long offset = 0;
+  if (start > 0) {  // resume after the line that straddles the range start
+    std::fseek(f, start - 1, SEEK_SET);
+    offset = start - 1;
+    int c;
+    while ((c = std::fgetc(f)) != EOF) {
+      offset++;
+      if (c == '\n') break;
+    }
+  }
+  auto* e = new (std::nothrow) ll_host_edges();
+  if (!e) {
+    std::fclose(f);
+    return fail(LL_GPU_ENOMEM, "out of host memory");
+  }
+  char* line = nullptr;
+  size_t cap = 0;
+  ssize_t r;
+  uint64_t errors = 0, vmax = 0;
+  bool done = offset >= size;
+  while (!done && (r = getline(&line, &cap, f)) != -1) {
+    offset += r;
+    // every consumed line counts towards the range end, skipped ones too
+    if (stop > 0 && offset >= stop) done = true;
+    const char c0 = line[0];
+    if (c0 == '\0' || c0 == '#' || c0 == '\n' || c0 == '\r') continue;
+    uint32_t t, h;
+    if (!net_parse(line, &t, &h)) {
+      if (++errors >= 100) {
+        std::free(line);
+        std::fclose(f);
+        delete e;
+        return fail(LL_GPU_EINVAL, "reached the limit of 100 parse errors");
+      }
+      continue;
+    }
+    e->e.src.push_back(t);
+    e->e.dst.push_back(h);
+    vmax = std::max<uint64_t>(vmax, std::max(t, h) + uint64_t(1));
+  }
+  std::free(line);
+  std::fclose(f);
+  if (num_nodes) *num_nodes = vmax;
+  if (parse_errors) *parse_errors = errors;
+  *out = e;
+  return LL_GPU_OK;
+}
+
+int ll_host_net_write(const char* path, const ll_host_edges* e) {
+  if (!path || !e) return fail(LL_GPU_EINVAL, "NULL argument");
+  FILE* f = std::fopen(path, "w");
+  if (!f) return fail(LL_GPU_EINVAL, std::string("cannot create ") + path);
+  std::fprintf(f, "# Directed graph: %lu edges\n# FromNodeId\tToNodeId\n",
+               (unsigned long)e->e.src.size());
+  for (size_t i = 0; i < e->e.src.size(); i++)
+    std::fprintf(f, "%u\t%u\n", e->e.src[i], e->e.dst[i]);
+  const bool ok = std::fclose(f) == 0;
+  return ok ? LL_GPU_OK : fail(LL_GPU_EINVAL, "write failed");
+}
+
+// ---- base + N splits of an edge list (utils/xs1-split.cpp) ----------------------
+// Each record (each tail/head pair of records with `undirected`) goes to part 0
+// with probability p_large, else to small part 1 + floor((r - p_large) /
+// p_small), p_small = (1 - p_large) / n_small; a part that already holds its
+// quota (floor(p * records), part 0 takes the remainder) passes the record to
+// the next part with room.  r comes from a seeded generator instead of rand().
+int ll_host_edges_split(const ll_host_edges* in, int n_small, double p_large, int undirected,
+                        uint64_t seed, ll_host_edges** parts) {
+  if (!in || !parts) return fail(LL_GPU_EINVAL, "NULL argument");
+  if (n_small <= 0) return fail(LL_GPU_EINVAL, "invalid number of small files");
+  if (!(p_large >= 0 && p_large <= 1)) return fail(LL_GPU_EINVAL, "p_large must be in [0, 1]");
+  const uint64_t n = in->e.src.size();
+  const bool w = in->w.size() == n;
+  if (undirected && n % 2) return fail(LL_GPU_EINVAL, "odd number of edges");
+  const double p_small = (1.0 - p_large) / n_small;
+  std::vector<uint64_t> expect(n_small + 1), have(n_small + 1, 0);
+  uint64_t sum = expect[0] = uint64_t(p_large * double(n));
+  for (int i = 1; i <= n_small; i++) sum += expect[i] = uint64_t(p_small * double(n));
+  if (sum > n) return fail(LL_GPU_ESTATE, "split quotas exceed the records");
+  expect[0] += n - sum;
+  std::vector<ll_host_edges*> out(n_small + 1, nullptr);
+  for (auto& p : out) {
+    p = new (std::nothrow) ll_host_edges();
+    if (!p) {
+      for (auto* q : out) delete q;
+      return fail(LL_GPU_ENOMEM, "out of host memory");
+    }
+  }
+  const uint64_t step = undirected ? 2 : 1;
+  for (uint64_t i = 0; i < n; i += step) {
+    if (undirected && (in->e.src[i + 1] != in->e.dst[i] || in->e.dst[i + 1] != in->e.src[i])) {
+      for (auto* q : out) delete q;
+      return fail(LL_GPU_EINVAL, "not an undirected edge pair");
+    }
+    const double r = double(llhost::rng_u64(seed, 0x5311, i) >> 11) * (1.0 / 9007199254740992.0);
+    int t = r < p_large ? 0 : 1 + int((r - p_large) / p_small);
+    if (t > n_small) t = n_small;
+    for (int k = 0; k <= n_small; k++) {
+      if (have[t] < expect[t]) break;
+      t = (t + 1) % (n_small + 1);
+    }
+    for (uint64_t j = i; j < i + step; j++) {
+      out[t]->e.src.push_back(in->e.src[j]);
+      out[t]->e.dst.push_back(in->e.dst[j]);
+      if (w) out[t]->w.push_back(in->w[j]);
+      have[t]++;
+    }
+  }
+  for (int i = 0; i <= n_small; i++) parts[i] = out[i];
+  return LL_GPU_OK;
+}

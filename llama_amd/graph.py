@@ -82,6 +82,25 @@ class Edges:
         N.check_host(N.host().ll_host_edges_fill_weights(self._h, seed))
         return self
 
+    @classmethod
+    def read_net(cls, path: str, part: int = 0, nparts: int = 0) -> tuple["Edges", int, int]:
+        """SNAP .net edge list: (edges, num_nodes, parse errors skipped)."""
+        h, nn, errs = C.c_void_p(), C.c_uint64(), C.c_uint64()
+        N.check_host(N.host().ll_host_net_read(path.encode(), part, nparts, C.byref(h),
+                                               C.byref(nn), C.byref(errs)))
+        return cls(h.value), int(nn.value), int(errs.value)
+
+    def write_net(self, path: str) -> None:
+        N.check_host(N.host().ll_host_net_write(path.encode(), self._h))
+
+    def split(self, n_small: int = 10, p_large: float = 0.8, undirected: bool = False,
+              seed: int = 1) -> list["Edges"]:
+        """Base + n_small parts the way utils/xs1-split.cpp cuts a file."""
+        arr = (C.c_void_p * (n_small + 1))()
+        N.check_host(N.host().ll_host_edges_split(self._h, n_small, p_large, int(undirected),
+                                                  seed, arr))
+        return [Edges(p) for p in arr]
+
     def write_xs1(self, path: str, num_nodes: int) -> None:
         N.check_host(N.host().ll_host_xs1_write(path.encode(), self._h, num_nodes))
 
