@@ -58,6 +58,7 @@ def lib() -> C.CDLL:
                                           f64, P(f64), P(f64)]),
             "orc_graph_add_weights": (C.c_int, [vp, u32, P(C.c_float), u64]),
             "orc_sssp_weighted": (i64, [vp, C.c_int, i64, P(C.c_float), P(f64), P(i64)]),
+            "orc_bc": (C.c_int, [vp, C.c_int, P(i64), i64, P(C.c_float)]),
             "orc_num_threads": (C.c_int, []),
         }
         for k, (r, a) in sig.items():
@@ -142,6 +143,19 @@ class OracleGraph:
         if n < 0:
             raise ValueError("sssp_weighted: bad level/root or unweighted graph")
         return dist, int(n), float(mx.value), int(rounds.value)
+
+    def bc(self, sources=None, level: int = -1) -> np.ndarray:
+        """Betweenness centrality in FP32 (bc_adj when sources is None, else the
+        given sources as bc_random uses its random picks)."""
+        out = np.zeros(self.V, np.float32)
+        if sources is None:
+            rc = lib().orc_bc(self._h, level, None, -1, _p(out, C.c_float))
+        else:
+            src = np.ascontiguousarray(sources, dtype=np.int64)
+            rc = lib().orc_bc(self._h, level, _p(src, C.c_int64), src.size, _p(out, C.c_float))
+        if rc != 0:
+            raise ValueError("bc: bad level or source")
+        return out
 
     def bfs(self, root: int, level: int = -1):
         dist = np.zeros(self.V, np.int32)

@@ -23,6 +23,9 @@
 //                     synchronous rounds, dist_nxt[s] = min(dist[n] + weight[e]) in
 //                     FP32 over the out-edges e of updated vertices, weights read
 //                     by edge ID from per-level arrays (the `weight` property)
+//   bc                ll_b_bc_adj / ll_b_bc_random (benchmark/benchmarks/bc_adj.h,
+//                     bc_random.h): Brandes in FP32, sigma along down-edges of the
+//                     BFS from each source, delta summed in iterator order
 //   ppr, wpr, adjoint FP64 restatements of the definitions in ll_gpu.h (no
 //                     reference counterpart)
 //
@@ -555,6 +558,58 @@ int64_t orc_sssp_weighted(const orc_graph* g, int level, int64_t root, float* di
   if (max_dist) *max_dist = mx;
   if (rounds) *rounds = r;
   return reached;
+}
+
+// bc_adj (sources == NULL: every vertex, ascending) / bc_random (the given
+// sources): per source a BFS over out-edges, sigma[w] += sigma[v] along
+// down-edges (level[w] = level[v] + 1) in BFS order (visit_fw), then in
+// reverse BFS order delta[v] = sum over v's down-edges in iterator order of
+// sigma[v] / sigma[w] * (1 + delta[w]) and BC[v] += delta[v] for v != source
+// (visit_rv).  All arithmetic in FP32 as the generated task code does.
+int orc_bc(const orc_graph* g, int level, const int64_t* sources, int64_t n, float* bc) {
+  level = resolve(g, level);
+  if (level < 0) return -1;
+  const uint64_t V = g->V;
+  const int64_t ns = sources ? n : int64_t(V);
+  std::vector<float> sigma(V), delta(V, 0.0f);
+  std::vector<int32_t> lv(V);
+  std::vector<uint64_t> order;
+  order.reserve(V);
+  for (uint64_t v = 0; v < V; v++) bc[v] = 0.0f;
+  for (int64_t i = 0; i < ns; i++) {
+    const uint64_t s = sources ? uint64_t(sources[i]) : uint64_t(i);
+    if (s >= V) return -2;
+    std::fill(lv.begin(), lv.end(), -1);
+    std::fill(sigma.begin(), sigma.end(), 0.0f);
+    order.clear();
+    lv[s] = 0;
+    sigma[s] = 1.0f;
+    order.push_back(s);
+    for (size_t h = 0; h < order.size(); h++) {
+      const uint64_t v = order[h];
+      foreach_nbr(g, 0, level, v, [&](uint64_t w) {
+        if (lv[w] < 0) {
+          lv[w] = lv[v] + 1;
+          order.push_back(w);
+        }
+      });
+    }
+    for (uint64_t v : order)
+      foreach_nbr(g, 0, level, v, [&](uint64_t w) {
+        if (lv[w] == lv[v] + 1) sigma[w] += sigma[v];
+      });
+    for (size_t h = order.size(); h-- > 0;) {
+      const uint64_t v = order[h];
+      if (v == s) continue;
+      float S = 0.0f;
+      foreach_nbr(g, 0, level, v, [&](uint64_t w) {
+        if (lv[w] == lv[v] + 1) S = S + sigma[v] / sigma[w] * (1 + delta[w]);
+      });
+      delta[v] = S;
+      bc[v] = bc[v] + delta[v];
+    }
+  }
+  return 0;
 }
 
 int orc_num_threads(void) { return omp_get_max_threads(); }
