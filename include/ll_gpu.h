@@ -145,6 +145,17 @@ int ll_gpu_graph_push_weights(ll_gpu_graph* g, uint32_t level, const float* w, u
  * int there); rounds = frontier rounds run.  All may be NULL. */
 int ll_gpu_sssp(ll_gpu_graph* g, int level, int64_t root, float* dist_out, uint64_t* reached,
                 double* max_dist, uint64_t* rounds);
+/* Replaces: tasks `bc_adj` = ll_b_bc_adj::run (benchmark/benchmarks/bc_adj.h:
+ * 124-181, every vertex a source) and `bc_random` = ll_b_bc_random::run
+ * (bc_random.h:158-193, K sources from pick_random_node): bc_out[max_nodes] =
+ * FP32 betweenness centrality (Brandes) over the snapshot's out-edges from
+ * sources[0..n) (sources NULL = every vertex, n ignored), summed over sources
+ * in the given order.  Path counts and dependencies are FP32 as in the
+ * reference; a vertex's dependency sums its next-level out-neighbours' terms
+ * in iterator order, so results match the reference bit for bit while path
+ * counts stay below 2^24 and no vertex has more than 1024 out-edges (longer
+ * rows sum in parallel: FP32 reassociation). */
+int ll_gpu_bc(ll_gpu_graph* g, int level, const int64_t* sources, int64_t n, float* bc_out);
 /* Replaces: task `sssp_unweighted` (benchmark/benchmarks/sssp.h:220-333) and
  * `bfs` (bfs.h:85-128): levels_out[max_nodes] (may be NULL) gets the BFS level,
  * INT_MAX-1 for unreached; reached = number of reached vertices (bfs_count);

@@ -58,6 +58,7 @@ GPU_SIGNATURES = {
     "ll_gpu_pagerank_push": (C.c_int, [vp, C.c_int, C.c_int, f64, P(f64), P(f64)]),
     "ll_gpu_graph_push_weights": (C.c_int, [vp, u32, P(f32), u64]),
     "ll_gpu_sssp": (C.c_int, [vp, C.c_int, i64, P(f32), P(u64), P(f64), P(u64)]),
+    "ll_gpu_bc": (C.c_int, [vp, C.c_int, P(i64), i64, P(f32)]),
     "ll_gpu_bfs": (C.c_int, [vp, C.c_int, i64, P(i32), P(u64), P(u64)]),
     "ll_gpu_bfs_ex": (C.c_int, [vp, C.c_int, i64, P(i32), P(u64), P(u64), P(u64)]),
     "ll_gpu_bfs_batch_ex": (C.c_int, [vp, C.c_int, P(i64), C.c_int, C.c_int, P(i32), P(u64),

@@ -40,6 +40,8 @@ const Entry kTasks[] = {
     {"gpu_sssp_unweighted", "BFS levels from -R (sssp_unweighted)"},
     {"gpu_sssp_weighted", "Weighted SSSP from -R over the .xs1 weights (sssp_weighted)"},
     {"gpu_bfs", "BFS reached count from -R (bfs)"},
+    {"gpu_bc_adj", "Betweenness centrality from every vertex (bc_adj)"},
+    {"gpu_bc_random", "Betweenness centrality from --sources K random sources (bc_random)"},
     {"gpu_tc", "Triangle counting on the symmetrised snapshot (tc_u / tc_od)"},
     {"gpu_scc", "Strongly connected components, min-ID labels (tarjan_scc)"},
     {"gpu_bfs_batch", "Multi-source BFS, --sources N roots, --lanes W"},
@@ -65,6 +67,7 @@ double now_ms() {
 int main(int argc, char** argv) {
   std::string task;
   int iters = -1, count = 1, sources = 64, lanes = 0, scale = 0, levels = 1;
+  bool sources_set = false;
   long root = 0;
   double tol = 1e-7;
   unsigned long seed = 1;
@@ -90,7 +93,7 @@ int main(int argc, char** argv) {
       case 'R': root = std::atol(optarg); break;
       case 'c': count = std::atoi(optarg); break;
       case 'o': out_file = optarg; break;
-      case 1: sources = std::atoi(optarg); break;
+      case 1: sources = std::atoi(optarg), sources_set = true; break;
       case 2: lanes = std::atoi(optarg); break;
       case 3: scale = std::atoi(optarg); break;
       case 4: levels = std::atoi(optarg); break;
@@ -178,6 +181,8 @@ int main(int argc, char** argv) {
   else if (task == "gpu_sssp_unweighted") t.reset(new gpu_sssp_unweighted<Graph>(root));
   else if (task == "gpu_sssp_weighted") t.reset(new gpu_sssp_weighted<Graph>(root));
   else if (task == "gpu_bfs") t.reset(new gpu_bfs<Graph>(root));
+  else if (task == "gpu_bc_adj") t.reset(new gpu_bc<Graph>());
+  else if (task == "gpu_bc_random") t.reset(new gpu_bc<Graph>(sources_set ? sources : 100, seed));
   else if (task == "gpu_tc") t.reset(new gpu_triangle_counting<Graph>());
   else if (task == "gpu_scc") t.reset(new gpu_scc<Graph>());
   else if (task == "gpu_bfs_batch") t.reset(new gpu_bfs_batch<Graph>(roots, lanes));

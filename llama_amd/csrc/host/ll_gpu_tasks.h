@@ -13,8 +13,8 @@
 //                            std::vector<uint32_t>& maxlev)
 //   bool      ll_gpu_weights(const Graph&, level, const float** w, uint64_t* n)
 //             (the level's `weight` property by out-edge index; false if none)
-// (overloads for the host layout are in ll_gpu_host_graph.h; INTEGRATION.md
-// shows the ones for ll_mlcsr_ro_graph).  The mirror is shared by all tasks of
+// (overloads for the host layout are in ll_gpu_host_graph.h; those for
+// ll_mlcsr_ro_graph are a few lines over its public accessors).  The mirror is shared by all tasks of
 // one graph and refreshed with each newly frozen level before run().
 //
 // Errors follow the reference convention: a failing call prints a message and
@@ -223,6 +223,56 @@ class gpu_sssp_weighted : public ll_gpu_task<Graph> {
   std::vector<float> dist_;
   uint64_t reached_ = 0;
   double max_ = 0;
+};
+
+// bc_adj (bc_adj.h:124-181) / bc_random (bc_random.h:158-193): FP32
+// betweenness centrality from every vertex, or from K sources picked with a
+// seeded generator (the reference's pick_random_node draws from its own
+// generator, so the picks differ); finalize = the largest value (:201-209).
+template <class Graph>
+class gpu_bc : public ll_gpu_task<Graph> {
+ public:
+  gpu_bc() : ll_gpu_task<Graph>("GPU Betweenness centrality - exact"), k_(-1) {}
+  gpu_bc(int k, uint64_t seed)
+      : ll_gpu_task<Graph>("GPU Betweenness centrality - randomized"), k_(k), seed_(seed) {}
+  void initialize(Graph* G) override {
+    ll_gpu_task<Graph>::initialize(G);
+    const uint64_t V = ll_gpu_max_nodes(*G);
+    bc_.assign(V, 0.0f);
+    sources_.clear();
+    for (int i = 0; i < k_ && V > 0; i++)
+      sources_.push_back(int64_t(mix64(seed_ * 0x9e3779b97f4a7c15ull + uint64_t(i)) % V));
+  }
+  double run() override {
+    this->m_->check(ll_gpu_bc(this->m_->graph(), -1, k_ < 0 ? nullptr : sources_.data(),
+                              int64_t(sources_.size()), bc_.data()),
+                    "ll_gpu_bc");
+    return 0;
+  }
+  double finalize() override {
+    float mx = 0;
+    for (float x : bc_)
+      if (x > mx) mx = x;
+    return mx;
+  }
+  void print_results(FILE* f) override {
+    std::vector<double> d(bc_.begin(), bc_.end());
+    print_part(f, d.data(), d.size());
+  }
+  const std::vector<int64_t>& sources() const { return sources_; }
+
+ private:
+  // splitmix64 finaliser (Steele, Lea, Flood 2014)
+  static uint64_t mix64(uint64_t z) {
+    z += 0x9e3779b97f4a7c15ull;
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+  }
+  int k_;
+  uint64_t seed_ = 1;
+  std::vector<int64_t> sources_;
+  std::vector<float> bc_;
 };
 
 // sssp_unweighted (sssp.h:220-333): dist = BFS level, INT_MAX-1 unreached;

@@ -405,6 +405,22 @@ class DeviceGraph:
             C.byref(reached), C.byref(mx), C.byref(rounds)))
         return dist, int(reached.value), float(mx.value), int(rounds.value)
 
+    def bc(self, sources=None, level: int = -1) -> np.ndarray:
+        """Betweenness centrality, FP32[V]: every vertex a source (bc_adj) when
+        sources is None, else the given sources in order (bc_random)."""
+        out = np.zeros(self.V, np.float32)
+        if sources is None:
+            src, n = None, 0
+        else:
+            src = np.ascontiguousarray(sources, dtype=np.int64)
+            n = src.size
+            if n == 0:
+                return out
+        self.ctx.check(self.lib.ll_gpu_bc(
+            self._h, level, src.ctypes.data_as(C.POINTER(C.c_int64)) if src is not None else None,
+            n, out.ctypes.data_as(C.POINTER(C.c_float))))
+        return out
+
     def bfs(self, root: int, level: int = -1, want_levels: bool = True):
         """Direction-optimising BFS: (levels or None, reached, edges examined)."""
         lv = np.zeros(self.V, np.int32) if want_levels else None

@@ -268,3 +268,16 @@ def test_cpp_task_driver_matches_oracle(tmp_path):
     assert np.array_equal(np.array(vals, np.float64).astype(np.float32), dist[:50])
     ret3 = [l for l in r3.stdout.splitlines() if l.startswith("Return value:")][0]
     assert int(float(ret3.split(":")[1])) == int(mx)
+    # betweenness centrality from every vertex
+    out4 = str(tmp_path / "bc.txt")
+    r4 = subprocess.run([str(exe), "-r", "gpu_bc_adj", "-o", out4, f0, f1],
+                        capture_output=True, text=True, timeout=120)
+    assert r4.returncode == 0, r4.stderr
+    bc = oracle.OracleGraph.from_host(g).bc()
+    vals = []
+    for line in open(out4):
+        if ":" in line:
+            vals += [float(x) for x in line.split(":")[1].split()]
+    assert np.allclose(vals, bc[:50], rtol=1e-6)
+    ret4 = [l for l in r4.stdout.splitlines() if l.startswith("Return value:")][0]
+    assert np.isclose(float(ret4.split(":")[1]), float(bc.max()), rtol=1e-6)
