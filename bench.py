@@ -115,6 +115,29 @@ def bench_sssp(dg, g, E: int, V: int, args) -> dict:
     return out
 
 
+def bench_bc(dg, g, E: int, V: int, args) -> dict:
+    """Betweenness centrality (bc_random's Brandes passes) from a few sources;
+    per source a BFS, a path-count pass and a dependency pass over the edges."""
+    roots = pick_sources(g, args.bc_sources, args.seed + 8)
+    dg.bc([int(roots[0])])  # warm the snapshot and row blocks
+    ctx = dg.ctx
+    ctx.sync()
+    ctx.profile(True)
+    t0 = time.perf_counter()
+    b = dg.bc([int(r) for r in roots])
+    t = time.perf_counter() - t0
+    n_p, ms_p = ctx.kernel_time("k_bc_pass")
+    n_b, ms_b = ctx.kernel_time("k_bu_step")
+    ctx.profile(False)
+    out = {"sources": len(roots), "seconds": round(t, 4),
+           "ms_per_source": round(t * 1e3 / len(roots), 2),
+           "gteps": round(E * len(roots) / t / 1e9, 3),
+           "pass_launches": n_p, "pass_kernel_ms": round(ms_p, 3),
+           "bfs_bottom_up_ms": round(ms_b, 3), "max_bc": float(b.max())}
+    log(f"BC: {out}")
+    return out
+
+
 def bench_bfs_batch(dg, g, V: int, args, single: dict | None) -> dict:
     """Multi-source BFS over the batch sources in 64-bit lanes (and the
     autotuned width), against the per-source cost of single-source runs."""
@@ -289,6 +312,8 @@ def main() -> int:
     ap.add_argument("--ppr-seeds", type=int, default=128, help="batched PPR seeds (0 = skip)")
     ap.add_argument("--sssp-sources", type=int, default=8,
                     help="weighted SSSP sources on the same graph (0 = skip)")
+    ap.add_argument("--bc-sources", type=int, default=8,
+                    help="betweenness centrality sources on the same graph (0 = skip)")
     ap.add_argument("--wpr-scale", type=int, default=23,
                     help="weighted PageRank + adjoint graph scale, single level (0 = skip)")
     args = ap.parse_args()
@@ -413,6 +438,8 @@ def main() -> int:
         secondary["bfs_batch"] = bench_bfs_batch(dg, g, V, args, secondary.get("bfs"))
     if args.sssp_sources > 0:
         secondary["sssp_weighted"] = bench_sssp(dg, g, E, V, args)
+    if args.bc_sources > 0:
+        secondary["bc_random"] = bench_bc(dg, g, E, V, args)
     if args.ppr_seeds > 0:
         secondary["ppr_batch"] = bench_ppr(dg, g, E, V, args)
 

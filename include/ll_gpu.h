@@ -153,8 +153,8 @@ int ll_gpu_sssp(ll_gpu_graph* g, int level, int64_t root, float* dist_out, uint6
  * in the given order.  Path counts and dependencies are FP32 as in the
  * reference; a vertex's dependency sums its next-level out-neighbours' terms
  * in iterator order, so results match the reference bit for bit while path
- * counts stay below 2^24 and no vertex has more than 1024 out-edges (longer
- * rows sum in parallel: FP32 reassociation). */
+ * counts stay below 2^24 (past that the reference's own atomic adds make
+ * them depend on thread timing). */
 int ll_gpu_bc(ll_gpu_graph* g, int level, const int64_t* sources, int64_t n, float* bc_out);
 /* Replaces: task `sssp_unweighted` (benchmark/benchmarks/sssp.h:220-333) and
  * `bfs` (bfs.h:85-128): levels_out[max_nodes] (may be NULL) gets the BFS level,
