@@ -91,26 +91,43 @@ def tuned_key(kind: str, dg, E: int | None, n: int) -> str:
 
 def bench_sssp(dg, g, E: int, V: int, args) -> dict:
     """Weighted SSSP (sssp_weighted) from a few sources over the levels'
-    U[0, 1) weights; distances stay on the device."""
+    U[0, 1) weights; distances stay on the device.  --sssp-sweep times the
+    listed near/far threshold steps (thousandths of the mean weight) as well."""
     roots = pick_sources(g, args.sssp_sources, args.seed + 7)
-    dg.sssp(int(roots[0]), want_dist=False)  # warm the weighted snapshot
     ctx = dg.ctx
-    ctx.sync()
-    ctx.profile(True)
-    reached = rounds = 0
-    t0 = time.perf_counter()
-    for r in roots:
-        _, n, _, k = dg.sssp(int(r), want_dist=False)
-        reached += n
-        rounds += k
-    ctx.sync()
-    t = time.perf_counter() - t0
-    n_rx, ms_rx = ctx.kernel_time("k_relax")
-    ctx.profile(False)
-    out = {"sources": len(roots), "seconds": round(t, 4),
-           "ms_per_source": round(t * 1e3 / len(roots), 2),
-           "gteps": round(E * len(roots) / t / 1e9, 3), "reached": reached,
-           "rounds": rounds, "relax_kernel_ms": round(ms_rx, 3)}
+
+    def timed(pct: int, minq: int = -1) -> dict:
+        ctx.set_option("sssp_delta_milli", pct)
+        ctx.set_option("sssp_min_queue", minq)
+        dg.sssp(int(roots[0]), want_dist=False)  # warm the weighted snapshot
+        ctx.sync()
+        ctx.profile(True)
+        reached = rounds = 0
+        t0 = time.perf_counter()
+        for r in roots:
+            _, n, _, k = dg.sssp(int(r), want_dist=False)
+            reached += n
+            rounds += k
+        ctx.sync()
+        t = time.perf_counter() - t0
+        n_rx, ms_rx = ctx.kernel_time("k_relax")
+        ctx.profile(False)
+        return {"sources": len(roots), "seconds": round(t, 4),
+                "ms_per_source": round(t * 1e3 / len(roots), 2),
+                "gteps": round(E * len(roots) / t / 1e9, 3), "reached": reached,
+                "rounds": rounds, "relax_kernel_ms": round(ms_rx, 3)}
+
+    sweep = {}
+    for item in [x.strip() for x in args.sssp_sweep.split(",") if x.strip()]:
+        pct, _, minq = item.partition(":")
+        sweep[item] = timed(int(pct), int(minq) if minq else -1)
+        log(f"SSSP delta {pct}/1000 min queue {minq or 'auto'}: {sweep[item]}")
+    out = timed(args.sssp_delta_milli)
+    out["delta_milli"] = args.sssp_delta_milli
+    if sweep:
+        out["delta_sweep_ms_per_source"] = {k: v["ms_per_source"] for k, v in sweep.items()}
+    ctx.set_option("sssp_delta_milli", -1)
+    ctx.set_option("sssp_min_queue", -1)
     log(f"SSSP: {out}")
     return out
 
@@ -312,6 +329,11 @@ def main() -> int:
     ap.add_argument("--ppr-seeds", type=int, default=128, help="batched PPR seeds (0 = skip)")
     ap.add_argument("--sssp-sources", type=int, default=8,
                     help="weighted SSSP sources on the same graph (0 = skip)")
+    ap.add_argument("--sssp-delta-milli", type=int, default=-1,
+                    help="weighted SSSP near/far step, thousandths of the mean weight (-1 = auto)")
+    ap.add_argument("--sssp-sweep", default="",
+                    help="comma list of near/far steps (thousandths[:min queue]) also timed, "
+                         "e.g. 0,10,10:0,20:65536")
     ap.add_argument("--bc-sources", type=int, default=8,
                     help="betweenness centrality sources on the same graph (0 = skip)")
     ap.add_argument("--wpr-scale", type=int, default=23,

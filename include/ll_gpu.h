@@ -233,7 +233,16 @@ int ll_gpu_tuned(ll_gpu_ctx* ctx, const char* prefix, int* value);
 /* Tuning knobs: "pr_reorder" (1 = gather PageRank contributions in
  * descending-out-degree slot order; default), "pr_cap" (default row-block
  * capacity, 2048), "pr_seg_bits" (pull over column segments of 2^N
- * contribution slots; 0 = off, -1 = automatic = off). */
+ * contribution slots; 0 = off, -1 = automatic = off), "pr_item_sort" (sort
+ * each row block's entries by slot; 1 on, 0 off, -1 automatic = off),
+ * "pr_hot_bits" (hot/cold split of the pull at slot 2^N; 0 = off, -1 =
+ * automatic = off), "sssp_delta_milli" (weighted SSSP near/far threshold
+ * step in thousandths of the mean edge weight; 0 = no far pile, plain
+ * Bellman-Ford rounds; -1 = automatic = 5), "sssp_min_queue" (once edges worth
+ * half the snapshot have been relaxed, the smallest queue a near/far phase
+ * starts with when the far pile holds that many -- smaller queues rejoin the
+ * pile; 0 = off, -1 = automatic = max_nodes / 32).  Results never depend on
+ * them. */
 int ll_gpu_set_option(ll_gpu_ctx* ctx, const char* name, long value);
 /* Drop the cached snapshot materialisations (the next call rebuilds them). */
 void ll_gpu_graph_drop_snapshots(ll_gpu_graph* g);

@@ -123,6 +123,7 @@ struct Csr {
   DevBuf adj;   // uint32_t[E]
   DevBuf w;     // float[E]: out-edge weights when every level up to `level` has them
   double ms = 0;
+  mutable double w_mean = -1;  // mean of w, computed on first use (weighted SSSP)
 };
 
 // Row blocks of a CSR for the load-balanced row-reduction kernels.

@@ -67,6 +67,47 @@ def test_sssp_matches_reference(ctx, name):
         assert int(mx) == int(ref["return"])
 
 
+def chain_with_shortcuts(n, seed):
+    """A long weighted chain plus a few forward shortcuts: distances spread
+    over thousands of threshold steps with a handful of vertices each."""
+    rng = np.random.default_rng(seed)
+    src = list(range(n - 1))
+    dst = list(range(1, n))
+    for _ in range(n // 20):
+        a = int(rng.integers(0, n - 10))
+        src.append(a)
+        dst.append(a + int(rng.integers(2, 10)))
+    pairs = sorted(set(zip(src, dst)))
+    w = rng.uniform(0.0, 4.0, len(pairs)).astype(np.float32)
+    g = HostGraph(n)
+    g.add_level([p[0] for p in pairs], [p[1] for p in pairs], weights=w)
+    return g
+
+
+@pytest.mark.parametrize("milli,minq", [(0, 0), (60, 0), (60, -1), (30, 1), (250, 64), (1000, 0),
+                                        (10, 1 << 30), (1, -1)])
+def test_sssp_schedules_match_oracle(ctx, milli, minq):
+    """Every near/far step and queue floor gives the same distances (the
+    schedule only changes how often edges are relaxed)."""
+    gpu_case(f"sssp schedule delta={milli}/1000 min_queue={minq}")
+    graphs = [weighted_multilevel(4096, 40000, 2000, 3, seed=23), chain_with_shortcuts(3000, 5)]
+    try:
+        ctx.set_option("sssp_delta_milli", milli)
+        ctx.set_option("sssp_min_queue", minq)
+        for g in graphs:
+            o = oracle.OracleGraph.from_host(g)
+            o.add_weights(g)
+            dg = DeviceGraph(ctx, g)
+            for root in (0, 7, 1500):
+                ref, n_ref, mx_ref, _ = o.sssp_weighted(root)
+                d, n, mx, _ = dg.sssp(root)
+                assert np.array_equal(d, ref), (milli, minq, root)
+                assert n == n_ref and mx == mx_ref
+    finally:
+        ctx.set_option("sssp_delta_milli", -1)
+        ctx.set_option("sssp_min_queue", -1)
+
+
 def test_sssp_edge_cases(ctx):
     gpu_case("sssp edge cases")
     g = HostGraph(6)
