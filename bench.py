@@ -132,6 +132,83 @@ def bench_sssp(dg, g, E: int, V: int, args) -> dict:
     return out
 
 
+def bench_cow_upload(ctx, args) -> dict:
+    """The level-freeze -> device pipeline on an R-MAT base plus levels
+    confined to windows of V/64 vertex IDs (1% of the base's edges inserted,
+    0.01% deleted, both ends in the window): per level the host freeze, the
+    push of whole vertex tables and the push of the copy-on-write pages the
+    level wrote; then freezes overlapped with page pushes on a second thread."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    from llama_amd.graph import DeviceGraph, build_multilevel
+    g = build_multilevel(args.cow_scale, levels=1, seed=args.seed + 11)
+    V = g.num_nodes
+    e0 = int(g.flatten(0)[1].size)
+    w = V // 64
+    whole = DeviceGraph(ctx, g)
+    cow = DeviceGraph(ctx, g, cow=True)
+    ctx.sync()
+
+    def window(k):
+        lo = (k * 7 * w) % (V - w)
+        g.add_window_level(e0 // 100, e0 // 10000, lo, lo + w, args.seed + 200 + k)
+
+    t = {"freeze": 0.0, "whole": 0.0, "cow": 0.0}
+    b0 = {"whole": whole.upload_stats()[0], "cow": cow.upload_stats()[0]}
+    for k in range(args.cow_levels):
+        t0 = time.perf_counter()
+        window(k)
+        t["freeze"] += time.perf_counter() - t0
+        for name, dg in (("whole", whole), ("cow", cow)):
+            t0 = time.perf_counter()
+            dg.sync_levels()  # the cow mirror lists the pages the freeze wrote
+            ctx.sync()
+            t[name] += time.perf_counter() - t0
+    n = args.cow_levels
+    b1 = {"whole": whole.upload_stats()[0] - b0["whole"], "cow": cow.upload_stats()[0] - b0["cow"]}
+    pages = [g.level_pages(d, g.num_levels - 1)[1] for d in (0, 1)]
+    ra, _ = whole.pagerank(3)
+    rb, _ = cow.pagerank(3)
+    out = {"graph": f"R-MAT s{args.cow_scale} EF16 base + {n} window levels "
+                    f"(window V/64, +1%/-0.01% of the base edges each)",
+           "vt_pages_per_level": (V + 1 + 511) // 512,
+           "written_pages_per_level_out_in": pages,
+           "freeze_ms_per_level": round(t["freeze"] * 1e3 / n, 2),
+           "whole_tables": {"bytes_per_level": b1["whole"] // n,
+                            "ms_per_level": round(t["whole"] * 1e3 / n, 2),
+                            "gb_per_s": round(b1["whole"] / t["whole"] / 1e9, 2)},
+           "cow_pages": {"bytes_per_level": b1["cow"] // n,
+                         "ms_per_level": round(t["cow"] * 1e3 / n, 2),
+                         "gb_per_s": round(b1["cow"] / t["cow"] / 1e9, 2)},
+           "identical_pagerank": bool((ra == rb).all())}
+    whole.close()
+    # the pipeline: level k's pages travel while the host freezes level k+1
+    pipe = DeviceGraph(ctx, g, cow=True)
+    ctx.sync()
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(1) as ex:
+        fut = None
+        for k in range(n, 2 * n):
+            window(k)
+            items = pipe.prepare_level(g.num_levels - 1)
+            if fut is not None:
+                fut.result()
+            fut = ex.submit(pipe.push_prepared, items)
+        fut.result()
+    ctx.sync()
+    t_pipe = time.perf_counter() - t0
+    out["overlapped_ms_per_level"] = round(t_pipe * 1e3 / n, 2)
+    out["sequential_ms_per_level"] = round((t["freeze"] + t["cow"]) * 1e3 / n, 2)
+    rc, _ = pipe.pagerank(3)
+    cow.sync_levels()
+    rd, _ = cow.pagerank(3)
+    out["pipelined_identical_pagerank"] = bool((rc == rd).all())
+    pipe.close()
+    cow.close()
+    log(f"COW upload: {out}")
+    return out
+
+
 def bench_bc(dg, g, E: int, V: int, args) -> dict:
     """Betweenness centrality (bc_random's Brandes passes) from a few sources;
     per source a BFS, a path-count pass and a dependency pass over the edges."""
@@ -336,6 +413,10 @@ def main() -> int:
                          "e.g. 0,10,10:0,20:65536")
     ap.add_argument("--bc-sources", type=int, default=8,
                     help="betweenness centrality sources on the same graph (0 = skip)")
+    ap.add_argument("--cow-scale", type=int, default=24,
+                    help="copy-on-write upload leg: base graph scale (0 = skip)")
+    ap.add_argument("--cow-levels", type=int, default=10,
+                    help="copy-on-write upload leg: window levels")
     ap.add_argument("--wpr-scale", type=int, default=23,
                     help="weighted PageRank + adjoint graph scale, single level (0 = skip)")
     args = ap.parse_args()
@@ -462,6 +543,8 @@ def main() -> int:
         secondary["sssp_weighted"] = bench_sssp(dg, g, E, V, args)
     if args.bc_sources > 0:
         secondary["bc_random"] = bench_bc(dg, g, E, V, args)
+    if args.cow_scale > 0:
+        secondary["cow_upload"] = bench_cow_upload(ctx, args)
     if args.ppr_seeds > 0:
         secondary["ppr_batch"] = bench_ppr(dg, g, E, V, args)
 

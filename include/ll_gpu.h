@@ -63,7 +63,8 @@ typedef struct {
   const void* vt;            /* 16-byte entries {int64 start; uint32 len; uint32 degree} */
   const uint64_t* et;        /* edge-table words                                 */
   uint32_t flags;            /* LL_GPU_HAS_DELETIONS | LL_GPU_HAS_CONT           */
-  const uint32_t* page_table;/* reserved for COW page upload; must be NULL       */
+  const uint32_t* page_table;/* reserved; must be NULL (COW pages: see
+                                ll_gpu_graph_push_level_pages)                 */
 } ll_gpu_level_view;
 
 /* Per-snapshot statistics (materialised on first use of a level). */
@@ -92,6 +93,17 @@ void ll_gpu_graph_destroy(ll_gpu_graph* g);
  * (ll_mlcsr_graph.h:1501-1831) / load_direct (loaders/ll_load_utils.h:289-880),
  * whose new VT_k / ET_k this call copies to HBM (async, pinned staging). */
 int ll_gpu_graph_push_level(ll_gpu_graph* g, const ll_gpu_level_view* view);
+/* Replaces: the same, for a level whose vertex table is the reference's
+ * copy-on-write array (ll_mem_array_swcow, ll_mem_array.h:918-1140: an
+ * indirection table of 512-entry pages; pages a level did not write are the
+ * previous level's).  pages[p], p < num_pages = ceil((max_nodes + 1) / 512),
+ * is the host address of page p (its _indirection entry) when the level wrote
+ * it, NULL when it is shared with the previous level.  Only the written pages
+ * are copied; the device keeps a page table per level that points at the
+ * previous level's copy of every shared page.  view->vt is not read (may be
+ * NULL); level 0 has no shared pages. */
+int ll_gpu_graph_push_level_pages(ll_gpu_graph* g, const ll_gpu_level_view* view,
+                                  const void* const* pages, uint64_t num_pages);
 /* Replaces: update_max_visible_level_lower_only on old levels
  * (ll_mlcsr_sp.h:849-863; ll_mlcsr_graph.h:1091-1108).  edge_ids use the
  * level<<38|index encoding (ll_mlcsr_helpers.h:77-79). */

@@ -73,11 +73,24 @@ int ll_host_graph_flatten_weights(ll_host_graph* g, int level, const float** w, 
  * live deletions); the batches stay readable through ll_host_graph_last_batch. */
 int ll_host_graph_add_random_level(ll_host_graph* g, double ins_frac, double del_frac,
                                    uint64_t seed);
+/* Freeze one level confined to the vertex window [lo, hi): n_ins new edges
+ * and n_del deletions of live edges (at most as many as there are), all with
+ * both ends in the window (seeded as ll_host_graph_add_random_level). */
+int ll_host_graph_add_window_level(ll_host_graph* g, uint64_t n_ins, uint64_t n_del, uint64_t lo,
+                                   uint64_t hi, uint64_t seed);
 int ll_host_graph_last_batch(const ll_host_graph* g, const ll_host_edges** ins,
                              const ll_host_edges** del);
 /* Borrowed view of one frozen level of one direction (LL_GPU_DIR_OUT/IN). */
 int ll_host_graph_level_view(const ll_host_graph* g, uint32_t dir, uint32_t level,
                              ll_gpu_level_view* out);
+/* Copy-on-write pages of one level's vertex table, for
+ * ll_gpu_graph_push_level_pages: pages[p] (num_pages = ceil((max_nodes+1)/512))
+ * points into the level's table where the freeze wrote page p, NULL where it
+ * did not (the page then equals the previous level's, as the reference's
+ * ll_mem_array_swcow shares it, ll_mem_array.h:1090-1140); *written (may be
+ * NULL) = non-NULL pages. */
+int ll_host_graph_level_pages(const ll_host_graph* g, uint32_t dir, uint32_t level,
+                              const void** pages, uint64_t num_pages, uint64_t* written);
 /* Deletion patches written while freezing `level` into older levels of `dir`. */
 int ll_host_graph_patches(const ll_host_graph* g, uint32_t dir, uint32_t level,
                           const uint64_t** edge_ids, const uint32_t** max_levels, uint64_t* n);

@@ -45,6 +45,7 @@ GPU_SIGNATURES = {
     "ll_gpu_graph_create": (C.c_int, [vp, u64, u32, P(vp)]),
     "ll_gpu_graph_destroy": (None, [vp]),
     "ll_gpu_graph_push_level": (C.c_int, [vp, P(LevelView)]),
+    "ll_gpu_graph_push_level_pages": (C.c_int, [vp, P(LevelView), P(vp), u64]),
     "ll_gpu_graph_patch_deletions": (C.c_int, [vp, u32, P(u64), P(u32), u64]),
     "ll_gpu_graph_num_levels": (C.c_int, [vp, u32]),
     "ll_gpu_graph_max_nodes": (u64, [vp]),
@@ -96,6 +97,8 @@ HOST_SIGNATURES = {
     "ll_host_graph_add_random_level": (C.c_int, [vp, f64, f64, u64]),
     "ll_host_graph_last_batch": (C.c_int, [vp, P(vp), P(vp)]),
     "ll_host_graph_level_view": (C.c_int, [vp, u32, u32, P(LevelView)]),
+    "ll_host_graph_level_pages": (C.c_int, [vp, u32, u32, P(vp), u64, P(u64)]),
+    "ll_host_graph_add_window_level": (C.c_int, [vp, u64, u64, u64, u64, u64]),
     "ll_host_graph_patches": (C.c_int, [vp, u32, u32, P(P(u64)), P(P(u32)), P(u64)]),
     "ll_host_graph_flatten": (C.c_int, [vp, u32, C.c_int, P(P(u64)), P(P(u32)), P(u64)]),
     "ll_host_gen_rmat": (C.c_int, [u32, f64, f64, f64, f64, u64, u64, P(vp)]),

@@ -110,6 +110,10 @@ int ll_host_graph_add_level_edges(ll_host_graph* g, const ll_host_edges* ins,
                                    nd ? del->e.dst.data() : nullptr, nd);
 }
 
+// Vertex-table page size of the reference's copy-on-write array
+// (LL_ENTRIES_PER_PAGE, ll_common.h:361-363).
+constexpr uint64_t kPageEntries = 512;
+
 int ll_host_graph_add_random_level(ll_host_graph* g, double ins_frac, double del_frac,
                                    uint64_t seed) {
   if (!g) return fail(LL_GPU_EINVAL, "graph is NULL");
@@ -122,6 +126,37 @@ int ll_host_graph_add_random_level(ll_host_graph* g, double ins_frac, double del
     if (!g->upd) g->upd.reset(new llhost::Updater(*g->g));
     llhost::EdgeList ins, del;
     g->upd->next(ins_frac, del_frac, seed, ins, del);
+    llhost::vec<float> w;
+    if (g->g->weighted()) uniform_weights(ins.src.size(), seed, w);  // U[0, 1), seeded
+    if (!g->g->add_level(ins.src.data(), ins.dst.data(), ins.src.size(), del.src.data(),
+                         del.dst.data(), del.src.size(), w.empty() ? nullptr : w.data())) {
+      g->upd.reset();
+      return fail(LL_GPU_ESTATE, g->g->last_error());
+    }
+    g->last_ins.e = std::move(ins);
+    g->last_ins.w = std::move(w);
+    g->last_del.e = std::move(del);
+    g->last_del.w.clear();
+  } catch (const std::bad_alloc&) {
+    return fail(LL_GPU_ENOMEM, "out of host memory while drawing a level");
+  }
+  return LL_GPU_OK;
+}
+
+int ll_host_graph_add_window_level(ll_host_graph* g, uint64_t n_ins, uint64_t n_del, uint64_t lo,
+                                   uint64_t hi, uint64_t seed) {
+  if (!g) return fail(LL_GPU_EINVAL, "graph is NULL");
+  if (g->g->num_levels() == 0) return fail(LL_GPU_ESTATE, "random levels need a base level");
+  if (n_del > 0 && !g->g->deletions())
+    return fail(LL_GPU_EINVAL, "deletions need a graph created with deletions on");
+  if (lo >= hi || hi > g->g->num_nodes() || hi - lo < 2)
+    return fail(LL_GPU_EINVAL, "window must satisfy lo + 2 <= hi <= max_nodes");
+  if (n_ins > (hi - lo) * (hi - lo - 1) / 2)
+    return fail(LL_GPU_EINVAL, "more inserts than the window can hold");
+  try {
+    if (!g->upd) g->upd.reset(new llhost::Updater(*g->g));
+    llhost::EdgeList ins, del;
+    g->upd->next_window(n_ins, n_del, lo, hi, seed, ins, del);
     llhost::vec<float> w;
     if (g->g->weighted()) uniform_weights(ins.src.size(), seed, w);  // U[0, 1), seeded
     if (!g->g->add_level(ins.src.data(), ins.dst.data(), ins.src.size(), del.src.data(),
@@ -163,6 +198,26 @@ int ll_host_graph_level_view(const ll_host_graph* g, uint32_t dir, uint32_t leve
   out->et = lv.et.data();
   out->flags = LL_GPU_HAS_CONT | (g->g->deletions() ? LL_GPU_HAS_DELETIONS : 0u);
   out->page_table = nullptr;
+  return LL_GPU_OK;
+}
+
+int ll_host_graph_level_pages(const ll_host_graph* g, uint32_t dir, uint32_t level,
+                              const void** pages, uint64_t num_pages, uint64_t* written) {
+  if (!g || !pages) return fail(LL_GPU_EINVAL, "NULL argument");
+  if (dir > 1) return fail(LL_GPU_EINVAL, "dir must be 0 (out) or 1 (in)");
+  if (int(level) >= g->g->num_levels()) return fail(LL_GPU_EINVAL, "no such level");
+  const auto& lv = g->g->level(int(dir), int(level));
+  const uint64_t np = lv.page_written.size();
+  if (num_pages != np) return fail(LL_GPU_EINVAL, "num_pages must be ceil((max_nodes + 1) / 512)");
+  // the builder records the pages it writes, as the reference's copy-on-write
+  // array allocates a page on its first write (ll_mem_array.h:1122-1140)
+  uint64_t n = 0;
+  for (uint64_t p = 0; p < np; p++) {
+    pages[p] = lv.page_written[p] ? static_cast<const void*>(lv.vt.data() + p * kPageEntries)
+                                  : nullptr;
+    n += lv.page_written[p] ? 1 : 0;
+  }
+  if (written) *written = n;
   return LL_GPU_OK;
 }
 

@@ -321,6 +321,20 @@ bool Graph::build_direction(int dir, uint32_t k, const vec<uint32_t>& new_cnt,
   }
   lv.vt[V_] = VtEntry{int64_t(edge_create(k, total)), 0, 0};
   lv.sorted_segments = sorted;
+  // pages of VT written by this level (the copy-on-write pages): any touched
+  // vertex, and the sentinel's page, which every level rewrites
+  const uint64_t np = (V_ + 1 + kVtPageEntries - 1) / kVtPageEntries;
+  lv.page_written.assign(np, k == 0 ? 1 : 0);
+  if (k > 0) {
+#pragma omp parallel for schedule(static)
+    for (uint64_t p = 0; p < np; p++) {
+      const uint64_t lo = p * kVtPageEntries, hi = std::min<uint64_t>(V_, lo + kVtPageEntries);
+      char any = 0;
+      for (uint64_t v = lo; v < hi && !any; v++) any = touched[v];
+      lv.page_written[p] = any;
+    }
+    lv.page_written[V_ / kVtPageEntries] = 1;
+  }
   levels.push_back(std::move(lv));
   return true;
 }
@@ -451,16 +465,44 @@ Updater::Updater(const Graph& g) : V_(g.num_nodes()) {
 
 void Updater::next(double ins_frac, double del_frac, uint64_t seed, EdgeList& ins, EdgeList& del) {
   const uint64_t E = adj_.size();
-  const uint64_t n_ins = uint64_t(ins_frac * double(E));
-  const uint64_t n_del = std::min<uint64_t>(E, uint64_t(del_frac * double(E)));
-  // Deletions: the first n_del distinct draws of a uniform live-edge index.
+  next_window(uint64_t(ins_frac * double(E)), std::min<uint64_t>(E, uint64_t(del_frac * double(E))),
+              0, V_, seed, ins, del);
+}
+
+void Updater::next_window(uint64_t n_ins, uint64_t n_del, uint64_t lo, uint64_t hi,
+                          uint64_t seed, EdgeList& ins, EdgeList& del) {
+  // deletions and inserts among edges with both ends in [lo, hi)
+  const uint64_t e_lo = ofs_[lo], E = ofs_[hi] - e_lo;
+  const uint64_t W = hi - lo;
+  const bool whole = lo == 0 && hi == V_;
+  auto eligible = [&](uint64_t idx) { return whole || (adj_[idx] >= lo && adj_[idx] < hi); };
+  uint64_t n_eligible = E;
+  if (!whole) {
+    n_eligible = 0;
+    for (uint64_t i = e_lo; i < e_lo + E; i++) n_eligible += eligible(i) ? 1 : 0;
+  }
+  n_del = std::min(n_del, n_eligible);
+  // Deletions: the first n_del distinct draws of a uniform live-edge index
+  // (whole graph); in a window, a seeded partial shuffle of its edges.
   std::vector<uint64_t> pick;
-  {
+  if (!whole) {
+    std::vector<uint64_t> elig;
+    elig.reserve(n_eligible);
+    for (uint64_t i = e_lo; i < e_lo + E; i++)
+      if (eligible(i)) elig.push_back(i);
+    for (uint64_t i = 0; i < n_del; i++) {
+      const uint64_t j = i + rng_u64(seed, 1, i) % (elig.size() - i);
+      std::swap(elig[i], elig[j]);
+    }
+    pick.assign(elig.begin(), elig.begin() + n_del);
+    std::sort(pick.begin(), pick.end());
+  } else {
     std::vector<std::pair<uint64_t, uint64_t>> cand;  // (index, draw number)
     uint64_t drawn = 0;
     while (true) {
       uint64_t want = n_del + n_del / 8 + 16;
-      for (uint64_t j = 0; j < want; j++, drawn++) cand.push_back({rng_u64(seed, 1, drawn) % std::max<uint64_t>(E, 1), drawn});
+      for (uint64_t j = 0; j < want; j++, drawn++)
+        cand.push_back({e_lo + rng_u64(seed, 1, drawn) % std::max<uint64_t>(E, 1), drawn});
       std::vector<std::pair<uint64_t, uint64_t>> s(cand);
       std::sort(s.begin(), s.end());
       std::vector<std::pair<uint64_t, uint64_t>> first;  // (draw number, index)
@@ -493,8 +535,8 @@ void Updater::next(double ins_frac, double del_frac, uint64_t seed, EdgeList& in
     const uint64_t base = drawn;
 #pragma omp parallel for
     for (uint64_t j = 0; j < want; j++) {
-      uint64_t u = rng_u64(seed, 2, base + j) % V_;
-      uint64_t v = rng_u64(seed, 3, base + j) % V_;
+      uint64_t u = lo + rng_u64(seed, 2, base + j) % W;
+      uint64_t v = lo + rng_u64(seed, 3, base + j) % W;
       if (u == v) continue;
       auto b = adj_.begin() + ofs_[u], e = adj_.begin() + ofs_[u + 1];
       if (std::binary_search(b, e, uint32_t(v))) continue;

@@ -65,12 +65,17 @@ struct VtEntry {
 };
 static_assert(sizeof(VtEntry) == 16, "VT entry must be 16 bytes");
 
+// Vertex-table page of the reference's copy-on-write array (LL_ENTRIES_PER_PAGE,
+// ll_common.h:361-363).
+constexpr uint64_t kVtPageEntries = 512;
+
 struct Level {
   vec<VtEntry> vt;     // V+1 entries; VT[V] is the sentinel
   vec<uint64_t> et;    // edge-table words incl. continuation records
   vec<float> wt;       // out direction of a weighted graph: weight per ET word (0 for
                        // continuation words), the level's `weight` edge property
   bool sorted_segments = true; // every segment ascending by neighbour
+  std::vector<char> page_written;  // per VT page of kVtPageEntries: written by this level
 };
 
 // Deletions applied while freezing a level: rewrites of older words.
@@ -155,6 +160,10 @@ class Updater {
   explicit Updater(const Graph& g);
   uint64_t num_live_edges() const { return adj_.size(); }
   void next(double ins_frac, double del_frac, uint64_t seed, EdgeList& ins, EdgeList& del);
+  // n_ins new edges with both ends in [lo, hi) and n_del live edges whose tail
+  // is there (a level confined to a window of vertex IDs).
+  void next_window(uint64_t n_ins, uint64_t n_del, uint64_t lo, uint64_t hi, uint64_t seed,
+                   EdgeList& ins, EdgeList& del);
   // Also accepts explicit batches so the snapshot stays in step with the graph.
   void apply(const EdgeList& ins, const EdgeList& del);
   const vec<uint64_t>& offsets() const { return ofs_; }

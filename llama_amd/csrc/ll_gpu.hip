@@ -271,8 +271,39 @@ void ll_gpu_graph_destroy(ll_gpu_graph* g) {
   // stream-ordered frees were queued by the buffers' destructors
 }
 
-int ll_gpu_graph_push_level(ll_gpu_graph* g, const ll_gpu_level_view* v) {
-  if (!g || !v) return LL_GPU_EINVAL;
+}  // extern "C"
+
+namespace {
+
+// Upload the listed host pages packed: page i of `src` lands at
+// dst + i * kVtPageEntries; entries[i] of it are valid (fewer than a page
+// only for the last page of a table).  Staged through the two pinned halves.
+void upload_pages(ll_gpu_ctx* c, llgpu::VtEntry* dst, const std::vector<const void*>& src,
+                  const std::vector<uint32_t>& entries) {
+  using llgpu::kVtPageEntries;
+  const size_t page_bytes = kVtPageEntries * sizeof(llgpu::VtEntry);
+  const size_t per_half = (c->pinned_bytes / 2) / page_bytes;
+  int slot = 0;
+  for (size_t i0 = 0; i0 < src.size(); i0 += per_half, slot ^= 1) {
+    const size_t n = std::min(per_half, src.size() - i0);
+    LLG_HIP(hipEventSynchronize(c->pinned_ev[slot]));
+    char* stage = static_cast<char*>(c->pinned) + slot * (c->pinned_bytes / 2);
+    for (size_t i = 0; i < n; i++)
+      std::memcpy(stage + i * page_bytes, src[i0 + i], entries[i0 + i] * sizeof(llgpu::VtEntry));
+    LLG_HIP(hipMemcpyAsync(dst + i0 * kVtPageEntries, stage, n * page_bytes,
+                           hipMemcpyHostToDevice, c->stream));
+    LLG_HIP(hipEventRecord(c->pinned_ev[slot], c->stream));
+  }
+}
+
+// Shared by the full and the page-wise push: `pages` is null for a full
+// vertex table in v->vt, else one host address per page (null = the previous
+// level's page).
+int push_level(ll_gpu_graph* g, const ll_gpu_level_view* v, const void* const* pages,
+               uint64_t num_pages) {
+  using llgpu::DevBuf;
+  using llgpu::kVtPageEntries;
+  using llgpu::VtEntry;
   ll_gpu_ctx* c = g->ctx;
   return guarded(c, [&]() {
     LLG_HIP(hipSetDevice(c->device));
@@ -280,32 +311,68 @@ int ll_gpu_graph_push_level(ll_gpu_graph* g, const ll_gpu_level_view* v) {
     auto& d = g->dir[v->dir];
     LLG_CHECK(v->level == d.vt.size(), LL_GPU_ESTATE,
               "levels must be pushed in order (expected level " + std::to_string(d.vt.size()) + ")");
-    LLG_CHECK(v->page_table == nullptr, LL_GPU_ENOTSUP, "COW page-table upload is not supported");
+    LLG_CHECK(v->page_table == nullptr, LL_GPU_EINVAL, "page_table is reserved and must be NULL");
     LLG_CHECK(v->num_nodes == g->V && v->num_vt_entries == g->V + 1, LL_GPU_EINVAL,
               "vertex table must have max_nodes+1 entries");
     LLG_CHECK(v->flags == g->flags, LL_GPU_EINVAL, "level flags differ from the graph flags");
-    LLG_CHECK(v->vt != nullptr && (v->num_et_words == 0 || v->et != nullptr), LL_GPU_EINVAL,
-              "NULL level arrays");
+    LLG_CHECK((pages != nullptr || v->vt != nullptr) && (v->num_et_words == 0 || v->et != nullptr),
+              LL_GPU_EINVAL, "NULL level arrays");
     LLG_CHECK(v->level < llgpu::kLevelMask - 2, LL_GPU_ERANGE, "level IDs exhausted");
+    const uint64_t np = (g->V + 1 + kVtPageEntries - 1) / kVtPageEntries;
+    std::vector<const void*> src;
+    std::vector<uint32_t> ents;
+    std::vector<uint64_t> slot(np, ~uint64_t(0));  // page -> packed index, or shared
+    auto valid = [&](uint64_t p) {
+      return uint32_t(std::min<uint64_t>(kVtPageEntries, g->V + 1 - p * kVtPageEntries));
+    };
+    if (pages) {
+      LLG_CHECK(num_pages == np, LL_GPU_EINVAL,
+                "num_pages must be ceil((max_nodes + 1) / 512) = " + std::to_string(np));
+      for (uint64_t p = 0; p < np; p++) {
+        if (!pages[p]) {
+          LLG_CHECK(v->level > 0, LL_GPU_EINVAL, "level 0 has no previous level to share pages with");
+          continue;
+        }
+        slot[p] = src.size();
+        src.push_back(pages[p]);
+        ents.push_back(valid(p));
+      }
+    }
     hipEvent_t e0, e1;
     LLG_HIP(hipEventCreate(&e0));
     LLG_HIP(hipEventCreate(&e1));
     LLG_HIP(hipEventRecord(e0, c->stream));
-    DevBuf vt, et;
-    const size_t vt_bytes = size_t(v->num_vt_entries) * sizeof(llgpu::VtEntry);
+    DevBuf vt, et, ptab;
     const size_t et_bytes = size_t(v->num_et_words) * sizeof(uint64_t);
-    vt.alloc(vt_bytes, c->stream);
+    size_t vt_bytes = 0;
+    std::vector<const VtEntry*> table(np);
+    if (pages) {
+      vt.alloc(std::max<size_t>(src.size(), 1) * kVtPageEntries * sizeof(VtEntry), c->stream);
+      upload_pages(c, vt.as<VtEntry>(), src, ents);
+      for (uint32_t x : ents) vt_bytes += size_t(x) * sizeof(VtEntry);
+      for (uint64_t p = 0; p < np; p++)
+        table[p] = slot[p] != ~uint64_t(0) ? vt.as<VtEntry>() + slot[p] * kVtPageEntries
+                                           : d.vt_pages.back()[p];
+    } else {
+      vt_bytes = size_t(v->num_vt_entries) * sizeof(VtEntry);
+      vt.alloc(np * kVtPageEntries * sizeof(VtEntry), c->stream);
+      upload(c, vt.p, v->vt, vt_bytes);
+      for (uint64_t p = 0; p < np; p++) table[p] = vt.as<VtEntry>() + p * kVtPageEntries;
+    }
+    ptab.alloc(np * sizeof(void*), c->stream);
+    upload(c, ptab.p, table.data(), np * sizeof(void*));
     et.alloc(std::max<size_t>(et_bytes, 8), c->stream);
-    upload(c, vt.p, v->vt, vt_bytes);
     if (et_bytes) upload(c, et.p, v->et, et_bytes);
     d.vt.push_back(std::move(vt));
+    d.vt_pages.push_back(std::move(table));
+    d.vt_ptab.push_back(std::move(ptab));
     d.et.push_back(std::move(et));
     d.et_words.push_back(v->num_et_words);
-    std::vector<const uint64_t*> table;
-    for (auto& b : d.et) table.push_back(b.as<const uint64_t>());
+    std::vector<const uint64_t*> etab;
+    for (auto& b : d.et) etab.push_back(b.as<const uint64_t>());
     DevBuf t;
-    t.alloc(table.size() * sizeof(void*), c->stream);
-    LLG_HIP(hipMemcpyAsync(t.p, table.data(), table.size() * sizeof(void*),
+    t.alloc(etab.size() * sizeof(void*), c->stream);
+    LLG_HIP(hipMemcpyAsync(t.p, etab.data(), etab.size() * sizeof(void*),
                            hipMemcpyHostToDevice, c->stream));
     d.et_table = std::move(t);
     DevBuf wt;
@@ -314,7 +381,7 @@ int ll_gpu_graph_push_level(ll_gpu_graph* g, const ll_gpu_level_view* v) {
                            hipMemcpyHostToDevice, c->stream));
     d.et_words_dev = std::move(wt);
     LLG_HIP(hipEventRecord(e1, c->stream));
-    LLG_HIP(hipEventSynchronize(e1));  // host arrays and the table are released on return
+    LLG_HIP(hipEventSynchronize(e1));  // host arrays and the tables are released on return
     float ms = 0;
     LLG_HIP(hipEventElapsedTime(&ms, e0, e1));
     (void)hipEventDestroy(e0);
@@ -323,6 +390,25 @@ int ll_gpu_graph_push_level(ll_gpu_graph* g, const ll_gpu_level_view* v) {
     g->upload_bytes += vt_bytes + et_bytes;
     llgpu::drop_snapshots(g);
   });
+}
+
+}  // namespace
+
+extern "C" {
+
+int ll_gpu_graph_push_level(ll_gpu_graph* g, const ll_gpu_level_view* v) {
+  if (!g || !v) return LL_GPU_EINVAL;
+  return push_level(g, v, nullptr, 0);
+}
+
+int ll_gpu_graph_push_level_pages(ll_gpu_graph* g, const ll_gpu_level_view* v,
+                                  const void* const* pages, uint64_t num_pages) {
+  if (!g || !v) return LL_GPU_EINVAL;
+  if (!pages) {
+    g->ctx->err = "NULL page list";
+    return LL_GPU_EINVAL;
+  }
+  return push_level(g, v, pages, num_pages);
 }
 
 int ll_gpu_graph_push_weights(ll_gpu_graph* g, uint32_t level, const float* w, uint64_t n) {

@@ -26,6 +26,19 @@ struct VtEntry {
 };
 static_assert(sizeof(VtEntry) == 16, "VT entry must be 16 bytes");
 
+// Vertex tables are held in pages of kVtPageEntries entries, as the reference's
+// copy-on-write vertex table (ll_mem_array_swcow, LL_ENTRIES_PER_PAGE): each
+// level has a table of page addresses, and pages a level did not write are
+// the previous level's.
+constexpr uint32_t kVtPageBits = 9;
+constexpr uint64_t kVtPageEntries = uint64_t(1) << kVtPageBits;
+struct VtPages {
+  const VtEntry* const* pages;
+  __device__ inline const VtEntry& operator[](uint64_t v) const {
+    return pages[v >> kVtPageBits][v & (kVtPageEntries - 1)];
+  }
+};
+
 __host__ __device__ inline uint32_t edge_level(uint64_t e) { return uint32_t(e >> kBitsIndex); }
 __host__ __device__ inline uint64_t edge_index(uint64_t e) { return e & kIndexMask; }
 __host__ __device__ inline uint32_t word_maxlev(uint64_t w) {
@@ -170,7 +183,9 @@ struct PrHotCold {
 };
 
 struct DirMirror {
-  std::vector<DevBuf> vt;   // VtEntry[V+1] per level
+  std::vector<DevBuf> vt;   // per level: the pages the level wrote (all of them for a full push)
+  std::vector<std::vector<const VtEntry*>> vt_pages;  // per level: device address of every page
+  std::vector<DevBuf> vt_ptab;  // per level: vt_pages on the device (VtPages)
   std::vector<DevBuf> et;   // uint64_t[words] per level
   std::vector<uint64_t> et_words;
   DevBuf et_table;          // const uint64_t*[levels] on device

@@ -33,7 +33,7 @@ constexpr int kWaves = kBlock / 64;
 
 __global__ void k_zero_u32(uint32_t* p) { *p = 0; }
 
-__global__ void k_degrees(const VtEntry* vt, uint32_t* deg, uint64_t V) {
+__global__ void k_degrees(VtPages vt, uint32_t* deg, uint64_t V) {
   uint64_t v = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (v < V) deg[v] = vt[v].degree;
 }
@@ -43,7 +43,7 @@ __global__ void k_degrees(const VtEntry* vt, uint32_t* deg, uint64_t V) {
 // lies inside its level's edge table; a violation flags the mirror as corrupt
 // and stops the walk instead of reading out of bounds.
 template <typename F>
-__device__ inline void walk_chain(const VtEntry* vt, const uint64_t* const* et,
+__device__ inline void walk_chain(VtPages vt, const uint64_t* const* et,
                                   const uint64_t* et_words, uint32_t level, uint64_t v,
                                   uint32_t* bad, F&& f) {
   VtEntry e = vt[v];
@@ -66,7 +66,7 @@ __device__ inline void walk_chain(const VtEntry* vt, const uint64_t* const* et,
   }
 }
 
-__global__ void k_seg_count(const VtEntry* vt, const uint64_t* const* et,
+__global__ void k_seg_count(VtPages vt, const uint64_t* const* et,
                             const uint64_t* et_words, uint32_t level, uint64_t V,
                             uint32_t* nseg, uint64_t* nwords, uint32_t* bad) {
   uint64_t v = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -81,7 +81,7 @@ __global__ void k_seg_count(const VtEntry* vt, const uint64_t* const* et,
   nwords[v] = words;
 }
 
-__global__ void k_seg_fill(const VtEntry* vt, const uint64_t* const* et,
+__global__ void k_seg_fill(VtPages vt, const uint64_t* const* et,
                            const uint64_t* et_words, uint32_t level, uint64_t V,
                            const uint64_t* seg_base, const uint64_t* word_base,
                            uint64_t* seg_addr, uint64_t* seg_vofs, uint32_t* bad) {
@@ -250,8 +250,9 @@ const uint32_t* ensure_degrees(ll_gpu_graph* g, uint32_t dir, int level) {
   g->deg[dir].alloc(g->V * sizeof(uint32_t), s);
   {
     TimedLaunch t(g->ctx, "k_degrees");
-    k_degrees<<<grid_for(g->V, 256), 256, 0, s>>>(g->dir[dir].vt[level].as<VtEntry>(),
-                                                   g->deg[dir].as<uint32_t>(), g->V);
+    k_degrees<<<grid_for(g->V, 256), 256, 0, s>>>(
+        VtPages{g->dir[dir].vt_ptab[level].as<const VtEntry* const>()}, g->deg[dir].as<uint32_t>(),
+        g->V);
   }
   LLG_HIP(hipGetLastError());
   g->deg_level[dir] = level;
@@ -266,7 +267,7 @@ const Csr& ensure_csr(ll_gpu_graph* g, uint32_t dir, int level) {
   ll_gpu_ctx* ctx = g->ctx;
   hipStream_t s = ctx->stream;
   const uint64_t V = g->V;
-  const VtEntry* vt = g->dir[dir].vt[level].as<VtEntry>();
+  const VtPages vt{g->dir[dir].vt_ptab[level].as<const VtEntry* const>()};
   const uint64_t* const* et = g->dir[dir].et_table.as<const uint64_t* const>();
   hipEvent_t e0, e1;
   LLG_HIP(hipEventCreate(&e0));

@@ -205,6 +205,21 @@ class HostGraph:
         N.check_host(N.host().ll_host_graph_level_view(self._h, direction, level, C.byref(v)))
         return v
 
+    def add_window_level(self, n_ins: int, n_del: int, lo: int, hi: int, seed: int) -> None:
+        """Freeze a seeded level confined to vertex IDs [lo, hi)."""
+        N.check_host(N.host().ll_host_graph_add_window_level(self._h, n_ins, n_del, lo, hi, seed))
+
+    def level_pages(self, direction: int, level: int):
+        """Copy-on-write pages of a level's vertex table: (ctypes array of host
+        page addresses, None where the page equals the previous level's;
+        number of written pages)."""
+        np_ = (self.num_nodes + 1 + 511) // 512
+        arr = (C.c_void_p * np_)()
+        n = C.c_uint64()
+        N.check_host(N.host().ll_host_graph_level_pages(self._h, direction, level, arr, np_,
+                                                        C.byref(n)))
+        return arr, int(n.value)
+
     def level_arrays(self, direction: int, level: int) -> LevelArrays:
         v = self.level_view(direction, level)
         vt_raw = (C.c_uint8 * (v.num_vt_entries * 16)).from_address(v.vt)
@@ -312,7 +327,10 @@ class GpuContext:
 class DeviceGraph:
     """HBM mirror of a HostGraph (ll_gpu_graph)."""
 
-    def __init__(self, ctx: GpuContext, host: HostGraph):
+    def __init__(self, ctx: GpuContext, host: HostGraph, cow: bool = False):
+        """cow: push levels after the first as copy-on-write vertex-table pages
+        (only the pages a level wrote travel), else whole tables."""
+        self.cow = cow
         self.ctx = ctx
         self.lib = ctx.lib
         self.host = host
@@ -328,16 +346,42 @@ class DeviceGraph:
         """Push every frozen host level not yet mirrored, with its patches."""
         have = self.lib.ll_gpu_graph_num_levels(self._h, DIR_OUT)
         for k in range(have, self.host.num_levels):
-            for d in (DIR_OUT, DIR_IN):
-                ids, ml = self.host.patches(d, k)
-                if ids.size:
-                    self.ctx.check(self.lib.ll_gpu_graph_patch_deletions(
-                        self._h, d, ids.ctypes.data_as(C.POINTER(C.c_uint64)),
-                        ml.ctypes.data_as(C.POINTER(C.c_uint32)), ids.size))
-                v = self.host.level_view(d, k)
-                self.ctx.check(self.lib.ll_gpu_graph_push_level(self._h, C.byref(v)))
-            w = self.host.level_weights(k)  # the level's `weight` property, if any
-            if w is not None:
+            self.push_prepared(self.prepare_level(k))
+
+    def prepare_level(self, k: int) -> list:
+        """Everything level k's push reads from the host graph (patches, level
+        views, copy-on-write page lists, weights), collected up front: the
+        push can then run on another thread while the host freezes level k+1
+        (frozen levels are not moved or rewritten by later freezes)."""
+        items = []
+        for d in (DIR_OUT, DIR_IN):
+            ids, ml = self.host.patches(d, k)
+            if ids.size:
+                items.append(("patch", d, ids, ml))
+            v = self.host.level_view(d, k)
+            pages = self.host.level_pages(d, k)[0] if self.cow else None
+            items.append(("level", v, pages))
+        w = self.host.level_weights(k)  # the level's `weight` property, if any
+        if w is not None:
+            items.append(("weights", k, w))
+        return items
+
+    def push_prepared(self, items: list) -> None:
+        for it in items:
+            if it[0] == "patch":
+                _, d, ids, ml = it
+                self.ctx.check(self.lib.ll_gpu_graph_patch_deletions(
+                    self._h, d, ids.ctypes.data_as(C.POINTER(C.c_uint64)),
+                    ml.ctypes.data_as(C.POINTER(C.c_uint32)), ids.size))
+            elif it[0] == "level":
+                _, v, pages = it
+                if pages is not None:
+                    self.ctx.check(self.lib.ll_gpu_graph_push_level_pages(
+                        self._h, C.byref(v), pages, len(pages)))
+                else:
+                    self.ctx.check(self.lib.ll_gpu_graph_push_level(self._h, C.byref(v)))
+            else:
+                _, k, w = it
                 self.ctx.check(self.lib.ll_gpu_graph_push_weights(
                     self._h, k, w.ctypes.data_as(C.POINTER(C.c_float)), w.size))
 
