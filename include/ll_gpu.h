@@ -104,6 +104,30 @@ int ll_gpu_graph_push_level(ll_gpu_graph* g, const ll_gpu_level_view* view);
  * NULL); level 0 has no shared pages. */
 int ll_gpu_graph_push_level_pages(ll_gpu_graph* g, const ll_gpu_level_view* view,
                                   const void* const* pages, uint64_t num_pages);
+/* Replaces: the LL_SLCSR configuration's single-level CSR (ll_slcsr,
+ * ll_slcsr.h:88-712; ll_common.h:280-297): one direction of the graph as
+ * begin[max_nodes+1] (the vertex table's adj_list_start, degree(n) =
+ * begin[n+1] - begin[n], :275-278) and values[num_edges] (the neighbour
+ * IDs, iterated begin[n]..begin[n+1]-1 in stored order, :300-350).  The graph
+ * must be created without LL_GPU_HAS_DELETIONS and hold no multi-level levels;
+ * its only snapshot level is 0.  Push the in-direction too for the tasks that
+ * read in-edges (the reference's needs_reverse_edges). */
+int ll_gpu_graph_push_slcsr(ll_gpu_graph* g, uint32_t dir, const int64_t* begin,
+                            const int64_t* values, uint64_t num_edges);
+/* Replaces: ll_mlcsr_ro_graph::set_min_level (LL_MIN_LEVEL,
+ * ll_mlcsr_graph.h:560-610 -> ll_mlcsr_sp.h:1801-1862): snapshots then visit
+ * only segments of levels >= m (iter_begin / descend, ll_mlcsr_sp.h:2021-2027,
+ * 2107-2117) and a vertex's degree is its live edges in the window (without
+ * deletions the reference's degree(n) = VT_L.degree - VT_(m-1).degree,
+ * ll_mlcsr_sp.h:1913-1925; the reference leaves the case with deletions
+ * unimplemented).  m never decreases. */
+int ll_gpu_graph_set_min_level(ll_gpu_graph* g, uint32_t m);
+/* Replaces: ll_mlcsr_ro_graph::delete_level (ll_mlcsr_sp.h:962-1000, allowed
+ * for level + 1 < min_level): frees the level's edge tables and weights in
+ * both directions, and its vertex-table pages unless a later level shares
+ * them.  The newest level stays.  A snapshot at a deleted level is refused;
+ * later deletion patches of its words are dropped (no window reaches them). */
+int ll_gpu_graph_delete_level(ll_gpu_graph* g, uint32_t level);
 /* Replaces: update_max_visible_level_lower_only on old levels
  * (ll_mlcsr_sp.h:849-863; ll_mlcsr_graph.h:1091-1108).  edge_ids use the
  * level<<38|index encoding (ll_mlcsr_helpers.h:77-79). */

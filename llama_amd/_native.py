@@ -46,6 +46,9 @@ GPU_SIGNATURES = {
     "ll_gpu_graph_destroy": (None, [vp]),
     "ll_gpu_graph_push_level": (C.c_int, [vp, P(LevelView)]),
     "ll_gpu_graph_push_level_pages": (C.c_int, [vp, P(LevelView), P(vp), u64]),
+    "ll_gpu_graph_set_min_level": (C.c_int, [vp, u32]),
+    "ll_gpu_graph_push_slcsr": (C.c_int, [vp, u32, P(i64), P(i64), u64]),
+    "ll_gpu_graph_delete_level": (C.c_int, [vp, u32]),
     "ll_gpu_graph_patch_deletions": (C.c_int, [vp, u32, P(u64), P(u32), u64]),
     "ll_gpu_graph_num_levels": (C.c_int, [vp, u32]),
     "ll_gpu_graph_max_nodes": (u64, [vp]),

@@ -6,6 +6,7 @@
 // pointers lets kernels follow a continuation record's edge ID
 // (level<<38 | index) into any older level, as iter_descend does
 // (ll_mlcsr_sp.h:2090-2134).
+#include <algorithm>
 #include <chrono>
 #include <cstring>
 
@@ -32,6 +33,22 @@ int guarded(ll_gpu_ctx* c, F&& f) {
     return fail(c, e);
   } catch (const std::bad_alloc&) {
     return fail(c, Error{LL_GPU_ENOMEM, "out of host memory"});
+  }
+}
+
+__global__ void k_zero_flag(uint32_t* p) { *p = 0; }
+
+// int64 neighbour IDs of a single-level CSR to the mirror's uint32 IDs.
+__global__ void k_narrow_ids(const int64_t* in, uint64_t n, uint64_t V, uint32_t* out,
+                             uint32_t* bad) {
+  const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int64_t x = in[i];
+  if (x < 0 || uint64_t(x) >= V) {
+    atomicOr(bad, 1u);
+    out[i] = 0;
+  } else {
+    out[i] = uint32_t(x);
   }
 }
 
@@ -93,11 +110,17 @@ TimedLaunch::~TimedLaunch() {
 }
 
 int resolve_level(const ll_gpu_graph* g, int level) {
+  if (g->slcsr) {
+    LLG_CHECK(level == -1 || level == 0, LL_GPU_EINVAL, "a single-level CSR has only level 0");
+    return 0;
+  }
   int L = int(g->dir[0].vt.size());
   LLG_CHECK(L > 0 && int(g->dir[1].vt.size()) == L, LL_GPU_ESTATE,
             "graph mirror needs the same number of out- and in-levels (>= 1)");
   if (level < 0) level = L - 1;
   LLG_CHECK(level < L, LL_GPU_EINVAL, "snapshot level beyond the pushed levels");
+  LLG_CHECK(size_t(level) >= g->level_deleted.size() || !g->level_deleted[level], LL_GPU_ESTATE,
+            "snapshot level was deleted (ll_gpu_graph_delete_level)");
   return level;
 }
 
@@ -308,6 +331,7 @@ int push_level(ll_gpu_graph* g, const ll_gpu_level_view* v, const void* const* p
   return guarded(c, [&]() {
     LLG_HIP(hipSetDevice(c->device));
     LLG_CHECK(v->dir <= 1, LL_GPU_EINVAL, "dir must be LL_GPU_DIR_OUT or LL_GPU_DIR_IN");
+    LLG_CHECK(!g->slcsr, LL_GPU_ESTATE, "the graph holds a single-level CSR");
     auto& d = g->dir[v->dir];
     LLG_CHECK(v->level == d.vt.size(), LL_GPU_ESTATE,
               "levels must be pushed in order (expected level " + std::to_string(d.vt.size()) + ")");
@@ -411,6 +435,119 @@ int ll_gpu_graph_push_level_pages(ll_gpu_graph* g, const ll_gpu_level_view* v,
   return push_level(g, v, pages, num_pages);
 }
 
+int ll_gpu_graph_push_slcsr(ll_gpu_graph* g, uint32_t dir, const int64_t* begin,
+                            const int64_t* values, uint64_t num_edges) {
+  if (!g) return LL_GPU_EINVAL;
+  ll_gpu_ctx* c = g->ctx;
+  return guarded(c, [&]() {
+    LLG_HIP(hipSetDevice(c->device));
+    LLG_CHECK(dir <= 1, LL_GPU_EINVAL, "dir must be LL_GPU_DIR_OUT or LL_GPU_DIR_IN");
+    LLG_CHECK(!(g->flags & LL_GPU_HAS_DELETIONS), LL_GPU_EINVAL,
+              "a single-level CSR has no deletions (LL_SLCSR excludes LL_DELETIONS)");
+    LLG_CHECK(g->dir[0].vt.empty() && g->dir[1].vt.empty(), LL_GPU_ESTATE,
+              "the graph already holds multi-level levels");
+    LLG_CHECK(!g->sl[dir].valid, LL_GPU_ESTATE, "this direction is already pushed");
+    LLG_CHECK(begin && (num_edges == 0 || values), LL_GPU_EINVAL, "NULL CSR arrays");
+    const uint64_t V = g->V;
+    LLG_CHECK(begin[0] == 0 && uint64_t(begin[V]) == num_edges, LL_GPU_EINVAL,
+              "begin[0] must be 0 and begin[max_nodes] the number of edges");
+    for (uint64_t v = 0; v < V; v++)
+      LLG_CHECK(begin[v + 1] >= begin[v], LL_GPU_EINVAL, "begin must not decrease");
+    llgpu::Csr cs;
+    cs.level = 0;
+    cs.V = V;
+    cs.E = cs.stored = num_edges;
+    cs.segments = 0;
+    cs.ofs.alloc((V + 1) * 8, c->stream);
+    upload(c, cs.ofs.p, begin, (V + 1) * 8);  // non-negative int64 = uint64 offsets
+    cs.adj.alloc(std::max<uint64_t>(num_edges, 1) * 4, c->stream);
+    if (num_edges) {
+      DevBuf wide, bad;
+      wide.alloc(num_edges * 8, c->stream);
+      bad.alloc(4, c->stream);
+      upload(c, wide.p, values, num_edges * 8);
+      k_zero_flag<<<1, 1, 0, c->stream>>>(bad.as<uint32_t>());
+      k_narrow_ids<<<llgpu::grid_for(num_edges, 256), 256, 0, c->stream>>>(
+          wide.as<int64_t>(), num_edges, V, cs.adj.as<uint32_t>(), bad.as<uint32_t>());
+      LLG_HIP(hipGetLastError());
+      uint32_t hbad = 0;
+      LLG_HIP(hipMemcpyAsync(&hbad, bad.p, 4, hipMemcpyDeviceToHost, c->stream));
+      LLG_HIP(hipStreamSynchronize(c->stream));
+      LLG_CHECK(hbad == 0, LL_GPU_EINVAL, "neighbour IDs must lie in [0, max_nodes)");
+    }
+    LLG_HIP(hipStreamSynchronize(c->stream));
+    cs.valid = true;
+    g->sl[dir] = std::move(cs);
+    g->slcsr = true;
+    g->upload_bytes += (V + 1) * 8 + num_edges * 8;
+    llgpu::drop_snapshots(g);
+  });
+}
+
+int ll_gpu_graph_set_min_level(ll_gpu_graph* g, uint32_t m) {
+  if (!g) return LL_GPU_EINVAL;
+  return guarded(g->ctx, [&]() {
+    LLG_CHECK(m >= g->min_level, LL_GPU_ERANGE, "the minimum level only moves forward");
+    g->min_level = m;
+    llgpu::drop_snapshots(g);
+  });
+}
+
+int ll_gpu_graph_delete_level(ll_gpu_graph* g, uint32_t level) {
+  if (!g) return LL_GPU_EINVAL;
+  ll_gpu_ctx* c = g->ctx;
+  return guarded(c, [&]() {
+    LLG_HIP(hipSetDevice(c->device));
+    LLG_CHECK(level + 1 < g->min_level, LL_GPU_ESTATE,
+              "only levels below min_level - 1 can be deleted");
+    LLG_CHECK(level + 1 < g->dir[0].vt.size(), LL_GPU_EINVAL,
+              "no such level, or the newest level (later pushes share its pages)");
+    if (g->level_deleted.size() < g->dir[0].vt.size()) g->level_deleted.resize(g->dir[0].vt.size());
+    if (g->level_deleted[level]) return;
+    for (auto& d : g->dir) {
+      // no snapshot inside the window walks into this level's edge table
+      d.et[level].release();
+      d.et_words[level] = 0;
+      if (level < d.wt.size()) d.wt[level].release();
+      // the level's own pages go unless a later level's page table shares them
+      const auto* lo = d.vt[level].as<const llgpu::VtEntry>();
+      const auto* hi = lo + d.vt[level].bytes / sizeof(llgpu::VtEntry);
+      bool shared = false;
+      for (size_t j = level + 1; j < d.vt_pages.size() && !shared; j++)
+        for (const llgpu::VtEntry* p : d.vt_pages[j])
+          if (p >= lo && p < hi) {
+            shared = true;
+            break;
+          }
+      if (!shared) d.vt[level].release();
+      std::vector<const uint64_t*> etab;
+      for (auto& b : d.et) etab.push_back(b.as<const uint64_t>());
+      llgpu::DevBuf t;
+      t.alloc(etab.size() * sizeof(void*), c->stream);
+      LLG_HIP(hipMemcpyAsync(t.p, etab.data(), etab.size() * sizeof(void*), hipMemcpyHostToDevice,
+                             c->stream));
+      d.et_table = std::move(t);
+      llgpu::DevBuf w;
+      w.alloc(d.et_words.size() * sizeof(uint64_t), c->stream);
+      LLG_HIP(hipMemcpyAsync(w.p, d.et_words.data(), d.et_words.size() * sizeof(uint64_t),
+                             hipMemcpyHostToDevice, c->stream));
+      d.et_words_dev = std::move(w);
+      if (!d.wt.empty()) {
+        std::vector<const float*> wtab(d.et.size(), nullptr);
+        for (size_t k = 0; k < d.wt.size(); k++) wtab[k] = d.wt[k].as<const float>();
+        llgpu::DevBuf wt;
+        wt.alloc(wtab.size() * sizeof(void*), c->stream);
+        LLG_HIP(hipMemcpyAsync(wt.p, wtab.data(), wtab.size() * sizeof(void*),
+                               hipMemcpyHostToDevice, c->stream));
+        d.wt_table = std::move(wt);
+      }
+    }
+    g->level_deleted[level] = 1;
+    LLG_HIP(hipStreamSynchronize(c->stream));  // the host tables are released on return
+    llgpu::drop_snapshots(g);
+  });
+}
+
 int ll_gpu_graph_push_weights(ll_gpu_graph* g, uint32_t level, const float* w, uint64_t n) {
   if (!g) return LL_GPU_EINVAL;
   ll_gpu_ctx* c = g->ctx;
@@ -453,11 +590,29 @@ int ll_gpu_graph_patch_deletions(ll_gpu_graph* g, uint32_t dir, const uint64_t* 
     LLG_CHECK(g->flags & LL_GPU_HAS_DELETIONS, LL_GPU_ENOTSUP,
               "graph was created without deletions");
     auto& d = g->dir[dir];
+    // words of deleted levels are outside every snapshot's window: their
+    // patches change nothing and are dropped
+    std::vector<uint64_t> keep_ids;
+    std::vector<uint32_t> keep_ml;
+    const bool any_deleted =
+        std::find(g->level_deleted.begin(), g->level_deleted.end(), 1) != g->level_deleted.end();
     for (uint64_t i = 0; i < n; i++) {
       uint32_t l = llgpu::edge_level(ids[i]);
-      LLG_CHECK(l < d.et.size() && llgpu::edge_index(ids[i]) < d.et_words[l], LL_GPU_EINVAL,
-                "patch edge ID outside the pushed levels");
+      LLG_CHECK(l < d.et.size(), LL_GPU_EINVAL, "patch edge ID outside the pushed levels");
       LLG_CHECK(maxlev[i] <= llgpu::kLevelMask, LL_GPU_EINVAL, "patch level out of range");
+      if (l < g->level_deleted.size() && g->level_deleted[l]) continue;
+      LLG_CHECK(llgpu::edge_index(ids[i]) < d.et_words[l], LL_GPU_EINVAL,
+                "patch edge ID outside the pushed levels");
+      if (any_deleted) {
+        keep_ids.push_back(ids[i]);
+        keep_ml.push_back(maxlev[i]);
+      }
+    }
+    if (any_deleted) {
+      ids = keep_ids.data();
+      maxlev = keep_ml.data();
+      n = keep_ids.size();
+      if (n == 0) return;
     }
     LLG_HIP(hipSetDevice(c->device));
     DevBuf di, dm;

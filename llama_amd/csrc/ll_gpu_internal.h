@@ -202,6 +202,11 @@ struct ll_gpu_graph {
   uint32_t flags = 0;
   llgpu::DirMirror dir[2];
   uint64_t upload_bytes = 0;
+  uint32_t min_level = 0;              // LL_MIN_LEVEL window (ll_gpu_graph_set_min_level)
+  // LL_SLCSR graphs (ll_gpu_graph_push_slcsr): one level given as a plain CSR
+  bool slcsr = false;
+  llgpu::Csr sl[2];
+  std::vector<char> level_deleted;     // per level: storage freed (ll_gpu_graph_delete_level)
   double last_upload_ms = 0;
   // snapshot caches
   int deg_level[2] = {-1, -1};

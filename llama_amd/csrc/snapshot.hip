@@ -19,8 +19,9 @@
 //   5. scan of the block counts; second pass writes the live payloads with a
 //      block-wide ordered compaction (wave ballots).          k_compact
 // Without deletions every stored word is live and step 4 is skipped.
-// The row offsets are the scan of the precomputed VT_L degrees; the live-word
-// total is checked against them.
+// The row offsets are the scan of the precomputed VT_L degrees (inside an
+// LL_MIN_LEVEL window, of the window's degrees); the live-word total is
+// checked against them.
 #include "ll_gpu_internal.h"
 
 namespace llgpu {
@@ -33,6 +34,11 @@ constexpr int kWaves = kBlock / 64;
 
 __global__ void k_zero_u32(uint32_t* p) { *p = 0; }
 
+__global__ void k_ofs_degrees(const uint64_t* ofs, uint32_t* deg, uint64_t V) {
+  uint64_t v = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (v < V) deg[v] = uint32_t(ofs[v + 1] - ofs[v]);
+}
+
 __global__ void k_degrees(VtPages vt, uint32_t* deg, uint64_t V) {
   uint64_t v = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (v < V) deg[v] = vt[v].degree;
@@ -44,14 +50,15 @@ __global__ void k_degrees(VtPages vt, uint32_t* deg, uint64_t V) {
 // and stops the walk instead of reading out of bounds.
 template <typename F>
 __device__ inline void walk_chain(VtPages vt, const uint64_t* const* et,
-                                  const uint64_t* et_words, uint32_t level, uint64_t v,
-                                  uint32_t* bad, F&& f) {
+                                  const uint64_t* et_words, uint32_t level, uint32_t min_level,
+                                  uint64_t v, uint32_t* bad, F&& f) {
   VtEntry e = vt[v];
   int64_t start = e.start;
   uint32_t len = e.len;
   uint32_t kprev = level + 1;
   while (start != -1 && len > 0) {
     uint32_t k = edge_level(uint64_t(start));
+    if (k < min_level) return;  // LL_MIN_LEVEL window (ll_mlcsr_sp.h:2021-2027, 2107-2111)
     uint64_t idx = edge_index(uint64_t(start));
     if (k >= kprev || idx + len + (k > 0 ? 2 : 0) > et_words[k]) {
       atomicOr(bad, 1u);
@@ -66,14 +73,37 @@ __device__ inline void walk_chain(VtPages vt, const uint64_t* const* et,
   }
 }
 
+// Degrees inside an LL_MIN_LEVEL window: a vertex's live words in the
+// segments of levels >= min_level (without deletions, the reference's
+// degree(n) = VT_L.degree - VT_(min-1).degree, ll_mlcsr_sp.h:1913-1925; with
+// deletions the reference leaves it unimplemented and the live words are
+// counted).
+__global__ void k_window_degrees(VtPages vt, const uint64_t* const* et, const uint64_t* et_words,
+                                 uint32_t level, uint32_t min_level, bool deletions, uint64_t V,
+                                 uint32_t* deg, uint32_t* bad) {
+  uint64_t v = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (v >= V) return;
+  uint32_t d = 0;
+  walk_chain(vt, et, et_words, level, min_level, v, bad, [&](uint64_t start, uint32_t len) {
+    if (!deletions) {
+      d += len;
+      return;
+    }
+    const uint64_t* w = et[edge_level(start)] + edge_index(start);
+    for (uint32_t j = 0; j < len; j++) d += word_maxlev(w[j]) > level ? 1u : 0u;
+  });
+  deg[v] = d;
+}
+
 __global__ void k_seg_count(VtPages vt, const uint64_t* const* et,
-                            const uint64_t* et_words, uint32_t level, uint64_t V,
+                            const uint64_t* et_words, uint32_t level, uint32_t min_level,
+                            uint64_t V,
                             uint32_t* nseg, uint64_t* nwords, uint32_t* bad) {
   uint64_t v = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (v >= V) return;
   uint32_t cnt = 0;
   uint64_t words = 0;
-  walk_chain(vt, et, et_words, level, v, bad, [&](uint64_t, uint32_t len) {
+  walk_chain(vt, et, et_words, level, min_level, v, bad, [&](uint64_t, uint32_t len) {
     cnt++;
     words += len;
   });
@@ -82,13 +112,14 @@ __global__ void k_seg_count(VtPages vt, const uint64_t* const* et,
 }
 
 __global__ void k_seg_fill(VtPages vt, const uint64_t* const* et,
-                           const uint64_t* et_words, uint32_t level, uint64_t V,
+                           const uint64_t* et_words, uint32_t level, uint32_t min_level,
+                           uint64_t V,
                            const uint64_t* seg_base, const uint64_t* word_base,
                            uint64_t* seg_addr, uint64_t* seg_vofs, uint32_t* bad) {
   uint64_t v = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (v >= V) return;
   uint64_t s = seg_base[v], w = word_base[v];
-  walk_chain(vt, et, et_words, level, v, bad, [&](uint64_t start, uint32_t len) {
+  walk_chain(vt, et, et_words, level, min_level, v, bad, [&](uint64_t start, uint32_t len) {
     seg_addr[s] = start;
     seg_vofs[s] = w;
     s++;
@@ -248,11 +279,33 @@ const uint32_t* ensure_degrees(ll_gpu_graph* g, uint32_t dir, int level) {
   if (g->deg_level[dir] == level) return g->deg[dir].as<uint32_t>();
   hipStream_t s = g->ctx->stream;
   g->deg[dir].alloc(g->V * sizeof(uint32_t), s);
+  if (g->slcsr) {  // LL_SLCSR degree(n) = begin[n+1] - begin[n] (ll_slcsr.h:275-278)
+    const Csr& c = ensure_csr(g, dir, level);
+    k_ofs_degrees<<<grid_for(g->V, 256), 256, 0, s>>>(c.ofs.as<uint64_t>(),
+                                                      g->deg[dir].as<uint32_t>(), g->V);
+    LLG_HIP(hipGetLastError());
+    g->deg_level[dir] = level;
+    return g->deg[dir].as<uint32_t>();
+  }
   {
     TimedLaunch t(g->ctx, "k_degrees");
-    k_degrees<<<grid_for(g->V, 256), 256, 0, s>>>(
-        VtPages{g->dir[dir].vt_ptab[level].as<const VtEntry* const>()}, g->deg[dir].as<uint32_t>(),
-        g->V);
+    const VtPages vt{g->dir[dir].vt_ptab[level].as<const VtEntry* const>()};
+    if (g->min_level == 0) {
+      k_degrees<<<grid_for(g->V, 256), 256, 0, s>>>(vt, g->deg[dir].as<uint32_t>(), g->V);
+    } else {
+      DevBuf bad;
+      bad.alloc(4, s);
+      k_zero_u32<<<1, 1, 0, s>>>(bad.as<uint32_t>());
+      k_window_degrees<<<grid_for(g->V, 256), 256, 0, s>>>(
+          vt, g->dir[dir].et_table.as<const uint64_t* const>(),
+          g->dir[dir].et_words_dev.as<const uint64_t>(), uint32_t(level), g->min_level,
+          (g->flags & LL_GPU_HAS_DELETIONS) != 0, g->V, g->deg[dir].as<uint32_t>(),
+          bad.as<uint32_t>());
+      uint32_t hbad = 0;
+      LLG_HIP(hipMemcpyAsync(&hbad, bad.p, 4, hipMemcpyDeviceToHost, s));
+      LLG_HIP(hipStreamSynchronize(s));
+      LLG_CHECK(hbad == 0, LL_GPU_ESTATE, "corrupt mirror: a segment chain leaves its bounds");
+    }
   }
   LLG_HIP(hipGetLastError());
   g->deg_level[dir] = level;
@@ -261,6 +314,11 @@ const uint32_t* ensure_degrees(ll_gpu_graph* g, uint32_t dir, int level) {
 
 const Csr& ensure_csr(ll_gpu_graph* g, uint32_t dir, int level) {
   level = resolve_level(g, level);
+  if (g->slcsr) {
+    LLG_CHECK(g->sl[dir].valid, LL_GPU_ESTATE,
+              "this direction of the single-level CSR is not pushed");
+    return g->sl[dir];
+  }
   Csr& c = g->csr[dir];
   if (c.valid && c.level == level) return c;
   c = Csr();
@@ -283,7 +341,8 @@ const Csr& ensure_csr(ll_gpu_graph* g, uint32_t dir, int level) {
   k_zero_u32<<<1, 1, 0, s>>>(bad.as<uint32_t>());
   {
     TimedLaunch t(ctx, "k_seg_count");
-    k_seg_count<<<grid_for(V, 256), 256, 0, s>>>(vt, et, et_words, uint32_t(level), V,
+    k_seg_count<<<grid_for(V, 256), 256, 0, s>>>(vt, et, et_words, uint32_t(level),
+                                                 g->min_level, V,
                                                  nseg.as<uint32_t>(), nwords.as<uint64_t>(),
                                                  bad.as<uint32_t>());
   }
@@ -308,7 +367,8 @@ const Csr& ensure_csr(ll_gpu_graph* g, uint32_t dir, int level) {
   seg_vofs.alloc((S + 1) * 8, s);
   {
     TimedLaunch t(ctx, "k_seg_fill");
-    k_seg_fill<<<grid_for(V, 256), 256, 0, s>>>(vt, et, et_words, uint32_t(level), V,
+    k_seg_fill<<<grid_for(V, 256), 256, 0, s>>>(vt, et, et_words, uint32_t(level),
+                                                g->min_level, V,
                                                 seg_base.as<uint64_t>(), word_base.as<uint64_t>(),
                                                 seg_addr.as<uint64_t>(), seg_vofs.as<uint64_t>(),
                                                 bad.as<uint32_t>());

@@ -389,6 +389,14 @@ class DeviceGraph:
     def num_levels(self) -> int:
         return int(self.lib.ll_gpu_graph_num_levels(self._h, DIR_OUT))
 
+    def set_min_level(self, m: int) -> None:
+        """LL_MIN_LEVEL window: snapshots visit only segments of levels >= m."""
+        self.ctx.check(self.lib.ll_gpu_graph_set_min_level(self._h, m))
+
+    def delete_level(self, level: int) -> None:
+        """Free a level below the window (level + 1 < min level)."""
+        self.ctx.check(self.lib.ll_gpu_graph_delete_level(self._h, level))
+
     def drop_snapshots(self) -> None:
         self.lib.ll_gpu_graph_drop_snapshots(self._h)
 
@@ -541,3 +549,40 @@ class DeviceGraph:
 
     def __del__(self):
         self.close()
+
+
+class SlcsrGraph(DeviceGraph):
+    """HBM mirror of a single-level CSR (the LL_SLCSR configuration): one
+    (begin[V+1], values[E]) pair per direction, snapshot level 0 only."""
+
+    def __init__(self, ctx: GpuContext, num_nodes: int, out_csr, in_csr=None):
+        self.cow = False
+        self.ctx = ctx
+        self.lib = ctx.lib
+        self.host = None
+        h = C.c_void_p()
+        ctx.check(self.lib.ll_gpu_graph_create(ctx._h, num_nodes, 0, C.byref(h)))
+        self._h = h
+        self.V = num_nodes
+        ctx._graphs.add(self)
+        for d, csr in ((DIR_OUT, out_csr), (DIR_IN, in_csr)):
+            if csr is None:
+                continue
+            b = np.ascontiguousarray(csr[0], dtype=np.int64)
+            v = np.ascontiguousarray(csr[1], dtype=np.int64)
+            ctx.check(self.lib.ll_gpu_graph_push_slcsr(
+                self._h, d, b.ctypes.data_as(C.POINTER(C.c_int64)),
+                v.ctypes.data_as(C.POINTER(C.c_int64)), v.size))
+
+    def sync_levels(self) -> None:
+        pass
+
+
+def csr_of(num_nodes: int, src, dst):
+    """(begin, values) of the edges grouped by src in the given order."""
+    src = np.asarray(src, dtype=np.int64)
+    dst = np.asarray(dst, dtype=np.int64)
+    order = np.argsort(src, kind="stable")
+    begin = np.zeros(num_nodes + 1, np.int64)
+    np.add.at(begin, src + 1, 1)
+    return np.cumsum(begin), dst[order]

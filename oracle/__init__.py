@@ -59,6 +59,7 @@ def lib() -> C.CDLL:
             "orc_graph_add_weights": (C.c_int, [vp, u32, P(C.c_float), u64]),
             "orc_sssp_weighted": (i64, [vp, C.c_int, i64, P(C.c_float), P(f64), P(i64)]),
             "orc_bc": (C.c_int, [vp, C.c_int, P(i64), i64, P(C.c_float)]),
+            "orc_graph_set_min_level": (C.c_int, [vp, u32]),
             "orc_num_threads": (C.c_int, []),
         }
         for k, (r, a) in sig.items():
@@ -103,6 +104,11 @@ class OracleGraph:
         o = cls(hg.num_nodes, hg.deletions, levels)
         o._host = hg  # keep the borrowed arrays alive
         return o
+
+    def set_min_level(self, m: int) -> None:
+        """LL_MIN_LEVEL window: later snapshots skip segments of levels < m."""
+        if lib().orc_graph_set_min_level(self._h, m) != 0:
+            raise ValueError("the minimum level only moves forward")
 
     def flatten(self, direction: int, level: int = -1):
         n = lib().orc_flatten(self._h, direction, level, None, None)

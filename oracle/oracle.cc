@@ -65,6 +65,7 @@ struct Lvl {
 struct orc_graph {
   uint64_t V;
   int deletions;
+  uint32_t min_level = 0;  // LL_MIN_LEVEL window: segments of older levels are not visited
   std::vector<Lvl> lv[2];
   std::vector<const float*> wt;  // out-direction `weight` property per level (by edge index)
   std::vector<uint64_t> n_wt;
@@ -101,7 +102,10 @@ void iter_descend(const orc_graph* g, int dir, Iter& it) {
   const uint64_t* c = it.ptr + 1;
   it.edge = int64_t(c[0]);
   it.left = uint32_t(c[1]);
-  if (it.left == 0) {
+  if (level_of(it.edge) < g->min_level) {  // LL_MIN_LEVEL, ll_mlcsr_sp.h:2107-2111
+    it.left = 0;
+    it.edge = -1;
+  } else if (it.left == 0) {
     it.edge = -1;
   } else {
     it.ptr = g->lv[dir][level_of(it.edge)].et + index_of(it.edge);
@@ -132,7 +136,10 @@ void iter_begin(const orc_graph* g, int dir, int level, uint64_t n, Iter& it) {
   it.left = b.len;
   it.last_node = -1;
   it.ptr = nullptr;
-  if (it.left == 0) {
+  if (level_of(it.edge) < g->min_level) {  // LL_MIN_LEVEL, ll_mlcsr_sp.h:2021-2027
+    it.left = 0;
+    it.edge = -1;
+  } else if (it.left == 0) {
     it.edge = -1;
   } else {
     it.ptr = g->lv[dir][level_of(it.edge)].et + index_of(it.edge);
@@ -162,7 +169,15 @@ inline void foreach_nbr(const orc_graph* g, int dir, int level, uint64_t n, F&& 
 }
 
 inline uint32_t degree(const orc_graph* g, int dir, int level, uint64_t n) {
-  return g->lv[dir][level].vt[n].degree;
+  if (g->min_level == 0) return g->lv[dir][level].vt[n].degree;
+  // LL_MIN_LEVEL degree(n), ll_mlcsr_sp.h:1913-1925: the stored degree less the
+  // one below the window; with deletions (unimplemented there) and for a
+  // snapshot below the window, the live edges the window shows
+  if (!g->deletions && int(g->min_level) - 1 <= level)
+    return g->lv[dir][level].vt[n].degree - g->lv[dir][g->min_level - 1].vt[n].degree;
+  uint32_t d = 0;
+  foreach_nbr(g, dir, level, n, [&](uint64_t) { d++; });
+  return d;
 }
 
 int resolve(const orc_graph* g, int level) {
@@ -609,6 +624,14 @@ int orc_bc(const orc_graph* g, int level, const int64_t* sources, int64_t n, flo
       bc[v] = bc[v] + delta[v];
     }
   }
+  return 0;
+}
+
+// LL_MIN_LEVEL: snapshots visit only segments of levels >= m (set_min_level,
+// ll_mlcsr_sp.h:1801-1862) and degrees count the window (degree above).
+int orc_graph_set_min_level(orc_graph* g, uint32_t m) {
+  if (m < g->min_level) return -1;  // the window only moves forward (:1806)
+  g->min_level = m;
   return 0;
 }
 
