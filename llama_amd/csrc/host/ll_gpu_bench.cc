@@ -51,7 +51,7 @@ const Entry kTasks[] = {
 void usage() {
   std::fprintf(stderr,
                "Usage: ll_gpu_bench -r TASK [-n ITERS] [-R ROOT] [-c COUNT] [-o FILE]\n"
-               "                    [--sources N] [--lanes W] [--tol T]\n"
+               "                    [-l MIN-[MAX]] [--sources N] [--lanes W] [--tol T]\n"
                "                    (FILE.xs1|FILE.net ... | --rmat SCALE [--levels L] [--seed S])\n"
                "Tasks:\n");
   for (const Entry& e : kTasks) std::fprintf(stderr, "  %-22s %s\n", e.id, e.name);
@@ -72,7 +72,9 @@ int main(int argc, char** argv) {
   double tol = 1e-7;
   unsigned long seed = 1;
   const char* out_file = nullptr;
+  int min_level = 0, max_level = -1;
   static option opts[] = {{"run", required_argument, 0, 'r'},
+                          {"levels-window", required_argument, 0, 'l'},
                           {"num-iters", required_argument, 0, 'n'},
                           {"root", required_argument, 0, 'R'},
                           {"count", required_argument, 0, 'c'},
@@ -86,13 +88,23 @@ int main(int argc, char** argv) {
                           {"help", no_argument, 0, 'h'},
                           {0, 0, 0, 0}};
   int c;
-  while ((c = getopt_long(argc, argv, "r:n:R:c:o:h", opts, nullptr)) != -1) {
+  while ((c = getopt_long(argc, argv, "r:n:R:c:o:l:h", opts, nullptr)) != -1) {
     switch (c) {
       case 'r': task = optarg; break;
       case 'n': iters = std::atoi(optarg); break;
       case 'R': root = std::atol(optarg); break;
       case 'c': count = std::atoi(optarg); break;
       case 'o': out_file = optarg; break;
+      case 'l': {  // "MAX" or "MIN-MAX" (benchmark.cc:1239-1250); MAX may be empty
+        const char* dash = std::strchr(optarg, '-');
+        if (!dash) {
+          max_level = std::atoi(optarg);
+        } else {
+          min_level = std::atoi(optarg);
+          if (dash[1]) max_level = std::atoi(dash + 1);
+        }
+        break;
+      }
       case 1: sources = std::atoi(optarg), sources_set = true; break;
       case 2: lanes = std::atoi(optarg); break;
       case 3: scale = std::atoi(optarg); break;
@@ -137,7 +149,9 @@ int main(int argc, char** argv) {
     }
     std::vector<ll_host_edges*> files;
     uint64_t V = 0;
-    for (int i = optind; i < argc; i++) {
+    // -l ...-MAX: the levels 0..MAX, one per file
+    const int last = max_level >= 0 ? std::min(argc, optind + max_level + 1) : argc;
+    for (int i = optind; i < last; i++) {
       ll_host_edges* e = nullptr;
       uint64_t n = 0;
       const std::string path = argv[i];
@@ -168,8 +182,9 @@ int main(int argc, char** argv) {
     }
   }
   const Graph& G = llhost::graph_of(hg);
-  std::printf("LOAD\nLoad Time  : %.3f seconds\nLevels     : %d\nNodes      : %lu\n\n",
-              (now_ms() - t0) / 1000.0, G.num_levels(), (unsigned long)G.num_nodes());
+  std::printf("LOAD\nLoad Time  : %.3f seconds\nLevels     : %d--%d\nNodes      : %lu\n\n",
+              (now_ms() - t0) / 1000.0, min_level, G.num_levels() - 1,
+              (unsigned long)G.num_nodes());
 
   std::vector<int64_t> roots;
   for (int i = 0; i < sources; i++)
@@ -188,6 +203,13 @@ int main(int argc, char** argv) {
   else if (task == "gpu_bfs_batch") t.reset(new gpu_bfs_batch<Graph>(roots, lanes));
   else t.reset(new gpu_ppr_batch<Graph>(roots, iters < 0 ? 30 : iters, 0.85, tol, lanes));
 
+  if (min_level > 0) {
+    if (min_level >= G.num_levels()) {
+      std::fprintf(stderr, "Error: -l %d-: the graph has %d levels\n", min_level, G.num_levels());
+      return 1;
+    }
+    Mirror::of(&G, G.num_nodes(), ll_gpu_flags(G)).set_window(uint32_t(min_level));
+  }
   std::vector<double> times;
   double ret = NAN;
   Graph& Gm = const_cast<Graph&>(G);

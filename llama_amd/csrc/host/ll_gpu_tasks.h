@@ -81,7 +81,29 @@ class Mirror {
       if (ll_gpu_weights(G, k, &w, &n))
         check(ll_gpu_graph_push_weights(g_, uint32_t(k), w, n), "ll_gpu_graph_push_weights");
     }
+    apply_window();
   }
+
+  // LL_MIN_LEVEL window, as the reference benchmark's `-l min-max` applies it
+  // (benchmark.cc:1684-1692): set_min_level(m), then delete the levels i with
+  // i + 1 < m.  Applied on the next refresh once those levels are mirrored.
+  void set_window(uint32_t m) {
+    window_ = m;
+    apply_window();
+  }
+
+ private:
+  void apply_window() {
+    if (window_ == 0) return;
+    const int L = ll_gpu_graph_num_levels(g_, LL_GPU_DIR_OUT);
+    if (L <= 0) return;
+    check(ll_gpu_graph_set_min_level(g_, window_), "ll_gpu_graph_set_min_level");
+    for (uint32_t i = deleted_; i + 1 < window_ && int(i) + 1 < L; i++, deleted_++)
+      check(ll_gpu_graph_delete_level(g_, i), "ll_gpu_graph_delete_level");
+  }
+  uint32_t window_ = 0, deleted_ = 0;
+
+ public:
 
  private:
   Mirror(uint64_t max_nodes, uint32_t flags) {

@@ -3,6 +3,8 @@ ll_gpu_graph_delete_level) against the oracle's window: adjacency
 bit-exact, PageRank to the FP32 bar, BFS levels exact, also after the levels
 below the window are deleted and on a copy-on-write mirror whose pages are
 shared with deleted levels."""
+from pathlib import Path
+
 import numpy as np
 import pytest
 
@@ -73,3 +75,35 @@ def test_window_on_cow_mirror_with_shared_pages(ctx):
     o2.set_min_level(4)
     dg.sync_levels()
     check(dg, o2, [g.num_levels - 1])
+
+
+def test_cli_window_option(tmp_path):
+    """ll_gpu_bench -l MIN-: the window the reference benchmark applies with
+    -l (set_min_level, then delete_level below it), on .xs1 levels."""
+    import subprocess
+    from llama_amd.graph import Edges, HostGraph
+    gpu_case("ll_gpu_bench -l window")
+    root = Path(__file__).resolve().parent.parent
+    exe = root / "llama_amd" / "ll_gpu_bench"
+    g, batches = batched_graph(200, 4, seed=8, deletions=False)
+    files = []
+    for k, (ins, _) in enumerate(batches):
+        f = str(tmp_path / f"l{k}.xs1")
+        Edges.from_arrays([e[0] for e in ins], [e[1] for e in ins]).write_xs1(f, 200)
+        files.append(f)
+    out = str(tmp_path / "pr.txt")
+    r = subprocess.run([str(exe), "-r", "gpu_pagerank", "-n", "10", "-l", "2-", "-o", out] + files,
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    h = HostGraph(200, deletions=False)
+    for ins, _ in batches:
+        h.add_level([e[0] for e in ins], [e[1] for e in ins])
+    o = oracle.OracleGraph.from_host(h)
+    o.set_min_level(2)
+    ref, _ = o.pagerank(10)
+    vals = []
+    for line in open(out):
+        if ":" in line:
+            vals += [float(x) for x in line.split(":")[1].split()]
+    assert np.allclose(vals, ref[:50], atol=1e-8)
+    assert "Levels     : 2--3" in r.stdout
