@@ -7,7 +7,8 @@ satisfy the two edge conditions that characterise BFS distances; SSSP
 distances are the FP32 fixed point on every edge; PageRank agrees between the
 pull (FP32 compensated) and push (FP64 atomics) paths; Brandes dependencies
 sum to the number of interior vertices of the shortest paths; every lane of
-the batched BFS equals the single-source run."""
+the batched BFS equals the single-source run; SCC labels equal scipy's strong
+components (canonical smallest-ID labels)."""
 import numpy as np
 import pytest
 
@@ -110,3 +111,20 @@ def test_full_batched_bfs_lanes(big):
     for i, r in enumerate(roots):
         lv, n, _ = dg.bfs(r)
         assert np.array_equal(lvs[i], lv) and int(reached[i]) == n
+
+
+def test_full_scc_matches_scipy(big):
+    g, dg, ofs, adj, _, _ = big
+    gpu_case("full size: SCC vs scipy strong components")
+    from scipy.sparse import csr_matrix
+    from scipy.sparse.csgraph import connected_components
+    V = g.num_nodes
+    A = csr_matrix((np.ones(adj.size, np.int8), adj.astype(np.int32), ofs.astype(np.int64)),
+                   shape=(V, V))
+    n, lab = connected_components(A, directed=True, connection="strong")
+    # canonical labels: the smallest vertex ID of each component
+    low = np.full(n, V, np.int64)
+    np.minimum.at(low, lab, np.arange(V, dtype=np.int64))
+    mine, count = dg.scc()
+    assert count == n
+    assert np.array_equal(mine, low[lab])
