@@ -629,9 +629,8 @@ int autotune_lanes(ll_gpu_graph* g, int level, const int64_t* roots, int n) {
   d_roots.alloc(size_t(m) * 8, s);
   d_reached.alloc(size_t(m) * 8, s);
   LLG_HIP(hipMemcpyAsync(d_roots.p, roots, size_t(m) * 8, hipMemcpyHostToDevice, s));
-  hipEvent_t e0, e1;
-  LLG_HIP(hipEventCreate(&e0));
-  LLG_HIP(hipEventCreate(&e1));
+  llgpu::EventPair ev;
+  hipEvent_t e0 = ev.a, e1 = ev.b;
   int best = 64;
   double best_t = 1e300;
   for (int w : {16, 32, 64, 128}) {
@@ -652,8 +651,6 @@ int autotune_lanes(ll_gpu_graph* g, int level, const int64_t* roots, int n) {
     const double est = double(ms) * double((n + w - 1) / w);
     if (est < best_t) { best_t = est; best = w; }
   }
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
   g->ctx->tuned[key] = best;
   return best;
 }

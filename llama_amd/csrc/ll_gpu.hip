@@ -362,9 +362,8 @@ int push_level(ll_gpu_graph* g, const ll_gpu_level_view* v, const void* const* p
         ents.push_back(valid(p));
       }
     }
-    hipEvent_t e0, e1;
-    LLG_HIP(hipEventCreate(&e0));
-    LLG_HIP(hipEventCreate(&e1));
+    llgpu::EventPair ev;
+    hipEvent_t e0 = ev.a, e1 = ev.b;
     LLG_HIP(hipEventRecord(e0, c->stream));
     DevBuf vt, et, ptab;
     const size_t et_bytes = size_t(v->num_et_words) * sizeof(uint64_t);
@@ -408,8 +407,6 @@ int push_level(ll_gpu_graph* g, const ll_gpu_level_view* v, const void* const* p
     LLG_HIP(hipEventSynchronize(e1));  // host arrays and the tables are released on return
     float ms = 0;
     LLG_HIP(hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     g->last_upload_ms = ms;
     g->upload_bytes += vt_bytes + et_bytes;
     llgpu::drop_snapshots(g);

@@ -64,6 +64,26 @@ struct Error {
   } while (0)
 
 // Stream-ordered device buffer (hipMallocAsync / hipFreeAsync on the context stream).
+// Two timing events, destroyed on scope exit (also when a helper throws).
+struct EventPair {
+  hipEvent_t a = nullptr, b = nullptr;
+  EventPair() {
+    LLG_HIP(hipEventCreate(&a));
+    const hipError_t e = hipEventCreate(&b);
+    if (e != hipSuccess) {
+      (void)hipEventDestroy(a);
+      a = nullptr;
+      LLG_HIP(e);
+    }
+  }
+  EventPair(const EventPair&) = delete;
+  EventPair& operator=(const EventPair&) = delete;
+  ~EventPair() {
+    if (a) (void)hipEventDestroy(a);
+    if (b) (void)hipEventDestroy(b);
+  }
+};
+
 struct DevBuf {
   void* p = nullptr;
   size_t bytes = 0;

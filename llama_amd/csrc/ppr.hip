@@ -322,9 +322,8 @@ double ppr_run(ll_gpu_graph* g, int level, const int64_t* seeds, int n, int iter
   PprPass pass;
   ppr_capture<B>(g, in, rb, b, iters, d, tol, pass);
   int done_max = 0;
-  hipEvent_t e0, e1;
-  LLG_HIP(hipEventCreate(&e0));
-  LLG_HIP(hipEventCreate(&e1));
+  llgpu::EventPair ev;
+  hipEvent_t e0 = ev.a, e1 = ev.b;
   double ms_total = 0;
   int batches = 0;
   for (int c0 = 0; c0 < n && batches < max_batches; c0 += B, batches++) {
@@ -350,8 +349,6 @@ double ppr_run(ll_gpu_graph* g, int level, const int64_t* seeds, int n, int iter
       LLG_HIP(hipStreamSynchronize(s));
     }
   }
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
   if (ctx->timer.on) {
     auto& acc = ctx->timer.acc["ppr_graph"];
     acc.first += batches;

@@ -327,9 +327,8 @@ const Csr& ensure_csr(ll_gpu_graph* g, uint32_t dir, int level) {
   const uint64_t V = g->V;
   const VtPages vt{g->dir[dir].vt_ptab[level].as<const VtEntry* const>()};
   const uint64_t* const* et = g->dir[dir].et_table.as<const uint64_t* const>();
-  hipEvent_t e0, e1;
-  LLG_HIP(hipEventCreate(&e0));
-  LLG_HIP(hipEventCreate(&e1));
+  llgpu::EventPair ev;
+  hipEvent_t e0 = ev.a, e1 = ev.b;
   LLG_HIP(hipEventRecord(e0, s));
 
   // 1-3: segments
@@ -445,8 +444,6 @@ const Csr& ensure_csr(ll_gpu_graph* g, uint32_t dir, int level) {
   LLG_HIP(hipEventSynchronize(e1));
   float ms = 0;
   LLG_HIP(hipEventElapsedTime(&ms, e0, e1));
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
   c.valid = true;
   c.level = level;
   c.V = V;
