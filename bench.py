@@ -368,6 +368,112 @@ def iteration_ms(ctx, iterations):
     return sum(ctx.kernel_time(k)[1] for k in PR_ITERATION_KERNELS) / max(iterations, 1)
 
 
+def bench_large(args) -> int:
+    """The large single-GPU configuration: R-MAT s27 EF16 plus 20 levels of
+    +0.5% / -0.05%, in- and out-CSR resident in HBM; each level's host freeze
+    and upload timed separately from traversal; PageRank 20 iterations, DO-BFS
+    from 16 sources and a 64-lane batched BFS.  Prints its own JSON line."""
+    import numpy as np
+
+    from llama_amd.graph import DeviceGraph, Edges, GpuContext, HostGraph
+    scale, levels = args.large_scale, args.large_levels
+    t0 = time.time()
+    log(f"large: generating R-MAT s{scale} EF16")
+    base = Edges.rmat(scale, 16.0, seed=args.seed, perm_seed=args.seed + 1)
+    g = HostGraph(1 << scale, deletions=True)
+    g.add_level_edges(base)
+    del base
+    log(f"large: base level frozen in {time.time() - t0:.1f}s")
+    ctx = GpuContext(0)
+    t1 = time.time()
+    dg = DeviceGraph(ctx, g)
+    ctx.sync()
+    b0 = dg.upload_stats()[0]
+    log(f"large: base level uploaded, {b0 / 1e9:.2f} GB in {time.time() - t1:.2f}s")
+    freeze, upload, up_bytes = [], [], []
+    e_expect = int(dg.snapshot_info().live_edges)
+    for k in range(1, levels):
+        t = time.time()
+        g.add_random_level(0.005, 0.0005, args.seed + 2 + k)
+        freeze.append(time.time() - t)
+        (ins, _), (dels, _) = g.last_batch()
+        e_expect += int(ins.size) - int(dels.size)
+        b = dg.upload_stats()[0]
+        t = time.time()
+        dg.sync_levels()
+        ctx.sync()
+        upload.append(time.time() - t)
+        up_bytes.append(dg.upload_stats()[0] - b)
+        log(f"large: level {k}: freeze {freeze[-1]:.2f}s, upload {up_bytes[-1] / 1e9:.2f} GB in "
+            f"{upload[-1]:.3f}s")
+    info = dg.snapshot_info()
+    E, V = int(info.live_edges), g.num_nodes
+    ctx.sync()
+    t = time.perf_counter()
+    dg.pagerank(20)  # first call also materialises the snapshot and its orderings
+    ctx.sync()
+    t_first = time.perf_counter() - t
+    ctx.profile(True)
+    t = time.perf_counter()
+    dg.pagerank(20)
+    ctx.sync()
+    t_pr = time.perf_counter() - t
+    n_it, ms_it = ctx.kernel_time("k_pr_pull")
+    ctx.profile(False)
+    log(f"large: PageRank 20 iterations {t_pr * 1e3:.1f} ms ({E * 20 / t_pr / 1e9:.1f} GTEPS)")
+    roots = pick_sources(g, 16, args.seed + 3)
+    dg.bfs(int(roots[0]), want_levels=False)
+    ctx.sync()
+    t = time.perf_counter()
+    trav = 0
+    for r in roots:
+        dg.bfs(int(r), want_levels=False)
+        trav += dg.last_bfs_traversed
+    ctx.sync()
+    t_bfs = time.perf_counter() - t
+    log(f"large: BFS x16 {t_bfs * 1e3:.1f} ms ({trav / t_bfs / 1e9:.1f} GTEPS)")
+    roots64 = pick_sources(g, 64, args.seed + 4)
+    dg.bfs_batch(roots64, lane_width=64, want_levels=False)
+    ctx.sync()
+    t = time.perf_counter()
+    _, reached, sweeps = dg.bfs_batch(roots64, lane_width=64, want_levels=False)
+    ctx.sync()
+    t_batch = time.perf_counter() - t
+    log(f"large: batched BFS x64 {t_batch * 1e3:.1f} ms")
+    # checks that would catch 32-bit index overflow past 2^31 edges
+    lv_b, _, _ = dg.bfs_batch(roots64[:2], lane_width=64)
+    lv0, n0, _ = dg.bfs(int(roots64[0]))
+    p, _ = dg.pagerank_push(20)
+    r, _ = dg.pagerank(20)
+    checks = {"live_edges_equal_base_plus_inserts_minus_deletes": E == e_expect,
+              "batched_lane_equals_single_source": bool(np.array_equal(lv_b[0], lv0)),
+              "bfs_reached_matches_levels": n0 == int((lv0 != 2**31 - 2).sum()),
+              "pagerank_pull_vs_push_l1rel": float(np.abs(r - p).sum() / np.abs(p).sum())}
+    log(f"large: checks {checks}")
+    print(json.dumps({
+        "metric": "large single-GPU scale (PageRank, DO-BFS, batched BFS; per-level upload)",
+        "config": {"graph": f"R-MAT s{scale} EF16, {levels} levels (+0.5%/-0.05% per level), "
+                            "deletion vector on", "num_nodes": V, "live_edges": E,
+                   "stored_words": int(info.stored_words)},
+        "data": "synthetic (seeded R-MAT + uniform incremental levels)",
+        "upload": {"base_bytes": b0, "per_level_bytes_mean": int(np.mean(up_bytes)),
+                   "per_level_upload_s_mean": round(float(np.mean(upload)), 4),
+                   "per_level_freeze_s_mean": round(float(np.mean(freeze)), 3),
+                   "upload_gb_per_s": round(sum(up_bytes) / sum(upload) / 1e9, 2)},
+        "pagerank": {"iterations": 20, "seconds": round(t_pr, 4),
+                     "gteps": round(E * 20 / t_pr / 1e9, 2),
+                     "first_call_seconds_incl_materialisation": round(t_first, 4),
+                     "k_pr_pull_ms": round(ms_it / max(n_it, 1), 3)},
+        "bfs": {"sources": 16, "seconds": round(t_bfs, 4), "gteps": round(trav / t_bfs / 1e9, 2)},
+        "bfs_batch": {"sources": 64, "lanes": 64, "seconds": round(t_batch, 4), "sweeps": sweeps,
+                      "reached_total": int(np.asarray(reached).sum())},
+        "checks": checks,
+    }), flush=True)
+    dg.close()
+    ctx.close()
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -413,6 +519,11 @@ def main() -> int:
                          "e.g. 0,10,10:0,20:65536")
     ap.add_argument("--bc-sources", type=int, default=8,
                     help="betweenness centrality sources on the same graph (0 = skip)")
+    ap.add_argument("--large-scale", type=int, default=0,
+                    help="run only the large single-GPU configuration at this R-MAT scale "
+                         "(27 for the configuration; prints its own JSON line)")
+    ap.add_argument("--large-levels", type=int, default=20,
+                    help="levels of the large configuration (base included)")
     ap.add_argument("--cow-scale", type=int, default=24,
                     help="copy-on-write upload leg: base graph scale (0 = skip)")
     ap.add_argument("--cow-levels", type=int, default=10,
@@ -425,6 +536,9 @@ def main() -> int:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 "
                           "(multi-GPU sharding is design-only)"}))
         return 2
+
+    if args.large_scale > 0:
+        return bench_large(args)
 
     import numpy as np
 
