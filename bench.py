@@ -368,6 +368,68 @@ def iteration_ms(ctx, iterations):
     return sum(ctx.kernel_time(k)[1] for k in PR_ITERATION_KERNELS) / max(iterations, 1)
 
 
+def reference_calibration(ctx, args) -> dict:
+    """The reference's own CPU-runnable configuration: plain PageRank
+    (pagerank_double, 10 iterations) on a single-level R-MAT s20 written as
+    .xs1, timed with the reference benchmark built in place under
+    oracle/_ref (its `Time` line, mean +- c95 of -c 3), beside the oracle
+    restatement and this build's GPU PageRank on the same graph."""
+    import re
+    import subprocess
+    import tempfile
+
+    import numpy as np
+
+    import oracle
+    from llama_amd.graph import DeviceGraph, Edges, HostGraph
+    ref = ROOT / "oracle" / "_ref" / "benchmark-memory"
+    if not ref.exists():
+        return {"skipped": "oracle/_ref/benchmark-memory is not built"}
+    cores = int(os.environ.get("OMP_NUM_THREADS", oracle.num_threads()))
+    e = Edges.rmat(20, 16.0, seed=args.seed, perm_seed=args.seed + 1)
+    out = {"graph": "R-MAT s20 EF16, 1 level, no deletions", "edges": len(e), "iterations": 10,
+           "cores": cores}
+    with tempfile.TemporaryDirectory() as tmp:
+        f = str(Path(tmp) / "s20.xs1")
+        e.write_xs1(f, 1 << 20)
+        env = dict(os.environ, OMP_NUM_THREADS=str(cores))
+        r = subprocess.run([str(ref), "-r", "pagerank_double", "-n", "10", "-c", "3", f],
+                           capture_output=True, text=True, timeout=600, cwd=tmp, env=env)
+    m = re.search(r"^Time\s*:\s*([0-9.]+) \+- ([0-9.]+) seconds", r.stdout, re.M)
+    if r.returncode != 0 or not m:
+        return {"skipped": "reference run failed: " + (r.stderr or r.stdout)[-300:]}
+    ref_s = float(m.group(1))
+    out["reference"] = {"seconds": ref_s, "c95": float(m.group(2)),
+                        "gteps": round(len(e) * 10 / ref_s / 1e9, 3)}
+    g = HostGraph(1 << 20, deletions=False)
+    g.add_level_edges(e)
+    o = oracle.OracleGraph.from_host(g)
+    ts = []
+    for _ in range(3):
+        t = time.perf_counter()
+        rp, _ = o.pagerank(10)
+        ts.append(time.perf_counter() - t)
+    out["port"] = {"seconds": round(float(np.mean(ts)), 4),
+                   "over_reference": round(float(np.mean(ts)) / ref_s, 3)}
+    dg = DeviceGraph(ctx, g)
+    dg.pagerank(10)
+    ctx.sync()
+    ts = []
+    for _ in range(3):
+        t = time.perf_counter()
+        rg, _ = dg.pagerank(10)
+        ts.append(time.perf_counter() - t)
+    ret = re.search(r"^Return value:\s*([0-9.eE+-]+)", r.stdout, re.M)
+    out["gpu"] = {"seconds": round(float(np.mean(ts)), 5),
+                  "speedup_over_reference": round(ref_s / float(np.mean(ts)), 1),
+                  "l1rel_vs_port": float(np.abs(rg - rp).sum() / np.abs(rp).sum()),
+                  "rank_sum": float(rg.sum()),
+                  "reference_return_value": float(ret.group(1)) if ret else None}
+    dg.close()
+    log(f"reference calibration: {out}")
+    return out
+
+
 def bench_large(args) -> int:
     """The large single-GPU configuration: R-MAT s27 EF16 plus 20 levels of
     +0.5% / -0.05%, in- and out-CSR resident in HBM; each level's host freeze
@@ -519,6 +581,9 @@ def main() -> int:
                          "e.g. 0,10,10:0,20:65536")
     ap.add_argument("--bc-sources", type=int, default=8,
                     help="betweenness centrality sources on the same graph (0 = skip)")
+    ap.add_argument("--ref-calibration", type=int, default=1,
+                    help="time the reference build on its own s20 PageRank configuration "
+                         "beside the oracle and the GPU (0 = skip)")
     ap.add_argument("--large-scale", type=int, default=0,
                     help="run only the large single-GPU configuration at this R-MAT scale "
                          "(27 for the configuration; prints its own JSON line)")
@@ -681,6 +746,8 @@ def main() -> int:
                          f"OpenMP dynamic,4096) over the same {args.levels}-level layout, "
                          f"{cpu_s:.2f}s"}
         log(f"CPU baseline {cpu['value']:.3f} GTEPS in {cpu_s:.1f}s")
+        if args.ref_calibration:
+            cpu["reference_calibration"] = reference_calibration(ctx, args)
 
     out = {
         "metric": METRIC,
