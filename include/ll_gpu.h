@@ -207,6 +207,14 @@ int ll_gpu_bfs_batch(ll_gpu_graph* g, int level, const int64_t* roots, int n, in
  * 682-831), extended to multi-level snapshots: triangles of the simple
  * undirected graph of live edges (out and in, self-loops and duplicates removed). */
 int ll_gpu_tc(ll_gpu_graph* g, int level, uint64_t* triangles);
+/* Replaces: task `tc_i` = ll_b_triangle_counting_LI::run
+ * (benchmark/benchmarks/triangle_counting.h:154-336; registered with reverse
+ * edges, benchmark.cc:906-908): for every pair u < v joined by a live edge,
+ * once per direction it is joined in, the common neighbours w > v (out or in)
+ * -- every triangle u<v<w weighs the number of directed edges between u and
+ * v.  The reference needs one level with sorted lists; any snapshot level
+ * works here. */
+int ll_gpu_tc_i(ll_gpu_graph* g, int level, uint64_t* count);
 /* Replaces: task `tarjan_scc` (benchmark/benchmarks/tarjan_scc.h:121-206) with
  * a correct SCC decomposition; labels_out[max_nodes] = smallest vertex ID of
  * each vertex's strongly connected component.  num_scc may be NULL. */

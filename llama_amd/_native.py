@@ -73,6 +73,7 @@ GPU_SIGNATURES = {
     "ll_gpu_set_option": (C.c_int, [vp, cp, C.c_long]),
     "ll_gpu_bfs_batch": (C.c_int, [vp, C.c_int, P(i64), C.c_int, C.c_int, P(i32), P(u64)]),
     "ll_gpu_tc": (C.c_int, [vp, C.c_int, P(u64)]),
+    "ll_gpu_tc_i": (C.c_int, [vp, C.c_int, P(u64)]),
     "ll_gpu_scc": (C.c_int, [vp, C.c_int, P(i64), P(u64)]),
     "ll_gpu_ppr_batch": (C.c_int, [vp, C.c_int, P(i64), C.c_int, C.c_int, f64, f64, C.c_int,
                                    P(f32), P(C.c_int)]),

@@ -43,6 +43,7 @@ const Entry kTasks[] = {
     {"gpu_bc_adj", "Betweenness centrality from every vertex (bc_adj)"},
     {"gpu_bc_random", "Betweenness centrality from --sources K random sources (bc_random)"},
     {"gpu_tc", "Triangle counting on the symmetrised snapshot (tc_u / tc_od)"},
+    {"gpu_tc_i", "Triangles counted once per direction of their lowest pair (tc_i)"},
     {"gpu_scc", "Strongly connected components, min-ID labels (tarjan_scc)"},
     {"gpu_bfs_batch", "Multi-source BFS, --sources N roots, --lanes W"},
     {"gpu_ppr_batch", "Batched personalized PageRank, --sources N seeds, -n iterations"},
@@ -199,6 +200,7 @@ int main(int argc, char** argv) {
   else if (task == "gpu_bc_adj") t.reset(new gpu_bc<Graph>());
   else if (task == "gpu_bc_random") t.reset(new gpu_bc<Graph>(sources_set ? sources : 100, seed));
   else if (task == "gpu_tc") t.reset(new gpu_triangle_counting<Graph>());
+  else if (task == "gpu_tc_i") t.reset(new gpu_tc_i<Graph>());
   else if (task == "gpu_scc") t.reset(new gpu_scc<Graph>());
   else if (task == "gpu_bfs_batch") t.reset(new gpu_bfs_batch<Graph>(roots, lanes));
   else t.reset(new gpu_ppr_batch<Graph>(roots, iters < 0 ? 30 : iters, 0.85, tol, lanes));

@@ -359,6 +359,24 @@ class gpu_triangle_counting : public ll_gpu_task<Graph> {
   uint64_t t_ = 0;
 };
 
+// tc_i (triangle_counting.h:154-336): run() returns the count, and
+// print_results writes it in the reference's words (:329-333).
+template <class Graph>
+class gpu_tc_i : public ll_gpu_task<Graph> {
+ public:
+  gpu_tc_i() : ll_gpu_task<Graph>("GPU Triangle Counting (tc_i)") {}
+  double run() override {
+    this->m_->check(ll_gpu_tc_i(this->m_->graph(), -1, &t_), "ll_gpu_tc_i");
+    return double(t_);
+  }
+  void print_results(FILE* f) override {
+    std::fprintf(f, "Number of triangles: %lld\n", (long long)t_);
+  }
+
+ private:
+  uint64_t t_ = 0;
+};
+
 // tarjan_scc replacement (tarjan_scc.h:121-206): label = min vertex ID per SCC;
 // finalize returns the largest label, as the reference's finalize does.
 template <class Graph>

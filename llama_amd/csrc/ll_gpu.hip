@@ -134,6 +134,7 @@ void drop_snapshots(ll_gpu_graph* g) {
   g->sym_level = -1;
   g->sym_ofs.release();
   g->sym_adj.release();
+  g->sym_mult.release();
   g->pr_level = -1;
   g->pr_inv.release();
   g->pr_adj.release();

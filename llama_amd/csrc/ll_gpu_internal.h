@@ -236,6 +236,7 @@ struct ll_gpu_graph {
   // derived structures (built on demand per snapshot level)
   int sym_level = -1;
   llgpu::DevBuf sym_ofs, sym_adj;  // degree-oriented DAG for triangle counting
+  llgpu::DevBuf sym_mult;          // uint8 per arc: directed live edges joining its ends (1 or 2)
   uint64_t sym_edges = 0;
   // PageRank locality order of the in-CSR (option pr_reorder)
   int pr_level = -1;

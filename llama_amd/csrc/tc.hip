@@ -16,6 +16,14 @@
 //      too long for a wave's LDS run on a whole workgroup, and beyond that
 //      search N+(a) in global memory.
 // Every triangle a<b<c (in that order) is counted once, at arc (a,b).
+//
+// Task tc_i (triangle_counting.h:154-336) visits the pair (u, v), u < v by
+// ID, once per directed edge between them (out-list, then in-list) and
+// counts the common neighbours w > v: every triangle weighs the number of
+// directed edges joining its two smallest IDs.  The DAG keeps that number per
+// arc (the run length of its key among the sorted, not yet deduplicated,
+// keys), and the weighted count adds it for the arc joining the two smallest
+// of {a, b, w}.
 #include "ll_gpu_internal.h"
 
 namespace llgpu {
@@ -56,36 +64,49 @@ __global__ void k_dag_offsets(const uint64_t* keys, uint64_t m, uint64_t V, uint
   if (i < m) nbr[i] = uint32_t(keys[i]);
 }
 
+// Run starts of the sorted arc keys (before deduplication).
+__global__ void k_run_starts(const uint64_t* sorted, uint64_t n, uint32_t* flag) {
+  const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) flag[i] = (i == 0 || sorted[i] != sorted[i - 1]) ? 1u : 0u;
+}
+
+// Arc multiplicity = length of its key's run: the run of unique key r starts
+// at sorted position start[r]; start[r + 1] (or n) ends it.
+__global__ void k_run_lengths(const uint32_t* flag, const uint64_t* rank, uint64_t n,
+                              uint64_t m, uint8_t* mult) {
+  const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n || !flag[i]) return;
+  const uint64_t r = rank[i];
+  if (r >= m) return;  // the self-loop sentinel's run
+  uint64_t j = i + 1;
+  while (j < n && !flag[j]) j++;
+  mult[r] = uint8_t(min<uint64_t>(j - i, 255));
+}
+
 // Make a wave's LDS writes visible to its other lanes before they read them.
 __device__ inline void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
   __builtin_amdgcn_wave_barrier();
 }
 
-__device__ inline bool lds_has(const uint32_t* list, uint32_t n, uint32_t x) {
+// Position of x in an ascending list of n entries, or n when absent.
+__device__ inline uint32_t find_in(const uint32_t* list, uint32_t n, uint32_t x) {
   uint32_t lo = 0, hi = n;
   while (lo < hi) {
     uint32_t mid = (lo + hi) >> 1;
     if (list[mid] < x) lo = mid + 1; else hi = mid;
   }
-  return lo < n && list[lo] == x;
-}
-
-__device__ inline bool global_has(const uint32_t* list, uint64_t n, uint32_t x) {
-  uint64_t lo = 0, hi = n;
-  while (lo < hi) {
-    uint64_t mid = (lo + hi) >> 1;
-    if (list[mid] < x) lo = mid + 1; else hi = mid;
-  }
-  return lo < n && list[lo] == x;
+  return lo < n && list[lo] == x ? lo : n;
 }
 
 // Count common out-neighbours for all arcs of row a, with `lanes` threads
 // cooperating (a wave or a workgroup); `na` = d+(a).  `pre` holds inclusive
-// prefix sums of d+(b) for the row's b's (in LDS when na fits).
-template <bool kLds>
+// prefix sums of d+(b) for the row's b's (in LDS when na fits).  kWeighted:
+// each triangle {a, b, w} adds mult of the arc joining its two smallest IDs.
+template <bool kLds, bool kWeighted>
 __device__ inline unsigned long long count_row(const uint64_t* ofs, const uint32_t* nbr,
-                                               uint64_t a, const uint32_t* la, uint32_t na,
+                                               const uint8_t* mult, uint64_t a,
+                                               const uint32_t* la, uint32_t na,
                                                const uint32_t* pre, uint32_t tid, uint32_t lanes) {
   const uint64_t a0 = ofs[a];
   const uint32_t work = na ? pre[na - 1] : 0;
@@ -99,14 +120,26 @@ __device__ inline unsigned long long count_row(const uint64_t* ofs, const uint32
     }
     const uint32_t b = kLds ? la[lo] : nbr[a0 + lo];
     const uint32_t j = x - (lo ? pre[lo - 1] : 0);
-    const uint32_t w = nbr[ofs[b] + j];
-    c += kLds ? lds_has(la, na, w) : global_has(nbr + a0, na, w);
+    const uint64_t bw = ofs[b] + j;
+    const uint32_t w = nbr[bw];
+    const uint32_t p = find_in(kLds ? la : nbr + a0, na, w);
+    if (p == na) continue;
+    if (!kWeighted) {
+      c++;
+    } else {
+      // the arc between the two smallest IDs: (a,b) unless w is not the largest
+      const uint64_t amax = uint64_t(a) > b && uint64_t(a) > w;
+      const uint64_t arc = amax ? bw : (b > w ? a0 + p : a0 + lo);
+      c += mult[arc];
+    }
   }
   return c;
 }
 
+template <bool kWeighted>
 __global__ __launch_bounds__(kBlock) void k_tc_wave(const uint64_t* ofs, const uint32_t* nbr,
-                                                    uint64_t V, unsigned long long* partial) {
+                                                    const uint8_t* mult, uint64_t V,
+                                                    unsigned long long* partial) {
   __shared__ uint32_t l_a[kBlock / 64][kWaveRow];
   __shared__ uint32_t l_pre[kBlock / 64][kWaveRow];
   __shared__ unsigned long long red[kBlock / 64];
@@ -136,7 +169,7 @@ __global__ __launch_bounds__(kBlock) void k_tc_wave(const uint64_t* ofs, const u
       carry += __shfl(v, 63, 64);
     }
     wave_sync();
-    c += count_row<true>(ofs, nbr, a, l_a[wave], na, l_pre[wave], lane, 64);
+    c += count_row<true, kWeighted>(ofs, nbr, mult, a, l_a[wave], na, l_pre[wave], lane, 64);
     wave_sync();
   }
   for (int o = 32; o > 0; o >>= 1) c += __shfl_down(c, o, 64);
@@ -150,9 +183,11 @@ __global__ __launch_bounds__(kBlock) void k_tc_wave(const uint64_t* ofs, const u
 }
 
 // Long rows (d+(a) > kWaveRow), one workgroup each.
+template <bool kWeighted>
 __global__ __launch_bounds__(kBlock) void k_tc_block(const uint64_t* ofs, const uint32_t* nbr,
-                                                     const uint32_t* rows, uint64_t nrows,
-                                                     uint32_t* gpre, unsigned long long* partial) {
+                                                     const uint8_t* mult, const uint32_t* rows,
+                                                     uint64_t nrows, uint32_t* gpre,
+                                                     unsigned long long* partial) {
   __shared__ uint32_t l_a[kBlockRow];
   __shared__ uint32_t l_pre[kBlockRow];
   __shared__ uint32_t wtot[kBlock / 64];
@@ -190,8 +225,8 @@ __global__ __launch_bounds__(kBlock) void k_tc_block(const uint64_t* ofs, const 
       carry += chunk;
       __syncthreads();
     }
-    if (lds) c += count_row<true>(ofs, nbr, a, l_a, na, l_pre, threadIdx.x, kBlock);
-    else c += count_row<false>(ofs, nbr, a, nullptr, na, pre, threadIdx.x, kBlock);
+    if (lds) c += count_row<true, kWeighted>(ofs, nbr, mult, a, l_a, na, l_pre, threadIdx.x, kBlock);
+    else c += count_row<false, kWeighted>(ofs, nbr, mult, a, nullptr, na, pre, threadIdx.x, kBlock);
     __syncthreads();
   }
   for (int o = 32; o > 0; o >>= 1) c += __shfl_down(c, o, 64);
@@ -282,6 +317,20 @@ void ensure_dag(ll_gpu_graph* g, int level) {
     LLG_HIP(hipStreamSynchronize(s));
     if (last == ~0ull) m--;
   }
+  // multiplicities (tc_i): runs of equal keys among the sorted keys
+  {
+    DevBuf flag, rank;
+    flag.alloc(std::max<uint64_t>(E, 1) * 4, s);
+    rank.alloc((E + 1) * 8, s);
+    g->sym_mult.alloc(std::max<uint64_t>(m, 1), s);
+    if (E) {
+      k_run_starts<<<grid_for(E, 256), 256, 0, s>>>(sorted.as<uint64_t>(), E, flag.as<uint32_t>());
+      scan_u32_to_u64(flag.as<uint32_t>(), rank.as<uint64_t>(), E, s);
+      k_run_lengths<<<grid_for(E, 256), 256, 0, s>>>(flag.as<uint32_t>(), rank.as<uint64_t>(), E, m,
+                                                     g->sym_mult.as<uint8_t>());
+      LLG_HIP(hipGetLastError());
+    }
+  }
   sorted.release();
   g->sym_ofs.alloc((V + 1) * 8, s);
   g->sym_adj.alloc(std::max<uint64_t>(m, 1) * 4, s);
@@ -293,13 +342,14 @@ void ensure_dag(ll_gpu_graph* g, int level) {
   g->sym_level = level;
 }
 
-TcResult tc_device(ll_gpu_graph* g, int level) {
+TcResult tc_device(ll_gpu_graph* g, int level, bool weighted) {
   ll_gpu_ctx* ctx = g->ctx;
   hipStream_t s = ctx->stream;
   const uint64_t V = g->V;
   ensure_dag(g, level);
   const uint64_t* ofs = g->sym_ofs.as<uint64_t>();
   const uint32_t* nbr = g->sym_adj.as<uint32_t>();
+  const uint8_t* mult = g->sym_mult.as<uint8_t>();
   TcResult r;
   r.arcs = g->sym_edges;
   DevBuf cnt, rows, gpre, part_w, part_b, total;
@@ -320,7 +370,10 @@ TcResult tc_device(ll_gpu_graph* g, int level) {
   part_w.alloc(gw * 8, s);
   {
     TimedLaunch t(ctx, "k_tc_wave");
-    k_tc_wave<<<gw, kBlock, 0, s>>>(ofs, nbr, V, part_w.as<unsigned long long>());
+    if (weighted)
+      k_tc_wave<true><<<gw, kBlock, 0, s>>>(ofs, nbr, mult, V, part_w.as<unsigned long long>());
+    else
+      k_tc_wave<false><<<gw, kBlock, 0, s>>>(ofs, nbr, mult, V, part_w.as<unsigned long long>());
   }
   const unsigned gb = unsigned(std::min<uint64_t>(std::max<uint64_t>(nlong, 1),
                                                   uint64_t(ctx->num_cus) * 8));
@@ -328,8 +381,12 @@ TcResult tc_device(ll_gpu_graph* g, int level) {
   if (nlong) {
     gpre.alloc(std::max<uint64_t>(g->sym_edges, 1) * 4, s);
     TimedLaunch t(ctx, "k_tc_block");
-    k_tc_block<<<gb, kBlock, 0, s>>>(ofs, nbr, rows.as<uint32_t>(), nlong, gpre.as<uint32_t>(),
-                                     part_b.as<unsigned long long>());
+    if (weighted)
+      k_tc_block<true><<<gb, kBlock, 0, s>>>(ofs, nbr, mult, rows.as<uint32_t>(), nlong,
+                                             gpre.as<uint32_t>(), part_b.as<unsigned long long>());
+    else
+      k_tc_block<false><<<gb, kBlock, 0, s>>>(ofs, nbr, mult, rows.as<uint32_t>(), nlong,
+                                              gpre.as<uint32_t>(), part_b.as<unsigned long long>());
   } else {
     k_zero_ull2<<<1, 1, 0, s>>>(part_b.as<unsigned long long>(), int(gb));
   }
@@ -355,7 +412,7 @@ int ll_gpu_tc_ex(ll_gpu_graph* g, int level, uint64_t* triangles, uint64_t* arcs
   if (!g || !triangles) return LL_GPU_EINVAL;
   try {
     LLG_HIP(hipSetDevice(g->ctx->device));
-    llgpu::TcResult r = llgpu::tc_device(g, level);
+    llgpu::TcResult r = llgpu::tc_device(g, level, false);
     *triangles = r.triangles;
     if (arcs) *arcs = r.arcs;
     if (work) *work = r.work;
@@ -368,6 +425,18 @@ int ll_gpu_tc_ex(ll_gpu_graph* g, int level, uint64_t* triangles, uint64_t* arcs
 
 int ll_gpu_tc(ll_gpu_graph* g, int level, uint64_t* triangles) {
   return ll_gpu_tc_ex(g, level, triangles, nullptr, nullptr);
+}
+
+int ll_gpu_tc_i(ll_gpu_graph* g, int level, uint64_t* count) {
+  if (!g || !count) return LL_GPU_EINVAL;
+  try {
+    LLG_HIP(hipSetDevice(g->ctx->device));
+    *count = llgpu::tc_device(g, level, true).triangles;
+    return LL_GPU_OK;
+  } catch (const llgpu::Error& e) {
+    g->ctx->err = e.msg;
+    return e.code;
+  }
 }
 
 }  // extern "C"

@@ -496,6 +496,12 @@ class DeviceGraph:
             reached.ctypes.data_as(C.POINTER(C.c_uint64)), C.byref(sweeps)))
         return lv, reached, int(sweeps.value)
 
+    def tc_i(self, level: int = -1) -> int:
+        """tc_i: triangles u<v<w counted once per directed edge between u and v."""
+        n = C.c_uint64()
+        self.ctx.check(self.lib.ll_gpu_tc_i(self._h, level, C.byref(n)))
+        return int(n.value)
+
     def triangles(self, level: int = -1, stats: bool = False):
         t, arcs, work = C.c_uint64(), C.c_uint64(), C.c_uint64()
         self.ctx.check(self.lib.ll_gpu_tc_ex(self._h, level, C.byref(t), C.byref(arcs),

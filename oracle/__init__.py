@@ -48,6 +48,7 @@ def lib() -> C.CDLL:
             "orc_pagerank_double": (C.c_int, [vp, C.c_int, C.c_int, f64, P(f64), P(f64)]),
             "orc_bfs_levels": (i64, [vp, C.c_int, i64, P(i32)]),
             "orc_triangles": (i64, [vp, C.c_int]),
+            "orc_tc_i": (i64, [vp, C.c_int]),
             "orc_symmetrise": (i64, [vp, C.c_int, P(u64), P(u32)]),
             "orc_scc": (i64, [vp, C.c_int, P(i64)]),
             "orc_ppr": (C.c_int, [vp, C.c_int, P(i64), C.c_int, C.c_int, f64, f64, P(f64),
@@ -170,6 +171,10 @@ class OracleGraph:
 
     def triangles(self, level: int = -1) -> int:
         return int(lib().orc_triangles(self._h, level))
+
+    def tc_i(self, level: int = -1) -> int:
+        """tc_i restated: triangles u<v<w counted once per direction of (u, v)."""
+        return int(lib().orc_tc_i(self._h, level))
 
     def symmetrise(self, level: int = -1):
         n = lib().orc_symmetrise(self._h, level, None, None)

@@ -355,6 +355,64 @@ int64_t orc_triangles(const orc_graph* g, int level) {
   return T;
 }
 
+// tc_i (ll_b_triangle_counting_LI, triangle_counting.h:154-336): for every
+// vertex u and every out-neighbour and then every in-neighbour v > u (a pair
+// joined in both directions is visited twice), count_for (:176-266) merges
+// the sorted out- and in-lists of u and of v and counts the common w > v.
+// The reference needs one level with sorted lists; the snapshot's lists are
+// sorted here, so any level works.
+int64_t orc_tc_i(const orc_graph* g, int level) {
+  level = resolve(g, level);
+  if (level < 0) return -1;
+  std::vector<uint64_t> oo, io;
+  std::vector<uint32_t> oa, ia;
+  for (int d = 0; d < 2; d++) {
+    auto& ofs = d == 0 ? oo : io;
+    auto& adj = d == 0 ? oa : ia;
+    ofs.assign(g->V + 1, 0);
+    adj.resize(uint64_t(orc_flatten(g, uint32_t(d), level, nullptr, nullptr)));
+    orc_flatten(g, uint32_t(d), level, ofs.data(), adj.data());
+#pragma omp parallel for schedule(dynamic, 4096)
+    for (uint64_t v = 0; v < g->V; v++) std::sort(adj.begin() + ofs[v], adj.begin() + ofs[v + 1]);
+  }
+  // the merged walk of count_for: both lists of each endpoint advance past
+  // the current minimum, so a neighbour in both lists counts once
+  auto count_for = [&](uint64_t u, uint64_t v) {
+    int64_t c = 0;
+    uint64_t a = oo[u], ae = oo[u + 1], b = io[u], be = io[u + 1];
+    uint64_t x = oo[v], xe = oo[v + 1], y = io[v], ye = io[v + 1];
+    while ((a < ae || b < be) && (x < xe || y < ye)) {
+      const uint64_t mu = a == ae ? ia[b] : (b == be ? oa[a] : std::min(oa[a], ia[b]));
+      const uint64_t mv = x == xe ? ia[y] : (y == ye ? oa[x] : std::min(oa[x], ia[y]));
+      const bool au = a < ae && oa[a] == mu, bu = b < be && ia[b] == mu;
+      const bool xv = x < xe && oa[x] == mv, yv = y < ye && ia[y] == mv;
+      if (mu < mv) {
+        a += au;
+        b += bu;
+      } else if (mv < mu) {
+        x += xv;
+        y += yv;
+      } else {
+        if (v < mv) c++;
+        a += au;
+        b += bu;
+        x += xv;
+        y += yv;
+      }
+    }
+    return c;
+  };
+  int64_t T = 0;
+#pragma omp parallel for reduction(+ : T) schedule(dynamic, 4096)
+  for (uint64_t u = 0; u < g->V; u++) {
+    for (uint64_t i = oo[u]; i < oo[u + 1]; i++)
+      if (u < oa[i]) T += count_for(u, oa[i]);
+    for (uint64_t i = io[u]; i < io[u + 1]; i++)
+      if (u < ia[i]) T += count_for(u, ia[i]);
+  }
+  return T;
+}
+
 // Symmetrised simple snapshot as CSR (for brute-force and degree checks).
 int64_t orc_symmetrise(const orc_graph* g, int level, uint64_t* ofs_out, uint32_t* adj_out) {
   level = resolve(g, level);
