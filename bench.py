@@ -112,10 +112,16 @@ def bench_sssp(dg, g, E: int, V: int, args) -> dict:
         t = time.perf_counter() - t0
         n_rx, ms_rx = ctx.kernel_time("k_relax")
         ctx.profile(False)
+        # algorithmic floor per source: every live edge relaxed once (4 B ID,
+        # 4 B weight, 4 B distance) and one 4 B distance per vertex
+        alg = (12 * E + 4 * V) * len(roots)
         return {"sources": len(roots), "seconds": round(t, 4),
                 "ms_per_source": round(t * 1e3 / len(roots), 2),
                 "gteps": round(E * len(roots) / t / 1e9, 3), "reached": reached,
-                "rounds": rounds, "relax_kernel_ms": round(ms_rx, 3)}
+                "rounds": rounds, "relax_kernel_ms": round(ms_rx, 3),
+                "alg_bytes": alg, "alg_gbs": round(alg / t / 1e9, 2),
+                "frac_hbm": round(alg / t / 1e9 / HBM_PEAK_GBS, 4),
+                "relax_alg_gbs": round(alg / max(ms_rx, 1e-9) / 1e6, 2)}
 
     sweep = {}
     for item in [x.strip() for x in args.sssp_sweep.split(",") if x.strip()]:
@@ -223,12 +229,74 @@ def bench_bc(dg, g, E: int, V: int, args) -> dict:
     n_p, ms_p = ctx.kernel_time("k_bc_pass")
     n_b, ms_b = ctx.kernel_time("k_bu_step")
     ctx.profile(False)
+    # algorithmic floor per source of the two Brandes passes: every live edge
+    # read once as an in-edge and once as an out-edge (4 B ID + 4 B level each)
+    alg = 16 * E * len(roots)
     out = {"sources": len(roots), "seconds": round(t, 4),
            "ms_per_source": round(t * 1e3 / len(roots), 2),
            "gteps": round(E * len(roots) / t / 1e9, 3),
            "pass_launches": n_p, "pass_kernel_ms": round(ms_p, 3),
-           "bfs_bottom_up_ms": round(ms_b, 3), "max_bc": float(b.max())}
+           "bfs_bottom_up_ms": round(ms_b, 3), "max_bc": float(b.max()),
+           "pass_alg_bytes": alg, "pass_alg_gbs": round(alg / max(ms_p, 1e-9) / 1e6, 2),
+           "pass_frac_hbm": round(alg / max(ms_p, 1e-9) / 1e6 / HBM_PEAK_GBS, 4)}
     log(f"BC: {out}")
+    return out
+
+
+def bench_tc(dg, E: int, V: int, args) -> dict:
+    """Triangle counting (tc_u over the symmetrised snapshot) and tc_i.  The
+    first call builds the degree-oriented DAG; later calls reuse it."""
+    ctx = dg.ctx
+    ctx.sync()
+    t0 = time.perf_counter()
+    tri, arcs, work = dg.triangles(stats=True)
+    t_first = time.perf_counter() - t0
+    ctx.profile(True)
+    t0 = time.perf_counter()
+    for _ in range(args.tc_reps):
+        tri2 = dg.triangles()
+    ctx.sync()
+    t = (time.perf_counter() - t0) / args.tc_reps
+    _, ms_w = ctx.kernel_time("k_tc_wave")
+    _, ms_b = ctx.kernel_time("k_tc_block")
+    ctx.profile(False)
+    t0 = time.perf_counter()
+    tci = dg.tc_i()
+    t_tci_first = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    tci2 = dg.tc_i()
+    t_tci = time.perf_counter() - t0
+    # B_TC: every intersection element read once (4 B) and one offset pair per vertex
+    alg = 4 * work + 8 * V
+    k_ms = (ms_w + ms_b) / args.tc_reps
+    out = {"triangles": tri, "repeat_equal": tri2 == tri, "dag_arcs": arcs, "work": work,
+           "first_call_s": round(t_first, 4), "count_s": round(t, 4),
+           "kernel_ms": round(k_ms, 3), "alg_bytes": alg,
+           "alg_gbs": round(alg / max(k_ms, 1e-9) / 1e6, 2),
+           "frac_hbm": round(alg / max(k_ms, 1e-9) / 1e6 / HBM_PEAK_GBS, 4),
+           "tc_i": tci, "tc_i_repeat_equal": tci2 == tci,
+           "tc_i_first_call_s": round(t_tci_first, 4), "tc_i_count_s": round(t_tci, 4)}
+    log(f"TC: {out}")
+    return out
+
+
+def bench_scc(dg, E: int, V: int, args) -> dict:
+    """Strongly connected components of the snapshot (labels stay on the device)."""
+    ctx = dg.ctx
+    _, n0 = dg.scc(want_labels=False)  # warm both CSR directions
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.scc_reps):
+        _, n = dg.scc(want_labels=False)
+    ctx.sync()
+    t = (time.perf_counter() - t0) / args.scc_reps
+    # floor: both directions read once (4 B per edge each) plus 16 B per vertex
+    alg = 8 * E + 16 * V
+    out = {"num_scc": n, "repeat_equal": n == n0, "seconds": round(t, 4),
+           "sweeps": dg.last_scc_sweeps, "gteps": round(E / t / 1e9, 3),
+           "alg_bytes_floor": alg, "alg_gbs": round(alg / t / 1e9, 2),
+           "frac_hbm": round(alg / t / 1e9 / HBM_PEAK_GBS, 4)}
+    log(f"SCC: {out}")
     return out
 
 
@@ -581,6 +649,9 @@ def main() -> int:
                          "e.g. 0,10,10:0,20:65536")
     ap.add_argument("--bc-sources", type=int, default=8,
                     help="betweenness centrality sources on the same graph (0 = skip)")
+    ap.add_argument("--tc-reps", type=int, default=2,
+                    help="timed triangle counts after the DAG build (0 = skip)")
+    ap.add_argument("--scc-reps", type=int, default=2, help="timed SCC runs (0 = skip)")
     ap.add_argument("--ref-calibration", type=int, default=1,
                     help="time the reference build on its own s20 PageRank configuration "
                          "beside the oracle and the GPU (0 = skip)")
@@ -722,6 +793,10 @@ def main() -> int:
         secondary["sssp_weighted"] = bench_sssp(dg, g, E, V, args)
     if args.bc_sources > 0:
         secondary["bc_random"] = bench_bc(dg, g, E, V, args)
+    if args.tc_reps > 0:
+        secondary["tc"] = bench_tc(dg, E, V, args)
+    if args.scc_reps > 0:
+        secondary["scc"] = bench_scc(dg, E, V, args)
     if args.cow_scale > 0:
         secondary["cow_upload"] = bench_cow_upload(ctx, args)
     if args.ppr_seeds > 0:

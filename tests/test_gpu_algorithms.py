@@ -100,6 +100,25 @@ def test_triangles_closed_forms(ctx):
     assert t == o2.triangles() and work > 0
 
 
+def test_triangles_global_memory_rows(ctx):
+    """K_n with both directions of every pair: all degrees tie, so the oriented
+    DAG ranks by ID and vertex i keeps n-1-i arcs; at n = 4100 the three
+    lowest rows exceed the 4096-entry LDS row and take the global-memory
+    prefix path.  Closed forms: C(n,3) triangles, tc_i twice that (two directed
+    edges join every pair)."""
+    n = 4100
+    gpu_case(f"tc K_{n} global-memory rows")
+    iu, ju = np.nonzero(~np.eye(n, dtype=bool))
+    h = HostGraph(n)
+    h.add_level(iu.astype(np.uint32), ju.astype(np.uint32))
+    dg = DeviceGraph(ctx, h)
+    t, arcs, _ = dg.triangles(stats=True)
+    c3 = n * (n - 1) * (n - 2) // 6
+    assert arcs == n * (n - 1) // 2
+    assert t == c3
+    assert dg.tc_i() == 2 * c3
+
+
 # ---- SCC -----------------------------------------------------------------------------
 def test_scc_labels_bit_exact(ctx, ml):
     g, o = ml
