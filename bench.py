@@ -259,6 +259,7 @@ def bench_tc(dg, E: int, V: int, args) -> dict:
     t = (time.perf_counter() - t0) / args.tc_reps
     _, ms_w = ctx.kernel_time("k_tc_wave")
     _, ms_b = ctx.kernel_time("k_tc_block")
+    _, ms_g = ctx.kernel_time("k_tc_global")
     ctx.profile(False)
     t0 = time.perf_counter()
     tci = dg.tc_i()
@@ -268,10 +269,14 @@ def bench_tc(dg, E: int, V: int, args) -> dict:
     t_tci = time.perf_counter() - t0
     # B_TC: every intersection element read once (4 B) and one offset pair per vertex
     alg = 4 * work + 8 * V
-    k_ms = (ms_w + ms_b) / args.tc_reps
+    k_ms = (ms_w + ms_b + ms_g) / args.tc_reps
     out = {"triangles": tri, "repeat_equal": tri2 == tri, "dag_arcs": arcs, "work": work,
            "first_call_s": round(t_first, 4), "count_s": round(t, 4),
-           "kernel_ms": round(k_ms, 3), "alg_bytes": alg,
+           "kernel_ms": round(k_ms, 3),
+           "kernel_ms_by_row_class": {"wave": round(ms_w / args.tc_reps, 3),
+                                      "workgroup": round(ms_b / args.tc_reps, 3),
+                                      "global": round(ms_g / args.tc_reps, 3)},
+           "alg_bytes": alg,
            "alg_gbs": round(alg / max(k_ms, 1e-9) / 1e6, 2),
            "frac_hbm": round(alg / max(k_ms, 1e-9) / 1e6 / HBM_PEAK_GBS, 4),
            "tc_i": tci, "tc_i_repeat_equal": tci2 == tci,

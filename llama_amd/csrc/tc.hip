@@ -9,12 +9,16 @@
 //      lists by O(sqrt(E))), encoded as the 64-bit key a<<32|b;
 //   2. keys are radix-sorted and deduplicated: the oriented DAG as a CSR with
 //      ascending neighbour lists;
-//   3. for each arc (a,b) count |N+(a) & N+(b)|: one wave per row a stages
-//      N+(a) and the prefix sums of d+(b) over b in N+(a) in LDS, then streams
-//      the concatenated lists N+(b) 64 entries at a time (coalesced within each
-//      list) and binary-searches every entry in the LDS copy of N+(a).  Rows
-//      too long for a wave's LDS run on a whole workgroup, and beyond that
-//      search N+(a) in global memory.
+//   3. for each arc (a,b) count |N+(a) & N+(b)|: one wave per row a (a whole
+//      workgroup for rows of 257..4096 arcs) stages N+(a), a hashed
+//      membership bitmap of it, and for the arcs with d+(b) > 0 the prefix
+//      sums of d+(b) and the list starts ofs[b] in LDS.  The concatenated
+//      lists N+(b) are then streamed 256 entries per wave step (coalesced
+//      within each list): every entry finds its arc from a ballot of
+//      arc-end flags (no search), the four 64-entry loads of a step are in
+//      flight together, the bitmap rejects most non-members with one LDS read
+//      and a branchless binary search of N+(a) confirms the rest.  Rows
+//      longer than 4096 arcs search N+(a) in global memory.
 // Every triangle a<b<c (in that order) is counted once, at arc (a,b).
 //
 // Task tc_i (triangle_counting.h:154-336) visits the pair (u, v), u < v by
@@ -31,8 +35,12 @@ namespace llgpu {
 namespace {
 
 constexpr int kBlock = 256;
-constexpr uint32_t kWaveRow = 512;    // row length handled by one wave (LDS)
-constexpr uint32_t kBlockRow = 4096;  // row length handled by one workgroup (LDS)
+constexpr int kRowBlock = 512;         // workgroup of the kWaveRow..kBlockRow rows
+constexpr uint32_t kWaveRow = 256;     // row length handled by one wave (LDS)
+constexpr uint32_t kBlockRow = 4096;   // row length handled by one workgroup (LDS)
+constexpr uint32_t kWaveBmBits = 12;   // membership bitmap of a wave row: 4096 bits
+constexpr uint32_t kBlockBmBits = 15;  // of a workgroup row: 32768 bits
+constexpr uint32_t kU = 4;             // 64-element chunks a wave resolves together
 
 __global__ void k_orient(const uint64_t* ofs, const uint32_t* adj, const uint32_t* dout,
                          const uint32_t* din, uint64_t V, uint64_t* keys, uint64_t E) {
@@ -99,15 +107,16 @@ __device__ inline uint32_t find_in(const uint32_t* list, uint32_t n, uint32_t x)
   return lo < n && list[lo] == x ? lo : n;
 }
 
-// Count common out-neighbours for all arcs of row a, with `lanes` threads
-// cooperating (a wave or a workgroup); `na` = d+(a).  `pre` holds inclusive
-// prefix sums of d+(b) for the row's b's (in LDS when na fits).  kWeighted:
-// each triangle {a, b, w} adds mult of the arc joining its two smallest IDs.
-template <bool kLds, bool kWeighted>
-__device__ inline unsigned long long count_row(const uint64_t* ofs, const uint32_t* nbr,
-                                               const uint8_t* mult, uint64_t a,
-                                               const uint32_t* la, uint32_t na,
-                                               const uint32_t* pre, uint32_t tid, uint32_t lanes) {
+// Count common out-neighbours for all arcs of row a, with `lanes` threads of a
+// workgroup; `na` = d+(a) > kBlockRow, so N+(a) is searched in global memory
+// and `pre` (inclusive prefix sums of d+(b) over the row's b's) lies in a
+// global buffer at the row's own arc range.  kWeighted: each triangle
+// {a, b, w} adds mult of the arc joining its two smallest IDs.
+template <bool kWeighted>
+__device__ inline unsigned long long count_row_global(const uint64_t* ofs, const uint32_t* nbr,
+                                                      const uint8_t* mult, uint64_t a,
+                                                      uint32_t na, const uint32_t* pre,
+                                                      uint32_t tid, uint32_t lanes) {
   const uint64_t a0 = ofs[a];
   const uint32_t work = na ? pre[na - 1] : 0;
   unsigned long long c = 0;
@@ -118,11 +127,11 @@ __device__ inline unsigned long long count_row(const uint64_t* ofs, const uint32
       uint32_t mid = (lo + hi) >> 1;
       if (pre[mid] <= x) lo = mid + 1; else hi = mid;
     }
-    const uint32_t b = kLds ? la[lo] : nbr[a0 + lo];
+    const uint32_t b = nbr[a0 + lo];
     const uint32_t j = x - (lo ? pre[lo - 1] : 0);
     const uint64_t bw = ofs[b] + j;
     const uint32_t w = nbr[bw];
-    const uint32_t p = find_in(kLds ? la : nbr + a0, na, w);
+    const uint32_t p = find_in(nbr + a0, na, w);
     if (p == na) continue;
     if (!kWeighted) {
       c++;
@@ -136,40 +145,183 @@ __device__ inline unsigned long long count_row(const uint64_t* ofs, const uint32
   return c;
 }
 
+// Membership bitmap of N+(a) in LDS: one bit per multiplicative hash of a
+// neighbour ID.  A clear bit proves absence; a set bit is confirmed by a
+// binary search of the sorted list.
+__device__ inline uint32_t bm_hash(uint32_t w, uint32_t bits) {
+  return (w * 0x9E3779B1u) >> (32u - bits);
+}
+
+__device__ inline uint64_t lanes_below(uint32_t lane) {
+  return lane ? (~0ull >> (64u - lane)) : 0ull;
+}
+
+// The staged row a (LDS): la[0..na) = N+(a) ascending, bm = its bitmap,
+// lp[0..cnt) = inclusive prefix sums of d+(b) over the arcs with d+(b) > 0
+// ("compacted arcs", in any order), lofs[k] = ofs[b] of compacted arc k and
+// li[k] = the position of its b in la (weighted count only).  Element x in
+// [0, lp[cnt-1]) is entry x - lp[k-1] of N+(b) for the compacted arc k with
+// lp[k-1] <= x < lp[k].
+//
+// One wave counts the elements x in [base0, base0 + 64kU) below `end`, lane
+// `lane` taking x = base0 + 64u + lane (u < kU).  *lo = the first compacted
+// arc with lp > base0 on entry, the first with lp > base0 + 64kU on return.
+// Every compacted arc holds at least one element, so at most 64kU arcs end
+// inside the range, and they are arcs lo .. lo + 64kU - 1: the lanes load
+// those, flag the range position after each one's last element, and a ballot
+// of the flags per 64 positions plus popcounts give every element its arc
+// without a search.  The kU neighbour loads are then issued together, tested
+// against the bitmap, and the candidates confirmed by a branchless binary
+// search of la (the same depth for every lane, the kU searches interleaved).
+template <bool kWeighted>
+__device__ inline unsigned long long count_group(const uint32_t* nbr, const uint8_t* mult,
+                                                 uint64_t a, uint64_t a0, const uint32_t* la,
+                                                 uint32_t na, const uint32_t* lp,
+                                                 const uint64_t* lofs, const uint16_t* li,
+                                                 uint32_t cnt, const uint32_t* bm,
+                                                 uint32_t bm_bits, uint32_t* fl, uint32_t base0,
+                                                 uint32_t end, uint32_t* lo, uint32_t lane) {
+  const uint32_t lo0 = *lo;
+#pragma unroll
+  for (uint32_t u = 0; u < kU; u++) fl[u * 64 + lane] = 0;
+  wave_sync();
+#pragma unroll
+  for (uint32_t k = 0; k < kU; k++) {
+    const uint32_t t = lo0 + k * 64 + lane;
+    if (t < cnt) {
+      const uint32_t q = lp[t] - base0 - 1u;  // lp[t] > base0 for t >= lo0
+      if (q < 64u * kU) fl[q] = 1;
+    }
+  }
+  wave_sync();
+  uint64_t M[kU];
+#pragma unroll
+  for (uint32_t u = 0; u < kU; u++) M[u] = __ballot(fl[u * 64 + lane] != 0);
+  wave_sync();  // every flag read before the next group clears them
+  uint64_t bw[kU];
+  uint32_t arc[kU];
+  bool act[kU];
+  uint32_t before = 0;
+#pragma unroll
+  for (uint32_t u = 0; u < kU; u++) {
+    const uint32_t r = lo0 + before + uint32_t(__popcll(M[u] & lanes_below(lane)));
+    const uint32_t x = base0 + 64u * u + lane;
+    act[u] = x < end;
+    arc[u] = r;
+    bw[u] = 0;
+    if (act[u]) bw[u] = lofs[r] + (x - (r ? lp[r - 1] : 0u));
+    before += uint32_t(__popcll(M[u]));
+  }
+  *lo = lo0 + before;
+  uint32_t w[kU];
+  bool cand[kU];
+  bool any = false;
+#pragma unroll
+  for (uint32_t u = 0; u < kU; u++) w[u] = act[u] ? nbr[bw[u]] : 0u;
+#pragma unroll
+  for (uint32_t u = 0; u < kU; u++) {
+    const uint32_t h = bm_hash(w[u], bm_bits);
+    cand[u] = act[u] && ((bm[h >> 5] >> (h & 31u)) & 1u);
+    any |= cand[u];
+  }
+  if (!__ballot(any)) return 0;
+  uint32_t pos[kU];
+#pragma unroll
+  for (uint32_t u = 0; u < kU; u++) pos[u] = 0;
+  for (uint32_t n = na; n > 1;) {  // pos + n <= na throughout
+    const uint32_t half = n >> 1;
+#pragma unroll
+    for (uint32_t u = 0; u < kU; u++) pos[u] = la[pos[u] + half] < w[u] ? pos[u] + half : pos[u];
+    n -= half;
+  }
+  unsigned long long c = 0;
+#pragma unroll
+  for (uint32_t u = 0; u < kU; u++) {
+    const uint32_t pw = pos[u] + (la[pos[u]] < w[u] ? 1u : 0u);  // lower bound
+    if (!cand[u] || pw >= na || la[pw] != w[u]) continue;
+    if (!kWeighted) {
+      c++;
+    } else {
+      const uint32_t ik = li[arc[u]];
+      const uint32_t b = la[ik];
+      const bool amax = a > b && a > w[u];
+      const uint64_t e = amax ? bw[u] : (b > w[u] ? a0 + pw : a0 + ik);
+      c += mult[e];
+    }
+  }
+  return c;
+}
+
+// Rows with 2 <= d+(a) <= kWaveRow, one wave each (a single arc closes no
+// triangle).
 template <bool kWeighted>
 __global__ __launch_bounds__(kBlock) void k_tc_wave(const uint64_t* ofs, const uint32_t* nbr,
                                                     const uint8_t* mult, uint64_t V,
                                                     unsigned long long* partial) {
-  __shared__ uint32_t l_a[kBlock / 64][kWaveRow];
-  __shared__ uint32_t l_pre[kBlock / 64][kWaveRow];
-  __shared__ unsigned long long red[kBlock / 64];
+  constexpr uint32_t kWaves = kBlock / 64;
+  constexpr uint32_t kBmWords = (1u << kWaveBmBits) / 32u;
+  __shared__ uint32_t l_a[kWaves][kWaveRow];
+  __shared__ uint32_t l_p[kWaves][kWaveRow];
+  __shared__ uint64_t l_o[kWaves][kWaveRow];
+  __shared__ uint16_t l_i[kWaves][kWeighted ? kWaveRow : 1];
+  __shared__ uint32_t l_bm[kWaves][kBmWords];
+  __shared__ uint32_t l_fl[kWaves][kU * 64];
+  __shared__ unsigned long long red[kWaves];
   const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const uint64_t nwaves = uint64_t(gridDim.x) * (kBlock / 64);
+  uint32_t* la = l_a[wave];
+  uint32_t* lp = l_p[wave];
+  uint64_t* lofs = l_o[wave];
+  uint16_t* li = l_i[wave];
+  uint32_t* bm = l_bm[wave];
+  const uint64_t nwaves = uint64_t(gridDim.x) * kWaves;
   unsigned long long c = 0;
-  for (uint64_t a = uint64_t(blockIdx.x) * (kBlock / 64) + wave; a < V; a += nwaves) {
+  for (uint64_t a = uint64_t(blockIdx.x) * kWaves + wave; a < V; a += nwaves) {
     const uint64_t a0 = ofs[a];
     const uint32_t na = uint32_t(ofs[a + 1] - a0);
-    if (na < 2 || na > kWaveRow) continue;  // a single arc closes no triangle
-    for (uint32_t i = lane; i < na; i += 64) {
-      const uint32_t b = nbr[a0 + i];
-      l_a[wave][i] = b;
-      l_pre[wave][i] = uint32_t(ofs[b + 1] - ofs[b]);
+    if (na < 2 || na > kWaveRow) continue;
+    for (uint32_t k = lane; k < kBmWords; k += 64) bm[k] = 0;
+    wave_sync();
+    uint32_t cnt = 0;
+    for (uint32_t i0 = 0; i0 < na; i0 += 64) {
+      const uint32_t i = i0 + lane;
+      uint32_t d = 0;
+      uint64_t ob = 0;
+      if (i < na) {
+        const uint32_t b = nbr[a0 + i];
+        la[i] = b;
+        ob = ofs[b];
+        d = uint32_t(ofs[b + 1] - ob);
+        const uint32_t h = bm_hash(b, kWaveBmBits);
+        atomicOr(&bm[h >> 5], 1u << (h & 31u));
+      }
+      const uint64_t m = __ballot(d != 0);
+      if (d) {
+        const uint32_t pos = cnt + uint32_t(__popcll(m & lanes_below(lane)));
+        lp[pos] = d;
+        lofs[pos] = ob;
+        if (kWeighted) li[pos] = uint16_t(i);
+      }
+      cnt += uint32_t(__popcll(m));
     }
     wave_sync();
-    // inclusive scan of d+(b) in chunks of 64
+    // inclusive scan of lp[0..cnt) in chunks of 64
     uint32_t carry = 0;
-    for (uint32_t base = 0; base < na; base += 64) {
+    for (uint32_t base = 0; base < cnt; base += 64) {
       const uint32_t i = base + lane;
-      uint32_t v = i < na ? l_pre[wave][i] : 0;
+      uint32_t v = i < cnt ? lp[i] : 0;
       for (int o = 1; o < 64; o <<= 1) {
         uint32_t t = __shfl_up(v, o, 64);
         if (lane >= uint32_t(o)) v += t;
       }
-      if (i < na) l_pre[wave][i] = v + carry;
+      if (i < cnt) lp[i] = v + carry;
       carry += __shfl(v, 63, 64);
     }
     wave_sync();
-    c += count_row<true, kWeighted>(ofs, nbr, mult, a, l_a[wave], na, l_pre[wave], lane, 64);
+    const uint32_t total = carry;
+    uint32_t lo = 0;
+    for (uint32_t base0 = 0; base0 < total; base0 += 64u * kU)
+      c += count_group<kWeighted>(nbr, mult, a, a0, la, na, lp, lofs, li, cnt, bm, kWaveBmBits,
+                                  l_fl[wave], base0, total, &lo, lane);
     wave_sync();
   }
   for (int o = 32; o > 0; o >>= 1) c += __shfl_down(c, o, 64);
@@ -177,19 +329,113 @@ __global__ __launch_bounds__(kBlock) void k_tc_wave(const uint64_t* ofs, const u
   __syncthreads();
   if (threadIdx.x == 0) {
     unsigned long long t = 0;
-    for (int i = 0; i < kBlock / 64; i++) t += red[i];
+    for (uint32_t i = 0; i < kWaves; i++) t += red[i];
     partial[blockIdx.x] = t;
   }
 }
 
-// Long rows (d+(a) > kWaveRow), one workgroup each.
+// Rows with kWaveRow < d+(a) <= kBlockRow, one workgroup each: the row is
+// staged once and its groups of kU chunks are dealt to the waves in turn.
 template <bool kWeighted>
-__global__ __launch_bounds__(kBlock) void k_tc_block(const uint64_t* ofs, const uint32_t* nbr,
-                                                     const uint8_t* mult, const uint32_t* rows,
-                                                     uint64_t nrows, uint32_t* gpre,
-                                                     unsigned long long* partial) {
-  __shared__ uint32_t l_a[kBlockRow];
-  __shared__ uint32_t l_pre[kBlockRow];
+__global__ __launch_bounds__(kRowBlock) void k_tc_block(const uint64_t* ofs, const uint32_t* nbr,
+                                                        const uint8_t* mult, const uint32_t* rows,
+                                                        uint64_t nrows,
+                                                        unsigned long long* partial) {
+  constexpr uint32_t kWaves = kRowBlock / 64;
+  constexpr uint32_t kBmWords = (1u << kBlockBmBits) / 32u;
+  __shared__ uint32_t la[kBlockRow];
+  __shared__ uint32_t lp[kBlockRow];
+  __shared__ uint64_t lofs[kBlockRow];
+  __shared__ uint16_t li[kWeighted ? kBlockRow : 1];
+  __shared__ uint32_t bm[kBmWords];
+  __shared__ uint32_t l_fl[kWaves][kU * 64];
+  __shared__ uint32_t wtot[kWaves];
+  __shared__ uint32_t ncomp;
+  __shared__ unsigned long long red[kWaves];
+  const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  unsigned long long c = 0;
+  for (uint64_t r = blockIdx.x; r < nrows; r += gridDim.x) {
+    const uint64_t a = rows[r];
+    const uint64_t a0 = ofs[a];
+    const uint32_t na = uint32_t(ofs[a + 1] - a0);  // in (kWaveRow, kBlockRow]
+    for (uint32_t k = tid; k < kBmWords; k += kRowBlock) bm[k] = 0;
+    if (tid == 0) ncomp = 0;
+    __syncthreads();
+    for (uint32_t i0 = 0; i0 < na; i0 += kRowBlock) {
+      const uint32_t i = i0 + tid;
+      uint32_t d = 0;
+      uint64_t ob = 0;
+      if (i < na) {
+        const uint32_t b = nbr[a0 + i];
+        la[i] = b;
+        ob = ofs[b];
+        d = uint32_t(ofs[b + 1] - ob);
+        const uint32_t h = bm_hash(b, kBlockBmBits);
+        atomicOr(&bm[h >> 5], 1u << (h & 31u));
+      }
+      const uint64_t m = __ballot(d != 0);
+      uint32_t wbase = 0;
+      if (lane == 0 && m) wbase = atomicAdd(&ncomp, uint32_t(__popcll(m)));
+      wbase = uint32_t(__shfl(int(wbase), 0, 64));
+      if (d) {
+        const uint32_t pos = wbase + uint32_t(__popcll(m & lanes_below(lane)));
+        lp[pos] = d;
+        lofs[pos] = ob;
+        if (kWeighted) li[pos] = uint16_t(i);
+      }
+    }
+    __syncthreads();
+    const uint32_t cnt = ncomp;
+    // block-wide inclusive scan of lp[0..cnt) in chunks of kRowBlock
+    uint32_t carry = 0;
+    for (uint32_t base = 0; base < cnt; base += kRowBlock) {
+      const uint32_t i = base + tid;
+      uint32_t v = i < cnt ? lp[i] : 0;
+      for (int o = 1; o < 64; o <<= 1) {
+        uint32_t t = __shfl_up(v, o, 64);
+        if (lane >= uint32_t(o)) v += t;
+      }
+      if (lane == 63) wtot[wave] = v;
+      __syncthreads();
+      uint32_t off = carry;
+      for (uint32_t k = 0; k < wave; k++) off += wtot[k];
+      uint32_t chunk = 0;
+      for (uint32_t k = 0; k < kWaves; k++) chunk += wtot[k];
+      if (i < cnt) lp[i] = v + off;
+      carry += chunk;
+      __syncthreads();
+    }
+    const uint32_t total = carry;
+    // wave w takes the contiguous elements [w * per, (w + 1) * per)
+    const uint32_t per = ((total + kWaves - 1) / kWaves + 63u) & ~63u;
+    const uint32_t wb = min(total, wave * per), we = min(total, wb + per);
+    uint32_t lo = 0, hi = cnt;  // first compacted arc with lp > wb
+    while (lo < hi) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (lp[mid] <= wb) lo = mid + 1; else hi = mid;
+    }
+    for (uint32_t base0 = wb; base0 < we; base0 += 64u * kU)
+      c += count_group<kWeighted>(nbr, mult, a, a0, la, na, lp, lofs, li, cnt, bm, kBlockBmBits,
+                                  l_fl[wave], base0, we, &lo, lane);
+    __syncthreads();  // the row's LDS is read before the next row overwrites it
+  }
+  for (int o = 32; o > 0; o >>= 1) c += __shfl_down(c, o, 64);
+  if (lane == 0) red[wave] = c;
+  __syncthreads();
+  if (tid == 0) {
+    unsigned long long t = 0;
+    for (uint32_t i = 0; i < kWaves; i++) t += red[i];
+    partial[blockIdx.x] = t;
+  }
+}
+
+// Rows with d+(a) > kBlockRow, one workgroup each; the prefix sums of d+(b)
+// live in gpre at the row's own arc range and N+(a) is searched in place.
+template <bool kWeighted>
+__global__ __launch_bounds__(kBlock) void k_tc_global(const uint64_t* ofs, const uint32_t* nbr,
+                                                      const uint8_t* mult, const uint32_t* rows,
+                                                      uint64_t nrows, uint32_t* gpre,
+                                                      unsigned long long* partial) {
   __shared__ uint32_t wtot[kBlock / 64];
   __shared__ unsigned long long red[kBlock / 64];
   const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -198,11 +444,9 @@ __global__ __launch_bounds__(kBlock) void k_tc_block(const uint64_t* ofs, const 
     const uint64_t a = rows[r];
     const uint64_t a0 = ofs[a];
     const uint32_t na = uint32_t(ofs[a + 1] - a0);
-    const bool lds = na <= kBlockRow;
-    uint32_t* pre = lds ? l_pre : gpre + a0;  // global prefix slot = the row's own arc range
+    uint32_t* pre = gpre + a0;
     for (uint32_t i = threadIdx.x; i < na; i += kBlock) {
       const uint32_t b = nbr[a0 + i];
-      if (lds) l_a[i] = b;
       pre[i] = uint32_t(ofs[b + 1] - ofs[b]);
     }
     __syncthreads();
@@ -225,8 +469,7 @@ __global__ __launch_bounds__(kBlock) void k_tc_block(const uint64_t* ofs, const 
       carry += chunk;
       __syncthreads();
     }
-    if (lds) c += count_row<true, kWeighted>(ofs, nbr, mult, a, l_a, na, l_pre, threadIdx.x, kBlock);
-    else c += count_row<false, kWeighted>(ofs, nbr, mult, a, nullptr, na, pre, threadIdx.x, kBlock);
+    c += count_row_global<kWeighted>(ofs, nbr, mult, a, na, pre, threadIdx.x, kBlock);
     __syncthreads();
   }
   for (int o = 32; o > 0; o >>= 1) c += __shfl_down(c, o, 64);
@@ -239,13 +482,15 @@ __global__ __launch_bounds__(kBlock) void k_tc_block(const uint64_t* ofs, const 
   }
 }
 
-__global__ void k_long_rows(const uint64_t* ofs, uint64_t V, uint32_t* rows,
-                            unsigned long long* n, unsigned long long* work) {
+// Rows by length: kWaveRow < d+ <= kBlockRow from the front of `rows`
+// (count n[0]), d+ > kBlockRow from the back (count n[2]).
+__global__ void k_classify_rows(const uint64_t* ofs, uint64_t V, uint32_t* rows,
+                                unsigned long long* n) {
   const uint64_t a = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (a >= V) return;
   const uint64_t na = ofs[a + 1] - ofs[a];
-  if (na > kWaveRow) rows[atomicAdd(n, 1ull)] = uint32_t(a);
-  (void)work;
+  if (na > kBlockRow) rows[V - 1 - atomicAdd(n + 2, 1ull)] = uint32_t(a);
+  else if (na > kWaveRow) rows[atomicAdd(n, 1ull)] = uint32_t(a);
 }
 
 // W = sum over arcs (a,b) of (d+(a) + d+(b)): intersection element reads.
@@ -352,19 +597,22 @@ TcResult tc_device(ll_gpu_graph* g, int level, bool weighted) {
   const uint8_t* mult = g->sym_mult.as<uint8_t>();
   TcResult r;
   r.arcs = g->sym_edges;
-  DevBuf cnt, rows, gpre, part_w, part_b, total;
+  DevBuf cnt, rows, gpre, part_w, part_b, part_g, total;
   cnt.alloc(3 * 8, s);
   k_zero_ull2<<<1, 1, 0, s>>>(cnt.as<unsigned long long>(), 3);
-  rows.alloc(V * 4, s);
-  k_long_rows<<<grid_for(V, 256), 256, 0, s>>>(ofs, V, rows.as<uint32_t>(),
-                                               cnt.as<unsigned long long>(), nullptr);
+  rows.alloc(std::max<uint64_t>(V, 1) * 4, s);
+  k_classify_rows<<<grid_for(V, 256), 256, 0, s>>>(ofs, V, rows.as<uint32_t>(),
+                                                   cnt.as<unsigned long long>());
   k_tc_work<<<grid_for(V, 256), 256, 0, s>>>(ofs, nbr, V, cnt.as<unsigned long long>() + 1);
   LLG_HIP(hipGetLastError());
-  unsigned long long h[2];
-  LLG_HIP(hipMemcpyAsync(h, cnt.p, 16, hipMemcpyDeviceToHost, s));
+  unsigned long long h[3];
+  LLG_HIP(hipMemcpyAsync(h, cnt.p, 24, hipMemcpyDeviceToHost, s));
   LLG_HIP(hipStreamSynchronize(s));
-  const uint64_t nlong = h[0];
+  const uint64_t nblock = h[0], nglobal = h[2];
   r.work = h[1];
+  LLG_CHECK(nblock + nglobal <= V, LL_GPU_ESTATE, "row classification overflow");
+  const uint32_t* rows_b = rows.as<uint32_t>();
+  const uint32_t* rows_g = rows.as<uint32_t>() + (V - nglobal);
   const unsigned gw = unsigned(std::min<uint64_t>(std::max<uint64_t>((V + 3) / 4, 1),
                                                   uint64_t(ctx->num_cus) * 32));
   part_w.alloc(gw * 8, s);
@@ -375,31 +623,47 @@ TcResult tc_device(ll_gpu_graph* g, int level, bool weighted) {
     else
       k_tc_wave<false><<<gw, kBlock, 0, s>>>(ofs, nbr, mult, V, part_w.as<unsigned long long>());
   }
-  const unsigned gb = unsigned(std::min<uint64_t>(std::max<uint64_t>(nlong, 1),
-                                                  uint64_t(ctx->num_cus) * 8));
+  const unsigned gb = unsigned(std::min<uint64_t>(std::max<uint64_t>(nblock, 1),
+                                                  uint64_t(ctx->num_cus) * 4));
   part_b.alloc(gb * 8, s);
-  if (nlong) {
-    gpre.alloc(std::max<uint64_t>(g->sym_edges, 1) * 4, s);
+  if (nblock) {
     TimedLaunch t(ctx, "k_tc_block");
     if (weighted)
-      k_tc_block<true><<<gb, kBlock, 0, s>>>(ofs, nbr, mult, rows.as<uint32_t>(), nlong,
-                                             gpre.as<uint32_t>(), part_b.as<unsigned long long>());
+      k_tc_block<true><<<gb, kRowBlock, 0, s>>>(ofs, nbr, mult, rows_b, nblock,
+                                                part_b.as<unsigned long long>());
     else
-      k_tc_block<false><<<gb, kBlock, 0, s>>>(ofs, nbr, mult, rows.as<uint32_t>(), nlong,
-                                              gpre.as<uint32_t>(), part_b.as<unsigned long long>());
+      k_tc_block<false><<<gb, kRowBlock, 0, s>>>(ofs, nbr, mult, rows_b, nblock,
+                                                 part_b.as<unsigned long long>());
   } else {
     k_zero_ull2<<<1, 1, 0, s>>>(part_b.as<unsigned long long>(), int(gb));
   }
+  const unsigned gg = unsigned(std::min<uint64_t>(std::max<uint64_t>(nglobal, 1),
+                                                  uint64_t(ctx->num_cus) * 8));
+  part_g.alloc(gg * 8, s);
+  if (nglobal) {
+    gpre.alloc(std::max<uint64_t>(g->sym_edges, 1) * 4, s);
+    TimedLaunch t(ctx, "k_tc_global");
+    if (weighted)
+      k_tc_global<true><<<gg, kBlock, 0, s>>>(ofs, nbr, mult, rows_g, nglobal,
+                                              gpre.as<uint32_t>(), part_g.as<unsigned long long>());
+    else
+      k_tc_global<false><<<gg, kBlock, 0, s>>>(ofs, nbr, mult, rows_g, nglobal,
+                                               gpre.as<uint32_t>(), part_g.as<unsigned long long>());
+  } else {
+    k_zero_ull2<<<1, 1, 0, s>>>(part_g.as<unsigned long long>(), int(gg));
+  }
   LLG_HIP(hipGetLastError());
-  total.alloc(16, s);
+  total.alloc(24, s);
   k_sum_partials<<<1, 1024, 0, s>>>(part_w.as<unsigned long long>(), gw,
                                     total.as<unsigned long long>());
   k_sum_partials<<<1, 1024, 0, s>>>(part_b.as<unsigned long long>(), gb,
                                     total.as<unsigned long long>() + 1);
+  k_sum_partials<<<1, 1024, 0, s>>>(part_g.as<unsigned long long>(), gg,
+                                    total.as<unsigned long long>() + 2);
   LLG_HIP(hipGetLastError());
-  LLG_HIP(hipMemcpyAsync(h, total.p, 16, hipMemcpyDeviceToHost, s));
+  LLG_HIP(hipMemcpyAsync(h, total.p, 24, hipMemcpyDeviceToHost, s));
   LLG_HIP(hipStreamSynchronize(s));
-  r.triangles = h[0] + h[1];
+  r.triangles = h[0] + h[1] + h[2];
   return r;
 }
 
