@@ -35,12 +35,14 @@ namespace llgpu {
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int kRowBlock = 512;         // workgroup of the kWaveRow..kBlockRow rows
+constexpr int kRowBlock = 1024;        // workgroup of the kWaveRow..kBlockRow rows
 constexpr uint32_t kWaveRow = 256;     // row length handled by one wave (LDS)
 constexpr uint32_t kBlockRow = 4096;   // row length handled by one workgroup (LDS)
 constexpr uint32_t kWaveBmBits = 12;   // membership bitmap of a wave row: 4096 bits
 constexpr uint32_t kBlockBmBits = 15;  // of a workgroup row: 32768 bits
 constexpr uint32_t kU = 4;             // 64-element chunks a wave resolves together
+constexpr uint32_t kPosShift = 48;     // staged list start | position of b in N+(a) << 48
+constexpr uint64_t kOfsMask = (1ull << kPosShift) - 1;
 
 __global__ void k_orient(const uint64_t* ofs, const uint32_t* adj, const uint32_t* dout,
                          const uint32_t* din, uint64_t V, uint64_t* keys, uint64_t E) {
@@ -158,8 +160,9 @@ __device__ inline uint64_t lanes_below(uint32_t lane) {
 
 // The staged row a (LDS): la[0..na) = N+(a) ascending, bm = its bitmap,
 // lp[0..cnt) = inclusive prefix sums of d+(b) over the arcs with d+(b) > 0
-// ("compacted arcs", in any order), lofs[k] = ofs[b] of compacted arc k and
-// li[k] = the position of its b in la (weighted count only).  Element x in
+// ("compacted arcs", in any order), lofs[k] = ofs[b] of compacted arc k, with
+// the position of its b in la in the bits from kPosShift up (weighted count
+// only; list starts stay below 2^kPosShift).  Element x in
 // [0, lp[cnt-1]) is entry x - lp[k-1] of N+(b) for the compacted arc k with
 // lp[k-1] <= x < lp[k].
 //
@@ -177,9 +180,9 @@ template <bool kWeighted>
 __device__ inline unsigned long long count_group(const uint32_t* nbr, const uint8_t* mult,
                                                  uint64_t a, uint64_t a0, const uint32_t* la,
                                                  uint32_t na, const uint32_t* lp,
-                                                 const uint64_t* lofs, const uint16_t* li,
-                                                 uint32_t cnt, const uint32_t* bm,
-                                                 uint32_t bm_bits, uint32_t* fl, uint32_t base0,
+                                                 const uint64_t* lofs, uint32_t cnt,
+                                                 const uint32_t* bm, uint32_t bm_bits,
+                                                 uint8_t* fl, uint32_t base0,
                                                  uint32_t end, uint32_t* lo, uint32_t lane) {
   const uint32_t lo0 = *lo;
 #pragma unroll
@@ -209,7 +212,7 @@ __device__ inline unsigned long long count_group(const uint32_t* nbr, const uint
     act[u] = x < end;
     arc[u] = r;
     bw[u] = 0;
-    if (act[u]) bw[u] = lofs[r] + (x - (r ? lp[r - 1] : 0u));
+    if (act[u]) bw[u] = (lofs[r] & kOfsMask) + (x - (r ? lp[r - 1] : 0u));
     before += uint32_t(__popcll(M[u]));
   }
   *lo = lo0 + before;
@@ -242,7 +245,7 @@ __device__ inline unsigned long long count_group(const uint32_t* nbr, const uint
     if (!kWeighted) {
       c++;
     } else {
-      const uint32_t ik = li[arc[u]];
+      const uint32_t ik = uint32_t(lofs[arc[u]] >> kPosShift);
       const uint32_t b = la[ik];
       const bool amax = a > b && a > w[u];
       const uint64_t e = amax ? bw[u] : (b > w[u] ? a0 + pw : a0 + ik);
@@ -263,15 +266,13 @@ __global__ __launch_bounds__(kBlock) void k_tc_wave(const uint64_t* ofs, const u
   __shared__ uint32_t l_a[kWaves][kWaveRow];
   __shared__ uint32_t l_p[kWaves][kWaveRow];
   __shared__ uint64_t l_o[kWaves][kWaveRow];
-  __shared__ uint16_t l_i[kWaves][kWeighted ? kWaveRow : 1];
   __shared__ uint32_t l_bm[kWaves][kBmWords];
-  __shared__ uint32_t l_fl[kWaves][kU * 64];
+  __shared__ uint8_t l_fl[kWaves][kU * 64];
   __shared__ unsigned long long red[kWaves];
   const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   uint32_t* la = l_a[wave];
   uint32_t* lp = l_p[wave];
   uint64_t* lofs = l_o[wave];
-  uint16_t* li = l_i[wave];
   uint32_t* bm = l_bm[wave];
   const uint64_t nwaves = uint64_t(gridDim.x) * kWaves;
   unsigned long long c = 0;
@@ -298,8 +299,7 @@ __global__ __launch_bounds__(kBlock) void k_tc_wave(const uint64_t* ofs, const u
       if (d) {
         const uint32_t pos = cnt + uint32_t(__popcll(m & lanes_below(lane)));
         lp[pos] = d;
-        lofs[pos] = ob;
-        if (kWeighted) li[pos] = uint16_t(i);
+        lofs[pos] = kWeighted ? ob | (uint64_t(i) << kPosShift) : ob;
       }
       cnt += uint32_t(__popcll(m));
     }
@@ -320,7 +320,7 @@ __global__ __launch_bounds__(kBlock) void k_tc_wave(const uint64_t* ofs, const u
     const uint32_t total = carry;
     uint32_t lo = 0;
     for (uint32_t base0 = 0; base0 < total; base0 += 64u * kU)
-      c += count_group<kWeighted>(nbr, mult, a, a0, la, na, lp, lofs, li, cnt, bm, kWaveBmBits,
+      c += count_group<kWeighted>(nbr, mult, a, a0, la, na, lp, lofs, cnt, bm, kWaveBmBits,
                                   l_fl[wave], base0, total, &lo, lane);
     wave_sync();
   }
@@ -346,9 +346,8 @@ __global__ __launch_bounds__(kRowBlock) void k_tc_block(const uint64_t* ofs, con
   __shared__ uint32_t la[kBlockRow];
   __shared__ uint32_t lp[kBlockRow];
   __shared__ uint64_t lofs[kBlockRow];
-  __shared__ uint16_t li[kWeighted ? kBlockRow : 1];
   __shared__ uint32_t bm[kBmWords];
-  __shared__ uint32_t l_fl[kWaves][kU * 64];
+  __shared__ uint8_t l_fl[kWaves][kU * 64];
   __shared__ uint32_t wtot[kWaves];
   __shared__ uint32_t ncomp;
   __shared__ unsigned long long red[kWaves];
@@ -380,8 +379,7 @@ __global__ __launch_bounds__(kRowBlock) void k_tc_block(const uint64_t* ofs, con
       if (d) {
         const uint32_t pos = wbase + uint32_t(__popcll(m & lanes_below(lane)));
         lp[pos] = d;
-        lofs[pos] = ob;
-        if (kWeighted) li[pos] = uint16_t(i);
+        lofs[pos] = kWeighted ? ob | (uint64_t(i) << kPosShift) : ob;
       }
     }
     __syncthreads();
@@ -415,7 +413,7 @@ __global__ __launch_bounds__(kRowBlock) void k_tc_block(const uint64_t* ofs, con
       if (lp[mid] <= wb) lo = mid + 1; else hi = mid;
     }
     for (uint32_t base0 = wb; base0 < we; base0 += 64u * kU)
-      c += count_group<kWeighted>(nbr, mult, a, a0, la, na, lp, lofs, li, cnt, bm, kBlockBmBits,
+      c += count_group<kWeighted>(nbr, mult, a, a0, la, na, lp, lofs, cnt, bm, kBlockBmBits,
                                   l_fl[wave], base0, we, &lo, lane);
     __syncthreads();  // the row's LDS is read before the next row overwrites it
   }
@@ -595,6 +593,7 @@ TcResult tc_device(ll_gpu_graph* g, int level, bool weighted) {
   const uint64_t* ofs = g->sym_ofs.as<uint64_t>();
   const uint32_t* nbr = g->sym_adj.as<uint32_t>();
   const uint8_t* mult = g->sym_mult.as<uint8_t>();
+  LLG_CHECK(g->sym_edges <= kOfsMask, LL_GPU_ERANGE, "oriented arcs beyond 2^48");
   TcResult r;
   r.arcs = g->sym_edges;
   DevBuf cnt, rows, gpre, part_w, part_b, part_g, total;
