@@ -44,23 +44,28 @@ constexpr uint32_t kU = 4;             // 64-element chunks a wave resolves toge
 constexpr uint32_t kPosShift = 48;     // staged list start | position of b in N+(a) << 48
 constexpr uint64_t kOfsMask = (1ull << kPosShift) - 1;
 
-__global__ void k_orient(const uint64_t* ofs, const uint32_t* adj, const uint32_t* dout,
-                         const uint32_t* din, uint64_t V, uint64_t* keys, uint64_t E) {
-  // one thread per vertex row; arcs written at the row's own positions
-  const uint64_t u = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (u >= V) return;
-  const uint64_t du = uint64_t(dout[u]) + din[u];
-  for (uint64_t e = ofs[u], end = ofs[u + 1]; e < end; e++) {
-    const uint64_t v = adj[e];
-    uint64_t key = ~0ull;  // self-loops sort last and are cut
-    if (v != u) {
-      const uint64_t dv = uint64_t(dout[v]) + din[v];
-      const bool uv = du < dv || (du == dv && u < v);
-      key = uv ? (u << 32) | v : (v << 32) | u;
+// One wave per vertex row (grid-stride), lanes over the row's edges: a hub's
+// row is spread over 64 lanes instead of serialising one thread.  Arcs are
+// written at the row's own edge positions.
+__global__ __launch_bounds__(256) void k_orient(const uint64_t* ofs, const uint32_t* adj,
+                                                const uint32_t* dout, const uint32_t* din,
+                                                uint64_t V, uint64_t* keys) {
+  const uint32_t lane = threadIdx.x & 63;
+  const uint64_t nw = uint64_t(gridDim.x) * (blockDim.x / 64);
+  for (uint64_t u = uint64_t(blockIdx.x) * (blockDim.x / 64) + (threadIdx.x >> 6); u < V;
+       u += nw) {
+    const uint64_t du = uint64_t(dout[u]) + din[u];
+    for (uint64_t e = ofs[u] + lane, end = ofs[u + 1]; e < end; e += 64) {
+      const uint64_t v = adj[e];
+      uint64_t key = ~0ull;  // self-loops sort last and are cut
+      if (v != u) {
+        const uint64_t dv = uint64_t(dout[v]) + din[v];
+        const bool uv = du < dv || (du == dv && u < v);
+        key = uv ? (u << 32) | v : (v << 32) | u;
+      }
+      keys[e] = key;
     }
-    keys[e] = key;
   }
-  (void)E;
 }
 
 // Row offsets of the sorted arc keys: ofs[x] = first arc with tail >= x.
@@ -492,19 +497,27 @@ __global__ void k_classify_rows(const uint64_t* ofs, uint64_t V, uint32_t* rows,
 }
 
 // W = sum over arcs (a,b) of (d+(a) + d+(b)): intersection element reads.
-__global__ void k_tc_work(const uint64_t* ofs, const uint32_t* nbr, uint64_t V,
-                          unsigned long long* work) {
-  const uint64_t a = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+// One wave per row (grid-stride), one atomic per workgroup.
+__global__ __launch_bounds__(256) void k_tc_work(const uint64_t* ofs, const uint32_t* nbr,
+                                                 uint64_t V, unsigned long long* work) {
+  __shared__ unsigned long long red[4];
+  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint64_t nw = uint64_t(gridDim.x) * 4;
   unsigned long long w = 0;
-  if (a < V) {
+  for (uint64_t a = uint64_t(blockIdx.x) * 4 + wave; a < V; a += nw) {
     const uint64_t a0 = ofs[a], na = ofs[a + 1] - a0;
-    for (uint64_t i = 0; i < na; i++) {
+    for (uint64_t i = lane; i < na; i += 64) {
       const uint32_t b = nbr[a0 + i];
       w += na + (ofs[b + 1] - ofs[b]);
     }
   }
   for (int o = 32; o > 0; o >>= 1) w += __shfl_down(w, o, 64);
-  if ((threadIdx.x & 63) == 0 && w) atomicAdd(work, w);
+  if (lane == 0) red[wave] = w;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const unsigned long long t = red[0] + red[1] + red[2] + red[3];
+    if (t) atomicAdd(work, t);
+  }
 }
 
 __global__ void k_sum_partials(const unsigned long long* p, uint64_t n, unsigned long long* out) {
@@ -546,8 +559,10 @@ void ensure_dag(ll_gpu_graph* g, int level) {
   keys.alloc(std::max<uint64_t>(E, 1) * 8, s);
   {
     TimedLaunch t(ctx, "k_orient");
-    k_orient<<<grid_for(V, 256), 256, 0, s>>>(out.ofs.as<uint64_t>(), out.adj.as<uint32_t>(),
-                                              dout, din, V, keys.as<uint64_t>(), E);
+    const unsigned go = unsigned(std::min<uint64_t>(std::max<uint64_t>((V + 3) / 4, 1),
+                                                    uint64_t(ctx->num_cus) * 32));
+    k_orient<<<go, 256, 0, s>>>(out.ofs.as<uint64_t>(), out.adj.as<uint32_t>(), dout, din, V,
+                                keys.as<uint64_t>());
   }
   LLG_HIP(hipGetLastError());
   sorted.alloc(std::max<uint64_t>(E, 1) * 8, s);
@@ -602,7 +617,9 @@ TcResult tc_device(ll_gpu_graph* g, int level, bool weighted) {
   rows.alloc(std::max<uint64_t>(V, 1) * 4, s);
   k_classify_rows<<<grid_for(V, 256), 256, 0, s>>>(ofs, V, rows.as<uint32_t>(),
                                                    cnt.as<unsigned long long>());
-  k_tc_work<<<grid_for(V, 256), 256, 0, s>>>(ofs, nbr, V, cnt.as<unsigned long long>() + 1);
+  k_tc_work<<<unsigned(std::min<uint64_t>(std::max<uint64_t>((V + 3) / 4, 1),
+                                           uint64_t(ctx->num_cus) * 32)),
+              256, 0, s>>>(ofs, nbr, V, cnt.as<unsigned long long>() + 1);
   LLG_HIP(hipGetLastError());
   unsigned long long h[3];
   LLG_HIP(hipMemcpyAsync(h, cnt.p, 24, hipMemcpyDeviceToHost, s));
