@@ -546,6 +546,8 @@ int ll_gpu_bfs_ex(ll_gpu_graph* g, int level, int64_t root, int32_t* levels_out,
   } catch (const llgpu::Error& e) {
     g->ctx->err = e.msg;
     return e.code;
+  } catch (...) {
+    return llgpu::api_exception(g->ctx);
   }
 }
 

@@ -708,6 +708,8 @@ int ll_gpu_bfs_batch_ex(ll_gpu_graph* g, int level, const int64_t* roots, int n,
   } catch (const Error& e) {
     g->ctx->err = e.msg;
     return e.code;
+  } catch (...) {
+    return llgpu::api_exception(g->ctx);
   }
 }
 

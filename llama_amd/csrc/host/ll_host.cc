@@ -48,9 +48,27 @@ const Graph& graph_of(const ll_host_graph* h) { return *h->g; }
 
 namespace {
 thread_local std::string g_err;
-int fail(int code, const std::string& msg) {
-  g_err = msg;
+int fail(int code, const char* msg) noexcept {
+  try {
+    g_err = msg;
+  } catch (...) {
+  }
   return code;
+}
+int fail(int code, const std::string& msg) noexcept { return fail(code, msg.c_str()); }
+// Status of the exception in flight inside an entry point's catch (...):
+// std::bad_alloc -> LL_GPU_ENOMEM, anything else -> LL_GPU_ESTATE.  No
+// exception crosses the C ABI.
+int host_exception() noexcept {
+  try {
+    throw;
+  } catch (const std::bad_alloc&) {
+    return fail(LL_GPU_ENOMEM, "out of host memory");
+  } catch (const std::exception& e) {
+    return fail(LL_GPU_ESTATE, e.what());
+  } catch (...) {
+    return fail(LL_GPU_ESTATE, "unknown exception");
+  }
 }
 }  // namespace
 
@@ -58,7 +76,7 @@ extern "C" {
 
 const char* ll_host_last_error(void) { return g_err.c_str(); }
 
-int ll_host_graph_create(uint64_t num_nodes, int deletions, ll_host_graph** out) {
+int ll_host_graph_create(uint64_t num_nodes, int deletions, ll_host_graph** out) try {
   if (!out) return fail(LL_GPU_EINVAL, "out is NULL");
   if (num_nodes == 0 || num_nodes >= (uint64_t(1) << 32))
     return fail(LL_GPU_ERANGE, "num_nodes must be in [1, 2^32)");
@@ -67,6 +85,8 @@ int ll_host_graph_create(uint64_t num_nodes, int deletions, ll_host_graph** out)
   h->g.reset(new llhost::Graph(num_nodes, deletions != 0));
   *out = h;
   return LL_GPU_OK;
+} catch (...) {
+  return host_exception();
 }
 
 void ll_host_graph_destroy(ll_host_graph* g) { delete g; }
@@ -100,7 +120,7 @@ int ll_host_graph_add_level_w(ll_host_graph* g, const uint32_t* ins_src, const u
 }
 
 int ll_host_graph_add_level_edges(ll_host_graph* g, const ll_host_edges* ins,
-                                  const ll_host_edges* del) {
+                                  const ll_host_edges* del) try {
   if (!g) return fail(LL_GPU_EINVAL, "graph is NULL");
   const uint64_t ni = ins ? ins->e.src.size() : 0, nd = del ? del->e.src.size() : 0;
   const bool w = ni && ins->w.size() == ni;  // .xs1 records carry weights
@@ -108,6 +128,8 @@ int ll_host_graph_add_level_edges(ll_host_graph* g, const ll_host_edges* ins,
                                    ni ? ins->e.dst.data() : nullptr, w ? ins->w.data() : nullptr,
                                    ni, nd ? del->e.src.data() : nullptr,
                                    nd ? del->e.dst.data() : nullptr, nd);
+} catch (...) {
+  return host_exception();
 }
 
 // Vertex-table page size of the reference's copy-on-write array
@@ -175,15 +197,17 @@ int ll_host_graph_add_window_level(ll_host_graph* g, uint64_t n_ins, uint64_t n_
 }
 
 int ll_host_graph_last_batch(const ll_host_graph* g, const ll_host_edges** ins,
-                             const ll_host_edges** del) {
+                             const ll_host_edges** del) try {
   if (!g || !ins || !del) return fail(LL_GPU_EINVAL, "NULL argument");
   *ins = &g->last_ins;
   *del = &g->last_del;
   return LL_GPU_OK;
+} catch (...) {
+  return host_exception();
 }
 
 int ll_host_graph_level_view(const ll_host_graph* g, uint32_t dir, uint32_t level,
-                             ll_gpu_level_view* out) {
+                             ll_gpu_level_view* out) try {
   if (!g || !out) return fail(LL_GPU_EINVAL, "NULL argument");
   if (dir > 1) return fail(LL_GPU_EINVAL, "dir must be 0 (out) or 1 (in)");
   if (int(level) >= g->g->num_levels()) return fail(LL_GPU_EINVAL, "no such level");
@@ -199,10 +223,12 @@ int ll_host_graph_level_view(const ll_host_graph* g, uint32_t dir, uint32_t leve
   out->flags = LL_GPU_HAS_CONT | (g->g->deletions() ? LL_GPU_HAS_DELETIONS : 0u);
   out->page_table = nullptr;
   return LL_GPU_OK;
+} catch (...) {
+  return host_exception();
 }
 
 int ll_host_graph_level_pages(const ll_host_graph* g, uint32_t dir, uint32_t level,
-                              const void** pages, uint64_t num_pages, uint64_t* written) {
+                              const void** pages, uint64_t num_pages, uint64_t* written) try {
   if (!g || !pages) return fail(LL_GPU_EINVAL, "NULL argument");
   if (dir > 1) return fail(LL_GPU_EINVAL, "dir must be 0 (out) or 1 (in)");
   if (int(level) >= g->g->num_levels()) return fail(LL_GPU_EINVAL, "no such level");
@@ -219,6 +245,8 @@ int ll_host_graph_level_pages(const ll_host_graph* g, uint32_t dir, uint32_t lev
   }
   if (written) *written = n;
   return LL_GPU_OK;
+} catch (...) {
+  return host_exception();
 }
 
 int ll_host_edges_fill_weights(ll_host_edges* e, uint64_t seed) {
@@ -234,16 +262,18 @@ int ll_host_edges_fill_weights(ll_host_edges* e, uint64_t seed) {
 int ll_host_graph_is_weighted(const ll_host_graph* g) { return g ? int(g->g->weighted()) : 0; }
 
 int ll_host_graph_level_weights(const ll_host_graph* g, uint32_t level, const float** w,
-                                uint64_t* n) {
+                                uint64_t* n) try {
   if (!g || !w || !n) return fail(LL_GPU_EINVAL, "NULL argument");
   if (int(level) >= g->g->num_levels()) return fail(LL_GPU_EINVAL, "no such level");
   const auto& lv = g->g->level(0, int(level));
   *w = lv.wt.empty() ? nullptr : lv.wt.data();
   *n = lv.wt.size();
   return LL_GPU_OK;
+} catch (...) {
+  return host_exception();
 }
 
-int ll_host_graph_flatten_weights(ll_host_graph* g, int level, const float** w, uint64_t* n) {
+int ll_host_graph_flatten_weights(ll_host_graph* g, int level, const float** w, uint64_t* n) try {
   if (!g || !w || !n) return fail(LL_GPU_EINVAL, "NULL argument");
   const int L = g->g->num_levels();
   if (L == 0) return fail(LL_GPU_ESTATE, "graph has no levels");
@@ -253,10 +283,12 @@ int ll_host_graph_flatten_weights(ll_host_graph* g, int level, const float** w, 
   *w = g->flat_w.data();
   *n = g->flat_w.size();
   return LL_GPU_OK;
+} catch (...) {
+  return host_exception();
 }
 
 int ll_host_graph_patches(const ll_host_graph* g, uint32_t dir, uint32_t level,
-                          const uint64_t** edge_ids, const uint32_t** max_levels, uint64_t* n) {
+                          const uint64_t** edge_ids, const uint32_t** max_levels, uint64_t* n) try {
   if (!g || !edge_ids || !max_levels || !n) return fail(LL_GPU_EINVAL, "NULL argument");
   if (dir > 1 || int(level) >= g->g->num_levels()) return fail(LL_GPU_EINVAL, "no such level");
   const auto& p = g->g->patches(int(dir), int(level));
@@ -264,10 +296,12 @@ int ll_host_graph_patches(const ll_host_graph* g, uint32_t dir, uint32_t level,
   *edge_ids = p.edge_ids.data();
   *max_levels = p.maxlev.data();
   return LL_GPU_OK;
+} catch (...) {
+  return host_exception();
 }
 
 int ll_host_graph_flatten(ll_host_graph* g, uint32_t dir, int level, const uint64_t** offsets,
-                          const uint32_t** neighbours, uint64_t* num_edges) {
+                          const uint32_t** neighbours, uint64_t* num_edges) try {
   if (!g || !offsets || !neighbours || !num_edges) return fail(LL_GPU_EINVAL, "NULL argument");
   const int L = g->g->num_levels();
   if (L == 0) return fail(LL_GPU_ESTATE, "graph has no levels");
@@ -278,6 +312,8 @@ int ll_host_graph_flatten(ll_host_graph* g, uint32_t dir, int level, const uint6
   *neighbours = g->flat_adj.data();
   *num_edges = g->flat_adj.size();
   return LL_GPU_OK;
+} catch (...) {
+  return host_exception();
 }
 
 int ll_host_gen_rmat(uint32_t scale, double edge_factor, double a, double b, double c,
@@ -297,7 +333,7 @@ int ll_host_gen_rmat(uint32_t scale, double edge_factor, double a, double b, dou
 }
 
 int ll_host_edges_create(const uint32_t* src, const uint32_t* dst, const float* weight,
-                         uint64_t n, ll_host_edges** out) {
+                         uint64_t n, ll_host_edges** out) try {
   if (!out || (n && (!src || !dst))) return fail(LL_GPU_EINVAL, "NULL argument");
   auto* e = new (std::nothrow) ll_host_edges();
   if (!e) return fail(LL_GPU_ENOMEM, "out of host memory");
@@ -306,6 +342,8 @@ int ll_host_edges_create(const uint32_t* src, const uint32_t* dst, const float* 
   if (weight) e->w.assign(weight, weight + n);
   *out = e;
   return LL_GPU_OK;
+} catch (...) {
+  return host_exception();
 }
 
 void ll_host_edges_destroy(ll_host_edges* e) { delete e; }
@@ -318,7 +356,7 @@ const float* ll_host_edges_weight(const ll_host_edges* e) {
 
 // X-Stream type 1: packed {u32 tail, u32 head, f32 weight}; the .ini side file
 // carries vertices= and edges= (loaders/ll_load_xstream1.h:146-152, 243-273).
-int ll_host_xs1_write(const char* path, const ll_host_edges* e, uint64_t num_nodes) {
+int ll_host_xs1_write(const char* path, const ll_host_edges* e, uint64_t num_nodes) try {
   if (!path || !e) return fail(LL_GPU_EINVAL, "NULL argument");
   FILE* f = std::fopen(path, "wb");
   if (!f) return fail(LL_GPU_EINVAL, std::string("cannot open ") + path);
@@ -346,9 +384,11 @@ int ll_host_xs1_write(const char* path, const ll_host_edges* e, uint64_t num_nod
                (unsigned long long)num_nodes, (unsigned long long)n);
   std::fclose(fi);
   return LL_GPU_OK;
+} catch (...) {
+  return host_exception();
 }
 
-int ll_host_xs1_read(const char* path, ll_host_edges** out, uint64_t* num_nodes) {
+int ll_host_xs1_read(const char* path, ll_host_edges** out, uint64_t* num_nodes) try {
   if (!path || !out) return fail(LL_GPU_EINVAL, "NULL argument");
   FILE* f = std::fopen(path, "rb");
   if (!f) return fail(LL_GPU_EINVAL, std::string("cannot open ") + path);
@@ -389,6 +429,8 @@ int ll_host_xs1_read(const char* path, ll_host_edges** out, uint64_t* num_nodes)
   if (num_nodes) *num_nodes = std::max(vmax, v_ini);
   *out = e;
   return LL_GPU_OK;
+} catch (...) {
+  return host_exception();
 }
 
 }  // extern "C"
@@ -428,7 +470,7 @@ bool net_parse(const char* line, uint32_t* tail, uint32_t* head) {
 }  // namespace
 
 int ll_host_net_read(const char* path, int part, int nparts, ll_host_edges** out,
-                     uint64_t* num_nodes, uint64_t* parse_errors) {
+                     uint64_t* num_nodes, uint64_t* parse_errors) try {
   if (!path || !out) return fail(LL_GPU_EINVAL, "NULL argument");
   if (nparts < 0 || (nparts > 0 && (part < 1 || part > nparts)))
     return fail(LL_GPU_EINVAL, "the partial load part ID is out of bounds");
@@ -488,9 +530,11 @@ int ll_host_net_read(const char* path, int part, int nparts, ll_host_edges** out
   if (parse_errors) *parse_errors = errors;
   *out = e;
   return LL_GPU_OK;
+} catch (...) {
+  return host_exception();
 }
 
-int ll_host_net_write(const char* path, const ll_host_edges* e) {
+int ll_host_net_write(const char* path, const ll_host_edges* e) try {
   if (!path || !e) return fail(LL_GPU_EINVAL, "NULL argument");
   FILE* f = std::fopen(path, "w");
   if (!f) return fail(LL_GPU_EINVAL, std::string("cannot create ") + path);
@@ -500,6 +544,8 @@ int ll_host_net_write(const char* path, const ll_host_edges* e) {
     std::fprintf(f, "%u\t%u\n", e->e.src[i], e->e.dst[i]);
   const bool ok = std::fclose(f) == 0;
   return ok ? LL_GPU_OK : fail(LL_GPU_EINVAL, "write failed");
+} catch (...) {
+  return host_exception();
 }
 
 // ---- base + N splits of an edge list (utils/xs1-split.cpp) ----------------------
@@ -509,7 +555,7 @@ int ll_host_net_write(const char* path, const ll_host_edges* e) {
 // quota (floor(p * records), part 0 takes the remainder) passes the record to
 // the next part with room.  r comes from a seeded generator instead of rand().
 int ll_host_edges_split(const ll_host_edges* in, int n_small, double p_large, int undirected,
-                        uint64_t seed, ll_host_edges** parts) {
+                        uint64_t seed, ll_host_edges** parts) try {
   if (!in || !parts) return fail(LL_GPU_EINVAL, "NULL argument");
   if (n_small <= 0) return fail(LL_GPU_EINVAL, "invalid number of small files");
   if (!(p_large >= 0 && p_large <= 1)) return fail(LL_GPU_EINVAL, "p_large must be in [0, 1]");
@@ -552,4 +598,6 @@ int ll_host_edges_split(const ll_host_edges* in, int n_small, double p_large, in
   }
   for (int i = 0; i <= n_small; i++) parts[i] = out[i];
   return LL_GPU_OK;
+} catch (...) {
+  return host_exception();
 }

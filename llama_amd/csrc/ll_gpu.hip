@@ -31,8 +31,8 @@ int guarded(ll_gpu_ctx* c, F&& f) {
     return LL_GPU_OK;
   } catch (const Error& e) {
     return fail(c, e);
-  } catch (const std::bad_alloc&) {
-    return fail(c, Error{LL_GPU_ENOMEM, "out of host memory"});
+  } catch (...) {
+    return llgpu::api_exception(c);
   }
 }
 
@@ -228,13 +228,17 @@ int ll_gpu_set_option(ll_gpu_ctx* c, const char* name, long value) {
   if (!c || !name) return LL_GPU_EINVAL;
   static const char* known[] = {"pr_reorder", "pr_cap", "pr_seg_bits", "pr_item_sort",
                                 "pr_hot_bits", "sssp_delta_milli", "sssp_min_queue"};
-  for (const char* k : known)
-    if (std::strcmp(k, name) == 0) {
-      c->options[name] = value;
-      return LL_GPU_OK;
-    }
-  c->err = std::string("unknown option ") + name;
-  return LL_GPU_EINVAL;
+  try {
+    for (const char* k : known)
+      if (std::strcmp(k, name) == 0) {
+        c->options[name] = value;
+        return LL_GPU_OK;
+      }
+    c->err = std::string("unknown option ") + name;
+    return LL_GPU_EINVAL;
+  } catch (...) {
+    return llgpu::api_exception(c);
+  }
 }
 
 int ll_gpu_profile(ll_gpu_ctx* c, int on) {

@@ -7,6 +7,8 @@
 #include <list>
 #include <map>
 #include <memory>
+#include <exception>
+#include <new>
 #include <string>
 #include <vector>
 
@@ -144,6 +146,26 @@ struct ll_gpu_ctx {
   std::map<std::string, int> tuned;    // autotuner decisions
   std::map<std::string, long> options; // ll_gpu_set_option knobs
 };
+
+namespace llgpu {
+// Status of the exception in flight inside an entry point's catch (...):
+// std::bad_alloc -> LL_GPU_ENOMEM, anything else -> LL_GPU_ESTATE, with the
+// message kept in the context.  No exception crosses the C ABI.
+inline int api_exception(ll_gpu_ctx* c) noexcept {
+  try {
+    throw;
+  } catch (const std::bad_alloc&) {
+    if (c) c->err = "out of host memory";
+    return LL_GPU_ENOMEM;
+  } catch (const std::exception& e) {
+    if (c) c->err = e.what();
+    return LL_GPU_ESTATE;
+  } catch (...) {
+    if (c) c->err = "unknown exception";
+    return LL_GPU_ESTATE;
+  }
+}
+}  // namespace llgpu
 
 namespace llgpu {
 

@@ -539,6 +539,8 @@ int ll_gpu_pagerank_ex(ll_gpu_graph* g, int level, int iters, double damping, ui
   } catch (const llgpu::Error& e) {
     g->ctx->err = e.msg;
     return e.code;
+  } catch (...) {
+    return llgpu::api_exception(g->ctx);
   }
 }
 

@@ -135,6 +135,8 @@ int ll_gpu_pagerank_push(ll_gpu_graph* g, int level, int iters, double damping,
   } catch (const Error& e) {
     g->ctx->err = e.msg;
     return e.code;
+  } catch (...) {
+    return llgpu::api_exception(g->ctx);
   }
 }
 

@@ -421,6 +421,8 @@ int ll_gpu_ppr_batch(ll_gpu_graph* g, int level, const int64_t* seeds, int n, in
   } catch (const Error& e) {
     g->ctx->err = e.msg;
     return e.code;
+  } catch (...) {
+    return llgpu::api_exception(g->ctx);
   }
 }
 

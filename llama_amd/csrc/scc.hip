@@ -317,6 +317,8 @@ int ll_gpu_scc_ex(ll_gpu_graph* g, int level, int64_t* labels_out, uint64_t* num
   } catch (const llgpu::Error& e) {
     g->ctx->err = e.msg;
     return e.code;
+  } catch (...) {
+    return llgpu::api_exception(g->ctx);
   }
 }
 

@@ -482,6 +482,8 @@ int ll_gpu_snapshot_stats(ll_gpu_graph* g, int level, ll_gpu_snapshot_info* info
   } catch (const llgpu::Error& e) {
     g->ctx->err = e.msg;
     return e.code;
+  } catch (...) {
+    return llgpu::api_exception(g->ctx);
   }
 }
 
@@ -501,6 +503,8 @@ int ll_gpu_snapshot_adjacency(ll_gpu_graph* g, int level, uint32_t dir, uint64_t
   } catch (const llgpu::Error& e) {
     g->ctx->err = e.msg;
     return e.code;
+  } catch (...) {
+    return llgpu::api_exception(g->ctx);
   }
 }
 
@@ -515,6 +519,8 @@ int ll_gpu_snapshot_degrees(ll_gpu_graph* g, int level, uint32_t dir, uint32_t* 
   } catch (const llgpu::Error& e) {
     g->ctx->err = e.msg;
     return e.code;
+  } catch (...) {
+    return llgpu::api_exception(g->ctx);
   }
 }
 

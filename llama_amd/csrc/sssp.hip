@@ -530,6 +530,8 @@ int ll_gpu_sssp(ll_gpu_graph* g, int level, int64_t root, float* dist_out, uint6
   } catch (const Error& e) {
     g->ctx->err = e.msg;
     return e.code;
+  } catch (...) {
+    return llgpu::api_exception(g->ctx);
   }
 }
 

@@ -700,6 +700,8 @@ int ll_gpu_tc_ex(ll_gpu_graph* g, int level, uint64_t* triangles, uint64_t* arcs
   } catch (const llgpu::Error& e) {
     g->ctx->err = e.msg;
     return e.code;
+  } catch (...) {
+    return llgpu::api_exception(g->ctx);
   }
 }
 
@@ -716,6 +718,8 @@ int ll_gpu_tc_i(ll_gpu_graph* g, int level, uint64_t* count) {
   } catch (const llgpu::Error& e) {
     g->ctx->err = e.msg;
     return e.code;
+  } catch (...) {
+    return llgpu::api_exception(g->ctx);
   }
 }
 

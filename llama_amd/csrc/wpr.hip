@@ -287,6 +287,8 @@ int ll_gpu_wpr_forward(ll_gpu_graph* g, int level, const float* edge_w, const fl
   } catch (const llgpu::Error& e) {
     g->ctx->err = e.msg;
     return e.code;
+  } catch (...) {
+    return llgpu::api_exception(g->ctx);
   }
 }
 
@@ -305,6 +307,8 @@ int ll_gpu_wpr_adjoint(ll_gpu_graph* g, int level, const float* edge_w, const fl
   } catch (const llgpu::Error& e) {
     g->ctx->err = e.msg;
     return e.code;
+  } catch (...) {
+    return llgpu::api_exception(g->ctx);
   }
 }
 
