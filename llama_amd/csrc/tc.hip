@@ -496,27 +496,40 @@ __global__ void k_classify_rows(const uint64_t* ofs, uint64_t V, uint32_t* rows,
   else if (na > kWaveRow) rows[atomicAdd(n, 1ull)] = uint32_t(a);
 }
 
-// W = sum over arcs (a,b) of (d+(a) + d+(b)): intersection element reads.
-// One wave per row (grid-stride), one atomic per workgroup.
+// W = sum over arcs (a,b) of (d+(a) + d+(b)): intersection element reads,
+// into *work; the largest per-row element count sum_b d+(b) into *max_row
+// (the counting kernels keep it in 32 bits).  One wave per row (grid-stride),
+// one atomic of each kind per workgroup.
 __global__ __launch_bounds__(256) void k_tc_work(const uint64_t* ofs, const uint32_t* nbr,
-                                                 uint64_t V, unsigned long long* work) {
-  __shared__ unsigned long long red[4];
+                                                 uint64_t V, unsigned long long* work,
+                                                 unsigned long long* max_row) {
+  __shared__ unsigned long long red[4], rmax[4];
   const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const uint64_t nw = uint64_t(gridDim.x) * 4;
-  unsigned long long w = 0;
+  unsigned long long w = 0, mx = 0;
   for (uint64_t a = uint64_t(blockIdx.x) * 4 + wave; a < V; a += nw) {
     const uint64_t a0 = ofs[a], na = ofs[a + 1] - a0;
+    unsigned long long el = 0;
     for (uint64_t i = lane; i < na; i += 64) {
       const uint32_t b = nbr[a0 + i];
-      w += na + (ofs[b + 1] - ofs[b]);
+      el += ofs[b + 1] - ofs[b];
     }
+    w += na * ((na + 63 - lane) / 64) + el;  // na per arc handled by this lane
+    for (int o = 32; o > 0; o >>= 1) el += __shfl_xor(el, o, 64);
+    mx = el > mx ? el : mx;
   }
   for (int o = 32; o > 0; o >>= 1) w += __shfl_down(w, o, 64);
-  if (lane == 0) red[wave] = w;
+  if (lane == 0) {
+    red[wave] = w;
+    rmax[wave] = mx;
+  }
   __syncthreads();
   if (threadIdx.x == 0) {
     const unsigned long long t = red[0] + red[1] + red[2] + red[3];
+    unsigned long long m = rmax[0];
+    for (int i = 1; i < 4; i++) m = rmax[i] > m ? rmax[i] : m;
     if (t) atomicAdd(work, t);
+    if (m) atomicMax(max_row, m);
   }
 }
 
@@ -612,20 +625,23 @@ TcResult tc_device(ll_gpu_graph* g, int level, bool weighted) {
   TcResult r;
   r.arcs = g->sym_edges;
   DevBuf cnt, rows, gpre, part_w, part_b, part_g, total;
-  cnt.alloc(3 * 8, s);
-  k_zero_ull2<<<1, 1, 0, s>>>(cnt.as<unsigned long long>(), 3);
+  cnt.alloc(4 * 8, s);  // block rows, W, global rows, largest row element count
+  k_zero_ull2<<<1, 1, 0, s>>>(cnt.as<unsigned long long>(), 4);
   rows.alloc(std::max<uint64_t>(V, 1) * 4, s);
   k_classify_rows<<<grid_for(V, 256), 256, 0, s>>>(ofs, V, rows.as<uint32_t>(),
                                                    cnt.as<unsigned long long>());
   k_tc_work<<<unsigned(std::min<uint64_t>(std::max<uint64_t>((V + 3) / 4, 1),
                                            uint64_t(ctx->num_cus) * 32)),
-              256, 0, s>>>(ofs, nbr, V, cnt.as<unsigned long long>() + 1);
+              256, 0, s>>>(ofs, nbr, V, cnt.as<unsigned long long>() + 1,
+                           cnt.as<unsigned long long>() + 3);
   LLG_HIP(hipGetLastError());
-  unsigned long long h[3];
-  LLG_HIP(hipMemcpyAsync(h, cnt.p, 24, hipMemcpyDeviceToHost, s));
+  unsigned long long h[4];
+  LLG_HIP(hipMemcpyAsync(h, cnt.p, 32, hipMemcpyDeviceToHost, s));
   LLG_HIP(hipStreamSynchronize(s));
   const uint64_t nblock = h[0], nglobal = h[2];
   r.work = h[1];
+  LLG_CHECK(h[3] <= 0xFFFFFFFFull, LL_GPU_ERANGE,
+            "a DAG row's neighbour lists hold 2^32 or more entries in total");
   LLG_CHECK(nblock + nglobal <= V, LL_GPU_ESTATE, "row classification overflow");
   const uint32_t* rows_b = rows.as<uint32_t>();
   const uint32_t* rows_g = rows.as<uint32_t>() + (V - nglobal);
