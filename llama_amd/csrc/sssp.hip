@@ -102,6 +102,7 @@ __global__ __launch_bounds__(kBlock) void k_relax(const uint32_t* q, const uint3
   const uint64_t m = qofs[nq];
   const uint64_t nblk = (m + kEdgesPerBlock - 1) / kEdgesPerBlock;
   const int lane = threadIdx.x & 63;
+  if (blockIdx.x == 0 && threadIdx.x == 0) n_next[1] = m;  // the round's edges, read back with n_next[0]
   // equal-sized edge blocks: a fixed grid takes them in turn
   for (uint64_t bi = blockIdx.x; bi < nblk; bi += gridDim.x) {
     const uint64_t x0 = bi * kEdgesPerBlock;
@@ -451,9 +452,7 @@ int ll_gpu_sssp(ll_gpu_graph* g, int level, int64_t root, float* dist_out, uint6
               t_bits, int32_t(nrounds), q[cur ^ 1].as<uint32_t>(), cnt.as<unsigned long long>());
         }
         LLG_HIP(hipGetLastError());
-        LLG_HIP(hipMemcpyAsync(ctx->readback, cnt.p, 8, hipMemcpyDeviceToHost, s));
-        LLG_HIP(hipMemcpyAsync(static_cast<char*>(ctx->readback) + 8, qofs.as<uint64_t>() + nq, 8,
-                               hipMemcpyDeviceToHost, s));
+        LLG_HIP(hipMemcpyAsync(ctx->readback, cnt.p, 16, hipMemcpyDeviceToHost, s));
         LLG_HIP(hipStreamSynchronize(s));
         nq = rb[0];
         work_done += rb[1];
