@@ -10,15 +10,15 @@
 //   2. keys are radix-sorted and deduplicated: the oriented DAG as a CSR with
 //      ascending neighbour lists;
 //   3. for each arc (a,b) count |N+(a) & N+(b)|: one wave per row a (a whole
-//      workgroup for rows of 257..4096 arcs) stages N+(a), a hashed
-//      membership bitmap of it, and for the arcs with d+(b) > 0 the prefix
-//      sums of d+(b) and the list starts ofs[b] in LDS.  The concatenated
-//      lists N+(b) are then streamed 256 entries per wave step (coalesced
-//      within each list): every entry finds its arc from a ballot of
-//      arc-end flags (no search), the four 64-entry loads of a step are in
-//      flight together, the bitmap rejects most non-members with one LDS read
-//      and a branchless binary search of N+(a) confirms the rest.  Rows
-//      longer than 4096 arcs search N+(a) in global memory.
+//      workgroup for rows of 257..4096 arcs) stages N+(a), a bucket index of
+//      it (bucket = ID >> shift, about one entry per bucket), and for the arcs
+//      with d+(b) > 0 the prefix sums of d+(b) and the list starts ofs[b] in
+//      LDS.  The concatenated lists N+(b) are then streamed 256 entries per
+//      wave step (coalesced within each list): every entry finds its arc from
+//      a ballot of arc-end flags (no search), the four 64-entry loads of a
+//      step are in flight together, and each loaded ID is searched in its
+//      bucket's short run of N+(a).  Rows longer than 4096 arcs search N+(a)
+//      in global memory.
 // Every triangle a<b<c (in that order) is counted once, at arc (a,b).
 //
 // Task tc_i (triangle_counting.h:154-336) visits the pair (u, v), u < v by
@@ -38,8 +38,6 @@ constexpr int kBlock = 256;
 constexpr int kRowBlock = 1024;        // workgroup of the kWaveRow..kBlockRow rows
 constexpr uint32_t kWaveRow = 256;     // row length handled by one wave (LDS)
 constexpr uint32_t kBlockRow = 4096;   // row length handled by one workgroup (LDS)
-constexpr uint32_t kWaveBmBits = 12;   // membership bitmap of a wave row: 4096 bits
-constexpr uint32_t kBlockBmBits = 15;  // of a workgroup row: 32768 bits
 constexpr uint32_t kU = 4;             // 64-element chunks a wave resolves together
 constexpr uint32_t kPosShift = 48;     // staged list start | position of b in N+(a) << 48
 constexpr uint64_t kOfsMask = (1ull << kPosShift) - 1;
@@ -152,18 +150,34 @@ __device__ inline unsigned long long count_row_global(const uint64_t* ofs, const
   return c;
 }
 
-// Membership bitmap of N+(a) in LDS: one bit per multiplicative hash of a
-// neighbour ID.  A clear bit proves absence; a set bit is confirmed by a
-// binary search of the sorted list.
-__device__ inline uint32_t bm_hash(uint32_t w, uint32_t bits) {
-  return (w * 0x9E3779B1u) >> (32u - bits);
+// Bucket index of a staged row: N+(a) is sorted, so the IDs of bucket
+// b = id >> shift (nb = 2^ceil(log2 na) buckets over the ID range) form the
+// run la[bs[b] .. bs[b+1]).  With IDs spread over [0, V) a bucket holds about
+// one entry: a lookup reads two bucket starts and searches the short run,
+// and an absent ID usually finds an empty run at once.  Clustered IDs only
+// lengthen the runs (the search stays exact).
+__device__ inline uint32_t bucket_shift(uint64_t V, uint32_t na) {
+  const uint32_t vbits = V > 1 ? uint32_t(64 - __clzll((unsigned long long)(V - 1))) : 1u;
+  const uint32_t nbits = na > 1 ? uint32_t(32 - __clz(int(na - 1))) : 0u;
+  return vbits > nbits ? vbits - nbits : 0u;
+}
+
+// bs[b] = first i with (la[i] >> shift) >= b for b in [0, 2^nbits]; `lanes`
+// threads from `tid` (la[0..na) staged and visible).
+__device__ inline void build_buckets(const uint32_t* la, uint32_t na, uint32_t shift,
+                                     uint32_t nb, uint16_t* bs, uint32_t tid, uint32_t lanes) {
+  for (uint32_t i = tid; i < na; i += lanes) {
+    const uint32_t bi = la[i] >> shift;
+    for (uint32_t b = i ? (la[i - 1] >> shift) + 1u : 0u; b <= bi; b++) bs[b] = uint16_t(i);
+  }
+  for (uint32_t b = (la[na - 1] >> shift) + 1u + tid; b <= nb; b += lanes) bs[b] = uint16_t(na);
 }
 
 __device__ inline uint64_t lanes_below(uint32_t lane) {
   return lane ? (~0ull >> (64u - lane)) : 0ull;
 }
 
-// The staged row a (LDS): la[0..na) = N+(a) ascending, bm = its bitmap,
+// The staged row a (LDS): la[0..na) = N+(a) ascending, bs = its bucket index,
 // lp[0..cnt) = inclusive prefix sums of d+(b) over the arcs with d+(b) > 0
 // ("compacted arcs", in any order), lofs[k] = ofs[b] of compacted arc k, with
 // the position of its b in la in the bits from kPosShift up (weighted count
@@ -178,15 +192,15 @@ __device__ inline uint64_t lanes_below(uint32_t lane) {
 // inside the range, and they are arcs lo .. lo + 64kU - 1: the lanes load
 // those, flag the range position after each one's last element, and a ballot
 // of the flags per 64 positions plus popcounts give every element its arc
-// without a search.  The kU neighbour loads are then issued together, tested
-// against the bitmap, and the candidates confirmed by a branchless binary
-// search of la (the same depth for every lane, the kU searches interleaved).
+// without a search.  The kU neighbour loads are then issued together and
+// each loaded ID is looked up in its bucket's run of la (the kU searches
+// interleaved).
 template <bool kWeighted>
 __device__ inline unsigned long long count_group(const uint32_t* nbr, const uint8_t* mult,
                                                  uint64_t a, uint64_t a0, const uint32_t* la,
                                                  uint32_t na, const uint32_t* lp,
                                                  const uint64_t* lofs, uint32_t cnt,
-                                                 const uint32_t* bm, uint32_t bm_bits,
+                                                 const uint16_t* bs, uint32_t shift,
                                                  uint8_t* fl, uint32_t base0,
                                                  uint32_t end, uint32_t* lo, uint32_t lane) {
   const uint32_t lo0 = *lo;
@@ -222,31 +236,34 @@ __device__ inline unsigned long long count_group(const uint32_t* nbr, const uint
   }
   *lo = lo0 + before;
   uint32_t w[kU];
-  bool cand[kU];
-  bool any = false;
 #pragma unroll
   for (uint32_t u = 0; u < kU; u++) w[u] = act[u] ? nbr[bw[u]] : 0u;
+  // lower bound of w in its bucket's run [bl, be)
+  uint32_t bl[kU], bh[kU], be[kU];
 #pragma unroll
   for (uint32_t u = 0; u < kU; u++) {
-    const uint32_t h = bm_hash(w[u], bm_bits);
-    cand[u] = act[u] && ((bm[h >> 5] >> (h & 31u)) & 1u);
-    any |= cand[u];
+    const uint32_t b = w[u] >> shift;
+    bl[u] = act[u] ? uint32_t(bs[b]) : 0u;
+    be[u] = act[u] ? uint32_t(bs[b + 1]) : 0u;
+    bh[u] = be[u];
   }
-  if (!__ballot(any)) return 0;
-  uint32_t pos[kU];
+  for (;;) {
+    bool open = false;
 #pragma unroll
-  for (uint32_t u = 0; u < kU; u++) pos[u] = 0;
-  for (uint32_t n = na; n > 1;) {  // pos + n <= na throughout
-    const uint32_t half = n >> 1;
+    for (uint32_t u = 0; u < kU; u++) open |= bl[u] < bh[u];
+    if (!__ballot(open)) break;
 #pragma unroll
-    for (uint32_t u = 0; u < kU; u++) pos[u] = la[pos[u] + half] < w[u] ? pos[u] + half : pos[u];
-    n -= half;
+    for (uint32_t u = 0; u < kU; u++)
+      if (bl[u] < bh[u]) {
+        const uint32_t mid = (bl[u] + bh[u]) >> 1;
+        if (la[mid] < w[u]) bl[u] = mid + 1; else bh[u] = mid;
+      }
   }
   unsigned long long c = 0;
 #pragma unroll
   for (uint32_t u = 0; u < kU; u++) {
-    const uint32_t pw = pos[u] + (la[pos[u]] < w[u] ? 1u : 0u);  // lower bound
-    if (!cand[u] || pw >= na || la[pw] != w[u]) continue;
+    const uint32_t pw = bl[u];
+    if (pw >= be[u] || la[pw] != w[u]) continue;
     if (!kWeighted) {
       c++;
     } else {
@@ -267,26 +284,23 @@ __global__ __launch_bounds__(kBlock) void k_tc_wave(const uint64_t* ofs, const u
                                                     const uint8_t* mult, uint64_t V,
                                                     unsigned long long* partial) {
   constexpr uint32_t kWaves = kBlock / 64;
-  constexpr uint32_t kBmWords = (1u << kWaveBmBits) / 32u;
   __shared__ uint32_t l_a[kWaves][kWaveRow];
   __shared__ uint32_t l_p[kWaves][kWaveRow];
   __shared__ uint64_t l_o[kWaves][kWaveRow];
-  __shared__ uint32_t l_bm[kWaves][kBmWords];
+  __shared__ uint16_t l_bs[kWaves][kWaveRow + 1];
   __shared__ uint8_t l_fl[kWaves][kU * 64];
   __shared__ unsigned long long red[kWaves];
   const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   uint32_t* la = l_a[wave];
   uint32_t* lp = l_p[wave];
   uint64_t* lofs = l_o[wave];
-  uint32_t* bm = l_bm[wave];
+  uint16_t* bs = l_bs[wave];
   const uint64_t nwaves = uint64_t(gridDim.x) * kWaves;
   unsigned long long c = 0;
   for (uint64_t a = uint64_t(blockIdx.x) * kWaves + wave; a < V; a += nwaves) {
     const uint64_t a0 = ofs[a];
     const uint32_t na = uint32_t(ofs[a + 1] - a0);
     if (na < 2 || na > kWaveRow) continue;
-    for (uint32_t k = lane; k < kBmWords; k += 64) bm[k] = 0;
-    wave_sync();
     uint32_t cnt = 0;
     for (uint32_t i0 = 0; i0 < na; i0 += 64) {
       const uint32_t i = i0 + lane;
@@ -297,8 +311,6 @@ __global__ __launch_bounds__(kBlock) void k_tc_wave(const uint64_t* ofs, const u
         la[i] = b;
         ob = ofs[b];
         d = uint32_t(ofs[b + 1] - ob);
-        const uint32_t h = bm_hash(b, kWaveBmBits);
-        atomicOr(&bm[h >> 5], 1u << (h & 31u));
       }
       const uint64_t m = __ballot(d != 0);
       if (d) {
@@ -308,6 +320,9 @@ __global__ __launch_bounds__(kBlock) void k_tc_wave(const uint64_t* ofs, const u
       }
       cnt += uint32_t(__popcll(m));
     }
+    wave_sync();
+    const uint32_t shift = bucket_shift(V, na);
+    build_buckets(la, na, shift, 1u << (32 - __clz(int(na - 1))), bs, lane, 64);
     wave_sync();
     // inclusive scan of lp[0..cnt) in chunks of 64
     uint32_t carry = 0;
@@ -325,7 +340,7 @@ __global__ __launch_bounds__(kBlock) void k_tc_wave(const uint64_t* ofs, const u
     const uint32_t total = carry;
     uint32_t lo = 0;
     for (uint32_t base0 = 0; base0 < total; base0 += 64u * kU)
-      c += count_group<kWeighted>(nbr, mult, a, a0, la, na, lp, lofs, cnt, bm, kWaveBmBits,
+      c += count_group<kWeighted>(nbr, mult, a, a0, la, na, lp, lofs, cnt, bs, shift,
                                   l_fl[wave], base0, total, &lo, lane);
     wave_sync();
   }
@@ -344,14 +359,13 @@ __global__ __launch_bounds__(kBlock) void k_tc_wave(const uint64_t* ofs, const u
 template <bool kWeighted>
 __global__ __launch_bounds__(kRowBlock) void k_tc_block(const uint64_t* ofs, const uint32_t* nbr,
                                                         const uint8_t* mult, const uint32_t* rows,
-                                                        uint64_t nrows,
+                                                        uint64_t nrows, uint64_t V,
                                                         unsigned long long* partial) {
   constexpr uint32_t kWaves = kRowBlock / 64;
-  constexpr uint32_t kBmWords = (1u << kBlockBmBits) / 32u;
   __shared__ uint32_t la[kBlockRow];
   __shared__ uint32_t lp[kBlockRow];
   __shared__ uint64_t lofs[kBlockRow];
-  __shared__ uint32_t bm[kBmWords];
+  __shared__ uint16_t bs[kBlockRow + 1];
   __shared__ uint8_t l_fl[kWaves][kU * 64];
   __shared__ uint32_t wtot[kWaves];
   __shared__ uint32_t ncomp;
@@ -362,7 +376,6 @@ __global__ __launch_bounds__(kRowBlock) void k_tc_block(const uint64_t* ofs, con
     const uint64_t a = rows[r];
     const uint64_t a0 = ofs[a];
     const uint32_t na = uint32_t(ofs[a + 1] - a0);  // in (kWaveRow, kBlockRow]
-    for (uint32_t k = tid; k < kBmWords; k += kRowBlock) bm[k] = 0;
     if (tid == 0) ncomp = 0;
     __syncthreads();
     for (uint32_t i0 = 0; i0 < na; i0 += kRowBlock) {
@@ -374,8 +387,6 @@ __global__ __launch_bounds__(kRowBlock) void k_tc_block(const uint64_t* ofs, con
         la[i] = b;
         ob = ofs[b];
         d = uint32_t(ofs[b + 1] - ob);
-        const uint32_t h = bm_hash(b, kBlockBmBits);
-        atomicOr(&bm[h >> 5], 1u << (h & 31u));
       }
       const uint64_t m = __ballot(d != 0);
       uint32_t wbase = 0;
@@ -389,6 +400,9 @@ __global__ __launch_bounds__(kRowBlock) void k_tc_block(const uint64_t* ofs, con
     }
     __syncthreads();
     const uint32_t cnt = ncomp;
+    const uint32_t shift = bucket_shift(V, na);
+    build_buckets(la, na, shift, 1u << (32 - __clz(int(na - 1))), bs, tid, kRowBlock);
+    __syncthreads();
     // block-wide inclusive scan of lp[0..cnt) in chunks of kRowBlock
     uint32_t carry = 0;
     for (uint32_t base = 0; base < cnt; base += kRowBlock) {
@@ -418,7 +432,7 @@ __global__ __launch_bounds__(kRowBlock) void k_tc_block(const uint64_t* ofs, con
       if (lp[mid] <= wb) lo = mid + 1; else hi = mid;
     }
     for (uint32_t base0 = wb; base0 < we; base0 += 64u * kU)
-      c += count_group<kWeighted>(nbr, mult, a, a0, la, na, lp, lofs, cnt, bm, kBlockBmBits,
+      c += count_group<kWeighted>(nbr, mult, a, a0, la, na, lp, lofs, cnt, bs, shift,
                                   l_fl[wave], base0, we, &lo, lane);
     __syncthreads();  // the row's LDS is read before the next row overwrites it
   }
@@ -661,10 +675,10 @@ TcResult tc_device(ll_gpu_graph* g, int level, bool weighted) {
   if (nblock) {
     TimedLaunch t(ctx, "k_tc_block");
     if (weighted)
-      k_tc_block<true><<<gb, kRowBlock, 0, s>>>(ofs, nbr, mult, rows_b, nblock,
+      k_tc_block<true><<<gb, kRowBlock, 0, s>>>(ofs, nbr, mult, rows_b, nblock, V,
                                                 part_b.as<unsigned long long>());
     else
-      k_tc_block<false><<<gb, kRowBlock, 0, s>>>(ofs, nbr, mult, rows_b, nblock,
+      k_tc_block<false><<<gb, kRowBlock, 0, s>>>(ofs, nbr, mult, rows_b, nblock, V,
                                                  part_b.as<unsigned long long>());
   } else {
     k_zero_ull2<<<1, 1, 0, s>>>(part_b.as<unsigned long long>(), int(gb));
