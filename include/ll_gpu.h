@@ -205,7 +205,10 @@ int ll_gpu_bfs_batch(ll_gpu_graph* g, int level, const int64_t* roots, int n, in
                      int32_t* levels_out, uint64_t* reached_out);
 /* Replaces: tasks `tc_u` / `tc_od` (benchmark/benchmarks/triangle_counting.h:343-470,
  * 682-831), extended to multi-level snapshots: triangles of the simple
- * undirected graph of live edges (out and in, self-loops and duplicates removed). */
+ * undirected graph of live edges (out and in, self-loops and duplicates removed).
+ * LL_GPU_ERANGE when one row of the degree-oriented DAG reaches 2^32 or more
+ * neighbour-list entries in total (sum of d+(b) over its arcs), or the DAG
+ * holds 2^48 or more arcs. */
 int ll_gpu_tc(ll_gpu_graph* g, int level, uint64_t* triangles);
 /* Replaces: task `tc_i` = ll_b_triangle_counting_LI::run
  * (benchmark/benchmarks/triangle_counting.h:154-336; registered with reverse
@@ -213,7 +216,7 @@ int ll_gpu_tc(ll_gpu_graph* g, int level, uint64_t* triangles);
  * once per direction it is joined in, the common neighbours w > v (out or in)
  * -- every triangle u<v<w weighs the number of directed edges between u and
  * v.  The reference needs one level with sorted lists; any snapshot level
- * works here. */
+ * works here.  Same limits as ll_gpu_tc. */
 int ll_gpu_tc_i(ll_gpu_graph* g, int level, uint64_t* count);
 /* Replaces: task `tarjan_scc` (benchmark/benchmarks/tarjan_scc.h:121-206) with
  * a correct SCC decomposition; labels_out[max_nodes] = smallest vertex ID of
