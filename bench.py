@@ -77,6 +77,18 @@ def bench_bfs(dg, g, E: int, V: int, args) -> dict:
     }
 
 
+def cpu_model() -> str:
+    """The host CPU's model name (first /proc/cpuinfo entry)."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def tuned_key(kind: str, dg, E: int | None, n: int) -> str:
     """Key of the autotuner decision for a request of n sources/seeds on the
     latest snapshot: kind:V:E:level:bucket, bucket = smallest power of two
@@ -279,6 +291,7 @@ def bench_tc(dg, E: int, V: int, args) -> dict:
            "alg_bytes": alg,
            "alg_gbs": round(alg / max(k_ms, 1e-9) / 1e6, 2),
            "frac_hbm": round(alg / max(k_ms, 1e-9) / 1e6 / HBM_PEAK_GBS, 4),
+           "triangles_per_s": round(tri / t), "work_per_s": round(work / t),
            "tc_i": tci, "tc_i_repeat_equal": tci2 == tci,
            "tc_i_first_call_s": round(t_tci_first, 4), "tc_i_count_s": round(t_tci, 4)}
     log(f"TC: {out}")
@@ -295,12 +308,16 @@ def bench_scc(dg, E: int, V: int, args) -> dict:
         _, n = dg.scc(want_labels=False)
     ctx.sync()
     t = (time.perf_counter() - t0) / args.scc_reps
-    # floor: both directions read once (4 B per edge each) plus 16 B per vertex
-    alg = 8 * E + 16 * V
+    # B_SCC: every full-graph sweep (trim, propagation, BFS) reads the live
+    # edges and writes per-vertex state: sweeps x (8 B per edge + 16 B per vertex)
+    sweeps = dg.last_scc_sweeps
+    per_sweep = 8 * E + 16 * V
+    alg = sweeps * per_sweep
     out = {"num_scc": n, "repeat_equal": n == n0, "seconds": round(t, 4),
-           "sweeps": dg.last_scc_sweeps, "gteps": round(E / t / 1e9, 3),
-           "alg_bytes_floor": alg, "alg_gbs": round(alg / t / 1e9, 2),
-           "frac_hbm": round(alg / t / 1e9 / HBM_PEAK_GBS, 4)}
+           "sweeps": sweeps, "gteps": round(E / t / 1e9, 3),
+           "alg_bytes": alg, "alg_gbs": round(alg / t / 1e9, 2),
+           "frac_hbm": round(alg / t / 1e9 / HBM_PEAK_GBS, 4),
+           "floor_one_sweep_frac_hbm": round(per_sweep / t / 1e9 / HBM_PEAK_GBS, 4)}
     log(f"SCC: {out}")
     return out
 
@@ -639,6 +656,8 @@ def main() -> int:
                     help="PageRank hot/cold split at slot 2^N (0 = off, -1 = automatic)")
     ap.add_argument("--cpu-iters", type=int, default=6,
                     help="PageRank iterations timed for the CPU baseline (0 = skip)")
+    ap.add_argument("--cpu-runs", type=int, default=3,
+                    help="timed repetitions of the CPU baseline sample (mean +- c95)")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--bfs-sources", type=int, default=64,
                     help="single-source DO-BFS runs timed (0 = skip)")
@@ -817,15 +836,26 @@ def main() -> int:
         o = oracle.OracleGraph.from_host(g)
         log(f"CPU baseline: oracle pagerank_double, {args.cpu_iters} iterations, "
             f"{oracle.num_threads()} threads")
-        tc = time.perf_counter()
-        o.pagerank(args.cpu_iters)
-        cpu_s = time.perf_counter() - tc
+        runs = []
+        for _ in range(args.cpu_runs):
+            tc = time.perf_counter()
+            o.pagerank(args.cpu_iters)
+            runs.append(time.perf_counter() - tc)
+        cpu_s = sum(runs) / len(runs)
+        # the reference's ll_c95: 1.96 * population stdev / sqrt(n)
+        # (llama/include/llama/ll_utils.h:200-221)
+        sd = (sum((x - cpu_s) ** 2 for x in runs) / len(runs)) ** 0.5
         cpu = {"value": E * args.cpu_iters / cpu_s / 1e9, "unit": "GTEPS",
                "cores": oracle.num_threads(), "kind": "port",
                "sample": f"{args.cpu_iters} pull iterations of pagerank_double (FP64, "
                          f"OpenMP dynamic,4096) over the same {args.levels}-level layout, "
-                         f"{cpu_s:.2f}s"}
-        log(f"CPU baseline {cpu['value']:.3f} GTEPS in {cpu_s:.1f}s")
+                         f"mean {cpu_s:.2f}s +- {1.96 * sd / len(runs) ** 0.5:.2f}s (c95) "
+                         f"over {len(runs)} runs",
+               "seconds_runs": [round(x, 3) for x in runs],
+               "cpu_model": cpu_model(),
+               "omp": {k: os.environ.get(k) for k in ("OMP_NUM_THREADS", "OMP_PROC_BIND",
+                                                       "OMP_PLACES")}}
+        log(f"CPU baseline {cpu['value']:.3f} GTEPS, mean of {len(runs)} runs of {cpu_s:.1f}s")
         if args.ref_calibration:
             cpu["reference_calibration"] = reference_calibration(ctx, args)
 
