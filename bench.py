@@ -77,6 +77,26 @@ def bench_bfs(dg, g, E: int, V: int, args) -> dict:
     }
 
 
+def measured_copy_gbs(mib: int = 2048, reps: int = 5) -> dict:
+    """STREAM-like device copy of `mib` MiB (read + write bytes / time), timed
+    with events on torch's current stream, the stream the copy runs on."""
+    import torch
+    n = mib << 20
+    src = torch.ones(n, dtype=torch.uint8, device="cuda:0")
+    dst = torch.empty_like(src)
+    dst.copy_(src)  # warm
+    t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    t1.record()
+    torch.cuda.synchronize()
+    ms = t0.elapsed_time(t1) / reps
+    del src, dst
+    torch.cuda.empty_cache()
+    return {"bytes": 2 * n, "ms": round(ms, 4), "gbs": round(2 * n / (ms * 1e-3) / 1e9, 1)}
+
+
 def cpu_model() -> str:
     """The host CPU's model name (first /proc/cpuinfo entry)."""
     try:
@@ -808,6 +828,9 @@ def main() -> int:
     used = [k for k, v in it_kernels.items() if v[0] > 0]
     traffic = measured_traffic(args, V, E, used)
 
+    copy_bw = measured_copy_gbs()
+    log(f"device copy: {copy_bw['gbs']} GB/s (read + write)")
+
     secondary = {}
     if args.bfs_sources > 0:
         secondary["bfs"] = bench_bfs(dg, g, E, V, args)
@@ -890,6 +913,7 @@ def main() -> int:
             "launches": n_pr,
             "per_kernel_avg_ms": {k: round(v[1] / max(n_pr, 1), 5)
                                   for k, v in it_kernels.items()},
+            "measured_copy": copy_bw,
         },
         "cpu_baseline": cpu,
         "breakdown_ms_per_step": {
