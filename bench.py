@@ -363,6 +363,19 @@ def bench_bfs_batch(dg, g, V: int, args, single: dict | None) -> dict:
             f"autotuned_lanes{dg.tuned(tuned_key('bfs_lanes', dg, None, len(roots)))}"
         out[key] = {"seconds": round(t, 4), "sweeps": sweeps, "kernel_ms": round(ms_k, 3),
                     "reached_total": int(reached.sum())}
+    # Graph500 numerator per source, E_comp(s) = sum of live out-degrees of the
+    # vertices s reaches, from single-source runs outside the timed regions;
+    # the reached counts double as a cross-check of the batched lanes
+    e_comp, same_reach = 0, True
+    for i, r in enumerate(roots):
+        _, n_r, _ = dg.bfs(int(r), want_levels=False)
+        e_comp += dg.last_bfs_traversed
+        same_reach &= n_r == int(reached[i])
+    out["edges_traversed_total"] = e_comp
+    out["reached_equal_single_source"] = bool(same_reach)
+    for k, v in list(out.items()):
+        if isinstance(v, dict) and "seconds" in v:
+            v["aggregate_gteps"] = round(e_comp / v["seconds"] / 1e9, 2)
     if single:
         per_src = single["seconds"] / single["sources"]
         seq = per_src * len(roots)
