@@ -376,12 +376,13 @@ int ll_gpu_bc(ll_gpu_graph* g, int level, const int64_t* sources, int64_t n, flo
     uint32_t* c_in = cnt.as<uint32_t>();
     uint32_t* c_out = c_in + 32;
     uint32_t* c_long = c_in + 64;
+    const uint64_t* dead = ensure_dead_rows(g, in);
     for (int64_t i = 0; i < ns; i++) {
       const uint32_t src = uint32_t(sources ? sources[i] : i);
       // the search's step count bounds the deepest level (a last step may
       // find nothing): a level without rows has an empty list
       const int32_t D =
-          bfs_core(g, out, in, in_rb, outdeg, src, nullptr, visited, lv.as<int32_t>()).levels;
+          bfs_core(g, out, in, in_rb, outdeg, src, dead, visited, lv.as<int32_t>()).levels;
       k_bc_source_init<<<grid_for(V, 256), 256, 0, s>>>(sigma.as<float>(), delta.as<float>(), V,
                                                         src);
       k_fill_u64<<<1, 64, 0, s>>>(reinterpret_cast<unsigned long long*>(c_in), 48);

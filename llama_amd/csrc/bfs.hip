@@ -375,6 +375,16 @@ __global__ __launch_bounds__(kBlock) void k_bitmap_to_queue(const uint64_t* bm, 
   for (uint64_t b = keep; b; b &= b - 1) q[off++] = uint32_t(w * 64 + (__ffsll((long long)b) - 1));
 }
 
+// Word w of the dead-row bitmap: bit i set when row 64w+i of the CSR has no
+// entry.  A wave covers 64 consecutive rows (blocks of 256 start at multiples
+// of 64), so one ballot forms the word.
+__global__ void k_dead_rows(const uint64_t* ofs, uint64_t V, uint64_t* out) {
+  const uint64_t v = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const bool dead = v < V && ofs[v + 1] == ofs[v];
+  const uint64_t m = __ballot(dead);
+  if ((threadIdx.x & 63) == 0 && v < V) out[v >> 6] = m;
+}
+
 // Reached vertices (grid-stride over a fixed grid: one atomic per workgroup).
 __global__ void k_count_reached(const int32_t* lv, uint64_t V, unsigned long long* out) {
   unsigned long long n = 0;
@@ -400,6 +410,19 @@ Counters read_counters(ll_gpu_ctx* ctx, const DevBuf& c) {
 }
 
 }  // namespace
+
+const uint64_t* ensure_dead_rows(ll_gpu_graph* g, const Csr& in) {
+  if (g->dead_level != in.level || !g->dead_in.p) {
+    hipStream_t s = g->ctx->stream;
+    const uint64_t V = g->V;
+    g->dead_in.alloc(((V + 63) / 64) * 8, s);
+    k_dead_rows<<<grid_for(V, 256), 256, 0, s>>>(in.ofs.as<uint64_t>(), V,
+                                                 g->dead_in.as<uint64_t>());
+    LLG_HIP(hipGetLastError());
+    g->dead_level = in.level;
+  }
+  return g->dead_in.as<uint64_t>();
+}
 
 // bfs_core: see ll_gpu_internal.h.
 BfsResult bfs_core(ll_gpu_graph* g, const Csr& fwd, const Csr& rev, const RowBlocks& rev_rb,
@@ -509,7 +532,8 @@ BfsResult bfs_device(ll_gpu_graph* g, int level, uint32_t root, DevBuf& lv) {
   const RowBlocks& in_rb = ensure_rowblocks(g, LL_GPU_DIR_IN, level, kBuCap, kPiece);
   lv.alloc(V * 4, s);
   DevBuf visited;
-  BfsResult r = bfs_core(g, out, in, in_rb, outdeg, root, nullptr, visited, lv.as<int32_t>());
+  BfsResult r = bfs_core(g, out, in, in_rb, outdeg, root, ensure_dead_rows(g, in), visited,
+                         lv.as<int32_t>());
   DevBuf red;
   red.alloc(8, s);
   k_fill_u64<<<1, 1, 0, s>>>(red.as<uint64_t>(), 1, 0);

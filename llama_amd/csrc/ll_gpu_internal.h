@@ -260,6 +260,9 @@ struct ll_gpu_graph {
   llgpu::DevBuf sym_ofs, sym_adj;  // degree-oriented DAG for triangle counting
   llgpu::DevBuf sym_mult;          // uint8 per arc: directed live edges joining its ends (1 or 2)
   uint64_t sym_edges = 0;
+  // rows of the in-CSR with no live in-edge (bitmap), excluded from BFS
+  int dead_level = -1;
+  llgpu::DevBuf dead_in;
   // PageRank locality order of the in-CSR (option pr_reorder)
   int pr_level = -1;
   llgpu::DevBuf pr_inv, pr_adj;
@@ -296,6 +299,11 @@ struct BfsResult {
 BfsResult bfs_core(ll_gpu_graph* g, const Csr& fwd, const Csr& rev, const RowBlocks& rev_rb,
                    const uint32_t* fdeg, uint32_t root, const uint64_t* excluded,
                    DevBuf& visited, int32_t* lv);
+// Bitmap of the in-CSR rows (of `in`, the snapshot's in-edges) with no
+// entry: no search from another vertex can reach them, so single-source
+// searches pass it as `excluded` and the bottom-up steps skip their row
+// blocks.  Cached per snapshot level.
+const uint64_t* ensure_dead_rows(ll_gpu_graph* g, const Csr& in);
 constexpr uint32_t kBfsRowCap = 1024;  // row-block capacity used by bottom-up steps
 
 int resolve_level(const ll_gpu_graph* g, int level);
