@@ -635,6 +635,24 @@ def bench_large(args) -> int:
               "bfs_reached_matches_levels": n0 == int((lv0 != 2**31 - 2).sum()),
               "pagerank_pull_vs_push_l1rel": float(np.abs(r - p).sum() / np.abs(p).sum())}
     log(f"large: checks {checks}")
+    tc = None
+    if args.large_tc:
+        # the PageRank orderings and both CSRs go first: the DAG build's sort
+        # needs ~ 4 x 8 B per live edge beside the mirror
+        dg.drop_snapshots()
+        ctx.sync()
+        t = time.perf_counter()
+        tri, arcs, work = dg.triangles(stats=True)
+        t_tc = time.perf_counter() - t
+        t = time.perf_counter()
+        tci = dg.tc_i()
+        t_tci = time.perf_counter() - t
+        tc = {"triangles": tri, "dag_arcs": arcs, "work": work,
+              "seconds_incl_dag_build": round(t_tc, 3), "tc_i": tci,
+              "tc_i_seconds": round(t_tci, 3)}
+        # each DAG arc joins its ends by one or two directed live edges
+        checks["tc_le_tc_i_le_2tc"] = tri <= tci <= 2 * tri
+        log(f"large: TC {tc}")
     print(json.dumps({
         "metric": "large single-GPU scale (PageRank, DO-BFS, batched BFS; per-level upload)",
         "config": {"graph": f"R-MAT s{scale} EF16, {levels} levels (+0.5%/-0.05% per level), "
@@ -652,6 +670,7 @@ def bench_large(args) -> int:
         "bfs": {"sources": 16, "seconds": round(t_bfs, 4), "gteps": round(trav / t_bfs / 1e9, 2)},
         "bfs_batch": {"sources": 64, "lanes": 64, "seconds": round(t_batch, 4), "sweeps": sweeps,
                       "reached_total": int(np.asarray(reached).sum())},
+        "triangle_counting": tc,
         "checks": checks,
     }), flush=True)
     dg.close()
@@ -715,6 +734,8 @@ def main() -> int:
     ap.add_argument("--large-scale", type=int, default=0,
                     help="run only the large single-GPU configuration at this R-MAT scale "
                          "(27 for the configuration; prints its own JSON line)")
+    ap.add_argument("--large-tc", type=int, default=1,
+                    help="with --large-scale: also count triangles (tc_u, tc_i)")
     ap.add_argument("--large-levels", type=int, default=20,
                     help="levels of the large configuration (base included)")
     ap.add_argument("--cow-scale", type=int, default=24,
