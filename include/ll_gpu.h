@@ -206,7 +206,7 @@ int ll_gpu_bfs_batch(ll_gpu_graph* g, int level, const int64_t* roots, int n, in
 /* Replaces: tasks `tc_u` / `tc_od` (benchmark/benchmarks/triangle_counting.h:343-470,
  * 682-831), extended to multi-level snapshots: triangles of the simple
  * undirected graph of live edges (out and in, self-loops and duplicates removed).
- * LL_GPU_ERANGE when one row of the degree-oriented DAG reaches 2^32 or more
+ * LL_GPU_ERANGE when one row of the degree-oriented DAG reaches 2^31 or more
  * neighbour-list entries in total (sum of d+(b) over its arcs), or the DAG
  * holds 2^48 or more arcs. */
 int ll_gpu_tc(ll_gpu_graph* g, int level, uint64_t* triangles);

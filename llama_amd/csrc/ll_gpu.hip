@@ -135,6 +135,7 @@ void drop_snapshots(ll_gpu_graph* g) {
   g->sym_ofs.release();
   g->sym_adj.release();
   g->sym_mult.release();
+  g->sym_rows.release();
   g->dead_level = -1;
   g->dead_in.release();
   g->pr_level = -1;

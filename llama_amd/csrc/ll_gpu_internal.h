@@ -260,6 +260,10 @@ struct ll_gpu_graph {
   llgpu::DevBuf sym_ofs, sym_adj;  // degree-oriented DAG for triangle counting
   llgpu::DevBuf sym_mult;          // uint8 per arc: directed live edges joining its ends (1 or 2)
   uint64_t sym_edges = 0;
+  // DAG rows by length (workgroup rows from the front, global-memory rows from
+  // the back), W and the largest row element count, built with the DAG
+  llgpu::DevBuf sym_rows;
+  uint64_t sym_nblock = 0, sym_nglobal = 0, sym_work = 0, sym_maxrow = 0;
   // rows of the in-CSR with no live in-edge (bitmap), excluded from BFS
   int dead_level = -1;
   llgpu::DevBuf dead_in;

@@ -512,7 +512,7 @@ __global__ void k_classify_rows(const uint64_t* ofs, uint64_t V, uint32_t* rows,
 
 // W = sum over arcs (a,b) of (d+(a) + d+(b)): intersection element reads,
 // into *work; the largest per-row element count sum_b d+(b) into *max_row
-// (the counting kernels keep it in 32 bits).  One wave per row (grid-stride),
+// (the counting kernels keep element positions below 2^31).  One wave per row (grid-stride),
 // one atomic of each kind per workgroup.
 __global__ __launch_bounds__(256) void k_tc_work(const uint64_t* ofs, const uint32_t* nbr,
                                                  uint64_t V, unsigned long long* work,
@@ -624,6 +624,29 @@ void ensure_dag(ll_gpu_graph* g, int level) {
                                                      g->sym_adj.as<uint32_t>());
   LLG_HIP(hipGetLastError());
   g->sym_edges = m;
+  // rows by length, W and the largest row element count (with the DAG)
+  {
+    const uint64_t* dofs = g->sym_ofs.as<uint64_t>();
+    const uint32_t* dnbr = g->sym_adj.as<uint32_t>();
+    DevBuf cnt;
+    cnt.alloc(4 * 8, s);  // block rows, W, global rows, largest row element count
+    k_zero_ull2<<<1, 1, 0, s>>>(cnt.as<unsigned long long>(), 4);
+    g->sym_rows.alloc(std::max<uint64_t>(V, 1) * 4, s);
+    k_classify_rows<<<grid_for(V, 256), 256, 0, s>>>(dofs, V, g->sym_rows.as<uint32_t>(),
+                                                     cnt.as<unsigned long long>());
+    k_tc_work<<<unsigned(std::min<uint64_t>(std::max<uint64_t>((V + 3) / 4, 1),
+                                             uint64_t(ctx->num_cus) * 32)),
+                256, 0, s>>>(dofs, dnbr, V, cnt.as<unsigned long long>() + 1,
+                             cnt.as<unsigned long long>() + 3);
+    LLG_HIP(hipGetLastError());
+    unsigned long long h[4];
+    LLG_HIP(hipMemcpyAsync(h, cnt.p, 32, hipMemcpyDeviceToHost, s));
+    LLG_HIP(hipStreamSynchronize(s));
+    g->sym_nblock = h[0];
+    g->sym_work = h[1];
+    g->sym_nglobal = h[2];
+    g->sym_maxrow = h[3];
+  }
   g->sym_level = level;
 }
 
@@ -638,27 +661,17 @@ TcResult tc_device(ll_gpu_graph* g, int level, bool weighted) {
   LLG_CHECK(g->sym_edges <= kOfsMask, LL_GPU_ERANGE, "oriented arcs beyond 2^48");
   TcResult r;
   r.arcs = g->sym_edges;
-  DevBuf cnt, rows, gpre, part_w, part_b, part_g, total;
-  cnt.alloc(4 * 8, s);  // block rows, W, global rows, largest row element count
-  k_zero_ull2<<<1, 1, 0, s>>>(cnt.as<unsigned long long>(), 4);
-  rows.alloc(std::max<uint64_t>(V, 1) * 4, s);
-  k_classify_rows<<<grid_for(V, 256), 256, 0, s>>>(ofs, V, rows.as<uint32_t>(),
-                                                   cnt.as<unsigned long long>());
-  k_tc_work<<<unsigned(std::min<uint64_t>(std::max<uint64_t>((V + 3) / 4, 1),
-                                           uint64_t(ctx->num_cus) * 32)),
-              256, 0, s>>>(ofs, nbr, V, cnt.as<unsigned long long>() + 1,
-                           cnt.as<unsigned long long>() + 3);
-  LLG_HIP(hipGetLastError());
-  unsigned long long h[4];
-  LLG_HIP(hipMemcpyAsync(h, cnt.p, 32, hipMemcpyDeviceToHost, s));
-  LLG_HIP(hipStreamSynchronize(s));
-  const uint64_t nblock = h[0], nglobal = h[2];
-  r.work = h[1];
-  LLG_CHECK(h[3] <= 0xFFFFFFFFull, LL_GPU_ERANGE,
-            "a DAG row's neighbour lists hold 2^32 or more entries in total");
+  DevBuf gpre, part_w, part_b, part_g, total;
+  unsigned long long h[3];
+  const uint64_t nblock = g->sym_nblock, nglobal = g->sym_nglobal;
+  r.work = g->sym_work;
+  // element positions of a row stay below 2^31, so every offset added to one
+  // (a wave step, a per-wave range) still fits in 32 bits
+  LLG_CHECK(g->sym_maxrow < (1ull << 31), LL_GPU_ERANGE,
+            "a DAG row's neighbour lists hold 2^31 or more entries in total");
   LLG_CHECK(nblock + nglobal <= V, LL_GPU_ESTATE, "row classification overflow");
-  const uint32_t* rows_b = rows.as<uint32_t>();
-  const uint32_t* rows_g = rows.as<uint32_t>() + (V - nglobal);
+  const uint32_t* rows_b = g->sym_rows.as<uint32_t>();
+  const uint32_t* rows_g = g->sym_rows.as<uint32_t>() + (V - nglobal);
   const unsigned gw = unsigned(std::min<uint64_t>(std::max<uint64_t>((V + 3) / 4, 1),
                                                   uint64_t(ctx->num_cus) * 32));
   part_w.alloc(gw * 8, s);
@@ -707,7 +720,7 @@ TcResult tc_device(ll_gpu_graph* g, int level, bool weighted) {
   k_sum_partials<<<1, 1024, 0, s>>>(part_g.as<unsigned long long>(), gg,
                                     total.as<unsigned long long>() + 2);
   LLG_HIP(hipGetLastError());
-  LLG_HIP(hipMemcpyAsync(h, total.p, 24, hipMemcpyDeviceToHost, s));
+  LLG_HIP(hipMemcpyAsync(h, total.p, sizeof(h), hipMemcpyDeviceToHost, s));
   LLG_HIP(hipStreamSynchronize(s));
   r.triangles = h[0] + h[1] + h[2];
   return r;
