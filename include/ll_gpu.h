@@ -288,8 +288,11 @@ int ll_gpu_tuned(ll_gpu_ctx* ctx, const char* prefix, int* value);
  * Bellman-Ford rounds; -1 = automatic = 5), "sssp_min_queue" (once edges worth
  * half the snapshot have been relaxed, the smallest queue a near/far phase
  * starts with when the far pile holds that many -- smaller queues rejoin the
- * pile; 0 = off, -1 = automatic = max_nodes / 32).  Results never depend on
- * them. */
+ * pile; 0 = off, -1 = automatic = max_nodes / 32), "bfs_alpha" / "bfs_beta"
+ * (direction-optimising BFS switches to bottom-up when the frontier's
+ * out-edges exceed the unexplored edges / alpha, back to top-down when a
+ * shrinking frontier holds under max_nodes / beta vertices; <= 0 = defaults
+ * 14 and 24).  Results never depend on them. */
 int ll_gpu_set_option(ll_gpu_ctx* ctx, const char* name, long value);
 /* Drop the cached snapshot materialisations (the next call rebuilds them). */
 void ll_gpu_graph_drop_snapshots(ll_gpu_graph* g);

@@ -17,8 +17,9 @@
 //     unvisited row tests its bit range; new vertices are set in the next
 //     frontier bitmap with an atomic OR whose old value keeps counts exact.
 // The switch follows Beamer et al. (SC'12): top-down -> bottom-up when the
-// frontier's out-edges exceed the unexplored edges / 14, back when the
-// frontier shrinks below V / 24.
+// frontier's out-edges exceed the unexplored edges / alpha, back when the
+// frontier shrinks below V / beta (options "bfs_alpha", "bfs_beta"; defaults
+// 14 and 24).
 #include <climits>
 
 #include "ll_gpu_internal.h"
@@ -458,9 +459,12 @@ BfsResult bfs_core(ll_gpu_graph* g, const Csr& fwd, const Csr& rev, const RowBlo
   int cur = 0;
   int32_t lvl = 0;
   const uint64_t E = fwd.E;
+  const long a_opt = ctx->options.count("bfs_alpha") ? ctx->options["bfs_alpha"] : -1;
+  const long b_opt = ctx->options.count("bfs_beta") ? ctx->options["bfs_beta"] : -1;
+  const uint64_t alpha = a_opt > 0 ? uint64_t(a_opt) : 14, beta = b_opt > 0 ? uint64_t(b_opt) : 24;
   while (n_f > 0 && m_f > 0) {
     const uint64_t m_u = E > m_visited ? E - m_visited : 0;
-    if (!bottom_up && m_f > m_u / 14) {
+    if (!bottom_up && m_f > m_u / alpha) {
       bottom_up = true;
       k_fill_u64<<<grid_for(words, 256), 256, 0, s>>>(fbm.as<uint64_t>(), words, 0);
       k_queue_to_bitmap<<<grid_for(nq, 256), 256, 0, s>>>(q[cur].as<uint32_t>(), nq,
@@ -501,7 +505,7 @@ BfsResult bfs_core(ll_gpu_graph* g, const Csr& fwd, const Csr& rev, const RowBlo
       m_f = h.m_next;
       std::swap(fbm, nbm);
       if (n_f == 0) break;
-      if (shrinking && n_f < V / 24) {
+      if (shrinking && n_f < V / beta) {
         bottom_up = false;
         k_zero_counters<<<1, kStripes, 0, s>>>(cnt.as<Counters>());
         k_bitmap_to_queue<<<grid_for(words, kBlock), kBlock, 0, s>>>(
