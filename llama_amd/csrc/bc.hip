@@ -35,13 +35,21 @@ constexpr int kBlock = 256;
 constexpr uint32_t kCap = kBfsRowCap;  // row blocks shared with the bottom-up BFS
 constexpr uint32_t kMaxEdges = kCap + kCap / 2;
 
+// Per-vertex Brandes state in one 16-byte record, so visiting a neighbour is
+// one load (level, path count and dependency together) instead of a level
+// load followed by dependent path-count and dependency loads.
+struct alignas(16) VState {
+  int32_t lv;    // BFS level from the source, kUnreachedLevel when unreached
+  float sigma;   // shortest-path count
+  float delta;   // dependency
+  float pad;
+};
+
 struct BcArgs {
   const RowItem* items;
   const uint64_t* ofs;
   const uint32_t* adj;
-  const int32_t* lv;
-  float* sigma;
-  float* delta;
+  VState* st;
   float* bc;
   float* part;      // partials of split rows, per item
   float* hub;       // reverse pass: terms of long rows, by out-edge position
@@ -56,17 +64,28 @@ constexpr int32_t kUnreachedLevel = 2147483646;  // INT_MAX - 1, as the BFS mark
 // in-neighbour one level up; reverse, the reference's
 // sigma[r] / sigma[w] * (1 + delta[w]) for an out-neighbour one level down.
 template <bool kFwd>
-__device__ inline float term(const BcArgs& a, uint32_t nb, int32_t l, float srow) {
-  if (kFwd) return l == a.d - 1 ? a.sigma[nb] : 0.0f;
-  return l == a.d + 1 ? srow / a.sigma[nb] * (1 + a.delta[nb]) : 0.0f;
+__device__ inline float term(const BcArgs& a, const VState& w, float srow) {
+  if (kFwd) return w.lv == a.d - 1 ? w.sigma : 0.0f;
+  return w.lv == a.d + 1 ? srow / w.sigma * (1 + w.delta) : 0.0f;
+}
+
+// The state of neighbour nb, or an off-level placeholder for a skipped entry.
+__device__ inline VState load_state(const BcArgs& a, bool need, uint32_t nb) {
+  if (need) return a.st[nb];
+  VState v;
+  v.lv = -1;
+  v.sigma = 0.0f;
+  v.delta = 0.0f;
+  v.pad = 0.0f;
+  return v;
 }
 
 template <bool kFwd>
 __device__ inline void finish(const BcArgs& a, uint32_t r, float S) {
   if (kFwd) {
-    a.sigma[r] = S;
+    a.st[r].sigma = S;
   } else {
-    a.delta[r] = S;
+    a.st[r].delta = S;
     if (r != a.source) a.bc[r] = a.bc[r] + S;
   }
 }
@@ -84,7 +103,7 @@ __device__ inline void bc_item(const BcArgs& a, uint64_t item) {
   if (it.npieces == 0) {
     const uint32_t nrows = it.row_end - it.row_begin;
     const uint32_t r = it.row_begin + threadIdx.x;
-    const bool on = threadIdx.x < nrows && a.lv[r] == a.d;
+    const bool on = threadIdx.x < nrows && a.st[r].lv == a.d;
     if (!__syncthreads_or(on)) return;
     for (uint32_t i = threadIdx.x; i <= nrows; i += kBlock) rofs[i] = a.ofs[it.row_begin + i];
     __syncthreads();
@@ -94,16 +113,16 @@ __device__ inline void bc_item(const BcArgs& a, uint64_t item) {
     const uint32_t hi = threadIdx.x < nrows ? uint32_t(rofs[threadIdx.x + 1] - e0) : 0u;
     if (threadIdx.x < nrows) {
       onf[threadIdx.x] = on ? 1 : 0;
-      srow[threadIdx.x] = on && !kFwd ? a.sigma[r] : 0.0f;
+      srow[threadIdx.x] = on && !kFwd ? a.st[r].sigma : 0.0f;
       if (on)
         for (uint32_t j = lo; j < hi; j++) rowof[j] = uint8_t(threadIdx.x);
     }
     __syncthreads();
     // entries of rows off the level are neither loaded nor staged (rowof is
     // written for rows on the level only, so the row's range is checked too);
-    // all neighbour IDs first, then their levels, so the loads overlap
+    // all neighbour IDs first, then their states, so the loads overlap
     uint32_t nb[kPer], rr[kPer];
-    int32_t l[kPer];
+    VState w[kPer];
     bool need[kPer];
 #pragma unroll
     for (uint32_t k = 0; k < kPer; k++) {
@@ -118,10 +137,10 @@ __device__ inline void bc_item(const BcArgs& a, uint64_t item) {
       nb[k] = need[k] ? a.adj[e0 + i] : 0u;
     }
 #pragma unroll
-    for (uint32_t k = 0; k < kPer; k++) l[k] = need[k] ? a.lv[nb[k]] : -1;
+    for (uint32_t k = 0; k < kPer; k++) w[k] = load_state(a, need[k], nb[k]);
 #pragma unroll
     for (uint32_t k = 0; k < kPer; k++)
-      if (need[k]) vals[0][k * kBlock + threadIdx.x] = term<kFwd>(a, nb[k], l[k], srow[rr[k]]);
+      if (need[k]) vals[0][k * kBlock + threadIdx.x] = term<kFwd>(a, w[k], srow[rr[k]]);
     __syncthreads();
     if (on) {
       // the row's terms in edge order, as the reference's loop adds them
@@ -135,7 +154,7 @@ __device__ inline void bc_item(const BcArgs& a, uint64_t item) {
     // order.  Waves 1-3 copy chunk c into LDS while lane 0 of wave 0 adds
     // chunk c-1, so the dependency is the reference's sequential sum.
     const uint32_t r = it.row_begin;
-    if (it.piece != 0 || a.lv[r] != a.d) return;  // uniform over the workgroup
+    if (it.piece != 0 || a.st[r].lv != a.d) return;  // uniform over the workgroup
     const uint64_t rb = a.ofs[r], re = a.ofs[r + 1];
     const uint32_t nch = uint32_t((re - rb + kMaxEdges - 1) / kMaxEdges);
     constexpr uint32_t kGather = kBlock - 64;
@@ -190,13 +209,13 @@ __device__ inline void bc_item(const BcArgs& a, uint64_t item) {
     // Forward pass, long row: path counts are whole numbers, exact in any
     // order below 2^24; pieces reduce in parallel, k_bc_fixup joins them.
     const uint32_t r = it.row_begin;
-    if (a.lv[r] != a.d) return;  // uniform over the workgroup
+    if (a.st[r].lv != a.d) return;  // uniform over the workgroup
     const uint64_t lo = a.ofs[r] + uint64_t(it.piece) * kPiece;
     const uint64_t hi = min(a.ofs[r + 1], lo + kPiece);
     float S = 0.0f;
     for (uint64_t c0 = lo; c0 < hi; c0 += kMaxEdges) {
       uint32_t nb[kPer];
-      int32_t l[kPer];
+      VState w[kPer];
 #pragma unroll
       for (uint32_t k = 0; k < kPer; k++) {
         const uint64_t e = c0 + k * kBlock + threadIdx.x;
@@ -204,9 +223,9 @@ __device__ inline void bc_item(const BcArgs& a, uint64_t item) {
       }
 #pragma unroll
       for (uint32_t k = 0; k < kPer; k++)
-        l[k] = c0 + k * kBlock + threadIdx.x < hi ? a.lv[nb[k]] : -1;
+        w[k] = load_state(a, c0 + k * kBlock + threadIdx.x < hi, nb[k]);
 #pragma unroll
-      for (uint32_t k = 0; k < kPer; k++) S = S + term<true>(a, nb[k], l[k], 0.0f);
+      for (uint32_t k = 0; k < kPer; k++) S = S + term<true>(a, w[k], 0.0f);
     }
     for (int o = 32; o > 0; o >>= 1) S = S + __shfl_down(S, o, 64);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = S;
@@ -232,25 +251,24 @@ __global__ __launch_bounds__(kBlock) void k_bc_pass(BcArgs a, const uint32_t* li
 __global__ __launch_bounds__(kBlock) void k_bc_hub_terms(BcArgs a, const uint32_t* list) {
   const RowItem it = a.items[list[blockIdx.x]];
   const uint32_t r = it.row_begin;
-  if (a.lv[r] != a.d) return;
+  if (a.st[r].lv != a.d) return;
   const uint64_t lo = a.ofs[r] + uint64_t(it.piece) * kPiece;
   const uint64_t hi = min(a.ofs[r + 1], lo + kPiece);
-  const float sr = a.sigma[r];
+  const float sr = a.st[r].sigma;
   for (uint64_t c0 = lo; c0 < hi; c0 += kMaxEdges) {
     uint32_t nb[kPer];
-    int32_t l[kPer];
+    VState w[kPer];
 #pragma unroll
     for (uint32_t k = 0; k < kPer; k++) {
       const uint64_t e = c0 + k * kBlock + threadIdx.x;
       nb[k] = e < hi ? a.adj[e] : 0u;
     }
 #pragma unroll
-    for (uint32_t k = 0; k < kPer; k++)
-      l[k] = c0 + k * kBlock + threadIdx.x < hi ? a.lv[nb[k]] : -1;
+    for (uint32_t k = 0; k < kPer; k++) w[k] = load_state(a, c0 + k * kBlock + threadIdx.x < hi, nb[k]);
 #pragma unroll
     for (uint32_t k = 0; k < kPer; k++) {
       const uint64_t e = c0 + k * kBlock + threadIdx.x;
-      if (e < hi) a.hub[e] = term<false>(a, nb[k], l[k], sr);
+      if (e < hi) a.hub[e] = term<false>(a, w[k], sr);
     }
   }
 }
@@ -310,18 +328,23 @@ __global__ void k_bc_fixup(BcArgs a, uint64_t nitems) {
   const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= nitems) return;
   const RowItem it = a.items[i];
-  if (it.npieces > 1 && it.piece == 0 && a.lv[it.row_begin] == a.d) {
+  if (it.npieces > 1 && it.piece == 0 && a.st[it.row_begin].lv == a.d) {
     float t = a.part[i];
     for (uint32_t p = 1; p < it.npieces; p++) t = t + a.part[i + p];
     finish<true>(a, it.row_begin, t);
   }
 }
 
-__global__ void k_bc_source_init(float* sigma, float* delta, uint64_t V, uint32_t s) {
+// A source's state: the BFS levels, one path to the source, no dependency.
+__global__ void k_bc_source_init(const int32_t* lv, VState* st, uint64_t V, uint32_t s) {
   const uint64_t v = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (v < V) {
-    sigma[v] = v == s ? 1.0f : 0.0f;
-    delta[v] = 0.0f;
+    VState x;
+    x.lv = lv[v];
+    x.sigma = v == s ? 1.0f : 0.0f;
+    x.delta = 0.0f;
+    x.pad = 0.0f;
+    st[v] = x;
   }
 }
 
@@ -360,10 +383,9 @@ int ll_gpu_bc(ll_gpu_graph* g, int level, const int64_t* sources, int64_t n, flo
     const RowBlocks& out_rb = ensure_rowblocks(g, LL_GPU_DIR_OUT, level, kCap, kPiece);
     const uint32_t* outdeg = ensure_degrees(g, LL_GPU_DIR_OUT, level);
     const uint64_t nin = std::max<uint64_t>(in_rb.n, 1), nout = std::max<uint64_t>(out_rb.n, 1);
-    DevBuf lv, visited, sigma, delta, bc, part, in_mask, out_mask, in_list, out_list, cnt, hub;
+    DevBuf lv, visited, st, bc, part, in_mask, out_mask, in_list, out_list, cnt, hub;
     lv.alloc(V * 4, s);
-    sigma.alloc(V * 4, s);
-    delta.alloc(V * 4, s);
+    st.alloc(V * sizeof(VState), s);
     bc.alloc(V * 4, s);
     part.alloc(std::max(nin, nout) * 4, s);
     in_mask.alloc(nin * 4, s);
@@ -383,7 +405,7 @@ int ll_gpu_bc(ll_gpu_graph* g, int level, const int64_t* sources, int64_t n, flo
       // find nothing): a level without rows has an empty list
       const int32_t D =
           bfs_core(g, out, in, in_rb, outdeg, src, dead, visited, lv.as<int32_t>()).levels;
-      k_bc_source_init<<<grid_for(V, 256), 256, 0, s>>>(sigma.as<float>(), delta.as<float>(), V,
+      k_bc_source_init<<<grid_for(V, 256), 256, 0, s>>>(lv.as<int32_t>(), st.as<VState>(), V,
                                                         src);
       k_fill_u64<<<1, 64, 0, s>>>(reinterpret_cast<unsigned long long*>(c_in), 48);
       k_item_levels<<<mask_grid, 256, 0, s>>>(in_rb.items.as<RowItem>(), in_rb.n,
@@ -411,9 +433,7 @@ int ll_gpu_bc(ll_gpu_graph* g, int level, const int64_t* sources, int64_t n, flo
       for (int l = 0; l < 32 && !hub.p; l++)
         if (h[64 + l]) hub.alloc(out.E * 4, s);
       BcArgs a;
-      a.lv = lv.as<int32_t>();
-      a.sigma = sigma.as<float>();
-      a.delta = delta.as<float>();
+      a.st = st.as<VState>();
       a.bc = bc.as<float>();
       a.part = part.as<float>();
       a.hub = hub.as<float>();
