@@ -276,26 +276,42 @@ __global__ __launch_bounds__(kBlock) void k_bc_hub_terms(BcArgs a, const uint32_
 // Level lists: list[l * n + j] = the j-th item with a row on level l (levels
 // >= 31 share list 31), counts in cnt[l].  Called for the long rows' pieces
 // first and then for the other items, so each list starts with its long rows.
-// Positions come from one atomic per wave and level.
-__global__ void k_level_lists(const RowItem* items, uint64_t n, const uint32_t* mask,
-                              bool long_rows, uint32_t* list, uint32_t* cnt) {
+// Positions come from one global atomic per workgroup and level (the waves
+// take their slots in the workgroup's share from an LDS counter).
+constexpr unsigned kListBlock = 1024;  // workgroup of k_level_lists
+
+__global__ __launch_bounds__(kListBlock) void k_level_lists(const RowItem* items, uint64_t n,
+                                                            const uint32_t* mask, bool long_rows,
+                                                            uint32_t* list, uint32_t* cnt) {
+  constexpr uint32_t kWaves = kListBlock / 64;
+  __shared__ uint32_t bcount[32], bbase[32], woff[kWaves][32];
   const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  const uint32_t lane = threadIdx.x & 63;
+  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   uint32_t m = 0;
   if (i < n && (items[i].npieces != 0) == long_rows) m = mask[i];
+  if (threadIdx.x < 32) bcount[threadIdx.x] = 0;
+  __syncthreads();
   uint32_t wm = m;
   for (int o = 32; o > 0; o >>= 1) wm |= __shfl_xor(wm, o, 64);
-  while (wm) {
-    const uint32_t l = uint32_t(__builtin_ctz(wm));
-    wm &= wm - 1;
+  // the wave's slot in the workgroup's share of each level's list
+  for (uint32_t x = wm; x;) {
+    const uint32_t l = uint32_t(__builtin_ctz(x));
+    x &= x - 1;
     const uint64_t b = __ballot((m >> l) & 1u);
-    const uint32_t leader = uint32_t(__builtin_ctzll(b));
-    uint32_t base = 0;
-    if (lane == leader) base = atomicAdd(&cnt[l], uint32_t(__popcll(b)));
-    base = uint32_t(__shfl(int(base), int(leader), 64));
+    if (lane == uint32_t(__builtin_ctzll(b))) woff[wave][l] = atomicAdd(&bcount[l], uint32_t(__popcll(b)));
+  }
+  __syncthreads();
+  // one global atomic per workgroup and level
+  if (threadIdx.x < 32 && bcount[threadIdx.x])
+    bbase[threadIdx.x] = atomicAdd(&cnt[threadIdx.x], bcount[threadIdx.x]);
+  __syncthreads();
+  for (uint32_t x = wm; x;) {
+    const uint32_t l = uint32_t(__builtin_ctz(x));
+    x &= x - 1;
+    const uint64_t b = __ballot((m >> l) & 1u);
     if ((m >> l) & 1u) {
       const uint64_t below = lane ? (b & (~0ull >> (64 - lane))) : 0ull;
-      list[uint64_t(l) * n + base + uint32_t(__popcll(below))] = uint32_t(i);
+      list[uint64_t(l) * n + bbase[l] + woff[wave][l] + uint32_t(__popcll(below))] = uint32_t(i);
     }
   }
 }
@@ -413,14 +429,14 @@ int ll_gpu_bc(ll_gpu_graph* g, int level, const int64_t* sources, int64_t n, flo
       k_item_levels<<<mask_grid, 256, 0, s>>>(out_rb.items.as<RowItem>(), out_rb.n,
                                               lv.as<int32_t>(), out_mask.as<uint32_t>());
       for (bool long_rows : {true, false})
-        k_level_lists<<<grid_for(in_rb.n, 256), 256, 0, s>>>(
+        k_level_lists<<<grid_for(in_rb.n, kListBlock), kListBlock, 0, s>>>(
             in_rb.items.as<RowItem>(), in_rb.n, in_mask.as<uint32_t>(), long_rows,
             in_list.as<uint32_t>(), c_in);
-      k_level_lists<<<grid_for(out_rb.n, 256), 256, 0, s>>>(
+      k_level_lists<<<grid_for(out_rb.n, kListBlock), kListBlock, 0, s>>>(
           out_rb.items.as<RowItem>(), out_rb.n, out_mask.as<uint32_t>(), true,
           out_list.as<uint32_t>(), c_out);
       k_copy_u32<<<1, 32, 0, s>>>(c_out, c_long, 32);
-      k_level_lists<<<grid_for(out_rb.n, 256), 256, 0, s>>>(
+      k_level_lists<<<grid_for(out_rb.n, kListBlock), kListBlock, 0, s>>>(
           out_rb.items.as<RowItem>(), out_rb.n, out_mask.as<uint32_t>(), false,
           out_list.as<uint32_t>(), c_out);
       LLG_HIP(hipGetLastError());
