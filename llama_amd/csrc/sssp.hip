@@ -87,8 +87,11 @@ __global__ void k_relax_blocks(const uint64_t* qofs, uint64_t nq, uint64_t* blk)
 }
 
 // Relax the queue's out-edges.  An improved vertex with out-edges is
-// appended to the next queue once per round (stamp, one atomic per wave) when
-// its distance is below the threshold (bits t_bits), else flagged far.
+// appended to the next queue once per round (stamp) when its distance is
+// below the threshold (bits t_bits), else flagged far.  An edge block's
+// appends are staged in LDS and reserved in the next queue with one atomic,
+// not one per wave: the queue counter is a single address every workgroup
+// adds to.
 __global__ __launch_bounds__(kBlock) void k_relax(const uint32_t* q, const uint32_t* qd,
                                                   const uint64_t* qofs, uint64_t nq,
                                                   const uint64_t* blk, const uint64_t* ofs,
@@ -99,6 +102,9 @@ __global__ __launch_bounds__(kBlock) void k_relax(const uint32_t* q, const uint3
   __shared__ uint64_t l_qofs[kEdgesPerBlock + 1];
   __shared__ uint64_t l_base[kEdgesPerBlock];
   __shared__ float l_dist[kEdgesPerBlock];
+  __shared__ uint32_t l_q[kEdgesPerBlock];  // at most one append per edge
+  __shared__ uint32_t l_qn;
+  __shared__ unsigned long long l_qbase;
   const uint64_t m = qofs[nq];
   const uint64_t nblk = (m + kEdgesPerBlock - 1) / kEdgesPerBlock;
   const int lane = threadIdx.x & 63;
@@ -116,6 +122,7 @@ __global__ __launch_bounds__(kBlock) void k_relax(const uint32_t* q, const uint3
         l_base[i] = ofs[q[lo + i]];
       }
     }
+    if (threadIdx.x == 0) l_qn = 0;
     __syncthreads();
     for (uint64_t base = x0; base < x1; base += kBlock) {
       const uint64_t x = base + threadIdx.x;
@@ -146,13 +153,17 @@ __global__ __launch_bounds__(kBlock) void k_relax(const uint32_t* q, const uint3
       const uint64_t mk = __ballot(take);
       if (mk) {
         const int leader = __ffsll((long long)mk) - 1;
-        unsigned long long qb = 0;
-        if (lane == leader) qb = atomicAdd(n_next, (unsigned long long)__popcll(mk));
-        qb = __shfl(qb, leader, 64);
-        if (take) q_next[qb + __popcll(mk & ((uint64_t(1) << lane) - 1))] = v;
+        uint32_t qb = 0;
+        if (lane == leader) qb = atomicAdd(&l_qn, uint32_t(__popcll(mk)));
+        qb = uint32_t(__shfl(int(qb), leader, 64));
+        if (take) l_q[qb + uint32_t(__popcll(mk & ((uint64_t(1) << lane) - 1)))] = v;
       }
     }
-    __syncthreads();  // the LDS slice is reloaded for the next block
+    __syncthreads();
+    if (threadIdx.x == 0 && l_qn) l_qbase = atomicAdd(n_next, (unsigned long long)l_qn);
+    __syncthreads();
+    for (uint32_t k = threadIdx.x; k < l_qn; k += kBlock) q_next[l_qbase + k] = l_q[k];
+    __syncthreads();  // the LDS slice and queue are reloaded for the next block
   }
 }
 
@@ -424,7 +435,7 @@ int ll_gpu_sssp(ll_gpu_graph* g, int level, int64_t root, float* dist_out, uint6
                                                  far.as<uint8_t>(), relaxed.as<uint32_t>(), V,
                                                  uint32_t(root), q[0].as<uint32_t>());
     LLG_HIP(hipGetLastError());
-    const unsigned relax_grid = 4u * unsigned(ctx->num_cus);  // 40 KB LDS per workgroup
+    const unsigned relax_grid = 3u * unsigned(ctx->num_cus);  // 48 KB LDS per workgroup
     uint64_t nq = 1, nrounds = 0;
     int cur = 0;
     double T = near_far ? delta : double(kUnreachedDist);
