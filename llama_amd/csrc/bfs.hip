@@ -91,22 +91,26 @@ __global__ void k_queue_degrees(const uint32_t* q, uint64_t n, const uint32_t* o
   if (i < n) qdeg[i] = outdeg[q[i]];
 }
 
-// Append the calling lane's vertex to the queue with one atomic per wave.
-__device__ inline void wave_append(bool take, uint32_t v, uint32_t outdeg, uint32_t* q,
-                                   Counters* c) {
+// Stage the calling lane's vertex (when it has out-edges) in the workgroup's
+// LDS queue lq (count *lqn) with one LDS atomic per wave; the frontier
+// out-edge count goes to the striped counters.  The workgroup later reserves
+// its whole share of the next queue with one atomic: the queue counter is a
+// single address every workgroup adds to.
+__device__ inline void wave_stage(bool take, uint32_t v, uint32_t outdeg, uint32_t* lq,
+                                  uint32_t* lqn, Counters* c) {
   const uint64_t m = __ballot(take);
   const uint64_t mq = __ballot(take && outdeg > 0);
   const int lane = threadIdx.x & 63;
   const int leader = m ? __ffsll((long long)m) - 1 : 0;
-  unsigned long long qbase = 0;
+  uint32_t qbase = 0;
   uint64_t md = take ? outdeg : 0;
   for (int o = 32; o > 0; o >>= 1) md += __shfl_xor(md, o, 64);
   if (lane == leader && m) {
     atomicAdd(&stripe(c)->m_next, (unsigned long long)md);
-    if (mq) qbase = atomicAdd(&c->n_next, (unsigned long long)__popcll(mq));
+    if (mq) qbase = atomicAdd(lqn, uint32_t(__popcll(mq)));
   }
-  qbase = __shfl(qbase, leader, 64);
-  if (take && outdeg > 0) q[qbase + __popcll(mq & ((uint64_t(1) << lane) - 1))] = v;
+  qbase = uint32_t(__shfl(int(qbase), leader, 64));
+  if (take && outdeg > 0) lq[qbase + uint32_t(__popcll(mq & ((uint64_t(1) << lane) - 1)))] = v;
 }
 
 __global__ __launch_bounds__(kBlock) void k_td_expand(
@@ -116,6 +120,9 @@ __global__ __launch_bounds__(kBlock) void k_td_expand(
   __shared__ uint64_t l_qofs[kEdgesPerBlock + 1];
   __shared__ uint64_t l_base[kEdgesPerBlock];
   __shared__ uint64_t sh_lo, sh_hi;
+  __shared__ uint32_t l_q[kEdgesPerBlock];  // at most one append per edge
+  __shared__ uint32_t l_qn;
+  __shared__ unsigned long long l_qbase;
   const uint64_t x0 = uint64_t(blockIdx.x) * kEdgesPerBlock;
   const uint64_t x1 = min(m_f, x0 + kEdgesPerBlock);
   if (threadIdx.x == 0) {
@@ -125,6 +132,7 @@ __global__ __launch_bounds__(kBlock) void k_td_expand(
     uint64_t lo2 = lo, hi2 = nq;
     while (hi2 - lo2 > 1) { uint64_t mid = (lo2 + hi2) / 2; if (qofs[mid] <= x1 - 1) lo2 = mid; else hi2 = mid; }
     sh_hi = lo2;
+    l_qn = 0;
   }
   __syncthreads();
   const uint64_t lo = sh_lo;
@@ -152,9 +160,15 @@ __global__ __launch_bounds__(kBlock) void k_td_expand(
         }
       }
     }
-    wave_append(take, v, od, q_next, c);
+    wave_stage(take, v, od, l_q, &l_qn, c);
   }
-  if (threadIdx.x == 0) atomicAdd(&stripe(c)->examined, (unsigned long long)(x1 - x0));
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    atomicAdd(&stripe(c)->examined, (unsigned long long)(x1 - x0));
+    if (l_qn) l_qbase = atomicAdd(&c->n_next, (unsigned long long)l_qn);
+  }
+  __syncthreads();
+  for (uint32_t k = threadIdx.x; k < l_qn; k += kBlock) q_next[l_qbase + k] = l_q[k];
 }
 
 __global__ void k_queue_to_bitmap(const uint32_t* q, uint64_t n, uint64_t* bm) {
