@@ -21,6 +21,7 @@
 // frontier shrinks below V / beta (options "bfs_alpha", "bfs_beta"; defaults
 // 14 and 24).
 #include <climits>
+#include <optional>
 
 #include "ll_gpu_internal.h"
 #include "rowblocks.h"
@@ -269,7 +270,6 @@ __global__ __launch_bounds__(kBlock) void k_bu_rows(const RowItem* items, const 
       for (uint32_t k = 0; k < kPer; k++) fw[k] = need[k] ? fbm[u[k] >> 6] : 0ull;
 #pragma unroll
       for (uint32_t k = 0; k < kPer; k++) {
-        const uint32_t i = k * kBlock + threadIdx.x;
         const uint64_t m = __ballot(need[k] && ((fw[k] >> (u[k] & 63)) & 1));
         if (lane == 0) bits[k * (kBlock / 64) + wave] = m;
       }
@@ -409,10 +409,10 @@ __global__ void k_count_reached(const int32_t* lv, uint64_t V, unsigned long lon
   block_atomic_add(out, n);
 }
 
-Counters read_counters(ll_gpu_ctx* ctx, const DevBuf& c) {
+Counters read_counters(ll_gpu_ctx* ctx, const Counters* c) {
   static_assert(kStripes * sizeof(Counters) <= kReadbackBytes, "stripes fit the read-back buffer");
   const Counters* st = static_cast<const Counters*>(ctx->readback);
-  LLG_HIP(hipMemcpyAsync(ctx->readback, c.p, kStripes * sizeof(Counters), hipMemcpyDeviceToHost,
+  LLG_HIP(hipMemcpyAsync(ctx->readback, c, kStripes * sizeof(Counters), hipMemcpyDeviceToHost,
                          ctx->stream));
   LLG_HIP(hipStreamSynchronize(ctx->stream));
   Counters h = st[0];
@@ -439,30 +439,48 @@ const uint64_t* ensure_dead_rows(ll_gpu_graph* g, const Csr& in) {
   return g->dead_in.as<uint64_t>();
 }
 
-// bfs_core: see ll_gpu_internal.h.
-BfsResult bfs_core(ll_gpu_graph* g, const Csr& fwd, const Csr& rev, const RowBlocks& rev_rb,
-                   const uint32_t* fdeg, uint32_t root, const uint64_t* excluded,
-                   DevBuf& visited, int32_t* lv) {
+namespace {
+
+// Device memory of one search: bitmaps of ceil(V/64) words, queues of V
+// entries, queue offsets of V+1, the striped counters and the scan's
+// temporary memory.  `timed` records per-kernel times when profiling is on.
+struct BfsWork {
+  uint64_t* visited;
+  uint64_t* fbm;
+  uint64_t* nbm;
+  uint32_t* q[2];
+  uint32_t* qdeg;
+  uint64_t* qofs;
+  Counters* cnt;
+  void* scan_tmp;
+  bool timed;
+};
+
+// Launch timer that records only for searches that may allocate.
+struct MaybeTimed {
+  std::optional<TimedLaunch> t;
+  MaybeTimed(ll_gpu_ctx* ctx, const char* name, bool on) {
+    if (on) t.emplace(ctx, name);
+  }
+};
+
+// bfs_core: see ll_gpu_internal.h; this form runs in caller-provided memory.
+BfsResult bfs_core_ws(ll_gpu_graph* g, const Csr& fwd, const Csr& rev, const RowBlocks& rev_rb,
+                      const uint32_t* fdeg, uint32_t root, const uint64_t* excluded,
+                      const BfsWork& w, int32_t* lv) {
   ll_gpu_ctx* ctx = g->ctx;
   hipStream_t s = ctx->stream;
   const uint64_t V = g->V;
   const uint64_t words = (V + 63) / 64;
-  DevBuf fbm, nbm, q[2], qdeg, qofs, cnt;
-  visited.alloc(words * 8, s);
-  fbm.alloc(words * 8, s);
-  nbm.alloc(words * 8, s);
-  q[0].alloc(V * 4, s);
-  q[1].alloc(V * 4, s);
-  qdeg.alloc(V * 4, s);
-  qofs.alloc((V + 1) * 8, s);
-  cnt.alloc(kStripes * sizeof(Counters), s);
-  k_bfs_init<<<grid_for(std::max(V, words), 256), 256, 0, s>>>(lv, visited.as<uint64_t>(),
+  uint64_t* fbm = w.fbm;  // frontier / next-frontier bitmaps, swapped per bottom-up level
+  uint64_t* nbm = w.nbm;
+  k_bfs_init<<<grid_for(std::max(V, words), 256), 256, 0, s>>>(lv, w.visited,
                                                               excluded, V, words);
-  k_zero_counters<<<1, kStripes, 0, s>>>(cnt.as<Counters>());
-  k_bfs_root<<<1, 1, 0, s>>>(lv, visited.as<uint64_t>(), q[0].as<uint32_t>(), root, fdeg,
-                             cnt.as<Counters>());
+  k_zero_counters<<<1, kStripes, 0, s>>>(w.cnt);
+  k_bfs_root<<<1, 1, 0, s>>>(lv, w.visited, w.q[0], root, fdeg,
+                             w.cnt);
   LLG_HIP(hipGetLastError());
-  Counters h = read_counters(ctx, cnt);
+  Counters h = read_counters(ctx, w.cnt);
   BfsResult r;
   uint64_t nq = h.n_next;       // queue entries (vertices with out-edges)
   uint64_t n_f = 1;             // frontier vertices
@@ -480,34 +498,34 @@ BfsResult bfs_core(ll_gpu_graph* g, const Csr& fwd, const Csr& rev, const RowBlo
     const uint64_t m_u = E > m_visited ? E - m_visited : 0;
     if (!bottom_up && m_f > m_u / alpha) {
       bottom_up = true;
-      k_fill_u64<<<grid_for(words, 256), 256, 0, s>>>(fbm.as<uint64_t>(), words, 0);
-      k_queue_to_bitmap<<<grid_for(nq, 256), 256, 0, s>>>(q[cur].as<uint32_t>(), nq,
-                                                          fbm.as<uint64_t>());
+      k_fill_u64<<<grid_for(words, 256), 256, 0, s>>>(fbm, words, 0);
+      k_queue_to_bitmap<<<grid_for(nq, 256), 256, 0, s>>>(w.q[cur], nq,
+                                                          fbm);
     }
-    k_zero_counters<<<1, kStripes, 0, s>>>(cnt.as<Counters>());
+    k_zero_counters<<<1, kStripes, 0, s>>>(w.cnt);
     if (bottom_up) {
-      k_fill_u64<<<grid_for(words, 256), 256, 0, s>>>(nbm.as<uint64_t>(), words, 0);
+      k_fill_u64<<<grid_for(words, 256), 256, 0, s>>>(nbm, words, 0);
       {
-        TimedLaunch t(ctx, "k_bu_step");
+        MaybeTimed t(ctx, "k_bu_step", w.timed);
         k_bu_rows<<<unsigned(rev_rb.n), kBlock, 0, s>>>(
             rev_rb.items.as<RowItem>(), rev.ofs.as<uint64_t>(), rev.adj.as<uint32_t>(), fdeg,
-            fbm.as<uint64_t>(), visited.as<uint64_t>(), nbm.as<uint64_t>(), lv, lvl + 1,
-            cnt.as<Counters>());
+            fbm, w.visited, nbm, lv, lvl + 1,
+            w.cnt);
       }
-      k_visited_or<<<grid_for(words, 256), 256, 0, s>>>(visited.as<uint64_t>(),
-                                                        nbm.as<uint64_t>(), words);
+      k_visited_or<<<grid_for(words, 256), 256, 0, s>>>(w.visited,
+                                                        nbm, words);
     } else {
-      k_queue_degrees<<<grid_for(nq, 256), 256, 0, s>>>(q[cur].as<uint32_t>(), nq, fdeg,
-                                                        qdeg.as<uint32_t>());
-      scan_u32_to_u64(qdeg.as<uint32_t>(), qofs.as<uint64_t>(), nq, s);
-      TimedLaunch t(ctx, "k_td_expand");
+      k_queue_degrees<<<grid_for(nq, 256), 256, 0, s>>>(w.q[cur], nq, fdeg,
+                                                        w.qdeg);
+      scan_u32_to_u64_ws(w.qdeg, w.qofs, nq, w.scan_tmp, s);
+      MaybeTimed t(ctx, "k_td_expand", w.timed);
       k_td_expand<<<grid_for(m_f, kEdgesPerBlock), kBlock, 0, s>>>(
-          q[cur].as<uint32_t>(), qofs.as<uint64_t>(), nq, m_f, fwd.ofs.as<uint64_t>(),
-          fwd.adj.as<uint32_t>(), fdeg, visited.as<uint64_t>(), lv, lvl + 1,
-          q[cur ^ 1].as<uint32_t>(), cnt.as<Counters>());
+          w.q[cur], w.qofs, nq, m_f, fwd.ofs.as<uint64_t>(),
+          fwd.adj.as<uint32_t>(), fdeg, w.visited, lv, lvl + 1,
+          w.q[cur ^ 1], w.cnt);
     }
     LLG_HIP(hipGetLastError());
-    h = read_counters(ctx, cnt);
+    h = read_counters(ctx, w.cnt);
     r.examined += h.examined;
     const uint64_t n_next = h.n_next;
     lvl++;
@@ -521,11 +539,11 @@ BfsResult bfs_core(ll_gpu_graph* g, const Csr& fwd, const Csr& rev, const RowBlo
       if (n_f == 0) break;
       if (shrinking && n_f < V / beta) {
         bottom_up = false;
-        k_zero_counters<<<1, kStripes, 0, s>>>(cnt.as<Counters>());
+        k_zero_counters<<<1, kStripes, 0, s>>>(w.cnt);
         k_bitmap_to_queue<<<grid_for(words, kBlock), kBlock, 0, s>>>(
-            fbm.as<uint64_t>(), V, fdeg, q[cur].as<uint32_t>(), cnt.as<Counters>());
+            fbm, V, fdeg, w.q[cur], w.cnt);
         LLG_HIP(hipGetLastError());
-        nq = read_counters(ctx, cnt).n_next;
+        nq = read_counters(ctx, w.cnt).n_next;
       }
     } else {
       // top-down n_next counts the queued vertices (those with out-edges)
@@ -540,29 +558,122 @@ BfsResult bfs_core(ll_gpu_graph* g, const Csr& fwd, const Csr& rev, const RowBlo
   return r;
 }
 
+}  // namespace
+
+BfsResult bfs_core(ll_gpu_graph* g, const Csr& fwd, const Csr& rev, const RowBlocks& rev_rb,
+                   const uint32_t* fdeg, uint32_t root, const uint64_t* excluded,
+                   DevBuf& visited, int32_t* lv) {
+  hipStream_t s = g->ctx->stream;
+  const uint64_t V = g->V;
+  const uint64_t words = (V + 63) / 64;
+  DevBuf fbm, nbm, q[2], qdeg, qofs, cnt, stmp;
+  visited.alloc(words * 8, s);
+  fbm.alloc(words * 8, s);
+  nbm.alloc(words * 8, s);
+  q[0].alloc(V * 4, s);
+  q[1].alloc(V * 4, s);
+  qdeg.alloc(V * 4, s);
+  qofs.alloc((V + 1) * 8, s);
+  cnt.alloc(kStripes * sizeof(Counters), s);
+  const size_t sb = scan_tmp_bytes(V);
+  if (sb) stmp.alloc(sb, s);
+  BfsWork w{visited.as<uint64_t>(), fbm.as<uint64_t>(), nbm.as<uint64_t>(),
+            {q[0].as<uint32_t>(), q[1].as<uint32_t>()}, qdeg.as<uint32_t>(), qofs.as<uint64_t>(),
+            cnt.as<Counters>(), stmp.p, true};
+  return bfs_core_ws(g, fwd, rev, rev_rb, fdeg, root, excluded, w, lv);
+}
+
+// Working memory of one single-source search, carved into 256-byte aligned
+// regions: three bitmaps, two queues and the queue degrees, the queue
+// offsets, the striped counters, the reached counter and the scan's
+// temporary memory.
+namespace {
+struct WsLayout {
+  size_t visited, fbm, nbm, q0, q1, qdeg, qofs, cnt, red, stmp, total;
+};
+WsLayout ws_layout(uint64_t V) {
+  const uint64_t words = (V + 63) / 64;
+  auto up = [](size_t x) { return (x + 255) & ~size_t(255); };
+  WsLayout L{};
+  size_t o = 0;
+  auto take = [&](size_t bytes) { const size_t at = o; o += up(bytes); return at; };
+  L.visited = take(words * 8);
+  L.fbm = take(words * 8);
+  L.nbm = take(words * 8);
+  L.q0 = take(V * 4);
+  L.q1 = take(V * 4);
+  L.qdeg = take(V * 4);
+  L.qofs = take((V + 1) * 8);
+  L.cnt = take(kStripes * sizeof(Counters));
+  L.red = take(8);
+  L.stmp = take(scan_tmp_bytes(V));
+  L.total = o;
+  return L;
+}
+}  // namespace
+
+size_t bfs_work_bytes(uint64_t V) { return ws_layout(V).total; }
+
+bool bfs_peek(ll_gpu_graph* g, int level, BfsPrepared* p) {
+  level = resolve_level(g, level);
+  const Csr* c[2];
+  for (uint32_t d = 0; d < 2; d++) {
+    c[d] = g->slcsr ? &g->sl[d] : &g->csr[d];
+    if (!c[d]->valid || (!g->slcsr && c[d]->level != level)) return false;
+  }
+  if (g->deg_level[LL_GPU_DIR_OUT] != level || !g->deg[LL_GPU_DIR_OUT].p) return false;
+  const RowBlocks* rb = nullptr;
+  for (const RowBlocks& r : g->rb[LL_GPU_DIR_IN])
+    if (r.valid && r.cap == kBuCap && r.piece == kPiece) rb = &r;
+  if (!rb) return false;
+  if (g->dead_level != c[LL_GPU_DIR_IN]->level || !g->dead_in.p) return false;
+  *p = BfsPrepared{c[LL_GPU_DIR_OUT], c[LL_GPU_DIR_IN], g->deg[LL_GPU_DIR_OUT].as<uint32_t>(), rb,
+                   g->dead_in.as<uint64_t>()};
+  return true;
+}
+
+BfsPrepared bfs_prepare(ll_gpu_graph* g, int level) {
+  BfsPrepared p;
+  p.out = &ensure_csr(g, LL_GPU_DIR_OUT, level);
+  p.in = &ensure_csr(g, LL_GPU_DIR_IN, level);
+  p.outdeg = ensure_degrees(g, LL_GPU_DIR_OUT, level);
+  p.in_rb = &ensure_rowblocks(g, LL_GPU_DIR_IN, level, kBuCap, kPiece);
+  p.dead = ensure_dead_rows(g, *p.in);
+  return p;
+}
+
+BfsResult bfs_single_ws(ll_gpu_graph* g, const BfsPrepared& p, uint32_t root, void* ws,
+                        int32_t* lv, bool timed) {
+  hipStream_t s = g->ctx->stream;
+  const uint64_t V = g->V;
+  const WsLayout L = ws_layout(V);
+  char* b = static_cast<char*>(ws);
+  BfsWork w{reinterpret_cast<uint64_t*>(b + L.visited), reinterpret_cast<uint64_t*>(b + L.fbm),
+            reinterpret_cast<uint64_t*>(b + L.nbm),
+            {reinterpret_cast<uint32_t*>(b + L.q0), reinterpret_cast<uint32_t*>(b + L.q1)},
+            reinterpret_cast<uint32_t*>(b + L.qdeg), reinterpret_cast<uint64_t*>(b + L.qofs),
+            reinterpret_cast<Counters*>(b + L.cnt), b + L.stmp, timed};
+  BfsResult r = bfs_core_ws(g, *p.out, *p.in, *p.in_rb, p.outdeg, root, p.dead, w, lv);
+  // top-down levels count only queued vertices: recount the reached set
+  unsigned long long* red = reinterpret_cast<unsigned long long*>(b + L.red);
+  k_fill_u64<<<1, 1, 0, s>>>(reinterpret_cast<uint64_t*>(red), 1, 0);
+  k_count_reached<<<std::min(grid_for(V, 256), 4u * unsigned(g->ctx->num_cus)), 256, 0, s>>>(
+      lv, V, red);
+  LLG_HIP(hipGetLastError());
+  LLG_HIP(hipMemcpyAsync(g->ctx->readback, red, 8, hipMemcpyDeviceToHost, s));
+  LLG_HIP(hipStreamSynchronize(s));
+  r.reached = *static_cast<const uint64_t*>(g->ctx->readback);
+  return r;
+}
+
 // Single-source BFS; per-vertex levels stay in `lv` (device, int32[V]).
 BfsResult bfs_device(ll_gpu_graph* g, int level, uint32_t root, DevBuf& lv) {
   hipStream_t s = g->ctx->stream;
-  const uint64_t V = g->V;
-  const Csr& out = ensure_csr(g, LL_GPU_DIR_OUT, level);
-  const Csr& in = ensure_csr(g, LL_GPU_DIR_IN, level);
-  const uint32_t* outdeg = ensure_degrees(g, LL_GPU_DIR_OUT, level);
-  const RowBlocks& in_rb = ensure_rowblocks(g, LL_GPU_DIR_IN, level, kBuCap, kPiece);
-  lv.alloc(V * 4, s);
-  DevBuf visited;
-  BfsResult r = bfs_core(g, out, in, in_rb, outdeg, root, ensure_dead_rows(g, in), visited,
-                         lv.as<int32_t>());
-  DevBuf red;
-  red.alloc(8, s);
-  k_fill_u64<<<1, 1, 0, s>>>(red.as<uint64_t>(), 1, 0);
-  k_count_reached<<<std::min(grid_for(V, 256), 4u * unsigned(g->ctx->num_cus)), 256, 0, s>>>(
-      lv.as<int32_t>(), V, red.as<unsigned long long>());
-  LLG_HIP(hipGetLastError());
-  uint64_t hr = 0;
-  LLG_HIP(hipMemcpyAsync(&hr, red.p, 8, hipMemcpyDeviceToHost, s));
-  LLG_HIP(hipStreamSynchronize(s));
-  r.reached = hr;
-  return r;
+  const BfsPrepared p = bfs_prepare(g, level);
+  lv.alloc(g->V * 4, s);
+  DevBuf ws;
+  ws.alloc(bfs_work_bytes(g->V), s);
+  return bfs_single_ws(g, p, root, ws.p, lv.as<int32_t>(), true);
 }
 
 }  // namespace llgpu

@@ -310,6 +310,28 @@ BfsResult bfs_core(ll_gpu_graph* g, const Csr& fwd, const Csr& rev, const RowBlo
 const uint64_t* ensure_dead_rows(ll_gpu_graph* g, const Csr& in);
 constexpr uint32_t kBfsRowCap = 1024;  // row-block capacity used by bottom-up steps
 
+// What a single-source search reads besides its working memory: both CSRs
+// of the snapshot, the out-degrees, the in-CSR row blocks and the dead-row
+// bitmap.
+struct BfsPrepared {
+  const Csr* out = nullptr;
+  const Csr* in = nullptr;
+  const uint32_t* outdeg = nullptr;
+  const RowBlocks* in_rb = nullptr;
+  const uint64_t* dead = nullptr;
+};
+// Builds what is missing (allocates).
+BfsPrepared bfs_prepare(ll_gpu_graph* g, int level);
+// The cached structures of `level` only: false when any is missing.
+bool bfs_peek(ll_gpu_graph* g, int level, BfsPrepared* p);
+// Bytes of one search's working memory for V vertices.
+size_t bfs_work_bytes(uint64_t V);
+// Single-source search in `ws` (bfs_work_bytes(V) bytes, 256-byte aligned):
+// levels into lv (device int32[V]); reached = vertices with a level.  Makes
+// no device allocation; `timed` records per-kernel times when profiling.
+BfsResult bfs_single_ws(ll_gpu_graph* g, const BfsPrepared& p, uint32_t root, void* ws,
+                        int32_t* lv, bool timed);
+
 int resolve_level(const ll_gpu_graph* g, int level);
 const uint32_t* ensure_degrees(ll_gpu_graph* g, uint32_t dir, int level);
 const Csr& ensure_csr(ll_gpu_graph* g, uint32_t dir, int level);
@@ -329,6 +351,9 @@ void build_pr_hotcold(ll_gpu_graph* g, const Csr& in, const uint32_t* slot_adj, 
 
 // Device-wide exclusive scan of n values into out (n+1 entries, total last).
 void scan_u32_to_u64(const uint32_t* in, uint64_t* out, uint64_t n, hipStream_t s);
+// The same in caller memory: tmp holds scan_tmp_bytes(n) bytes (8-byte aligned).
+size_t scan_tmp_bytes(uint64_t n);
+void scan_u32_to_u64_ws(const uint32_t* in, uint64_t* out, uint64_t n, void* tmp, hipStream_t s);
 void scan_u64(const uint64_t* in, uint64_t* out, uint64_t n, hipStream_t s);
 void sort_pairs_u64_u32(const uint64_t* keys_in, uint64_t* keys_out, const uint32_t* vals_in,
                         uint32_t* vals_out, uint64_t n, int end_bit, hipStream_t s);

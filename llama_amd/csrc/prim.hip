@@ -1,7 +1,20 @@
-// Device-wide scan and sort plumbing (rocPRIM through hipCUB).  These carry no
-// graph logic: they turn per-vertex / per-block counts into offsets and order
-// key arrays for the snapshot and triangle-counting preprocessing.
-#include <hipcub/hipcub.hpp>
+// Device-wide scans, stable radix sorts and run compaction, written for
+// 64-wide waves.  They carry no graph logic: they turn per-vertex / per-block
+// counts into offsets and order key arrays for the snapshot, PageRank and
+// triangle-counting preprocessing.
+//
+// Scan: tiles of 4096 items (256 threads x 16 rounds); tile totals are
+// scanned recursively, then every tile rescans its items from its prefix.
+// Integer sums, so the result does not depend on the schedule.
+//
+// Radix sort: least-significant digit first, 8-bit digits, stable.  One
+// pre-pass counts every digit position at once; passes whose digit is the
+// same for all keys are skipped (the host reads the 8 x 256 counts back).  A
+// pass counts digits per tile, scans the counts digit-major (digit, tile)
+// and scatters: each round of 256 keys ranks a key among the equal digits
+// of its wave with 8 ballots, adds the counts of the waves before it (LDS)
+// and the tile's running offset of that digit.
+#include <algorithm>
 
 #include "ll_gpu_internal.h"
 
@@ -9,89 +22,305 @@ namespace llgpu {
 
 namespace {
 
-struct WidenU32 {
-  __host__ __device__ uint64_t operator()(uint32_t x) const { return uint64_t(x); }
+constexpr int kB = 256;
+constexpr int kRounds = 16;
+constexpr uint64_t kTile = uint64_t(kB) * kRounds;
+
+struct SrcU32 {
+  const uint32_t* p;
+  __device__ uint64_t operator()(uint64_t i) const { return p[i]; }
+};
+struct SrcU64 {
+  const uint64_t* p;
+  __device__ uint64_t operator()(uint64_t i) const { return p[i]; }
+};
+// 1 where a sorted run of equal values starts
+struct SrcRunStart {
+  const uint64_t* p;
+  __device__ uint64_t operator()(uint64_t i) const { return (i == 0 || p[i] != p[i - 1]) ? 1 : 0; }
 };
 
-__global__ void k_write_total_u32(const uint32_t* in, uint64_t* out, uint64_t n) {
-  out[n] = n ? out[n - 1] + uint64_t(in[n - 1]) : 0;
+// Exclusive prefix of x over the workgroup (all threads call); *total = sum.
+__device__ inline uint64_t block_exclusive_scan(uint64_t x, uint64_t* total) {
+  __shared__ uint64_t wsum[kB / 64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  uint64_t inc = x;
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint64_t y = __shfl_up(inc, o, 64);
+    if (lane >= o) inc += y;
+  }
+  if (lane == 63) wsum[wave] = inc;
+  __syncthreads();
+  uint64_t wbase = 0, t = 0;
+  for (int i = 0; i < kB / 64; i++) {
+    const uint64_t w = wsum[i];
+    if (i < wave) wbase += w;
+    t += w;
+  }
+  __syncthreads();  // wsum is reused by the next call
+  *total = t;
+  return wbase + inc - x;
 }
-__global__ void k_write_total_u64(const uint64_t* in, uint64_t* out, uint64_t n) {
-  out[n] = n ? out[n - 1] + in[n - 1] : 0;
+
+template <typename Src>
+__global__ __launch_bounds__(kB) void k_tile_sums(Src src, uint64_t n, uint64_t* sums) {
+  const uint64_t base = uint64_t(blockIdx.x) * kTile;
+  uint64_t x = 0;
+  for (int r = 0; r < kRounds; r++) {
+    const uint64_t i = base + uint64_t(r) * kB + threadIdx.x;
+    if (i < n) x += src(i);
+  }
+  uint64_t tot;
+  (void)block_exclusive_scan(x, &tot);
+  if (threadIdx.x == 0) sums[blockIdx.x] = tot;
+}
+
+// out[i] = prefix[tile] + exclusive prefix of src within the tile; the last
+// tile also writes out[n] = total.
+template <typename Src>
+__global__ __launch_bounds__(kB) void k_tile_scan(Src src, uint64_t n, const uint64_t* prefix,
+                                                  uint64_t* out) {
+  uint64_t carry = prefix ? prefix[blockIdx.x] : 0;
+  const uint64_t base = uint64_t(blockIdx.x) * kTile;
+  for (int r = 0; r < kRounds; r++) {
+    if (base + uint64_t(r) * kB >= n) break;  // uniform over the workgroup
+    const uint64_t i = base + uint64_t(r) * kB + threadIdx.x;
+    const uint64_t x = i < n ? src(i) : 0;
+    uint64_t tot;
+    const uint64_t ex = block_exclusive_scan(x, &tot);
+    if (i < n) out[i] = carry + ex;
+    carry += tot;
+  }
+  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) out[n] = carry;
+}
+
+uint64_t ntiles_of(uint64_t n) { return (n + kTile - 1) / kTile; }
+
+template <typename Src>
+void scan_rec(Src src, uint64_t n, uint64_t* out, uint64_t* tmp, hipStream_t s) {
+  if (n <= kTile) {
+    k_tile_scan<<<1, kB, 0, s>>>(src, n, nullptr, out);
+    LLG_HIP(hipGetLastError());
+    return;
+  }
+  const uint64_t nt = ntiles_of(n);
+  uint64_t* sums = tmp;
+  uint64_t* pre = tmp + nt;  // nt + 1 entries
+  k_tile_sums<<<unsigned(nt), kB, 0, s>>>(src, n, sums);
+  LLG_HIP(hipGetLastError());
+  scan_rec(SrcU64{sums}, nt, pre, tmp + 2 * nt + 1, s);
+  k_tile_scan<<<unsigned(nt), kB, 0, s>>>(src, n, pre, out);
+  LLG_HIP(hipGetLastError());
+}
+
+template <typename Src>
+void scan_alloc(Src src, uint64_t n, uint64_t* out, hipStream_t s) {
+  DevBuf t;
+  const size_t b = scan_tmp_bytes(n);
+  if (b) t.alloc(b, s);
+  scan_rec(src, n, out, t.as<uint64_t>(), s);
+}
+
+// --- radix sort ----------------------------------------------------------------
+
+// Counts of every digit position (P = sizeof(K) bytes) over all keys.
+template <typename K>
+__global__ __launch_bounds__(kB) void k_rs_all_hist(const K* keys, uint64_t n,
+                                                    unsigned long long* ghist) {
+  constexpr int P = sizeof(K);
+  __shared__ uint32_t h[P][256];
+  for (int p = 0; p < P; p++) h[p][threadIdx.x] = 0;
+  __syncthreads();
+  for (uint64_t i = uint64_t(blockIdx.x) * kB + threadIdx.x; i < n;
+       i += uint64_t(gridDim.x) * kB) {
+    const K k = keys[i];
+#pragma unroll
+    for (int p = 0; p < P; p++) atomicAdd(&h[p][uint32_t(k >> (8 * p)) & 255u], 1u);
+  }
+  __syncthreads();
+  for (int p = 0; p < P; p++)
+    if (h[p][threadIdx.x]) atomicAdd(&ghist[p * 256 + threadIdx.x], (unsigned long long)h[p][threadIdx.x]);
+}
+
+// Per-tile digit counts, digit-major: th[d * ntiles + tile].
+template <typename K>
+__global__ __launch_bounds__(kB) void k_rs_tile_hist(const K* keys, uint64_t n, int shift,
+                                                     uint32_t* th, uint64_t ntiles) {
+  __shared__ uint32_t h[256];
+  h[threadIdx.x] = 0;
+  __syncthreads();
+  const uint64_t base = uint64_t(blockIdx.x) * kTile;
+  for (int r = 0; r < kRounds; r++) {
+    const uint64_t i = base + uint64_t(r) * kB + threadIdx.x;
+    if (i < n) atomicAdd(&h[uint32_t(keys[i] >> shift) & 255u], 1u);
+  }
+  __syncthreads();
+  th[uint64_t(threadIdx.x) * ntiles + blockIdx.x] = h[threadIdx.x];
+}
+
+template <typename K, bool kVals>
+__global__ __launch_bounds__(kB) void k_rs_scatter(const K* kin, const uint32_t* vin, K* kout,
+                                                   uint32_t* vout, uint64_t n, int shift,
+                                                   const uint64_t* tofs, uint64_t ntiles) {
+  __shared__ uint64_t run[256];
+  __shared__ uint32_t wcnt[kB / 64][256];
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  run[t] = tofs[uint64_t(t) * ntiles + blockIdx.x];
+  for (int w = 0; w < kB / 64; w++) wcnt[w][t] = 0;
+  __syncthreads();
+  const uint64_t base = uint64_t(blockIdx.x) * kTile;
+  for (int r = 0; r < kRounds; r++) {
+    if (base + uint64_t(r) * kB >= n) break;  // uniform over the workgroup
+    const uint64_t i = base + uint64_t(r) * kB + t;
+    const bool valid = i < n;
+    const K key = valid ? kin[i] : K(0);
+    uint32_t val = 0;
+    if (kVals && valid) val = vin[i];
+    const uint32_t d = uint32_t(key >> shift) & 255u;
+    uint64_t peers = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; b++) {
+      const uint64_t m = __ballot((d >> b) & 1u);
+      peers &= ((d >> b) & 1u) ? m : ~m;
+    }
+    const uint64_t below = peers & ((uint64_t(1) << lane) - 1);
+    if (valid && below == 0) wcnt[wave][d] = uint32_t(__popcll(peers));
+    __syncthreads();
+    if (valid) {
+      uint64_t pos = run[d] + uint64_t(__popcll(below));
+      for (int w = 0; w < wave; w++) pos += wcnt[w][d];
+      kout[pos] = key;
+      if (kVals) vout[pos] = val;
+    }
+    __syncthreads();
+    uint32_t add = 0;
+    for (int w = 0; w < kB / 64; w++) {
+      add += wcnt[w][t];
+      wcnt[w][t] = 0;
+    }
+    run[t] += add;
+    __syncthreads();
+  }
+}
+
+template <typename K, bool kVals>
+void radix_sort(const K* kin, K* kout, const uint32_t* vin, uint32_t* vout, uint64_t n,
+                int end_bit, hipStream_t s) {
+  if (n == 0) return;
+  constexpr int P = sizeof(K);
+  const int npass_max = std::min(P, (end_bit + 7) / 8);
+  // which passes move anything
+  DevBuf gh;
+  gh.alloc(P * 256 * sizeof(unsigned long long), s);
+  LLG_HIP(hipMemsetAsync(gh.p, 0, gh.bytes, s));
+  {
+    int dev = 0, cus = 0;
+    LLG_HIP(hipGetDevice(&dev));
+    LLG_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    const uint64_t g = std::min<uint64_t>((n + kB - 1) / kB, uint64_t(std::max(cus, 1)) * 8);
+    k_rs_all_hist<K><<<unsigned(g), kB, 0, s>>>(kin, n, gh.as<unsigned long long>());
+    LLG_HIP(hipGetLastError());
+  }
+  std::vector<unsigned long long> h(P * 256);
+  LLG_HIP(hipMemcpyAsync(h.data(), gh.p, gh.bytes, hipMemcpyDeviceToHost, s));
+  LLG_HIP(hipStreamSynchronize(s));
+  std::vector<int> passes;
+  for (int p = 0; p < npass_max; p++) {
+    bool trivial = false;
+    for (int d = 0; d < 256; d++)
+      if (h[p * 256 + d] == n) trivial = true;
+    if (!trivial) passes.push_back(p);
+  }
+  const uint64_t nt = ntiles_of(n);
+  if (passes.empty()) {
+    LLG_HIP(hipMemcpyAsync(kout, kin, n * sizeof(K), hipMemcpyDeviceToDevice, s));
+    if (kVals) LLG_HIP(hipMemcpyAsync(vout, vin, n * 4, hipMemcpyDeviceToDevice, s));
+    return;
+  }
+  DevBuf th, tofs, kt, vt, stmp;
+  th.alloc(256 * nt * 4, s);
+  tofs.alloc((256 * nt + 1) * 8, s);
+  const size_t sb = scan_tmp_bytes(256 * nt);
+  if (sb) stmp.alloc(sb, s);
+  if (passes.size() > 1) {
+    kt.alloc(n * sizeof(K), s);
+    if (kVals) vt.alloc(n * 4, s);
+  }
+  const int np = int(passes.size());
+  const K* ksrc = kin;
+  const uint32_t* vsrc = vin;
+  for (int j = 0; j < np; j++) {
+    // the last pass lands in the output; earlier ones alternate so that it does
+    const bool to_out = ((np - 1 - j) % 2) == 0;
+    K* kdst = to_out ? kout : kt.as<K>();
+    uint32_t* vdst = to_out ? vout : vt.as<uint32_t>();
+    const int shift = 8 * passes[j];
+    k_rs_tile_hist<K><<<unsigned(nt), kB, 0, s>>>(ksrc, n, shift, th.as<uint32_t>(), nt);
+    LLG_HIP(hipGetLastError());
+    scan_rec(SrcU32{th.as<uint32_t>()}, 256 * nt, tofs.as<uint64_t>(), stmp.as<uint64_t>(), s);
+    k_rs_scatter<K, kVals><<<unsigned(nt), kB, 0, s>>>(ksrc, vsrc, kdst, vdst, n, shift,
+                                                       tofs.as<uint64_t>(), nt);
+    LLG_HIP(hipGetLastError());
+    ksrc = kdst;
+    vsrc = vdst;
+  }
+}
+
+__global__ void k_unique_scatter(const uint64_t* in, const uint64_t* pos, uint64_t n,
+                                 uint64_t* out) {
+  const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n && (i == 0 || in[i] != in[i - 1])) out[pos[i]] = in[i];
 }
 
 }  // namespace
 
-void scan_u32_to_u64(const uint32_t* in, uint64_t* out, uint64_t n, hipStream_t s) {
-  if (n > 0) {
-    hipcub::TransformInputIterator<uint64_t, WidenU32, const uint32_t*> it(in, WidenU32());
-    size_t tmp = 0;
-    LLG_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, it, out, n, s));
-    DevBuf t;
-    t.alloc(tmp, s);
-    LLG_HIP(hipcub::DeviceScan::ExclusiveSum(t.p, tmp, it, out, n, s));
+size_t scan_tmp_bytes(uint64_t n) {
+  size_t w = 0;
+  while (n > kTile) {
+    const uint64_t nt = ntiles_of(n);
+    w += 2 * nt + 1;
+    n = nt;
   }
-  k_write_total_u32<<<1, 1, 0, s>>>(in, out, n);
-  LLG_HIP(hipGetLastError());
+  return w * 8;
+}
+
+void scan_u32_to_u64_ws(const uint32_t* in, uint64_t* out, uint64_t n, void* tmp,
+                        hipStream_t s) {
+  scan_rec(SrcU32{in}, n, out, static_cast<uint64_t*>(tmp), s);
+}
+
+void scan_u32_to_u64(const uint32_t* in, uint64_t* out, uint64_t n, hipStream_t s) {
+  scan_alloc(SrcU32{in}, n, out, s);
 }
 
 void scan_u64(const uint64_t* in, uint64_t* out, uint64_t n, hipStream_t s) {
-  if (n > 0) {
-    size_t tmp = 0;
-    LLG_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, in, out, n, s));
-    DevBuf t;
-    t.alloc(tmp, s);
-    LLG_HIP(hipcub::DeviceScan::ExclusiveSum(t.p, tmp, in, out, n, s));
-  }
-  k_write_total_u64<<<1, 1, 0, s>>>(in, out, n);
-  LLG_HIP(hipGetLastError());
+  scan_alloc(SrcU64{in}, n, out, s);
 }
 
 void sort_pairs_u64_u32(const uint64_t* keys_in, uint64_t* keys_out, const uint32_t* vals_in,
                         uint32_t* vals_out, uint64_t n, int end_bit, hipStream_t s) {
-  if (n == 0) return;
-  size_t tmp = 0;
-  LLG_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, keys_in, keys_out, vals_in, vals_out,
-                                             n, 0, end_bit, s));
-  DevBuf t;
-  t.alloc(tmp, s);
-  LLG_HIP(hipcub::DeviceRadixSort::SortPairs(t.p, tmp, keys_in, keys_out, vals_in, vals_out, n,
-                                             0, end_bit, s));
+  radix_sort<uint64_t, true>(keys_in, keys_out, vals_in, vals_out, n, end_bit, s);
 }
 
 void sort_pairs_u32_u32(const uint32_t* keys_in, uint32_t* keys_out, const uint32_t* vals_in,
                         uint32_t* vals_out, uint64_t n, int end_bit, hipStream_t s) {
-  if (n == 0) return;
-  size_t tmp = 0;
-  LLG_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, keys_in, keys_out, vals_in, vals_out,
-                                             n, 0, end_bit, s));
-  DevBuf t;
-  t.alloc(tmp, s);
-  LLG_HIP(hipcub::DeviceRadixSort::SortPairs(t.p, tmp, keys_in, keys_out, vals_in, vals_out, n,
-                                             0, end_bit, s));
+  radix_sort<uint32_t, true>(keys_in, keys_out, vals_in, vals_out, n, end_bit, s);
 }
 
 void sort_keys_u64(const uint64_t* keys_in, uint64_t* keys_out, uint64_t n, int end_bit,
                    hipStream_t s) {
-  if (n == 0) return;
-  size_t tmp = 0;
-  LLG_HIP(hipcub::DeviceRadixSort::SortKeys(nullptr, tmp, keys_in, keys_out, n, 0, end_bit, s));
-  DevBuf t;
-  t.alloc(tmp, s);
-  LLG_HIP(hipcub::DeviceRadixSort::SortKeys(t.p, tmp, keys_in, keys_out, n, 0, end_bit, s));
+  radix_sort<uint64_t, false>(keys_in, keys_out, nullptr, nullptr, n, end_bit, s);
 }
 
 uint64_t unique_u64(const uint64_t* in, uint64_t* out, uint64_t n, hipStream_t s) {
   if (n == 0) return 0;
-  DevBuf cnt;
-  cnt.alloc(sizeof(uint64_t), s);
-  size_t tmp = 0;
-  LLG_HIP(hipcub::DeviceSelect::Unique(nullptr, tmp, in, out, cnt.as<uint64_t>(), n, s));
-  DevBuf t;
-  t.alloc(tmp, s);
-  LLG_HIP(hipcub::DeviceSelect::Unique(t.p, tmp, in, out, cnt.as<uint64_t>(), n, s));
+  DevBuf pos;
+  pos.alloc((n + 1) * 8, s);
+  scan_alloc(SrcRunStart{in}, n, pos.as<uint64_t>(), s);
+  k_unique_scatter<<<grid_for(n, 256), 256, 0, s>>>(in, pos.as<uint64_t>(), n, out);
+  LLG_HIP(hipGetLastError());
   uint64_t h = 0;
-  LLG_HIP(hipMemcpyAsync(&h, cnt.p, sizeof h, hipMemcpyDeviceToHost, s));
+  LLG_HIP(hipMemcpyAsync(&h, pos.as<uint64_t>() + n, sizeof h, hipMemcpyDeviceToHost, s));
   LLG_HIP(hipStreamSynchronize(s));
   return h;
 }

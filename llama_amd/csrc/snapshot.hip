@@ -322,6 +322,7 @@ const Csr& ensure_csr(ll_gpu_graph* g, uint32_t dir, int level) {
   Csr& c = g->csr[dir];
   if (c.valid && c.level == level) return c;
   c = Csr();
+  g->rb[dir].clear();  // row blocks describe the previous level's rows
   ll_gpu_ctx* ctx = g->ctx;
   hipStream_t s = ctx->stream;
   const uint64_t V = g->V;
