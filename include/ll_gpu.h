@@ -198,6 +198,20 @@ int ll_gpu_bc(ll_gpu_graph* g, int level, const int64_t* sources, int64_t n, flo
  * edges_examined = edges the direction-optimising traversal inspected. */
 int ll_gpu_bfs(ll_gpu_graph* g, int level, int64_t root, int32_t* levels_out, uint64_t* reached,
                uint64_t* edges_examined);
+/* No counterpart in the reference: ll_gpu_bfs_ex for callers that cannot
+ * allocate device memory (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc).  The search works in a fixed 48 MiB
+ * device array of the library (one per GPU, held until the process ends) and
+ * calls no allocating routine.  levels_dev is a DEVICE pointer (int32 per
+ * vertex, max_nodes entries, INT_MAX-1 for unreached) on the context's
+ * device; reached and edges_examined (may be NULL) as in ll_gpu_bfs.
+ * LL_GPU_ESTATE until ll_gpu_bfs has run once on `level` (it builds the
+ * snapshot structures this call reads); LL_GPU_ERANGE when the working
+ * memory of max_nodes vertices (about 20.4 bytes each) exceeds the array:
+ * above about 2.4 million vertices.  All calls on one GPU share the array:
+ * calls that overlap on one GPU (two contexts or threads) race. */
+int ll_gpu_bfs_noalloc(ll_gpu_graph* g, int level, int64_t root, int32_t* levels_dev,
+                       uint64_t* reached, uint64_t* edges_examined);
 /* Multi-source BFS, sources in lanes of lane_width bits (64, or 0 = autotune
  * among 16/32/64/128).  levels_out (may be NULL) is n x max_nodes int32,
  * row-major by source; reached_out[n] (may be NULL) per-source reached counts. */

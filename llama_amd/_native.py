@@ -65,6 +65,7 @@ GPU_SIGNATURES = {
     "ll_gpu_bc": (C.c_int, [vp, C.c_int, P(i64), i64, P(f32)]),
     "ll_gpu_bfs": (C.c_int, [vp, C.c_int, i64, P(i32), P(u64), P(u64)]),
     "ll_gpu_bfs_ex": (C.c_int, [vp, C.c_int, i64, P(i32), P(u64), P(u64), P(u64)]),
+    "ll_gpu_bfs_noalloc": (C.c_int, [vp, C.c_int, i64, vp, P(u64), P(u64)]),
     "ll_gpu_bfs_batch_ex": (C.c_int, [vp, C.c_int, P(i64), C.c_int, C.c_int, P(i32), P(u64),
                                       P(u64)]),
     "ll_gpu_tc_ex": (C.c_int, [vp, C.c_int, P(u64), P(u64), P(u64)]),

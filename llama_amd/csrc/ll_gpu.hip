@@ -167,6 +167,7 @@ int ll_gpu_create(int device, ll_gpu_ctx** out) {
     LLG_CHECK(device >= 0 && device < n, LL_GPU_EINVAL, "no such device");
     LLG_HIP(hipSetDevice(device));
     c->device = device;
+    c->scratch = llgpu::scratch_setup(device);
     hipDeviceProp_t prop;
     LLG_HIP(hipGetDeviceProperties(&prop, device));
     c->num_cus = prop.multiProcessorCount;

@@ -133,6 +133,13 @@ struct KernelTimer {
 
 constexpr size_t kReadbackBytes = size_t(64) << 10;
 
+namespace llgpu {
+// Size of the fixed device array ll_gpu_bfs_noalloc works in (one per GPU).
+constexpr size_t kScratchBytes = size_t(48) << 20;
+// Address of that array on `device` (the current device), looked up once.
+void* scratch_setup(int device);
+}  // namespace llgpu
+
 struct ll_gpu_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -145,6 +152,7 @@ struct ll_gpu_ctx {
   void* readback = nullptr;    // small pinned buffer for per-level counter reads
   std::map<std::string, int> tuned;    // autotuner decisions
   std::map<std::string, long> options; // ll_gpu_set_option knobs
+  void* scratch = nullptr;     // device address of the library's fixed array on `device`
 };
 
 namespace llgpu {
