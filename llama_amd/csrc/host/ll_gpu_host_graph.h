@@ -44,3 +44,14 @@ inline bool ll_gpu_weights(const llhost::Graph& G, int level, const float** w, u
   *n = lv.wt.size();
   return true;
 }
+// bc_random's sources on the host layout: a seeded splitmix64 stream (the
+// reference draws them with pick_random_node from libc rand(),
+// ll_mlcsr_graph.h:677-682; the reference-side overload in
+// ll_gpu_reference.h calls that function, so its picks are the reference's).
+inline int64_t ll_gpu_pick_source(const llhost::Graph& G, uint64_t seed, int i) {
+  uint64_t z = seed * 0x9e3779b97f4a7c15ull + uint64_t(i) + 0x9e3779b97f4a7c15ull;
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;  // splitmix64 finaliser (Steele, Lea,
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;  // Flood 2014)
+  z ^= z >> 31;
+  return G.num_nodes() ? int64_t(z % G.num_nodes()) : 0;
+}

@@ -13,6 +13,8 @@
 //                            std::vector<uint32_t>& maxlev)
 //   bool      ll_gpu_weights(const Graph&, level, const float** w, uint64_t* n)
 //             (the level's `weight` property by out-edge index; false if none)
+//   int64_t   ll_gpu_pick_source(const Graph&, uint64_t seed, int i)
+//             (bc_random's i-th source; the reference graph's pick_random_node)
 // (overloads for the host layout are in ll_gpu_host_graph.h; those for
 // ll_mlcsr_ro_graph are a few lines over its public accessors).  The mirror is shared by all tasks of
 // one graph and refreshed with each newly frozen level before run().
@@ -248,9 +250,10 @@ class gpu_sssp_weighted : public ll_gpu_task<Graph> {
 };
 
 // bc_adj (bc_adj.h:124-181) / bc_random (bc_random.h:158-193): FP32
-// betweenness centrality from every vertex, or from K sources picked with a
-// seeded generator (the reference's pick_random_node draws from its own
-// generator, so the picks differ); finalize = the largest value (:201-209).
+// betweenness centrality from every vertex, or from K sources given by the
+// graph's ll_gpu_pick_source (on the reference graph its pick_random_node,
+// bc_random.h:177, so the sources are the reference task's); finalize = the
+// largest value (:201-209).
 template <class Graph>
 class gpu_bc : public ll_gpu_task<Graph> {
  public:
@@ -262,8 +265,7 @@ class gpu_bc : public ll_gpu_task<Graph> {
     const uint64_t V = ll_gpu_max_nodes(*G);
     bc_.assign(V, 0.0f);
     sources_.clear();
-    for (int i = 0; i < k_ && V > 0; i++)
-      sources_.push_back(int64_t(mix64(seed_ * 0x9e3779b97f4a7c15ull + uint64_t(i)) % V));
+    for (int i = 0; i < k_ && V > 0; i++) sources_.push_back(ll_gpu_pick_source(*G, seed_, i));
   }
   double run() override {
     this->m_->check(ll_gpu_bc(this->m_->graph(), -1, k_ < 0 ? nullptr : sources_.data(),
@@ -284,13 +286,6 @@ class gpu_bc : public ll_gpu_task<Graph> {
   const std::vector<int64_t>& sources() const { return sources_; }
 
  private:
-  // splitmix64 finaliser (Steele, Lea, Flood 2014)
-  static uint64_t mix64(uint64_t z) {
-    z += 0x9e3779b97f4a7c15ull;
-    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
-    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
-    return z ^ (z >> 31);
-  }
   int k_;
   uint64_t seed_ = 1;
   std::vector<int64_t> sources_;

@@ -17,10 +17,15 @@
 //                    "tarjan_scc" | "tc_u" | "tc_od"
 // Output (stdout, text): "VT dir level k" blocks of "start len degree" lines,
 // "ET dir level n" blocks of words, then per task "TASK name arg", its
-// print_results text, and "RETURN value".
+// print_results text, and "RETURN value".  With LL_DUMP_EACH=1 in the
+// environment the tables are dumped right after each level's freeze instead
+// ("FREEZE k", the new level's vertex tables and every level's edge tables as
+// they stand then) -- the state a device push at that freeze reads, which is
+// what differs between the vertex-table containers (LL_FLAT_VT, LL_ONE_VT).
 #include <llama.h>
 
 #include <cstdio>
+#include <cstdlib>
 #include <unistd.h>
 #include <cstring>
 #include <string>
@@ -40,6 +45,26 @@ static void run_task(Task& t, ll_mlcsr_ro_graph& G, const char* name, long arg) 
   double f = t.finalize();
   t.print_results(stdout);
   printf("RETURN %.17g %.17g\n", r, f);
+}
+
+static void dump_vt(ll_mlcsr_ro_graph& G, int dir, int l) {
+  auto& csr = dir == 0 ? G.out() : G.in();
+  auto* vt = csr.vertex_table(l);
+  size_t n = vt->size();
+  printf("VT %d %d %zu\n", dir, l, n + 1);
+  for (size_t v = 0; v <= n; v++) {
+    const ll_mlcsr_core__begin_t& e = (*vt)[v];
+    printf("%lld %u %u\n", (long long)e.adj_list_start, (unsigned)e.level_length,
+           (unsigned)e.degree);
+  }
+}
+
+static void dump_et(ll_mlcsr_ro_graph& G, int dir, int l) {
+  auto& csr = dir == 0 ? G.out() : G.in();
+  auto* et = csr.edge_table(l);
+  size_t w = csr.max_edges(l);
+  printf("ET %d %d %zu\n", dir, l, w);
+  for (size_t i = 0; i < w; i++) printf("%llu\n", (unsigned long long)(*et)[i]);
 }
 
 static void dump(ll_mlcsr_ro_graph& G) {
@@ -64,6 +89,8 @@ static void dump(ll_mlcsr_ro_graph& G) {
 
 int main() {
   long V = 0, L = 0;
+  const char* each_env = getenv("LL_DUMP_EACH");
+  const bool each = each_env && each_env[0] == '1';
   if (scanf("%ld %ld", &V, &L) != 2) return 2;
   ll_database db;
   ll_writable_graph& W = *db.graph();
@@ -92,10 +119,18 @@ int main() {
     for (auto& p : ins) W.add_edge(p.first, p.second);
     W.tx_commit();
     W.checkpoint();
+    if (each) {
+      ll_mlcsr_ro_graph& R = W.ro_graph();
+      printf("FREEZE %ld\n", k);
+      for (int dir = 0; dir < 2; dir++) {
+        dump_vt(R, dir, int(k));
+        for (int l = 0; l <= int(k); l++) dump_et(R, dir, l);
+      }
+    }
   }
   ll_mlcsr_ro_graph& G = W.ro_graph();
   printf("LEVELS %d NODES %ld\n", G.num_levels(), (long)G.max_nodes());
-  dump(G);
+  if (!each) dump(G);
   char task[64];
   while (scanf("%63s", task) == 1) {
     long arg = 0;

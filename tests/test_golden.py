@@ -19,7 +19,8 @@ from llama_amd.graph import DIR_IN, DIR_OUT, HostGraph
 
 GOLDEN = Path(__file__).resolve().parent / "golden"
 FIXTURES = sorted(p.stem for p in GOLDEN.glob("*.json")
-                  if not p.stem.startswith(("sssp_", "net_", "bc_", "slcsr_", "tci_")))
+                  if not p.stem.startswith(("sssp_", "net_", "bc_", "slcsr_", "tci_", "tc_",
+                                            "flat_", "onevt_")))
 
 
 def load(name):
