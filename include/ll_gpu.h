@@ -291,7 +291,14 @@ int ll_gpu_profile_get(ll_gpu_ctx* ctx, const char* kernel, uint64_t* launches,
                        double* total_ms);
 /* Autotuner decision whose key starts with `prefix` ("ppr:", "bfs_lanes:"). */
 int ll_gpu_tuned(ll_gpu_ctx* ctx, const char* prefix, int* value);
-/* Tuning knobs: "pr_reorder" (1 = gather PageRank contributions in
+/* Tuning knobs: "pr_bin_bits" (PageRank as the binned pull over the
+ * out-CSR with destination bins of 2^N vertices, N in 12..14; 0 = the
+ * row-block pull over the in-CSR, which the knobs below tune; -1 = automatic
+ * = 13 while the snapshot has fewer than 2^32 live edges, else 0),
+ * "bfs_reach" (1 = single-source searches whose root reaches only the
+ * snapshot's reach set -- the vertices reachable from its highest-degree
+ * vertex -- treat the other rows as visited; 0 = dead rows only; default 1),
+ * "pr_reorder" (1 = gather PageRank contributions in
  * descending-out-degree slot order; default), "pr_cap" (default row-block
  * capacity, 2048), "pr_seg_bits" (pull over column segments of 2^N
  * contribution slots; 0 = off, -1 = automatic = off), "pr_item_sort" (sort

@@ -6,8 +6,11 @@ Native pieces (built in-tree by ``python -m llama_amd.build``):
 
 Python pieces:
   graph     HostGraph / DeviceGraph / GpuContext (ctypes over the C ABIs)
-  tasks     benchmark-task registry mirroring benchmark/benchmark.cc
   autograd  torch.autograd.Function for weighted PageRank (forward + adjoint)
+
+The benchmark-task registry (the reference's task interface) is C++:
+csrc/host/ll_gpu_tasks.h, with the CLI csrc/host/ll_gpu_bench.cc and the
+binding into the reference's own driver, csrc/host/ll_gpu_reference.h.
 """
 from .graph import (DIR_IN, DIR_OUT, UNREACHED, DeviceGraph, Edges, GpuContext, GpuError,
                     HostGraph, build_multilevel)

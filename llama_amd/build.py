@@ -40,7 +40,7 @@ HOST_FLAGS = ["-O3", "-std=c++17", "-fopenmp", "-fPIC", "-Wall", "-Wextra", "-DN
               f"-I{INCLUDE}"]
 
 HIP_SOURCES = ["ll_gpu.hip", "prim.hip", "snapshot.hip", "rowblocks.hip", "pagerank.hip",
-               "pagerank_push.hip", "pr_segments.hip",
+               "pagerank_push.hip", "pr_segments.hip", "pr_bins.hip",
                "bfs.hip", "bfs_batch.hip", "tc.hip", "scc.hip", "ppr.hip", "wpr.hip",
                "sssp.hip", "bc.hip", "noalloc.hip"]
 HOST_SOURCES = ["host/ll_layout.cc", "host/ll_host.cc"]

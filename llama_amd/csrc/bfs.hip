@@ -9,13 +9,20 @@
 //     degrees and processed edge-parallel (2048 edges per workgroup; each edge
 //     finds its frontier vertex by binary search in an LDS copy of the
 //     workgroup's slice of the scan), claiming vertices with an atomic OR on
-//     the visited bitmap and appending them with one wave-aggregated atomic;
-//   * bottom-up over row blocks of the in-edges: workgroups whose rows are all
-//     visited return at once; the others read their neighbour IDs coalesced and
-//     pack "neighbour in the frontier bitmap" into 64-bit words with one wave
-//     ballot per 64 edges (the V/8-byte bitmap stays in L2), and every
-//     unvisited row tests its bit range; new vertices are set in the next
-//     frontier bitmap with an atomic OR whose old value keeps counts exact.
+//     the visited bitmap; each workgroup stages its new vertices in LDS and
+//     reserves their queue slots with one atomic;
+//   * bottom-up over the row blocks of the in-edges that still hold an open
+//     row (unvisited and not excluded): each level's kernel flags the blocks
+//     that stay open, and a scan compacts them into the next level's list;
+//     a block reads its neighbour IDs coalesced and packs "neighbour in the
+//     frontier bitmap" into 64-bit words with one wave ballot per 64 edges
+//     (the V/8-byte bitmap stays in L2), and every unvisited row tests its
+//     bit range; new vertices are set in the next frontier bitmap with an
+//     atomic OR whose old value keeps counts exact.
+// Rows no search from the root can enter start as visited: rows without a
+// live in-edge always, and, when the root's successors all lie in the
+// snapshot's reach set (the vertices reachable from its highest-degree
+// vertex, closed under successors), every row outside that set.
 // The switch follows Beamer et al. (SC'12): top-down -> bottom-up when the
 // frontier's out-edges exceed the unexplored edges / alpha, back when the
 // frontier shrinks below V / beta (options "bfs_alpha", "bfs_beta"; defaults
@@ -45,14 +52,19 @@ struct alignas(128) Counters {
   unsigned long long n_next;     // vertices discovered this level
   unsigned long long m_next;     // their out-edges
   unsigned long long examined;   // edges inspected this level
+  unsigned long long aux;        // stripe 0: 1 when the search excludes the reach complement
 };
 static_assert(sizeof(Counters) == 128, "one stripe per 128-byte line");
 
 __device__ inline Counters* stripe(Counters* c) { return c + (blockIdx.x & (kStripes - 1)); }
 
-__global__ void k_bfs_init(int32_t* lv, uint64_t* visited, const uint64_t* excluded, uint64_t V,
-                           uint64_t words) {
+// `excluded` (may be null) marks rows no search from this root enters; when
+// `alt` is given and the root's check set use_alt (Counters::aux of stripe
+// 0), `alt` is used instead.
+__global__ void k_bfs_init(int32_t* lv, uint64_t* visited, const uint64_t* excluded,
+                           const uint64_t* alt, const Counters* c, uint64_t V, uint64_t words) {
   uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (alt && c->aux) excluded = alt;
   if (lv && i < V) lv[i] = kUnreached;
   if (i < words) {
     // bits beyond V (and excluded vertices) count as visited, so no step claims them
@@ -84,6 +96,25 @@ __global__ void k_zero_counters(Counters* c) {  // <<<1, kStripes>>>
   c->n_next = 0;
   c->m_next = 0;
   c->examined = 0;
+}
+
+// Counters::aux of stripe 0 = 1 when every vertex the search from `root` can
+// reach lies in the reach set R (the complement bitmap `notr` clear): the
+// root is in R, or all its out-neighbours are (R is closed under successors).
+// One workgroup.
+__global__ void k_reach_check(uint32_t root, const uint64_t* notr, const uint64_t* ofs,
+                              const uint32_t* adj, Counters* c) {
+  __shared__ int bad;
+  if (threadIdx.x == 0) bad = 0;
+  __syncthreads();
+  const bool in_r = !((notr[root >> 6] >> (root & 63)) & 1);
+  if (!in_r)
+    for (uint64_t e = ofs[root] + threadIdx.x; e < ofs[root + 1]; e += blockDim.x) {
+      const uint32_t w = adj[e];
+      if ((notr[w >> 6] >> (w & 63)) & 1) bad = 1;
+    }
+  __syncthreads();
+  if (threadIdx.x == 0) c->aux = (in_r || !bad) ? 1ull : 0ull;
 }
 
 __global__ void k_queue_degrees(const uint32_t* q, uint64_t n, const uint32_t* outdeg,
@@ -200,7 +231,11 @@ __device__ inline uint32_t block_sum_u32(uint32_t x) {
   return t;
 }
 
-__global__ __launch_bounds__(kBlock) void k_bu_rows(const RowItem* items, const uint64_t* ofs,
+// `list` (null = every item) holds the row blocks that still had an open row
+// (unvisited, not excluded) after the previous bottom-up level; open[i] is
+// set for list entry i when its item keeps an open row after this level.
+__global__ __launch_bounds__(kBlock) void k_bu_rows(const RowItem* items, const uint32_t* list,
+                                                    uint32_t* open, const uint64_t* ofs,
                                                     const uint32_t* adj, const uint32_t* outdeg,
                                                     const uint64_t* fbm, const uint64_t* visited,
                                                     uint64_t* nbm, int32_t* lv, int32_t next_lv,
@@ -209,14 +244,17 @@ __global__ __launch_bounds__(kBlock) void k_bu_rows(const RowItem* items, const 
   __shared__ uint64_t rofs[kRowsPerItem + 1];
   __shared__ uint64_t bits[kMaxEdges / 64];
   __shared__ unsigned long long red[3][kBlock / 64];
-  const RowItem it = items[blockIdx.x];
+  const RowItem it = items[list ? list[blockIdx.x] : blockIdx.x];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   unsigned long long n_new = 0, m_new = 0, exam = 0;
   if (it.npieces == 0) {
     const uint32_t nrows = it.row_end - it.row_begin;
     const uint32_t r = it.row_begin + threadIdx.x;
     const bool unv = threadIdx.x < nrows && !((visited[r >> 6] >> (r & 63)) & 1);
-    if (!__syncthreads_or(unv)) return;
+    if (!__syncthreads_or(unv)) {
+      if (threadIdx.x == 0) open[blockIdx.x] = 0;
+      return;
+    }
     for (uint32_t i = threadIdx.x; i <= nrows; i += kBlock) rofs[i] = ofs[it.row_begin + i];
     __syncthreads();
     const uint64_t e0 = rofs[0];
@@ -290,6 +328,8 @@ __global__ __launch_bounds__(kBlock) void k_bu_rows(const RowItem* items, const 
       n_new = 1;  // a stream row belongs to this item only: no other writer sets its bit
       m_new = outdeg[r];
     }
+    const bool still = __syncthreads_or(unv && !found);
+    if (threadIdx.x == 0) open[blockIdx.x] = still ? 1u : 0u;
     // the wave's rows are consecutive: its found bits land in at most two
     // bitmap words, set with one atomic each
     const uint64_t fm = __ballot(found);
@@ -302,7 +342,10 @@ __global__ __launch_bounds__(kBlock) void k_bu_rows(const RowItem* items, const 
     }
   } else {
     const uint32_t r = it.row_begin;
-    if ((visited[r >> 6] >> (r & 63)) & 1) return;
+    if ((visited[r >> 6] >> (r & 63)) & 1) {
+      if (threadIdx.x == 0) open[blockIdx.x] = 0;
+      return;
+    }
     const uint64_t lo = ofs[r] + uint64_t(it.piece) * kPiece;
     const uint64_t hi = min(ofs[r + 1], lo + kPiece);
     bool f = false;
@@ -312,6 +355,7 @@ __global__ __launch_bounds__(kBlock) void k_bu_rows(const RowItem* items, const 
     }
     const bool found = __syncthreads_or(f);
     if (threadIdx.x == 0) {
+      open[blockIdx.x] = found ? 0u : 1u;
       exam = hi - lo;
       if (found) {
         if (lv) lv[r] = next_lv;
@@ -350,6 +394,53 @@ __global__ __launch_bounds__(kBlock) void k_bu_rows(const RowItem* items, const 
 __global__ void k_visited_or(uint64_t* visited, const uint64_t* nbm, uint64_t words) {
   const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i < words) visited[i] |= nbm[i];
+}
+
+// Next bottom-up list: the entries of `list` (null = identity) whose open flag
+// is set, in order (pos = exclusive scan of the flags).
+__global__ void k_list_compact(const uint32_t* list, const uint32_t* open, const uint64_t* pos,
+                               uint64_t n, uint32_t* out) {
+  const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n && open[i]) out[pos[i]] = list ? list[i] : uint32_t(i);
+}
+
+// Highest (out + in) degree, smallest ID among equals: max of deg << 32 | ~v.
+__global__ void k_hub(const uint32_t* dout, const uint32_t* din, uint64_t V,
+                      unsigned long long* best) {
+  unsigned long long m = 0;
+  for (uint64_t v = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; v < V;
+       v += uint64_t(gridDim.x) * blockDim.x) {
+    const unsigned long long k = (uint64_t(dout[v]) + din[v]) << 32 | (0xffffffffull - v);
+    m = k > m ? k : m;
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    const unsigned long long y = __shfl_xor(m, o, 64);
+    m = y > m ? y : m;
+  }
+  if ((threadIdx.x & 63) == 0) atomicMax(best, m);
+}
+
+// Complement of the reach set from the searched `visited` bitmap (which also
+// holds the excluded dead rows and the padding bits beyond V): notr = rows
+// outside R, padding included.
+__global__ void k_reach_complement(const uint64_t* visited, const uint64_t* dead, uint64_t V,
+                                   uint64_t words, uint64_t* notr) {
+  const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= words) return;
+  uint64_t pad = 0;
+  const uint64_t lo = i * 64;
+  if (lo + 64 > V) pad = V > lo ? ~((uint64_t(1) << (V - lo)) - 1) : ~uint64_t(0);
+  notr[i] = ~visited[i] | dead[i] | pad;
+}
+
+// Out-edges of the vertices in R (grid-stride, one atomic per workgroup).
+__global__ void k_reach_edges(const uint64_t* notr, const uint32_t* dout, uint64_t V,
+                              unsigned long long* out) {
+  unsigned long long n = 0;
+  for (uint64_t v = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; v < V;
+       v += uint64_t(gridDim.x) * blockDim.x)
+    if (!((notr[v >> 6] >> (v & 63)) & 1)) n += dout[v];
+  block_atomic_add(out, n);
 }
 
 // Frontier bitmap -> queue of its vertices with out-edges.  One thread per
@@ -409,12 +500,20 @@ __global__ void k_count_reached(const int32_t* lv, uint64_t V, unsigned long lon
   block_atomic_add(out, n);
 }
 
-Counters read_counters(ll_gpu_ctx* ctx, const Counters* c) {
-  static_assert(kStripes * sizeof(Counters) <= kReadbackBytes, "stripes fit the read-back buffer");
+// Summed stripes (aux from stripe 0); with `extra`, one more device word is
+// read back in the same round trip into *extra_out.
+Counters read_counters(ll_gpu_ctx* ctx, const Counters* c, const uint64_t* extra = nullptr,
+                       uint64_t* extra_out = nullptr) {
+  static_assert(kStripes * sizeof(Counters) + 8 <= kReadbackBytes,
+                "stripes fit the read-back buffer");
   const Counters* st = static_cast<const Counters*>(ctx->readback);
+  uint64_t* ex = reinterpret_cast<uint64_t*>(static_cast<char*>(ctx->readback) +
+                                             kStripes * sizeof(Counters));
   LLG_HIP(hipMemcpyAsync(ctx->readback, c, kStripes * sizeof(Counters), hipMemcpyDeviceToHost,
                          ctx->stream));
+  if (extra) LLG_HIP(hipMemcpyAsync(ex, extra, 8, hipMemcpyDeviceToHost, ctx->stream));
   LLG_HIP(hipStreamSynchronize(ctx->stream));
+  if (extra) *extra_out = *ex;
   Counters h = st[0];
   for (int i = 1; i < kStripes; i++) {
     h.n_next += st[i].n_next;
@@ -452,7 +551,10 @@ struct BfsWork {
   uint32_t* qdeg;
   uint64_t* qofs;
   Counters* cnt;
-  void* scan_tmp;
+  void* scan_tmp;       // scan_tmp_bytes(max(V, row blocks))
+  uint32_t* list[2];    // open row blocks of the bottom-up levels (row-block count each)
+  uint32_t* open;       // per list entry: still open after the level
+  uint64_t* open_pos;   // exclusive scan of `open` (row-block count + 1)
   bool timed;
 };
 
@@ -465,23 +567,35 @@ struct MaybeTimed {
 };
 
 // bfs_core: see ll_gpu_internal.h; this form runs in caller-provided memory.
+// `notr` (may be null) is the complement of a reach set R (closed under
+// successors) with `e_reach` out-edges inside R: when the root's successors
+// all lie in R, rows outside R are excluded instead of `excluded`.
 BfsResult bfs_core_ws(ll_gpu_graph* g, const Csr& fwd, const Csr& rev, const RowBlocks& rev_rb,
                       const uint32_t* fdeg, uint32_t root, const uint64_t* excluded,
-                      const BfsWork& w, int32_t* lv) {
+                      const uint64_t* notr, uint64_t e_reach, const BfsWork& w, int32_t* lv) {
   ll_gpu_ctx* ctx = g->ctx;
   hipStream_t s = ctx->stream;
   const uint64_t V = g->V;
   const uint64_t words = (V + 63) / 64;
   uint64_t* fbm = w.fbm;  // frontier / next-frontier bitmaps, swapped per bottom-up level
   uint64_t* nbm = w.nbm;
-  k_bfs_init<<<grid_for(std::max(V, words), 256), 256, 0, s>>>(lv, w.visited,
-                                                              excluded, V, words);
   k_zero_counters<<<1, kStripes, 0, s>>>(w.cnt);
+  if (notr)
+    k_reach_check<<<1, 256, 0, s>>>(root, notr, fwd.ofs.as<uint64_t>(), fwd.adj.as<uint32_t>(),
+                                    w.cnt);
+  else
+    k_fill_u64<<<1, 1, 0, s>>>(reinterpret_cast<uint64_t*>(&w.cnt->aux), 1, 0);
+  k_bfs_init<<<grid_for(std::max(V, words), 256), 256, 0, s>>>(lv, w.visited, excluded, notr,
+                                                              w.cnt, V, words);
   k_bfs_root<<<1, 1, 0, s>>>(lv, w.visited, w.q[0], root, fdeg,
                              w.cnt);
   LLG_HIP(hipGetLastError());
   Counters h = read_counters(ctx, w.cnt);
   BfsResult r;
+  // open row blocks of the bottom-up levels: all of them at first
+  const uint32_t* list = nullptr;
+  uint64_t n_list = rev_rb.n;
+  int lc = 0;
   uint64_t nq = h.n_next;       // queue entries (vertices with out-edges)
   uint64_t n_f = 1;             // frontier vertices
   uint64_t m_f = h.m_next;      // frontier out-edges
@@ -490,7 +604,8 @@ BfsResult bfs_core_ws(ll_gpu_graph* g, const Csr& fwd, const Csr& rev, const Row
   bool bottom_up = false;
   int cur = 0;
   int32_t lvl = 0;
-  const uint64_t E = fwd.E;
+  // out-edges a search can still reach: those inside R when R excludes the rest
+  const uint64_t E = h.aux ? e_reach : fwd.E;
   const long a_opt = ctx->options.count("bfs_alpha") ? ctx->options["bfs_alpha"] : -1;
   const long b_opt = ctx->options.count("bfs_beta") ? ctx->options["bfs_beta"] : -1;
   const uint64_t alpha = a_opt > 0 ? uint64_t(a_opt) : 14, beta = b_opt > 0 ? uint64_t(b_opt) : 24;
@@ -503,17 +618,21 @@ BfsResult bfs_core_ws(ll_gpu_graph* g, const Csr& fwd, const Csr& rev, const Row
                                                           fbm);
     }
     k_zero_counters<<<1, kStripes, 0, s>>>(w.cnt);
+    uint64_t n_open = 0;
     if (bottom_up) {
       k_fill_u64<<<grid_for(words, 256), 256, 0, s>>>(nbm, words, 0);
-      {
-        MaybeTimed t(ctx, "k_bu_step", w.timed);
-        k_bu_rows<<<unsigned(rev_rb.n), kBlock, 0, s>>>(
-            rev_rb.items.as<RowItem>(), rev.ofs.as<uint64_t>(), rev.adj.as<uint32_t>(), fdeg,
-            fbm, w.visited, nbm, lv, lvl + 1,
-            w.cnt);
+      if (n_list) {
+        {
+          MaybeTimed t(ctx, "k_bu_step", w.timed);
+          k_bu_rows<<<unsigned(n_list), kBlock, 0, s>>>(
+              rev_rb.items.as<RowItem>(), list, w.open, rev.ofs.as<uint64_t>(),
+              rev.adj.as<uint32_t>(), fdeg, fbm, w.visited, nbm, lv, lvl + 1, w.cnt);
+        }
+        k_visited_or<<<grid_for(words, 256), 256, 0, s>>>(w.visited, nbm, words);
+        scan_u32_to_u64_ws(w.open, w.open_pos, n_list, w.scan_tmp, s);
+        k_list_compact<<<grid_for(n_list, 256), 256, 0, s>>>(list, w.open, w.open_pos, n_list,
+                                                             w.list[lc]);
       }
-      k_visited_or<<<grid_for(words, 256), 256, 0, s>>>(w.visited,
-                                                        nbm, words);
     } else {
       k_queue_degrees<<<grid_for(nq, 256), 256, 0, s>>>(w.q[cur], nq, fdeg,
                                                         w.qdeg);
@@ -525,11 +644,17 @@ BfsResult bfs_core_ws(ll_gpu_graph* g, const Csr& fwd, const Csr& rev, const Row
           w.q[cur ^ 1], w.cnt);
     }
     LLG_HIP(hipGetLastError());
-    h = read_counters(ctx, w.cnt);
+    const bool listed = bottom_up && n_list > 0;
+    h = listed ? read_counters(ctx, w.cnt, w.open_pos + n_list, &n_open) : read_counters(ctx, w.cnt);
     r.examined += h.examined;
     const uint64_t n_next = h.n_next;
     lvl++;
     m_visited += h.m_next;
+    if (listed) {  // the open row blocks after this level
+      list = w.list[lc];
+      lc ^= 1;
+      n_list = n_open;
+    }
     if (bottom_up) {
       r.reached += n_next;
       const bool shrinking = n_next < n_f;
@@ -575,12 +700,19 @@ BfsResult bfs_core(ll_gpu_graph* g, const Csr& fwd, const Csr& rev, const RowBlo
   qdeg.alloc(V * 4, s);
   qofs.alloc((V + 1) * 8, s);
   cnt.alloc(kStripes * sizeof(Counters), s);
-  const size_t sb = scan_tmp_bytes(V);
+  const uint64_t ni = std::max<uint64_t>(rev_rb.n, 1);
+  const size_t sb = scan_tmp_bytes(std::max(V, ni));
   if (sb) stmp.alloc(sb, s);
+  DevBuf lists[2], open, open_pos;
+  lists[0].alloc(ni * 4, s);
+  lists[1].alloc(ni * 4, s);
+  open.alloc(ni * 4, s);
+  open_pos.alloc((ni + 1) * 8, s);
   BfsWork w{visited.as<uint64_t>(), fbm.as<uint64_t>(), nbm.as<uint64_t>(),
             {q[0].as<uint32_t>(), q[1].as<uint32_t>()}, qdeg.as<uint32_t>(), qofs.as<uint64_t>(),
-            cnt.as<Counters>(), stmp.p, true};
-  return bfs_core_ws(g, fwd, rev, rev_rb, fdeg, root, excluded, w, lv);
+            cnt.as<Counters>(), stmp.p, {lists[0].as<uint32_t>(), lists[1].as<uint32_t>()},
+            open.as<uint32_t>(), open_pos.as<uint64_t>(), true};
+  return bfs_core_ws(g, fwd, rev, rev_rb, fdeg, root, excluded, nullptr, 0, w, lv);
 }
 
 // Working memory of one single-source search, carved into 256-byte aligned
@@ -589,9 +721,10 @@ BfsResult bfs_core(ll_gpu_graph* g, const Csr& fwd, const Csr& rev, const RowBlo
 // temporary memory.
 namespace {
 struct WsLayout {
-  size_t visited, fbm, nbm, q0, q1, qdeg, qofs, cnt, red, stmp, total;
+  size_t visited, fbm, nbm, q0, q1, qdeg, qofs, cnt, red, stmp, l0, l1, open, open_pos, total;
 };
-WsLayout ws_layout(uint64_t V) {
+WsLayout ws_layout(uint64_t V, uint64_t items) {
+  const uint64_t ni = std::max<uint64_t>(items, 1);
   const uint64_t words = (V + 63) / 64;
   auto up = [](size_t x) { return (x + 255) & ~size_t(255); };
   WsLayout L{};
@@ -606,13 +739,58 @@ WsLayout ws_layout(uint64_t V) {
   L.qofs = take((V + 1) * 8);
   L.cnt = take(kStripes * sizeof(Counters));
   L.red = take(8);
-  L.stmp = take(scan_tmp_bytes(V));
+  L.stmp = take(scan_tmp_bytes(std::max(V, ni)));
+  L.l0 = take(ni * 4);
+  L.l1 = take(ni * 4);
+  L.open = take(ni * 4);
+  L.open_pos = take((ni + 1) * 8);
   L.total = o;
   return L;
 }
 }  // namespace
 
-size_t bfs_work_bytes(uint64_t V) { return ws_layout(V).total; }
+size_t bfs_work_bytes(uint64_t V, uint64_t items) { return ws_layout(V, items).total; }
+
+namespace {
+
+// The reach set of the snapshot: R = the vertices reachable from its hub, the
+// vertex of highest out + in degree (smallest ID among equals).  R is closed
+// under successors, so a search from any vertex of R -- or from a vertex all
+// of whose out-neighbours are in R -- never leaves R, and its bottom-up steps
+// can treat rows outside R as visited from the start (exactly as the dead
+// rows, which R never holds).  Built with one search per snapshot level.
+void build_reach(ll_gpu_graph* g, int level, const BfsPrepared& p) {
+  ll_gpu_ctx* ctx = g->ctx;
+  hipStream_t s = ctx->stream;
+  const uint64_t V = g->V, words = (V + 63) / 64;
+  LLG_CHECK(V > 0, LL_GPU_EINVAL, "a graph without vertices has no search");
+  const uint32_t* din = ensure_degrees(g, LL_GPU_DIR_IN, level);
+  DevBuf best;
+  best.alloc(16, s);
+  k_fill_u64<<<1, 2, 0, s>>>(best.as<uint64_t>(), 2, 0);
+  const unsigned gr = std::min(grid_for(V, 256), 4u * unsigned(ctx->num_cus));
+  k_hub<<<gr, 256, 0, s>>>(p.outdeg, din, V, best.as<unsigned long long>());
+  LLG_HIP(hipGetLastError());
+  LLG_HIP(hipMemcpyAsync(ctx->readback, best.p, 8, hipMemcpyDeviceToHost, s));
+  LLG_HIP(hipStreamSynchronize(s));
+  const uint64_t key = *static_cast<const uint64_t*>(ctx->readback);
+  const uint32_t hub = 0xffffffffu - uint32_t(key & 0xffffffffu);
+  DevBuf visited;
+  (void)bfs_core(g, *p.out, *p.in, *p.in_rb, p.outdeg, hub, p.dead, visited, nullptr);
+  g->reach_notr.alloc(words * 8, s);
+  k_reach_complement<<<grid_for(words, 256), 256, 0, s>>>(visited.as<uint64_t>(), p.dead, V,
+                                                          words, g->reach_notr.as<uint64_t>());
+  k_reach_edges<<<gr, 256, 0, s>>>(g->reach_notr.as<uint64_t>(), p.outdeg, V,
+                                   best.as<unsigned long long>() + 1);
+  LLG_HIP(hipGetLastError());
+  LLG_HIP(hipMemcpyAsync(ctx->readback, best.as<uint64_t>() + 1, 8, hipMemcpyDeviceToHost, s));
+  LLG_HIP(hipStreamSynchronize(s));
+  g->reach_e = *static_cast<const uint64_t*>(ctx->readback);
+  g->reach_hub = hub;
+  g->reach_level = level;
+}
+
+}  // namespace
 
 bool bfs_peek(ll_gpu_graph* g, int level, BfsPrepared* p) {
   level = resolve_level(g, level);
@@ -627,18 +805,26 @@ bool bfs_peek(ll_gpu_graph* g, int level, BfsPrepared* p) {
     if (r.valid && r.cap == kBuCap && r.piece == kPiece) rb = &r;
   if (!rb) return false;
   if (g->dead_level != c[LL_GPU_DIR_IN]->level || !g->dead_in.p) return false;
+  if (g->reach_level != level || !g->reach_notr.p) return false;
   *p = BfsPrepared{c[LL_GPU_DIR_OUT], c[LL_GPU_DIR_IN], g->deg[LL_GPU_DIR_OUT].as<uint32_t>(), rb,
-                   g->dead_in.as<uint64_t>()};
+                   g->dead_in.as<uint64_t>(), g->reach_notr.as<uint64_t>(), g->reach_e};
   return true;
 }
 
 BfsPrepared bfs_prepare(ll_gpu_graph* g, int level) {
+  level = resolve_level(g, level);
   BfsPrepared p;
   p.out = &ensure_csr(g, LL_GPU_DIR_OUT, level);
   p.in = &ensure_csr(g, LL_GPU_DIR_IN, level);
   p.outdeg = ensure_degrees(g, LL_GPU_DIR_OUT, level);
   p.in_rb = &ensure_rowblocks(g, LL_GPU_DIR_IN, level, kBuCap, kPiece);
   p.dead = ensure_dead_rows(g, *p.in);
+  if (g->reach_level != level || !g->reach_notr.p) {
+    TimedLaunch t(g->ctx, "bfs_reach_index");
+    build_reach(g, level, p);
+  }
+  p.notr = g->reach_notr.as<uint64_t>();
+  p.e_reach = g->reach_e;
   return p;
 }
 
@@ -646,14 +832,21 @@ BfsResult bfs_single_ws(ll_gpu_graph* g, const BfsPrepared& p, uint32_t root, vo
                         int32_t* lv, bool timed) {
   hipStream_t s = g->ctx->stream;
   const uint64_t V = g->V;
-  const WsLayout L = ws_layout(V);
+  const WsLayout L = ws_layout(V, p.in_rb->n);
   char* b = static_cast<char*>(ws);
   BfsWork w{reinterpret_cast<uint64_t*>(b + L.visited), reinterpret_cast<uint64_t*>(b + L.fbm),
             reinterpret_cast<uint64_t*>(b + L.nbm),
             {reinterpret_cast<uint32_t*>(b + L.q0), reinterpret_cast<uint32_t*>(b + L.q1)},
             reinterpret_cast<uint32_t*>(b + L.qdeg), reinterpret_cast<uint64_t*>(b + L.qofs),
-            reinterpret_cast<Counters*>(b + L.cnt), b + L.stmp, timed};
-  BfsResult r = bfs_core_ws(g, *p.out, *p.in, *p.in_rb, p.outdeg, root, p.dead, w, lv);
+            reinterpret_cast<Counters*>(b + L.cnt), b + L.stmp,
+            {reinterpret_cast<uint32_t*>(b + L.l0), reinterpret_cast<uint32_t*>(b + L.l1)},
+            reinterpret_cast<uint32_t*>(b + L.open), reinterpret_cast<uint64_t*>(b + L.open_pos),
+            timed};
+  // option bfs_reach 0: dead rows only (for comparison)
+  const auto& opts = g->ctx->options;
+  const bool use_reach = !opts.count("bfs_reach") || opts.at("bfs_reach") != 0;
+  BfsResult r = bfs_core_ws(g, *p.out, *p.in, *p.in_rb, p.outdeg, root, p.dead,
+                            use_reach ? p.notr : nullptr, p.e_reach, w, lv);
   // top-down levels count only queued vertices: recount the reached set
   unsigned long long* red = reinterpret_cast<unsigned long long*>(b + L.red);
   k_fill_u64<<<1, 1, 0, s>>>(reinterpret_cast<uint64_t*>(red), 1, 0);
@@ -672,7 +865,7 @@ BfsResult bfs_device(ll_gpu_graph* g, int level, uint32_t root, DevBuf& lv) {
   const BfsPrepared p = bfs_prepare(g, level);
   lv.alloc(g->V * 4, s);
   DevBuf ws;
-  ws.alloc(bfs_work_bytes(g->V), s);
+  ws.alloc(bfs_work_bytes(g->V, p.in_rb->n), s);
   return bfs_single_ws(g, p, root, ws.p, lv.as<int32_t>(), true);
 }
 

@@ -261,11 +261,13 @@ class gpu_bc : public ll_gpu_task<Graph> {
   gpu_bc(int k, uint64_t seed)
       : ll_gpu_task<Graph>("GPU Betweenness centrality - randomized"), k_(k), seed_(seed) {}
   void initialize(Graph* G) override {
-    ll_gpu_task<Graph>::initialize(G);
+    // sources first: the reference's pick draws from libc rand(), and the
+    // device set-up below must not come between the driver and the draws
     const uint64_t V = ll_gpu_max_nodes(*G);
-    bc_.assign(V, 0.0f);
     sources_.clear();
     for (int i = 0; i < k_ && V > 0; i++) sources_.push_back(ll_gpu_pick_source(*G, seed_, i));
+    ll_gpu_task<Graph>::initialize(G);
+    bc_.assign(V, 0.0f);
   }
   double run() override {
     this->m_->check(ll_gpu_bc(this->m_->graph(), -1, k_ < 0 ? nullptr : sources_.data(),

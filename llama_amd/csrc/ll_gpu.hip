@@ -138,6 +138,8 @@ void drop_snapshots(ll_gpu_graph* g) {
   g->sym_rows.release();
   g->dead_level = -1;
   g->dead_in.release();
+  g->reach_level = -1;
+  g->reach_notr.release();
   g->pr_level = -1;
   g->pr_inv.release();
   g->pr_adj.release();
@@ -146,6 +148,7 @@ void drop_snapshots(ll_gpu_graph* g) {
   g->pr_sort_cap = 0;
   g->pr_seg = llgpu::PrSegments();
   g->pr_hc = llgpu::PrHotCold();
+  g->pr_bins = llgpu::PrBins();
   g->w_level = -1;
   g->wcsr = Csr();
   g->wperm.release();
@@ -231,8 +234,8 @@ void* ll_gpu_stream(ll_gpu_ctx* c) { return c ? static_cast<void*>(c->stream) : 
 int ll_gpu_set_option(ll_gpu_ctx* c, const char* name, long value) {
   if (!c || !name) return LL_GPU_EINVAL;
   static const char* known[] = {"pr_reorder", "pr_cap", "pr_seg_bits", "pr_item_sort",
-                                "pr_hot_bits", "sssp_delta_milli", "sssp_min_queue",
-                                "bfs_alpha", "bfs_beta"};
+                                "pr_hot_bits", "pr_bin_bits", "sssp_delta_milli",
+                                "sssp_min_queue", "bfs_alpha", "bfs_beta", "bfs_reach"};
   try {
     for (const char* k : known)
       if (std::strcmp(k, name) == 0) {

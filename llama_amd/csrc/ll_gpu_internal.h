@@ -232,6 +232,19 @@ struct PrHotCold {
   RowBlocks hot_rb, cold_rb;
 };
 
+// Binned PageRank layout of the out-CSR (built by build_pr_bins): destination
+// bins of 2^bb vertices; every out-edge owns one slot of the binned value
+// array, slots grouped by bin.
+struct PrBins {
+  bool valid = false;
+  int level = -1;
+  uint32_t bb = 0, nbins = 0;
+  DevBuf binpos;  // uint32_t[E]: slot of out-edge e
+  DevBuf ldst;    // uint16_t[E]: slot -> destination offset in its bin
+  DevBuf srcl;    // uint16_t[E]: out-edge -> source offset in its 2^bb range
+  DevBuf bstart;  // uint64_t[nbins+1]: first slot of each bin
+};
+
 struct DirMirror {
   std::vector<DevBuf> vt;   // per level: the pages the level wrote (all of them for a full push)
   std::vector<std::vector<const VtEntry*>> vt_pages;  // per level: device address of every page
@@ -275,6 +288,11 @@ struct ll_gpu_graph {
   // rows of the in-CSR with no live in-edge (bitmap), excluded from BFS
   int dead_level = -1;
   llgpu::DevBuf dead_in;
+  // complement of the vertices reachable from the hub (bitmap), their out-edges
+  int reach_level = -1;
+  llgpu::DevBuf reach_notr;
+  uint64_t reach_e = 0;
+  uint32_t reach_hub = 0;
   // PageRank locality order of the in-CSR (option pr_reorder)
   int pr_level = -1;
   llgpu::DevBuf pr_inv, pr_adj;
@@ -282,6 +300,7 @@ struct ll_gpu_graph {
   uint32_t pr_sort_cap = 0;         // row-block capacity the sorted copy was built for
   llgpu::PrSegments pr_seg;   // column segments of the in-CSR in slot order
   llgpu::PrHotCold pr_hc;     // hot/cold split of the in-CSR in slot order
+  llgpu::PrBins pr_bins;      // binned PageRank layout of the out-CSR
   // weighted in-CSR (rows = heads, sources ascending) + map to out-edge positions
   int w_level = -1;
   llgpu::Csr wcsr;
@@ -327,13 +346,16 @@ struct BfsPrepared {
   const uint32_t* outdeg = nullptr;
   const RowBlocks* in_rb = nullptr;
   const uint64_t* dead = nullptr;
+  const uint64_t* notr = nullptr;  // complement of the snapshot's reach set
+  uint64_t e_reach = 0;            // out-edges inside the reach set
 };
 // Builds what is missing (allocates).
 BfsPrepared bfs_prepare(ll_gpu_graph* g, int level);
 // The cached structures of `level` only: false when any is missing.
 bool bfs_peek(ll_gpu_graph* g, int level, BfsPrepared* p);
-// Bytes of one search's working memory for V vertices.
-size_t bfs_work_bytes(uint64_t V);
+// Bytes of one search's working memory for V vertices and `items` in-CSR
+// row blocks.
+size_t bfs_work_bytes(uint64_t V, uint64_t items);
 // Single-source search in `ws` (bfs_work_bytes(V) bytes, 256-byte aligned):
 // levels into lv (device int32[V]); reached = vertices with a level.  Makes
 // no device allocation; `timed` records per-kernel times when profiling.
@@ -356,6 +378,12 @@ const RowBlocks& pr_segment_rowblocks(ll_gpu_ctx* ctx, PrSegments& ps, uint32_t 
 // Hot/cold split of `in` whose adjacency holds slots (`slot_adj`): hot = slot < 2^hb.
 void build_pr_hotcold(ll_gpu_graph* g, const Csr& in, const uint32_t* slot_adj, uint32_t hb,
                       PrHotCold& hc);
+
+// Binned PageRank (pr_bins.hip): layout of `out` with bins of 2^bb
+// destinations, and `iters` iterations over it (FP32 ranks in rank_final).
+void build_pr_bins(ll_gpu_graph* g, const Csr& out, uint32_t bb, PrBins& pb);
+void pagerank_bins(ll_gpu_graph* g, const Csr& out, const uint32_t* outdeg, PrBins& pb,
+                   int iters, double damping, DevBuf& rank_final, double* last_diff);
 
 // Device-wide exclusive scan of n values into out (n+1 entries, total last).
 void scan_u32_to_u64(const uint32_t* in, uint64_t* out, uint64_t n, hipStream_t s);

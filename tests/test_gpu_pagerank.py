@@ -1,5 +1,8 @@
 """HIP path vs the oracle: snapshot materialisation (bit-exact) and PageRank
-(L1 relative error <= 1e-6 against FP64 pagerank_double)."""
+(L1 relative error <= 1e-6 against FP64 pagerank_double), for the binned pull
+(the default; exact fixed-point row sums, so results are bit-identical for
+every bin size and run) and for the row-block pull and its variants (option
+pr_bin_bits = 0)."""
 import numpy as np
 import pytest
 
@@ -17,13 +20,29 @@ def l1rel(a, b):
 
 
 @pytest.fixture
-def seg_bits(ctx):
-    """Set the PageRank column-segment option for one test, then restore the
-    automatic default (-1)."""
+def rowpull(ctx):
+    """The row-block pull (binned pull off) for one test."""
+    ctx.set_option("pr_bin_bits", 0)
+    yield
+    ctx.set_option("pr_bin_bits", -1)
+
+
+@pytest.fixture
+def seg_bits(ctx, rowpull):
+    """Set the PageRank column-segment option of the row-block pull for one
+    test, then restore the automatic default (-1)."""
     def set_bits(n):
         ctx.set_option("pr_seg_bits", n)
     yield set_bits
     ctx.set_option("pr_seg_bits", -1)
+
+
+@pytest.fixture
+def bin_bits(ctx):
+    def set_bits(n):
+        ctx.set_option("pr_bin_bits", n)
+    yield set_bits
+    ctx.set_option("pr_bin_bits", -1)
 
 
 @pytest.fixture(scope="module")
@@ -202,7 +221,7 @@ def test_pagerank_segments_by_vertex_id(ctx, ml, seg_bits):
 
 
 @pytest.mark.parametrize("cap", [512, 4096])
-def test_pagerank_item_sorted_gathers(ctx, ml, cap):
+def test_pagerank_item_sorted_gathers(ctx, ml, cap, rowpull):
     """Option pr_item_sort: every row-block item's entries sorted by slot
     (LDS restores row order; pieces of long rows summed in slot order)."""
     g, o = ml
@@ -220,7 +239,7 @@ def test_pagerank_item_sorted_gathers(ctx, ml, cap):
     assert np.array_equal(a, b)
 
 
-def test_pagerank_item_sorted_hub(ctx):
+def test_pagerank_item_sorted_hub(ctx, rowpull):
     gpu_case("pagerank pr_item_sort hub 20000")
     k = 20000
     n = k + 3
@@ -241,7 +260,7 @@ def test_pagerank_item_sorted_hub(ctx):
 
 @pytest.mark.parametrize("cap", [512, 4096])
 @pytest.mark.parametrize("hot", [6, 9])
-def test_pagerank_hot_cold_split(ctx, ml, cap, hot):
+def test_pagerank_hot_cold_split(ctx, ml, cap, hot, rowpull):
     """Option pr_hot_bits: a hot pass over in-edges from slots < 2^hot writes
     per-vertex partials, the cold pass continues from them and finalises."""
     g, o = ml
@@ -262,7 +281,7 @@ def test_pagerank_hot_cold_split(ctx, ml, cap, hot):
         ctx.set_option("pr_hot_bits", -1)
 
 
-def test_pagerank_hot_cold_hub_and_empty(ctx):
+def test_pagerank_hot_cold_hub_and_empty(ctx, rowpull):
     gpu_case("pagerank hot/cold hub 20000 + empty")
     k = 20000
     n = k + 3
@@ -283,6 +302,96 @@ def test_pagerank_hot_cold_hub_and_empty(ctx):
         assert np.allclose(r0, 0.15 / 5, rtol=1e-6)
     finally:
         ctx.set_option("pr_hot_bits", -1)
+
+
+@pytest.fixture(scope="module")
+def ml15():
+    g = build_multilevel(15, levels=5, ins_frac=0.03, del_frac=0.01, seed=11)
+    return g, oracle.OracleGraph.from_host(g)
+
+
+def test_pagerank_bins_every_bin_size(ctx, ml15, bin_bits):
+    """Binned pull with bins of 2^12, 2^13 and 2^14 destinations (8, 4 and 2
+    bins at s15): within the bar at the first and last level, the final
+    iteration's L1 change as the oracle's, and bit-identical ranks for every
+    bin size and after a rebuild of the layout (the row sums are exact)."""
+    g, o = ml15
+    dg = DeviceGraph(ctx, g)
+    for L in (0, g.num_levels - 1):
+        ref, ref_diff = o.pagerank(20, level=L)
+        first = None
+        for bb in (12, 13, 14):
+            gpu_case(f"pagerank bins bb={bb} level {L}")
+            bin_bits(bb)
+            r, diff = dg.pagerank(20, level=L)
+            err = l1rel(r, ref)
+            print(f"bins bb {bb} level {L} L1rel {err:.3e}")
+            assert err <= PR_TOL
+            assert abs(diff - ref_diff) <= 1e-3 * ref_diff + 1e-9
+            if first is None:
+                first = r
+            assert np.array_equal(r, first)
+    dg.drop_snapshots()
+    r2, _ = dg.pagerank(20)
+    assert np.array_equal(r2, first)
+
+
+def test_pagerank_bins_ragged_and_tiny(ctx, bin_bits):
+    """Vertex counts that are not a multiple of the bin (last bin partial),
+    smaller than one bin, isolated vertices, an empty graph, one vertex, and a
+    hub with in-degree 20000 inside one bin."""
+    bin_bits(12)
+    rng = np.random.default_rng(4)
+    V = 20000 + 37
+    src = rng.integers(0, V, 150000)
+    dst = rng.integers(0, V, 150000)
+    keep = src != dst
+    g = HostGraph(V)
+    g.add_level(src[keep], dst[keep])
+    g.add_random_level(0.05, 0.02, 3)
+    ref, _ = oracle.OracleGraph.from_host(g).pagerank(15)
+    gpu_case("pagerank bins ragged V=20037")
+    r, _ = DeviceGraph(ctx, g).pagerank(15)
+    assert l1rel(r, ref) <= PR_TOL
+    k = 20000
+    n = k + 3
+    hs = np.concatenate([rng.permutation(np.arange(1, k + 1)), [0, n - 1]])
+    hd = np.concatenate([np.zeros(k, int), [n - 2, 0]])
+    h = HostGraph(n)
+    h.add_level(hs, hd)
+    ref, _ = oracle.OracleGraph.from_host(h).pagerank(5)
+    gpu_case("pagerank bins hub 20000")
+    r, _ = DeviceGraph(ctx, h).pagerank(5)
+    assert l1rel(r, ref) <= PR_TOL
+    gpu_case("pagerank bins empty / one vertex / one edge")
+    e = HostGraph(5)
+    e.add_level([], [])
+    r, d = DeviceGraph(ctx, e).pagerank(1)
+    assert np.allclose(r, 0.15 / 5, rtol=1e-6)
+    one = HostGraph(1)
+    one.add_level([], [])
+    r, _ = DeviceGraph(ctx, one).pagerank(3)
+    assert np.allclose(r, [0.15], rtol=1e-6)
+    t = HostGraph(300)
+    t.add_level([7], [290])
+    ref, _ = oracle.OracleGraph.from_host(t).pagerank(2)
+    r, _ = DeviceGraph(ctx, t).pagerank(2)
+    assert l1rel(r, ref) <= PR_TOL
+
+
+def test_pagerank_bins_equal_row_pull(ctx, ml, bin_bits):
+    """Binned and row-block pulls on the same snapshot: both within the bar,
+    and within 2e-7 of each other (different FP32 roundings of the same sums)."""
+    g, o = ml
+    dg = DeviceGraph(ctx, g)
+    gpu_case("pagerank bins vs row-block pull")
+    bin_bits(13)
+    a, _ = dg.pagerank(20)
+    bin_bits(0)
+    b, _ = dg.pagerank(20)
+    ref, _ = o.pagerank(20)
+    assert l1rel(a, ref) <= PR_TOL and l1rel(b, ref) <= PR_TOL
+    assert l1rel(a, b) <= 2e-7
 
 
 def test_pagerank_push_matches_fp64(ctx, ml):
