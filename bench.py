@@ -109,6 +109,24 @@ def cpu_model() -> str:
     return "unknown"
 
 
+def host_cpu_layout() -> dict:
+    """Sockets, cores and threads of the host (lscpu) and the CPUs this
+    process may run on (its affinity mask)."""
+    import subprocess
+    out = {"affinity_cpus": len(os.sched_getaffinity(0))}
+    try:
+        txt = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        for line in txt.splitlines():
+            k, _, v = line.partition(":")
+            k = k.strip()
+            if k in ("Socket(s)", "Core(s) per socket", "Thread(s) per core", "CPU(s)",
+                     "Model name", "NUMA node(s)"):
+                out[k] = v.strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return out
+
+
 def tuned_key(kind: str, dg, E: int | None, n: int) -> str:
     """Key of the autotuner decision for a request of n sources/seeds on the
     latest snapshot: kind:V:E:level:bucket, bucket = smallest power of two
@@ -363,26 +381,30 @@ def bench_bfs_batch(dg, g, V: int, args, single: dict | None) -> dict:
             f"autotuned_lanes{dg.tuned(tuned_key('bfs_lanes', dg, None, len(roots)))}"
         out[key] = {"seconds": round(t, 4), "sweeps": sweeps, "kernel_ms": round(ms_k, 3),
                     "reached_total": int(reached.sum())}
-    # Graph500 numerator per source, E_comp(s) = sum of live out-degrees of the
-    # vertices s reaches, from single-source runs outside the timed regions;
-    # the reached counts double as a cross-check of the batched lanes
+    # The same sources as sequential single-source runs, timed: they also give
+    # the Graph500 numerator per source, E_comp(s) = sum of live out-degrees
+    # of the vertices s reaches, and their reached counts cross-check the
+    # batched lanes
     e_comp, same_reach = 0, True
+    dg.bfs(int(roots[0]), want_levels=False)  # prepared snapshot, as for the batches
+    ctx.sync()
+    t0 = time.perf_counter()
     for i, r in enumerate(roots):
         _, n_r, _ = dg.bfs(int(r), want_levels=False)
         e_comp += dg.last_bfs_traversed
         same_reach &= n_r == int(reached[i])
+    ctx.sync()
+    seq = time.perf_counter() - t0
     out["edges_traversed_total"] = e_comp
     out["reached_equal_single_source"] = bool(same_reach)
+    out["sequential_seconds"] = round(seq, 4)
+    out["sequential_gteps"] = round(e_comp / seq / 1e9, 2)
     for k, v in list(out.items()):
         if isinstance(v, dict) and "seconds" in v:
             v["aggregate_gteps"] = round(e_comp / v["seconds"] / 1e9, 2)
-    if single:
-        per_src = single["seconds"] / single["sources"]
-        seq = per_src * len(roots)
-        out["sequential_estimate_seconds"] = round(seq, 4)
-        out["speedup_vs_sequential_lanes64"] = round(seq / out["lanes64"]["seconds"], 2)
-        best = min(v["seconds"] for k, v in out.items() if isinstance(v, dict))
-        out["speedup_vs_sequential_best"] = round(seq / best, 2)
+    out["speedup_vs_sequential_lanes64"] = round(seq / out["lanes64"]["seconds"], 2)
+    best = min(v["seconds"] for k, v in out.items() if isinstance(v, dict))
+    out["speedup_vs_sequential_best"] = round(seq / best, 2)
     log(f"batched BFS x{len(roots)}: {out}")
     return out
 
@@ -458,7 +480,7 @@ def bench_wpr(ctx, args) -> dict:
 # one PageRank pull iteration: pull (per vertex, per (row, segment) pair, or
 # the hot pass of a hot/cold split), the cold pass of a split, split-row
 # fix-ups, and the segment merge when the pull is segmented
-PR_ITERATION_KERNELS = ("k_pr_pull", "k_pr_pull_cold", "k_pr_fixup", "k_pr_merge")
+PR_ITERATION_KERNELS = ("k_pb_iter", "k_pr_pull", "k_pr_pull_cold", "k_pr_fixup", "k_pr_merge")
 
 
 TRAFFIC_PROFILE = ROOT / "profiles" / "k_pr_pull_traffic_rmat_s24_l11.json"
@@ -475,7 +497,7 @@ def measured_traffic(args, V: int, E: int, used: list) -> dict | None:
         return None
     w = prof["workload"]
     plain = (args.reorder == 1 and args.seg_bits in (-1, 0) and args.hot_bits in (-1, 0)
-             and args.item_sort in (-1, 0) and args.cap in (0, 2048))
+             and args.item_sort in (-1, 0) and args.cap in (0, 2048) and args.bin_bits == 0)
     same = (w["num_nodes"] == V and w["live_edges"] == E and w["seed"] == args.seed and plain
             and used == ["k_pr_pull", "k_pr_fixup"])
     if not same:
@@ -698,6 +720,9 @@ def main() -> int:
     ap.add_argument("--sweep-configs", default="",
                     help="with --sweep-pr: comma-separated reorder:seg_bits:cap[:item_sort[:hot_bits]] "
                          "(default: the full grid)")
+    ap.add_argument("--bin-bits", type=int, default=-1,
+                    help="binned PageRank bins of 2^N destinations (0 = row-block pull, "
+                         "-1 = automatic)")
     ap.add_argument("--reorder", type=int, default=1,
                     help="1 = degree-ordered contribution slots in PageRank (default)")
     ap.add_argument("--seg-bits", type=int, default=-1,
@@ -706,7 +731,7 @@ def main() -> int:
                     help="PageRank items sorted by slot (1), not (0), automatic (-1)")
     ap.add_argument("--hot-bits", type=int, default=-1,
                     help="PageRank hot/cold split at slot 2^N (0 = off, -1 = automatic)")
-    ap.add_argument("--cpu-iters", type=int, default=6,
+    ap.add_argument("--cpu-iters", type=int, default=20,
                     help="PageRank iterations timed for the CPU baseline (0 = skip)")
     ap.add_argument("--cpu-runs", type=int, default=3,
                     help="timed repetitions of the CPU baseline sample (mean +- c95)")
@@ -746,6 +771,12 @@ def main() -> int:
                     help="weighted PageRank + adjoint graph scale, single level (0 = skip)")
     args = ap.parse_args()
 
+    # OpenMP placement of the CPU baseline (and of the host graph builder):
+    # threads bound close, one per core, as the reference's CPU runs are
+    # measured; set before any OpenMP runtime is loaded
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
+
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 "
                           "(multi-GPU sharding is design-only)"}))
@@ -766,6 +797,7 @@ def main() -> int:
     log(f"host graph built in {time.time() - t0:.1f}s")
 
     ctx = GpuContext(0)
+    ctx.set_option("pr_bin_bits", args.bin_bits)
     ctx.set_option("pr_reorder", args.reorder)
     ctx.set_option("pr_seg_bits", args.seg_bits)
     ctx.set_option("pr_item_sort", args.item_sort)
@@ -847,11 +879,14 @@ def main() -> int:
         step()
     ctx.sync()
     te = time.perf_counter()
-    n_pr, ms_pr = ctx.kernel_time("k_pr_pull")
+    n_pr, ms_pr = ctx.kernel_time("k_pb_iter")
+    if n_pr == 0:
+        n_pr, ms_pr = ctx.kernel_time("k_pr_pull")
     it_kernels = {k: ctx.kernel_time(k) for k in PR_ITERATION_KERNELS}
     mat = {k: ctx.kernel_time(k) for k in ("k_seg_count", "k_seg_fill", "k_live_count",
-                                           "k_compact", "k_degrees", "pr_reorder",
-                                           "pr_segments", "pr_item_sort", "pr_hotcold")}
+                                           "k_compact", "k_degrees", "pr_bins_build",
+                                           "k_pb_init", "pr_reorder", "pr_segments",
+                                           "pr_item_sort", "pr_hotcold")}
     avg_it_ms = iteration_ms(ctx, args.iters * args.steps)
     ctx.profile(False)
     log(f"timed {args.steps} steps done")
@@ -861,6 +896,9 @@ def main() -> int:
     achieved = b_pr / (avg_it_ms * 1e-3) / 1e9
     used = [k for k, v in it_kernels.items() if v[0] > 0]
     traffic = measured_traffic(args, V, E, used)
+
+    # the headline configuration's ranks, for the parity check against FP64
+    r_head, _ = dg.pagerank(args.iters, 0.85, cap=args.cap)
 
     copy_bw = measured_copy_gbs()
     log(f"device copy: {copy_bw['gbs']} GB/s (read + write)")
@@ -888,16 +926,26 @@ def main() -> int:
         secondary["wpr_adjoint"] = bench_wpr(ctx, args)
 
     cpu = None
+    parity = None
     if args.cpu_iters > 0:
         import oracle
         o = oracle.OracleGraph.from_host(g)
         log(f"CPU baseline: oracle pagerank_double, {args.cpu_iters} iterations, "
             f"{oracle.num_threads()} threads")
         runs = []
+        ref_ranks = None
         for _ in range(args.cpu_runs):
             tc = time.perf_counter()
-            o.pagerank(args.cpu_iters)
+            ref_ranks, _ = o.pagerank(args.cpu_iters)
             runs.append(time.perf_counter() - tc)
+        if args.cpu_iters == args.iters:
+            # the headline's ranks against the FP64 restatement on the same graph
+            parity = {"l1_rel_vs_fp64": float(np.abs(r_head - ref_ranks).sum() /
+                                              np.abs(ref_ranks).sum()),
+                      "bar": 1e-6,
+                      "against": f"oracle pagerank_double (FP64), {args.iters} iterations, "
+                                 f"same graph"}
+            log(f"headline PageRank L1 relative error vs FP64: {parity['l1_rel_vs_fp64']:.3e}")
         cpu_s = sum(runs) / len(runs)
         # the reference's ll_c95: 1.96 * population stdev / sqrt(n)
         # (llama/include/llama/ll_utils.h:200-221)
@@ -910,11 +958,21 @@ def main() -> int:
                          f"over {len(runs)} runs",
                "seconds_runs": [round(x, 3) for x in runs],
                "cpu_model": cpu_model(),
+               "host": host_cpu_layout(),
                "omp": {k: os.environ.get(k) for k in ("OMP_NUM_THREADS", "OMP_PROC_BIND",
                                                        "OMP_PLACES")}}
         log(f"CPU baseline {cpu['value']:.3f} GTEPS, mean of {len(runs)} runs of {cpu_s:.1f}s")
         if args.ref_calibration:
-            cpu["reference_calibration"] = reference_calibration(ctx, args)
+            cal = reference_calibration(ctx, args)
+            cpu["reference_calibration"] = cal
+            if "port" in cal:
+                # the reference build ran the same algorithm port/ref times faster
+                # on the calibration graph: scale the port's rate by that ratio
+                q = cal["port"]["over_reference"]
+                cpu["reference_equivalent"] = {
+                    "value": round(cpu["value"] * q, 4), "unit": "GTEPS",
+                    "how": f"port rate x {q} (port time / reference time on the "
+                           f"reference's own s20 case, same box, same threads)"}
 
     out = {
         "metric": METRIC,
@@ -950,6 +1008,7 @@ def main() -> int:
             "measured_copy": copy_bw,
         },
         "cpu_baseline": cpu,
+        "parity": parity,
         "breakdown_ms_per_step": {
             **{k: round(v[1] / args.steps, 4) for k, v in it_kernels.items()},
             **{k: round(v[1] / args.steps, 4) for k, v in mat.items()},

@@ -538,12 +538,31 @@ int orc_ppr(const orc_graph* g, int level, const int64_t* seeds, int n, int iter
 // weights w in out-edge order: r0 = p, r' = (1-d) p + d * W^T r.
 int orc_wpr_forward(uint64_t V, const uint64_t* ofs, const uint32_t* adj, const double* w,
                     const double* p, int iters, double d, double* r_out) {
+  // r' = (1-d) p + d W^T r as a pull over the transpose (built once, heads'
+  // in-edges in out-edge order), so every head is summed by one thread.
+  const uint64_t E = ofs[V];
+  std::vector<uint64_t> tofs(V + 1, 0);
+  for (uint64_t e = 0; e < E; e++) tofs[adj[e] + 1]++;
+  for (uint64_t v = 0; v < V; v++) tofs[v + 1] += tofs[v];
+  std::vector<uint32_t> tsrc(E);
+  std::vector<double> tw(E);
+  {
+    std::vector<uint64_t> cur(tofs.begin(), tofs.end() - 1);
+    for (uint64_t u = 0; u < V; u++)
+      for (uint64_t e = ofs[u]; e < ofs[u + 1]; e++) {
+        const uint64_t k = cur[adj[e]]++;
+        tsrc[k] = uint32_t(u);
+        tw[k] = w[e];
+      }
+  }
   std::vector<double> r(p, p + V), nx(V);
   for (int it = 0; it < iters; it++) {
-    std::fill(nx.begin(), nx.end(), 0.0);
-    for (uint64_t u = 0; u < V; u++)
-      for (uint64_t e = ofs[u]; e < ofs[u + 1]; e++) nx[adj[e]] += w[e] * r[u];
-    for (uint64_t v = 0; v < V; v++) nx[v] = (1 - d) * p[v] + d * nx[v];
+#pragma omp parallel for schedule(dynamic, 4096)
+    for (uint64_t v = 0; v < V; v++) {
+      double s = 0;
+      for (uint64_t k = tofs[v]; k < tofs[v + 1]; k++) s += tw[k] * r[tsrc[k]];
+      nx[v] = (1 - d) * p[v] + d * s;
+    }
     r.swap(nx);
   }
   std::copy(r.begin(), r.end(), r_out);
@@ -558,6 +577,7 @@ int orc_wpr_adjoint(uint64_t V, const uint64_t* ofs, const uint32_t* adj, const 
                     double* dp) {
   std::vector<double> lam(gr, gr + V), nx(V);
   for (int it = 0; it < iters; it++) {
+#pragma omp parallel for schedule(dynamic, 4096)
     for (uint64_t u = 0; u < V; u++) {
       double s = 0;
       for (uint64_t e = ofs[u]; e < ofs[u + 1]; e++) s += w[e] * lam[adj[e]];
@@ -565,8 +585,10 @@ int orc_wpr_adjoint(uint64_t V, const uint64_t* ofs, const uint32_t* adj, const 
     }
     lam.swap(nx);
   }
+#pragma omp parallel for schedule(dynamic, 4096)
   for (uint64_t u = 0; u < V; u++)
     for (uint64_t e = ofs[u]; e < ofs[u + 1]; e++) dw[e] = d * r[u] * lam[adj[e]];
+#pragma omp parallel for
   for (uint64_t v = 0; v < V; v++) dp[v] = (1 - d) * lam[v];
   return 0;
 }

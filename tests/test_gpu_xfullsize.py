@@ -1,14 +1,20 @@
 """Parity at the full benchmark size: R-MAT scale 24, edge factor 16, plus 10
 levels of +1% / -0.1% (deletion vector on, U[0, 1) weights) -- the graph
-bench.py measures.  The oracle's per-edge restatement is too slow for the
-algorithms at this size, so each result is pinned by a property that fixes it
-exactly: the snapshot equals the oracle's iterator bit for bit; BFS levels
-satisfy the two edge conditions that characterise BFS distances; SSSP
-distances are the FP32 fixed point on every edge; PageRank agrees between the
-pull (FP32 compensated) and push (FP64 atomics) paths; Brandes dependencies
+bench.py measures.  Pinned here: the snapshot equals the oracle's iterator bit
+for bit; PageRank (20 iterations) is within 1e-6 L1 of the oracle's FP64
+pagerank_double and agrees between the pull (FP32 compensated) and push (FP64
+atomics) paths; the triangle counts equal the oracle's tc_u / tc_i, written
+into tests/golden/tc_rmat_s24_l11.json by tests/golden/make_golden_tc_s24.py;
+BFS levels satisfy the two edge conditions that characterise BFS distances;
+SSSP distances are the FP32 fixed point on every edge; Brandes dependencies
 sum to the number of interior vertices of the shortest paths; every lane of
 the batched BFS equals the single-source run; SCC labels equal scipy's strong
-components (canonical smallest-ID labels)."""
+components (canonical smallest-ID labels); batched personalized PageRank with
+128 seeds is one step of its recurrence from the previous iterate in every
+column (sampled rows) and within 1e-5 L1 of the FP64 oracle in two columns."""
+import json
+from pathlib import Path
+
 import numpy as np
 import pytest
 
@@ -18,6 +24,7 @@ from llama_amd.graph import DIR_IN, DIR_OUT, UNREACHED, DeviceGraph, build_multi
 
 pytestmark = pytest.mark.gpu
 SCALE, LEVELS = 24, 11
+GOLDEN = Path(__file__).resolve().parent / "golden"
 
 
 @pytest.fixture(scope="module")
@@ -34,10 +41,15 @@ def big(ctx):
     dg.close()
 
 
-def test_full_snapshot_equals_oracle(big):
+@pytest.fixture(scope="module")
+def orc(big):
+    return oracle.OracleGraph.from_host(big[0])
+
+
+def test_full_snapshot_equals_oracle(big, orc):
     g, dg, ofs, adj, _, _ = big
     gpu_case("full size: snapshot vs oracle iterator")
-    o = oracle.OracleGraph.from_host(g)
+    o = orc
     r_ofs, r_adj = o.flatten(DIR_OUT)
     assert np.array_equal(ofs, r_ofs) and np.array_equal(adj, r_adj)
     i_ofs, i_adj = dg.adjacency(DIR_IN)
@@ -82,6 +94,63 @@ def test_full_sssp_is_the_fixed_point(big):
     assert best[hit].all() and dist[root] == 0
     assert reached == int((dist != unreached).sum())
     assert mx == float(dist[dist != unreached].max())
+
+
+def test_full_pagerank_vs_fp64_oracle(big, orc):
+    _, dg, _, _, _, _ = big
+    gpu_case("full size: PageRank 20 iterations vs the FP64 oracle")
+    r, _ = dg.pagerank(20)
+    ref, _ = orc.pagerank(20)
+    err = float(np.abs(r - ref).sum() / np.abs(ref).sum())
+    print(f"full size PageRank L1 relative error vs FP64 pagerank_double: {err:.3e}")
+    assert err <= 1e-6
+
+
+def test_full_triangles_pinned(big):
+    g, dg, ofs, _, _, _ = big
+    gold = json.loads((GOLDEN / "tc_rmat_s24_l11.json").read_text())
+    assert gold["num_nodes"] == g.num_nodes and gold["live_edges"] == int(ofs[-1])
+    gpu_case("full size: triangle counts vs the golden oracle counts")
+    t = dg.triangles()
+    ti = dg.tc_i()
+    assert t == gold["tc_u"] and ti == gold["tc_i"]
+    assert t <= ti <= 2 * t
+
+
+def test_full_ppr_128_seeds(big, orc):
+    g, dg, ofs, _, _, _ = big
+    V, d = g.num_nodes, 0.85
+    i_ofs, i_adj = dg.adjacency(DIR_IN)
+    outdeg = np.diff(ofs.astype(np.int64))
+    indeg = np.diff(i_ofs.astype(np.int64))
+    seeds = np.random.default_rng(5).choice(np.flatnonzero(outdeg + indeg > 0), 128,
+                                            replace=False)
+    gpu_case("full size: batched PPR, 128 seeds, 29 and 30 iterations")
+    R29, it29 = dg.ppr_batch(seeds, iters=29, tol=1e-7, batch_width=0)
+    R30, it30 = dg.ppr_batch(seeds, iters=30, tol=1e-7, batch_width=0)
+    assert (it29, it30) == (29, 30)  # no column reaches 1e-7 this early
+    # every column: iterate 30 is one step of the recurrence from iterate 29
+    rows = np.unique(np.concatenate([np.random.default_rng(6).choice(V, 1 << 16), seeds]))
+    rows = rows[indeg[rows] > 0]
+    lens = indeg[rows]
+    starts = i_ofs[rows].astype(np.int64)
+    pos = np.repeat(starts - np.concatenate([[0], np.cumsum(lens)[:-1]]), lens) + \
+        np.arange(int(lens.sum()))
+    nb = i_adj[pos]
+    seg = np.concatenate([[0], np.cumsum(lens)[:-1]])
+    pred = np.add.reduceat(R29[nb].astype(np.float64) / outdeg[nb][:, None], seg, axis=0) * d
+    tele = rows[:, None] == seeds[None, :]
+    pred += (1 - d) * tele
+    got = R30[rows].astype(np.float64)
+    err = np.abs(got - pred).sum(axis=0) / np.abs(pred).sum(axis=0)
+    print(f"PPR one-step residual over {rows.size} rows: max column L1 rel {err.max():.2e}")
+    assert err.max() <= 1e-5
+    del R29
+    gpu_case("full size: PPR columns 0-1 vs the FP64 oracle")
+    ref, _ = orc.ppr(seeds[:2], 30, tol=1e-7)
+    err2 = np.abs(R30[:, :2] - ref).sum(axis=0) / np.abs(ref).sum(axis=0)
+    print(f"PPR vs FP64 oracle: column L1 rel {err2}")
+    assert err2.max() <= 1e-5
 
 
 def test_full_pagerank_pull_equals_push(big):
