@@ -121,9 +121,19 @@ inline void ll_gpu_patches(const ll_mlcsr_ro_graph& Gc, uint32_t dir, int k,
 #endif
 }
 
-// bc_random's sources (bc_random.h:177): the reference's own pick.
-inline int64_t ll_gpu_pick_source(const ll_mlcsr_ro_graph& G, uint64_t, int) {
-  return int64_t(const_cast<ll_mlcsr_ro_graph&>(G).pick_random_node());  // ll_mlcsr_graph.h:677
+// bc_random's sources (bc_random.h:177): the reference task's own draws.
+// bc_random is registered without reverse edges (benchmark.cc:882-884), so its
+// pick_random_node (ll_mlcsr_graph.h:677-682) accepts a draw of
+// ll_rand64_positive (libc rand(), ll_utils.h:322-328) when the vertex has
+// out-edges at the latest level; the GPU task needs the in-edges loaded, so it
+// makes the same draws against the out-edges alone, which gives the reference
+// task's sources.
+inline int64_t ll_gpu_pick_source(const ll_mlcsr_ro_graph& Gc, uint64_t, int) {
+  ll_mlcsr_ro_graph& G = const_cast<ll_mlcsr_ro_graph&>(Gc);
+  for (;;) {
+    const node_t n = ll_rand64_positive() % G.max_nodes();
+    if (G.out().node_exists(n)) return int64_t(n);
+  }
 }
 
 #if !defined(LL_FLAT_VT) && !defined(LL_SLCSR)

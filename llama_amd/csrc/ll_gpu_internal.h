@@ -232,17 +232,18 @@ struct PrHotCold {
   RowBlocks hot_rb, cold_rb;
 };
 
-// Binned PageRank layout of the out-CSR (built by build_pr_bins): destination
-// bins of 2^bb vertices; every out-edge owns one slot of the binned value
-// array, slots grouped by bin.
+// Binned PageRank layout of the out-CSR (built by build_pr_bins): vertices in
+// bins of 2^bb; every out-edge owns one slot of the binned value array, the
+// slots of a destination bin grouped by source bin (chunks), in out-edge
+// order within a chunk.
 struct PrBins {
   bool valid = false;
   int level = -1;
   uint32_t bb = 0, nbins = 0;
-  DevBuf binpos;  // uint32_t[E]: slot of out-edge e
   DevBuf ldst;    // uint16_t[E]: slot -> destination offset in its bin
-  DevBuf srcl;    // uint16_t[E]: out-edge -> source offset in its 2^bb range
-  DevBuf bstart;  // uint64_t[nbins+1]: first slot of each bin
+  DevBuf sidx;    // uint16_t[E]: slot -> source offset in its bin
+  DevBuf bstart;  // uint64_t[nbins+1]: first slot of each destination bin
+  DevBuf ct;      // uint2[nbins * nbins]: [source bin][destination bin] = {first slot, slots}
 };
 
 struct DirMirror {

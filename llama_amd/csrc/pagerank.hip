@@ -315,11 +315,13 @@ void pagerank_device(ll_gpu_graph* g, int level, int iters, double damping, uint
   hipStream_t s = ctx->stream;
   const uint64_t V = g->V;
   // Binned pull over the out-CSR (pr_bins.hip) unless turned off: bins of
-  // 2^pr_bin_bits destinations; automatic (-1) = 13 while E < 2^32.
+  // 2^pr_bin_bits vertices; automatic (-1) = 13 while E < 2^32 and the bins
+  // are at most 2^14 (V <= 2^27).
   {
     long bin_bits = ctx->options.count("pr_bin_bits") ? ctx->options["pr_bin_bits"] : -1;
     const Csr& out = ensure_csr(g, LL_GPU_DIR_OUT, level);
-    if (bin_bits < 0) bin_bits = out.E < (uint64_t(1) << 32) ? 13 : 0;
+    if (bin_bits < 0)
+      bin_bits = (out.E < (uint64_t(1) << 32) && V <= (uint64_t(1) << 27)) ? 13 : 0;
     if (bin_bits > 0) {
       LLG_CHECK(bin_bits >= 12 && bin_bits <= 14, LL_GPU_EINVAL, "pr_bin_bits must be 0 or 12..14");
       const uint32_t* od = ensure_degrees(g, LL_GPU_DIR_OUT, level);

@@ -322,6 +322,12 @@ double ppr_run(ll_gpu_graph* g, int level, const int64_t* seeds, int n, int iter
   PprPass pass;
   ppr_capture<B>(g, in, rb, b, iters, d, tol, pass);
   int done_max = 0;
+  // several batches gather into one device block [V x n] (device-side strided
+  // copies), copied to the host in one piece: a strided copy straight into
+  // host memory moves one V-row at a time
+  DevBuf gathered;
+  const bool direct = ranks_out && n <= B;
+  if (ranks_out && !direct) gathered.alloc(V * uint64_t(n) * 4, s);
   llgpu::EventPair ev;
   hipEvent_t e0 = ev.a, e1 = ev.b;
   double ms_total = 0;
@@ -344,10 +350,19 @@ double ppr_run(ll_gpu_graph* g, int level, const int64_t* seeds, int n, int iter
     done_max = std::max(done_max, done);
     if (ranks_out) {
       const float* src = b.r[done & 1].as<float>();
-      LLG_HIP(hipMemcpy2DAsync(ranks_out + c0, size_t(n) * 4, src, size_t(B) * 4, size_t(m) * 4,
-                               V, hipMemcpyDeviceToHost, s));
-      LLG_HIP(hipStreamSynchronize(s));
+      if (direct) {
+        LLG_HIP(hipMemcpy2DAsync(ranks_out, size_t(n) * 4, src, size_t(B) * 4, size_t(m) * 4,
+                                 V, hipMemcpyDeviceToHost, s));
+        LLG_HIP(hipStreamSynchronize(s));
+      } else {
+        LLG_HIP(hipMemcpy2DAsync(gathered.as<float>() + c0, size_t(n) * 4, src, size_t(B) * 4,
+                                 size_t(m) * 4, V, hipMemcpyDeviceToDevice, s));
+      }
     }
+  }
+  if (ranks_out && !direct) {
+    LLG_HIP(hipMemcpyAsync(ranks_out, gathered.p, V * uint64_t(n) * 4, hipMemcpyDeviceToHost, s));
+    LLG_HIP(hipStreamSynchronize(s));
   }
   if (ctx->timer.on) {
     auto& acc = ctx->timer.acc["ppr_graph"];

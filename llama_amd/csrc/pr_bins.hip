@@ -4,34 +4,34 @@
 // The plain pull gathers one 4-byte contribution per in-edge from a V-entry
 // vector: at R-MAT scale 24 that vector is 16x an XCD's L2, every gather costs
 // a cache-line request and the sweep is bound by request latency, not bytes.
-// Here no iteration gathers at random.  Destinations are cut into bins of
-// 2^bb consecutive vertices; every out-edge (u, v) owns one slot of a binned
-// value array, grouped by v's bin.  One launch per iteration, one workgroup per
-// bin b:
+// Here no iteration gathers at random.  Vertices are cut into bins of 2^bb
+// consecutive IDs.  Every out-edge (u, v) owns one slot of a binned value
+// array; the slots of destination bin b' hold the edges into b', grouped by
+// the source's bin b (a "chunk" (b', b), contiguous), in out-edge order.  One
+// launch per iteration, one workgroup per bin b:
 //   A. streams the slots of bin b -- value (the source's contribution, FP32)
 //      and the destination's offset in the bin (u16) -- and adds each value,
 //      as a 2^-61 fixed-point integer, into a per-destination 64-bit
 //      accumulator in LDS;
 //   B. finalises the bin's vertices: rank = (1-d)/N + d*S, contribution =
 //      rank/outdeg (FP32), kept in LDS;
-//   C. as vertices of the same 2^bb range are sources too, scatters their
-//      contributions along their out-edges into the slots of the next
-//      iteration's binned array (positions precomputed: binpos, and the
-//      source's offset in its range: srcl).
+//   C. as the same vertices are the sources of chunks (b', b) for every b',
+//      writes the next iteration's values into those chunks, chunk by chunk
+//      (each slot names its source's offset in the bin, u16), so the writes
+//      of a wave are contiguous.
 // Integer addition is associative, so a row sum does not depend on the order
-// the slots are added in: every run gives the same bits, whatever the
-// schedule.  Each term is the FP32 contribution truncated to a multiple of
-// 2^-61 (at most 2^-61 absolute per term; the rank sum never exceeds 1, so the
-// accumulators cannot overflow).
+// its slots are added in: every run gives the same bits, whatever the
+// schedule or bin size.  Each term is the FP32 contribution truncated to a
+// multiple of 2^-61 (at most 2^-61 absolute per term; the rank sum never
+// exceeds 1, so the accumulators cannot overflow).
 //
-// Bytes per iteration: 4 B value read + 2 B destination offset + 4 B binpos +
-// 2 B source offset + 4 B value written per edge, and 4 B out-degree + 4 B rank
-// written per vertex.  The layout (binpos, destination offsets, bin starts,
-// source offsets) is built once per snapshot: a per-tile histogram of the
-// destination bins, a scan of the (bin, tile) counts, and a placement pass in
-// which each tile hands out positions within its (bin, tile) ranges with LDS
-// counters -- slots of one bin in any order, since the sums do not depend on
-// it.
+// Bytes per iteration: 4 B value read + 2 B destination offset + 2 B source
+// offset + 4 B value written per edge; 4 B out-degree + 4 B rank written per
+// vertex; 8 B per chunk of the chunk table.  The layout is built once per
+// snapshot: a stable sort of the out-edges by destination bin (the slots), a
+// per-(source bin, destination bin) count scanned into the chunk table.
+#include <algorithm>
+
 #include "ll_gpu_internal.h"
 
 namespace llgpu {
@@ -41,51 +41,11 @@ namespace {
 constexpr int kIterBlock = 1024;
 constexpr int kBuildBlock = 256;
 
-__global__ __launch_bounds__(kBuildBlock) void k_pb_hist(const uint32_t* adj, uint64_t E,
-                                                         uint32_t bb, uint32_t nbins, uint64_t T,
-                                                         uint32_t* th, uint64_t ntiles) {
-  extern __shared__ uint32_t h[];
-  for (uint32_t i = threadIdx.x; i < nbins; i += blockDim.x) h[i] = 0;
-  __syncthreads();
-  const uint64_t e0 = uint64_t(blockIdx.x) * T;
-  const uint64_t e1 = min(E, e0 + T);
-  for (uint64_t e = e0 + threadIdx.x; e < e1; e += blockDim.x) atomicAdd(&h[adj[e] >> bb], 1u);
-  __syncthreads();
-  for (uint32_t i = threadIdx.x; i < nbins; i += blockDim.x)
-    th[uint64_t(i) * ntiles + blockIdx.x] = h[i];
-}
-
-__global__ __launch_bounds__(kBuildBlock) void k_pb_place(const uint32_t* adj, uint64_t E,
-                                                          uint32_t bb, uint32_t nbins, uint64_t T,
-                                                          const uint64_t* toff, uint64_t ntiles,
-                                                          uint32_t* binpos, uint16_t* ldst) {
-  extern __shared__ uint32_t cur[];
-  for (uint32_t i = threadIdx.x; i < nbins; i += blockDim.x) cur[i] = 0;
-  __syncthreads();
-  const uint64_t e0 = uint64_t(blockIdx.x) * T;
-  const uint64_t e1 = min(E, e0 + T);
-  const uint32_t mask = (1u << bb) - 1;
-  for (uint64_t e = e0 + threadIdx.x; e < e1; e += blockDim.x) {
-    const uint32_t v = adj[e];
-    const uint32_t b = v >> bb;
-    const uint32_t loc = atomicAdd(&cur[b], 1u);
-    const uint64_t pos = toff[uint64_t(b) * ntiles + blockIdx.x] + loc;
-    binpos[e] = uint32_t(pos);
-    ldst[pos] = uint16_t(v & mask);
-  }
-}
-
-// bstart[b] = first slot of bin b (the offset of its first tile), bstart[nbins] = E.
-__global__ void k_pb_bstart(const uint64_t* toff, uint64_t ntiles, uint32_t nbins, uint64_t E,
-                            uint64_t* bstart) {
-  const uint64_t b = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (b < nbins) bstart[b] = toff[b * ntiles];
-  if (b == nbins) bstart[b] = E;
-}
-
-// srcl[e] = offset of out-edge e's source in its 2^bb range; one wave per 64
-// rows, all lanes on one row at a time (coalesced for long rows).
-__global__ void k_pb_srcl(const uint64_t* ofs, uint64_t V, uint32_t bb, uint16_t* srcl) {
+// Per out-edge (one wave per 64 rows, all lanes on one row at a time): the
+// destination bin as the sort key, and (source offset << 16 | destination
+// offset) as the value.
+__global__ void k_pb_keys(const uint64_t* ofs, const uint32_t* adj, uint64_t V, uint32_t bb,
+                          uint32_t* key, uint32_t* val) {
   const uint64_t w = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
   const uint32_t lane = threadIdx.x & 63;
   const uint64_t r0 = w * 64;
@@ -94,23 +54,66 @@ __global__ void k_pb_srcl(const uint64_t* ofs, uint64_t V, uint32_t bb, uint16_t
   const uint64_t r1 = min(V, r0 + 64);
   for (uint64_t r = r0; r < r1; r++) {
     const uint64_t lo = ofs[r], hi = ofs[r + 1];
-    for (uint64_t e = lo + lane; e < hi; e += 64) srcl[e] = uint16_t(r & mask);
+    const uint32_t sl = uint32_t(r & mask) << 16;
+    for (uint64_t e = lo + lane; e < hi; e += 64) {
+      const uint32_t v = adj[e];
+      key[e] = v >> bb;
+      val[e] = sl | (v & mask);
+    }
   }
+}
+
+__global__ void k_pb_split(const uint32_t* val, uint64_t E, uint16_t* ldst, uint16_t* sidx) {
+  const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < E) {
+    const uint32_t x = val[i];
+    ldst[i] = uint16_t(x & 0xffffu);
+    sidx[i] = uint16_t(x >> 16);
+  }
+}
+
+// Chunk sizes: th[b' * nb + b] = out-edges from source bin b into
+// destination bin b' (one workgroup per source bin, counts in LDS).
+__global__ __launch_bounds__(kBuildBlock) void k_pb_count(const uint64_t* ofs, const uint32_t* adj,
+                                                          uint64_t V, uint32_t bb, uint32_t nb,
+                                                          uint32_t* th) {
+  extern __shared__ uint32_t h[];
+  for (uint32_t i = threadIdx.x; i < nb; i += blockDim.x) h[i] = 0;
+  __syncthreads();
+  const uint64_t b = blockIdx.x;
+  const uint64_t e0 = ofs[b << bb], e1 = ofs[min(V, (b + 1) << bb)];
+  for (uint64_t e = e0 + threadIdx.x; e < e1; e += blockDim.x) atomicAdd(&h[adj[e] >> bb], 1u);
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < nb; i += blockDim.x) th[uint64_t(i) * nb + b] = h[i];
+}
+
+// Chunk table by source bin: ct[b * nb + b'] = {first slot, slots} of chunk
+// (b', b); bstart[b'] = first slot of destination bin b', bstart[nb] = E.
+__global__ void k_pb_table(const uint32_t* th, const uint64_t* toff, uint32_t nb, uint64_t E,
+                           uint2* ct, uint64_t* bstart) {
+  const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < uint64_t(nb) * nb) {
+    const uint64_t b = i / nb, bp = i % nb;
+    const uint64_t k = bp * nb + b;
+    ct[i] = make_uint2(uint32_t(toff[k]), th[k]);
+  }
+  if (i < nb) bstart[i] = toff[i * nb];
+  if (i == nb) bstart[nb] = E;
 }
 
 struct PbArgs {
   const float* val_in;
   float* val_out;
   const uint16_t* ldst;
+  const uint16_t* sidx;
   const uint64_t* bstart;
-  const uint32_t* binpos;
-  const uint16_t* srcl;
-  const uint64_t* ofs_out;
+  const uint2* ct;
   const uint32_t* outdeg;
-  const float* rank_in;   // read when diff is accumulated
+  const float* rank_in;  // read when diff is accumulated
   float* rank_out;
   double* diff;
   uint64_t V;
+  uint32_t nb;
   double base, d, r0;
 };
 
@@ -126,6 +129,7 @@ __global__ __launch_bounds__(kIterBlock) void k_pb_iter(PbArgs a) {
   const uint64_t b = blockIdx.x;
   const uint64_t v0 = b << BB;
   const uint32_t nrows = uint32_t(min(uint64_t(R), a.V - v0));
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   if (!kInit) {
     for (uint32_t i = threadIdx.x; i < R; i += kIterBlock) acc[i] = 0;
     __syncthreads();
@@ -158,15 +162,27 @@ __global__ __launch_bounds__(kIterBlock) void k_pb_iter(PbArgs a) {
 #pragma unroll
     for (uint32_t j = 0; j < kPer; j++) cl[threadIdx.x + j * kIterBlock] = c[j];
     __syncthreads();
-    const uint64_t e0 = a.ofs_out[v0], e1 = a.ofs_out[v0 + nrows];
-    for (uint64_t e = e0 + threadIdx.x; e < e1; e += kIterBlock)
-      a.val_out[a.binpos[e]] = cl[a.srcl[e]];
+    // chunks (b', b): 64 table entries per wave step, then each chunk's slots
+    // 64 at a time
+    const uint2* row = a.ct + b * a.nb;
+    for (uint32_t c0 = uint32_t(wave) * 64; c0 < a.nb; c0 += (kIterBlock / 64) * 64) {
+      const uint32_t mine = c0 + lane;
+      const uint2 t = mine < a.nb ? row[mine] : make_uint2(0u, 0u);
+      const uint32_t n = min(64u, a.nb - c0);
+      for (uint32_t j = 0; j < n; j++) {
+        const uint32_t st = __shfl(t.x, int(j), 64), len = __shfl(t.y, int(j), 64);
+        for (uint32_t k = uint32_t(lane); k < len; k += 64) {
+          const uint32_t slot = st + k;
+          a.val_out[slot] = cl[a.sidx[slot]];
+        }
+      }
+    }
   }
   if (kDiff) {
     for (int o = 32; o > 0; o >>= 1) dsum += __shfl_down(dsum, o, 64);
     __syncthreads();  // the LDS words below are free again
     float* red = reinterpret_cast<float*>(acc);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = dsum;
+    if (lane == 0) red[wave] = dsum;
     __syncthreads();
     if (threadIdx.x == 0) {
       float t = 0.f;
@@ -187,11 +203,6 @@ void launch_iter(bool init, bool scatter, bool diff, unsigned nb, const PbArgs& 
 
 __global__ void k_zero_f64(double* p) { *p = 0.0; }
 
-__global__ void k_zero_u32(uint32_t* p, uint64_t n) {
-  const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = 0;
-}
-
 }  // namespace
 
 void build_pr_bins(ll_gpu_graph* g, const Csr& out, uint32_t bb, PrBins& pb) {
@@ -199,57 +210,55 @@ void build_pr_bins(ll_gpu_graph* g, const Csr& out, uint32_t bb, PrBins& pb) {
   hipStream_t s = ctx->stream;
   const uint64_t V = g->V, E = out.E;
   LLG_CHECK(E < (uint64_t(1) << 32), LL_GPU_ERANGE, "binned PageRank needs E < 2^32");
-  LLG_CHECK(bb >= 10 && bb <= 16, LL_GPU_EINVAL, "bin bits must be in [10, 16]");
+  LLG_CHECK(bb >= 12 && bb <= 14, LL_GPU_EINVAL, "bin bits must be 12, 13 or 14");
   pb = PrBins();
-  const uint32_t nbins = uint32_t((V + (uint64_t(1) << bb) - 1) >> bb);
-  // tiles large enough that the (bin, tile) table stays small
-  const uint64_t T = std::max<uint64_t>(16384, uint64_t(nbins) * 64);
-  const uint64_t ntiles = std::max<uint64_t>((E + T - 1) / T, 1);
-  const size_t lds = size_t(nbins) * 4;
-  LLG_CHECK(lds <= 160 * 1024, LL_GPU_ERANGE, "too many PageRank bins for one workgroup's LDS");
-  DevBuf th, toff;
-  th.alloc(uint64_t(nbins) * ntiles * 4, s);
-  toff.alloc((uint64_t(nbins) * ntiles + 1) * 8, s);
-  pb.binpos.alloc(std::max<uint64_t>(E, 1) * 4, s);
-  pb.ldst.alloc(std::max<uint64_t>(E, 1) * 2, s);
-  pb.srcl.alloc(std::max<uint64_t>(E, 1) * 2, s);
-  pb.bstart.alloc((uint64_t(nbins) + 1) * 8, s);
-  if (lds > 64 * 1024) {
-    LLG_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_pb_hist),
-                                hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
-    LLG_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_pb_place),
-                                hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
-  }
+  const uint32_t nb = uint32_t((V + (uint64_t(1) << bb) - 1) >> bb);
+  const uint64_t nc = uint64_t(nb) * nb;
+  const size_t lds = size_t(nb) * 4;
+  LLG_CHECK(lds <= 64 * 1024, LL_GPU_ERANGE, "too many PageRank bins for one workgroup's LDS");
   const uint32_t* adj = out.adj.as<uint32_t>();
+  const uint64_t* ofs = out.ofs.as<uint64_t>();
+  pb.ldst.alloc(std::max<uint64_t>(E, 1) * 2, s);
+  pb.sidx.alloc(std::max<uint64_t>(E, 1) * 2, s);
+  pb.bstart.alloc((uint64_t(nb) + 1) * 8, s);
+  pb.ct.alloc(nc * 8, s);
+  TimedLaunch t(ctx, "pr_bins_build");
   {
-    TimedLaunch t(ctx, "pr_bins_build");
-    if (E) {
-      k_pb_hist<<<unsigned(ntiles), kBuildBlock, lds, s>>>(adj, E, bb, nbins, T,
-                                                           th.as<uint32_t>(), ntiles);
-      LLG_HIP(hipGetLastError());
-    } else {
-      k_zero_u32<<<grid_for(uint64_t(nbins) * ntiles, 256), 256, 0, s>>>(
-          th.as<uint32_t>(), uint64_t(nbins) * ntiles);
-    }
-    scan_u32_to_u64(th.as<uint32_t>(), toff.as<uint64_t>(), uint64_t(nbins) * ntiles, s);
-    if (E) {
-      k_pb_place<<<unsigned(ntiles), kBuildBlock, lds, s>>>(adj, E, bb, nbins, T,
-                                                            toff.as<uint64_t>(), ntiles,
-                                                            pb.binpos.as<uint32_t>(),
-                                                            pb.ldst.as<uint16_t>());
-      LLG_HIP(hipGetLastError());
-    }
-    k_pb_bstart<<<grid_for(uint64_t(nbins) + 1, 256), 256, 0, s>>>(toff.as<uint64_t>(), ntiles,
-                                                                   nbins, E,
-                                                                   pb.bstart.as<uint64_t>());
-    k_pb_srcl<<<grid_for(((V + 63) / 64) * 64, 256), 256, 0, s>>>(out.ofs.as<uint64_t>(), V, bb,
-                                                                  pb.srcl.as<uint16_t>());
+    // slots: out-edges stably sorted by destination bin
+    DevBuf key, val, skey, sval;
+    key.alloc(std::max<uint64_t>(E, 1) * 4, s);
+    val.alloc(std::max<uint64_t>(E, 1) * 4, s);
+    skey.alloc(std::max<uint64_t>(E, 1) * 4, s);
+    sval.alloc(std::max<uint64_t>(E, 1) * 4, s);
+    k_pb_keys<<<grid_for(((V + 63) / 64) * 64, 256), 256, 0, s>>>(ofs, adj, V, bb,
+                                                                  key.as<uint32_t>(),
+                                                                  val.as<uint32_t>());
+    LLG_HIP(hipGetLastError());
+    int kbits = 1;
+    while (kbits < 32 && (uint64_t(nb - 1) >> kbits) != 0) kbits++;
+    sort_pairs_u32_u32(key.as<uint32_t>(), skey.as<uint32_t>(), val.as<uint32_t>(),
+                       sval.as<uint32_t>(), E, kbits, s);
+    k_pb_split<<<grid_for(E, 256), 256, 0, s>>>(sval.as<uint32_t>(), E, pb.ldst.as<uint16_t>(),
+                                                pb.sidx.as<uint16_t>());
+    LLG_HIP(hipGetLastError());
+  }
+  {
+    // chunk table: (source bin, destination bin) counts, scanned destination-major
+    DevBuf th, toff;
+    th.alloc(nc * 4, s);
+    toff.alloc((nc + 1) * 8, s);
+    k_pb_count<<<nb, kBuildBlock, lds, s>>>(ofs, adj, V, bb, nb, th.as<uint32_t>());
+    LLG_HIP(hipGetLastError());
+    scan_u32_to_u64(th.as<uint32_t>(), toff.as<uint64_t>(), nc, s);
+    k_pb_table<<<grid_for(std::max<uint64_t>(nc, nb + 1), 256), 256, 0, s>>>(
+        th.as<uint32_t>(), toff.as<uint64_t>(), nb, E, pb.ct.as<uint2>(),
+        pb.bstart.as<uint64_t>());
     LLG_HIP(hipGetLastError());
   }
   pb.valid = true;
   pb.level = out.level;
   pb.bb = bb;
-  pb.nbins = nbins;
+  pb.nbins = nb;
 }
 
 void pagerank_bins(ll_gpu_graph* g, const Csr& out, const uint32_t* outdeg, PrBins& pb,
@@ -266,12 +275,12 @@ void pagerank_bins(ll_gpu_graph* g, const Csr& out, const uint32_t* outdeg, PrBi
   k_zero_f64<<<1, 1, 0, s>>>(diff.as<double>());
   PbArgs a{};
   a.ldst = pb.ldst.as<uint16_t>();
+  a.sidx = pb.sidx.as<uint16_t>();
   a.bstart = pb.bstart.as<uint64_t>();
-  a.binpos = pb.binpos.as<uint32_t>();
-  a.srcl = pb.srcl.as<uint16_t>();
-  a.ofs_out = out.ofs.as<uint64_t>();
+  a.ct = pb.ct.as<uint2>();
   a.outdeg = outdeg;
   a.V = V;
+  a.nb = pb.nbins;
   a.base = (1.0 - damping) / double(V);
   a.d = damping;
   a.r0 = 1.0 / double(V);
@@ -303,11 +312,9 @@ void pagerank_bins(ll_gpu_graph* g, const Csr& out, const uint32_t* outdeg, PrBi
   }
   rank_final = std::move(rank[iters & 1]);
   if (last_diff) {
-    double h = 0;
     LLG_HIP(hipMemcpyAsync(ctx->readback, diff.p, 8, hipMemcpyDeviceToHost, s));
     LLG_HIP(hipStreamSynchronize(s));
-    h = *static_cast<const double*>(ctx->readback);
-    *last_diff = iters > 0 ? h : 0.0;
+    *last_diff = iters > 0 ? *static_cast<const double*>(ctx->readback) : 0.0;
   }
 }
 

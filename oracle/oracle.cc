@@ -334,21 +334,23 @@ int64_t orc_triangles(const orc_graph* g, int level) {
   std::vector<uint64_t> ofs;
   std::vector<uint32_t> adj;
   symmetrise(g, level, ofs, adj);
+  // parallel over list entries (u's i-th neighbour), not vertices, so a hub's
+  // merges spread over the threads; u = the row holding entry i
   int64_t T = 0;
-#pragma omp parallel for reduction(+ : T) schedule(dynamic, 4096)
-  for (uint64_t u = 0; u < g->V; u++) {
-    for (uint64_t i = ofs[u]; i < ofs[u + 1]; i++) {
-      uint64_t v = adj[i];
-      if (u >= v) continue;
-      uint64_t a = ofs[u], ae = ofs[u + 1], b = ofs[v], be = ofs[v + 1];
-      while (a < ae && b < be) {
-        if (adj[a] < adj[b]) a++;
-        else if (adj[b] < adj[a]) b++;
-        else {
-          if (v < adj[b]) T++;
-          a++;
-          b++;
-        }
+  const uint64_t M = ofs[g->V];
+#pragma omp parallel for reduction(+ : T) schedule(dynamic, 1024)
+  for (uint64_t i = 0; i < M; i++) {
+    const uint64_t u = uint64_t(std::upper_bound(ofs.begin(), ofs.end(), i) - ofs.begin()) - 1;
+    const uint64_t v = adj[i];
+    if (u >= v) continue;
+    uint64_t a = ofs[u], ae = ofs[u + 1], b = ofs[v], be = ofs[v + 1];
+    while (a < ae && b < be) {
+      if (adj[a] < adj[b]) a++;
+      else if (adj[b] < adj[a]) b++;
+      else {
+        if (v < adj[b]) T++;
+        a++;
+        b++;
       }
     }
   }
@@ -402,13 +404,17 @@ int64_t orc_tc_i(const orc_graph* g, int level) {
     }
     return c;
   };
+  // parallel over list entries, out-lists then in-lists (see orc_triangles)
   int64_t T = 0;
-#pragma omp parallel for reduction(+ : T) schedule(dynamic, 4096)
-  for (uint64_t u = 0; u < g->V; u++) {
-    for (uint64_t i = oo[u]; i < oo[u + 1]; i++)
-      if (u < oa[i]) T += count_for(u, oa[i]);
-    for (uint64_t i = io[u]; i < io[u + 1]; i++)
-      if (u < ia[i]) T += count_for(u, ia[i]);
+  const uint64_t Mo = oo[g->V], Mi = io[g->V];
+#pragma omp parallel for reduction(+ : T) schedule(dynamic, 1024)
+  for (uint64_t j = 0; j < Mo + Mi; j++) {
+    const bool out = j < Mo;
+    const std::vector<uint64_t>& o = out ? oo : io;
+    const uint64_t i = out ? j : j - Mo;
+    const uint64_t u = uint64_t(std::upper_bound(o.begin(), o.end(), i) - o.begin()) - 1;
+    const uint64_t v = out ? oa[i] : ia[i];
+    if (u < v) T += count_for(u, v);
   }
   return T;
 }

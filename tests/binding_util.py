@@ -11,14 +11,22 @@ GPU_DRIVER = ROOT / "oracle" / "_ref" / "benchmark-memory-wd-gpu"
 CPU_DRIVER = ROOT / "oracle" / "_ref" / "benchmark-memory-wd"
 
 
-def write_levels(tmp: Path, scale: int, levels: int, seed: int) -> list[str]:
+def write_levels(tmp: Path, scale: int, levels: int, seed: int, oneway: bool = False) -> list[str]:
     """A seeded R-MAT base level plus insert-only levels as weighted .xs1
-    files (the reference loader makes one level per file)."""
+    files (the reference loader makes one level per file).  oneway: keep one
+    direction of each reciprocal pair of the base level (the input tc_u's -U
+    loading expects: it adds the reverse of every edge itself)."""
     import numpy as np
     from llama_amd.graph import Edges
     base = Edges.rmat(scale, 8.0, seed=seed, perm_seed=seed + 1).fill_weights(seed)
     s, d = base.arrays()
     V = 1 << scale
+    if oneway:
+        pairs = set(zip(s.tolist(), d.tolist()))
+        keep = np.array([not (u > v and (v, u) in pairs) for u, v in zip(s.tolist(), d.tolist())],
+                        dtype=bool)
+        base = Edges.from_arrays(s[keep], d[keep]).fill_weights(seed)
+        s, d = base.arrays()
     rng = np.random.default_rng(seed)
     have = set(zip(s.tolist(), d.tolist()))
     files = [str(tmp / "l0.xs1")]

@@ -62,8 +62,11 @@ def test_pagerank(files, tmp_path):
 
 
 def test_triangles_single_level(tmp_path):
-    one = write_levels(tmp_path, 10, 1, seed=9)
+    # tc_u needs one level loaded with -U (the loader adds each edge's
+    # reverse); the input keeps one direction of every pair, so the loaded
+    # lists are simple.  gpu_tc symmetrises the same loaded graph itself.
+    one = write_levels(tmp_path, 10, 1, seed=9, oneway=True)
     gpu_case("reference driver: gpu_tc vs tc_u (-U)")
     c = run(GPU_DRIVER, "tc_u", one, tmp_path / "cpu.txt", "-U")
-    g = run(GPU_DRIVER, "gpu_tc", one, tmp_path / "gpu.txt")
+    g = run(GPU_DRIVER, "gpu_tc", one, tmp_path / "gpu.txt", "-U")
     assert c[0] == g[0] and float(g[0]) > 0

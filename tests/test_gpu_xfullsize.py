@@ -125,12 +125,14 @@ def test_full_ppr_128_seeds(big, orc):
     indeg = np.diff(i_ofs.astype(np.int64))
     seeds = np.random.default_rng(5).choice(np.flatnonzero(outdeg + indeg > 0), 128,
                                             replace=False)
-    gpu_case("full size: batched PPR, 128 seeds, 29 and 30 iterations")
-    R29, it29 = dg.ppr_batch(seeds, iters=29, tol=1e-7, batch_width=0)
-    R30, it30 = dg.ppr_batch(seeds, iters=30, tol=1e-7, batch_width=0)
+    gpu_case("full size: batched PPR, 128 seeds, 29 iterations")
+    R29, it29 = dg.ppr_batch(seeds, iters=29, tol=1e-7, batch_width=128)
+    gpu_case("full size: batched PPR, 128 seeds, 30 iterations")
+    R30, it30 = dg.ppr_batch(seeds, iters=30, tol=1e-7, batch_width=64)
     assert (it29, it30) == (29, 30)  # no column reaches 1e-7 this early
+    gpu_case("full size: PPR one-step recurrence on sampled rows")
     # every column: iterate 30 is one step of the recurrence from iterate 29
-    rows = np.unique(np.concatenate([np.random.default_rng(6).choice(V, 1 << 16), seeds]))
+    rows = np.unique(np.concatenate([np.random.default_rng(6).choice(V, 1 << 15), seeds]))
     rows = rows[indeg[rows] > 0]
     lens = indeg[rows]
     starts = i_ofs[rows].astype(np.int64)
