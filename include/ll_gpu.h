@@ -200,15 +200,15 @@ int ll_gpu_bfs(ll_gpu_graph* g, int level, int64_t root, int32_t* levels_out, ui
                uint64_t* edges_examined);
 /* No counterpart in the reference: ll_gpu_bfs_ex for callers that cannot
  * allocate device memory (code under HIP graph capture, serving loops,
- * harnesses that forbid hipMalloc).  The search works in a fixed 48 MiB
+ * harnesses that forbid hipMalloc).  The search works in a fixed 64 MiB
  * device array of the library (one per GPU, held until the process ends) and
  * calls no allocating routine.  levels_dev is a DEVICE pointer (int32 per
  * vertex, max_nodes entries, INT_MAX-1 for unreached) on the context's
  * device; reached and edges_examined (may be NULL) as in ll_gpu_bfs.
  * LL_GPU_ESTATE until ll_gpu_bfs has run once on `level` (it builds the
  * snapshot structures this call reads); LL_GPU_ERANGE when the working
- * memory of max_nodes vertices (about 20.4 bytes each) exceeds the array:
- * above about 2.4 million vertices.  All calls on one GPU share the array:
+ * memory of max_nodes vertices (about 28.4 bytes each) exceeds the array:
+ * above about 2.3 million vertices.  All calls on one GPU share the array:
  * calls that overlap on one GPU (two contexts or threads) race. */
 int ll_gpu_bfs_noalloc(ll_gpu_graph* g, int level, int64_t root, int32_t* levels_dev,
                        uint64_t* reached, uint64_t* edges_examined);

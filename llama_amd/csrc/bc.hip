@@ -32,7 +32,7 @@ namespace llgpu {
 namespace {
 
 constexpr int kBlock = 256;
-constexpr uint32_t kCap = kBfsRowCap;  // row blocks shared with the bottom-up BFS
+constexpr uint32_t kCap = 1024;  // row-block capacity of the path-count and dependency passes
 constexpr uint32_t kMaxEdges = kCap + kCap / 2;
 
 // Per-vertex Brandes state in one 16-byte record, so visiting a neighbour is
@@ -420,7 +420,7 @@ int ll_gpu_bc(ll_gpu_graph* g, int level, const int64_t* sources, int64_t n, flo
       // the search's step count bounds the deepest level (a last step may
       // find nothing): a level without rows has an empty list
       const int32_t D =
-          bfs_core(g, out, in, in_rb, outdeg, src, dead, visited, lv.as<int32_t>()).levels;
+          bfs_core(g, out, in, outdeg, src, dead, visited, lv.as<int32_t>()).levels;
       k_bc_source_init<<<grid_for(V, 256), 256, 0, s>>>(lv.as<int32_t>(), st.as<VState>(), V,
                                                         src);
       k_fill_u64<<<1, 64, 0, s>>>(reinterpret_cast<unsigned long long*>(c_in), 48);

@@ -31,7 +31,6 @@
 #include <optional>
 
 #include "ll_gpu_internal.h"
-#include "rowblocks.h"
 
 namespace llgpu {
 
@@ -40,7 +39,6 @@ namespace {
 constexpr int kBlock = 256;
 constexpr int kEdgesPerBlock = 2048;
 constexpr int32_t kUnreached = INT_MAX - 1;
-constexpr uint32_t kBuCap = kBfsRowCap;
 
 // Level counters, striped over kStripes cache lines: workgroups add their
 // totals to stripe blockIdx % kStripes, so hundreds of thousands of
@@ -53,6 +51,8 @@ struct alignas(128) Counters {
   unsigned long long m_next;     // their out-edges
   unsigned long long examined;   // edges inspected this level
   unsigned long long aux;        // stripe 0: 1 when the search excludes the reach complement
+  unsigned long long n_short;    // stripe 0: open rows walked by a thread each
+  unsigned long long n_long;     // stripe 0: open rows walked by a wave each
 };
 static_assert(sizeof(Counters) == 128, "one stripe per 128-byte line");
 
@@ -96,6 +96,8 @@ __global__ void k_zero_counters(Counters* c) {  // <<<1, kStripes>>>
   c->n_next = 0;
   c->m_next = 0;
   c->examined = 0;
+  c->n_short = 0;
+  c->n_long = 0;
 }
 
 // Counters::aux of stripe 0 = 1 when every vertex the search from `root` can
@@ -211,172 +213,91 @@ __global__ void k_queue_to_bitmap(const uint32_t* q, uint64_t n, uint64_t* bm) {
   }
 }
 
-// Bottom-up level over row blocks of the reverse CSR: a workgroup first
-// checks whether any of its rows is unvisited (else it returns at once), then
-// loads its run of neighbour IDs with coalesced reads and packs "neighbour is
-// in the frontier" into bits with one ballot per 64 edges (the frontier bitmap
-// is V/8 bytes and stays in L2); each unvisited row then tests its bit range.
-// No early exit per vertex, but no divergent per-lane edge walks either --
-// except in items whose open rows hold under a quarter of the edges, where
-// each open row walks its own edges and stops at the first parent.
-// Sum over the workgroup, returned to every thread.
-__device__ inline uint32_t block_sum_u32(uint32_t x) {
-  __shared__ uint32_t part[kBlock / 64];
-  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
-  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = x;
-  __syncthreads();
-  uint32_t t = 0;
-  for (int i = 0; i < kBlock / 64; i++) t += part[i];
-  __syncthreads();  // part may be reused by the next call
-  return t;
-}
+// Bottom-up level over the open rows of the reverse CSR (unvisited and not
+// excluded), each walking its in-neighbours until one is in the frontier
+// (bitmap of V/8 bytes, L2-resident) -- most rows that are found stop after a
+// few edges.  Rows of in-degree <= kLongRow: a thread per row, four
+// neighbours loaded before their frontier words; longer rows: a wave per row,
+// 64 neighbours per step, stopping once a ballot finds one.  Found rows take
+// the next level and their bit of the next frontier.  The row lists are
+// rebuilt from the visited bitmap before each bottom-up level, their lengths
+// stay on the device (stripe 0 of the counters), and both kernels run a
+// fixed grid over them.
+constexpr uint32_t kLongRow = 32;
 
-// `list` (null = every item) holds the row blocks that still had an open row
-// (unvisited, not excluded) after the previous bottom-up level; open[i] is
-// set for list entry i when its item keeps an open row after this level.
-__global__ __launch_bounds__(kBlock) void k_bu_rows(const RowItem* items, const uint32_t* list,
-                                                    uint32_t* open, const uint64_t* ofs,
-                                                    const uint32_t* adj, const uint32_t* outdeg,
-                                                    const uint64_t* fbm, const uint64_t* visited,
-                                                    uint64_t* nbm, int32_t* lv, int32_t next_lv,
-                                                    Counters* c) {
-  constexpr uint32_t kMaxEdges = kBuCap + kBuCap / 2;
-  __shared__ uint64_t rofs[kRowsPerItem + 1];
-  __shared__ uint64_t bits[kMaxEdges / 64];
-  __shared__ unsigned long long red[3][kBlock / 64];
-  const RowItem it = items[list ? list[blockIdx.x] : blockIdx.x];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  unsigned long long n_new = 0, m_new = 0, exam = 0;
-  if (it.npieces == 0) {
-    const uint32_t nrows = it.row_end - it.row_begin;
-    const uint32_t r = it.row_begin + threadIdx.x;
-    const bool unv = threadIdx.x < nrows && !((visited[r >> 6] >> (r & 63)) & 1);
-    if (!__syncthreads_or(unv)) {
-      if (threadIdx.x == 0) open[blockIdx.x] = 0;
-      return;
-    }
-    for (uint32_t i = threadIdx.x; i <= nrows; i += kBlock) rofs[i] = ofs[it.row_begin + i];
-    __syncthreads();
-    const uint64_t e0 = rofs[0];
-    const uint32_t ne = uint32_t(rofs[nrows] - e0);
-    const uint32_t lo = threadIdx.x < nrows ? uint32_t(rofs[threadIdx.x] - e0) : 0u;
-    const uint32_t hi = threadIdx.x < nrows ? uint32_t(rofs[threadIdx.x + 1] - e0) : 0u;
-    // Late levels leave a few open rows per item (mostly vertices the search
-    // never reaches): when their edges are under a quarter of the item's, each
-    // open row walks its own edges and stops at the first frontier parent.
-    const uint32_t open_edges = block_sum_u32(unv ? hi - lo : 0u);
-    bool found = false;
-    if (open_edges * 4 < ne) {
-      if (unv) {
-        for (uint32_t j = lo; j < hi; j++) {
-          const uint32_t u = adj[e0 + j];
-          exam++;
-          if ((fbm[u >> 6] >> (u & 63)) & 1) {
-            found = true;
-            break;
-          }
-        }
-      }
+// Lists of the open rows, short and long, from the visited bitmap: a thread
+// per bitmap word, one list reservation (atomic) per workgroup and list.
+__global__ __launch_bounds__(kBlock) void k_open_rows(const uint64_t* visited, uint64_t words,
+                                                      const uint64_t* ofs, uint32_t* rows_short,
+                                                      uint32_t* rows_long, Counters* c) {
+  __shared__ uint32_t wsum[2][kBlock / 64];
+  __shared__ unsigned long long base[2];
+  const uint64_t w = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
+  const uint64_t open = w < words ? ~visited[w] : 0ull;  // padding bits are set in visited
+  uint32_t ns = 0, nl = 0;
+  uint64_t longm = 0;
+  for (uint64_t b = open; b; b &= b - 1) {
+    const int i = __ffsll((long long)b) - 1;
+    const uint64_t r = w * 64 + uint64_t(i);
+    if (ofs[r + 1] - ofs[r] > kLongRow) {
+      longm |= uint64_t(1) << i;
+      nl++;
     } else {
-      // entries of visited rows are neither loaded nor tested: rowof maps an
-      // entry to its row in the item (written for unvisited rows only)
-      __shared__ uint8_t rowof[kMaxEdges];
-      __shared__ uint8_t openf[kRowsPerItem];
-      if (threadIdx.x < nrows) {
-        openf[threadIdx.x] = unv ? 1 : 0;
-        if (unv)
-          for (uint32_t j = lo; j < hi; j++) rowof[j] = uint8_t(threadIdx.x);
-      }
-      __syncthreads();
-      // all neighbour IDs first, then all bitmap words, so loads overlap
-      constexpr uint32_t kPer = kMaxEdges / kBlock;
-      uint32_t u[kPer];
-      uint64_t fw[kPer];
-      bool need[kPer];
-#pragma unroll
-      for (uint32_t k = 0; k < kPer; k++) {
-        const uint32_t i = k * kBlock + threadIdx.x;
-        need[k] = false;
-        if (i < ne) {
-          const uint32_t rr = rowof[i];
-          need[k] = rr < nrows && openf[rr] && i >= uint32_t(rofs[rr] - e0) &&
-                    i < uint32_t(rofs[rr + 1] - e0);
-        }
-        u[k] = need[k] ? adj[e0 + i] : 0u;
-      }
-#pragma unroll
-      for (uint32_t k = 0; k < kPer; k++) fw[k] = need[k] ? fbm[u[k] >> 6] : 0ull;
-#pragma unroll
-      for (uint32_t k = 0; k < kPer; k++) {
-        const uint64_t m = __ballot(need[k] && ((fw[k] >> (u[k] & 63)) & 1));
-        if (lane == 0) bits[k * (kBlock / 64) + wave] = m;
-      }
-      __syncthreads();
-      if (unv) {
-        exam = hi - lo;
-        for (uint32_t j = lo; j < hi && !found;) {
-          const uint32_t w = j >> 6, b0 = j & 63;
-          const uint32_t nb = min(64u - b0, hi - j);
-          const uint64_t mask = (nb == 64 ? ~0ull : ((1ull << nb) - 1)) << b0;
-          found = (bits[w] & mask) != 0;
-          j += nb;
-        }
-      }
-    }
-    if (found) {
-      if (lv) lv[r] = next_lv;
-      n_new = 1;  // a stream row belongs to this item only: no other writer sets its bit
-      m_new = outdeg[r];
-    }
-    const bool still = __syncthreads_or(unv && !found);
-    if (threadIdx.x == 0) open[blockIdx.x] = still ? 1u : 0u;
-    // the wave's rows are consecutive: its found bits land in at most two
-    // bitmap words, set with one atomic each
-    const uint64_t fm = __ballot(found);
-    if (lane == 0 && fm) {
-      const uint32_t r0 = it.row_begin + (threadIdx.x & ~63u);
-      const uint32_t sh = r0 & 63;
-      atomicOr((unsigned long long*)&nbm[r0 >> 6], (unsigned long long)(fm << sh));
-      if (sh && (fm >> (64 - sh)))
-        atomicOr((unsigned long long*)&nbm[(r0 >> 6) + 1], (unsigned long long)(fm >> (64 - sh)));
-    }
-  } else {
-    const uint32_t r = it.row_begin;
-    if ((visited[r >> 6] >> (r & 63)) & 1) {
-      if (threadIdx.x == 0) open[blockIdx.x] = 0;
-      return;
-    }
-    const uint64_t lo = ofs[r] + uint64_t(it.piece) * kPiece;
-    const uint64_t hi = min(ofs[r + 1], lo + kPiece);
-    bool f = false;
-    for (uint64_t e = lo + threadIdx.x; e < hi && !f; e += kBlock) {
-      const uint32_t u = adj[e];
-      f = (fbm[u >> 6] >> (u & 63)) & 1;
-    }
-    const bool found = __syncthreads_or(f);
-    if (threadIdx.x == 0) {
-      open[blockIdx.x] = found ? 0u : 1u;
-      exam = hi - lo;
-      if (found) {
-        if (lv) lv[r] = next_lv;
-        const uint64_t bit = uint64_t(1) << (r & 63);
-        const unsigned long long old = atomicOr((unsigned long long*)&nbm[r >> 6], (unsigned long long)bit);
-        if (!(old & bit)) {
-          n_new = 1;
-          m_new = outdeg[r];
-        }
-      }
+      ns++;
     }
   }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  uint32_t xs = ns, xl = nl;
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t ys = __shfl_up(xs, o, 64), yl = __shfl_up(xl, o, 64);
+    if (lane >= o) {
+      xs += ys;
+      xl += yl;
+    }
+  }
+  if (lane == 63) {
+    wsum[0][wave] = xs;
+    wsum[1][wave] = xl;
+  }
+  __syncthreads();
+  if (threadIdx.x < 2) {
+    uint32_t t = 0;
+    for (int k = 0; k < kBlock / 64; k++) t += wsum[threadIdx.x][k];
+    unsigned long long* dst = threadIdx.x == 0 ? &c->n_short : &c->n_long;
+    base[threadIdx.x] = t ? atomicAdd(dst, (unsigned long long)t) : 0ull;
+  }
+  __syncthreads();
+  unsigned long long os = base[0] + xs - ns, ol = base[1] + xl - nl;
+  for (int k = 0; k < wave; k++) {
+    os += wsum[0][k];
+    ol += wsum[1][k];
+  }
+  for (uint64_t b = open; b; b &= b - 1) {
+    const int i = __ffsll((long long)b) - 1;
+    const uint32_t r = uint32_t(w * 64 + uint64_t(i));
+    if ((longm >> i) & 1) rows_long[ol++] = r;
+    else rows_short[os++] = r;
+  }
+}
+
+__device__ inline void found_row(uint32_t r, int32_t* lv, int32_t next_lv, uint64_t* nbm) {
+  if (lv) lv[r] = next_lv;
+  atomicOr((unsigned long long*)&nbm[r >> 6], (unsigned long long)(uint64_t(1) << (r & 63)));
+}
+
+// Add the workgroup's level totals to its stripe (one atomic per total).
+__device__ inline void block_counters(unsigned long long n_new, unsigned long long m_new,
+                                      unsigned long long exam, Counters* c) {
+  __shared__ unsigned long long red[3][kBlock / 64];
   for (int o = 32; o > 0; o >>= 1) {
     n_new += __shfl_down(n_new, o, 64);
     m_new += __shfl_down(m_new, o, 64);
     exam += __shfl_down(exam, o, 64);
   }
-  if (lane == 0) {
-    red[0][wave] = n_new;
-    red[1][wave] = m_new;
-    red[2][wave] = exam;
+  if ((threadIdx.x & 63) == 0) {
+    red[0][threadIdx.x >> 6] = n_new;
+    red[1][threadIdx.x >> 6] = m_new;
+    red[2][threadIdx.x >> 6] = exam;
   }
   __syncthreads();
   if (threadIdx.x < 3) {
@@ -391,17 +312,78 @@ __global__ __launch_bounds__(kBlock) void k_bu_rows(const RowItem* items, const 
   }
 }
 
+__global__ __launch_bounds__(kBlock) void k_bu_short(const uint32_t* rows, const uint64_t* ofs,
+                                                     const uint32_t* adj, const uint32_t* outdeg,
+                                                     const uint64_t* fbm, uint64_t* nbm,
+                                                     int32_t* lv, int32_t next_lv, Counters* c) {
+  const uint64_t n = c->n_short;  // stripe 0
+  unsigned long long n_new = 0, m_new = 0, exam = 0;
+  for (uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x; i < n;
+       i += uint64_t(gridDim.x) * kBlock) {
+    const uint32_t r = rows[i];
+    uint64_t e = ofs[r];
+    const uint64_t hi = ofs[r + 1];
+    bool found = false;
+    while (e < hi && !found) {
+      const uint32_t m = uint32_t(min(uint64_t(4), hi - e));
+      uint32_t u[4];
+      uint64_t fw[4];
+#pragma unroll
+      for (uint32_t k = 0; k < 4; k++) u[k] = adj[e + (k < m ? k : 0)];
+#pragma unroll
+      for (uint32_t k = 0; k < 4; k++) fw[k] = fbm[u[k] >> 6];
+      uint32_t hit = 0;  // 1 + position of the first neighbour in the frontier
+#pragma unroll
+      for (int k = 3; k >= 0; k--)
+        if (uint32_t(k) < m && ((fw[k] >> (u[k] & 63)) & 1)) hit = uint32_t(k) + 1;
+      found = hit != 0;
+      exam += found ? hit : m;
+      e += m;
+    }
+    if (found) {
+      found_row(r, lv, next_lv, nbm);
+      n_new++;
+      m_new += outdeg[r];
+    }
+  }
+  block_counters(n_new, m_new, exam, c);
+}
+
+__global__ __launch_bounds__(kBlock) void k_bu_long(const uint32_t* rows, const uint64_t* ofs,
+                                                    const uint32_t* adj, const uint32_t* outdeg,
+                                                    const uint64_t* fbm, uint64_t* nbm,
+                                                    int32_t* lv, int32_t next_lv, Counters* c) {
+  const uint64_t n = c->n_long;  // stripe 0
+  const int lane = threadIdx.x & 63;
+  const uint64_t nw = uint64_t(gridDim.x) * (kBlock / 64);
+  unsigned long long n_new = 0, m_new = 0, exam = 0;
+  for (uint64_t i = uint64_t(blockIdx.x) * (kBlock / 64) + (threadIdx.x >> 6); i < n; i += nw) {
+    const uint32_t r = rows[i];
+    const uint64_t lo = ofs[r], hi = ofs[r + 1];
+    bool found = false;
+    for (uint64_t e0 = lo; e0 < hi && !found; e0 += 64) {
+      const uint64_t e = e0 + uint64_t(lane);
+      bool f = false;
+      if (e < hi) {
+        const uint32_t u = adj[e];
+        f = (fbm[u >> 6] >> (u & 63)) & 1;
+      }
+      const uint64_t m = __ballot(f);
+      found = m != 0;
+      if (lane == 0) exam += found ? uint64_t(__ffsll((long long)m)) : min(uint64_t(64), hi - e0);
+    }
+    if (found && lane == 0) {
+      found_row(r, lv, next_lv, nbm);
+      n_new++;
+      m_new += outdeg[r];
+    }
+  }
+  block_counters(n_new, m_new, exam, c);
+}
+
 __global__ void k_visited_or(uint64_t* visited, const uint64_t* nbm, uint64_t words) {
   const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i < words) visited[i] |= nbm[i];
-}
-
-// Next bottom-up list: the entries of `list` (null = identity) whose open flag
-// is set, in order (pos = exclusive scan of the flags).
-__global__ void k_list_compact(const uint32_t* list, const uint32_t* open, const uint64_t* pos,
-                               uint64_t n, uint32_t* out) {
-  const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i < n && open[i]) out[pos[i]] = list ? list[i] : uint32_t(i);
 }
 
 // Highest (out + in) degree, smallest ID among equals: max of deg << 32 | ~v.
@@ -551,10 +533,9 @@ struct BfsWork {
   uint32_t* qdeg;
   uint64_t* qofs;
   Counters* cnt;
-  void* scan_tmp;       // scan_tmp_bytes(max(V, row blocks))
-  uint32_t* list[2];    // open row blocks of the bottom-up levels (row-block count each)
-  uint32_t* open;       // per list entry: still open after the level
-  uint64_t* open_pos;   // exclusive scan of `open` (row-block count + 1)
+  void* scan_tmp;       // scan_tmp_bytes(V)
+  uint32_t* rows_short; // open rows of a bottom-up level (V entries each)
+  uint32_t* rows_long;
   bool timed;
 };
 
@@ -570,9 +551,9 @@ struct MaybeTimed {
 // `notr` (may be null) is the complement of a reach set R (closed under
 // successors) with `e_reach` out-edges inside R: when the root's successors
 // all lie in R, rows outside R are excluded instead of `excluded`.
-BfsResult bfs_core_ws(ll_gpu_graph* g, const Csr& fwd, const Csr& rev, const RowBlocks& rev_rb,
-                      const uint32_t* fdeg, uint32_t root, const uint64_t* excluded,
-                      const uint64_t* notr, uint64_t e_reach, const BfsWork& w, int32_t* lv) {
+BfsResult bfs_core_ws(ll_gpu_graph* g, const Csr& fwd, const Csr& rev, const uint32_t* fdeg,
+                      uint32_t root, const uint64_t* excluded, const uint64_t* notr,
+                      uint64_t e_reach, const BfsWork& w, int32_t* lv) {
   ll_gpu_ctx* ctx = g->ctx;
   hipStream_t s = ctx->stream;
   const uint64_t V = g->V;
@@ -592,10 +573,7 @@ BfsResult bfs_core_ws(ll_gpu_graph* g, const Csr& fwd, const Csr& rev, const Row
   LLG_HIP(hipGetLastError());
   Counters h = read_counters(ctx, w.cnt);
   BfsResult r;
-  // open row blocks of the bottom-up levels: all of them at first
-  const uint32_t* list = nullptr;
-  uint64_t n_list = rev_rb.n;
-  int lc = 0;
+  const unsigned bu_grid = 8u * unsigned(ctx->num_cus);
   uint64_t nq = h.n_next;       // queue entries (vertices with out-edges)
   uint64_t n_f = 1;             // frontier vertices
   uint64_t m_f = h.m_next;      // frontier out-edges
@@ -618,21 +596,20 @@ BfsResult bfs_core_ws(ll_gpu_graph* g, const Csr& fwd, const Csr& rev, const Row
                                                           fbm);
     }
     k_zero_counters<<<1, kStripes, 0, s>>>(w.cnt);
-    uint64_t n_open = 0;
     if (bottom_up) {
       k_fill_u64<<<grid_for(words, 256), 256, 0, s>>>(nbm, words, 0);
-      if (n_list) {
-        {
-          MaybeTimed t(ctx, "k_bu_step", w.timed);
-          k_bu_rows<<<unsigned(n_list), kBlock, 0, s>>>(
-              rev_rb.items.as<RowItem>(), list, w.open, rev.ofs.as<uint64_t>(),
-              rev.adj.as<uint32_t>(), fdeg, fbm, w.visited, nbm, lv, lvl + 1, w.cnt);
-        }
-        k_visited_or<<<grid_for(words, 256), 256, 0, s>>>(w.visited, nbm, words);
-        scan_u32_to_u64_ws(w.open, w.open_pos, n_list, w.scan_tmp, s);
-        k_list_compact<<<grid_for(n_list, 256), 256, 0, s>>>(list, w.open, w.open_pos, n_list,
-                                                             w.list[lc]);
+      {
+        MaybeTimed t(ctx, "k_bu_step", w.timed);
+        k_open_rows<<<grid_for(words, kBlock), kBlock, 0, s>>>(
+            w.visited, words, rev.ofs.as<uint64_t>(), w.rows_short, w.rows_long, w.cnt);
+        k_bu_short<<<bu_grid, kBlock, 0, s>>>(w.rows_short, rev.ofs.as<uint64_t>(),
+                                              rev.adj.as<uint32_t>(), fdeg, fbm, nbm, lv,
+                                              lvl + 1, w.cnt);
+        k_bu_long<<<bu_grid, kBlock, 0, s>>>(w.rows_long, rev.ofs.as<uint64_t>(),
+                                             rev.adj.as<uint32_t>(), fdeg, fbm, nbm, lv,
+                                             lvl + 1, w.cnt);
       }
+      k_visited_or<<<grid_for(words, 256), 256, 0, s>>>(w.visited, nbm, words);
     } else {
       k_queue_degrees<<<grid_for(nq, 256), 256, 0, s>>>(w.q[cur], nq, fdeg,
                                                         w.qdeg);
@@ -644,17 +621,11 @@ BfsResult bfs_core_ws(ll_gpu_graph* g, const Csr& fwd, const Csr& rev, const Row
           w.q[cur ^ 1], w.cnt);
     }
     LLG_HIP(hipGetLastError());
-    const bool listed = bottom_up && n_list > 0;
-    h = listed ? read_counters(ctx, w.cnt, w.open_pos + n_list, &n_open) : read_counters(ctx, w.cnt);
+    h = read_counters(ctx, w.cnt);
     r.examined += h.examined;
     const uint64_t n_next = h.n_next;
     lvl++;
     m_visited += h.m_next;
-    if (listed) {  // the open row blocks after this level
-      list = w.list[lc];
-      lc ^= 1;
-      n_list = n_open;
-    }
     if (bottom_up) {
       r.reached += n_next;
       const bool shrinking = n_next < n_f;
@@ -685,9 +656,8 @@ BfsResult bfs_core_ws(ll_gpu_graph* g, const Csr& fwd, const Csr& rev, const Row
 
 }  // namespace
 
-BfsResult bfs_core(ll_gpu_graph* g, const Csr& fwd, const Csr& rev, const RowBlocks& rev_rb,
-                   const uint32_t* fdeg, uint32_t root, const uint64_t* excluded,
-                   DevBuf& visited, int32_t* lv) {
+BfsResult bfs_core(ll_gpu_graph* g, const Csr& fwd, const Csr& rev, const uint32_t* fdeg,
+                   uint32_t root, const uint64_t* excluded, DevBuf& visited, int32_t* lv) {
   hipStream_t s = g->ctx->stream;
   const uint64_t V = g->V;
   const uint64_t words = (V + 63) / 64;
@@ -700,19 +670,15 @@ BfsResult bfs_core(ll_gpu_graph* g, const Csr& fwd, const Csr& rev, const RowBlo
   qdeg.alloc(V * 4, s);
   qofs.alloc((V + 1) * 8, s);
   cnt.alloc(kStripes * sizeof(Counters), s);
-  const uint64_t ni = std::max<uint64_t>(rev_rb.n, 1);
-  const size_t sb = scan_tmp_bytes(std::max(V, ni));
+  const size_t sb = scan_tmp_bytes(V);
   if (sb) stmp.alloc(sb, s);
-  DevBuf lists[2], open, open_pos;
-  lists[0].alloc(ni * 4, s);
-  lists[1].alloc(ni * 4, s);
-  open.alloc(ni * 4, s);
-  open_pos.alloc((ni + 1) * 8, s);
+  DevBuf rs, rl;
+  rs.alloc(std::max<uint64_t>(V, 1) * 4, s);
+  rl.alloc(std::max<uint64_t>(V, 1) * 4, s);
   BfsWork w{visited.as<uint64_t>(), fbm.as<uint64_t>(), nbm.as<uint64_t>(),
             {q[0].as<uint32_t>(), q[1].as<uint32_t>()}, qdeg.as<uint32_t>(), qofs.as<uint64_t>(),
-            cnt.as<Counters>(), stmp.p, {lists[0].as<uint32_t>(), lists[1].as<uint32_t>()},
-            open.as<uint32_t>(), open_pos.as<uint64_t>(), true};
-  return bfs_core_ws(g, fwd, rev, rev_rb, fdeg, root, excluded, nullptr, 0, w, lv);
+            cnt.as<Counters>(), stmp.p, rs.as<uint32_t>(), rl.as<uint32_t>(), true};
+  return bfs_core_ws(g, fwd, rev, fdeg, root, excluded, nullptr, 0, w, lv);
 }
 
 // Working memory of one single-source search, carved into 256-byte aligned
@@ -721,10 +687,9 @@ BfsResult bfs_core(ll_gpu_graph* g, const Csr& fwd, const Csr& rev, const RowBlo
 // temporary memory.
 namespace {
 struct WsLayout {
-  size_t visited, fbm, nbm, q0, q1, qdeg, qofs, cnt, red, stmp, l0, l1, open, open_pos, total;
+  size_t visited, fbm, nbm, q0, q1, qdeg, qofs, cnt, red, stmp, rs, rl, total;
 };
-WsLayout ws_layout(uint64_t V, uint64_t items) {
-  const uint64_t ni = std::max<uint64_t>(items, 1);
+WsLayout ws_layout(uint64_t V) {
   const uint64_t words = (V + 63) / 64;
   auto up = [](size_t x) { return (x + 255) & ~size_t(255); };
   WsLayout L{};
@@ -739,17 +704,15 @@ WsLayout ws_layout(uint64_t V, uint64_t items) {
   L.qofs = take((V + 1) * 8);
   L.cnt = take(kStripes * sizeof(Counters));
   L.red = take(8);
-  L.stmp = take(scan_tmp_bytes(std::max(V, ni)));
-  L.l0 = take(ni * 4);
-  L.l1 = take(ni * 4);
-  L.open = take(ni * 4);
-  L.open_pos = take((ni + 1) * 8);
+  L.stmp = take(scan_tmp_bytes(V));
+  L.rs = take(std::max<uint64_t>(V, 1) * 4);
+  L.rl = take(std::max<uint64_t>(V, 1) * 4);
   L.total = o;
   return L;
 }
 }  // namespace
 
-size_t bfs_work_bytes(uint64_t V, uint64_t items) { return ws_layout(V, items).total; }
+size_t bfs_work_bytes(uint64_t V) { return ws_layout(V).total; }
 
 namespace {
 
@@ -776,7 +739,7 @@ void build_reach(ll_gpu_graph* g, int level, const BfsPrepared& p) {
   const uint64_t key = *static_cast<const uint64_t*>(ctx->readback);
   const uint32_t hub = 0xffffffffu - uint32_t(key & 0xffffffffu);
   DevBuf visited;
-  (void)bfs_core(g, *p.out, *p.in, *p.in_rb, p.outdeg, hub, p.dead, visited, nullptr);
+  (void)bfs_core(g, *p.out, *p.in, p.outdeg, hub, p.dead, visited, nullptr);
   g->reach_notr.alloc(words * 8, s);
   k_reach_complement<<<grid_for(words, 256), 256, 0, s>>>(visited.as<uint64_t>(), p.dead, V,
                                                           words, g->reach_notr.as<uint64_t>());
@@ -800,13 +763,9 @@ bool bfs_peek(ll_gpu_graph* g, int level, BfsPrepared* p) {
     if (!c[d]->valid || (!g->slcsr && c[d]->level != level)) return false;
   }
   if (g->deg_level[LL_GPU_DIR_OUT] != level || !g->deg[LL_GPU_DIR_OUT].p) return false;
-  const RowBlocks* rb = nullptr;
-  for (const RowBlocks& r : g->rb[LL_GPU_DIR_IN])
-    if (r.valid && r.cap == kBuCap && r.piece == kPiece) rb = &r;
-  if (!rb) return false;
   if (g->dead_level != c[LL_GPU_DIR_IN]->level || !g->dead_in.p) return false;
   if (g->reach_level != level || !g->reach_notr.p) return false;
-  *p = BfsPrepared{c[LL_GPU_DIR_OUT], c[LL_GPU_DIR_IN], g->deg[LL_GPU_DIR_OUT].as<uint32_t>(), rb,
+  *p = BfsPrepared{c[LL_GPU_DIR_OUT], c[LL_GPU_DIR_IN], g->deg[LL_GPU_DIR_OUT].as<uint32_t>(),
                    g->dead_in.as<uint64_t>(), g->reach_notr.as<uint64_t>(), g->reach_e};
   return true;
 }
@@ -817,7 +776,6 @@ BfsPrepared bfs_prepare(ll_gpu_graph* g, int level) {
   p.out = &ensure_csr(g, LL_GPU_DIR_OUT, level);
   p.in = &ensure_csr(g, LL_GPU_DIR_IN, level);
   p.outdeg = ensure_degrees(g, LL_GPU_DIR_OUT, level);
-  p.in_rb = &ensure_rowblocks(g, LL_GPU_DIR_IN, level, kBuCap, kPiece);
   p.dead = ensure_dead_rows(g, *p.in);
   if (g->reach_level != level || !g->reach_notr.p) {
     TimedLaunch t(g->ctx, "bfs_reach_index");
@@ -832,20 +790,18 @@ BfsResult bfs_single_ws(ll_gpu_graph* g, const BfsPrepared& p, uint32_t root, vo
                         int32_t* lv, bool timed) {
   hipStream_t s = g->ctx->stream;
   const uint64_t V = g->V;
-  const WsLayout L = ws_layout(V, p.in_rb->n);
+  const WsLayout L = ws_layout(V);
   char* b = static_cast<char*>(ws);
   BfsWork w{reinterpret_cast<uint64_t*>(b + L.visited), reinterpret_cast<uint64_t*>(b + L.fbm),
             reinterpret_cast<uint64_t*>(b + L.nbm),
             {reinterpret_cast<uint32_t*>(b + L.q0), reinterpret_cast<uint32_t*>(b + L.q1)},
             reinterpret_cast<uint32_t*>(b + L.qdeg), reinterpret_cast<uint64_t*>(b + L.qofs),
             reinterpret_cast<Counters*>(b + L.cnt), b + L.stmp,
-            {reinterpret_cast<uint32_t*>(b + L.l0), reinterpret_cast<uint32_t*>(b + L.l1)},
-            reinterpret_cast<uint32_t*>(b + L.open), reinterpret_cast<uint64_t*>(b + L.open_pos),
-            timed};
+            reinterpret_cast<uint32_t*>(b + L.rs), reinterpret_cast<uint32_t*>(b + L.rl), timed};
   // option bfs_reach 0: dead rows only (for comparison)
   const auto& opts = g->ctx->options;
   const bool use_reach = !opts.count("bfs_reach") || opts.at("bfs_reach") != 0;
-  BfsResult r = bfs_core_ws(g, *p.out, *p.in, *p.in_rb, p.outdeg, root, p.dead,
+  BfsResult r = bfs_core_ws(g, *p.out, *p.in, p.outdeg, root, p.dead,
                             use_reach ? p.notr : nullptr, p.e_reach, w, lv);
   // top-down levels count only queued vertices: recount the reached set
   unsigned long long* red = reinterpret_cast<unsigned long long*>(b + L.red);
@@ -865,7 +821,7 @@ BfsResult bfs_device(ll_gpu_graph* g, int level, uint32_t root, DevBuf& lv) {
   const BfsPrepared p = bfs_prepare(g, level);
   lv.alloc(g->V * 4, s);
   DevBuf ws;
-  ws.alloc(bfs_work_bytes(g->V, p.in_rb->n), s);
+  ws.alloc(bfs_work_bytes(g->V), s);
   return bfs_single_ws(g, p, root, ws.p, lv.as<int32_t>(), true);
 }
 

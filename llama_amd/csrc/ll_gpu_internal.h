@@ -135,7 +135,7 @@ constexpr size_t kReadbackBytes = size_t(64) << 10;
 
 namespace llgpu {
 // Size of the fixed device array ll_gpu_bfs_noalloc works in (one per GPU).
-constexpr size_t kScratchBytes = size_t(48) << 20;
+constexpr size_t kScratchBytes = size_t(64) << 20;
 // Address of that array on `device` (the current device), looked up once.
 void* scratch_setup(int device);
 }  // namespace llgpu
@@ -325,27 +325,23 @@ struct BfsResult {
   int levels = 0;
 };
 // Direction-optimising BFS from `root` over `fwd` (top-down) / `rev`
-// (bottom-up, on its row blocks `rev_rb` of capacity kBfsRowCap); `fdeg` = degrees of `fwd`; `excluded` (bitmap or null) is never
+// (bottom-up); `fdeg` = degrees of `fwd`; `excluded` (bitmap or null) is never
 // entered; `visited` receives the reached set (plus excluded/padding bits);
 // `lv` (int32[V] or null) the levels.
-BfsResult bfs_core(ll_gpu_graph* g, const Csr& fwd, const Csr& rev, const RowBlocks& rev_rb,
-                   const uint32_t* fdeg, uint32_t root, const uint64_t* excluded,
-                   DevBuf& visited, int32_t* lv);
+BfsResult bfs_core(ll_gpu_graph* g, const Csr& fwd, const Csr& rev, const uint32_t* fdeg,
+                   uint32_t root, const uint64_t* excluded, DevBuf& visited, int32_t* lv);
 // Bitmap of the in-CSR rows (of `in`, the snapshot's in-edges) with no
 // entry: no search from another vertex can reach them, so single-source
-// searches pass it as `excluded` and the bottom-up steps skip their row
-// blocks.  Cached per snapshot level.
+// searches pass it as `excluded` and the bottom-up steps never open them.
+// Cached per snapshot level.
 const uint64_t* ensure_dead_rows(ll_gpu_graph* g, const Csr& in);
-constexpr uint32_t kBfsRowCap = 1024;  // row-block capacity used by bottom-up steps
 
 // What a single-source search reads besides its working memory: both CSRs
-// of the snapshot, the out-degrees, the in-CSR row blocks and the dead-row
-// bitmap.
+// of the snapshot, the out-degrees, the dead-row bitmap and the reach set.
 struct BfsPrepared {
   const Csr* out = nullptr;
   const Csr* in = nullptr;
   const uint32_t* outdeg = nullptr;
-  const RowBlocks* in_rb = nullptr;
   const uint64_t* dead = nullptr;
   const uint64_t* notr = nullptr;  // complement of the snapshot's reach set
   uint64_t e_reach = 0;            // out-edges inside the reach set
@@ -354,9 +350,8 @@ struct BfsPrepared {
 BfsPrepared bfs_prepare(ll_gpu_graph* g, int level);
 // The cached structures of `level` only: false when any is missing.
 bool bfs_peek(ll_gpu_graph* g, int level, BfsPrepared* p);
-// Bytes of one search's working memory for V vertices and `items` in-CSR
-// row blocks.
-size_t bfs_work_bytes(uint64_t V, uint64_t items);
+// Bytes of one search's working memory for V vertices.
+size_t bfs_work_bytes(uint64_t V);
 // Single-source search in `ws` (bfs_work_bytes(V) bytes, 256-byte aligned):
 // levels into lv (device int32[V]); reached = vertices with a level.  Makes
 // no device allocation; `timed` records per-kernel times when profiling.

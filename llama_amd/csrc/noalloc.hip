@@ -3,7 +3,7 @@
 // Callers that cannot allocate device memory (code under HIP graph capture,
 // tight serving loops, harnesses that forbid hipMalloc) get the same search as
 // ll_gpu_bfs_ex with its working memory in `scratch`, a fixed array of this
-// library on every GPU it runs on, held until the process ends.  In host code
+// library on every GPU it runs on (64 MiB), held until the process ends.  In host code
 // the array's name is not a device address: ll_gpu_create asks HIP for the
 // array's address on its device (hipGetSymbolAddress, once per device, with
 // that device current) and every use goes through that address.  All calls on
@@ -52,20 +52,15 @@ int ll_gpu_bfs_noalloc(ll_gpu_graph* g, int level, int64_t root, int32_t* levels
     LLG_CHECK(levels_dev != nullptr, LL_GPU_EINVAL, "levels_dev (device int32[max_nodes]) is required");
     LLG_CHECK(root >= 0 && uint64_t(root) < g->V, LL_GPU_EINVAL, "root outside [0, max_nodes)");
     LLG_CHECK(ctx->scratch != nullptr, LL_GPU_ESTATE, "context has no scratch address");
-    // bound by the vertex count first (row blocks <= vertices + entries / 8192)
-    LLG_CHECK(llgpu::bfs_work_bytes(g->V, 1) <= llgpu::kScratchBytes, LL_GPU_ERANGE,
-              "ll_gpu_bfs_noalloc: " + std::to_string(g->V) + " vertices need more working "
-              "memory than the fixed array's " + std::to_string(llgpu::kScratchBytes) + " bytes");
+    const size_t need = llgpu::bfs_work_bytes(g->V);
+    LLG_CHECK(need <= llgpu::kScratchBytes, LL_GPU_ERANGE,
+              "ll_gpu_bfs_noalloc: " + std::to_string(g->V) + " vertices need " +
+                  std::to_string(need) + " bytes of working memory; the fixed array holds " +
+                  std::to_string(llgpu::kScratchBytes));
     llgpu::BfsPrepared p;
     LLG_CHECK(llgpu::bfs_peek(g, level, &p), LL_GPU_ESTATE,
               "ll_gpu_bfs_noalloc: the snapshot's BFS structures are not built; call ll_gpu_bfs "
               "on this level once first");
-    const size_t need = llgpu::bfs_work_bytes(g->V, p.in_rb->n);
-    LLG_CHECK(need <= llgpu::kScratchBytes, LL_GPU_ERANGE,
-              "ll_gpu_bfs_noalloc: " + std::to_string(g->V) + " vertices and " +
-                  std::to_string(p.in_rb->n) + " row blocks need " + std::to_string(need) +
-                  " bytes of working memory; the fixed array holds " +
-                  std::to_string(llgpu::kScratchBytes));
     const llgpu::BfsResult r =
         llgpu::bfs_single_ws(g, p, uint32_t(root), ctx->scratch, levels_dev, false);
     if (reached) *reached = r.reached;

@@ -7,13 +7,16 @@
 // scanned recursively, then every tile rescans its items from its prefix.
 // Integer sums, so the result does not depend on the schedule.
 //
-// Radix sort: least-significant digit first, 8-bit digits, stable.  One
-// pre-pass counts every digit position at once; passes whose digit is the
-// same for all keys are skipped (the host reads the 8 x 256 counts back).  A
-// pass counts digits per tile, scans the counts digit-major (digit, tile)
-// and scatters: each round of 256 keys ranks a key among the equal digits
-// of its wave with 8 ballots, adds the counts of the waves before it (LDS)
-// and the tile's running offset of that digit.
+// Radix sort: least-significant digit first, stable.  One pre-pass counts
+// every 8-bit digit position at once; byte positions whose value is the same
+// for all keys are dropped (the host reads the 8 x 256 counts back), and the
+// remaining bits are cut into the fewest passes of at most 8 bits, as even
+// as possible (11 bits: 6 + 5), so each tile's runs per digit stay long
+// enough to fill cache lines.  A pass counts digits per tile, scans the counts
+// digit-major (digit, tile) and scatters: each round of 256 keys ranks a key
+// among the equal digits of its wave with one ballot per digit bit, adds the
+// counts of the waves before it (LDS) and the tile's running offset of that
+// digit.
 #include <algorithm>
 
 #include "ll_gpu_internal.h"
@@ -142,30 +145,33 @@ __global__ __launch_bounds__(kB) void k_rs_all_hist(const K* keys, uint64_t n,
     if (h[p][threadIdx.x]) atomicAdd(&ghist[p * 256 + threadIdx.x], (unsigned long long)h[p][threadIdx.x]);
 }
 
-// Per-tile digit counts, digit-major: th[d * ntiles + tile].
+// Per-tile digit counts, digit-major: th[d * ntiles + tile] for the
+// 2^bits digits of (key >> shift).
 template <typename K>
 __global__ __launch_bounds__(kB) void k_rs_tile_hist(const K* keys, uint64_t n, int shift,
-                                                     uint32_t* th, uint64_t ntiles) {
+                                                     int bits, uint32_t* th, uint64_t ntiles) {
   __shared__ uint32_t h[256];
   h[threadIdx.x] = 0;
   __syncthreads();
+  const uint32_t dm = (1u << bits) - 1;
   const uint64_t base = uint64_t(blockIdx.x) * kTile;
   for (int r = 0; r < kRounds; r++) {
     const uint64_t i = base + uint64_t(r) * kB + threadIdx.x;
-    if (i < n) atomicAdd(&h[uint32_t(keys[i] >> shift) & 255u], 1u);
+    if (i < n) atomicAdd(&h[uint32_t(keys[i] >> shift) & dm], 1u);
   }
   __syncthreads();
-  th[uint64_t(threadIdx.x) * ntiles + blockIdx.x] = h[threadIdx.x];
+  if (threadIdx.x <= dm) th[uint64_t(threadIdx.x) * ntiles + blockIdx.x] = h[threadIdx.x];
 }
 
 template <typename K, bool kVals>
 __global__ __launch_bounds__(kB) void k_rs_scatter(const K* kin, const uint32_t* vin, K* kout,
-                                                   uint32_t* vout, uint64_t n, int shift,
+                                                   uint32_t* vout, uint64_t n, int shift, int bits,
                                                    const uint64_t* tofs, uint64_t ntiles) {
   __shared__ uint64_t run[256];
   __shared__ uint32_t wcnt[kB / 64][256];
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-  run[t] = tofs[uint64_t(t) * ntiles + blockIdx.x];
+  const uint32_t dm = (1u << bits) - 1;
+  run[t] = uint32_t(t) <= dm ? tofs[uint64_t(t) * ntiles + blockIdx.x] : 0;
   for (int w = 0; w < kB / 64; w++) wcnt[w][t] = 0;
   __syncthreads();
   const uint64_t base = uint64_t(blockIdx.x) * kTile;
@@ -176,10 +182,9 @@ __global__ __launch_bounds__(kB) void k_rs_scatter(const K* kin, const uint32_t*
     const K key = valid ? kin[i] : K(0);
     uint32_t val = 0;
     if (kVals && valid) val = vin[i];
-    const uint32_t d = uint32_t(key >> shift) & 255u;
+    const uint32_t d = uint32_t(key >> shift) & dm;
     uint64_t peers = __ballot(valid);
-#pragma unroll
-    for (int b = 0; b < 8; b++) {
+    for (int b = 0; b < bits; b++) {
       const uint64_t m = __ballot((d >> b) & 1u);
       peers &= ((d >> b) & 1u) ? m : ~m;
     }
@@ -224,12 +229,45 @@ void radix_sort(const K* kin, K* kout, const uint32_t* vin, uint32_t* vout, uint
   std::vector<unsigned long long> h(P * 256);
   LLG_HIP(hipMemcpyAsync(h.data(), gh.p, gh.bytes, hipMemcpyDeviceToHost, s));
   LLG_HIP(hipStreamSynchronize(s));
-  std::vector<int> passes;
-  for (int p = 0; p < npass_max; p++) {
+  // the bits to sort: bytes whose value differs between keys, cut into
+  // the fewest even passes of at most 8 bits within each run of such bytes
+  struct Pass {
+    int shift, bits;
+  };
+  std::vector<Pass> passes;
+  for (int p = 0; p < npass_max;) {
     bool trivial = false;
     for (int d = 0; d < 256; d++)
       if (h[p * 256 + d] == n) trivial = true;
-    if (!trivial) passes.push_back(p);
+    if (trivial) {
+      p++;
+      continue;
+    }
+    int q = p;  // bytes p..q-1 all vary
+    while (q < npass_max) {
+      bool t2 = false;
+      for (int d = 0; d < 256; d++)
+        if (h[q * 256 + d] == n) t2 = true;
+      if (t2) break;
+      q++;
+    }
+    int lo = 8 * p, hi = std::min(8 * q, end_bit);
+    // the highest byte may carry fewer significant bits than 8
+    if (q == npass_max && hi > lo) {
+      const int top = 8 * (q - 1);
+      int tb = 0;
+      for (int d = 0; d < 256; d++)
+        if (h[(q - 1) * 256 + d]) tb = std::max(tb, 32 - __builtin_clz(unsigned(d) | 1u));
+      hi = std::min(hi, top + tb);
+    }
+    const int nbits = hi - lo;
+    const int np = (nbits + 7) / 8;
+    for (int k = 0, at = lo; k < np; k++) {
+      const int w = (nbits - (at - lo) + (np - k) - 1) / (np - k);
+      passes.push_back({at, w});
+      at += w;
+    }
+    p = q;
   }
   const uint64_t nt = ntiles_of(n);
   if (passes.empty()) {
@@ -254,11 +292,12 @@ void radix_sort(const K* kin, K* kout, const uint32_t* vin, uint32_t* vout, uint
     const bool to_out = ((np - 1 - j) % 2) == 0;
     K* kdst = to_out ? kout : kt.as<K>();
     uint32_t* vdst = to_out ? vout : vt.as<uint32_t>();
-    const int shift = 8 * passes[j];
-    k_rs_tile_hist<K><<<unsigned(nt), kB, 0, s>>>(ksrc, n, shift, th.as<uint32_t>(), nt);
+    const int shift = passes[j].shift, bits = passes[j].bits;
+    const uint64_t nd = uint64_t(1) << bits;
+    k_rs_tile_hist<K><<<unsigned(nt), kB, 0, s>>>(ksrc, n, shift, bits, th.as<uint32_t>(), nt);
     LLG_HIP(hipGetLastError());
-    scan_rec(SrcU32{th.as<uint32_t>()}, 256 * nt, tofs.as<uint64_t>(), stmp.as<uint64_t>(), s);
-    k_rs_scatter<K, kVals><<<unsigned(nt), kB, 0, s>>>(ksrc, vsrc, kdst, vdst, n, shift,
+    scan_rec(SrcU32{th.as<uint32_t>()}, nd * nt, tofs.as<uint64_t>(), stmp.as<uint64_t>(), s);
+    k_rs_scatter<K, kVals><<<unsigned(nt), kB, 0, s>>>(ksrc, vsrc, kdst, vdst, n, shift, bits,
                                                        tofs.as<uint64_t>(), nt);
     LLG_HIP(hipGetLastError());
     ksrc = kdst;

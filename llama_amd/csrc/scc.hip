@@ -17,7 +17,6 @@
 //      component.  Each round retires at least the component of the largest
 //      active ID, so the loop terminates.
 #include "ll_gpu_internal.h"
-#include "rowblocks.h"
 
 namespace llgpu {
 
@@ -232,10 +231,8 @@ SccResult scc_device(ll_gpu_graph* g, int level, DevBuf& labels64) {
     k_inactive_bitmap<<<grid_for(words * 64, 256), 256, 0, s>>>(label.as<uint32_t>(), V,
                                                                excl.as<uint64_t>());
     LLG_HIP(hipGetLastError());
-    const RowBlocks& in_rb = ensure_rowblocks(g, LL_GPU_DIR_IN, level, kBfsRowCap, kPiece);
-    const RowBlocks& out_rb = ensure_rowblocks(g, LL_GPU_DIR_OUT, level, kBfsRowCap, kPiece);
-    bfs_core(g, out, in, in_rb, dout, pivot, excl.as<uint64_t>(), fw, nullptr);
-    bfs_core(g, in, out, out_rb, din, pivot, excl.as<uint64_t>(), bw, nullptr);
+    bfs_core(g, out, in, dout, pivot, excl.as<uint64_t>(), fw, nullptr);
+    bfs_core(g, in, out, din, pivot, excl.as<uint64_t>(), bw, nullptr);
     r.sweeps += 2;
     DevBuf mn;
     mn.alloc(8, s);
