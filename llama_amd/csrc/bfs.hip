@@ -226,26 +226,29 @@ __global__ void k_queue_to_bitmap(const uint32_t* q, uint64_t n, uint64_t* bm) {
 constexpr uint32_t kLongRow = 32;
 
 // Lists of the open rows, short and long, from the visited bitmap: a thread
-// per bitmap word, one list reservation (atomic) per workgroup and list.
+// per bitmap word, one list reservation (atomic) per workgroup and list.  A
+// row is long by the snapshot's long-row bitmap when one is given, else by
+// its offsets.
 __global__ __launch_bounds__(kBlock) void k_open_rows(const uint64_t* visited, uint64_t words,
-                                                      const uint64_t* ofs, uint32_t* rows_short,
-                                                      uint32_t* rows_long, Counters* c) {
+                                                      const uint64_t* ofs,
+                                                      const uint64_t* longrows,
+                                                      uint32_t* rows_short, uint32_t* rows_long,
+                                                      Counters* c) {
   __shared__ uint32_t wsum[2][kBlock / 64];
   __shared__ unsigned long long base[2];
   const uint64_t w = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
   const uint64_t open = w < words ? ~visited[w] : 0ull;  // padding bits are set in visited
-  uint32_t ns = 0, nl = 0;
   uint64_t longm = 0;
-  for (uint64_t b = open; b; b &= b - 1) {
-    const int i = __ffsll((long long)b) - 1;
-    const uint64_t r = w * 64 + uint64_t(i);
-    if (ofs[r + 1] - ofs[r] > kLongRow) {
-      longm |= uint64_t(1) << i;
-      nl++;
-    } else {
-      ns++;
+  if (longrows) {
+    longm = w < words ? open & longrows[w] : 0ull;
+  } else {
+    for (uint64_t b = open; b; b &= b - 1) {
+      const int i = __ffsll((long long)b) - 1;
+      const uint64_t r = w * 64 + uint64_t(i);
+      if (ofs[r + 1] - ofs[r] > kLongRow) longm |= uint64_t(1) << i;
     }
   }
+  const uint32_t nl = uint32_t(__popcll(longm)), ns = uint32_t(__popcll(open & ~longm));
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   uint32_t xs = ns, xl = nl;
   for (int o = 1; o < 64; o <<= 1) {
@@ -464,13 +467,18 @@ __global__ __launch_bounds__(kBlock) void k_bitmap_to_queue(const uint64_t* bm, 
 }
 
 // Word w of the dead-row bitmap: bit i set when row 64w+i of the CSR has no
-// entry.  A wave covers 64 consecutive rows (blocks of 256 start at multiples
-// of 64), so one ballot forms the word.
-__global__ void k_dead_rows(const uint64_t* ofs, uint64_t V, uint64_t* out) {
+// entry; word w of the long-row bitmap (at out + words): bit i set when the
+// row has more than kLongRow entries.  A wave covers 64 consecutive rows
+// (blocks of 256 start at multiples of 64), so one ballot forms each word.
+__global__ void k_dead_rows(const uint64_t* ofs, uint64_t V, uint64_t words, uint64_t* out) {
   const uint64_t v = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  const bool dead = v < V && ofs[v + 1] == ofs[v];
-  const uint64_t m = __ballot(dead);
-  if ((threadIdx.x & 63) == 0 && v < V) out[v >> 6] = m;
+  const uint64_t n = v < V ? ofs[v + 1] - ofs[v] : 0;
+  const uint64_t m = __ballot(v < V && n == 0);
+  const uint64_t l = __ballot(n > kLongRow);
+  if ((threadIdx.x & 63) == 0 && v < V) {
+    out[v >> 6] = m;
+    out[words + (v >> 6)] = l;
+  }
 }
 
 // Reached vertices (grid-stride over a fixed grid: one atomic per workgroup).
@@ -511,8 +519,9 @@ const uint64_t* ensure_dead_rows(ll_gpu_graph* g, const Csr& in) {
   if (g->dead_level != in.level || !g->dead_in.p) {
     hipStream_t s = g->ctx->stream;
     const uint64_t V = g->V;
-    g->dead_in.alloc(((V + 63) / 64) * 8, s);
-    k_dead_rows<<<grid_for(V, 256), 256, 0, s>>>(in.ofs.as<uint64_t>(), V,
+    const uint64_t words = (V + 63) / 64;
+    g->dead_in.alloc(2 * words * 8, s);  // dead rows, then long rows
+    k_dead_rows<<<grid_for(V, 256), 256, 0, s>>>(in.ofs.as<uint64_t>(), V, words,
                                                  g->dead_in.as<uint64_t>());
     LLG_HIP(hipGetLastError());
     g->dead_level = in.level;
@@ -537,6 +546,7 @@ struct BfsWork {
   uint32_t* rows_short; // open rows of a bottom-up level (V entries each)
   uint32_t* rows_long;
   bool timed;
+  const uint64_t* longrows = nullptr;  // long rows of `rev`, when known
 };
 
 // Launch timer that records only for searches that may allocate.
@@ -601,7 +611,8 @@ BfsResult bfs_core_ws(ll_gpu_graph* g, const Csr& fwd, const Csr& rev, const uin
       {
         MaybeTimed t(ctx, "k_bu_step", w.timed);
         k_open_rows<<<grid_for(words, kBlock), kBlock, 0, s>>>(
-            w.visited, words, rev.ofs.as<uint64_t>(), w.rows_short, w.rows_long, w.cnt);
+            w.visited, words, rev.ofs.as<uint64_t>(), w.longrows, w.rows_short, w.rows_long,
+            w.cnt);
         k_bu_short<<<bu_grid, kBlock, 0, s>>>(w.rows_short, rev.ofs.as<uint64_t>(),
                                               rev.adj.as<uint32_t>(), fdeg, fbm, nbm, lv,
                                               lvl + 1, w.cnt);
@@ -766,7 +777,8 @@ bool bfs_peek(ll_gpu_graph* g, int level, BfsPrepared* p) {
   if (g->dead_level != c[LL_GPU_DIR_IN]->level || !g->dead_in.p) return false;
   if (g->reach_level != level || !g->reach_notr.p) return false;
   *p = BfsPrepared{c[LL_GPU_DIR_OUT], c[LL_GPU_DIR_IN], g->deg[LL_GPU_DIR_OUT].as<uint32_t>(),
-                   g->dead_in.as<uint64_t>(), g->reach_notr.as<uint64_t>(), g->reach_e};
+                   g->dead_in.as<uint64_t>(), g->reach_notr.as<uint64_t>(), g->reach_e,
+                   g->dead_in.as<uint64_t>() + (g->V + 63) / 64};
   return true;
 }
 
@@ -777,6 +789,7 @@ BfsPrepared bfs_prepare(ll_gpu_graph* g, int level) {
   p.in = &ensure_csr(g, LL_GPU_DIR_IN, level);
   p.outdeg = ensure_degrees(g, LL_GPU_DIR_OUT, level);
   p.dead = ensure_dead_rows(g, *p.in);
+  p.longrows = p.dead + (g->V + 63) / 64;
   if (g->reach_level != level || !g->reach_notr.p) {
     TimedLaunch t(g->ctx, "bfs_reach_index");
     build_reach(g, level, p);
@@ -797,7 +810,8 @@ BfsResult bfs_single_ws(ll_gpu_graph* g, const BfsPrepared& p, uint32_t root, vo
             {reinterpret_cast<uint32_t*>(b + L.q0), reinterpret_cast<uint32_t*>(b + L.q1)},
             reinterpret_cast<uint32_t*>(b + L.qdeg), reinterpret_cast<uint64_t*>(b + L.qofs),
             reinterpret_cast<Counters*>(b + L.cnt), b + L.stmp,
-            reinterpret_cast<uint32_t*>(b + L.rs), reinterpret_cast<uint32_t*>(b + L.rl), timed};
+            reinterpret_cast<uint32_t*>(b + L.rs), reinterpret_cast<uint32_t*>(b + L.rl), timed,
+            p.longrows};
   // option bfs_reach 0: dead rows only (for comparison)
   const auto& opts = g->ctx->options;
   const bool use_reach = !opts.count("bfs_reach") || opts.at("bfs_reach") != 0;

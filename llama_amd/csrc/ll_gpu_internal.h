@@ -286,7 +286,8 @@ struct ll_gpu_graph {
   // the back), W and the largest row element count, built with the DAG
   llgpu::DevBuf sym_rows;
   uint64_t sym_nblock = 0, sym_nglobal = 0, sym_work = 0, sym_maxrow = 0;
-  // rows of the in-CSR with no live in-edge (bitmap), excluded from BFS
+  // rows of the in-CSR with no live in-edge (bitmap, excluded from BFS), then
+  // the rows with more than 32 (bitmap, walked a wave per row bottom-up)
   int dead_level = -1;
   llgpu::DevBuf dead_in;
   // complement of the vertices reachable from the hub (bitmap), their out-edges
@@ -345,6 +346,7 @@ struct BfsPrepared {
   const uint64_t* dead = nullptr;
   const uint64_t* notr = nullptr;  // complement of the snapshot's reach set
   uint64_t e_reach = 0;            // out-edges inside the reach set
+  const uint64_t* longrows = nullptr;  // in-CSR rows of more than 32 entries
 };
 // Builds what is missing (allocates).
 BfsPrepared bfs_prepare(ll_gpu_graph* g, int level);
@@ -393,6 +395,9 @@ void sort_keys_u64(const uint64_t* keys_in, uint64_t* keys_out, uint64_t n, int 
                    hipStream_t s);
 void sort_pairs_u32_u32(const uint32_t* keys_in, uint32_t* keys_out, const uint32_t* vals_in,
                         uint32_t* vals_out, uint64_t n, int end_bit, hipStream_t s);
+// The same with bits [0, end_bit) sorted as given (no pre-pass over the keys).
+void sort_pairs_u32_u32_bits(const uint32_t* keys_in, uint32_t* keys_out, const uint32_t* vals_in,
+                             uint32_t* vals_out, uint64_t n, int end_bit, hipStream_t s);
 uint64_t unique_u64(const uint64_t* in, uint64_t* out, uint64_t n, hipStream_t s);
 
 // Sum x over the workgroup and add it to *p with a single atomic.  Every

@@ -28,9 +28,11 @@
 // Bytes per iteration: 4 B value read + 2 B destination offset + 2 B source
 // offset + 4 B value written per edge; 4 B out-degree + 4 B rank written per
 // vertex; 8 B per chunk of the chunk table.  The layout is built once per
-// snapshot: a stable sort of the out-edges by destination bin (the slots), a
-// per-(source bin, destination bin) count scanned into the chunk table.
+// snapshot: the out-edges stably sorted by destination bin (the slots), and
+// per-(piece of a source bin, destination bin) counts scanned
+// destination-major into the chunk table.
 #include <algorithm>
+#include <vector>
 
 #include "ll_gpu_internal.h"
 
@@ -41,25 +43,61 @@ namespace {
 constexpr int kIterBlock = 1024;
 constexpr int kBuildBlock = 256;
 
-// Per out-edge (one wave per 64 rows, all lanes on one row at a time): the
-// destination bin as the sort key, and (source offset << 16 | destination
-// offset) as the value.
-__global__ void k_pb_keys(const uint64_t* ofs, const uint32_t* adj, uint64_t V, uint32_t bb,
-                          uint32_t* key, uint32_t* val) {
-  const uint64_t w = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
-  const uint32_t lane = threadIdx.x & 63;
-  const uint64_t r0 = w * 64;
-  if (r0 >= V) return;
-  const uint32_t mask = (1u << bb) - 1;
-  const uint64_t r1 = min(V, r0 + 64);
-  for (uint64_t r = r0; r < r1; r++) {
-    const uint64_t lo = ofs[r], hi = ofs[r + 1];
-    const uint32_t sl = uint32_t(r & mask) << 16;
-    for (uint64_t e = lo + lane; e < hi; e += 64) {
-      const uint32_t v = adj[e];
-      key[e] = v >> bb;
-      val[e] = sl | (v & mask);
+// Build items: pieces of one source bin's out-edges, a bin holding more than
+// max(2^18, E / 2048) cut into equal pieces, so that no workgroup of the build
+// walks a hub bin alone.
+constexpr uint64_t kMinPiece = uint64_t(1) << 18;
+
+struct PbItem {
+  uint32_t bin, e0, e1, pad;  // out-edges [e0, e1) of source bin `bin`
+};
+
+// bo[b] = first out-edge of source bin b, b = 0..nb (bo[nb] = E).
+__global__ void k_pb_bounds(const uint64_t* ofs, uint64_t V, uint32_t bb, uint32_t nb,
+                            uint64_t* bo) {
+  const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i <= nb) bo[i] = ofs[min(V, i << bb)];
+}
+
+// th[b' * ni + i] = out-edges of item i into destination bin b' (counts in LDS).
+__global__ __launch_bounds__(kBuildBlock) void k_pb_count(const uint32_t* adj, const PbItem* items,
+                                                          uint32_t ni, uint32_t bb, uint32_t nb,
+                                                          uint32_t* th) {
+  extern __shared__ uint32_t h[];
+  for (uint32_t i = threadIdx.x; i < nb; i += blockDim.x) h[i] = 0;
+  __syncthreads();
+  const PbItem it = items[blockIdx.x];
+  for (uint64_t e = it.e0 + threadIdx.x; e < it.e1; e += blockDim.x) atomicAdd(&h[adj[e] >> bb], 1u);
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < nb; i += blockDim.x) th[uint64_t(i) * ni + blockIdx.x] = h[i];
+}
+
+// Per out-edge of an item: the destination bin as the sort key and (source
+// offset << 16 | destination offset) as the value; the source row is found
+// by binary search over the bin's row offsets, staged in LDS.  Reads and
+// writes follow the edge order (coalesced).
+template <uint32_t BB>
+__global__ __launch_bounds__(kBuildBlock) void k_pb_keys(const uint64_t* ofs, const uint32_t* adj,
+                                                         const PbItem* items, uint64_t V,
+                                                         uint32_t* key, uint32_t* val) {
+  constexpr uint32_t R = 1u << BB;
+  __shared__ uint32_t ro[R + 1];
+  const PbItem it = items[blockIdx.x];
+  const uint64_t v0 = uint64_t(it.bin) << BB;
+  const uint32_t nrows = uint32_t(min(uint64_t(R), V - v0));
+  for (uint32_t i = threadIdx.x; i <= nrows; i += blockDim.x) ro[i] = uint32_t(ofs[v0 + i]);
+  __syncthreads();
+  for (uint64_t e = it.e0 + threadIdx.x; e < it.e1; e += blockDim.x) {
+    const uint32_t v = adj[e];
+    // the row r with ro[r] <= e < ro[r + 1]; ro[0] <= e0 <= e < e1 <= ro[nrows]
+    uint32_t lo = 0, hi = nrows;
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (ro[mid] <= e) lo = mid;
+      else hi = mid;
     }
+    key[e] = v >> BB;
+    val[e] = lo << 16 | (v & (R - 1));
   }
 }
 
@@ -72,32 +110,21 @@ __global__ void k_pb_split(const uint32_t* val, uint64_t E, uint16_t* ldst, uint
   }
 }
 
-// Chunk sizes: th[b' * nb + b] = out-edges from source bin b into
-// destination bin b' (one workgroup per source bin, counts in LDS).
-__global__ __launch_bounds__(kBuildBlock) void k_pb_count(const uint64_t* ofs, const uint32_t* adj,
-                                                          uint64_t V, uint32_t bb, uint32_t nb,
-                                                          uint32_t* th) {
-  extern __shared__ uint32_t h[];
-  for (uint32_t i = threadIdx.x; i < nb; i += blockDim.x) h[i] = 0;
-  __syncthreads();
-  const uint64_t b = blockIdx.x;
-  const uint64_t e0 = ofs[b << bb], e1 = ofs[min(V, (b + 1) << bb)];
-  for (uint64_t e = e0 + threadIdx.x; e < e1; e += blockDim.x) atomicAdd(&h[adj[e] >> bb], 1u);
-  __syncthreads();
-  for (uint32_t i = threadIdx.x; i < nb; i += blockDim.x) th[uint64_t(i) * nb + b] = h[i];
-}
-
 // Chunk table by source bin: ct[b * nb + b'] = {first slot, slots} of chunk
-// (b', b); bstart[b'] = first slot of destination bin b', bstart[nb] = E.
-__global__ void k_pb_table(const uint32_t* th, const uint64_t* toff, uint32_t nb, uint64_t E,
-                           uint2* ct, uint64_t* bstart) {
+// (b', b), from X, the exclusive prefix of the item counts in (destination
+// bin, item) order: the slots are the out-edges stably sorted by destination
+// bin, so item i's out-edges into b' start at slot X[b' * ni + i], and the
+// items of source bin b (first[b] .. first[b + 1] - 1, consecutive) fill
+// chunk (b', b); bstart[b'] = first slot of destination bin b', bstart[nb] = E.
+__global__ void k_pb_table(const uint64_t* X, const uint32_t* first, uint32_t ni, uint32_t nb,
+                           uint64_t E, uint2* ct, uint64_t* bstart) {
   const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i < uint64_t(nb) * nb) {
     const uint64_t b = i / nb, bp = i % nb;
-    const uint64_t k = bp * nb + b;
-    ct[i] = make_uint2(uint32_t(toff[k]), th[k]);
+    const uint64_t a = X[bp * ni + first[b]], z = X[bp * ni + first[b + 1]];
+    ct[i] = make_uint2(uint32_t(a), uint32_t(z - a));
   }
-  if (i < nb) bstart[i] = toff[i * nb];
+  if (i < nb) bstart[i] = X[i * ni];
   if (i == nb) bstart[nb] = E;
 }
 
@@ -223,6 +250,39 @@ void build_pr_bins(ll_gpu_graph* g, const Csr& out, uint32_t bb, PrBins& pb) {
   pb.bstart.alloc((uint64_t(nb) + 1) * 8, s);
   pb.ct.alloc(nc * 8, s);
   TimedLaunch t(ctx, "pr_bins_build");
+  // items: each source bin cut into pieces of at most `piece` out-edges
+  std::vector<uint64_t> hb(size_t(nb) + 1);
+  {
+    DevBuf bo;
+    bo.alloc((uint64_t(nb) + 1) * 8, s);
+    k_pb_bounds<<<grid_for(uint64_t(nb) + 1, 256), 256, 0, s>>>(ofs, V, bb, nb, bo.as<uint64_t>());
+    LLG_HIP(hipGetLastError());
+    LLG_HIP(hipMemcpyAsync(hb.data(), bo.p, hb.size() * 8, hipMemcpyDeviceToHost, s));
+    LLG_HIP(hipStreamSynchronize(s));
+  }
+  const uint64_t piece = std::max(kMinPiece, (E + 2047) / 2048);
+  std::vector<PbItem> items;
+  std::vector<uint32_t> first(size_t(nb) + 1);
+  for (uint32_t b = 0; b < nb; b++) {
+    first[b] = uint32_t(items.size());
+    const uint64_t n = hb[b + 1] - hb[b];
+    const uint64_t k = std::max<uint64_t>(1, (n + piece - 1) / piece);
+    for (uint64_t j = 0; j < k; j++)
+      items.push_back({b, uint32_t(hb[b] + n * j / k), uint32_t(hb[b] + n * (j + 1) / k), 0u});
+  }
+  first[nb] = uint32_t(items.size());
+  const uint32_t ni = uint32_t(items.size());
+  const uint64_t nx = uint64_t(nb) * ni;
+  DevBuf d_items, d_first, th, X;
+  d_items.alloc(items.size() * sizeof(PbItem), s);
+  d_first.alloc(first.size() * 4, s);
+  LLG_HIP(hipMemcpyAsync(d_items.p, items.data(), d_items.bytes, hipMemcpyHostToDevice, s));
+  LLG_HIP(hipMemcpyAsync(d_first.p, first.data(), d_first.bytes, hipMemcpyHostToDevice, s));
+  th.alloc(nx * 4, s);
+  X.alloc((nx + 1) * 8, s);
+  k_pb_count<<<ni, kBuildBlock, lds, s>>>(adj, d_items.as<PbItem>(), ni, bb, nb, th.as<uint32_t>());
+  LLG_HIP(hipGetLastError());
+  scan_u32_to_u64(th.as<uint32_t>(), X.as<uint64_t>(), nx, s);
   {
     // slots: out-edges stably sorted by destination bin
     DevBuf key, val, skey, sval;
@@ -230,31 +290,27 @@ void build_pr_bins(ll_gpu_graph* g, const Csr& out, uint32_t bb, PrBins& pb) {
     val.alloc(std::max<uint64_t>(E, 1) * 4, s);
     skey.alloc(std::max<uint64_t>(E, 1) * 4, s);
     sval.alloc(std::max<uint64_t>(E, 1) * 4, s);
-    k_pb_keys<<<grid_for(((V + 63) / 64) * 64, 256), 256, 0, s>>>(ofs, adj, V, bb,
-                                                                  key.as<uint32_t>(),
-                                                                  val.as<uint32_t>());
+    const PbItem* di = d_items.as<PbItem>();
+    switch (bb) {
+      case 12: k_pb_keys<12><<<ni, kBuildBlock, 0, s>>>(ofs, adj, di, V, key.as<uint32_t>(), val.as<uint32_t>()); break;
+      case 13: k_pb_keys<13><<<ni, kBuildBlock, 0, s>>>(ofs, adj, di, V, key.as<uint32_t>(), val.as<uint32_t>()); break;
+      default: k_pb_keys<14><<<ni, kBuildBlock, 0, s>>>(ofs, adj, di, V, key.as<uint32_t>(), val.as<uint32_t>()); break;
+    }
     LLG_HIP(hipGetLastError());
     int kbits = 1;
     while (kbits < 32 && (uint64_t(nb - 1) >> kbits) != 0) kbits++;
-    sort_pairs_u32_u32(key.as<uint32_t>(), skey.as<uint32_t>(), val.as<uint32_t>(),
-                       sval.as<uint32_t>(), E, kbits, s);
+    sort_pairs_u32_u32_bits(key.as<uint32_t>(), skey.as<uint32_t>(), val.as<uint32_t>(),
+                            sval.as<uint32_t>(), E, kbits, s);
     k_pb_split<<<grid_for(E, 256), 256, 0, s>>>(sval.as<uint32_t>(), E, pb.ldst.as<uint16_t>(),
                                                 pb.sidx.as<uint16_t>());
     LLG_HIP(hipGetLastError());
   }
-  {
-    // chunk table: (source bin, destination bin) counts, scanned destination-major
-    DevBuf th, toff;
-    th.alloc(nc * 4, s);
-    toff.alloc((nc + 1) * 8, s);
-    k_pb_count<<<nb, kBuildBlock, lds, s>>>(ofs, adj, V, bb, nb, th.as<uint32_t>());
-    LLG_HIP(hipGetLastError());
-    scan_u32_to_u64(th.as<uint32_t>(), toff.as<uint64_t>(), nc, s);
-    k_pb_table<<<grid_for(std::max<uint64_t>(nc, nb + 1), 256), 256, 0, s>>>(
-        th.as<uint32_t>(), toff.as<uint64_t>(), nb, E, pb.ct.as<uint2>(),
-        pb.bstart.as<uint64_t>());
-    LLG_HIP(hipGetLastError());
-  }
+  k_pb_table<<<grid_for(std::max<uint64_t>(nc, nb + 1), 256), 256, 0, s>>>(
+      X.as<uint64_t>(), d_first.as<uint32_t>(), ni, nb, E, pb.ct.as<uint2>(),
+      pb.bstart.as<uint64_t>());
+  LLG_HIP(hipGetLastError());
+  // the host tables above are read by the copies
+  LLG_HIP(hipStreamSynchronize(s));
   pb.valid = true;
   pb.level = out.level;
   pb.bb = bb;

@@ -12,11 +12,12 @@
 // for all keys are dropped (the host reads the 8 x 256 counts back), and the
 // remaining bits are cut into the fewest passes of at most 8 bits, as even
 // as possible (11 bits: 6 + 5), so each tile's runs per digit stay long
-// enough to fill cache lines.  A pass counts digits per tile, scans the counts
-// digit-major (digit, tile) and scatters: each round of 256 keys ranks a key
-// among the equal digits of its wave with one ballot per digit bit, adds the
-// counts of the waves before it (LDS) and the tile's running offset of that
-// digit.
+// enough to fill cache lines.  Callers that know the key width skip the
+// pre-pass.  A pass counts digits per tile, scans the counts digit-major
+// (digit, tile) and scatters: the tile is ordered by digit in LDS (each round
+// of 256 keys ranks a key among the equal digits of its wave with one ballot
+// per digit bit, adds the counts of the waves before it and the digit's
+// running count) and then written out run by run.
 #include <algorithm>
 
 #include "ll_gpu_internal.h"
@@ -163,25 +164,43 @@ __global__ __launch_bounds__(kB) void k_rs_tile_hist(const K* keys, uint64_t n, 
   if (threadIdx.x <= dm) th[uint64_t(threadIdx.x) * ntiles + blockIdx.x] = h[threadIdx.x];
 }
 
+// Scatter of one tile.  The tile's keys are first ordered by digit in LDS,
+// stably: each round of 256 keys ranks a key among the equal digits of its
+// wave with one ballot per digit bit, then adds the counts of the waves
+// before it and the digit's running count in the tile.  They are then written
+// out in that order, so consecutive lanes write consecutive positions of one
+// digit's run (a tile's run per digit is written as one contiguous piece).
 template <typename K, bool kVals>
 __global__ __launch_bounds__(kB) void k_rs_scatter(const K* kin, const uint32_t* vin, K* kout,
                                                    uint32_t* vout, uint64_t n, int shift, int bits,
-                                                   const uint64_t* tofs, uint64_t ntiles) {
-  __shared__ uint64_t run[256];
+                                                   const uint32_t* th, const uint64_t* tofs,
+                                                   uint64_t ntiles) {
+  __shared__ K skey[kTile];
+  __shared__ uint32_t sval[kVals ? kTile : 1];
+  __shared__ uint64_t gbase[256];  // first output position of the tile's run per digit
+  __shared__ uint32_t lbase[256];  // first LDS position of the digit in the tile
+  __shared__ uint32_t run[256];
   __shared__ uint32_t wcnt[kB / 64][256];
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const uint32_t dm = (1u << bits) - 1;
-  run[t] = uint32_t(t) <= dm ? tofs[uint64_t(t) * ntiles + blockIdx.x] : 0;
+  const uint64_t tile = blockIdx.x;
+  const bool mine = uint32_t(t) <= dm;
+  const uint64_t cnt = mine ? th[uint64_t(t) * ntiles + tile] : 0;
+  uint64_t tot;
+  lbase[t] = uint32_t(block_exclusive_scan(cnt, &tot));
+  gbase[t] = mine ? tofs[uint64_t(t) * ntiles + tile] : 0;
+  run[t] = 0;
   for (int w = 0; w < kB / 64; w++) wcnt[w][t] = 0;
   __syncthreads();
-  const uint64_t base = uint64_t(blockIdx.x) * kTile;
+  const uint64_t base = tile * kTile;
+  const uint32_t nt = uint32_t(min(kTile, n - base));
   for (int r = 0; r < kRounds; r++) {
-    if (base + uint64_t(r) * kB >= n) break;  // uniform over the workgroup
-    const uint64_t i = base + uint64_t(r) * kB + t;
-    const bool valid = i < n;
-    const K key = valid ? kin[i] : K(0);
+    if (uint32_t(r) * kB >= nt) break;  // uniform over the workgroup
+    const uint32_t j = uint32_t(r) * kB + uint32_t(t);
+    const bool valid = j < nt;
+    const K key = valid ? kin[base + j] : K(0);
     uint32_t val = 0;
-    if (kVals && valid) val = vin[i];
+    if (kVals && valid) val = vin[base + j];
     const uint32_t d = uint32_t(key >> shift) & dm;
     uint64_t peers = __ballot(valid);
     for (int b = 0; b < bits; b++) {
@@ -192,10 +211,10 @@ __global__ __launch_bounds__(kB) void k_rs_scatter(const K* kin, const uint32_t*
     if (valid && below == 0) wcnt[wave][d] = uint32_t(__popcll(peers));
     __syncthreads();
     if (valid) {
-      uint64_t pos = run[d] + uint64_t(__popcll(below));
+      uint32_t pos = lbase[d] + run[d] + uint32_t(__popcll(below));
       for (int w = 0; w < wave; w++) pos += wcnt[w][d];
-      kout[pos] = key;
-      if (kVals) vout[pos] = val;
+      skey[pos] = key;
+      if (kVals) sval[pos] = val;
     }
     __syncthreads();
     uint32_t add = 0;
@@ -206,60 +225,29 @@ __global__ __launch_bounds__(kB) void k_rs_scatter(const K* kin, const uint32_t*
     run[t] += add;
     __syncthreads();
   }
+  for (uint32_t j = uint32_t(t); j < nt; j += kB) {
+    const K key = skey[j];
+    const uint32_t d = uint32_t(key >> shift) & dm;
+    const uint64_t pos = gbase[d] + (j - lbase[d]);
+    kout[pos] = key;
+    if (kVals) vout[pos] = sval[j];
+  }
 }
 
+// probe: count the digit positions first and skip the bits that never vary;
+// without it, bits [0, end_bit) are sorted as given.
 template <typename K, bool kVals>
 void radix_sort(const K* kin, K* kout, const uint32_t* vin, uint32_t* vout, uint64_t n,
-                int end_bit, hipStream_t s) {
+                int end_bit, hipStream_t s, bool probe = true) {
   if (n == 0) return;
   constexpr int P = sizeof(K);
   const int npass_max = std::min(P, (end_bit + 7) / 8);
-  // which passes move anything
-  DevBuf gh;
-  gh.alloc(P * 256 * sizeof(unsigned long long), s);
-  LLG_HIP(hipMemsetAsync(gh.p, 0, gh.bytes, s));
-  {
-    int dev = 0, cus = 0;
-    LLG_HIP(hipGetDevice(&dev));
-    LLG_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    const uint64_t g = std::min<uint64_t>((n + kB - 1) / kB, uint64_t(std::max(cus, 1)) * 8);
-    k_rs_all_hist<K><<<unsigned(g), kB, 0, s>>>(kin, n, gh.as<unsigned long long>());
-    LLG_HIP(hipGetLastError());
-  }
-  std::vector<unsigned long long> h(P * 256);
-  LLG_HIP(hipMemcpyAsync(h.data(), gh.p, gh.bytes, hipMemcpyDeviceToHost, s));
-  LLG_HIP(hipStreamSynchronize(s));
-  // the bits to sort: bytes whose value differs between keys, cut into
-  // the fewest even passes of at most 8 bits within each run of such bytes
   struct Pass {
     int shift, bits;
   };
   std::vector<Pass> passes;
-  for (int p = 0; p < npass_max;) {
-    bool trivial = false;
-    for (int d = 0; d < 256; d++)
-      if (h[p * 256 + d] == n) trivial = true;
-    if (trivial) {
-      p++;
-      continue;
-    }
-    int q = p;  // bytes p..q-1 all vary
-    while (q < npass_max) {
-      bool t2 = false;
-      for (int d = 0; d < 256; d++)
-        if (h[q * 256 + d] == n) t2 = true;
-      if (t2) break;
-      q++;
-    }
-    int lo = 8 * p, hi = std::min(8 * q, end_bit);
-    // the highest byte may carry fewer significant bits than 8
-    if (q == npass_max && hi > lo) {
-      const int top = 8 * (q - 1);
-      int tb = 0;
-      for (int d = 0; d < 256; d++)
-        if (h[(q - 1) * 256 + d]) tb = std::max(tb, 32 - __builtin_clz(unsigned(d) | 1u));
-      hi = std::min(hi, top + tb);
-    }
+  // bits [lo, hi) in the fewest even passes of at most 8 bits
+  auto cut = [&](int lo, int hi) {
     const int nbits = hi - lo;
     const int np = (nbits + 7) / 8;
     for (int k = 0, at = lo; k < np; k++) {
@@ -267,7 +255,55 @@ void radix_sort(const K* kin, K* kout, const uint32_t* vin, uint32_t* vout, uint
       passes.push_back({at, w});
       at += w;
     }
-    p = q;
+  };
+  if (!probe) {
+    cut(0, std::min(end_bit, 8 * P));
+  } else {
+    // which passes move anything
+    DevBuf gh;
+    gh.alloc(P * 256 * sizeof(unsigned long long), s);
+    LLG_HIP(hipMemsetAsync(gh.p, 0, gh.bytes, s));
+    {
+      int dev = 0, cus = 0;
+      LLG_HIP(hipGetDevice(&dev));
+      LLG_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+      const uint64_t g = std::min<uint64_t>((n + kB - 1) / kB, uint64_t(std::max(cus, 1)) * 8);
+      k_rs_all_hist<K><<<unsigned(g), kB, 0, s>>>(kin, n, gh.as<unsigned long long>());
+      LLG_HIP(hipGetLastError());
+    }
+    std::vector<unsigned long long> h(P * 256);
+    LLG_HIP(hipMemcpyAsync(h.data(), gh.p, gh.bytes, hipMemcpyDeviceToHost, s));
+    LLG_HIP(hipStreamSynchronize(s));
+    // the bits to sort: bytes whose value differs between keys, cut into
+    // the fewest even passes of at most 8 bits within each run of such bytes
+    for (int p = 0; p < npass_max;) {
+      bool trivial = false;
+      for (int d = 0; d < 256; d++)
+        if (h[p * 256 + d] == n) trivial = true;
+      if (trivial) {
+        p++;
+        continue;
+      }
+      int q = p;  // bytes p..q-1 all vary
+      while (q < npass_max) {
+        bool t2 = false;
+        for (int d = 0; d < 256; d++)
+          if (h[q * 256 + d] == n) t2 = true;
+        if (t2) break;
+        q++;
+      }
+      int lo = 8 * p, hi = std::min(8 * q, end_bit);
+      // the highest byte may carry fewer significant bits than 8
+      if (q == npass_max && hi > lo) {
+        const int top = 8 * (q - 1);
+        int tb = 0;
+        for (int d = 0; d < 256; d++)
+          if (h[(q - 1) * 256 + d]) tb = std::max(tb, 32 - __builtin_clz(unsigned(d) | 1u));
+        hi = std::min(hi, top + tb);
+      }
+      cut(lo, hi);
+      p = q;
+    }
   }
   const uint64_t nt = ntiles_of(n);
   if (passes.empty()) {
@@ -298,7 +334,7 @@ void radix_sort(const K* kin, K* kout, const uint32_t* vin, uint32_t* vout, uint
     LLG_HIP(hipGetLastError());
     scan_rec(SrcU32{th.as<uint32_t>()}, nd * nt, tofs.as<uint64_t>(), stmp.as<uint64_t>(), s);
     k_rs_scatter<K, kVals><<<unsigned(nt), kB, 0, s>>>(ksrc, vsrc, kdst, vdst, n, shift, bits,
-                                                       tofs.as<uint64_t>(), nt);
+                                                       th.as<uint32_t>(), tofs.as<uint64_t>(), nt);
     LLG_HIP(hipGetLastError());
     ksrc = kdst;
     vsrc = vdst;
@@ -344,6 +380,11 @@ void sort_pairs_u64_u32(const uint64_t* keys_in, uint64_t* keys_out, const uint3
 void sort_pairs_u32_u32(const uint32_t* keys_in, uint32_t* keys_out, const uint32_t* vals_in,
                         uint32_t* vals_out, uint64_t n, int end_bit, hipStream_t s) {
   radix_sort<uint32_t, true>(keys_in, keys_out, vals_in, vals_out, n, end_bit, s);
+}
+
+void sort_pairs_u32_u32_bits(const uint32_t* keys_in, uint32_t* keys_out, const uint32_t* vals_in,
+                             uint32_t* vals_out, uint64_t n, int end_bit, hipStream_t s) {
+  radix_sort<uint32_t, true>(keys_in, keys_out, vals_in, vals_out, n, end_bit, s, false);
 }
 
 void sort_keys_u64(const uint64_t* keys_in, uint64_t* keys_out, uint64_t n, int end_bit,

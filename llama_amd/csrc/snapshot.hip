@@ -31,6 +31,8 @@ namespace {
 constexpr int kBlock = 256;
 constexpr int kWordsPerBlock = 2048;
 constexpr int kWaves = kBlock / 64;
+constexpr int kPerThread = kWordsPerBlock / kBlock;
+static_assert(kWordsPerBlock % kBlock == 0, "whole rounds per block");
 
 __global__ void k_zero_u32(uint32_t* p) { *p = 0; }
 
@@ -213,10 +215,18 @@ __global__ __launch_bounds__(kBlock) void k_live_count(const uint64_t* const* et
   const uint64_t x0 = uint64_t(blockIdx.x) * kWordsPerBlock;
   const uint64_t x1 = min(total, x0 + kWordsPerBlock);
   BlockSegs bs = load_block_segs(seg_vofs, seg_addr, blk_seg, l_vofs, l_addr);
+  // all of a thread's words are requested before any is tested
+  uint64_t w[kPerThread];
+#pragma unroll
+  for (int j = 0; j < kPerThread; j++) {
+    const uint64_t x = x0 + uint64_t(j) * kBlock + threadIdx.x;
+    w[j] = x < x1 ? load_word(et, l_vofs, l_addr, bs.n, x) : 0;
+  }
   uint32_t cnt = 0;
-  for (uint64_t x = x0 + threadIdx.x; x < x1; x += kBlock) {
-    uint64_t w = load_word(et, l_vofs, l_addr, bs.n, x);
-    cnt += word_maxlev(w) > level ? 1u : 0u;
+#pragma unroll
+  for (int j = 0; j < kPerThread; j++) {
+    const uint64_t x = x0 + uint64_t(j) * kBlock + threadIdx.x;
+    cnt += x < x1 && word_maxlev(w[j]) > level ? 1u : 0u;
   }
   for (int o = 32; o > 0; o >>= 1) cnt += __shfl_down(cnt, o, 64);
   if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = cnt;
@@ -246,13 +256,25 @@ __global__ __launch_bounds__(kBlock) void k_compact(const uint64_t* const* et,
   BlockSegs bs = load_block_segs(seg_vofs, seg_addr, blk_seg, l_vofs, l_addr);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   uint64_t out = kDeletions ? blk_ofs[blockIdx.x] : x0;
-  // Rounds of 256 consecutive words keep the output in virtual order.
-  for (uint64_t base = x0; base < x1; base += kBlock) {
-    uint64_t x = base + threadIdx.x;
-    bool valid = x < x1;
+  // all of a thread's words (and weights) are requested first
+  uint64_t wd[kPerThread];
+  float wv[kPerThread];
+#pragma unroll
+  for (int j = 0; j < kPerThread; j++) {
+    const uint64_t x = x0 + uint64_t(j) * kBlock + threadIdx.x;
     uint32_t lvl = 0;
     uint64_t idx = 0;
-    uint64_t w = valid ? load_word_at(et, l_vofs, l_addr, bs.n, x, lvl, idx) : 0;
+    wd[j] = x < x1 ? load_word_at(et, l_vofs, l_addr, bs.n, x, lvl, idx) : 0;
+    wv[j] = kWeights && x < x1 ? wt[lvl][idx] : 0.f;
+  }
+  // Rounds of 256 consecutive words keep the output in virtual order.
+#pragma unroll
+  for (int j = 0; j < kPerThread; j++) {
+    const uint64_t base = x0 + uint64_t(j) * kBlock;
+    if (base >= x1) break;  // uniform over the workgroup
+    const uint64_t x = base + threadIdx.x;
+    const bool valid = x < x1;
+    const uint64_t w = wd[j];
     bool live = valid && (!kDeletions || word_maxlev(w) > level);
     uint64_t mask = __ballot(live);
     uint32_t rank = __popcll(mask & ((uint64_t(1) << lane) - 1));
@@ -265,7 +287,7 @@ __global__ __launch_bounds__(kBlock) void k_compact(const uint64_t* const* et,
     }
     if (live) {
       adj[out + before + rank] = uint32_t(w & kIndexMask);
-      if (kWeights) wout[out + before + rank] = wt[lvl][idx];
+      if (kWeights) wout[out + before + rank] = wv[j];
     }
     out += round;
     __syncthreads();

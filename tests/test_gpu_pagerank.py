@@ -338,8 +338,9 @@ def test_pagerank_bins_every_bin_size(ctx, ml15, bin_bits):
 
 def test_pagerank_bins_ragged_and_tiny(ctx, bin_bits):
     """Vertex counts that are not a multiple of the bin (last bin partial),
-    smaller than one bin, isolated vertices, an empty graph, one vertex, and a
-    hub with in-degree 20000 inside one bin."""
+    smaller than one bin, isolated vertices, an empty graph, one vertex, a hub
+    with in-degree 20000 inside one bin, and a source bin whose out-edges are
+    cut into several build pieces (one cut inside a 300000-edge row)."""
     bin_bits(12)
     rng = np.random.default_rng(4)
     V = 20000 + 37
@@ -363,6 +364,22 @@ def test_pagerank_bins_ragged_and_tiny(ctx, bin_bits):
     gpu_case("pagerank bins hub 20000")
     r, _ = DeviceGraph(ctx, h).pagerank(5)
     assert l1rel(r, ref) <= PR_TOL
+    # a source bin of more than 2^18 out-edges is built in several pieces, one
+    # of them starting inside the out-row of vertex 0 (300000 out-edges)
+    n = 6000
+    ps = np.concatenate([np.zeros(300000, int), rng.integers(1, 4096, 120000)])
+    pd = rng.integers(1, n, ps.size)
+    p = HostGraph(n)
+    p.add_level(ps, pd)
+    ref, _ = oracle.OracleGraph.from_host(p).pagerank(10)
+    dp = DeviceGraph(ctx, p)
+    gpu_case("pagerank bins source bin in two build pieces")
+    r12, _ = dp.pagerank(10)
+    assert l1rel(r12, ref) <= PR_TOL
+    bin_bits(13)
+    r13, _ = dp.pagerank(10)
+    assert np.array_equal(r12, r13)
+    bin_bits(12)
     gpu_case("pagerank bins empty / one vertex / one edge")
     e = HostGraph(5)
     e.add_level([], [])
