@@ -5,9 +5,10 @@ Workload (default): R-MAT scale 24, edge factor 16 (a,b,c,d = .57,.19,.19,.05),
 permuted IDs, no self-loops/duplicates, base level + 10 incremental levels of
 +1% uniform inserts / -0.1% uniform deletes (deletion vector on), out- and
 in-edges mirrored in HBM.  One step = one full PageRank run from the HBM
-mirror: snapshot materialisation of the in-edges (the multi-level iterator,
-run on the device) + 20 pull iterations at d = 0.85 (task pagerank,
-benchmark/benchmarks/pagerank.h:98-182).  GTEPS = E_live * 20 / step time.
+mirror: snapshot materialisation of the out-edges (the multi-level iterator,
+run on the device), the binned layout built from them, and 20 pull
+iterations at d = 0.85 (task pagerank, benchmark/benchmarks/pagerank.h:98-182).
+GTEPS = E_live * 20 / step time.
 
 Prints one JSON line.  The CPU baseline is the oracle's OpenMP restatement of
 pagerank_double walking the same multi-level layout, on a bounded number of
@@ -483,28 +484,37 @@ def bench_wpr(ctx, args) -> dict:
 PR_ITERATION_KERNELS = ("k_pb_iter", "k_pr_pull", "k_pr_pull_cold", "k_pr_fixup", "k_pr_merge")
 
 
-TRAFFIC_PROFILE = ROOT / "profiles" / "k_pr_pull_traffic_rmat_s24_l11.json"
+# committed rocprofv3 FETCH_SIZE / WRITE_SIZE summaries, one per iteration kernel
+TRAFFIC_PROFILES = {
+    "k_pb_iter": ROOT / "profiles" / "k_pb_iter_traffic_rmat_s24_l11.json",
+    "k_pr_pull": ROOT / "profiles" / "k_pr_pull_traffic_rmat_s24_l11.json",
+}
 
 
 def measured_traffic(args, V: int, E: int, used: list) -> dict | None:
-    """HBM bytes per k_pr_pull dispatch from the committed rocprofv3 FETCH_SIZE /
-    WRITE_SIZE passes (each scaled by what the counter reads for a 1 GiB device
-    copy in the same run), when this run is the workload and PageRank variant
-    they were measured on; else None."""
+    """HBM bytes per dispatch of the PageRank iteration kernel from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE passes (each scaled by what the
+    counter reads for a 1 GiB device copy in the same run), when this run is
+    the workload and PageRank variant they were measured on; else None."""
+    if used == ["k_pb_iter"]:
+        kernel, variant = "k_pb_iter", args.bin_bits in (-1, 13)
+    elif used == ["k_pr_pull", "k_pr_fixup"]:
+        kernel = "k_pr_pull"
+        variant = (args.reorder == 1 and args.seg_bits in (-1, 0) and args.hot_bits in (-1, 0)
+                   and args.item_sort in (-1, 0) and args.cap in (0, 2048) and args.bin_bits == 0)
+    else:
+        return None
+    path = TRAFFIC_PROFILES[kernel]
     try:
-        prof = json.loads(TRAFFIC_PROFILE.read_text())
+        prof = json.loads(path.read_text())
     except (OSError, ValueError):
         return None
     w = prof["workload"]
-    plain = (args.reorder == 1 and args.seg_bits in (-1, 0) and args.hot_bits in (-1, 0)
-             and args.item_sort in (-1, 0) and args.cap in (0, 2048) and args.bin_bits == 0)
-    same = (w["num_nodes"] == V and w["live_edges"] == E and w["seed"] == args.seed and plain
-            and used == ["k_pr_pull", "k_pr_fixup"])
-    if not same:
+    if not (variant and w["num_nodes"] == V and w["live_edges"] == E and w["seed"] == args.seed):
         return None
     return {"bytes": int(prof["traffic_bytes_per_dispatch"]),
-            "source": f"profiles/{TRAFFIC_PROFILE.name}: FETCH_SIZE x{prof['fetch_scale']:.2f} + "
-                      f"WRITE_SIZE x{prof['write_scale']:.2f} KiB per k_pr_pull dispatch"}
+            "source": f"profiles/{path.name}: FETCH_SIZE x{prof['fetch_scale']:.2f} + "
+                      f"WRITE_SIZE x{prof['write_scale']:.2f} KiB per {kernel} dispatch"}
 
 
 def iteration_ms(ctx, iterations):

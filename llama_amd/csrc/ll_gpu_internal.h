@@ -414,6 +414,18 @@ __device__ inline void block_atomic_add(unsigned long long* p, unsigned long lon
   }
 }
 
+// Workgroups are dealt to the chip's 8 XCDs round-robin by ID (workgroup i
+// runs on XCD i mod 8).  xcd_block(i, n) renumbers workgroup i of n so that
+// every XCD takes a contiguous range of work items, in order: items that are
+// neighbours in memory then share one XCD's L2, where the partial cache lines
+// that neighbouring items write merge before they go to HBM.
+constexpr uint32_t kXcds = 8;
+__device__ inline uint32_t xcd_block(uint32_t i, uint32_t n) {
+  const uint32_t x = i % kXcds, q = i / kXcds;
+  const uint32_t per = n / kXcds, rem = n % kXcds;
+  return x * per + (x < rem ? x : rem) + q;
+}
+
 inline unsigned grid_for(uint64_t n, unsigned block) {
   uint64_t g = (n + block - 1) / block;
   return unsigned(g == 0 ? 1 : g);

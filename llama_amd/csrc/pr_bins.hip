@@ -41,6 +41,8 @@ namespace llgpu {
 namespace {
 
 constexpr int kIterBlock = 1024;
+constexpr int kUnrollA = 4;  // phase A: slots per thread in flight
+constexpr int kGroupC = 4;   // phase C: chunks per wave in flight
 constexpr int kBuildBlock = 256;
 
 // Build items: pieces of one source bin's out-edges, a bin holding more than
@@ -66,10 +68,23 @@ __global__ __launch_bounds__(kBuildBlock) void k_pb_count(const uint32_t* adj, c
   extern __shared__ uint32_t h[];
   for (uint32_t i = threadIdx.x; i < nb; i += blockDim.x) h[i] = 0;
   __syncthreads();
-  const PbItem it = items[blockIdx.x];
-  for (uint64_t e = it.e0 + threadIdx.x; e < it.e1; e += blockDim.x) atomicAdd(&h[adj[e] >> bb], 1u);
+  // neighbouring items write neighbouring counts: keep them on one XCD
+  const uint32_t item = xcd_block(blockIdx.x, gridDim.x);
+  const PbItem it = items[item];
+  // four edges per thread requested before they are counted
+  for (uint64_t e = it.e0 + threadIdx.x; e < it.e1; e += 4 * uint64_t(blockDim.x)) {
+    uint32_t v[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const uint64_t x = e + uint64_t(k) * blockDim.x;
+      v[k] = x < it.e1 ? adj[x] : 0u;
+    }
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+      if (e + uint64_t(k) * blockDim.x < it.e1) atomicAdd(&h[v[k] >> bb], 1u);
+  }
   __syncthreads();
-  for (uint32_t i = threadIdx.x; i < nb; i += blockDim.x) th[uint64_t(i) * ni + blockIdx.x] = h[i];
+  for (uint32_t i = threadIdx.x; i < nb; i += blockDim.x) th[uint64_t(i) * ni + item] = h[i];
 }
 
 // Per out-edge of an item: the destination bin as the sort key and (source
@@ -153,7 +168,9 @@ __global__ __launch_bounds__(kIterBlock) void k_pb_iter(PbArgs a) {
   constexpr uint32_t kPer = R / kIterBlock;
   static_assert(kPer >= 1 && R % kIterBlock == 0, "bin rows must be a multiple of the block");
   __shared__ unsigned long long acc[R];
-  const uint64_t b = blockIdx.x;
+  // neighbouring source bins write neighbouring chunks of every destination
+  // bin: keep them on one XCD
+  const uint64_t b = xcd_block(blockIdx.x, gridDim.x);
   const uint64_t v0 = b << BB;
   const uint32_t nrows = uint32_t(min(uint64_t(R), a.V - v0));
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -161,9 +178,20 @@ __global__ __launch_bounds__(kIterBlock) void k_pb_iter(PbArgs a) {
     for (uint32_t i = threadIdx.x; i < R; i += kIterBlock) acc[i] = 0;
     __syncthreads();
     const uint64_t k0 = a.bstart[b], k1 = a.bstart[b + 1];
-    for (uint64_t k = k0 + threadIdx.x; k < k1; k += kIterBlock) {
-      const float x = a.val_in[k];
-      atomicAdd(&acc[a.ldst[k]], (unsigned long long)(double(x) * 0x1p61));
+    // kUnrollA slots per thread requested before any is added
+    for (uint64_t k = k0 + threadIdx.x; k < k1; k += kUnrollA * uint64_t(kIterBlock)) {
+      float x[kUnrollA];
+      uint16_t d[kUnrollA];
+#pragma unroll
+      for (int u = 0; u < kUnrollA; u++) {
+        const uint64_t q = k + uint64_t(u) * kIterBlock;
+        x[u] = q < k1 ? a.val_in[q] : 0.f;
+        d[u] = q < k1 ? a.ldst[q] : uint16_t(0);
+      }
+#pragma unroll
+      for (int u = 0; u < kUnrollA; u++)
+        if (k + uint64_t(u) * kIterBlock < k1)
+          atomicAdd(&acc[d[u]], (unsigned long long)(double(x[u]) * 0x1p61));
     }
     __syncthreads();
   }
@@ -189,19 +217,39 @@ __global__ __launch_bounds__(kIterBlock) void k_pb_iter(PbArgs a) {
 #pragma unroll
     for (uint32_t j = 0; j < kPer; j++) cl[threadIdx.x + j * kIterBlock] = c[j];
     __syncthreads();
-    // chunks (b', b): 64 table entries per wave step, then each chunk's slots
-    // 64 at a time
+    // chunks (b', b): 64 table entries per wave step, then kGroupC chunks at a
+    // time, the first 128 slots of each requested together (two per lane)
+    // before any is written; longer chunks finish 64 slots at a time
     const uint2* row = a.ct + b * a.nb;
     for (uint32_t c0 = uint32_t(wave) * 64; c0 < a.nb; c0 += (kIterBlock / 64) * 64) {
       const uint32_t mine = c0 + lane;
       const uint2 t = mine < a.nb ? row[mine] : make_uint2(0u, 0u);
       const uint32_t n = min(64u, a.nb - c0);
-      for (uint32_t j = 0; j < n; j++) {
-        const uint32_t st = __shfl(t.x, int(j), 64), len = __shfl(t.y, int(j), 64);
-        for (uint32_t k = uint32_t(lane); k < len; k += 64) {
-          const uint32_t slot = st + k;
-          a.val_out[slot] = cl[a.sidx[slot]];
+      for (uint32_t j = 0; j < n; j += kGroupC) {
+        uint32_t st[kGroupC], len[kGroupC];
+        uint16_t s0[kGroupC], s1[kGroupC];
+#pragma unroll
+        for (int q = 0; q < kGroupC; q++) {
+          const int src = int(min(j + uint32_t(q), n - 1));
+          st[q] = __shfl(t.x, src, 64);
+          len[q] = j + uint32_t(q) < n ? __shfl(t.y, src, 64) : 0u;
         }
+#pragma unroll
+        for (int q = 0; q < kGroupC; q++) {
+          s0[q] = uint32_t(lane) < len[q] ? a.sidx[st[q] + lane] : uint16_t(0);
+          s1[q] = uint32_t(lane) + 64 < len[q] ? a.sidx[st[q] + lane + 64] : uint16_t(0);
+        }
+#pragma unroll
+        for (int q = 0; q < kGroupC; q++) {
+          if (uint32_t(lane) < len[q]) a.val_out[st[q] + lane] = cl[s0[q]];
+          if (uint32_t(lane) + 64 < len[q]) a.val_out[st[q] + lane + 64] = cl[s1[q]];
+        }
+#pragma unroll
+        for (int q = 0; q < kGroupC; q++)
+          for (uint32_t k = 128 + uint32_t(lane); k < len[q]; k += 64) {
+            const uint32_t slot = st[q] + k;
+            a.val_out[slot] = cl[a.sidx[slot]];
+          }
       }
     }
   }

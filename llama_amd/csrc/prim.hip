@@ -156,9 +156,16 @@ __global__ __launch_bounds__(kB) void k_rs_tile_hist(const K* keys, uint64_t n, 
   __syncthreads();
   const uint32_t dm = (1u << bits) - 1;
   const uint64_t base = uint64_t(blockIdx.x) * kTile;
+  K k[kRounds];  // every key of the thread requested before any is counted
+#pragma unroll
   for (int r = 0; r < kRounds; r++) {
     const uint64_t i = base + uint64_t(r) * kB + threadIdx.x;
-    if (i < n) atomicAdd(&h[uint32_t(keys[i] >> shift) & dm], 1u);
+    k[r] = i < n ? keys[i] : K(0);
+  }
+#pragma unroll
+  for (int r = 0; r < kRounds; r++) {
+    const uint64_t i = base + uint64_t(r) * kB + threadIdx.x;
+    if (i < n) atomicAdd(&h[uint32_t(k[r] >> shift) & dm], 1u);
   }
   __syncthreads();
   if (threadIdx.x <= dm) th[uint64_t(threadIdx.x) * ntiles + blockIdx.x] = h[threadIdx.x];
@@ -183,7 +190,9 @@ __global__ __launch_bounds__(kB) void k_rs_scatter(const K* kin, const uint32_t*
   __shared__ uint32_t wcnt[kB / 64][256];
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const uint32_t dm = (1u << bits) - 1;
-  const uint64_t tile = blockIdx.x;
+  // neighbouring tiles write neighbouring pieces of every digit's run: keep
+  // them on one XCD
+  const uint64_t tile = xcd_block(blockIdx.x, gridDim.x);
   const bool mine = uint32_t(t) <= dm;
   const uint64_t cnt = mine ? th[uint64_t(t) * ntiles + tile] : 0;
   uint64_t tot;
@@ -194,13 +203,22 @@ __global__ __launch_bounds__(kB) void k_rs_scatter(const K* kin, const uint32_t*
   __syncthreads();
   const uint64_t base = tile * kTile;
   const uint32_t nt = uint32_t(min(kTile, n - base));
+  // every key (and value) of the thread is requested before any is ranked
+  K kr[kRounds];
+  uint32_t vr[kVals ? kRounds : 1];
+#pragma unroll
+  for (int r = 0; r < kRounds; r++) {
+    const uint32_t j = uint32_t(r) * kB + uint32_t(t);
+    kr[r] = j < nt ? kin[base + j] : K(0);
+    if (kVals) vr[r] = j < nt ? vin[base + j] : 0u;
+  }
+#pragma unroll
   for (int r = 0; r < kRounds; r++) {
     if (uint32_t(r) * kB >= nt) break;  // uniform over the workgroup
     const uint32_t j = uint32_t(r) * kB + uint32_t(t);
     const bool valid = j < nt;
-    const K key = valid ? kin[base + j] : K(0);
-    uint32_t val = 0;
-    if (kVals && valid) val = vin[base + j];
+    const K key = kr[r];
+    const uint32_t val = kVals ? vr[r] : 0u;
     const uint32_t d = uint32_t(key >> shift) & dm;
     uint64_t peers = __ballot(valid);
     for (int b = 0; b < bits; b++) {
