@@ -61,46 +61,27 @@ __global__ void k_pb_bounds(const uint64_t* ofs, uint64_t V, uint32_t bb, uint32
   if (i <= nb) bo[i] = ofs[min(V, i << bb)];
 }
 
-// th[b' * ni + i] = out-edges of item i into destination bin b' (counts in LDS).
-__global__ __launch_bounds__(kBuildBlock) void k_pb_count(const uint32_t* adj, const PbItem* items,
-                                                          uint32_t ni, uint32_t bb, uint32_t nb,
-                                                          uint32_t* th) {
-  extern __shared__ uint32_t h[];
-  for (uint32_t i = threadIdx.x; i < nb; i += blockDim.x) h[i] = 0;
-  __syncthreads();
-  // neighbouring items write neighbouring counts: keep them on one XCD
-  const uint32_t item = xcd_block(blockIdx.x, gridDim.x);
-  const PbItem it = items[item];
-  // four edges per thread requested before they are counted
-  for (uint64_t e = it.e0 + threadIdx.x; e < it.e1; e += 4 * uint64_t(blockDim.x)) {
-    uint32_t v[4];
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-      const uint64_t x = e + uint64_t(k) * blockDim.x;
-      v[k] = x < it.e1 ? adj[x] : 0u;
-    }
-#pragma unroll
-    for (int k = 0; k < 4; k++)
-      if (e + uint64_t(k) * blockDim.x < it.e1) atomicAdd(&h[v[k] >> bb], 1u);
-  }
-  __syncthreads();
-  for (uint32_t i = threadIdx.x; i < nb; i += blockDim.x) th[uint64_t(i) * ni + item] = h[i];
-}
-
 // Per out-edge of an item: the destination bin as the sort key and (source
 // offset << 16 | destination offset) as the value; the source row is found
 // by binary search over the bin's row offsets, staged in LDS.  Reads and
 // writes follow the edge order (coalesced).
+// The same pass counts the item's out-edges per destination bin (LDS), into
+// th[b' * ni + i].
 template <uint32_t BB>
 __global__ __launch_bounds__(kBuildBlock) void k_pb_keys(const uint64_t* ofs, const uint32_t* adj,
-                                                         const PbItem* items, uint64_t V,
-                                                         uint32_t* key, uint32_t* val) {
+                                                         const PbItem* items, uint32_t ni,
+                                                         uint64_t V, uint32_t nb, uint32_t* key,
+                                                         uint32_t* val, uint32_t* th) {
   constexpr uint32_t R = 1u << BB;
   __shared__ uint32_t ro[R + 1];
-  const PbItem it = items[blockIdx.x];
+  extern __shared__ uint32_t h[];
+  // neighbouring items write neighbouring counts: keep them on one XCD
+  const uint32_t item = xcd_block(blockIdx.x, gridDim.x);
+  const PbItem it = items[item];
   const uint64_t v0 = uint64_t(it.bin) << BB;
   const uint32_t nrows = uint32_t(min(uint64_t(R), V - v0));
   for (uint32_t i = threadIdx.x; i <= nrows; i += blockDim.x) ro[i] = uint32_t(ofs[v0 + i]);
+  for (uint32_t i = threadIdx.x; i < nb; i += blockDim.x) h[i] = 0;
   __syncthreads();
   for (uint64_t e = it.e0 + threadIdx.x; e < it.e1; e += blockDim.x) {
     const uint32_t v = adj[e];
@@ -113,16 +94,10 @@ __global__ __launch_bounds__(kBuildBlock) void k_pb_keys(const uint64_t* ofs, co
     }
     key[e] = v >> BB;
     val[e] = lo << 16 | (v & (R - 1));
+    atomicAdd(&h[v >> BB], 1u);
   }
-}
-
-__global__ void k_pb_split(const uint32_t* val, uint64_t E, uint16_t* ldst, uint16_t* sidx) {
-  const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i < E) {
-    const uint32_t x = val[i];
-    ldst[i] = uint16_t(x & 0xffffu);
-    sidx[i] = uint16_t(x >> 16);
-  }
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < nb; i += blockDim.x) th[uint64_t(i) * ni + item] = h[i];
 }
 
 // Chunk table by source bin: ct[b * nb + b'] = {first slot, slots} of chunk
@@ -291,6 +266,15 @@ void build_pr_bins(ll_gpu_graph* g, const Csr& out, uint32_t bb, PrBins& pb) {
   const uint64_t nc = uint64_t(nb) * nb;
   const size_t lds = size_t(nb) * 4;
   LLG_CHECK(lds <= 64 * 1024, LL_GPU_ERANGE, "too many PageRank bins for one workgroup's LDS");
+  {
+    // the key pass holds the bin's row offsets and the bin counters in LDS
+    int dev = 0, max_lds = 0;
+    LLG_HIP(hipGetDevice(&dev));
+    LLG_HIP(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    LLG_CHECK(lds + ((size_t(1) << bb) + 1) * 4 <= size_t(max_lds), LL_GPU_ERANGE,
+              "binned PageRank: " + std::to_string(nb) + " bins of 2^" + std::to_string(bb) +
+                  " vertices need more LDS per workgroup than the device has");
+  }
   const uint32_t* adj = out.adj.as<uint32_t>();
   const uint64_t* ofs = out.ofs.as<uint64_t>();
   pb.ldst.alloc(std::max<uint64_t>(E, 1) * 2, s);
@@ -328,30 +312,29 @@ void build_pr_bins(ll_gpu_graph* g, const Csr& out, uint32_t bb, PrBins& pb) {
   LLG_HIP(hipMemcpyAsync(d_first.p, first.data(), d_first.bytes, hipMemcpyHostToDevice, s));
   th.alloc(nx * 4, s);
   X.alloc((nx + 1) * 8, s);
-  k_pb_count<<<ni, kBuildBlock, lds, s>>>(adj, d_items.as<PbItem>(), ni, bb, nb, th.as<uint32_t>());
-  LLG_HIP(hipGetLastError());
-  scan_u32_to_u64(th.as<uint32_t>(), X.as<uint64_t>(), nx, s);
   {
-    // slots: out-edges stably sorted by destination bin
-    DevBuf key, val, skey, sval;
+    // keys and counts in one pass, then the slots: out-edges stably sorted by
+    // destination bin, the last sort pass writing the two u16 arrays
+    DevBuf key, val, kw, vw;
     key.alloc(std::max<uint64_t>(E, 1) * 4, s);
     val.alloc(std::max<uint64_t>(E, 1) * 4, s);
-    skey.alloc(std::max<uint64_t>(E, 1) * 4, s);
-    sval.alloc(std::max<uint64_t>(E, 1) * 4, s);
+    kw.alloc(std::max<uint64_t>(E, 1) * 4, s);
+    vw.alloc(std::max<uint64_t>(E, 1) * 4, s);
     const PbItem* di = d_items.as<PbItem>();
+    uint32_t* tk = key.as<uint32_t>();
+    uint32_t* tv = val.as<uint32_t>();
+    uint32_t* tc = th.as<uint32_t>();
     switch (bb) {
-      case 12: k_pb_keys<12><<<ni, kBuildBlock, 0, s>>>(ofs, adj, di, V, key.as<uint32_t>(), val.as<uint32_t>()); break;
-      case 13: k_pb_keys<13><<<ni, kBuildBlock, 0, s>>>(ofs, adj, di, V, key.as<uint32_t>(), val.as<uint32_t>()); break;
-      default: k_pb_keys<14><<<ni, kBuildBlock, 0, s>>>(ofs, adj, di, V, key.as<uint32_t>(), val.as<uint32_t>()); break;
+      case 12: k_pb_keys<12><<<ni, kBuildBlock, lds, s>>>(ofs, adj, di, ni, V, nb, tk, tv, tc); break;
+      case 13: k_pb_keys<13><<<ni, kBuildBlock, lds, s>>>(ofs, adj, di, ni, V, nb, tk, tv, tc); break;
+      default: k_pb_keys<14><<<ni, kBuildBlock, lds, s>>>(ofs, adj, di, ni, V, nb, tk, tv, tc); break;
     }
     LLG_HIP(hipGetLastError());
+    scan_u32_to_u64(th.as<uint32_t>(), X.as<uint64_t>(), nx, s);
     int kbits = 1;
     while (kbits < 32 && (uint64_t(nb - 1) >> kbits) != 0) kbits++;
-    sort_pairs_u32_u32_bits(key.as<uint32_t>(), skey.as<uint32_t>(), val.as<uint32_t>(),
-                            sval.as<uint32_t>(), E, kbits, s);
-    k_pb_split<<<grid_for(E, 256), 256, 0, s>>>(sval.as<uint32_t>(), E, pb.ldst.as<uint16_t>(),
-                                                pb.sidx.as<uint16_t>());
-    LLG_HIP(hipGetLastError());
+    sort_pairs_u32_u32_split(tk, kw.as<uint32_t>(), tv, vw.as<uint32_t>(), E, kbits,
+                             pb.ldst.as<uint16_t>(), pb.sidx.as<uint16_t>(), s);
   }
   k_pb_table<<<grid_for(std::max<uint64_t>(nc, nb + 1), 256), 256, 0, s>>>(
       X.as<uint64_t>(), d_first.as<uint32_t>(), ni, nb, E, pb.ct.as<uint2>(),

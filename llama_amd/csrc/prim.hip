@@ -177,11 +177,14 @@ __global__ __launch_bounds__(kB) void k_rs_tile_hist(const K* keys, uint64_t n, 
 // before it and the digit's running count in the tile.  They are then written
 // out in that order, so consecutive lanes write consecutive positions of one
 // digit's run (a tile's run per digit is written as one contiguous piece).
+// With lo16 given, the pass writes no keys and each value as its two 16-bit
+// halves (lo16, hi16) instead.
 template <typename K, bool kVals>
 __global__ __launch_bounds__(kB) void k_rs_scatter(const K* kin, const uint32_t* vin, K* kout,
                                                    uint32_t* vout, uint64_t n, int shift, int bits,
                                                    const uint32_t* th, const uint64_t* tofs,
-                                                   uint64_t ntiles) {
+                                                   uint64_t ntiles, uint16_t* lo16,
+                                                   uint16_t* hi16) {
   __shared__ K skey[kTile];
   __shared__ uint32_t sval[kVals ? kTile : 1];
   __shared__ uint64_t gbase[256];  // first output position of the tile's run per digit
@@ -247,17 +250,28 @@ __global__ __launch_bounds__(kB) void k_rs_scatter(const K* kin, const uint32_t*
     const K key = skey[j];
     const uint32_t d = uint32_t(key >> shift) & dm;
     const uint64_t pos = gbase[d] + (j - lbase[d]);
-    kout[pos] = key;
-    if (kVals) vout[pos] = sval[j];
+    if (kVals && lo16) {
+      const uint32_t v = sval[j];
+      lo16[pos] = uint16_t(v & 0xffffu);
+      hi16[pos] = uint16_t(v >> 16);
+    } else {
+      kout[pos] = key;
+      if (kVals) vout[pos] = sval[j];
+    }
   }
 }
 
 // probe: count the digit positions first and skip the bits that never vary;
-// without it, bits [0, end_bit) are sorted as given.
+// without it, bits [0, end_bit) are sorted as given.  lo16 / hi16: the last
+// pass writes the values' halves there instead of keys and values (kout and
+// vout then only serve as working memory).
 template <typename K, bool kVals>
 void radix_sort(const K* kin, K* kout, const uint32_t* vin, uint32_t* vout, uint64_t n,
-                int end_bit, hipStream_t s, bool probe = true) {
+                int end_bit, hipStream_t s, bool probe = true, uint16_t* lo16 = nullptr,
+                uint16_t* hi16 = nullptr) {
   if (n == 0) return;
+  LLG_CHECK(!lo16 || (!probe && end_bit > 0), LL_GPU_EINVAL,
+            "split output needs a sort of known, non-empty key width");
   constexpr int P = sizeof(K);
   const int npass_max = std::min(P, (end_bit + 7) / 8);
   struct Pass {
@@ -351,8 +365,11 @@ void radix_sort(const K* kin, K* kout, const uint32_t* vin, uint32_t* vout, uint
     k_rs_tile_hist<K><<<unsigned(nt), kB, 0, s>>>(ksrc, n, shift, bits, th.as<uint32_t>(), nt);
     LLG_HIP(hipGetLastError());
     scan_rec(SrcU32{th.as<uint32_t>()}, nd * nt, tofs.as<uint64_t>(), stmp.as<uint64_t>(), s);
+    const bool split = lo16 && j == np - 1;
     k_rs_scatter<K, kVals><<<unsigned(nt), kB, 0, s>>>(ksrc, vsrc, kdst, vdst, n, shift, bits,
-                                                       th.as<uint32_t>(), tofs.as<uint64_t>(), nt);
+                                                       th.as<uint32_t>(), tofs.as<uint64_t>(), nt,
+                                                       split ? lo16 : nullptr,
+                                                       split ? hi16 : nullptr);
     LLG_HIP(hipGetLastError());
     ksrc = kdst;
     vsrc = vdst;
@@ -403,6 +420,13 @@ void sort_pairs_u32_u32(const uint32_t* keys_in, uint32_t* keys_out, const uint3
 void sort_pairs_u32_u32_bits(const uint32_t* keys_in, uint32_t* keys_out, const uint32_t* vals_in,
                              uint32_t* vals_out, uint64_t n, int end_bit, hipStream_t s) {
   radix_sort<uint32_t, true>(keys_in, keys_out, vals_in, vals_out, n, end_bit, s, false);
+}
+
+void sort_pairs_u32_u32_split(const uint32_t* keys_in, uint32_t* keys_work,
+                              const uint32_t* vals_in, uint32_t* vals_work, uint64_t n,
+                              int end_bit, uint16_t* lo16, uint16_t* hi16, hipStream_t s) {
+  radix_sort<uint32_t, true>(keys_in, keys_work, vals_in, vals_work, n, end_bit, s, false, lo16,
+                             hi16);
 }
 
 void sort_keys_u64(const uint64_t* keys_in, uint64_t* keys_out, uint64_t n, int end_bit,

@@ -13,15 +13,17 @@
 //   2. scans give each segment its slot and virtual word offset;
 //   3. thread per vertex: write segment records (edge ID of the run start and
 //      virtual offset);                                      k_seg_fill
-//   4. block per 2048 virtual words: find each word's segment by binary search
-//      in an LDS copy of the block's segment range, read the word (coalesced
-//      within segments), test liveness, count;               k_live_count
-//   5. scan of the block counts; second pass writes the live payloads with a
-//      block-wide ordered compaction (wave ballots).          k_compact
-// Without deletions every stored word is live and step 4 is skipped.
-// The row offsets are the scan of the precomputed VT_L degrees (inside an
-// LL_MIN_LEVEL window, of the window's degrees); the live-word total is
-// checked against them.
+//   4. the row offsets are the scan of the precomputed VT_L degrees (inside
+//      an LL_MIN_LEVEL window, of the window's degrees), which already count
+//      only live words, so compaction blocks that start at a vertex know
+//      where their output starts: block b takes the vertices whose first
+//      virtual word lies in the b-th window of 2048 words;
+//   5. per block, each word's segment by binary search in an LDS copy of the
+//      block's segment range, the word read (coalesced within segments) and
+//      tested, the live ones written in order with a block-wide ballot
+//      compaction from the first vertex's row offset;        k_compact
+// The live-word total is summed on the device and checked against the row
+// offsets (a mismatch means a missing deletion patch).
 #include "ll_gpu_internal.h"
 
 namespace llgpu {
@@ -35,6 +37,7 @@ constexpr int kPerThread = kWordsPerBlock / kBlock;
 static_assert(kWordsPerBlock % kBlock == 0, "whole rounds per block");
 
 __global__ void k_zero_u32(uint32_t* p) { *p = 0; }
+__global__ void k_zero_u64(uint64_t* p) { *p = 0; }
 
 __global__ void k_ofs_degrees(const uint64_t* ofs, uint32_t* deg, uint64_t V) {
   uint64_t v = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -143,37 +146,44 @@ __device__ inline uint64_t seg_search_global(const uint64_t* vofs, uint64_t lo, 
   return lo;
 }
 
-struct BlockSegs {
-  uint64_t s_lo;
-  uint32_t n;  // segments overlapping the block
-};
+// Compaction blocks start at a vertex: block b takes the vertices whose first
+// virtual word lies in [b * kWordsPerBlock, (b + 1) * kWordsPerBlock), bv[b]
+// being the first of them (bv[nblk] = V), so its live words start at that
+// vertex's row offset and no count pass is needed.  A vertex longer than the
+// window makes its block longer; the windows it covers hold no vertex start
+// and their blocks are empty.
+__global__ void k_block_verts(const uint64_t* word_base, uint64_t V, uint64_t nblk,
+                              uint64_t* bv) {
+  const uint64_t b = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (b > nblk) return;
+  if (b == nblk) {
+    bv[b] = V;
+    return;
+  }
+  const uint64_t x = b * kWordsPerBlock;
+  uint64_t lo = 0, hi = V;  // first v with word_base[v] >= x
+  while (lo < hi) {
+    const uint64_t mid = lo + (hi - lo) / 2;
+    if (word_base[mid] < x) lo = mid + 1;
+    else hi = mid;
+  }
+  bv[b] = lo;
+}
 
-// First and last segment of every block of kWordsPerBlock virtual words
-// (one thread per block: the binary searches run in parallel instead of
-// serially at the head of each compaction workgroup).
-__global__ void k_block_segs(const uint64_t* seg_vofs, uint64_t nseg, uint64_t total,
-                             uint64_t nblk, uint64_t* blk_seg) {
+// First and last segment of every non-empty block (one thread per block: the
+// binary searches run in parallel instead of at the head of each workgroup).
+__global__ void k_block_segs(const uint64_t* seg_vofs, uint64_t nseg, const uint64_t* word_base,
+                             const uint64_t* bv, uint64_t nblk, uint64_t* blk_seg) {
   const uint64_t b = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (b >= nblk) return;
-  const uint64_t x0 = b * kWordsPerBlock;
-  const uint64_t x1 = min(total, x0 + kWordsPerBlock);
+  const uint64_t x0 = word_base[bv[b]], x1 = word_base[bv[b + 1]];
+  if (x0 == x1) {
+    blk_seg[2 * b] = blk_seg[2 * b + 1] = 0;
+    return;
+  }
   const uint64_t lo = seg_search_global(seg_vofs, 0, nseg, x0);
   blk_seg[2 * b] = lo;
   blk_seg[2 * b + 1] = seg_search_global(seg_vofs, lo, nseg, x1 - 1);
-}
-
-// Load the block's segment range (from k_block_segs) into LDS; returns it.
-__device__ inline BlockSegs load_block_segs(const uint64_t* seg_vofs, const uint64_t* seg_addr,
-                                            const uint64_t* blk_seg, uint64_t* l_vofs,
-                                            uint64_t* l_addr) {
-  const uint64_t lo = blk_seg[2 * blockIdx.x];
-  const uint32_t n = uint32_t(blk_seg[2 * blockIdx.x + 1] - lo + 1);
-  for (uint32_t i = threadIdx.x; i <= n; i += blockDim.x) {
-    l_vofs[i] = seg_vofs[lo + i];
-    if (i < n) l_addr[i] = seg_addr[lo + i];
-  }
-  __syncthreads();
-  return BlockSegs{lo, n};
 }
 
 __device__ inline uint32_t seg_search_lds(const uint64_t* l_vofs, uint32_t n, uint64_t x) {
@@ -185,113 +195,98 @@ __device__ inline uint32_t seg_search_lds(const uint64_t* l_vofs, uint32_t n, ui
   return lo;
 }
 
-__device__ inline uint64_t load_word(const uint64_t* const* et, const uint64_t* l_vofs,
-                                     const uint64_t* l_addr, uint32_t n, uint64_t x) {
-  uint32_t s = seg_search_lds(l_vofs, n, x);
-  uint64_t a = l_addr[s];
-  return et[edge_level(a)][edge_index(a) + (x - l_vofs[s])];
-}
-
-// The same word plus its level and index (for the per-level weight arrays).
-__device__ inline uint64_t load_word_at(const uint64_t* const* et, const uint64_t* l_vofs,
-                                        const uint64_t* l_addr, uint32_t n, uint64_t x,
-                                        uint32_t& lvl, uint64_t& idx) {
-  uint32_t s = seg_search_lds(l_vofs, n, x);
-  uint64_t a = l_addr[s];
-  lvl = edge_level(a);
-  idx = edge_index(a) + (x - l_vofs[s]);
-  return et[lvl][idx];
-}
-
-__global__ __launch_bounds__(kBlock) void k_live_count(const uint64_t* const* et,
-                                                       const uint64_t* seg_vofs,
-                                                       const uint64_t* seg_addr,
-                                                       const uint64_t* blk_seg,
-                                                       uint64_t total, uint32_t level,
-                                                       uint32_t* blk_cnt) {
-  __shared__ uint64_t l_vofs[kWordsPerBlock + 1];
-  __shared__ uint64_t l_addr[kWordsPerBlock];
-  __shared__ uint32_t wsum[kWaves];
-  const uint64_t x0 = uint64_t(blockIdx.x) * kWordsPerBlock;
-  const uint64_t x1 = min(total, x0 + kWordsPerBlock);
-  BlockSegs bs = load_block_segs(seg_vofs, seg_addr, blk_seg, l_vofs, l_addr);
-  // all of a thread's words are requested before any is tested
-  uint64_t w[kPerThread];
-#pragma unroll
-  for (int j = 0; j < kPerThread; j++) {
-    const uint64_t x = x0 + uint64_t(j) * kBlock + threadIdx.x;
-    w[j] = x < x1 ? load_word(et, l_vofs, l_addr, bs.n, x) : 0;
-  }
-  uint32_t cnt = 0;
-#pragma unroll
-  for (int j = 0; j < kPerThread; j++) {
-    const uint64_t x = x0 + uint64_t(j) * kBlock + threadIdx.x;
-    cnt += x < x1 && word_maxlev(w[j]) > level ? 1u : 0u;
-  }
-  for (int o = 32; o > 0; o >>= 1) cnt += __shfl_down(cnt, o, 64);
-  if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = cnt;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    uint32_t t = 0;
-    for (int i = 0; i < kWaves; i++) t += wsum[i];
-    blk_cnt[blockIdx.x] = t;
-  }
-}
-
 // kWeights: also write each live word's weight (the out direction's
 // per-level weight arrays, same level and index as the word) to wout.
+// Block b: the vertices [bv[b], bv[b + 1]), their virtual words in rounds of
+// kWordsPerBlock (each thread requests its kPerThread words of a round before
+// any is tested), live words written in virtual order from the first
+// vertex's row offset.  The block's segments are looked up in LDS (in global
+// memory when more than kWordsPerBlock overlap it).  Writes stay below E
+// whatever the mirror holds; the live total is added to *live, which the host
+// checks against E.
 template <bool kDeletions, bool kWeights>
 __global__ __launch_bounds__(kBlock) void k_compact(const uint64_t* const* et,
                                                     const uint64_t* seg_vofs,
                                                     const uint64_t* seg_addr,
                                                     const uint64_t* blk_seg,
-                                                    uint64_t total, uint32_t level,
-                                                    const uint64_t* blk_ofs, uint32_t* adj,
-                                                    const float* const* wt, float* wout) {
+                                                    const uint64_t* word_base, const uint64_t* bv,
+                                                    const uint64_t* ofs, uint64_t E,
+                                                    uint32_t level, uint32_t* adj,
+                                                    const float* const* wt, float* wout,
+                                                    unsigned long long* live) {
   __shared__ uint64_t l_vofs[kWordsPerBlock + 1];
   __shared__ uint64_t l_addr[kWordsPerBlock];
   __shared__ uint32_t wcnt[kWaves];
-  const uint64_t x0 = uint64_t(blockIdx.x) * kWordsPerBlock;
-  const uint64_t x1 = min(total, x0 + kWordsPerBlock);
-  BlockSegs bs = load_block_segs(seg_vofs, seg_addr, blk_seg, l_vofs, l_addr);
+  const uint64_t va = bv[blockIdx.x], vb = bv[blockIdx.x + 1];
+  const uint64_t x0 = word_base[va], x1 = word_base[vb];
+  if (x0 == x1) return;  // uniform: a window inside a long vertex
+  const uint64_t s_lo = blk_seg[2 * blockIdx.x];
+  const uint64_t n = blk_seg[2 * blockIdx.x + 1] - s_lo + 1;
+  const bool in_lds = n <= kWordsPerBlock;
+  if (in_lds) {
+    for (uint32_t i = threadIdx.x; i <= uint32_t(n); i += blockDim.x) {
+      l_vofs[i] = seg_vofs[s_lo + i];
+      if (i < n) l_addr[i] = seg_addr[s_lo + i];
+    }
+  }
+  __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  uint64_t out = kDeletions ? blk_ofs[blockIdx.x] : x0;
-  // all of a thread's words (and weights) are requested first
-  uint64_t wd[kPerThread];
-  float wv[kPerThread];
+  uint64_t out = ofs[va];
+  unsigned long long nlive = 0;
+  for (uint64_t xb = x0; xb < x1; xb += kWordsPerBlock) {
+    uint64_t wd[kPerThread];
+    float wv[kPerThread];
 #pragma unroll
-  for (int j = 0; j < kPerThread; j++) {
-    const uint64_t x = x0 + uint64_t(j) * kBlock + threadIdx.x;
-    uint32_t lvl = 0;
-    uint64_t idx = 0;
-    wd[j] = x < x1 ? load_word_at(et, l_vofs, l_addr, bs.n, x, lvl, idx) : 0;
-    wv[j] = kWeights && x < x1 ? wt[lvl][idx] : 0.f;
-  }
-  // Rounds of 256 consecutive words keep the output in virtual order.
+    for (int j = 0; j < kPerThread; j++) {
+      const uint64_t x = xb + uint64_t(j) * kBlock + threadIdx.x;
+      wd[j] = 0;
+      wv[j] = 0.f;
+      if (x < x1) {
+        uint64_t a, v;
+        if (in_lds) {
+          const uint32_t sg = seg_search_lds(l_vofs, uint32_t(n), x);
+          a = l_addr[sg];
+          v = l_vofs[sg];
+        } else {
+          const uint64_t sg = seg_search_global(seg_vofs, s_lo, s_lo + n, x);
+          a = seg_addr[sg];
+          v = seg_vofs[sg];
+        }
+        const uint32_t lvl = edge_level(a);
+        const uint64_t idx = edge_index(a) + (x - v);
+        wd[j] = et[lvl][idx];
+        if (kWeights) wv[j] = wt[lvl][idx];
+      }
+    }
+    // rounds of 256 consecutive words keep the output in virtual order
 #pragma unroll
-  for (int j = 0; j < kPerThread; j++) {
-    const uint64_t base = x0 + uint64_t(j) * kBlock;
-    if (base >= x1) break;  // uniform over the workgroup
-    const uint64_t x = base + threadIdx.x;
-    const bool valid = x < x1;
-    const uint64_t w = wd[j];
-    bool live = valid && (!kDeletions || word_maxlev(w) > level);
-    uint64_t mask = __ballot(live);
-    uint32_t rank = __popcll(mask & ((uint64_t(1) << lane) - 1));
-    if (lane == 0) wcnt[wave] = __popcll(mask);
-    __syncthreads();
-    uint32_t before = 0, round = 0;
-    for (int i = 0; i < kWaves; i++) {
-      if (i < wave) before += wcnt[i];
-      round += wcnt[i];
+    for (int j = 0; j < kPerThread; j++) {
+      const uint64_t base = xb + uint64_t(j) * kBlock;
+      if (base >= x1) break;  // uniform over the workgroup
+      const uint64_t x = base + threadIdx.x;
+      const bool valid = x < x1;
+      const uint64_t w = wd[j];
+      const bool alive = valid && (!kDeletions || word_maxlev(w) > level);
+      const uint64_t mask = __ballot(alive);
+      const uint32_t rank = __popcll(mask & ((uint64_t(1) << lane) - 1));
+      if (lane == 0) wcnt[wave] = __popcll(mask);
+      __syncthreads();
+      uint32_t before = 0, round = 0;
+      for (int i = 0; i < kWaves; i++) {
+        if (i < wave) before += wcnt[i];
+        round += wcnt[i];
+      }
+      const uint64_t pos = out + before + rank;
+      if (alive && pos < E) {
+        adj[pos] = uint32_t(w & kIndexMask);
+        if (kWeights) wout[pos] = wv[j];
+      }
+      if (threadIdx.x == 0) nlive += round;
+      out += round;
+      __syncthreads();
     }
-    if (live) {
-      adj[out + before + rank] = uint32_t(w & kIndexMask);
-      if (kWeights) wout[out + before + rank] = wv[j];
-    }
-    out += round;
-    __syncthreads();
   }
+  block_atomic_add(live, nlive);
 }
 
 }  // namespace
@@ -398,7 +393,6 @@ const Csr& ensure_csr(ll_gpu_graph* g, uint32_t dir, int level) {
   k_seg_total<<<1, 1, 0, s>>>(seg_vofs.as<uint64_t>(), S, W);
   LLG_HIP(hipGetLastError());
   seg_base.release();
-  word_base.release();
 
   // Row offsets from the precomputed degrees.
   const uint32_t* deg = ensure_degrees(g, dir, level);
@@ -408,7 +402,7 @@ const Csr& ensure_csr(ll_gpu_graph* g, uint32_t dir, int level) {
   LLG_HIP(hipMemcpyAsync(&E, c.ofs.as<uint64_t>() + V, 8, hipMemcpyDeviceToHost, s));
   LLG_HIP(hipStreamSynchronize(s));
 
-  // 4-5: compaction of live words
+  // 4-5: compaction of live words, blocks aligned to vertices
   const uint64_t nblk = (W + kWordsPerBlock - 1) / kWordsPerBlock;
   const bool dels = (g->flags & LL_GPU_HAS_DELETIONS) != 0;
   c.adj.alloc(std::max<uint64_t>(E, 1) * 4, s);
@@ -416,53 +410,55 @@ const Csr& ensure_csr(ll_gpu_graph* g, uint32_t dir, int level) {
   bool weighted = dir == LL_GPU_DIR_OUT && int(g->dir[dir].wt.size()) > level;
   for (int k = 0; weighted && k <= level; k++) weighted = g->dir[dir].wt[k].p != nullptr;
   if (weighted) c.w.alloc(std::max<uint64_t>(E, 1) * 4, s);
-  uint64_t live = W;
   if (nblk > 0) {
-    DevBuf blk_cnt, blk_ofs, blk_seg;
+    LLG_CHECK(dels || W == E, LL_GPU_ESTATE,
+              "snapshot has " + std::to_string(W) + " stored words but its degrees sum to " +
+                  std::to_string(E) + " and the graph has no deletions");
+    DevBuf bv, blk_seg, live;
+    bv.alloc((nblk + 1) * 8, s);
     blk_seg.alloc(nblk * 16, s);
-    k_block_segs<<<grid_for(nblk, 256), 256, 0, s>>>(seg_vofs.as<uint64_t>(), S, W, nblk,
-                                                     blk_seg.as<uint64_t>());
-    if (dels) {
-      blk_cnt.alloc(nblk * 4, s);
-      blk_ofs.alloc((nblk + 1) * 8, s);
-      {
-        TimedLaunch t(ctx, "k_live_count");
-        k_live_count<<<unsigned(nblk), kBlock, 0, s>>>(et, seg_vofs.as<uint64_t>(),
-                                                        seg_addr.as<uint64_t>(),
-                                                        blk_seg.as<uint64_t>(), W,
-                                                        uint32_t(level), blk_cnt.as<uint32_t>());
-      }
-      LLG_HIP(hipGetLastError());
-      scan_u32_to_u64(blk_cnt.as<uint32_t>(), blk_ofs.as<uint64_t>(), nblk, s);
-      LLG_HIP(hipMemcpyAsync(&live, blk_ofs.as<uint64_t>() + nblk, 8, hipMemcpyDeviceToHost, s));
-      LLG_HIP(hipStreamSynchronize(s));
-    }
-    LLG_CHECK(live == E, LL_GPU_ESTATE,
-              "snapshot has " + std::to_string(live) + " live words but its degrees sum to " +
-                  std::to_string(E) + " (missing deletion patches?)");
+    live.alloc(8, s);
+    k_zero_u64<<<1, 1, 0, s>>>(live.as<uint64_t>());
+    k_block_verts<<<grid_for(nblk + 1, 256), 256, 0, s>>>(word_base.as<uint64_t>(), V, nblk,
+                                                         bv.as<uint64_t>());
+    k_block_segs<<<grid_for(nblk, 256), 256, 0, s>>>(seg_vofs.as<uint64_t>(), S,
+                                                     word_base.as<uint64_t>(), bv.as<uint64_t>(),
+                                                     nblk, blk_seg.as<uint64_t>());
+    LLG_HIP(hipGetLastError());
     {
       TimedLaunch t(ctx, "k_compact");
-      const uint64_t* bo = dels ? blk_ofs.as<uint64_t>() : nullptr;
       const float* const* wt = weighted ? g->dir[dir].wt_table.as<const float* const>() : nullptr;
       float* wout = weighted ? c.w.as<float>() : nullptr;
       const unsigned nb = unsigned(nblk);
       const uint64_t* sv = seg_vofs.as<uint64_t>();
       const uint64_t* sa = seg_addr.as<uint64_t>();
       const uint64_t* bs = blk_seg.as<uint64_t>();
+      const uint64_t* wb = word_base.as<uint64_t>();
+      const uint64_t* bvp = bv.as<uint64_t>();
+      const uint64_t* of = c.ofs.as<uint64_t>();
+      unsigned long long* lv = live.as<unsigned long long>();
       uint32_t* adj = c.adj.as<uint32_t>();
+      const uint32_t L = uint32_t(level);
       if (dels && weighted)
-        k_compact<true, true><<<nb, kBlock, 0, s>>>(et, sv, sa, bs, W, uint32_t(level), bo, adj, wt, wout);
+        k_compact<true, true><<<nb, kBlock, 0, s>>>(et, sv, sa, bs, wb, bvp, of, E, L, adj, wt, wout, lv);
       else if (dels)
-        k_compact<true, false><<<nb, kBlock, 0, s>>>(et, sv, sa, bs, W, uint32_t(level), bo, adj, wt, wout);
+        k_compact<true, false><<<nb, kBlock, 0, s>>>(et, sv, sa, bs, wb, bvp, of, E, L, adj, wt, wout, lv);
       else if (weighted)
-        k_compact<false, true><<<nb, kBlock, 0, s>>>(et, sv, sa, bs, W, uint32_t(level), bo, adj, wt, wout);
+        k_compact<false, true><<<nb, kBlock, 0, s>>>(et, sv, sa, bs, wb, bvp, of, E, L, adj, wt, wout, lv);
       else
-        k_compact<false, false><<<nb, kBlock, 0, s>>>(et, sv, sa, bs, W, uint32_t(level), bo, adj, wt, wout);
+        k_compact<false, false><<<nb, kBlock, 0, s>>>(et, sv, sa, bs, wb, bvp, of, E, L, adj, wt, wout, lv);
     }
     LLG_HIP(hipGetLastError());
+    uint64_t nlive = 0;
+    LLG_HIP(hipMemcpyAsync(&nlive, live.p, 8, hipMemcpyDeviceToHost, s));
+    LLG_HIP(hipStreamSynchronize(s));
+    LLG_CHECK(nlive == E, LL_GPU_ESTATE,
+              "snapshot has " + std::to_string(nlive) + " live words but its degrees sum to " +
+                  std::to_string(E) + " (missing deletion patches?)");
   } else {
     LLG_CHECK(E == 0, LL_GPU_ESTATE, "degrees are non-zero but no segments are reachable");
   }
+  word_base.release();
   LLG_HIP(hipEventRecord(e1, s));
   LLG_HIP(hipEventSynchronize(e1));
   float ms = 0;

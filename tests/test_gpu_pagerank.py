@@ -81,6 +81,32 @@ def test_incremental_push_matches_full_push(ctx):
             assert np.array_equal(ga, oa) and np.array_equal(go, oo)
 
 
+def test_missing_deletion_patches_are_reported(ctx):
+    """A level pushed without the deletion patches it made leaves older words
+    live that its degrees no longer count: materialising the snapshot reports
+    the mismatch (and writes nothing past the row offsets) instead of
+    returning a wrong adjacency.  Pushed with the patches, the same level is
+    the oracle's."""
+    from llama_amd.graph import GpuError
+    g = build_multilevel(10, levels=2, ins_frac=0.05, del_frac=0.02, seed=7)
+    dg = DeviceGraph(ctx, g)
+    g.add_random_level(0.04, 0.03, 61)
+    items = dg.prepare_level(g.num_levels - 1)
+    assert any(it[0] == "patch" for it in items)
+    dg.push_prepared([it for it in items if it[0] != "patch"])
+    gpu_case("snapshot without the level's deletion patches")
+    with pytest.raises(GpuError, match="missing deletion patches"):
+        dg.adjacency(DIR_OUT)
+    dg.close()
+    dg = DeviceGraph(ctx, g)
+    o = oracle.OracleGraph.from_host(g)
+    for d in (DIR_OUT, DIR_IN):
+        oo, oa = o.flatten(d)
+        go, ga = dg.adjacency(d)
+        assert np.array_equal(ga, oa) and np.array_equal(go, oo)
+    dg.close()
+
+
 @pytest.mark.parametrize("seg", [0, 6, 9])
 @pytest.mark.parametrize("cap", [512, 1024, 2048, 4096])
 def test_pagerank_multilevel(ctx, ml, cap, seg, seg_bits):
