@@ -523,6 +523,28 @@ def iteration_ms(ctx, iterations):
     return sum(ctx.kernel_time(k)[1] for k in PR_ITERATION_KERNELS) / max(iterations, 1)
 
 
+def bound_env() -> dict:
+    """Environment of the CPU baseline processes: OpenMP threads bound close,
+    one per core, as the reference's CPU runs are measured (unless the caller
+    set these variables)."""
+    env = dict(os.environ)
+    env.setdefault("OMP_PROC_BIND", "close")
+    env.setdefault("OMP_PLACES", "cores")
+    return env
+
+
+def cpu_child(argv: list) -> dict:
+    """Run oracle/cpu_baseline.py in a process of its own (OpenMP bound, no GPU
+    runtime or torch loaded there); its progress lines pass through to stderr,
+    its JSON line is returned."""
+    import subprocess
+    r = subprocess.run([sys.executable, str(ROOT / "oracle" / "cpu_baseline.py"), *argv],
+                       stdout=subprocess.PIPE, text=True, timeout=900, env=bound_env())
+    if r.returncode != 0:
+        raise RuntimeError(f"CPU baseline process failed (exit {r.returncode})")
+    return json.loads(r.stdout.strip().splitlines()[-1])
+
+
 def reference_calibration(ctx, args) -> dict:
     """The reference's own CPU-runnable configuration: plain PageRank
     (pagerank_double, 10 iterations) on a single-level R-MAT s20 written as
@@ -547,25 +569,25 @@ def reference_calibration(ctx, args) -> dict:
     with tempfile.TemporaryDirectory() as tmp:
         f = str(Path(tmp) / "s20.xs1")
         e.write_xs1(f, 1 << 20)
-        env = dict(os.environ, OMP_NUM_THREADS=str(cores))
+        env = dict(bound_env(), OMP_NUM_THREADS=str(cores))
         r = subprocess.run([str(ref), "-r", "pagerank_double", "-n", "10", "-c", "3", f],
                            capture_output=True, text=True, timeout=600, cwd=tmp, env=env)
-    m = re.search(r"^Time\s*:\s*([0-9.]+) \+- ([0-9.]+) seconds", r.stdout, re.M)
-    if r.returncode != 0 or not m:
-        return {"skipped": "reference run failed: " + (r.stderr or r.stdout)[-300:]}
-    ref_s = float(m.group(1))
-    out["reference"] = {"seconds": ref_s, "c95": float(m.group(2)),
-                        "gteps": round(len(e) * 10 / ref_s / 1e9, 3)}
+        m = re.search(r"^Time\s*:\s*([0-9.]+) \+- ([0-9.]+) seconds", r.stdout, re.M)
+        if r.returncode != 0 or not m:
+            return {"skipped": "reference run failed: " + (r.stderr or r.stdout)[-300:]}
+        ref_s = float(m.group(1))
+        out["reference"] = {"seconds": ref_s, "c95": float(m.group(2)),
+                            "gteps": round(len(e) * 10 / ref_s / 1e9, 3)}
+        # the oracle restatement on the same graph, in a bound process of its own
+        rfile = str(Path(tmp) / "port_ranks.npy")
+        port = cpu_child(["calibration", "--seed", str(args.seed), "--ranks-out", rfile])
+        rp = np.load(rfile)
+    ts = port["seconds"]
+    out["port"] = {"seconds": round(float(np.mean(ts)), 4),
+                   "over_reference": round(float(np.mean(ts)) / ref_s, 3), "omp": port["omp"]}
+    out["omp"] = {k: env.get(k) for k in ("OMP_NUM_THREADS", "OMP_PROC_BIND", "OMP_PLACES")}
     g = HostGraph(1 << 20, deletions=False)
     g.add_level_edges(e)
-    o = oracle.OracleGraph.from_host(g)
-    ts = []
-    for _ in range(3):
-        t = time.perf_counter()
-        rp, _ = o.pagerank(10)
-        ts.append(time.perf_counter() - t)
-    out["port"] = {"seconds": round(float(np.mean(ts)), 4),
-                   "over_reference": round(float(np.mean(ts)) / ref_s, 3)}
     dg = DeviceGraph(ctx, g)
     dg.pagerank(10)
     ctx.sync()
@@ -781,11 +803,6 @@ def main() -> int:
                     help="weighted PageRank + adjoint graph scale, single level (0 = skip)")
     args = ap.parse_args()
 
-    # OpenMP placement of the CPU baseline (and of the host graph builder):
-    # threads bound close, one per core, as the reference's CPU runs are
-    # measured; set before any OpenMP runtime is loaded
-    os.environ.setdefault("OMP_PROC_BIND", "close")
-    os.environ.setdefault("OMP_PLACES", "cores")
 
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 "
@@ -893,10 +910,10 @@ def main() -> int:
     if n_pr == 0:
         n_pr, ms_pr = ctx.kernel_time("k_pr_pull")
     it_kernels = {k: ctx.kernel_time(k) for k in PR_ITERATION_KERNELS}
-    mat = {k: ctx.kernel_time(k) for k in ("k_seg_count", "k_seg_fill", "k_live_count",
-                                           "k_compact", "k_degrees", "pr_bins_build",
-                                           "k_pb_init", "pr_reorder", "pr_segments",
-                                           "pr_item_sort", "pr_hotcold")}
+    mat = {k: ctx.kernel_time(k) for k in ("k_seg_count", "k_seg_fill", "k_compact",
+                                           "k_degrees", "pr_bins_build", "k_pb_init",
+                                           "pr_reorder", "pr_segments", "pr_item_sort",
+                                           "pr_hotcold")}
     avg_it_ms = iteration_ms(ctx, args.iters * args.steps)
     ctx.profile(False)
     log(f"timed {args.steps} steps done")
@@ -938,20 +955,28 @@ def main() -> int:
     cpu = None
     parity = None
     if args.cpu_iters > 0:
+        import tempfile
+
         import oracle
-        o = oracle.OracleGraph.from_host(g)
         log(f"CPU baseline: oracle pagerank_double, {args.cpu_iters} iterations, "
-            f"{oracle.num_threads()} threads")
-        runs = []
-        ref_ranks = None
-        for _ in range(args.cpu_runs):
-            tc = time.perf_counter()
-            ref_ranks, _ = o.pagerank(args.cpu_iters)
-            runs.append(time.perf_counter() - tc)
+            f"{oracle.num_threads()} threads, in a process of its own")
+        with tempfile.TemporaryDirectory() as tmp:
+            rfile = str(Path(tmp) / "gpu_ranks.npy")
+            np.save(rfile, r_head)
+            argv = ["pagerank", "--scale", str(args.scale), "--levels", str(args.levels),
+                    "--edge-factor", str(args.edge_factor), "--seed", str(args.seed),
+                    "--iters", str(args.cpu_iters), "--runs", str(args.cpu_runs),
+                    "--ranks", rfile]
+            if args.sssp_sources > 0:
+                argv.append("--weighted")  # the same generator calls as the graph above
+            child = cpu_child(argv)
+        if child["live_edges"] != E or child["num_nodes"] != V:
+            raise RuntimeError(f"CPU baseline graph differs: {child['num_nodes']} vertices, "
+                               f"{child['live_edges']} live edges")
+        runs = child["runs"]
         if args.cpu_iters == args.iters:
             # the headline's ranks against the FP64 restatement on the same graph
-            parity = {"l1_rel_vs_fp64": float(np.abs(r_head - ref_ranks).sum() /
-                                              np.abs(ref_ranks).sum()),
+            parity = {"l1_rel_vs_fp64": child["l1_rel_vs_fp64"],
                       "bar": 1e-6,
                       "against": f"oracle pagerank_double (FP64), {args.iters} iterations, "
                                  f"same graph"}
@@ -961,16 +986,15 @@ def main() -> int:
         # (llama/include/llama/ll_utils.h:200-221)
         sd = (sum((x - cpu_s) ** 2 for x in runs) / len(runs)) ** 0.5
         cpu = {"value": E * args.cpu_iters / cpu_s / 1e9, "unit": "GTEPS",
-               "cores": oracle.num_threads(), "kind": "port",
+               "cores": child["threads"], "kind": "port",
                "sample": f"{args.cpu_iters} pull iterations of pagerank_double (FP64, "
                          f"OpenMP dynamic,4096) over the same {args.levels}-level layout, "
                          f"mean {cpu_s:.2f}s +- {1.96 * sd / len(runs) ** 0.5:.2f}s (c95) "
-                         f"over {len(runs)} runs",
+                         f"over {len(runs)} runs, in a process of its own",
                "seconds_runs": [round(x, 3) for x in runs],
                "cpu_model": cpu_model(),
                "host": host_cpu_layout(),
-               "omp": {k: os.environ.get(k) for k in ("OMP_NUM_THREADS", "OMP_PROC_BIND",
-                                                       "OMP_PLACES")}}
+               "omp": child["omp"]}
         log(f"CPU baseline {cpu['value']:.3f} GTEPS, mean of {len(runs)} runs of {cpu_s:.1f}s")
         if args.ref_calibration:
             cal = reference_calibration(ctx, args)
