@@ -106,6 +106,18 @@ struct DevBuf {
     if (n == 0) return;
     hipError_t e = hipMallocAsync(&p, n, stream);
     if (e != hipSuccess) {
+      // The device's stream-ordered pool keeps freed memory (the context sets
+      // no release threshold), in blocks that may each be smaller than this
+      // request: hand them back to the device and try once more.
+      (void)hipGetLastError();
+      int dev = 0;
+      hipMemPool_t pool = nullptr;
+      if (hipStreamSynchronize(stream) == hipSuccess && hipGetDevice(&dev) == hipSuccess &&
+          hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess && pool &&
+          hipMemPoolTrimTo(pool, 0) == hipSuccess)
+        e = hipMallocAsync(&p, n, stream);
+    }
+    if (e != hipSuccess) {
       p = nullptr;
       bytes = 0;
       (void)hipGetLastError();
