@@ -41,8 +41,8 @@ namespace llgpu {
 namespace {
 
 constexpr int kIterBlock = 1024;
-constexpr int kUnrollA = 4;  // phase A: slots per thread in flight
-constexpr int kGroupC = 4;   // phase C: chunks per wave in flight
+constexpr int kUnrollA = 8;  // phase A: slots per thread in flight
+constexpr int kGroupC = 8;   // phase C: chunks per wave in flight
 constexpr int kBuildBlock = 256;
 
 // Build items: pieces of one source bin's out-edges, a bin holding more than
