@@ -293,8 +293,10 @@ int ll_gpu_profile_get(ll_gpu_ctx* ctx, const char* kernel, uint64_t* launches,
 int ll_gpu_tuned(ll_gpu_ctx* ctx, const char* prefix, int* value);
 /* Tuning knobs: "pr_bin_bits" (PageRank as the binned pull over the
  * out-CSR with destination bins of 2^N vertices, N in 12..14; 0 = the
- * row-block pull over the in-CSR, which the knobs below tune; -1 = automatic
- * = 13 while the snapshot has fewer than 2^32 live edges, else 0),
+ * row-block pull over the in-CSR, which the knobs below tune; -1 = automatic:
+ * while the snapshot has fewer than 2^32 live edges and at most 2^28
+ * vertices, 13 when the edges between two bins of 2^13 number 32 or more on
+ * average, else 14; otherwise 0),
  * "bfs_reach" (1 = single-source searches whose root reaches only the
  * snapshot's reach set -- the vertices reachable from its highest-degree
  * vertex -- treat the other rows as visited; 0 = dead rows only; default 1),

@@ -22,8 +22,8 @@
 //      block's segment range, the word read (coalesced within segments) and
 //      tested, the live ones written in order with a block-wide ballot
 //      compaction from the first vertex's row offset;        k_compact
-// The live-word total is summed on the device and checked against the row
-// offsets (a mismatch means a missing deletion patch).
+// Every block checks its live words against its rows' degrees (a mismatch
+// means a missing deletion patch).
 #include "ll_gpu_internal.h"
 
 namespace llgpu {
@@ -37,7 +37,6 @@ constexpr int kPerThread = kWordsPerBlock / kBlock;
 static_assert(kWordsPerBlock % kBlock == 0, "whole rounds per block");
 
 __global__ void k_zero_u32(uint32_t* p) { *p = 0; }
-__global__ void k_zero_u64(uint64_t* p) { *p = 0; }
 
 __global__ void k_ofs_degrees(const uint64_t* ofs, uint32_t* deg, uint64_t V) {
   uint64_t v = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -201,9 +200,9 @@ __device__ inline uint32_t seg_search_lds(const uint64_t* l_vofs, uint32_t n, ui
 // kWordsPerBlock (each thread requests its kPerThread words of a round before
 // any is tested), live words written in virtual order from the first
 // vertex's row offset.  The block's segments are looked up in LDS (in global
-// memory when more than kWordsPerBlock overlap it).  Writes stay below E
-// whatever the mirror holds; the live total is added to *live, which the host
-// checks against E.
+// memory when more than kWordsPerBlock overlap it).  Writes stay inside the
+// block's rows whatever the mirror holds; a block whose live words differ in
+// number from its rows' degrees counts itself in *bad.
 template <bool kDeletions, bool kWeights>
 __global__ __launch_bounds__(kBlock) void k_compact(const uint64_t* const* et,
                                                     const uint64_t* seg_vofs,
@@ -213,7 +212,7 @@ __global__ __launch_bounds__(kBlock) void k_compact(const uint64_t* const* et,
                                                     const uint64_t* ofs, uint64_t E,
                                                     uint32_t level, uint32_t* adj,
                                                     const float* const* wt, float* wout,
-                                                    unsigned long long* live) {
+                                                    uint32_t* bad) {
   __shared__ uint64_t l_vofs[kWordsPerBlock + 1];
   __shared__ uint64_t l_addr[kWordsPerBlock];
   __shared__ uint32_t wcnt[kWaves];
@@ -231,8 +230,8 @@ __global__ __launch_bounds__(kBlock) void k_compact(const uint64_t* const* et,
   }
   __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  uint64_t out = ofs[va];
-  unsigned long long nlive = 0;
+  const uint64_t out0 = ofs[va], out1 = min(ofs[vb], E);
+  uint64_t out = out0;
   for (uint64_t xb = x0; xb < x1; xb += kWordsPerBlock) {
     uint64_t wd[kPerThread];
     float wv[kPerThread];
@@ -277,16 +276,15 @@ __global__ __launch_bounds__(kBlock) void k_compact(const uint64_t* const* et,
         round += wcnt[i];
       }
       const uint64_t pos = out + before + rank;
-      if (alive && pos < E) {
+      if (alive && pos < out1) {
         adj[pos] = uint32_t(w & kIndexMask);
         if (kWeights) wout[pos] = wv[j];
       }
-      if (threadIdx.x == 0) nlive += round;
       out += round;
       __syncthreads();
     }
   }
-  block_atomic_add(live, nlive);
+  if (threadIdx.x == 0 && out != ofs[vb]) atomicAdd(bad, 1u);
 }
 
 }  // namespace
@@ -414,11 +412,11 @@ const Csr& ensure_csr(ll_gpu_graph* g, uint32_t dir, int level) {
     LLG_CHECK(dels || W == E, LL_GPU_ESTATE,
               "snapshot has " + std::to_string(W) + " stored words but its degrees sum to " +
                   std::to_string(E) + " and the graph has no deletions");
-    DevBuf bv, blk_seg, live;
+    DevBuf bv, blk_seg, nbad;
     bv.alloc((nblk + 1) * 8, s);
     blk_seg.alloc(nblk * 16, s);
-    live.alloc(8, s);
-    k_zero_u64<<<1, 1, 0, s>>>(live.as<uint64_t>());
+    nbad.alloc(4, s);
+    k_zero_u32<<<1, 1, 0, s>>>(nbad.as<uint32_t>());
     k_block_verts<<<grid_for(nblk + 1, 256), 256, 0, s>>>(word_base.as<uint64_t>(), V, nblk,
                                                          bv.as<uint64_t>());
     k_block_segs<<<grid_for(nblk, 256), 256, 0, s>>>(seg_vofs.as<uint64_t>(), S,
@@ -436,7 +434,7 @@ const Csr& ensure_csr(ll_gpu_graph* g, uint32_t dir, int level) {
       const uint64_t* wb = word_base.as<uint64_t>();
       const uint64_t* bvp = bv.as<uint64_t>();
       const uint64_t* of = c.ofs.as<uint64_t>();
-      unsigned long long* lv = live.as<unsigned long long>();
+      uint32_t* lv = nbad.as<uint32_t>();
       uint32_t* adj = c.adj.as<uint32_t>();
       const uint32_t L = uint32_t(level);
       if (dels && weighted)
@@ -449,12 +447,12 @@ const Csr& ensure_csr(ll_gpu_graph* g, uint32_t dir, int level) {
         k_compact<false, false><<<nb, kBlock, 0, s>>>(et, sv, sa, bs, wb, bvp, of, E, L, adj, wt, wout, lv);
     }
     LLG_HIP(hipGetLastError());
-    uint64_t nlive = 0;
-    LLG_HIP(hipMemcpyAsync(&nlive, live.p, 8, hipMemcpyDeviceToHost, s));
+    uint32_t hbad = 0;
+    LLG_HIP(hipMemcpyAsync(&hbad, nbad.p, 4, hipMemcpyDeviceToHost, s));
     LLG_HIP(hipStreamSynchronize(s));
-    LLG_CHECK(nlive == E, LL_GPU_ESTATE,
-              "snapshot has " + std::to_string(nlive) + " live words but its degrees sum to " +
-                  std::to_string(E) + " (missing deletion patches?)");
+    LLG_CHECK(hbad == 0, LL_GPU_ESTATE,
+              "snapshot: " + std::to_string(hbad) + " blocks of vertices hold more or fewer "
+              "live words than their degrees count (missing deletion patches?)");
   } else {
     LLG_CHECK(E == 0, LL_GPU_ESTATE, "degrees are non-zero but no segments are reachable");
   }
