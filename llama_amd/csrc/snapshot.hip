@@ -33,8 +33,7 @@ namespace {
 constexpr int kBlock = 256;
 constexpr int kWordsPerBlock = 2048;
 constexpr int kWaves = kBlock / 64;
-constexpr int kPerThread = kWordsPerBlock / kBlock;
-static_assert(kWordsPerBlock % kBlock == 0, "whole rounds per block");
+constexpr int kBatch = kWordsPerBlock / kBlock + 1;  // rounds per compaction batch
 
 __global__ void k_zero_u32(uint32_t* p) { *p = 0; }
 
@@ -169,20 +168,27 @@ __global__ void k_block_verts(const uint64_t* word_base, uint64_t V, uint64_t nb
   bv[b] = lo;
 }
 
-// First and last segment of every non-empty block (one thread per block: the
-// binary searches run in parallel instead of at the head of each workgroup).
-__global__ void k_block_segs(const uint64_t* seg_vofs, uint64_t nseg, const uint64_t* word_base,
-                             const uint64_t* bv, uint64_t nblk, uint64_t* blk_seg) {
+// Everything a compaction block needs before it starts, in one record: its
+// virtual words [x0, x1), its rows' output range [out0, out1) and the
+// segments overlapping it (s_lo, n; n = 0 for an empty block).  One thread per
+// block: the lookups and binary searches run in parallel instead of at the
+// head of each workgroup.
+struct BlockPlan {
+  uint64_t x0, x1, out0, out1, s_lo, n;
+};
+
+__global__ void k_block_plan(const uint64_t* seg_vofs, uint64_t nseg, const uint64_t* word_base,
+                             const uint64_t* bv, const uint64_t* ofs, uint64_t nblk,
+                             BlockPlan* plan) {
   const uint64_t b = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (b >= nblk) return;
-  const uint64_t x0 = word_base[bv[b]], x1 = word_base[bv[b + 1]];
-  if (x0 == x1) {
-    blk_seg[2 * b] = blk_seg[2 * b + 1] = 0;
-    return;
+  const uint64_t va = bv[b], vb = bv[b + 1];
+  BlockPlan p{word_base[va], word_base[vb], ofs[va], ofs[vb], 0, 0};
+  if (p.x0 < p.x1) {
+    p.s_lo = seg_search_global(seg_vofs, 0, nseg, p.x0);
+    p.n = seg_search_global(seg_vofs, p.s_lo, nseg, p.x1 - 1) - p.s_lo + 1;
   }
-  const uint64_t lo = seg_search_global(seg_vofs, 0, nseg, x0);
-  blk_seg[2 * b] = lo;
-  blk_seg[2 * b + 1] = seg_search_global(seg_vofs, lo, nseg, x1 - 1);
+  plan[b] = p;
 }
 
 __device__ inline uint32_t seg_search_lds(const uint64_t* l_vofs, uint32_t n, uint64_t x) {
@@ -196,10 +202,10 @@ __device__ inline uint32_t seg_search_lds(const uint64_t* l_vofs, uint32_t n, ui
 
 // kWeights: also write each live word's weight (the out direction's
 // per-level weight arrays, same level and index as the word) to wout.
-// Block b: the vertices [bv[b], bv[b + 1]), their virtual words in rounds of
-// kWordsPerBlock (each thread requests its kPerThread words of a round before
-// any is tested), live words written in virtual order from the first
-// vertex's row offset.  The block's segments are looked up in LDS (in global
+// Block b: the vertices [bv[b], bv[b + 1]) (its plan record), their virtual
+// words in batches of kBatch rounds of 256 (each thread requests its words of
+// a batch before any is tested), live words written in virtual order from the
+// first vertex's row offset.  The block's segments are looked up in LDS (in global
 // memory when more than kWordsPerBlock overlap it).  Writes stay inside the
 // block's rows whatever the mirror holds; a block whose live words differ in
 // number from its rows' degrees counts itself in *bad.
@@ -207,20 +213,17 @@ template <bool kDeletions, bool kWeights>
 __global__ __launch_bounds__(kBlock) void k_compact(const uint64_t* const* et,
                                                     const uint64_t* seg_vofs,
                                                     const uint64_t* seg_addr,
-                                                    const uint64_t* blk_seg,
-                                                    const uint64_t* word_base, const uint64_t* bv,
-                                                    const uint64_t* ofs, uint64_t E,
+                                                    const BlockPlan* plan, uint64_t E,
                                                     uint32_t level, uint32_t* adj,
                                                     const float* const* wt, float* wout,
                                                     uint32_t* bad) {
   __shared__ uint64_t l_vofs[kWordsPerBlock + 1];
   __shared__ uint64_t l_addr[kWordsPerBlock];
   __shared__ uint32_t wcnt[kWaves];
-  const uint64_t va = bv[blockIdx.x], vb = bv[blockIdx.x + 1];
-  const uint64_t x0 = word_base[va], x1 = word_base[vb];
+  const BlockPlan pl = plan[blockIdx.x];
+  const uint64_t x0 = pl.x0, x1 = pl.x1;
   if (x0 == x1) return;  // uniform: a window inside a long vertex
-  const uint64_t s_lo = blk_seg[2 * blockIdx.x];
-  const uint64_t n = blk_seg[2 * blockIdx.x + 1] - s_lo + 1;
+  const uint64_t s_lo = pl.s_lo, n = pl.n;
   const bool in_lds = n <= kWordsPerBlock;
   if (in_lds) {
     for (uint32_t i = threadIdx.x; i <= uint32_t(n); i += blockDim.x) {
@@ -230,13 +233,15 @@ __global__ __launch_bounds__(kBlock) void k_compact(const uint64_t* const* et,
   }
   __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const uint64_t out0 = ofs[va], out1 = min(ofs[vb], E);
-  uint64_t out = out0;
-  for (uint64_t xb = x0; xb < x1; xb += kWordsPerBlock) {
-    uint64_t wd[kPerThread];
-    float wv[kPerThread];
+  const uint64_t out1 = min(pl.out1, E);
+  uint64_t out = pl.out0;
+  // batches of kBatch rounds: a block holds kWordsPerBlock words plus the
+  // tail of its last vertex, so most blocks take one batch
+  for (uint64_t xb = x0; xb < x1; xb += uint64_t(kBatch) * kBlock) {
+    uint64_t wd[kBatch];
+    float wv[kBatch];
 #pragma unroll
-    for (int j = 0; j < kPerThread; j++) {
+    for (int j = 0; j < kBatch; j++) {
       const uint64_t x = xb + uint64_t(j) * kBlock + threadIdx.x;
       wd[j] = 0;
       wv[j] = 0.f;
@@ -259,7 +264,7 @@ __global__ __launch_bounds__(kBlock) void k_compact(const uint64_t* const* et,
     }
     // rounds of 256 consecutive words keep the output in virtual order
 #pragma unroll
-    for (int j = 0; j < kPerThread; j++) {
+    for (int j = 0; j < kBatch; j++) {
       const uint64_t base = xb + uint64_t(j) * kBlock;
       if (base >= x1) break;  // uniform over the workgroup
       const uint64_t x = base + threadIdx.x;
@@ -284,7 +289,7 @@ __global__ __launch_bounds__(kBlock) void k_compact(const uint64_t* const* et,
       __syncthreads();
     }
   }
-  if (threadIdx.x == 0 && out != ofs[vb]) atomicAdd(bad, 1u);
+  if (threadIdx.x == 0 && out != pl.out1) atomicAdd(bad, 1u);
 }
 
 }  // namespace
@@ -412,16 +417,17 @@ const Csr& ensure_csr(ll_gpu_graph* g, uint32_t dir, int level) {
     LLG_CHECK(dels || W == E, LL_GPU_ESTATE,
               "snapshot has " + std::to_string(W) + " stored words but its degrees sum to " +
                   std::to_string(E) + " and the graph has no deletions");
-    DevBuf bv, blk_seg, nbad;
+    DevBuf bv, plan, nbad;
     bv.alloc((nblk + 1) * 8, s);
-    blk_seg.alloc(nblk * 16, s);
+    plan.alloc(nblk * sizeof(BlockPlan), s);
     nbad.alloc(4, s);
     k_zero_u32<<<1, 1, 0, s>>>(nbad.as<uint32_t>());
     k_block_verts<<<grid_for(nblk + 1, 256), 256, 0, s>>>(word_base.as<uint64_t>(), V, nblk,
                                                          bv.as<uint64_t>());
-    k_block_segs<<<grid_for(nblk, 256), 256, 0, s>>>(seg_vofs.as<uint64_t>(), S,
+    k_block_plan<<<grid_for(nblk, 256), 256, 0, s>>>(seg_vofs.as<uint64_t>(), S,
                                                      word_base.as<uint64_t>(), bv.as<uint64_t>(),
-                                                     nblk, blk_seg.as<uint64_t>());
+                                                     c.ofs.as<uint64_t>(), nblk,
+                                                     plan.as<BlockPlan>());
     LLG_HIP(hipGetLastError());
     {
       TimedLaunch t(ctx, "k_compact");
@@ -430,21 +436,18 @@ const Csr& ensure_csr(ll_gpu_graph* g, uint32_t dir, int level) {
       const unsigned nb = unsigned(nblk);
       const uint64_t* sv = seg_vofs.as<uint64_t>();
       const uint64_t* sa = seg_addr.as<uint64_t>();
-      const uint64_t* bs = blk_seg.as<uint64_t>();
-      const uint64_t* wb = word_base.as<uint64_t>();
-      const uint64_t* bvp = bv.as<uint64_t>();
-      const uint64_t* of = c.ofs.as<uint64_t>();
+      const BlockPlan* bp = plan.as<BlockPlan>();
       uint32_t* lv = nbad.as<uint32_t>();
       uint32_t* adj = c.adj.as<uint32_t>();
       const uint32_t L = uint32_t(level);
       if (dels && weighted)
-        k_compact<true, true><<<nb, kBlock, 0, s>>>(et, sv, sa, bs, wb, bvp, of, E, L, adj, wt, wout, lv);
+        k_compact<true, true><<<nb, kBlock, 0, s>>>(et, sv, sa, bp, E, L, adj, wt, wout, lv);
       else if (dels)
-        k_compact<true, false><<<nb, kBlock, 0, s>>>(et, sv, sa, bs, wb, bvp, of, E, L, adj, wt, wout, lv);
+        k_compact<true, false><<<nb, kBlock, 0, s>>>(et, sv, sa, bp, E, L, adj, wt, wout, lv);
       else if (weighted)
-        k_compact<false, true><<<nb, kBlock, 0, s>>>(et, sv, sa, bs, wb, bvp, of, E, L, adj, wt, wout, lv);
+        k_compact<false, true><<<nb, kBlock, 0, s>>>(et, sv, sa, bp, E, L, adj, wt, wout, lv);
       else
-        k_compact<false, false><<<nb, kBlock, 0, s>>>(et, sv, sa, bs, wb, bvp, of, E, L, adj, wt, wout, lv);
+        k_compact<false, false><<<nb, kBlock, 0, s>>>(et, sv, sa, bp, E, L, adj, wt, wout, lv);
     }
     LLG_HIP(hipGetLastError());
     uint32_t hbad = 0;
