@@ -191,6 +191,13 @@ __global__ void k_block_plan(const uint64_t* seg_vofs, uint64_t nseg, const uint
   plan[b] = p;
 }
 
+// A load through a pointer read from a table: the edge tables live in
+// device global memory, so the load is issued as a global (not a generic)
+// access and waits only on the vector memory counter.
+__device__ inline uint64_t load_global(const uint64_t* p) {
+  return *(const __attribute__((address_space(1))) uint64_t*)p;
+}
+
 __device__ inline uint32_t seg_search_lds(const uint64_t* l_vofs, uint32_t n, uint64_t x) {
   uint32_t lo = 0, hi = n;
   while (hi - lo > 1) {
@@ -209,30 +216,18 @@ __device__ inline uint32_t seg_search_lds(const uint64_t* l_vofs, uint32_t n, ui
 // memory when more than kWordsPerBlock overlap it).  Writes stay inside the
 // block's rows whatever the mirror holds; a block whose live words differ in
 // number from its rows' degrees counts itself in *bad.
-template <bool kDeletions, bool kWeights>
-__global__ __launch_bounds__(kBlock) void k_compact(const uint64_t* const* et,
-                                                    const uint64_t* seg_vofs,
-                                                    const uint64_t* seg_addr,
-                                                    const BlockPlan* plan, uint64_t E,
-                                                    uint32_t level, uint32_t* adj,
-                                                    const float* const* wt, float* wout,
-                                                    uint32_t* bad) {
-  __shared__ uint64_t l_vofs[kWordsPerBlock + 1];
-  __shared__ uint64_t l_addr[kWordsPerBlock];
-  __shared__ uint32_t wcnt[kWaves];
-  const BlockPlan pl = plan[blockIdx.x];
-  const uint64_t x0 = pl.x0, x1 = pl.x1;
-  if (x0 == x1) return;  // uniform: a window inside a long vertex
-  const uint64_t s_lo = pl.s_lo, n = pl.n;
-  const bool in_lds = n <= kWordsPerBlock;
-  if (in_lds) {
-    for (uint32_t i = threadIdx.x; i <= uint32_t(n); i += blockDim.x) {
-      l_vofs[i] = seg_vofs[s_lo + i];
-      if (i < n) l_addr[i] = seg_addr[s_lo + i];
-    }
-  }
-  __syncthreads();
+// The words of one block, live ones written in order.  kLds: the block's
+// segment records are in LDS (l_vofs / l_addr, n entries), else they are
+// searched in global memory; the two forms are separate code, so neither
+// pays for the other's address space.
+template <bool kDeletions, bool kWeights, bool kLds>
+__device__ inline void compact_block(const uint64_t* const* et, const uint64_t* seg_vofs,
+                                     const uint64_t* seg_addr, const BlockPlan& pl,
+                                     const uint64_t* l_vofs, const uint64_t* l_addr,
+                                     uint32_t* wcnt, uint64_t E, uint32_t level, uint32_t* adj,
+                                     const float* const* wt, float* wout, uint32_t* bad) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint64_t x0 = pl.x0, x1 = pl.x1, s_lo = pl.s_lo, n = pl.n;
   const uint64_t out1 = min(pl.out1, E);
   uint64_t out = pl.out0;
   // batches of kBatch rounds: a block holds kWordsPerBlock words plus the
@@ -247,7 +242,7 @@ __global__ __launch_bounds__(kBlock) void k_compact(const uint64_t* const* et,
       wv[j] = 0.f;
       if (x < x1) {
         uint64_t a, v;
-        if (in_lds) {
+        if (kLds) {
           const uint32_t sg = seg_search_lds(l_vofs, uint32_t(n), x);
           a = l_addr[sg];
           v = l_vofs[sg];
@@ -258,15 +253,17 @@ __global__ __launch_bounds__(kBlock) void k_compact(const uint64_t* const* et,
         }
         const uint32_t lvl = edge_level(a);
         const uint64_t idx = edge_index(a) + (x - v);
-        wd[j] = et[lvl][idx];
+        wd[j] = load_global(et[lvl] + idx);
         if (kWeights) wv[j] = wt[lvl][idx];
       }
     }
     // rounds of 256 consecutive words keep the output in virtual order
+    // (predicated rather than left early, so that the loop unrolls and wd[]
+    // stays in registers)
 #pragma unroll
     for (int j = 0; j < kBatch; j++) {
       const uint64_t base = xb + uint64_t(j) * kBlock;
-      if (base >= x1) break;  // uniform over the workgroup
+      if (base >= x1) continue;  // uniform over the workgroup
       const uint64_t x = base + threadIdx.x;
       const bool valid = x < x1;
       const uint64_t w = wd[j];
@@ -290,6 +287,33 @@ __global__ __launch_bounds__(kBlock) void k_compact(const uint64_t* const* et,
     }
   }
   if (threadIdx.x == 0 && out != pl.out1) atomicAdd(bad, 1u);
+}
+
+template <bool kDeletions, bool kWeights>
+__global__ __launch_bounds__(kBlock) void k_compact(const uint64_t* const* et,
+                                                    const uint64_t* seg_vofs,
+                                                    const uint64_t* seg_addr,
+                                                    const BlockPlan* plan, uint64_t E,
+                                                    uint32_t level, uint32_t* adj,
+                                                    const float* const* wt, float* wout,
+                                                    uint32_t* bad) {
+  __shared__ uint64_t l_vofs[kWordsPerBlock + 1];
+  __shared__ uint64_t l_addr[kWordsPerBlock];
+  __shared__ uint32_t wcnt[kWaves];
+  const BlockPlan pl = plan[blockIdx.x];
+  if (pl.x0 == pl.x1) return;  // uniform: a window inside a long vertex
+  if (pl.n <= kWordsPerBlock) {  // uniform
+    for (uint32_t i = threadIdx.x; i <= uint32_t(pl.n); i += blockDim.x) {
+      l_vofs[i] = seg_vofs[pl.s_lo + i];
+      if (i < pl.n) l_addr[i] = seg_addr[pl.s_lo + i];
+    }
+    __syncthreads();
+    compact_block<kDeletions, kWeights, true>(et, seg_vofs, seg_addr, pl, l_vofs, l_addr, wcnt,
+                                              E, level, adj, wt, wout, bad);
+  } else {
+    compact_block<kDeletions, kWeights, false>(et, seg_vofs, seg_addr, pl, l_vofs, l_addr, wcnt,
+                                               E, level, adj, wt, wout, bad);
+  }
 }
 
 }  // namespace
