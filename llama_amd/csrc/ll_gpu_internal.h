@@ -34,10 +34,26 @@ static_assert(sizeof(VtEntry) == 16, "VT entry must be 16 bytes");
 // the previous level's.
 constexpr uint32_t kVtPageBits = 9;
 constexpr uint64_t kVtPageEntries = uint64_t(1) << kVtPageBits;
+// A load through a pointer read from a table (page tables, per-level edge
+// tables): the memory is device global memory, so the load is issued as a
+// global access, which waits only on the vector memory counter (a generic
+// access also waits on the LDS counter).
+template <typename T>
+__device__ inline T ld_global(const T* p) {
+  return *(const __attribute__((address_space(1))) T*)p;
+}
+
 struct VtPages {
   const VtEntry* const* pages;
-  __device__ inline const VtEntry& operator[](uint64_t v) const {
-    return pages[v >> kVtPageBits][v & (kVtPageEntries - 1)];
+  __device__ inline VtEntry operator[](uint64_t v) const {
+    const uint64_t* p =
+        reinterpret_cast<const uint64_t*>(pages[v >> kVtPageBits] + (v & (kVtPageEntries - 1)));
+    const uint64_t a = ld_global(p), b = ld_global(p + 1);
+    VtEntry e;
+    e.start = int64_t(a);
+    e.len = uint32_t(b);
+    e.degree = uint32_t(b >> 32);
+    return e;
   }
 };
 

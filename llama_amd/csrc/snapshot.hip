@@ -70,8 +70,8 @@ __device__ inline void walk_chain(VtPages vt, const uint64_t* const* et,
     f(uint64_t(start), len);
     if (k == 0) return;
     const uint64_t* c = et[k] + idx + len;
-    start = int64_t(c[0]);
-    len = uint32_t(c[1]);
+    start = int64_t(ld_global(c));
+    len = uint32_t(ld_global(c + 1));
     kprev = k;
   }
 }
@@ -191,13 +191,6 @@ __global__ void k_block_plan(const uint64_t* seg_vofs, uint64_t nseg, const uint
   plan[b] = p;
 }
 
-// A load through a pointer read from a table: the edge tables live in
-// device global memory, so the load is issued as a global (not a generic)
-// access and waits only on the vector memory counter.
-__device__ inline uint64_t load_global(const uint64_t* p) {
-  return *(const __attribute__((address_space(1))) uint64_t*)p;
-}
-
 __device__ inline uint32_t seg_search_lds(const uint64_t* l_vofs, uint32_t n, uint64_t x) {
   uint32_t lo = 0, hi = n;
   while (hi - lo > 1) {
@@ -253,7 +246,7 @@ __device__ inline void compact_block(const uint64_t* const* et, const uint64_t* 
         }
         const uint32_t lvl = edge_level(a);
         const uint64_t idx = edge_index(a) + (x - v);
-        wd[j] = load_global(et[lvl] + idx);
+        wd[j] = ld_global(et[lvl] + idx);
         if (kWeights) wv[j] = wt[lvl][idx];
       }
     }
