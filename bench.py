@@ -63,7 +63,10 @@ def bench_bfs(dg, g, E: int, V: int, args) -> dict:
     ctx.sync()
     t = time.perf_counter() - t0
     n_td, ms_td = ctx.kernel_time("k_td_expand")
-    n_bu, ms_bu = ctx.kernel_time("k_bu_step")  # bottom-up launches (k_bu_rows)
+    # bottom-up level launches, including up to three launched past a search's
+    # last level (bottom-up levels go out four per read-back once a search
+    # stays bottom-up; those find nothing)
+    n_bu, ms_bu = ctx.kernel_time("k_bu_step")
     ctx.profile(False)
     levels_total = n_td + n_bu
     b_bfs = 4 * exam + 12 * reached + 2 * ((V + 7) // 8) * levels_total
@@ -71,7 +74,7 @@ def bench_bfs(dg, g, E: int, V: int, args) -> dict:
     return {
         "sources": len(roots), "seconds": round(t, 4),
         "gteps": round(trav / t / 1e9, 4), "edges_traversed": trav,
-        "edges_examined": exam, "reached": reached, "levels": levels_total,
+        "edges_examined": exam, "reached": reached, "level_launches": levels_total,
         "alg_bytes": b_bfs, "alg_gbs": round(b_bfs / t / 1e9, 2),
         "frac_hbm": round(b_bfs / t / 1e9 / HBM_PEAK_GBS, 4),
         "ms_top_down": round(ms_td, 3), "ms_bottom_up": round(ms_bu, 3),

@@ -32,7 +32,10 @@
 // the reach set it has not found yet, so each further level walks only those
 // and the lists shrink, while a return to top-down pays for the queue and the
 // frontier's out-edges (R-MAT s24, 64 sources: 115 against 101 GTEPS with
-// beta 24).
+// beta 24).  Such a search launches its bottom-up levels kBuBatch at a time,
+// each with its own counters, and reads them back once per batch: the only
+// decision left is where the search ended, and the levels launched past that
+// find nothing (their frontier is empty or has no out-edges).
 #include <climits>
 #include <optional>
 
@@ -52,6 +55,9 @@ constexpr int32_t kUnreached = INT_MAX - 1;
 // allocated from stripe 0's n_next (fetch-add needs one counter); the host
 // sums the stripes.
 constexpr int kStripes = 64;
+// bottom-up levels launched per host read-back once a search stays
+// bottom-up; each has its own set of striped counters
+constexpr int kBuBatch = 4;
 struct alignas(128) Counters {
   unsigned long long n_next;     // vertices discovered this level
   unsigned long long m_next;     // their out-edges
@@ -498,6 +504,26 @@ __global__ void k_count_reached(const int32_t* lv, uint64_t V, unsigned long lon
 
 // Summed stripes (aux from stripe 0); with `extra`, one more device word is
 // read back in the same round trip into *extra_out.
+// The summed stripes of n consecutive counter sets, one read-back.
+void read_counter_sets(ll_gpu_ctx* ctx, const Counters* c, int n, Counters* out) {
+  static_assert(kBuBatch * kStripes * sizeof(Counters) <= kReadbackBytes,
+                "counter sets fit the read-back buffer");
+  const Counters* st = static_cast<const Counters*>(ctx->readback);
+  LLG_HIP(hipMemcpyAsync(ctx->readback, c, size_t(n) * kStripes * sizeof(Counters),
+                         hipMemcpyDeviceToHost, ctx->stream));
+  LLG_HIP(hipStreamSynchronize(ctx->stream));
+  for (int k = 0; k < n; k++) {
+    const Counters* sk = st + k * kStripes;
+    Counters h = sk[0];
+    for (int i = 1; i < kStripes; i++) {
+      h.n_next += sk[i].n_next;
+      h.m_next += sk[i].m_next;
+      h.examined += sk[i].examined;
+    }
+    out[k] = h;
+  }
+}
+
 Counters read_counters(ll_gpu_ctx* ctx, const Counters* c, const uint64_t* extra = nullptr,
                        uint64_t* extra_out = nullptr) {
   static_assert(kStripes * sizeof(Counters) + 8 <= kReadbackBytes,
@@ -613,6 +639,44 @@ BfsResult bfs_core_ws(ll_gpu_graph* g, const Csr& fwd, const Csr& rev, const uin
       k_queue_to_bitmap<<<grid_for(nq, 256), 256, 0, s>>>(w.q[cur], nq,
                                                           fbm);
     }
+    if (bottom_up && beta == 0) {
+      // no return to top-down: kBuBatch levels per read-back; the levels
+      // after the one that finds nothing find nothing either
+      for (int k = 0; k < kBuBatch; k++) {
+        Counters* ck = w.cnt + k * kStripes;
+        k_zero_counters<<<1, kStripes, 0, s>>>(ck);
+        k_fill_u64<<<grid_for(words, 256), 256, 0, s>>>(nbm, words, 0);
+        {
+          MaybeTimed t(ctx, "k_bu_step", w.timed);
+          k_open_rows<<<grid_for(words, kBlock), kBlock, 0, s>>>(
+              w.visited, words, rev.ofs.as<uint64_t>(), w.longrows, w.rows_short, w.rows_long,
+              ck);
+          k_bu_short<<<bu_grid, kBlock, 0, s>>>(w.rows_short, rev.ofs.as<uint64_t>(),
+                                                rev.adj.as<uint32_t>(), fdeg, fbm, nbm, lv,
+                                                lvl + 1 + k, ck);
+          k_bu_long<<<bu_grid, kBlock, 0, s>>>(w.rows_long, rev.ofs.as<uint64_t>(),
+                                               rev.adj.as<uint32_t>(), fdeg, fbm, nbm, lv,
+                                               lvl + 1 + k, ck);
+        }
+        k_visited_or<<<grid_for(words, 256), 256, 0, s>>>(w.visited, nbm, words);
+        std::swap(fbm, nbm);
+      }
+      LLG_HIP(hipGetLastError());
+      Counters hs[kBuBatch];
+      read_counter_sets(ctx, w.cnt, kBuBatch, hs);
+      bool ended = false;
+      for (int k = 0; k < kBuBatch && !ended; k++) {
+        r.examined += hs[k].examined;
+        r.reached += hs[k].n_next;
+        m_visited += hs[k].m_next;
+        lvl++;
+        n_f = hs[k].n_next;
+        m_f = hs[k].m_next;
+        ended = n_f == 0 || m_f == 0;
+      }
+      if (ended) break;
+      continue;
+    }
     k_zero_counters<<<1, kStripes, 0, s>>>(w.cnt);
     if (bottom_up) {
       k_fill_u64<<<grid_for(words, 256), 256, 0, s>>>(nbm, words, 0);
@@ -688,7 +752,7 @@ BfsResult bfs_core(ll_gpu_graph* g, const Csr& fwd, const Csr& rev, const uint32
   q[1].alloc(V * 4, s);
   qdeg.alloc(V * 4, s);
   qofs.alloc((V + 1) * 8, s);
-  cnt.alloc(kStripes * sizeof(Counters), s);
+  cnt.alloc(kBuBatch * kStripes * sizeof(Counters), s);
   const size_t sb = scan_tmp_bytes(V);
   if (sb) stmp.alloc(sb, s);
   DevBuf rs, rl;
@@ -721,7 +785,7 @@ WsLayout ws_layout(uint64_t V) {
   L.q1 = take(V * 4);
   L.qdeg = take(V * 4);
   L.qofs = take((V + 1) * 8);
-  L.cnt = take(kStripes * sizeof(Counters));
+  L.cnt = take(kBuBatch * kStripes * sizeof(Counters));
   L.red = take(8);
   L.stmp = take(scan_tmp_bytes(V));
   L.rs = take(std::max<uint64_t>(V, 1) * 4);

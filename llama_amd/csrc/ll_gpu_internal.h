@@ -423,10 +423,8 @@ void sort_keys_u64(const uint64_t* keys_in, uint64_t* keys_out, uint64_t n, int 
                    hipStream_t s);
 void sort_pairs_u32_u32(const uint32_t* keys_in, uint32_t* keys_out, const uint32_t* vals_in,
                         uint32_t* vals_out, uint64_t n, int end_bit, hipStream_t s);
-// The same with bits [0, end_bit) sorted as given (no pre-pass over the keys).
-void sort_pairs_u32_u32_bits(const uint32_t* keys_in, uint32_t* keys_out, const uint32_t* vals_in,
-                             uint32_t* vals_out, uint64_t n, int end_bit, hipStream_t s);
-// The same, the sorted values written as their low and high 16-bit halves
+// The same with bits [0, end_bit) sorted as given (no pre-pass over the
+// keys), the sorted values written as their low and high 16-bit halves
 // (lo16, hi16); keys_work / vals_work (n entries each) are working memory.
 void sort_pairs_u32_u32_split(const uint32_t* keys_in, uint32_t* keys_work,
                               const uint32_t* vals_in, uint32_t* vals_work, uint64_t n,

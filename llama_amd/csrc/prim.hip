@@ -417,11 +417,6 @@ void sort_pairs_u32_u32(const uint32_t* keys_in, uint32_t* keys_out, const uint3
   radix_sort<uint32_t, true>(keys_in, keys_out, vals_in, vals_out, n, end_bit, s);
 }
 
-void sort_pairs_u32_u32_bits(const uint32_t* keys_in, uint32_t* keys_out, const uint32_t* vals_in,
-                             uint32_t* vals_out, uint64_t n, int end_bit, hipStream_t s) {
-  radix_sort<uint32_t, true>(keys_in, keys_out, vals_in, vals_out, n, end_bit, s, false);
-}
-
 void sort_pairs_u32_u32_split(const uint32_t* keys_in, uint32_t* keys_work,
                               const uint32_t* vals_in, uint32_t* vals_work, uint64_t n,
                               int end_bit, uint16_t* lo16, uint16_t* hi16, hipStream_t s) {
