@@ -62,11 +62,16 @@ __global__ void k_pb_bounds(const uint64_t* ofs, uint64_t V, uint32_t bb, uint32
 }
 
 // Per out-edge of an item: the destination bin as the sort key and (source
-// offset << 16 | destination offset) as the value; the source row is found
-// by binary search over the bin's row offsets, staged in LDS.  Reads and
-// writes follow the edge order (coalesced).
-// The same pass counts the item's out-edges per destination bin (LDS), into
-// th[b' * ni + i].
+// offset << 16 | destination offset) as the value.  The item is walked in
+// tiles of kKeyTile edges.  A tile's source rows: r0, the row holding its
+// first edge (one binary search over the bin's row offsets, staged in LDS),
+// and every later row starting inside the tile marks its first edge in LDS; a
+// running maximum over the tile then gives each edge its row.  Reads and
+// writes follow the edge order (coalesced).  The same pass counts the item's
+// out-edges per destination bin (LDS), into th[b' * ni + i].
+constexpr int kKeyPer = 8;
+constexpr uint32_t kKeyTile = kBuildBlock * kKeyPer;
+
 template <uint32_t BB>
 __global__ __launch_bounds__(kBuildBlock) void k_pb_keys(const uint64_t* ofs, const uint32_t* adj,
                                                          const PbItem* items, uint32_t ni,
@@ -74,6 +79,8 @@ __global__ __launch_bounds__(kBuildBlock) void k_pb_keys(const uint64_t* ofs, co
                                                          uint32_t* val, uint32_t* th) {
   constexpr uint32_t R = 1u << BB;
   __shared__ uint32_t ro[R + 1];
+  __shared__ uint32_t mark[kKeyTile];
+  __shared__ uint32_t wmax[kBuildBlock / 64];
   extern __shared__ uint32_t h[];
   // neighbouring items write neighbouring counts: keep them on one XCD
   const uint32_t item = xcd_block(blockIdx.x, gridDim.x);
@@ -82,21 +89,60 @@ __global__ __launch_bounds__(kBuildBlock) void k_pb_keys(const uint64_t* ofs, co
   const uint32_t nrows = uint32_t(min(uint64_t(R), V - v0));
   for (uint32_t i = threadIdx.x; i <= nrows; i += blockDim.x) ro[i] = uint32_t(ofs[v0 + i]);
   for (uint32_t i = threadIdx.x; i < nb; i += blockDim.x) h[i] = 0;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   __syncthreads();
-  for (uint64_t e = it.e0 + threadIdx.x; e < it.e1; e += blockDim.x) {
-    const uint32_t v = adj[e];
-    // the row r with ro[r] <= e < ro[r + 1]; ro[0] <= e0 <= e < e1 <= ro[nrows]
+  for (uint64_t t0 = it.e0; t0 < it.e1; t0 += kKeyTile) {
+    const uint64_t t1 = min(t0 + kKeyTile, uint64_t(it.e1));
+    // r0: the last row starting at or before t0 (ro[0] <= e0 <= t0 < ro[nrows])
     uint32_t lo = 0, hi = nrows;
     while (hi - lo > 1) {
       const uint32_t mid = (lo + hi) >> 1;
-      if (ro[mid] <= e) lo = mid;
+      if (ro[mid] <= t0) lo = mid;
       else hi = mid;
     }
-    key[e] = v >> BB;
-    val[e] = lo << 16 | (v & (R - 1));
-    atomicAdd(&h[v >> BB], 1u);
+    const uint32_t r0 = lo;
+#pragma unroll
+    for (int j = 0; j < kKeyPer; j++) mark[threadIdx.x + j * kBuildBlock] = 0;
+    __syncthreads();
+    // rows after r0 that start inside the tile (ro ascending: a thread stops
+    // at its first row starting past the tile)
+    for (uint32_t r = r0 + 1 + threadIdx.x; r < nrows && ro[r] < t1; r += kBuildBlock)
+      atomicMax(&mark[ro[r] - t0], r);
+    __syncthreads();
+    // running maximum over the tile, thread t owning [kKeyPer t, kKeyPer (t+1))
+    uint32_t m[kKeyPer];
+    uint32_t run = 0;
+#pragma unroll
+    for (int j = 0; j < kKeyPer; j++) {
+      run = max(run, mark[threadIdx.x * kKeyPer + j]);
+      m[j] = run;
+    }
+    uint32_t inc = run;
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t y = __shfl_up(inc, o, 64);
+      if (lane >= o) inc = max(inc, y);
+    }
+    if (lane == 63) wmax[wave] = inc;
+    __syncthreads();
+    uint32_t before = __shfl_up(inc, 1, 64);
+    if (lane == 0) before = 0;
+    for (int w = 0; w < wave; w++) before = max(before, wmax[w]);
+#pragma unroll
+    for (int j = 0; j < kKeyPer; j++) mark[threadIdx.x * kKeyPer + j] = max(before, m[j]);
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kKeyPer; j++) {
+      const uint64_t e = t0 + uint64_t(j) * kBuildBlock + threadIdx.x;
+      if (e < t1) {
+        const uint32_t v = adj[e];
+        const uint32_t row = max(r0, mark[uint32_t(e - t0)]);
+        key[e] = v >> BB;
+        val[e] = row << 16 | (v & (R - 1));
+        atomicAdd(&h[v >> BB], 1u);
+      }
+    }
+    __syncthreads();  // mark and wmax are rewritten by the next tile
   }
-  __syncthreads();
   for (uint32_t i = threadIdx.x; i < nb; i += blockDim.x) th[uint64_t(i) * ni + item] = h[i];
 }
 
@@ -267,11 +313,13 @@ void build_pr_bins(ll_gpu_graph* g, const Csr& out, uint32_t bb, PrBins& pb) {
   const size_t lds = size_t(nb) * 4;
   LLG_CHECK(lds <= 64 * 1024, LL_GPU_ERANGE, "too many PageRank bins for one workgroup's LDS");
   {
-    // the key pass holds the bin's row offsets and the bin counters in LDS
+    // the key pass holds the bin's row offsets, a tile of row marks and the
+    // bin counters in LDS
     int dev = 0, max_lds = 0;
     LLG_HIP(hipGetDevice(&dev));
     LLG_HIP(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
-    LLG_CHECK(lds + ((size_t(1) << bb) + 1) * 4 <= size_t(max_lds), LL_GPU_ERANGE,
+    LLG_CHECK(lds + ((size_t(1) << bb) + 1) * 4 + kKeyTile * 4 + 64 <= size_t(max_lds),
+              LL_GPU_ERANGE,
               "binned PageRank: " + std::to_string(nb) + " bins of 2^" + std::to_string(bb) +
                   " vertices need more LDS per workgroup than the device has");
   }
