@@ -16,9 +16,11 @@
 //   B. finalises the bin's vertices: rank = (1-d)/N + d*S, contribution =
 //      rank/outdeg (FP32), kept in LDS;
 //   C. as the same vertices are the sources of chunks (b', b) for every b',
-//      writes the next iteration's values into those chunks, chunk by chunk
-//      (each slot names its source's offset in the bin, u16), so the writes
-//      of a wave are contiguous.
+//      writes the next iteration's values into those chunks (each slot names
+//      its source's offset in the bin, u16): a wave takes the chunks kGroupC
+//      at a time, their first 128 slots loaded together, so its writes are
+//      contiguous runs and several loads are in flight.  Neighbouring bins
+//      run on one XCD, whose L2 merges the partial lines at chunk ends.
 // Integer addition is associative, so a row sum does not depend on the order
 // its slots are added in: every run gives the same bits, whatever the
 // schedule or bin size.  Each term is the FP32 contribution truncated to a
