@@ -314,8 +314,10 @@ int ll_gpu_tuned(ll_gpu_ctx* ctx, const char* prefix, int* value);
  * pile; 0 = off, -1 = automatic = max_nodes / 32), "bfs_alpha" / "bfs_beta"
  * (direction-optimising BFS switches to bottom-up when the frontier's
  * out-edges exceed the unexplored edges / alpha, back to top-down when a
- * shrinking frontier holds under max_nodes / beta vertices; <= 0 = defaults
- * 14 and 24).  Results never depend on them. */
+ * shrinking frontier holds under max_nodes / beta vertices; <= 0 = defaults:
+ * alpha 10; beta 24, or no return to top-down for a search that treats the
+ * rows outside the snapshot's reach set as visited).  Results never depend
+ * on them. */
 int ll_gpu_set_option(ll_gpu_ctx* ctx, const char* name, long value);
 /* Drop the cached snapshot materialisations (the next call rebuilds them). */
 void ll_gpu_graph_drop_snapshots(ll_gpu_graph* g);
