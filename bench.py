@@ -366,7 +366,9 @@ def bench_scc(dg, E: int, V: int, args) -> dict:
 
 def bench_bfs_batch(dg, g, V: int, args, single: dict | None) -> dict:
     """Multi-source BFS over the batch sources in 64-bit lanes (and the
-    autotuned width), against the per-source cost of single-source runs."""
+    autotuned width), against the per-source cost of single-source runs.
+    Sweeps: pushes from small frontiers, binned sweeps while the rows some
+    search has not reached hold many edges, pulls otherwise."""
     import numpy as np
     roots = pick_sources(g, args.batch_sources, args.seed + 4)
     ctx = dg.ctx
@@ -379,11 +381,15 @@ def bench_bfs_batch(dg, g, V: int, args, single: dict | None) -> dict:
         _, reached, sweeps = dg.bfs_batch(roots, lane_width=lanes, want_levels=False)
         ctx.sync()
         t = time.perf_counter() - t0
-        n_k, ms_k = ctx.kernel_time("k_ms_pull")
+        kt = {k: ctx.kernel_time(k) for k in ("k_ms_pull", "k_msb_sweep", "k_ms_push")}
         ctx.profile(False)
         key = "lanes64" if lanes == 64 else \
             f"autotuned_lanes{dg.tuned(tuned_key('bfs_lanes', dg, None, len(roots)))}"
-        out[key] = {"seconds": round(t, 4), "sweeps": sweeps, "kernel_ms": round(ms_k, 3),
+        out[key] = {"seconds": round(t, 4), "sweeps": sweeps,
+                    "pull_sweeps": kt["k_ms_pull"][0], "pull_ms": round(kt["k_ms_pull"][1], 3),
+                    "binned_sweeps": kt["k_msb_sweep"][0],
+                    "binned_ms": round(kt["k_msb_sweep"][1], 3),
+                    "push_kernel_ms": round(kt["k_ms_push"][1], 3),
                     "reached_total": int(reached.sum())}
     # The same sources as sequential single-source runs, timed: they also give
     # the Graph500 numerator per source, E_comp(s) = sum of live out-degrees

@@ -1,16 +1,22 @@
 // Multi-source BFS with sources packed into lanes of a per-vertex bit word
-// (16, 32, 64 or 128 lanes).  Level l+1 of every source is one pull sweep over
-// the snapshot in-edges:
+// (16, 32, 64 or 128 lanes).  Level l+1 of every source is one sweep:
 //   next[v] = (OR_{u in in(v)} frontier[u]) & ~visited[v]
-// run on the row blocks of the in-CSR: a workgroup stages frontier[u] for a
-// run of short rows in LDS with coalesced neighbour loads and ORs each row in
-// one thread; long rows are split into pieces whose partial ORs are combined
-// in piece order.  Vertices already visited by every lane are skipped.
+// taken one of three ways, chosen per level from the previous sweep's report:
+//  - push, while the frontier's out-edges are few: a queue of the frontier
+//    vertices, their out-edges OR-ed into next[] edge-parallel;
+//  - binned, while the rows some search has not reached hold many in-edges:
+//    over the binned out-edge layout, every slot streamed (no random reads);
+//  - pull otherwise, on the row blocks of the in-CSR: a workgroup stages
+//    frontier[u] for a run of short rows in LDS with coalesced neighbour loads
+//    and ORs each row in one thread; long rows are split into pieces whose
+//    partial ORs are combined in piece order.  Rows already visited by every
+//    lane still searching are skipped.
 // Levels are unique per source, so each lane's levels equal a single-source
 // BFS from that lane's root (sssp_unweighted semantics, sssp.h:220-333).
 #include <climits>
 #include <string>
 
+#include "bins.h"
 #include "ll_gpu_internal.h"
 #include "rowblocks.h"
 
@@ -95,20 +101,30 @@ struct MsArgs {
   unsigned long long* found;  // striped OR of the lanes that discovered vertices
 };
 
-// Row r's new lanes: visited |= new, next frontier = new, levels; adds the
-// out-degree of a newly discovered vertex to `work` (next frontier's edges).
+// Row r discovered by lanes nw (not zero) over visited lanes vr: visited |=
+// nw, levels; adds the out-degree of r to `work` (next frontier's edges).
 template <typename W>
-__device__ inline W finalize(const MsArgs<W>& a, uint32_t r, W acc, unsigned long long& work) {
-  const W vr = a.vis[r];
-  const W nw = acc & ~vr;
-  a.nx[r] = nw;
-  if (nw == Lane<W>::zero()) return nw;
+__device__ inline void discover(const MsArgs<W>& a, uint32_t r, W vr, W nw,
+                                unsigned long long& work) {
   a.vis[r] = vr | nw;
   work += a.outdeg[r];
   if (a.lv) {
     for (int s = 0; s < a.n_src; s++)
       if (Lane<W>::test(nw, s)) a.lv[uint64_t(a.src_base + s) * a.V + r] = a.next_lv;
   }
+}
+
+// Row r's new lanes: next frontier = new, then discover(); adds the row's
+// in-degree `deg` to `open` while a lane still searching has not reached it.
+template <typename W>
+__device__ inline W finalize(const MsArgs<W>& a, uint32_t r, W acc, unsigned long long& work,
+                             unsigned long long& open, uint64_t deg) {
+  const W vr = a.vis[r];
+  const W nw = acc & ~vr;
+  a.nx[r] = nw;
+  if (!((vr | nw | a.done) == a.full)) open += deg;
+  if (nw == Lane<W>::zero()) return nw;
+  discover(a, r, vr, nw, work);
   return nw;
 }
 
@@ -124,34 +140,38 @@ __device__ inline uint32_t block_sum_u32(uint32_t x) {
   return t;
 }
 
-// Sweep report: the lanes that discovered vertices (OR) and the out-degree
-// sum of the discovered vertices.  A workgroup reduces its threads' values and
-// adds them to its stripe (blockIdx % kFlagStripes, one 128-byte line each:
-// word 0/1 = lanes 0-63/64-127, word 2 = out-degree sum), so hundreds of
-// thousands of workgroups spread over many L2 lines; the host combines the
-// stripes.
+// Sweep report: the lanes that discovered vertices (OR), the out-degree sum
+// of the discovered vertices and the in-degree sum of the rows a lane still
+// searching has not reached (pull and binned sweeps).  A workgroup reduces
+// its threads' values and adds them to its stripe (blockIdx % kFlagStripes,
+// one 128-byte line each: word 0/1 = lanes 0-63/64-127, word 2 = out-degree
+// sum, word 3 = open in-degree sum), so hundreds of thousands of workgroups
+// spread over many L2 lines; the host combines the stripes.
 constexpr int kFlagStripes = 64;
 constexpr int kFlagStride = 16;  // u64 per 128-byte line
 
-template <typename W>
-__device__ inline void block_report(unsigned long long* flags, W x, unsigned long long work) {
-  __shared__ unsigned long long part[3][kBlock / 64];
+template <typename W, int kThreads = kBlock>
+__device__ inline void block_report(unsigned long long* flags, W x, unsigned long long work,
+                                    unsigned long long open) {
+  __shared__ unsigned long long part[4][kThreads / 64];
   unsigned long long lo = Lane<W>::lo64(x), hi = Lane<W>::hi64(x);
   for (int o = 32; o > 0; o >>= 1) {
     lo |= __shfl_xor(lo, o, 64);
     hi |= __shfl_xor(hi, o, 64);
     work += __shfl_xor(work, o, 64);
+    open += __shfl_xor(open, o, 64);
   }
   if ((threadIdx.x & 63) == 0) {
     part[0][threadIdx.x >> 6] = lo;
     part[1][threadIdx.x >> 6] = hi;
     part[2][threadIdx.x >> 6] = work;
+    part[3][threadIdx.x >> 6] = open;
   }
   __syncthreads();
-  if (threadIdx.x < 3) {
+  if (threadIdx.x < 4) {
     unsigned long long t = 0;
-    for (int i = 0; i < kBlock / 64; i++)
-      t = threadIdx.x < 2 ? (t | part[threadIdx.x][i]) : t + part[2][i];
+    for (int i = 0; i < kThreads / 64; i++)
+      t = threadIdx.x < 2 ? (t | part[threadIdx.x][i]) : t + part[threadIdx.x][i];
     unsigned long long* dst = &flags[(blockIdx.x % kFlagStripes) * kFlagStride + threadIdx.x];
     if (t) {
       if (threadIdx.x < 2) atomicOr(dst, t);
@@ -164,6 +184,7 @@ __global__ void k_zero_flags(unsigned long long* flags) {  // <<<1, kFlagStripes
   flags[threadIdx.x * kFlagStride] = 0ull;
   flags[threadIdx.x * kFlagStride + 1] = 0ull;
   flags[threadIdx.x * kFlagStride + 2] = 0ull;
+  flags[threadIdx.x * kFlagStride + 3] = 0ull;
 }
 
 template <typename W>
@@ -174,12 +195,13 @@ __global__ __launch_bounds__(kBlock) void k_ms_pull(MsArgs<W> a) {
   const RowItem it = a.items[blockIdx.x];
   W cnt = Lane<W>::zero();  // lanes this thread's row was discovered by
   unsigned long long work = 0;  // out-edges of the vertices it discovered
+  unsigned long long open = 0;  // in-edges of its row while a lane has not reached it
   if (it.npieces == 0) {
     const uint32_t nrows = it.row_end - it.row_begin;
     // rows already reached from every lane still searching need no gathers
-    const bool open =
+    const bool live =
         threadIdx.x < nrows && !((a.vis[it.row_begin + threadIdx.x] | a.done) == a.full);
-    if (!__syncthreads_or(open)) {
+    if (!__syncthreads_or(live)) {
       if (threadIdx.x < nrows) a.nx[it.row_begin + threadIdx.x] = Lane<W>::zero();
       return;
     }
@@ -191,23 +213,23 @@ __global__ __launch_bounds__(kBlock) void k_ms_pull(MsArgs<W> a) {
     // once every lane has reached it
     const uint32_t lo0 = threadIdx.x < nrows ? uint32_t(rofs[threadIdx.x] - e0) : 0u;
     const uint32_t hi0 = threadIdx.x < nrows ? uint32_t(rofs[threadIdx.x + 1] - e0) : 0u;
-    const uint32_t open_edges = block_sum_u32(open ? hi0 - lo0 : 0u);
+    const uint32_t open_edges = block_sum_u32(live ? hi0 - lo0 : 0u);
     if (open_edges * 4 < ne) {
       if (threadIdx.x < nrows) {
         const uint32_t r = it.row_begin + threadIdx.x;
-        if (open) {
+        if (live) {
           const W vr = a.vis[r];
           W acc = Lane<W>::zero();
           for (uint32_t j = lo0; j < hi0; j++) {
             acc = acc | a.fr[a.adj[e0 + j]];
             if ((acc | vr | a.done) == a.full) break;
           }
-          cnt = finalize(a, r, acc, work);
+          cnt = finalize(a, r, acc, work, open, hi0 - lo0);
         } else {
           a.nx[r] = Lane<W>::zero();
         }
       }
-      block_report(a.found, cnt, work);
+      block_report(a.found, cnt, work, open);
       return;
     }
     // entries of rows every lane has reached are neither loaded nor gathered:
@@ -215,8 +237,8 @@ __global__ __launch_bounds__(kBlock) void k_ms_pull(MsArgs<W> a) {
     __shared__ uint8_t rowof[kCap + kCap / 2];
     __shared__ uint8_t openf[kRowsPerItem];
     if (threadIdx.x < nrows) {
-      openf[threadIdx.x] = open ? 1 : 0;
-      if (open)
+      openf[threadIdx.x] = live ? 1 : 0;
+      if (live)
         for (uint32_t j = lo0; j < hi0; j++) rowof[j] = uint8_t(threadIdx.x);
     }
     __syncthreads();
@@ -247,7 +269,7 @@ __global__ __launch_bounds__(kBlock) void k_ms_pull(MsArgs<W> a) {
         W acc = Lane<W>::zero();
         const uint32_t lo = uint32_t(rofs[threadIdx.x] - e0), hi = uint32_t(rofs[threadIdx.x + 1] - e0);
         for (uint32_t j = lo; j < hi; j++) acc = acc | vals[j];
-        cnt = finalize(a, r, acc, work);
+        cnt = finalize(a, r, acc, work, open, hi - lo);
       } else {
         a.nx[r] = Lane<W>::zero();
       }
@@ -268,28 +290,29 @@ __global__ __launch_bounds__(kBlock) void k_ms_pull(MsArgs<W> a) {
       __syncthreads();
     }
     if (threadIdx.x == 0) {
-      if (it.npieces == 1) cnt = finalize(a, r, wred[0], work);
+      if (it.npieces == 1) cnt = finalize(a, r, wred[0], work, open, hi - lo);
       else a.part[blockIdx.x] = wred[0];
     }
     (void)red;
   }
-  block_report(a.found, cnt, work);
+  block_report(a.found, cnt, work, open);
 }
 
 template <typename W>
 __global__ void k_ms_fixup(MsArgs<W> a, uint64_t nitems) {
   const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   W cnt = Lane<W>::zero();
-  unsigned long long work = 0;
+  unsigned long long work = 0, open = 0;
   if (i < nitems) {
     const RowItem it = a.items[i];
     if (it.npieces > 1 && it.piece == 0) {
       W acc = a.part[i];
       for (uint32_t p = 1; p < it.npieces; p++) acc = acc | a.part[i + p];
-      cnt = finalize(a, it.row_begin, acc, work);
+      const uint32_t r = it.row_begin;
+      cnt = finalize(a, r, acc, work, open, a.ofs[r + 1] - a.ofs[r]);
     }
   }
-  block_report(a.found, cnt, work);
+  block_report(a.found, cnt, work, open);
 }
 
 template <typename W>
@@ -442,7 +465,8 @@ __global__ __launch_bounds__(kBlock) void k_ms_push(const uint32_t* q, const uin
   }
 }
 
-// After a push: every vertex settles its pushed lanes like a pull row would.
+// After a push: every vertex settles its pushed lanes like a pull row would
+// (rows reached by no push are not read: the open in-degree sum is not kept).
 template <typename W>
 __global__ __launch_bounds__(kBlock) void k_ms_settle(MsArgs<W> a) {
   const uint64_t r = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
@@ -450,9 +474,127 @@ __global__ __launch_bounds__(kBlock) void k_ms_settle(MsArgs<W> a) {
   unsigned long long work = 0;
   if (r < a.V) {
     const W acc = a.nx[r];
-    if (!(acc == Lane<W>::zero())) cnt = finalize(a, uint32_t(r), acc, work);
+    if (!(acc == Lane<W>::zero())) {
+      const W vr = a.vis[r];
+      const W nw = acc & ~vr;
+      a.nx[r] = nw;
+      if (!(nw == Lane<W>::zero())) discover(a, uint32_t(r), vr, nw, work);
+      cnt = nw;
+    }
   }
-  block_report(a.found, cnt, work);
+  block_report(a.found, cnt, work, 0ull);
+}
+
+// ---- binned sweeps for levels with many open rows ---------------------------
+// Over the binned out-edge layout (bins.h): a scatter launch writes each
+// source's frontier lanes into its slots (one workgroup per source bin, the
+// bin's lanes staged in LDS, the slots written in chunk runs), then a gather
+// launch ORs each destination bin's slots into per-vertex lane words in LDS
+// (streamed, no random reads) and settles the bin's vertices as a pull row
+// would.  Every slot is read and written once per sweep, whatever the rows'
+// state: taken while the rows still open hold a large share of the edges.
+constexpr int kBinBlock = 1024;
+constexpr int kBinUnroll = 8;  // gather: slots per thread in flight
+constexpr int kBinGroup = 8;   // scatter: chunks per wave in flight
+
+// LDS lane word of the gather (16-bit lanes OR into 32-bit words)
+template <typename W> struct BinAcc { using T = W; };
+template <> struct BinAcc<uint16_t> { using T = uint32_t; };
+
+__device__ inline void lds_or(uint32_t* p, uint32_t x) { atomicOr(p, x); }
+__device__ inline void lds_or(uint64_t* p, uint64_t x) {
+  atomicOr(reinterpret_cast<unsigned long long*>(p), (unsigned long long)x);
+}
+__device__ inline void lds_or(W128* p, W128 x) {
+  if (x.a) atomicOr(reinterpret_cast<unsigned long long*>(&p->a), (unsigned long long)x.a);
+  if (x.b) atomicOr(reinterpret_cast<unsigned long long*>(&p->b), (unsigned long long)x.b);
+}
+
+template <uint32_t BB, typename W>
+__global__ __launch_bounds__(kBinBlock) void k_msb_scatter(const W* fr, uint64_t V, uint32_t nb,
+                                                           const uint2* ct, const uint16_t* sidx,
+                                                           W* sv) {
+  constexpr uint32_t R = 1u << BB;
+  __shared__ W cl[R];
+  // neighbouring source bins write neighbouring chunks of every destination
+  // bin: keep them on one XCD
+  const uint64_t b = xcd_block(blockIdx.x, gridDim.x);
+  const uint64_t v0 = b << BB;
+  const uint32_t nrows = uint32_t(min(uint64_t(R), V - v0));
+  for (uint32_t i = threadIdx.x; i < R; i += kBinBlock)
+    cl[i] = i < nrows ? fr[v0 + i] : Lane<W>::zero();
+  __syncthreads();
+  scatter_chunks<kBinBlock, kBinGroup>(ct + b * nb, nb, sidx, cl, sv);
+}
+
+template <uint32_t BB, typename W>
+__global__ __launch_bounds__(kBinBlock) void k_msb_gather(MsArgs<W> a, const W* sv,
+                                                          const uint16_t* ldst,
+                                                          const uint64_t* bstart,
+                                                          const uint32_t* indeg) {
+  using A = typename BinAcc<W>::T;
+  constexpr uint32_t R = 1u << BB;
+  constexpr uint32_t kPer = R / kBinBlock;
+  static_assert(kPer >= 1 && R % kBinBlock == 0, "bin rows must be a multiple of the block");
+  __shared__ A acc[R];
+  const uint64_t b = xcd_block(blockIdx.x, gridDim.x);
+  const uint64_t v0 = b << BB;
+  const uint32_t nrows = uint32_t(min(uint64_t(R), a.V - v0));
+  for (uint32_t i = threadIdx.x; i < R; i += kBinBlock) acc[i] = A(Lane<W>::zero());
+  __syncthreads();
+  const uint64_t k0 = bstart[b], k1 = bstart[b + 1];
+  for (uint64_t k = k0 + threadIdx.x; k < k1; k += kBinUnroll * uint64_t(kBinBlock)) {
+    W x[kBinUnroll];
+    uint16_t d[kBinUnroll];
+#pragma unroll
+    for (int u = 0; u < kBinUnroll; u++) {
+      const uint64_t q = k + uint64_t(u) * kBinBlock;
+      x[u] = q < k1 ? sv[q] : Lane<W>::zero();
+      d[u] = q < k1 ? ldst[q] : uint16_t(0);
+    }
+#pragma unroll
+    for (int u = 0; u < kBinUnroll; u++)
+      if (!(x[u] == Lane<W>::zero())) lds_or(&acc[d[u]], A(x[u]));
+  }
+  __syncthreads();
+  W cnt = Lane<W>::zero();
+  unsigned long long work = 0, open = 0;
+#pragma unroll
+  for (uint32_t j = 0; j < kPer; j++) {
+    const uint32_t i = threadIdx.x + j * kBinBlock;
+    if (i < nrows) {
+      const uint32_t r = uint32_t(v0 + i);
+      const W vr = a.vis[r];
+      const W nw = W(acc[i]) & ~vr;
+      a.nx[r] = nw;
+      if (!((vr | nw | a.done) == a.full)) open += indeg[r];
+      if (!(nw == Lane<W>::zero())) {
+        discover(a, r, vr, nw, work);
+        cnt = cnt | nw;
+      }
+    }
+  }
+  block_report<W, kBinBlock>(a.found, cnt, work, open);
+}
+
+// The bin sizes a lane width can gather: a bin's lane words fill at most
+// 128 KiB of LDS.
+template <typename W>
+constexpr bool bins_fit(uint32_t bb) {
+  return (uint64_t(sizeof(typename BinAcc<W>::T)) << bb) <= 128 * 1024;
+}
+
+template <uint32_t BB, typename W>
+void launch_binned(const PrBins& pb, const W* fr, W* sv, const MsArgs<W>& a,
+                   const uint32_t* indeg, hipStream_t s) {
+  if constexpr (bins_fit<W>(BB)) {
+    k_msb_scatter<BB, W><<<pb.nbins, kBinBlock, 0, s>>>(fr, a.V, pb.nbins, pb.ct.as<uint2>(),
+                                                        pb.sidx.as<uint16_t>(), sv);
+    k_msb_gather<BB, W><<<pb.nbins, kBinBlock, 0, s>>>(a, sv, pb.ldst.as<uint16_t>(),
+                                                       pb.bstart.as<uint64_t>(), indeg);
+  } else {
+    throw Error{LL_GPU_EINVAL, "binned sweep: bins too large for the lane width"};
+  }
 }
 
 template <typename W>
@@ -517,22 +659,41 @@ void msbfs_batch(ll_gpu_graph* g, int level, const int64_t* d_roots, int n, int 
   LLG_HIP(hipGetLastError());
   static_assert(kFlagStripes * kFlagStride * 8 <= kReadbackBytes, "flags fit the read-back buffer");
   const uint64_t* fl = static_cast<const uint64_t*>(ctx->readback);
-  auto read_report = [&](uint64_t& lo, uint64_t& hi, uint64_t& work) {
+  auto read_report = [&](uint64_t& lo, uint64_t& hi, uint64_t& work, uint64_t& open) {
     LLG_HIP(hipMemcpyAsync(ctx->readback, found.p, kFlagStripes * kFlagStride * 8,
                            hipMemcpyDeviceToHost, s));
     LLG_HIP(hipStreamSynchronize(s));
-    lo = hi = work = 0;
+    lo = hi = work = open = 0;
     for (int i = 0; i < kFlagStripes; i++) {
       lo |= fl[i * kFlagStride];
       hi |= fl[i * kFlagStride + 1];
       work += fl[i * kFlagStride + 2];
+      open += fl[i * kFlagStride + 3];
     }
   };
-  uint64_t lo, hi, m_f;
-  read_report(lo, hi, m_f);  // m_f: out-edges of the frontier (the roots)
+  uint64_t lo, hi, m_f, open;
+  read_report(lo, hi, m_f, open);  // m_f: out-edges of the frontier (the roots)
   // Push while the frontier's out-edges are under 1/kPushRatio of the edges
-  // (the first sweeps from a handful of roots); pull otherwise.
+  // (the first sweeps from a handful of roots).  Otherwise a binned sweep
+  // while the rows some searching lane has not reached hold at least
+  // kBinOpen of the in-edges (a pull skips the other rows; the binned sweep
+  // streams every edge at about half a full pull's cost), else a pull.
   constexpr uint64_t kPushRatio = 16;
+  constexpr double kBinOpen = 0.4;
+  const long bins_opt = ctx->options.count("bfs_batch_bins") ? ctx->options["bfs_batch_bins"] : -1;
+  const PrBins* pb = nullptr;
+  const uint32_t* indeg = nullptr;
+  if (bins_opt != 0) {
+    const long bb = ctx->options.count("pr_bin_bits") ? ctx->options["pr_bin_bits"] : -1;
+    PrBins* p = ensure_pr_bins(g, level, bb);
+    if (!p && bins_opt == 1) p = ensure_pr_bins(g, level, 12);
+    if (p && bins_fit<W>(p->bb)) {
+      pb = p;
+      indeg = ensure_degrees(g, LL_GPU_DIR_IN, level);
+    }
+  }
+  uint64_t open_in = in.E;  // in-edges of the rows a searching lane has not reached
+  DevBuf sv;                // binned sweeps: the slots' lane words
   const Csr* out = nullptr;
   DevBuf q, qdeg, qofs, qn, blk;
   uint64_t lv_count = 0;
@@ -556,7 +717,10 @@ void msbfs_batch(ll_gpu_graph* g, int level, const int64_t* d_roots, int n, int 
     a.outdeg = outdeg;
     a.found = found.as<unsigned long long>();
     k_zero_flags<<<1, kFlagStripes, 0, s>>>(a.found);
-    if (m_f * kPushRatio < in.E) {
+    const bool push = m_f * kPushRatio < in.E;
+    const bool binned =
+        !push && pb && (bins_opt == 1 || double(open_in) >= kBinOpen * double(in.E));
+    if (push) {
       if (!out) {
         out = &ensure_csr(g, LL_GPU_DIR_OUT, level);
         q.alloc(V * 4, s);
@@ -591,6 +755,14 @@ void msbfs_batch(ll_gpu_graph* g, int level, const int64_t* d_roots, int n, int 
         }
       }
       k_ms_settle<W><<<grid_for(V, kBlock), kBlock, 0, s>>>(a);
+    } else if (binned) {
+      if (!sv.p) sv.alloc(std::max<uint64_t>(in.E, 1) * sizeof(W), s);
+      TimedLaunch t(ctx, "k_msb_sweep");
+      switch (pb->bb) {
+        case 12: launch_binned<12, W>(*pb, fr.as<W>(), sv.as<W>(), a, indeg, s); break;
+        case 13: launch_binned<13, W>(*pb, fr.as<W>(), sv.as<W>(), a, indeg, s); break;
+        default: launch_binned<14, W>(*pb, fr.as<W>(), sv.as<W>(), a, indeg, s); break;
+      }
     } else {
       {
         TimedLaunch t(ctx, "k_ms_pull");
@@ -599,7 +771,9 @@ void msbfs_batch(ll_gpu_graph* g, int level, const int64_t* d_roots, int n, int 
       k_ms_fixup<W><<<grid_for(rb.n, 256), 256, 0, s>>>(a, rb.n);
     }
     LLG_HIP(hipGetLastError());
-    read_report(lo, hi, m_f);  // lanes that discovered vertices = the next frontier's lanes
+    // lanes that discovered vertices = the next frontier's lanes
+    read_report(lo, hi, m_f, open);
+    if (!push) open_in = open;
     lv_count++;
     if ((lo | hi) == 0) break;
     done = full & ~Lane<W>::from64(lo, hi);

@@ -235,7 +235,8 @@ int ll_gpu_set_option(ll_gpu_ctx* c, const char* name, long value) {
   if (!c || !name) return LL_GPU_EINVAL;
   static const char* known[] = {"pr_reorder", "pr_cap", "pr_seg_bits", "pr_item_sort",
                                 "pr_hot_bits", "pr_bin_bits", "sssp_delta_milli",
-                                "sssp_min_queue", "bfs_alpha", "bfs_beta", "bfs_reach"};
+                                "sssp_min_queue", "bfs_alpha", "bfs_beta", "bfs_reach",
+                                "bfs_batch_bins"};
   try {
     for (const char* k : known)
       if (std::strcmp(k, name) == 0) {

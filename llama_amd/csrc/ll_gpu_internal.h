@@ -408,6 +408,12 @@ void build_pr_hotcold(ll_gpu_graph* g, const Csr& in, const uint32_t* slot_adj, 
 // Binned PageRank (pr_bins.hip): layout of `out` with bins of 2^bb
 // destinations, and `iters` iterations over it (FP32 ranks in rank_final).
 void build_pr_bins(ll_gpu_graph* g, const Csr& out, uint32_t bb, PrBins& pb);
+// The layout of level `level`'s out-CSR with bins of 2^bin_bits vertices
+// (12..14), built on first use; -1 = automatic (bins of 2^13 while the edges
+// between two such bins number 32 or more on average, else 2^14, while E <
+// 2^32 and the bins of 2^14 number at most 16384; otherwise none); nullptr
+// when there are none (0 or the automatic rule's "none").
+PrBins* ensure_pr_bins(ll_gpu_graph* g, int level, long bin_bits);
 void pagerank_bins(ll_gpu_graph* g, const Csr& out, const uint32_t* outdeg, PrBins& pb,
                    int iters, double damping, DevBuf& rank_final, double* last_diff);
 

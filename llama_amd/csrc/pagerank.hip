@@ -321,21 +321,11 @@ void pagerank_device(ll_gpu_graph* g, int level, int iters, double damping, uint
   // against 1.49 ms with 2^14; s27: 8.6 slots per chunk, 17.1 ms against
   // 9.2 ms with 2^14), as long as the bins of 2^14 number at most 16384.
   {
-    long bin_bits = ctx->options.count("pr_bin_bits") ? ctx->options["pr_bin_bits"] : -1;
-    const Csr& out = ensure_csr(g, LL_GPU_DIR_OUT, level);
-    if (bin_bits < 0) {
-      const uint64_t nb13 = (V + (uint64_t(1) << 13) - 1) >> 13;
-      const uint64_t nb14 = (V + (uint64_t(1) << 14) - 1) >> 14;
-      if (out.E >= (uint64_t(1) << 32) || nb14 > 16384) bin_bits = 0;
-      else bin_bits = out.E >= 32 * nb13 * nb13 ? 13 : 14;
-    }
-    if (bin_bits > 0) {
-      LLG_CHECK(bin_bits >= 12 && bin_bits <= 14, LL_GPU_EINVAL, "pr_bin_bits must be 0 or 12..14");
+    const long bin_bits = ctx->options.count("pr_bin_bits") ? ctx->options["pr_bin_bits"] : -1;
+    if (PrBins* pb = ensure_pr_bins(g, level, bin_bits)) {
+      const Csr& out = ensure_csr(g, LL_GPU_DIR_OUT, level);
       const uint32_t* od = ensure_degrees(g, LL_GPU_DIR_OUT, level);
-      PrBins& pb = g->pr_bins;
-      if (!(pb.valid && pb.level == out.level && pb.bb == uint32_t(bin_bits)))
-        build_pr_bins(g, out, uint32_t(bin_bits), pb);
-      pagerank_bins(g, out, od, pb, iters, damping, rank_final, last_diff);
+      pagerank_bins(g, out, od, *pb, iters, damping, rank_final, last_diff);
       return;
     }
   }

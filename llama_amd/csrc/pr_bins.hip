@@ -36,6 +36,7 @@
 #include <algorithm>
 #include <vector>
 
+#include "bins.h"
 #include "ll_gpu_internal.h"
 
 namespace llgpu {
@@ -240,41 +241,8 @@ __global__ __launch_bounds__(kIterBlock) void k_pb_iter(PbArgs a) {
 #pragma unroll
     for (uint32_t j = 0; j < kPer; j++) cl[threadIdx.x + j * kIterBlock] = c[j];
     __syncthreads();
-    // chunks (b', b): 64 table entries per wave step, then kGroupC chunks at a
-    // time, the first 128 slots of each requested together (two per lane)
-    // before any is written; longer chunks finish 64 slots at a time
-    const uint2* row = a.ct + b * a.nb;
-    for (uint32_t c0 = uint32_t(wave) * 64; c0 < a.nb; c0 += (kIterBlock / 64) * 64) {
-      const uint32_t mine = c0 + lane;
-      const uint2 t = mine < a.nb ? row[mine] : make_uint2(0u, 0u);
-      const uint32_t n = min(64u, a.nb - c0);
-      for (uint32_t j = 0; j < n; j += kGroupC) {
-        uint32_t st[kGroupC], len[kGroupC];
-        uint16_t s0[kGroupC], s1[kGroupC];
-#pragma unroll
-        for (int q = 0; q < kGroupC; q++) {
-          const int src = int(min(j + uint32_t(q), n - 1));
-          st[q] = __shfl(t.x, src, 64);
-          len[q] = j + uint32_t(q) < n ? __shfl(t.y, src, 64) : 0u;
-        }
-#pragma unroll
-        for (int q = 0; q < kGroupC; q++) {
-          s0[q] = uint32_t(lane) < len[q] ? a.sidx[st[q] + lane] : uint16_t(0);
-          s1[q] = uint32_t(lane) + 64 < len[q] ? a.sidx[st[q] + lane + 64] : uint16_t(0);
-        }
-#pragma unroll
-        for (int q = 0; q < kGroupC; q++) {
-          if (uint32_t(lane) < len[q]) a.val_out[st[q] + lane] = cl[s0[q]];
-          if (uint32_t(lane) + 64 < len[q]) a.val_out[st[q] + lane + 64] = cl[s1[q]];
-        }
-#pragma unroll
-        for (int q = 0; q < kGroupC; q++)
-          for (uint32_t k = 128 + uint32_t(lane); k < len[q]; k += 64) {
-            const uint32_t slot = st[q] + k;
-            a.val_out[slot] = cl[a.sidx[slot]];
-          }
-      }
-    }
+    // chunks (b', b) of every destination bin b'
+    scatter_chunks<kIterBlock, kGroupC>(a.ct + b * a.nb, a.nb, a.sidx, cl, a.val_out);
   }
   if (kDiff) {
     for (int o = 32; o > 0; o >>= 1) dsum += __shfl_down(dsum, o, 64);
@@ -396,6 +364,25 @@ void build_pr_bins(ll_gpu_graph* g, const Csr& out, uint32_t bb, PrBins& pb) {
   pb.level = out.level;
   pb.bb = bb;
   pb.nbins = nb;
+}
+
+PrBins* ensure_pr_bins(ll_gpu_graph* g, int level, long bin_bits) {
+  const Csr& out = ensure_csr(g, LL_GPU_DIR_OUT, level);
+  const uint64_t V = g->V;
+  if (bin_bits < 0) {
+    // bins of 2^13 while a chunk holds 32 slots on average, else 2^14, while
+    // E < 2^32 and the bins of 2^14 number at most 16384; otherwise none
+    const uint64_t nb13 = (V + (uint64_t(1) << 13) - 1) >> 13;
+    const uint64_t nb14 = (V + (uint64_t(1) << 14) - 1) >> 14;
+    if (out.E >= (uint64_t(1) << 32) || nb14 > 16384) bin_bits = 0;
+    else bin_bits = out.E >= 32 * nb13 * nb13 ? 13 : 14;
+  }
+  if (bin_bits == 0) return nullptr;
+  LLG_CHECK(bin_bits >= 12 && bin_bits <= 14, LL_GPU_EINVAL, "bin bits must be 0 or 12..14");
+  PrBins& pb = g->pr_bins;
+  if (!(pb.valid && pb.level == out.level && pb.bb == uint32_t(bin_bits)))
+    build_pr_bins(g, out, uint32_t(bin_bits), pb);
+  return &pb;
 }
 
 void pagerank_bins(ll_gpu_graph* g, const Csr& out, const uint32_t* outdeg, PrBins& pb,

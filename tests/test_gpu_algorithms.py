@@ -72,6 +72,35 @@ def test_bfs_batch_matches_single_source(ctx, ml):
     assert np.array_equal(reached0, reached[:40])
 
 
+def test_bfs_batch_binned_sweeps(ctx):
+    """Sweeps over the binned out-edge layout (bins of 2^12 over 2^14
+    vertices: several bins, chunks of every size) give every lane the
+    single-source levels: forced for every sweep that does not push (1), in
+    the automatic mix of push, binned and pull sweeps (-1), and off (0)."""
+    g = build_multilevel(14, levels=3, ins_frac=0.02, del_frac=0.01, seed=21, edge_factor=8)
+    o = oracle.OracleGraph.from_host(g)
+    dg = DeviceGraph(ctx, g)
+    roots = np.concatenate([roots_with_edges(g, 130, seed=5), [7]])
+    refs = {j: o.bfs(int(roots[j])) for j in (0, 1, 17, 63, 64, 100, 127, 128, 130)}
+    ctx.set_option("pr_bin_bits", 12)
+    try:
+        for mode in (1, -1, 0):
+            ctx.set_option("bfs_batch_bins", mode)
+            for lanes in (16, 32, 64, 128):
+                gpu_case(f"bfs_batch bins mode={mode} lanes={lanes}")
+                ctx.profile(True)
+                lv, reached, _ = dg.bfs_batch(roots, lane_width=lanes)
+                binned, _ = ctx.kernel_time("k_msb_sweep")
+                ctx.profile(False)
+                assert (binned > 0) == (mode != 0), (mode, lanes, binned)
+                for j, (ref, n) in refs.items():
+                    assert np.array_equal(lv[j], ref), (mode, lanes, j)
+                    assert reached[j] == n
+    finally:
+        ctx.set_option("bfs_batch_bins", -1)
+        ctx.set_option("pr_bin_bits", -1)
+
+
 # ---- triangles ---------------------------------------------------------------------
 def test_triangles_match_oracle(ctx, ml):
     g, o = ml
