@@ -320,9 +320,9 @@ int ll_gpu_tuned(ll_gpu_ctx* ctx, const char* prefix, int* value);
  * (multi-source BFS sweeps over the binned out-edge layout of "pr_bin_bits":
  * -1 = automatic = where that layout exists and fits the lane width in LDS,
  * for the sweeps that do not push while the rows some search has not reached
- * hold at least 40% of the in-edges; 0 = off; 1 = every sweep that does not
- * push, with bins of 2^12 where "pr_bin_bits" names none).  Results never
- * depend on them. */
+ * hold at least "bfs_batch_bin_open" percent of the in-edges (default 40);
+ * 0 = off; 1 = every sweep that does not push, with bins of 2^12 where
+ * "pr_bin_bits" names none).  Results never depend on them. */
 int ll_gpu_set_option(ll_gpu_ctx* ctx, const char* name, long value);
 /* Drop the cached snapshot materialisations (the next call rebuilds them). */
 void ll_gpu_graph_drop_snapshots(ll_gpu_graph* g);

@@ -102,29 +102,34 @@ struct MsArgs {
 };
 
 // Row r discovered by lanes nw (not zero) over visited lanes vr: visited |=
-// nw, levels; adds the out-degree of r to `work` (next frontier's edges).
+// nw, levels; adds the out-degree of r to `work` (next frontier's edges) and
+// returns it.
 template <typename W>
-__device__ inline void discover(const MsArgs<W>& a, uint32_t r, W vr, W nw,
-                                unsigned long long& work) {
+__device__ inline uint32_t discover(const MsArgs<W>& a, uint32_t r, W vr, W nw,
+                                    unsigned long long& work) {
   a.vis[r] = vr | nw;
-  work += a.outdeg[r];
+  const uint32_t od = a.outdeg[r];
+  work += od;
   if (a.lv) {
     for (int s = 0; s < a.n_src; s++)
       if (Lane<W>::test(nw, s)) a.lv[uint64_t(a.src_base + s) * a.V + r] = a.next_lv;
   }
+  return od;
 }
 
-// Row r's new lanes: next frontier = new, then discover(); adds the row's
-// in-degree `deg` to `open` while a lane still searching has not reached it.
+// Row r's new lanes: discover(), next frontier = the new lanes of a vertex
+// with out-edges (the frontier words of a vertex without any stay zero: no
+// sweep propagates from it); adds the row's in-degree `deg` to `open` while
+// a lane still searching has not reached it.  Returns the new lanes.
 template <typename W>
 __device__ inline W finalize(const MsArgs<W>& a, uint32_t r, W acc, unsigned long long& work,
                              unsigned long long& open, uint64_t deg) {
   const W vr = a.vis[r];
   const W nw = acc & ~vr;
-  a.nx[r] = nw;
   if (!((vr | nw | a.done) == a.full)) open += deg;
-  if (nw == Lane<W>::zero()) return nw;
-  discover(a, r, vr, nw, work);
+  W keep = Lane<W>::zero();
+  if (!(nw == Lane<W>::zero()) && discover(a, r, vr, nw, work) > 0) keep = nw;
+  a.nx[r] = keep;
   return nw;
 }
 
@@ -180,11 +185,16 @@ __device__ inline void block_report(unsigned long long* flags, W x, unsigned lon
   }
 }
 
+// Stripe 0's word 4 counts the vertices a push sweep reaches (kTouchCount),
+// word 5 those of them with out-edges, listed by the settle (kListCount);
+// word 6 is set when a push workgroup's edges span more queue entries than
+// it stages (kOverflow: an entry without out-edges, which no list holds).
+constexpr int kTouchCount = 4;
+constexpr int kListCount = 5;
+constexpr int kOverflow = 6;
+
 __global__ void k_zero_flags(unsigned long long* flags) {  // <<<1, kFlagStripes>>>
-  flags[threadIdx.x * kFlagStride] = 0ull;
-  flags[threadIdx.x * kFlagStride + 1] = 0ull;
-  flags[threadIdx.x * kFlagStride + 2] = 0ull;
-  flags[threadIdx.x * kFlagStride + 3] = 0ull;
+  for (int w = 0; w <= kOverflow; w++) flags[threadIdx.x * kFlagStride + w] = 0ull;
 }
 
 template <typename W>
@@ -316,11 +326,12 @@ __global__ void k_ms_fixup(MsArgs<W> a, uint64_t nitems) {
 }
 
 template <typename W>
-__global__ void k_ms_init(W* vis, W* fr, uint64_t V, W init) {
+__global__ void k_ms_init(W* vis, W* fr, W* nx, uint64_t V, W init) {
   const uint64_t v = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (v < V) {
     vis[v] = init;
     fr[v] = Lane<W>::zero();
+    nx[v] = Lane<W>::zero();
   }
 }
 
@@ -329,36 +340,43 @@ __global__ void k_fill_i32(int32_t* p, uint64_t n, int32_t x) {
   if (i < n) p[i] = x;
 }
 
+// The roots' lanes; the first frontier: the roots with out-edges, listed
+// once each, their count into stripe 0's word kListCount and their out-edges
+// into word 2.
 template <typename W>
 __global__ void k_ms_roots(W* vis, W* fr, const int64_t* roots, int n, int32_t* lv, uint64_t V,
-                           int src_base) {
+                           int src_base, const uint32_t* outdeg, uint32_t* lst,
+                           unsigned long long* flags) {
   // one thread: roots may repeat across lanes
+  unsigned long long nl = 0, work = 0;
   for (int s = 0; s < n; s++) {
     const uint64_t r = uint64_t(roots[s]);
     W bit = Lane<W>::low(s + 1) & ~Lane<W>::low(s);
     vis[r] = vis[r] | bit;
-    fr[r] = fr[r] | bit;
     if (lv) lv[uint64_t(src_base + s) * V + r] = 0;
+    const uint32_t od = outdeg[r];
+    if (od == 0) continue;
+    if (fr[r] == Lane<W>::zero()) {
+      lst[nl++] = uint32_t(r);
+      work += od;
+    }
+    fr[r] = fr[r] | bit;
   }
+  flags[kListCount] = nl;
+  flags[2] = work;
 }
 
 __global__ void k_zero_ull(unsigned long long* p) { *p = 0; }
 
-// Out-edges of the roots (first sweep's work), into stripe 0's word 2.
-__global__ void k_roots_work(const int64_t* roots, int n, const uint32_t* outdeg,
-                             unsigned long long* flags) {
-  unsigned long long w = 0;
-  for (int s = 0; s < n; s++) w += outdeg[roots[s]];
-  flags[2] = w;
-}
-
 // ---- top-down (push) sweeps for small frontiers ---------------------------
-// Frontier queue: vertices with frontier lanes and out-edges.  Fixed grid;
-// each workgroup owns a contiguous vertex range, counts it, takes one queue
+// A push sweep works from the list of the frontier's vertices (every vertex
+// with frontier lanes; all have out-edges) and lists the vertices it
+// discovers that have out-edges: the next sweep's list.  After a pull or
+// binned sweep the frontier is listed by a scan of every vertex: fixed grid;
+// each workgroup owns a contiguous vertex range, counts it, takes one list
 // allocation and writes its vertices in order.
 template <typename W>
-__global__ __launch_bounds__(kBlock) void k_ms_queue(const W* fr, const uint32_t* outdeg,
-                                                     uint64_t V, uint32_t* q,
+__global__ __launch_bounds__(kBlock) void k_ms_queue(const W* fr, uint64_t V, uint32_t* q,
                                                      unsigned long long* qn) {
   __shared__ unsigned long long base;
   __shared__ uint32_t wcnt[kBlock / 64];
@@ -366,7 +384,7 @@ __global__ __launch_bounds__(kBlock) void k_ms_queue(const W* fr, const uint32_t
   const uint64_t v0 = uint64_t(blockIdx.x) * per, v1 = min(V, v0 + per);
   uint32_t c = 0;
   for (uint64_t v = v0 + threadIdx.x; v < v1; v += kBlock)
-    c += (!(fr[v] == Lane<W>::zero()) && outdeg[v] > 0) ? 1u : 0u;
+    c += !(fr[v] == Lane<W>::zero()) ? 1u : 0u;
   const uint32_t tot = block_sum_u32(c);
   if (threadIdx.x == 0) base = tot ? atomicAdd(qn, (unsigned long long)tot) : 0ull;
   __syncthreads();
@@ -374,7 +392,7 @@ __global__ __launch_bounds__(kBlock) void k_ms_queue(const W* fr, const uint32_t
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   for (uint64_t r0 = v0; r0 < v1; r0 += kBlock) {
     const uint64_t v = r0 + threadIdx.x;
-    const bool in = v < v1 && !(fr[v] == Lane<W>::zero()) && outdeg[v] > 0;
+    const bool in = v < v1 && !(fr[v] == Lane<W>::zero());
     const uint64_t m = __ballot(in);
     if (lane == 0) wcnt[wave] = __popcll(m);
     __syncthreads();
@@ -416,33 +434,61 @@ __global__ void k_push_blocks(const uint64_t* qofs, uint64_t nq, uint64_t m_f, u
   blk[2 * b + 1] = search(lo, x1 - 1);
 }
 
-__device__ inline void lanes_or(uint16_t* p, uint16_t m) {
+// nx[v] |= m (m not zero); returns true for exactly one of the ORs into a
+// word that started at zero.  Up to 64 lanes: the OR that found the word
+// zero.  128 lanes take two atomics, so v's bit in `touched` decides.
+__device__ inline bool lanes_or(uint16_t* p, uint16_t m, uint32_t*, uint32_t) {
   // 16-bit lanes: OR into the containing aligned 32-bit word
   const uintptr_t a = reinterpret_cast<uintptr_t>(p);
-  atomicOr(reinterpret_cast<unsigned int*>(a & ~uintptr_t(3)), uint32_t(m) << ((a & 2) ? 16 : 0));
+  const int sh = (a & 2) ? 16 : 0;
+  const uint32_t old =
+      atomicOr(reinterpret_cast<unsigned int*>(a & ~uintptr_t(3)), uint32_t(m) << sh);
+  return ((old >> sh) & 0xffffu) == 0;
 }
-__device__ inline void lanes_or(uint32_t* p, uint32_t m) { atomicOr(p, m); }
-__device__ inline void lanes_or(uint64_t* p, uint64_t m) {
-  atomicOr(reinterpret_cast<unsigned long long*>(p), (unsigned long long)m);
+__device__ inline bool lanes_or(uint32_t* p, uint32_t m, uint32_t*, uint32_t) {
+  return atomicOr(p, m) == 0;
 }
-__device__ inline void lanes_or(W128* p, W128 m) {
+__device__ inline bool lanes_or(uint64_t* p, uint64_t m, uint32_t*, uint32_t) {
+  return atomicOr(reinterpret_cast<unsigned long long*>(p), (unsigned long long)m) == 0;
+}
+__device__ inline bool lanes_or(W128* p, W128 m, uint32_t* touched, uint32_t v) {
   if (m.a) atomicOr(reinterpret_cast<unsigned long long*>(&p->a), (unsigned long long)m.a);
   if (m.b) atomicOr(reinterpret_cast<unsigned long long*>(&p->b), (unsigned long long)m.b);
+  const uint32_t bit = 1u << (v & 31);
+  return (atomicOr(&touched[v >> 5], bit) & bit) == 0;
 }
+// Lane words whose first touch is read off the OR itself.
+template <typename W> constexpr bool kOrFindsFirst = true;
+template <> constexpr bool kOrFindsFirst<W128> = false;
 
-// Edge-parallel push over the queue's out-edges: nx[v] |= fr[u] & ~vis[v].
+// Edge-parallel push over the queue's out-edges: nx[v] |= fr[u] & ~vis[v]
+// (nx is all zero before the sweep).  The OR that first gives v lanes
+// (lanes_or) lists v in `tl`: the workgroup gathers its vertices in LDS and
+// takes their list positions with one atomic, so every vertex the sweep
+// reaches is listed once.  Every queue entry has out-edges, so a workgroup's
+// kPushEdges edges span at most kPushEdges + 1 entries.
 template <typename W>
 __global__ __launch_bounds__(kBlock) void k_ms_push(const uint32_t* q, const uint64_t* qofs,
                                                     const uint64_t* blk, uint64_t m_f,
                                                     const uint64_t* ofs, const uint32_t* adj,
-                                                    const W* fr, const W* vis, W* nx) {
+                                                    const W* fr, const W* vis, W* nx,
+                                                    uint32_t* touched, uint32_t* tl,
+                                                    unsigned long long* flags) {
   __shared__ uint64_t l_qofs[kPushEdges + 1];
   __shared__ uint64_t l_base[kPushEdges];
   __shared__ W l_fr[kPushEdges];
+  __shared__ uint32_t l_new[kPushEdges];
+  __shared__ uint32_t l_nnew;
+  __shared__ unsigned long long l_pos;
+  if (threadIdx.x == 0) l_nnew = 0;
   const uint64_t x0 = uint64_t(blockIdx.x) * kPushEdges;
   const uint64_t x1 = min(m_f, x0 + kPushEdges);
   const uint64_t lo = blk[2 * blockIdx.x];
-  const uint32_t n = uint32_t(blk[2 * blockIdx.x + 1] - lo + 1);
+  const uint64_t span = blk[2 * blockIdx.x + 1] - lo + 1;
+  // at most kPushEdges entries are staged; more is reported (the edges past
+  // the staged entries are skipped below)
+  if (span > kPushEdges && threadIdx.x == 0) flags[kOverflow] = 1ull;
+  const uint32_t n = uint32_t(min(span, uint64_t(kPushEdges)));
   for (uint32_t i = threadIdx.x; i <= n; i += kBlock) {
     l_qofs[i] = qofs[lo + i];
     if (i < n) {
@@ -452,7 +498,8 @@ __global__ __launch_bounds__(kBlock) void k_ms_push(const uint32_t* q, const uin
     }
   }
   __syncthreads();
-  for (uint64_t x = x0 + threadIdx.x; x < x1; x += kBlock) {
+  // x < l_qofs[n]: the edge belongs to one of this workgroup's queue entries
+  for (uint64_t x = x0 + threadIdx.x; x < x1 && x < l_qofs[n]; x += kBlock) {
     uint32_t a = 0, b = n;
     while (b - a > 1) {
       const uint32_t mid = (a + b) >> 1;
@@ -461,28 +508,62 @@ __global__ __launch_bounds__(kBlock) void k_ms_push(const uint32_t* q, const uin
     }
     const uint32_t v = adj[l_base[a] + (x - l_qofs[a])];
     const W m = l_fr[a] & ~vis[v];
-    if (!(m == Lane<W>::zero())) lanes_or(&nx[v], m);
+    if (!(m == Lane<W>::zero()) && lanes_or(&nx[v], m, touched, v))
+      l_new[atomicAdd(&l_nnew, 1u)] = v;  // at most one per edge: kPushEdges
   }
+  __syncthreads();
+  const uint32_t nn = l_nnew;
+  if (threadIdx.x == 0 && nn) l_pos = atomicAdd(&flags[kTouchCount], (unsigned long long)nn);
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < nn; i += kBlock) tl[l_pos + i] = l_new[i];
 }
 
-// After a push: every vertex settles its pushed lanes like a pull row would
-// (rows reached by no push are not read: the open in-degree sum is not kept).
+// After a push: each reached vertex (list `tl`, count in stripe 0's word
+// kTouchCount) settles its pushed lanes like a pull row would (and clears its
+// `touched` bit where lanes_or used it); those with out-edges are listed in
+// `lst` (count in word kListCount; one atomic per workgroup round), the
+// others' frontier words cleared.  The pushed lanes were unvisited, so nx[v]
+// is v's new lanes as it stands.  Fixed grid.  The open in-degree sum is not
+// kept.
 template <typename W>
-__global__ __launch_bounds__(kBlock) void k_ms_settle(MsArgs<W> a) {
-  const uint64_t r = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
+__global__ __launch_bounds__(kBlock) void k_ms_settle(MsArgs<W> a, const uint32_t* tl,
+                                                      uint32_t* touched, uint32_t* lst) {
+  __shared__ uint32_t l_keep[kBlock];
+  __shared__ uint32_t l_nkeep;
+  __shared__ unsigned long long l_pos;
+  const uint64_t n = a.found[kTouchCount];
   W cnt = Lane<W>::zero();
   unsigned long long work = 0;
-  if (r < a.V) {
-    const W acc = a.nx[r];
-    if (!(acc == Lane<W>::zero())) {
-      const W vr = a.vis[r];
-      const W nw = acc & ~vr;
-      a.nx[r] = nw;
-      if (!(nw == Lane<W>::zero())) discover(a, uint32_t(r), vr, nw, work);
-      cnt = nw;
+  const uint64_t stride = uint64_t(gridDim.x) * kBlock;
+  // rounds of kBlock vertices, the same for every thread of the workgroup
+  for (uint64_t i0 = uint64_t(blockIdx.x) * kBlock; i0 < n; i0 += stride) {
+    if (threadIdx.x == 0) l_nkeep = 0;
+    __syncthreads();
+    const uint64_t i = i0 + threadIdx.x;
+    if (i < n) {
+      const uint32_t v = tl[i];
+      const W vr = a.vis[v];
+      const W nw = a.nx[v];
+      if (discover(a, v, vr, nw, work) > 0) l_keep[atomicAdd(&l_nkeep, 1u)] = v;
+      else a.nx[v] = Lane<W>::zero();
+      cnt = cnt | nw;
+      if (!kOrFindsFirst<W>) atomicAnd(&touched[v >> 5], ~(1u << (v & 31)));
     }
+    __syncthreads();
+    const uint32_t nk = l_nkeep;
+    if (threadIdx.x == 0 && nk) l_pos = atomicAdd(&a.found[kListCount], (unsigned long long)nk);
+    __syncthreads();
+    if (threadIdx.x < nk) lst[l_pos + threadIdx.x] = l_keep[threadIdx.x];
+    __syncthreads();  // l_keep and l_nkeep are reused by the next round
   }
   block_report(a.found, cnt, work, 0ull);
+}
+
+// Clears the lane words of the listed vertices.
+template <typename W>
+__global__ void k_clear_listed(W* p, const uint32_t* lst, uint64_t n) {
+  const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) p[lst[i]] = Lane<W>::zero();
 }
 
 // ---- binned sweeps for levels with many open rows ---------------------------
@@ -566,12 +647,13 @@ __global__ __launch_bounds__(kBinBlock) void k_msb_gather(MsArgs<W> a, const W* 
       const uint32_t r = uint32_t(v0 + i);
       const W vr = a.vis[r];
       const W nw = W(acc[i]) & ~vr;
-      a.nx[r] = nw;
       if (!((vr | nw | a.done) == a.full)) open += indeg[r];
+      W keep = Lane<W>::zero();
       if (!(nw == Lane<W>::zero())) {
-        discover(a, r, vr, nw, work);
+        if (discover(a, r, vr, nw, work) > 0) keep = nw;
         cnt = cnt | nw;
       }
+      a.nx[r] = keep;
     }
   }
   block_report<W, kBinBlock>(a.found, cnt, work, open);
@@ -603,33 +685,47 @@ __global__ void k_zero_lanes(W* p, uint64_t n) {
   if (i < n) p[i] = Lane<W>::zero();
 }
 
-// Per-lane visited counts: lane s of each wave accumulates popcount(ballot(bit s)),
-// the workgroup sums its waves in LDS and adds once per lane.
-// Grid-stride over a fixed grid, so the per-lane totals take few atomics.
+// Per-lane visited counts: for each bit s of each 64-bit half of the lane
+// word, popcount(ballot(bit s)) over a wave's 64 vertices, kept by thread s
+// (the shifts are constants: the loop over s is unrolled, no branches).  A
+// workgroup sums its waves in LDS and adds once per lane.  Grid-stride over a
+// fixed grid of kReachBlock-thread workgroups.
+constexpr int kReachBlock = 1024;
+
 template <typename W>
-__global__ void k_ms_reached(const W* vis, uint64_t V, int n_src, unsigned long long* out) {
+__global__ __launch_bounds__(kReachBlock) void k_ms_reached(const W* vis, uint64_t V, int n_src,
+                                                            unsigned long long* out) {
+  constexpr int kHalves = Lane<W>::kBits > 64 ? 2 : 1;
+  constexpr int kBitsPerHalf = Lane<W>::kBits > 64 ? 64 : Lane<W>::kBits;
   __shared__ unsigned long long acc[128];
   const int lane = threadIdx.x & 63;
-  for (int i = threadIdx.x; i < 128; i += blockDim.x) acc[i] = 0;
+  for (int i = threadIdx.x; i < 128; i += kReachBlock) acc[i] = 0;
   __syncthreads();
-  unsigned long long c0 = 0, c1 = 0;
-  const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
-  const uint64_t nround = (V + stride - 1) / stride;  // uniform across the grid
-  for (uint64_t k = 0; k < nround; k++) {
-    const uint64_t v = k * stride + uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  uint32_t c[kHalves] = {};
+  const uint64_t stride = uint64_t(gridDim.x) * kReachBlock;
+  for (uint64_t v = uint64_t(blockIdx.x) * kReachBlock + threadIdx.x; v - threadIdx.x < V;
+       v += stride) {
+    // every lane of a wave runs the same rounds (v - threadIdx.x is uniform)
     const W w = v < V ? vis[v] : Lane<W>::zero();
-    for (int s = 0; s < n_src; s++) {
-      const uint64_t b = __ballot(v < V && Lane<W>::test(w, s));
-      if ((s & 63) == lane) {
-        if (s < 64) c0 += __popcll(b);
-        else c1 += __popcll(b);
+    uint64_t h[kHalves];
+    h[0] = Lane<W>::lo64(w);
+    if (kHalves > 1) h[kHalves - 1] = Lane<W>::hi64(w);
+#pragma unroll
+    for (int b = 0; b < kBitsPerHalf; b++) {
+#pragma unroll
+      for (int j = 0; j < kHalves; j++) {
+        const uint32_t k = uint32_t(__popcll(__ballot((h[j] >> b) & 1)));
+        c[j] += lane == b ? k : 0u;
       }
     }
   }
-  if (lane < n_src && c0) atomicAdd(&acc[lane], c0);
-  if (lane + 64 < n_src && c1) atomicAdd(&acc[lane + 64], c1);
+#pragma unroll
+  for (int j = 0; j < kHalves; j++) {
+    const int sl = j * 64 + lane;
+    if (sl < n_src && lane < kBitsPerHalf && c[j]) atomicAdd(&acc[sl], (unsigned long long)c[j]);
+  }
   __syncthreads();
-  for (int i = threadIdx.x; i < n_src; i += blockDim.x)
+  for (int i = threadIdx.x; i < n_src; i += kReachBlock)
     if (acc[i]) atomicAdd(&out[i], acc[i]);
 }
 
@@ -650,12 +746,22 @@ void msbfs_batch(ll_gpu_graph* g, int level, const int64_t* d_roots, int n, int 
   nx.alloc(V * sizeof(W) + 16, s);
   part.alloc(std::max<uint64_t>(rb.n, 1) * sizeof(W), s);
   found.alloc(kFlagStripes * kFlagStride * 8, s);
+  // vertex lists of the frontiers (two, alternating), the vertices a push
+  // reaches and its touched bits (all clear between sweeps)
+  DevBuf lst[2], tl, touched;
+  lst[0].alloc(V * 4, s);
+  lst[1].alloc(V * 4, s);
+  tl.alloc(V * 4, s);
+  const uint64_t tw = (V + 31) / 32;
+  touched.alloc(tw * 4, s);
+  k_fill_i32<<<grid_for(tw, 256), 256, 0, s>>>(touched.as<int32_t>(), tw, 0);
   const W full = Lane<W>::low(Lane<W>::kBits);
   const W used = Lane<W>::low(n);
-  k_ms_init<W><<<grid_for(V, 256), 256, 0, s>>>(vis.as<W>(), fr.as<W>(), V, full & ~used);
+  k_ms_init<W><<<grid_for(V, 256), 256, 0, s>>>(vis.as<W>(), fr.as<W>(), nx.as<W>(), V,
+                                                full & ~used);
   k_zero_flags<<<1, kFlagStripes, 0, s>>>(found.as<unsigned long long>());
-  k_ms_roots<W><<<1, 1, 0, s>>>(vis.as<W>(), fr.as<W>(), d_roots + src_base, n, d_lv, V, src_base);
-  k_roots_work<<<1, 1, 0, s>>>(d_roots + src_base, n, outdeg, found.as<unsigned long long>());
+  k_ms_roots<W><<<1, 1, 0, s>>>(vis.as<W>(), fr.as<W>(), d_roots + src_base, n, d_lv, V, src_base,
+                                outdeg, lst[0].as<uint32_t>(), found.as<unsigned long long>());
   LLG_HIP(hipGetLastError());
   static_assert(kFlagStripes * kFlagStride * 8 <= kReadbackBytes, "flags fit the read-back buffer");
   const uint64_t* fl = static_cast<const uint64_t*>(ctx->readback);
@@ -673,14 +779,23 @@ void msbfs_batch(ll_gpu_graph* g, int level, const int64_t* d_roots, int n, int 
   };
   uint64_t lo, hi, m_f, open;
   read_report(lo, hi, m_f, open);  // m_f: out-edges of the frontier (the roots)
+  // Which vertices hold lanes in each lane-word buffer: the list buffer
+  // (0/1) naming them, kUnlisted (a pull or binned sweep wrote every vertex),
+  // or kAllZero.  A push sweep needs its output buffer all zero and its
+  // frontier listed.
+  constexpr int kUnlisted = -1, kAllZero = -2;
+  int fr_list = 0, nx_list = kAllZero;
+  uint64_t fr_n = fl[kListCount], nx_n = 0;
   // Push while the frontier's out-edges are under 1/kPushRatio of the edges
   // (the first sweeps from a handful of roots).  Otherwise a binned sweep
   // while the rows some searching lane has not reached hold at least
   // kBinOpen of the in-edges (a pull skips the other rows; the binned sweep
   // streams every edge at about half a full pull's cost), else a pull.
   constexpr uint64_t kPushRatio = 16;
-  constexpr double kBinOpen = 0.4;
   const long bins_opt = ctx->options.count("bfs_batch_bins") ? ctx->options["bfs_batch_bins"] : -1;
+  const long open_opt =
+      ctx->options.count("bfs_batch_bin_open") ? ctx->options["bfs_batch_bin_open"] : -1;
+  const double kBinOpen = open_opt >= 0 ? double(open_opt) / 100.0 : 0.4;
   const PrBins* pb = nullptr;
   const uint32_t* indeg = nullptr;
   if (bins_opt != 0) {
@@ -695,7 +810,7 @@ void msbfs_batch(ll_gpu_graph* g, int level, const int64_t* d_roots, int n, int 
   uint64_t open_in = in.E;  // in-edges of the rows a searching lane has not reached
   DevBuf sv;                // binned sweeps: the slots' lane words
   const Csr* out = nullptr;
-  DevBuf q, qdeg, qofs, qn, blk;
+  DevBuf qdeg, qofs, qn, blk;
   uint64_t lv_count = 0;
   W done = full & ~used;  // lanes not searching (unused); later also finished ones
   for (int32_t l = 1;; l++) {
@@ -720,41 +835,50 @@ void msbfs_batch(ll_gpu_graph* g, int level, const int64_t* d_roots, int n, int 
     const bool push = m_f * kPushRatio < in.E;
     const bool binned =
         !push && pb && (bins_opt == 1 || double(open_in) >= kBinOpen * double(in.E));
+    int out_list = kUnlisted;
     if (push) {
       if (!out) {
         out = &ensure_csr(g, LL_GPU_DIR_OUT, level);
-        q.alloc(V * 4, s);
         qdeg.alloc(V * 4, s);
         qofs.alloc((V + 1) * 8, s);
         qn.alloc(8, s);
       }
-      k_zero_ull<<<1, 1, 0, s>>>(qn.as<unsigned long long>());
-      k_ms_queue<W><<<std::min(grid_for(V, kBlock), 4u * unsigned(ctx->num_cus)), kBlock, 0, s>>>(
-          fr.as<W>(), outdeg, V, q.as<uint32_t>(), qn.as<unsigned long long>());
-      LLG_HIP(hipMemcpyAsync(ctx->readback, qn.p, 8, hipMemcpyDeviceToHost, s));
-      LLG_HIP(hipStreamSynchronize(s));
-      const uint64_t nq = *static_cast<const uint64_t*>(ctx->readback);
-      k_zero_lanes<W><<<grid_for(V, 256), 256, 0, s>>>(nx.as<W>(), V);
-      if (nq) {
-        k_queue_deg<<<grid_for(nq, 256), 256, 0, s>>>(q.as<uint32_t>(), nq, outdeg,
-                                                      qdeg.as<uint32_t>());
-        scan_u32_to_u64(qdeg.as<uint32_t>(), qofs.as<uint64_t>(), nq, s);
-        LLG_HIP(hipMemcpyAsync(ctx->readback, qofs.as<uint64_t>() + nq, 8, hipMemcpyDeviceToHost, s));
-        LLG_HIP(hipStreamSynchronize(s));
-        const uint64_t me = *static_cast<const uint64_t*>(ctx->readback);
-        const uint64_t nblk = (me + kPushEdges - 1) / kPushEdges;
-        if (nblk) {
-          blk.alloc(nblk * 16, s);
-          k_push_blocks<<<grid_for(nblk, 256), 256, 0, s>>>(qofs.as<uint64_t>(), nq, me, nblk,
-                                                            blk.as<uint64_t>());
-          TimedLaunch t(ctx, "k_ms_push");
-          k_ms_push<W><<<unsigned(nblk), kBlock, 0, s>>>(
-              q.as<uint32_t>(), qofs.as<uint64_t>(), blk.as<uint64_t>(), me,
-              out->ofs.as<uint64_t>(), out->adj.as<uint32_t>(), fr.as<W>(), vis.as<W>(),
-              nx.as<W>());
-        }
+      // the output words start at zero: clear the listed vertices, or all
+      if (nx_list == kUnlisted || (nx_list >= 0 && nx_n > V / 16)) {
+        k_zero_lanes<W><<<grid_for(V, 256), 256, 0, s>>>(nx.as<W>(), V);
+      } else if (nx_list >= 0 && nx_n > 0) {
+        k_clear_listed<W><<<grid_for(nx_n, 256), 256, 0, s>>>(nx.as<W>(),
+                                                              lst[nx_list].as<uint32_t>(), nx_n);
       }
-      k_ms_settle<W><<<grid_for(V, kBlock), kBlock, 0, s>>>(a);
+      if (fr_list == kUnlisted) {
+        // the list buffers are free: the output's list (if any) was read above
+        k_zero_ull<<<1, 1, 0, s>>>(qn.as<unsigned long long>());
+        k_ms_queue<W><<<std::min(grid_for(V, kBlock), 4u * unsigned(ctx->num_cus)), kBlock, 0,
+                        s>>>(fr.as<W>(), V, lst[0].as<uint32_t>(), qn.as<unsigned long long>());
+        LLG_HIP(hipMemcpyAsync(ctx->readback, qn.p, 8, hipMemcpyDeviceToHost, s));
+        LLG_HIP(hipStreamSynchronize(s));
+        fr_list = 0;
+        fr_n = *static_cast<const uint64_t*>(ctx->readback);
+      }
+      out_list = 1 - fr_list;
+      const uint32_t* q = lst[fr_list].as<uint32_t>();
+      // m_f = the out-degree sum over the listed frontier (each sweep adds the
+      // out-degree of every vertex it discovers, once)
+      const uint64_t nblk = (m_f + kPushEdges - 1) / kPushEdges;
+      if (fr_n > 0 && nblk > 0) {
+        k_queue_deg<<<grid_for(fr_n, 256), 256, 0, s>>>(q, fr_n, outdeg, qdeg.as<uint32_t>());
+        scan_u32_to_u64(qdeg.as<uint32_t>(), qofs.as<uint64_t>(), fr_n, s);
+        blk.alloc(nblk * 16, s);
+        k_push_blocks<<<grid_for(nblk, 256), 256, 0, s>>>(qofs.as<uint64_t>(), fr_n, m_f, nblk,
+                                                          blk.as<uint64_t>());
+        TimedLaunch t(ctx, "k_ms_push");
+        k_ms_push<W><<<unsigned(nblk), kBlock, 0, s>>>(
+            q, qofs.as<uint64_t>(), blk.as<uint64_t>(), m_f, out->ofs.as<uint64_t>(),
+            out->adj.as<uint32_t>(), fr.as<W>(), vis.as<W>(), nx.as<W>(), touched.as<uint32_t>(),
+            tl.as<uint32_t>(), a.found);
+      }
+      k_ms_settle<W><<<std::min(grid_for(V, kBlock), 4u * unsigned(ctx->num_cus)), kBlock, 0, s>>>(
+          a, tl.as<uint32_t>(), touched.as<uint32_t>(), lst[out_list].as<uint32_t>());
     } else if (binned) {
       if (!sv.p) sv.alloc(std::max<uint64_t>(in.E, 1) * sizeof(W), s);
       TimedLaunch t(ctx, "k_msb_sweep");
@@ -773,15 +897,22 @@ void msbfs_batch(ll_gpu_graph* g, int level, const int64_t* d_roots, int n, int 
     LLG_HIP(hipGetLastError());
     // lanes that discovered vertices = the next frontier's lanes
     read_report(lo, hi, m_f, open);
+    LLG_CHECK(fl[kOverflow] == 0, LL_GPU_ESTATE,
+              "multi-source BFS: a push queue entry has no out-edges");
     if (!push) open_in = open;
     lv_count++;
     if ((lo | hi) == 0) break;
     done = full & ~Lane<W>::from64(lo, hi);
     std::swap(fr, nx);
+    // the input words keep their list (no sweep writes them)
+    nx_list = fr_list;
+    nx_n = fr_n;
+    fr_list = out_list;
+    fr_n = push ? fl[kListCount] : 0;
   }
   if (d_reached)
-    k_ms_reached<W><<<std::min(grid_for(V, 256), 4u * unsigned(ctx->num_cus)), 256, 0, s>>>(
-        vis.as<W>(), V, n, d_reached + src_base);
+    k_ms_reached<W><<<std::min(grid_for(V, kReachBlock), unsigned(ctx->num_cus)), kReachBlock, 0,
+                      s>>>(vis.as<W>(), V, n, d_reached + src_base);
   LLG_HIP(hipGetLastError());
   if (levels_run) *levels_run += lv_count;
 }

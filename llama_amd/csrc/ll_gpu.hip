@@ -236,7 +236,7 @@ int ll_gpu_set_option(ll_gpu_ctx* c, const char* name, long value) {
   static const char* known[] = {"pr_reorder", "pr_cap", "pr_seg_bits", "pr_item_sort",
                                 "pr_hot_bits", "pr_bin_bits", "sssp_delta_milli",
                                 "sssp_min_queue", "bfs_alpha", "bfs_beta", "bfs_reach",
-                                "bfs_batch_bins"};
+                                "bfs_batch_bins", "bfs_batch_bin_open"};
   try {
     for (const char* k : known)
       if (std::strcmp(k, name) == 0) {
