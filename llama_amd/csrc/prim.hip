@@ -189,8 +189,7 @@ __global__ __launch_bounds__(kB) void k_rs_scatter(const K* kin, const uint32_t*
   __shared__ uint32_t sval[kVals ? kTile : 1];
   __shared__ uint64_t gbase[256];  // first output position of the tile's run per digit
   __shared__ uint32_t lbase[256];  // first LDS position of the digit in the tile
-  __shared__ uint32_t run[256];
-  __shared__ uint32_t wcnt[kB / 64][256];
+  __shared__ uint32_t wcnt[kB / 64][256];  // per wave: digit counts, then their offsets
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const uint32_t dm = (1u << bits) - 1;
   // neighbouring tiles write neighbouring pieces of every digit's run: keep
@@ -201,51 +200,64 @@ __global__ __launch_bounds__(kB) void k_rs_scatter(const K* kin, const uint32_t*
   uint64_t tot;
   lbase[t] = uint32_t(block_exclusive_scan(cnt, &tot));
   gbase[t] = mine ? tofs[uint64_t(t) * ntiles + tile] : 0;
-  run[t] = 0;
   for (int w = 0; w < kB / 64; w++) wcnt[w][t] = 0;
   __syncthreads();
   const uint64_t base = tile * kTile;
   const uint32_t nt = uint32_t(min(kTile, n - base));
-  // every key (and value) of the thread is requested before any is ranked
+  // wave w owns the tile's keys [w * 64 * kRounds, (w + 1) * 64 * kRounds) and
+  // ranks them in order with its own running digit counts (no barrier); the
+  // waves' counts then give each wave's offset per digit, which keeps the
+  // tile's order within a digit.  Every key (and value) of the thread is
+  // requested before any is ranked.
+  const uint32_t w0 = uint32_t(wave) * 64u * kRounds;
   K kr[kRounds];
   uint32_t vr[kVals ? kRounds : 1];
+  uint32_t lp[kRounds];  // rank among the wave's keys of the same digit
 #pragma unroll
   for (int r = 0; r < kRounds; r++) {
-    const uint32_t j = uint32_t(r) * kB + uint32_t(t);
+    const uint32_t j = w0 + uint32_t(r) * 64u + uint32_t(lane);
     kr[r] = j < nt ? kin[base + j] : K(0);
     if (kVals) vr[r] = j < nt ? vin[base + j] : 0u;
   }
+  const uint64_t lt = (uint64_t(1) << lane) - 1;
 #pragma unroll
   for (int r = 0; r < kRounds; r++) {
-    if (uint32_t(r) * kB >= nt) break;  // uniform over the workgroup
-    const uint32_t j = uint32_t(r) * kB + uint32_t(t);
+    const uint32_t j = w0 + uint32_t(r) * 64u + uint32_t(lane);
     const bool valid = j < nt;
-    const K key = kr[r];
-    const uint32_t val = kVals ? vr[r] : 0u;
-    const uint32_t d = uint32_t(key >> shift) & dm;
+    const uint32_t d = uint32_t(kr[r] >> shift) & dm;
     uint64_t peers = __ballot(valid);
     for (int b = 0; b < bits; b++) {
       const uint64_t m = __ballot((d >> b) & 1u);
       peers &= ((d >> b) & 1u) ? m : ~m;
     }
-    const uint64_t below = peers & ((uint64_t(1) << lane) - 1);
-    if (valid && below == 0) wcnt[wave][d] = uint32_t(__popcll(peers));
-    __syncthreads();
-    if (valid) {
-      uint32_t pos = lbase[d] + run[d] + uint32_t(__popcll(below));
-      for (int w = 0; w < wave; w++) pos += wcnt[w][d];
-      skey[pos] = key;
-      if (kVals) sval[pos] = val;
-    }
-    __syncthreads();
-    uint32_t add = 0;
-    for (int w = 0; w < kB / 64; w++) {
-      add += wcnt[w][t];
-      wcnt[w][t] = 0;
-    }
-    run[t] += add;
-    __syncthreads();
+    // a wave's LDS accesses take effect in program order: every lane reads
+    // the count before the digit's first lane writes it back
+    const uint32_t c = valid ? wcnt[wave][d] : 0u;
+    const uint64_t below = peers & lt;
+    lp[r] = c + uint32_t(__popcll(below));
+    if (valid && below == 0) wcnt[wave][d] = c + uint32_t(__popcll(peers));
   }
+  __syncthreads();
+  if (mine) {
+    uint32_t acc = 0;
+    for (int w = 0; w < kB / 64; w++) {
+      const uint32_t c = wcnt[w][t];
+      wcnt[w][t] = lbase[t] + acc;
+      acc += c;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < kRounds; r++) {
+    const uint32_t j = w0 + uint32_t(r) * 64u + uint32_t(lane);
+    if (j < nt) {
+      const uint32_t d = uint32_t(kr[r] >> shift) & dm;
+      const uint32_t pos = wcnt[wave][d] + lp[r];
+      skey[pos] = kr[r];
+      if (kVals) sval[pos] = vr[r];
+    }
+  }
+  __syncthreads();
   for (uint32_t j = uint32_t(t); j < nt; j += kB) {
     const K key = skey[j];
     const uint32_t d = uint32_t(key >> shift) & dm;
