@@ -67,10 +67,12 @@ __global__ void k_pb_bounds(const uint64_t* ofs, uint64_t V, uint32_t bb, uint32
 // Per out-edge of an item: the destination bin as the sort key and (source
 // offset << 16 | destination offset) as the value.  The item is walked in
 // tiles of kKeyTile edges.  A tile's source rows: r0, the row holding its
-// first edge (one binary search over the bin's row offsets, staged in LDS),
-// and every later row starting inside the tile marks its first edge in LDS; a
+// first edge (a binary search over the bin's row offsets, staged in LDS, for
+// the item's first tile; then the row of the previous tile's last edge), and
+// every later row starting inside the tile marks its first edge in LDS; a
 // running maximum over the tile then gives each edge its row.  Reads and
-// writes follow the edge order (coalesced).  The same pass counts the item's
+// writes follow the edge order (coalesced); a tile's destinations are loaded
+// while the tile before it is processed.  The same pass counts the item's
 // out-edges per destination bin (LDS), into th[b' * ni + i].
 constexpr int kKeyPer = 8;
 constexpr uint32_t kKeyTile = kBuildBlock * kKeyPer;
@@ -94,16 +96,34 @@ __global__ __launch_bounds__(kBuildBlock) void k_pb_keys(const uint64_t* ofs, co
   for (uint32_t i = threadIdx.x; i < nb; i += blockDim.x) h[i] = 0;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   __syncthreads();
-  for (uint64_t t0 = it.e0; t0 < it.e1; t0 += kKeyTile) {
-    const uint64_t t1 = min(t0 + kKeyTile, uint64_t(it.e1));
-    // r0: the last row starting at or before t0 (ro[0] <= e0 <= t0 < ro[nrows])
+  // r0: the last row starting at or before the tile's first edge; found by a
+  // binary search for the item's first tile (ro[0] <= e0 < ro[nrows]), then
+  // carried: the next tile's r0 is the row of this tile's last edge
+  uint32_t r0;
+  {
     uint32_t lo = 0, hi = nrows;
     while (hi - lo > 1) {
       const uint32_t mid = (lo + hi) >> 1;
-      if (ro[mid] <= t0) lo = mid;
+      if (ro[mid] <= it.e0) lo = mid;
       else hi = mid;
     }
-    const uint32_t r0 = lo;
+    r0 = lo;
+  }
+  // each tile's destinations are requested while the tile before is ranked
+  uint32_t cur[kKeyPer];
+#pragma unroll
+  for (int j = 0; j < kKeyPer; j++) {
+    const uint64_t e = it.e0 + uint64_t(j) * kBuildBlock + threadIdx.x;
+    cur[j] = e < it.e1 ? adj[e] : 0u;
+  }
+  for (uint64_t t0 = it.e0; t0 < it.e1; t0 += kKeyTile) {
+    const uint64_t t1 = min(t0 + kKeyTile, uint64_t(it.e1));
+    uint32_t nxt[kKeyPer];
+#pragma unroll
+    for (int j = 0; j < kKeyPer; j++) {
+      const uint64_t e = t1 + uint64_t(j) * kBuildBlock + threadIdx.x;
+      nxt[j] = e < it.e1 ? adj[e] : 0u;
+    }
 #pragma unroll
     for (int j = 0; j < kKeyPer; j++) mark[threadIdx.x + j * kBuildBlock] = 0;
     __syncthreads();
@@ -129,7 +149,11 @@ __global__ __launch_bounds__(kBuildBlock) void k_pb_keys(const uint64_t* ofs, co
     __syncthreads();
     uint32_t before = __shfl_up(inc, 1, 64);
     if (lane == 0) before = 0;
-    for (int w = 0; w < wave; w++) before = max(before, wmax[w]);
+    uint32_t last = r0;  // the row of the tile's last edge
+    for (int w = 0; w < kBuildBlock / 64; w++) {
+      if (w < wave) before = max(before, wmax[w]);
+      last = max(last, wmax[w]);
+    }
 #pragma unroll
     for (int j = 0; j < kKeyPer; j++) mark[threadIdx.x * kKeyPer + j] = max(before, m[j]);
     __syncthreads();
@@ -137,13 +161,16 @@ __global__ __launch_bounds__(kBuildBlock) void k_pb_keys(const uint64_t* ofs, co
     for (int j = 0; j < kKeyPer; j++) {
       const uint64_t e = t0 + uint64_t(j) * kBuildBlock + threadIdx.x;
       if (e < t1) {
-        const uint32_t v = adj[e];
+        const uint32_t v = cur[j];
         const uint32_t row = max(r0, mark[uint32_t(e - t0)]);
         key[e] = v >> BB;
         val[e] = row << 16 | (v & (R - 1));
         atomicAdd(&h[v >> BB], 1u);
       }
     }
+    r0 = last;
+#pragma unroll
+    for (int j = 0; j < kKeyPer; j++) cur[j] = nxt[j];
     __syncthreads();  // mark and wmax are rewritten by the next tile
   }
   for (uint32_t i = threadIdx.x; i < nb; i += blockDim.x) th[uint64_t(i) * ni + item] = h[i];
