@@ -250,34 +250,39 @@ __device__ inline void compact_block(const uint64_t* const* et, const uint64_t* 
         if (kWeights) wv[j] = wt[lvl][idx];
       }
     }
-    // rounds of 256 consecutive words keep the output in virtual order
-    // (predicated rather than left early, so that the loop unrolls and wd[]
-    // stays in registers)
+    // the batch's words in rounds of 256 consecutive words: every wave counts
+    // its live words per round, then one barrier, and each word's output
+    // position is the live words before it in virtual order (rounds before,
+    // waves before in its round, lanes before in its wave)
+    bool alive[kBatch];
+    uint32_t rank[kBatch];
 #pragma unroll
     for (int j = 0; j < kBatch; j++) {
-      const uint64_t base = xb + uint64_t(j) * kBlock;
-      if (base >= x1) continue;  // uniform over the workgroup
-      const uint64_t x = base + threadIdx.x;
-      const bool valid = x < x1;
-      const uint64_t w = wd[j];
-      const bool alive = valid && (!kDeletions || word_maxlev(w) > level);
-      const uint64_t mask = __ballot(alive);
-      const uint32_t rank = __popcll(mask & ((uint64_t(1) << lane) - 1));
-      if (lane == 0) wcnt[wave] = __popcll(mask);
-      __syncthreads();
-      uint32_t before = 0, round = 0;
+      const uint64_t x = xb + uint64_t(j) * kBlock + threadIdx.x;
+      alive[j] = x < x1 && (!kDeletions || word_maxlev(wd[j]) > level);
+      const uint64_t mask = __ballot(alive[j]);
+      rank[j] = __popcll(mask & ((uint64_t(1) << lane) - 1));
+      if (lane == 0) wcnt[j * kWaves + wave] = __popcll(mask);
+    }
+    __syncthreads();
+    uint64_t before = out;
+#pragma unroll
+    for (int j = 0; j < kBatch; j++) {
+      uint32_t mine = 0, round = 0;
       for (int i = 0; i < kWaves; i++) {
-        if (i < wave) before += wcnt[i];
-        round += wcnt[i];
+        const uint32_t c = wcnt[j * kWaves + i];
+        if (i < wave) mine += c;
+        round += c;
       }
-      const uint64_t pos = out + before + rank;
-      if (alive && pos < out1) {
-        adj[pos] = uint32_t(w & kIndexMask);
+      const uint64_t pos = before + mine + rank[j];
+      if (alive[j] && pos < out1) {
+        adj[pos] = uint32_t(wd[j] & kIndexMask);
         if (kWeights) wout[pos] = wv[j];
       }
-      out += round;
-      __syncthreads();
+      before += round;
     }
+    out = before;
+    __syncthreads();  // wcnt is rewritten by the next batch
   }
   if (threadIdx.x == 0 && out != pl.out1) atomicAdd(bad, 1u);
 }
@@ -292,7 +297,7 @@ __global__ __launch_bounds__(kBlock) void k_compact(const uint64_t* const* et,
                                                     uint32_t* bad) {
   __shared__ uint64_t l_vofs[kWordsPerBlock + 1];
   __shared__ uint64_t l_addr[kWordsPerBlock];
-  __shared__ uint32_t wcnt[kWaves];
+  __shared__ uint32_t wcnt[kBatch * kWaves];
   const BlockPlan pl = plan[blockIdx.x];
   if (pl.x0 == pl.x1) return;  // uniform: a window inside a long vertex
   if (pl.n <= kWordsPerBlock) {  // uniform
