@@ -32,6 +32,11 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr int kWordsPerBlock = 2048;
+// Segment records a compaction block stages in LDS: a block overlaps about
+// 200 segments at R-MAT s24, so room for 512 keeps the LDS per workgroup at
+// 8 KiB (eight workgroups per CU); a block overlapping more searches them in
+// global memory.
+constexpr int kSegLds = 512;
 constexpr int kWaves = kBlock / 64;
 constexpr int kBatch = kWordsPerBlock / kBlock + 1;  // rounds per compaction batch
 
@@ -206,7 +211,7 @@ __device__ inline uint32_t seg_search_lds(const uint64_t* l_vofs, uint32_t n, ui
 // words in batches of kBatch rounds of 256 (each thread requests its words of
 // a batch before any is tested), live words written in virtual order from the
 // first vertex's row offset.  The block's segments are looked up in LDS (in global
-// memory when more than kWordsPerBlock overlap it).  Writes stay inside the
+// memory when more than kSegLds overlap it).  Writes stay inside the
 // block's rows whatever the mirror holds; a block whose live words differ in
 // number from its rows' degrees counts itself in *bad.
 // The words of one block, live ones written in order.  kLds: the block's
@@ -295,12 +300,12 @@ __global__ __launch_bounds__(kBlock) void k_compact(const uint64_t* const* et,
                                                     uint32_t level, uint32_t* adj,
                                                     const float* const* wt, float* wout,
                                                     uint32_t* bad) {
-  __shared__ uint64_t l_vofs[kWordsPerBlock + 1];
-  __shared__ uint64_t l_addr[kWordsPerBlock];
+  __shared__ uint64_t l_vofs[kSegLds + 1];
+  __shared__ uint64_t l_addr[kSegLds];
   __shared__ uint32_t wcnt[kBatch * kWaves];
   const BlockPlan pl = plan[blockIdx.x];
   if (pl.x0 == pl.x1) return;  // uniform: a window inside a long vertex
-  if (pl.n <= kWordsPerBlock) {  // uniform
+  if (pl.n <= kSegLds) {  // uniform
     for (uint32_t i = threadIdx.x; i <= uint32_t(pl.n); i += blockDim.x) {
       l_vofs[i] = seg_vofs[pl.s_lo + i];
       if (i < pl.n) l_addr[i] = seg_addr[pl.s_lo + i];
