@@ -46,7 +46,7 @@ namespace {
 constexpr int kIterBlock = 1024;
 constexpr int kUnrollA = 8;  // phase A: slots per thread in flight
 constexpr int kGroupC = 8;   // phase C: chunks per wave in flight
-constexpr int kBuildBlock = 256;
+constexpr int kBuildBlock = 512;
 
 // Build items: pieces of one source bin's out-edges, a bin holding more than
 // max(2^18, E / 2048) cut into equal pieces, so that no workgroup of the build
