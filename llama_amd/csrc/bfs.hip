@@ -238,29 +238,48 @@ __global__ void k_queue_to_bitmap(const uint32_t* q, uint64_t n, uint64_t* bm) {
 constexpr uint32_t kLongRow = 32;
 
 // Lists of the open rows, short and long, from the visited bitmap: a thread
-// per bitmap word, one list reservation (atomic) per workgroup and list.  A
+// per kOpenWords bitmap words, one list reservation (atomic) per workgroup and
+// list, rows in ascending order.  `merge` (may be null): a bitmap of rows
+// found by the level before, OR-ed into visited first (and written back).  A
 // row is long by the snapshot's long-row bitmap when one is given, else by
 // its offsets.
-__global__ __launch_bounds__(kBlock) void k_open_rows(const uint64_t* visited, uint64_t words,
-                                                      const uint64_t* ofs,
+constexpr int kOpenWords = 4;
+
+__global__ __launch_bounds__(kBlock) void k_open_rows(uint64_t* visited, const uint64_t* merge,
+                                                      uint64_t words, const uint64_t* ofs,
                                                       const uint64_t* longrows,
                                                       uint32_t* rows_short, uint32_t* rows_long,
                                                       Counters* c) {
   __shared__ uint32_t wsum[2][kBlock / 64];
   __shared__ unsigned long long base[2];
-  const uint64_t w = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
-  const uint64_t open = w < words ? ~visited[w] : 0ull;  // padding bits are set in visited
-  uint64_t longm = 0;
-  if (longrows) {
-    longm = w < words ? open & longrows[w] : 0ull;
-  } else {
-    for (uint64_t b = open; b; b &= b - 1) {
-      const int i = __ffsll((long long)b) - 1;
-      const uint64_t r = w * 64 + uint64_t(i);
-      if (ofs[r + 1] - ofs[r] > kLongRow) longm |= uint64_t(1) << i;
+  const uint64_t w0 = (uint64_t(blockIdx.x) * kBlock + threadIdx.x) * kOpenWords;
+  uint64_t open[kOpenWords], longm[kOpenWords];
+  uint32_t nl = 0, ns = 0;
+#pragma unroll
+  for (int k = 0; k < kOpenWords; k++) {
+    const uint64_t w = w0 + uint64_t(k);
+    open[k] = 0;
+    longm[k] = 0;
+    if (w < words) {
+      uint64_t v = visited[w];
+      if (merge) {
+        v |= merge[w];
+        visited[w] = v;
+      }
+      open[k] = ~v;  // padding bits are set in visited
+      if (longrows) {
+        longm[k] = open[k] & longrows[w];
+      } else {
+        for (uint64_t b = open[k]; b; b &= b - 1) {
+          const int i = __ffsll((long long)b) - 1;
+          const uint64_t r = w * 64 + uint64_t(i);
+          if (ofs[r + 1] - ofs[r] > kLongRow) longm[k] |= uint64_t(1) << i;
+        }
+      }
+      nl += uint32_t(__popcll(longm[k]));
+      ns += uint32_t(__popcll(open[k] & ~longm[k]));
     }
   }
-  const uint32_t nl = uint32_t(__popcll(longm)), ns = uint32_t(__popcll(open & ~longm));
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   uint32_t xs = ns, xl = nl;
   for (int o = 1; o < 64; o <<= 1) {
@@ -287,12 +306,14 @@ __global__ __launch_bounds__(kBlock) void k_open_rows(const uint64_t* visited, u
     os += wsum[0][k];
     ol += wsum[1][k];
   }
-  for (uint64_t b = open; b; b &= b - 1) {
-    const int i = __ffsll((long long)b) - 1;
-    const uint32_t r = uint32_t(w * 64 + uint64_t(i));
-    if ((longm >> i) & 1) rows_long[ol++] = r;
-    else rows_short[os++] = r;
-  }
+#pragma unroll
+  for (int k = 0; k < kOpenWords; k++)
+    for (uint64_t b = open[k]; b; b &= b - 1) {
+      const int i = __ffsll((long long)b) - 1;
+      const uint32_t r = uint32_t((w0 + uint64_t(k)) * 64 + uint64_t(i));
+      if ((longm[k] >> i) & 1) rows_long[ol++] = r;
+      else rows_short[os++] = r;
+    }
 }
 
 __device__ inline void found_row(uint32_t r, int32_t* lv, int32_t next_lv, uint64_t* nbm) {
@@ -648,9 +669,12 @@ BfsResult bfs_core_ws(ll_gpu_graph* g, const Csr& fwd, const Csr& rev, const uin
         k_fill_u64<<<grid_for(words, 256), 256, 0, s>>>(nbm, words, 0);
         {
           MaybeTimed t(ctx, "k_bu_step", w.timed);
-          k_open_rows<<<grid_for(words, kBlock), kBlock, 0, s>>>(
-              w.visited, words, rev.ofs.as<uint64_t>(), w.longrows, w.rows_short, w.rows_long,
-              ck);
+          // the rows the level before found join visited here (the frontier
+          // bitmap); the last level's are never merged: no later step reads
+          // visited in a search that stays bottom-up
+          k_open_rows<<<grid_for(words, kBlock * kOpenWords), kBlock, 0, s>>>(
+              w.visited, fbm, words, rev.ofs.as<uint64_t>(), w.longrows, w.rows_short,
+              w.rows_long, ck);
           k_bu_short<<<bu_grid, kBlock, 0, s>>>(w.rows_short, rev.ofs.as<uint64_t>(),
                                                 rev.adj.as<uint32_t>(), fdeg, fbm, nbm, lv,
                                                 lvl + 1 + k, ck);
@@ -658,7 +682,6 @@ BfsResult bfs_core_ws(ll_gpu_graph* g, const Csr& fwd, const Csr& rev, const uin
                                                rev.adj.as<uint32_t>(), fdeg, fbm, nbm, lv,
                                                lvl + 1 + k, ck);
         }
-        k_visited_or<<<grid_for(words, 256), 256, 0, s>>>(w.visited, nbm, words);
         std::swap(fbm, nbm);
       }
       LLG_HIP(hipGetLastError());
@@ -682,9 +705,9 @@ BfsResult bfs_core_ws(ll_gpu_graph* g, const Csr& fwd, const Csr& rev, const uin
       k_fill_u64<<<grid_for(words, 256), 256, 0, s>>>(nbm, words, 0);
       {
         MaybeTimed t(ctx, "k_bu_step", w.timed);
-        k_open_rows<<<grid_for(words, kBlock), kBlock, 0, s>>>(
-            w.visited, words, rev.ofs.as<uint64_t>(), w.longrows, w.rows_short, w.rows_long,
-            w.cnt);
+        k_open_rows<<<grid_for(words, kBlock * kOpenWords), kBlock, 0, s>>>(
+            w.visited, nullptr, words, rev.ofs.as<uint64_t>(), w.longrows, w.rows_short,
+            w.rows_long, w.cnt);
         k_bu_short<<<bu_grid, kBlock, 0, s>>>(w.rows_short, rev.ofs.as<uint64_t>(),
                                               rev.adj.as<uint32_t>(), fdeg, fbm, nbm, lv,
                                               lvl + 1, w.cnt);
