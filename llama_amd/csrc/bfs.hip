@@ -240,13 +240,15 @@ constexpr uint32_t kLongRow = 32;
 // Lists of the open rows, short and long, from the visited bitmap: a thread
 // per kOpenWords bitmap words, one list reservation (atomic) per workgroup and
 // list, rows in ascending order.  `merge` (may be null): a bitmap of rows
-// found by the level before, OR-ed into visited first (and written back).  A
-// row is long by the snapshot's long-row bitmap when one is given, else by
-// its offsets.
+// found by the level before, OR-ed into visited first (and written back);
+// `clear` (may be null): a bitmap zeroed over the same words (the level's
+// next frontier).  A row is long by the snapshot's long-row bitmap when one
+// is given, else by its offsets.
 constexpr int kOpenWords = 4;
 
 __global__ __launch_bounds__(kBlock) void k_open_rows(uint64_t* visited, const uint64_t* merge,
-                                                      uint64_t words, const uint64_t* ofs,
+                                                      uint64_t* clear, uint64_t words,
+                                                      const uint64_t* ofs,
                                                       const uint64_t* longrows,
                                                       uint32_t* rows_short, uint32_t* rows_long,
                                                       Counters* c) {
@@ -266,6 +268,7 @@ __global__ __launch_bounds__(kBlock) void k_open_rows(uint64_t* visited, const u
         v |= merge[w];
         visited[w] = v;
       }
+      if (clear) clear[w] = 0;
       open[k] = ~v;  // padding bits are set in visited
       if (longrows) {
         longm[k] = open[k] & longrows[w];
@@ -666,14 +669,14 @@ BfsResult bfs_core_ws(ll_gpu_graph* g, const Csr& fwd, const Csr& rev, const uin
       for (int k = 0; k < kBuBatch; k++) {
         Counters* ck = w.cnt + k * kStripes;
         k_zero_counters<<<1, kStripes, 0, s>>>(ck);
-        k_fill_u64<<<grid_for(words, 256), 256, 0, s>>>(nbm, words, 0);
         {
           MaybeTimed t(ctx, "k_bu_step", w.timed);
           // the rows the level before found join visited here (the frontier
           // bitmap); the last level's are never merged: no later step reads
-          // visited in a search that stays bottom-up
+          // visited in a search that stays bottom-up.  The next frontier's
+          // bitmap is cleared in the same pass.
           k_open_rows<<<grid_for(words, kBlock * kOpenWords), kBlock, 0, s>>>(
-              w.visited, fbm, words, rev.ofs.as<uint64_t>(), w.longrows, w.rows_short,
+              w.visited, fbm, nbm, words, rev.ofs.as<uint64_t>(), w.longrows, w.rows_short,
               w.rows_long, ck);
           k_bu_short<<<bu_grid, kBlock, 0, s>>>(w.rows_short, rev.ofs.as<uint64_t>(),
                                                 rev.adj.as<uint32_t>(), fdeg, fbm, nbm, lv,
@@ -702,11 +705,10 @@ BfsResult bfs_core_ws(ll_gpu_graph* g, const Csr& fwd, const Csr& rev, const uin
     }
     k_zero_counters<<<1, kStripes, 0, s>>>(w.cnt);
     if (bottom_up) {
-      k_fill_u64<<<grid_for(words, 256), 256, 0, s>>>(nbm, words, 0);
       {
         MaybeTimed t(ctx, "k_bu_step", w.timed);
         k_open_rows<<<grid_for(words, kBlock * kOpenWords), kBlock, 0, s>>>(
-            w.visited, nullptr, words, rev.ofs.as<uint64_t>(), w.longrows, w.rows_short,
+            w.visited, nullptr, nbm, words, rev.ofs.as<uint64_t>(), w.longrows, w.rows_short,
             w.rows_long, w.cnt);
         k_bu_short<<<bu_grid, kBlock, 0, s>>>(w.rows_short, rev.ofs.as<uint64_t>(),
                                               rev.adj.as<uint32_t>(), fdeg, fbm, nbm, lv,
