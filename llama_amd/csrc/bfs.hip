@@ -63,6 +63,7 @@ struct alignas(128) Counters {
   unsigned long long m_next;     // their out-edges
   unsigned long long examined;   // edges inspected this level
   unsigned long long aux;        // stripe 0: 1 when the search excludes the reach complement
+  unsigned long long found;      // vertices a top-down level claimed (queued or not)
   unsigned long long n_short;    // stripe 0: open rows walked by a thread each
   unsigned long long n_long;     // stripe 0: open rows walked by a wave each
 };
@@ -103,11 +104,12 @@ __global__ void k_fill_u64(uint64_t* p, uint64_t n, uint64_t v) {
   if (i < n) p[i] = v;
 }
 
-__global__ void k_zero_counters(Counters* c) {  // <<<1, kStripes>>>
-  c += threadIdx.x;
+__global__ void k_zero_counters(Counters* c) {  // <<<sets, kStripes>>>
+  c += uint64_t(blockIdx.x) * kStripes + threadIdx.x;
   c->n_next = 0;
   c->m_next = 0;
   c->examined = 0;
+  c->found = 0;
   c->n_short = 0;
   c->n_long = 0;
 }
@@ -153,6 +155,7 @@ __device__ inline void wave_stage(bool take, uint32_t v, uint32_t outdeg, uint32
   for (int o = 32; o > 0; o >>= 1) md += __shfl_xor(md, o, 64);
   if (lane == leader && m) {
     atomicAdd(&stripe(c)->m_next, (unsigned long long)md);
+    atomicAdd(&stripe(c)->found, (unsigned long long)__popcll(m));
     if (mq) qbase = atomicAdd(lqn, uint32_t(__popcll(mq)));
   }
   qbase = uint32_t(__shfl(int(qbase), leader, 64));
@@ -517,15 +520,6 @@ __global__ void k_dead_rows(const uint64_t* ofs, uint64_t V, uint64_t words, uin
   }
 }
 
-// Reached vertices (grid-stride over a fixed grid: one atomic per workgroup).
-__global__ void k_count_reached(const int32_t* lv, uint64_t V, unsigned long long* out) {
-  unsigned long long n = 0;
-  for (uint64_t v = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; v < V;
-       v += uint64_t(gridDim.x) * blockDim.x)
-    n += lv[v] != kUnreached ? 1ull : 0ull;
-  block_atomic_add(out, n);
-}
-
 // Summed stripes (aux from stripe 0); with `extra`, one more device word is
 // read back in the same round trip into *extra_out.
 // The summed stripes of n consecutive counter sets, one read-back.
@@ -543,6 +537,7 @@ void read_counter_sets(ll_gpu_ctx* ctx, const Counters* c, int n, Counters* out)
       h.n_next += sk[i].n_next;
       h.m_next += sk[i].m_next;
       h.examined += sk[i].examined;
+      h.found += sk[i].found;
     }
     out[k] = h;
   }
@@ -565,6 +560,7 @@ Counters read_counters(ll_gpu_ctx* ctx, const Counters* c, const uint64_t* extra
     h.n_next += st[i].n_next;
     h.m_next += st[i].m_next;
     h.examined += st[i].examined;
+    h.found += st[i].found;
   }
   return h;
 }
@@ -666,9 +662,9 @@ BfsResult bfs_core_ws(ll_gpu_graph* g, const Csr& fwd, const Csr& rev, const uin
     if (bottom_up && beta == 0) {
       // no return to top-down: kBuBatch levels per read-back; the levels
       // after the one that finds nothing find nothing either
+      k_zero_counters<<<kBuBatch, kStripes, 0, s>>>(w.cnt);  // the batch's counter sets
       for (int k = 0; k < kBuBatch; k++) {
         Counters* ck = w.cnt + k * kStripes;
-        k_zero_counters<<<1, kStripes, 0, s>>>(ck);
         {
           MaybeTimed t(ctx, "k_bu_step", w.timed);
           // the rows the level before found join visited here (the frontier
@@ -750,7 +746,9 @@ BfsResult bfs_core_ws(ll_gpu_graph* g, const Csr& fwd, const Csr& rev, const uin
         nq = read_counters(ctx, w.cnt).n_next;
       }
     } else {
-      // top-down n_next counts the queued vertices (those with out-edges)
+      // top-down n_next counts the queued vertices (those with out-edges),
+      // found every claimed vertex
+      r.reached += h.found;
       nq = n_next;
       n_f = n_next;
       m_f = h.m_next;
@@ -791,11 +789,11 @@ BfsResult bfs_core(ll_gpu_graph* g, const Csr& fwd, const Csr& rev, const uint32
 
 // Working memory of one single-source search, carved into 256-byte aligned
 // regions: three bitmaps, two queues and the queue degrees, the queue
-// offsets, the striped counters, the reached counter and the scan's
-// temporary memory.
+// offsets, the striped counters and the scan's temporary memory, and the two
+// lists of open rows.
 namespace {
 struct WsLayout {
-  size_t visited, fbm, nbm, q0, q1, qdeg, qofs, cnt, red, stmp, rs, rl, total;
+  size_t visited, fbm, nbm, q0, q1, qdeg, qofs, cnt, stmp, rs, rl, total;
 };
 WsLayout ws_layout(uint64_t V) {
   const uint64_t words = (V + 63) / 64;
@@ -811,7 +809,6 @@ WsLayout ws_layout(uint64_t V) {
   L.qdeg = take(V * 4);
   L.qofs = take((V + 1) * 8);
   L.cnt = take(kBuBatch * kStripes * sizeof(Counters));
-  L.red = take(8);
   L.stmp = take(scan_tmp_bytes(V));
   L.rs = take(std::max<uint64_t>(V, 1) * 4);
   L.rl = take(std::max<uint64_t>(V, 1) * 4);
@@ -898,9 +895,7 @@ BfsPrepared bfs_prepare(ll_gpu_graph* g, int level) {
 
 BfsResult bfs_single_ws(ll_gpu_graph* g, const BfsPrepared& p, uint32_t root, void* ws,
                         int32_t* lv, bool timed) {
-  hipStream_t s = g->ctx->stream;
-  const uint64_t V = g->V;
-  const WsLayout L = ws_layout(V);
+  const WsLayout L = ws_layout(g->V);
   char* b = static_cast<char*>(ws);
   BfsWork w{reinterpret_cast<uint64_t*>(b + L.visited), reinterpret_cast<uint64_t*>(b + L.fbm),
             reinterpret_cast<uint64_t*>(b + L.nbm),
@@ -912,18 +907,8 @@ BfsResult bfs_single_ws(ll_gpu_graph* g, const BfsPrepared& p, uint32_t root, vo
   // option bfs_reach 0: dead rows only (for comparison)
   const auto& opts = g->ctx->options;
   const bool use_reach = !opts.count("bfs_reach") || opts.at("bfs_reach") != 0;
-  BfsResult r = bfs_core_ws(g, *p.out, *p.in, p.outdeg, root, p.dead,
-                            use_reach ? p.notr : nullptr, p.e_reach, w, lv);
-  // top-down levels count only queued vertices: recount the reached set
-  unsigned long long* red = reinterpret_cast<unsigned long long*>(b + L.red);
-  k_fill_u64<<<1, 1, 0, s>>>(reinterpret_cast<uint64_t*>(red), 1, 0);
-  k_count_reached<<<std::min(grid_for(V, 256), 4u * unsigned(g->ctx->num_cus)), 256, 0, s>>>(
-      lv, V, red);
-  LLG_HIP(hipGetLastError());
-  LLG_HIP(hipMemcpyAsync(g->ctx->readback, red, 8, hipMemcpyDeviceToHost, s));
-  LLG_HIP(hipStreamSynchronize(s));
-  r.reached = *static_cast<const uint64_t*>(g->ctx->readback);
-  return r;
+  return bfs_core_ws(g, *p.out, *p.in, p.outdeg, root, p.dead, use_reach ? p.notr : nullptr,
+                     p.e_reach, w, lv);
 }
 
 // Single-source BFS; per-vertex levels stay in `lv` (device, int32[V]).
