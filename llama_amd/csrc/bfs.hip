@@ -104,14 +104,17 @@ __global__ void k_fill_u64(uint64_t* p, uint64_t n, uint64_t v) {
   if (i < n) p[i] = v;
 }
 
-__global__ void k_zero_counters(Counters* c) {  // <<<sets, kStripes>>>
-  c += uint64_t(blockIdx.x) * kStripes + threadIdx.x;
+__device__ inline void zero_stripe(Counters* c) {  // aux is left as it is
   c->n_next = 0;
   c->m_next = 0;
   c->examined = 0;
   c->found = 0;
   c->n_short = 0;
   c->n_long = 0;
+}
+
+__global__ void k_zero_counters(Counters* c) {  // <<<sets, kStripes>>>
+  zero_stripe(c + uint64_t(blockIdx.x) * kStripes + threadIdx.x);
 }
 
 // Counters::aux of stripe 0 = 1 when every vertex the search from `root` can
@@ -131,12 +134,6 @@ __global__ void k_reach_check(uint32_t root, const uint64_t* notr, const uint64_
     }
   __syncthreads();
   if (threadIdx.x == 0) c->aux = (in_r || !bad) ? 1ull : 0ull;
-}
-
-__global__ void k_queue_degrees(const uint32_t* q, uint64_t n, const uint32_t* outdeg,
-                                uint32_t* qdeg) {
-  uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i < n) qdeg[i] = outdeg[q[i]];
 }
 
 // Stage the calling lane's vertex (when it has out-edges) in the workgroup's
@@ -165,7 +162,7 @@ __device__ inline void wave_stage(bool take, uint32_t v, uint32_t outdeg, uint32
 __global__ __launch_bounds__(kBlock) void k_td_expand(
     const uint32_t* q, const uint64_t* qofs, uint64_t nq, uint64_t m_f, const uint64_t* ofs,
     const uint32_t* adj, const uint32_t* outdeg, uint64_t* visited, int32_t* lv, int32_t next_lv,
-    uint32_t* q_next, Counters* c) {
+    uint32_t* q_next, Counters* c, Counters* zero_next) {
   __shared__ uint64_t l_qofs[kEdgesPerBlock + 1];
   __shared__ uint64_t l_base[kEdgesPerBlock];
   __shared__ uint64_t sh_lo, sh_hi;
@@ -174,6 +171,9 @@ __global__ __launch_bounds__(kBlock) void k_td_expand(
   __shared__ unsigned long long l_qbase;
   const uint64_t x0 = uint64_t(blockIdx.x) * kEdgesPerBlock;
   const uint64_t x1 = min(m_f, x0 + kEdgesPerBlock);
+  // the counter set of the next top-down level (read by the host before this
+  // launch) starts at zero
+  if (blockIdx.x == 0 && threadIdx.x < kStripes) zero_stripe(zero_next + threadIdx.x);
   if (threadIdx.x == 0) {
     uint64_t lo = 0, hi = nq;
     while (hi - lo > 1) { uint64_t mid = (lo + hi) / 2; if (qofs[mid] <= x0) lo = mid; else hi = mid; }
@@ -591,7 +591,6 @@ struct BfsWork {
   uint64_t* fbm;
   uint64_t* nbm;
   uint32_t* q[2];
-  uint32_t* qdeg;
   uint64_t* qofs;
   Counters* cnt;
   void* scan_tmp;       // scan_tmp_bytes(V)
@@ -622,7 +621,9 @@ BfsResult bfs_core_ws(ll_gpu_graph* g, const Csr& fwd, const Csr& rev, const uin
   const uint64_t words = (V + 63) / 64;
   uint64_t* fbm = w.fbm;  // frontier / next-frontier bitmaps, swapped per bottom-up level
   uint64_t* nbm = w.nbm;
-  k_zero_counters<<<1, kStripes, 0, s>>>(w.cnt);
+  // set 0: the root and bottom-up levels; top-down levels alternate sets 1
+  // and 0, each expansion zeroing the other
+  k_zero_counters<<<2, kStripes, 0, s>>>(w.cnt);
   if (notr)
     k_reach_check<<<1, 256, 0, s>>>(root, notr, fwd.ofs.as<uint64_t>(), fwd.adj.as<uint32_t>(),
                                     w.cnt);
@@ -642,6 +643,7 @@ BfsResult bfs_core_ws(ll_gpu_graph* g, const Csr& fwd, const Csr& rev, const uin
   uint64_t m_visited = h.m_next;
   r.reached = 1;
   bool bottom_up = false;
+  int td = 1;  // counter set of the next top-down level (zeroed)
   int cur = 0;
   int32_t lvl = 0;
   // out-edges a search can still reach: those inside R when R excludes the rest
@@ -699,8 +701,9 @@ BfsResult bfs_core_ws(ll_gpu_graph* g, const Csr& fwd, const Csr& rev, const uin
       if (ended) break;
       continue;
     }
-    k_zero_counters<<<1, kStripes, 0, s>>>(w.cnt);
+    Counters* const cl = bottom_up ? w.cnt : w.cnt + td * kStripes;  // this level's counters
     if (bottom_up) {
+      k_zero_counters<<<1, kStripes, 0, s>>>(w.cnt);
       {
         MaybeTimed t(ctx, "k_bu_step", w.timed);
         k_open_rows<<<grid_for(words, kBlock * kOpenWords), kBlock, 0, s>>>(
@@ -715,17 +718,16 @@ BfsResult bfs_core_ws(ll_gpu_graph* g, const Csr& fwd, const Csr& rev, const uin
       }
       k_visited_or<<<grid_for(words, 256), 256, 0, s>>>(w.visited, nbm, words);
     } else {
-      k_queue_degrees<<<grid_for(nq, 256), 256, 0, s>>>(w.q[cur], nq, fdeg,
-                                                        w.qdeg);
-      scan_u32_to_u64_ws(w.qdeg, w.qofs, nq, w.scan_tmp, s);
+      // queue offsets: the prefix of the queued vertices' out-degrees
+      scan_gather_u32_to_u64_ws(w.q[cur], fdeg, w.qofs, nq, w.scan_tmp, s);
       MaybeTimed t(ctx, "k_td_expand", w.timed);
       k_td_expand<<<grid_for(m_f, kEdgesPerBlock), kBlock, 0, s>>>(
           w.q[cur], w.qofs, nq, m_f, fwd.ofs.as<uint64_t>(),
           fwd.adj.as<uint32_t>(), fdeg, w.visited, lv, lvl + 1,
-          w.q[cur ^ 1], w.cnt);
+          w.q[cur ^ 1], cl, w.cnt + (1 - td) * kStripes);
     }
     LLG_HIP(hipGetLastError());
-    h = read_counters(ctx, w.cnt);
+    h = read_counters(ctx, cl);
     r.examined += h.examined;
     const uint64_t n_next = h.n_next;
     lvl++;
@@ -739,11 +741,13 @@ BfsResult bfs_core_ws(ll_gpu_graph* g, const Csr& fwd, const Csr& rev, const uin
       if (n_f == 0) break;
       if (beta > 0 && shrinking && n_f < V / beta) {
         bottom_up = false;
-        k_zero_counters<<<1, kStripes, 0, s>>>(w.cnt);
+        // set 0 counts the queue; set 1 serves the next top-down level
+        k_zero_counters<<<2, kStripes, 0, s>>>(w.cnt);
         k_bitmap_to_queue<<<grid_for(words, kBlock), kBlock, 0, s>>>(
             fbm, V, fdeg, w.q[cur], w.cnt);
         LLG_HIP(hipGetLastError());
         nq = read_counters(ctx, w.cnt).n_next;
+        td = 1;
       }
     } else {
       // top-down n_next counts the queued vertices (those with out-edges),
@@ -753,6 +757,7 @@ BfsResult bfs_core_ws(ll_gpu_graph* g, const Csr& fwd, const Csr& rev, const uin
       n_f = n_next;
       m_f = h.m_next;
       cur ^= 1;
+      td ^= 1;
     }
   }
   r.levels = lvl;
@@ -767,13 +772,12 @@ BfsResult bfs_core(ll_gpu_graph* g, const Csr& fwd, const Csr& rev, const uint32
   hipStream_t s = g->ctx->stream;
   const uint64_t V = g->V;
   const uint64_t words = (V + 63) / 64;
-  DevBuf fbm, nbm, q[2], qdeg, qofs, cnt, stmp;
+  DevBuf fbm, nbm, q[2], qofs, cnt, stmp;
   visited.alloc(words * 8, s);
   fbm.alloc(words * 8, s);
   nbm.alloc(words * 8, s);
   q[0].alloc(V * 4, s);
   q[1].alloc(V * 4, s);
-  qdeg.alloc(V * 4, s);
   qofs.alloc((V + 1) * 8, s);
   cnt.alloc(kBuBatch * kStripes * sizeof(Counters), s);
   const size_t sb = scan_tmp_bytes(V);
@@ -782,18 +786,17 @@ BfsResult bfs_core(ll_gpu_graph* g, const Csr& fwd, const Csr& rev, const uint32
   rs.alloc(std::max<uint64_t>(V, 1) * 4, s);
   rl.alloc(std::max<uint64_t>(V, 1) * 4, s);
   BfsWork w{visited.as<uint64_t>(), fbm.as<uint64_t>(), nbm.as<uint64_t>(),
-            {q[0].as<uint32_t>(), q[1].as<uint32_t>()}, qdeg.as<uint32_t>(), qofs.as<uint64_t>(),
+            {q[0].as<uint32_t>(), q[1].as<uint32_t>()}, qofs.as<uint64_t>(),
             cnt.as<Counters>(), stmp.p, rs.as<uint32_t>(), rl.as<uint32_t>(), true};
   return bfs_core_ws(g, fwd, rev, fdeg, root, excluded, nullptr, 0, w, lv);
 }
 
 // Working memory of one single-source search, carved into 256-byte aligned
-// regions: three bitmaps, two queues and the queue degrees, the queue
-// offsets, the striped counters and the scan's temporary memory, and the two
-// lists of open rows.
+// regions: three bitmaps, two queues, the queue offsets, the striped
+// counters and the scan's temporary memory, and the two lists of open rows.
 namespace {
 struct WsLayout {
-  size_t visited, fbm, nbm, q0, q1, qdeg, qofs, cnt, stmp, rs, rl, total;
+  size_t visited, fbm, nbm, q0, q1, qofs, cnt, stmp, rs, rl, total;
 };
 WsLayout ws_layout(uint64_t V) {
   const uint64_t words = (V + 63) / 64;
@@ -806,7 +809,6 @@ WsLayout ws_layout(uint64_t V) {
   L.nbm = take(words * 8);
   L.q0 = take(V * 4);
   L.q1 = take(V * 4);
-  L.qdeg = take(V * 4);
   L.qofs = take((V + 1) * 8);
   L.cnt = take(kBuBatch * kStripes * sizeof(Counters));
   L.stmp = take(scan_tmp_bytes(V));
@@ -900,7 +902,7 @@ BfsResult bfs_single_ws(ll_gpu_graph* g, const BfsPrepared& p, uint32_t root, vo
   BfsWork w{reinterpret_cast<uint64_t*>(b + L.visited), reinterpret_cast<uint64_t*>(b + L.fbm),
             reinterpret_cast<uint64_t*>(b + L.nbm),
             {reinterpret_cast<uint32_t*>(b + L.q0), reinterpret_cast<uint32_t*>(b + L.q1)},
-            reinterpret_cast<uint32_t*>(b + L.qdeg), reinterpret_cast<uint64_t*>(b + L.qofs),
+            reinterpret_cast<uint64_t*>(b + L.qofs),
             reinterpret_cast<Counters*>(b + L.cnt), b + L.stmp,
             reinterpret_cast<uint32_t*>(b + L.rs), reinterpret_cast<uint32_t*>(b + L.rl), timed,
             p.longrows};

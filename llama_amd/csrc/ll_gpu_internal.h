@@ -422,6 +422,9 @@ void scan_u32_to_u64(const uint32_t* in, uint64_t* out, uint64_t n, hipStream_t 
 // The same in caller memory: tmp holds scan_tmp_bytes(n) bytes (8-byte aligned).
 size_t scan_tmp_bytes(uint64_t n);
 void scan_u32_to_u64_ws(const uint32_t* in, uint64_t* out, uint64_t n, void* tmp, hipStream_t s);
+// The same over val[idx[i]], i < n.
+void scan_gather_u32_to_u64_ws(const uint32_t* idx, const uint32_t* val, uint64_t* out,
+                               uint64_t n, void* tmp, hipStream_t s);
 void scan_u64(const uint64_t* in, uint64_t* out, uint64_t n, hipStream_t s);
 void sort_pairs_u64_u32(const uint64_t* keys_in, uint64_t* keys_out, const uint32_t* vals_in,
                         uint32_t* vals_out, uint64_t n, int end_bit, hipStream_t s);

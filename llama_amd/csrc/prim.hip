@@ -38,6 +38,12 @@ struct SrcU64 {
   const uint64_t* p;
   __device__ uint64_t operator()(uint64_t i) const { return p[i]; }
 };
+// val[idx[i]]
+struct SrcGatherU32 {
+  const uint32_t* idx;
+  const uint32_t* val;
+  __device__ uint64_t operator()(uint64_t i) const { return val[idx[i]]; }
+};
 // 1 where a sorted run of equal values starts
 struct SrcRunStart {
   const uint64_t* p;
@@ -409,6 +415,11 @@ size_t scan_tmp_bytes(uint64_t n) {
 void scan_u32_to_u64_ws(const uint32_t* in, uint64_t* out, uint64_t n, void* tmp,
                         hipStream_t s) {
   scan_rec(SrcU32{in}, n, out, static_cast<uint64_t*>(tmp), s);
+}
+
+void scan_gather_u32_to_u64_ws(const uint32_t* idx, const uint32_t* val, uint64_t* out,
+                               uint64_t n, void* tmp, hipStream_t s) {
+  scan_rec(SrcGatherU32{idx, val}, n, out, static_cast<uint64_t*>(tmp), s);
 }
 
 void scan_u32_to_u64(const uint32_t* in, uint64_t* out, uint64_t n, hipStream_t s) {
