@@ -354,39 +354,64 @@ __device__ inline void block_counters(unsigned long long n_new, unsigned long lo
   }
 }
 
+// A wave's found rows set their bits of the next frontier: lanes holding the
+// same bitmap word (the row lists ascend, so they are neighbours) OR their
+// bits together first, and the first lane of each such run issues the
+// atomic.  Every lane of the wave calls it.
+__device__ inline void wave_set_bits(uint64_t* bm, bool found, uint32_t r, bool valid) {
+  const int lane = threadIdx.x & 63;
+  const uint64_t word = valid ? uint64_t(r >> 6) : ~uint64_t(0);
+  uint64_t bits = found ? uint64_t(1) << (r & 63) : 0ull;
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint64_t ob = __shfl_down(bits, o, 64);
+    const uint64_t ow = __shfl_down(word, o, 64);
+    if (lane + o < 64 && ow == word) bits |= ob;
+  }
+  const uint64_t prev = __shfl_up(word, 1, 64);
+  if (bits && (lane == 0 || prev != word))
+    atomicOr((unsigned long long*)&bm[word], (unsigned long long)bits);
+}
+
 __global__ __launch_bounds__(kBlock) void k_bu_short(const uint32_t* rows, const uint64_t* ofs,
                                                      const uint32_t* adj, const uint32_t* outdeg,
                                                      const uint64_t* fbm, uint64_t* nbm,
                                                      int32_t* lv, int32_t next_lv, Counters* c) {
   const uint64_t n = c->n_short;  // stripe 0
   unsigned long long n_new = 0, m_new = 0, exam = 0;
-  for (uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x; i < n;
-       i += uint64_t(gridDim.x) * kBlock) {
-    const uint32_t r = rows[i];
-    uint64_t e = ofs[r];
-    const uint64_t hi = ofs[r + 1];
+  // rounds the same for every lane of a wave (wave_set_bits)
+  for (uint64_t base = uint64_t(blockIdx.x) * kBlock; base < n;
+       base += uint64_t(gridDim.x) * kBlock) {
+    const uint64_t i = base + threadIdx.x;
+    const bool valid = i < n;
+    uint32_t r = 0;
     bool found = false;
-    while (e < hi && !found) {
-      const uint32_t m = uint32_t(min(uint64_t(4), hi - e));
-      uint32_t u[4];
-      uint64_t fw[4];
+    if (valid) {
+      r = rows[i];
+      uint64_t e = ofs[r];
+      const uint64_t hi = ofs[r + 1];
+      while (e < hi && !found) {
+        const uint32_t m = uint32_t(min(uint64_t(4), hi - e));
+        uint32_t u[4];
+        uint64_t fw[4];
 #pragma unroll
-      for (uint32_t k = 0; k < 4; k++) u[k] = adj[e + (k < m ? k : 0)];
+        for (uint32_t k = 0; k < 4; k++) u[k] = adj[e + (k < m ? k : 0)];
 #pragma unroll
-      for (uint32_t k = 0; k < 4; k++) fw[k] = fbm[u[k] >> 6];
-      uint32_t hit = 0;  // 1 + position of the first neighbour in the frontier
+        for (uint32_t k = 0; k < 4; k++) fw[k] = fbm[u[k] >> 6];
+        uint32_t hit = 0;  // 1 + position of the first neighbour in the frontier
 #pragma unroll
-      for (int k = 3; k >= 0; k--)
-        if (uint32_t(k) < m && ((fw[k] >> (u[k] & 63)) & 1)) hit = uint32_t(k) + 1;
-      found = hit != 0;
-      exam += found ? hit : m;
-      e += m;
+        for (int k = 3; k >= 0; k--)
+          if (uint32_t(k) < m && ((fw[k] >> (u[k] & 63)) & 1)) hit = uint32_t(k) + 1;
+        found = hit != 0;
+        exam += found ? hit : m;
+        e += m;
+      }
+      if (found) {
+        if (lv) lv[r] = next_lv;
+        n_new++;
+        m_new += outdeg[r];
+      }
     }
-    if (found) {
-      found_row(r, lv, next_lv, nbm);
-      n_new++;
-      m_new += outdeg[r];
-    }
+    wave_set_bits(nbm, found, r, valid);
   }
   block_counters(n_new, m_new, exam, c);
 }
