@@ -11,14 +11,14 @@
 //     workgroup's slice of the scan), claiming vertices with an atomic OR on
 //     the visited bitmap; each workgroup stages its new vertices in LDS and
 //     reserves their queue slots with one atomic;
-//   * bottom-up over the row blocks of the in-edges that still hold an open
-//     row (unvisited and not excluded): each level's kernel flags the blocks
-//     that stay open, and a scan compacts them into the next level's list;
-//     a block reads its neighbour IDs coalesced and packs "neighbour in the
-//     frontier bitmap" into 64-bit words with one wave ballot per 64 edges
-//     (the V/8-byte bitmap stays in L2), and every unvisited row tests its
-//     bit range; new vertices are set in the next frontier bitmap with an
-//     atomic OR whose old value keeps counts exact.
+//   * bottom-up over lists of the open rows (unvisited and not excluded),
+//     listed in ascending order from the visited bitmap before each level:
+//     rows of at most 32 in-edges a thread each, four neighbour IDs and then
+//     their four frontier words at a time, stopping at the first neighbour in
+//     the frontier (the V/8-byte bitmap stays in L2); longer rows a wave
+//     each, 64 neighbours per step, stopping once a ballot finds one; a
+//     wave's found rows OR their next-frontier bits per bitmap word before
+//     one atomic per word.
 // Rows no search from the root can enter start as visited: rows without a
 // live in-edge always, and, when the root's successors all lie in the
 // snapshot's reach set (the vertices reachable from its highest-degree
@@ -31,8 +31,8 @@
 // bottom-up stays bottom-up -- the open rows it still lists are the rows of
 // the reach set it has not found yet, so each further level walks only those
 // and the lists shrink, while a return to top-down pays for the queue and the
-// frontier's out-edges (R-MAT s24, 64 sources: 115 against 101 GTEPS with
-// beta 24).  Such a search launches its bottom-up levels kBuBatch at a time,
+// frontier's out-edges (R-MAT s24, 64 sources, when the choice was measured:
+// 115 against 101 GTEPS with beta 24).  Such a search launches its bottom-up levels kBuBatch at a time,
 // each with its own counters, and reads them back once per batch: the only
 // decision left is where the search ended, and the levels launched past that
 // find nothing (their frontier is empty or has no out-edges).
