@@ -23,6 +23,7 @@ namespace llgpu {
 namespace {
 
 constexpr uint32_t kNone = 0xffffffffu;
+constexpr int kReduceBlock = 256;
 
 __global__ void k_fill_u32(uint32_t* p, uint64_t n, uint32_t x) {
   const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -51,20 +52,40 @@ __global__ void k_trim(const uint64_t* iofs, const uint32_t* iadj, const uint64_
   block_atomic_add(changed, t ? 1ull : 0ull);
 }
 
-__global__ void k_pivot(const uint32_t* label, const uint32_t* dout, const uint32_t* din,
-                        uint64_t V, unsigned long long* best) {
-  const uint64_t v = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  unsigned long long key = 0;
-  if (v < V && label[v] == kNone) {
-    unsigned long long p = (unsigned long long)(dout[v] + 1ull) * (din[v] + 1ull);
-    if (p > 0x7fffffffull) p = 0x7fffffffull;
-    key = ((p + 1) << 32) | v;
-  }
+// The workgroup's largest (kMax) or smallest x, in thread 0.  Every thread
+// calls it.  The reductions below run a fixed grid (grid-stride), so the
+// single result word takes one atomic per workgroup, not one per wave.
+template <bool kMax>
+__device__ inline unsigned long long block_extreme(unsigned long long x) {
+  __shared__ unsigned long long part[kReduceBlock / 64];
   for (int o = 32; o > 0; o >>= 1) {
-    unsigned long long t = __shfl_down(key, o, 64);
-    key = t > key ? t : key;
+    const unsigned long long t = __shfl_down(x, o, 64);
+    x = kMax ? (t > x ? t : x) : (t < x ? t : x);
   }
-  if ((threadIdx.x & 63) == 0 && key) atomicMax(best, key);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = x;
+  __syncthreads();
+  if (threadIdx.x == 0)
+    for (int i = 1; i < kReduceBlock / 64; i++)
+      x = kMax ? (part[i] > x ? part[i] : x) : (part[i] < x ? part[i] : x);
+  return x;
+}
+
+__global__ __launch_bounds__(kReduceBlock) void k_pivot(const uint32_t* label,
+                                                        const uint32_t* dout,
+                                                        const uint32_t* din, uint64_t V,
+                                                        unsigned long long* best) {
+  unsigned long long key = 0;
+  for (uint64_t v = uint64_t(blockIdx.x) * kReduceBlock + threadIdx.x; v < V;
+       v += uint64_t(gridDim.x) * kReduceBlock) {
+    if (label[v] == kNone) {
+      unsigned long long p = (unsigned long long)(dout[v] + 1ull) * (din[v] + 1ull);
+      if (p > 0x7fffffffull) p = 0x7fffffffull;
+      const unsigned long long k = ((p + 1) << 32) | v;
+      key = k > key ? k : key;
+    }
+  }
+  key = block_extreme<true>(key);
+  if (threadIdx.x == 0 && key) atomicMax(best, key);
 }
 
 // inactive (labelled) vertices as a bitmap; wave w builds word w
@@ -76,12 +97,17 @@ __global__ void k_inactive_bitmap(const uint32_t* label, uint64_t V, uint64_t* b
   if ((threadIdx.x & 63) == 0) bm[v >> 6] = m;
 }
 
-__global__ void k_fwbw_min(const uint64_t* fw, const uint64_t* bw, const uint64_t* excl,
-                           uint64_t V, unsigned long long* mn) {
-  const uint64_t v = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (v >= V) return;
-  const uint64_t w = v >> 6, b = uint64_t(1) << (v & 63);
-  if ((fw[w] & bw[w] & ~excl[w]) & b) atomicMin(mn, (unsigned long long)v);
+__global__ __launch_bounds__(kReduceBlock) void k_fwbw_min(const uint64_t* fw, const uint64_t* bw,
+                                                           const uint64_t* excl, uint64_t V,
+                                                           unsigned long long* mn) {
+  unsigned long long m = ~0ull;
+  for (uint64_t v = uint64_t(blockIdx.x) * kReduceBlock + threadIdx.x; v < V;
+       v += uint64_t(gridDim.x) * kReduceBlock) {
+    const uint64_t w = v >> 6, b = uint64_t(1) << (v & 63);
+    if (((fw[w] & bw[w] & ~excl[w]) & b) && v < m) m = v;
+  }
+  m = block_extreme<false>(m);
+  if (threadIdx.x == 0 && m != ~0ull) atomicMin(mn, m);
 }
 
 __global__ void k_fwbw_assign(const uint64_t* fw, const uint64_t* bw, const uint64_t* excl,
@@ -165,15 +191,17 @@ __global__ void k_comp_assign(uint32_t* label, const uint32_t* color, const uint
   if (v < V && label[v] == kNone && mark[v]) label[v] = minlab[color[v]];
 }
 
-__global__ void k_labels_out(const uint32_t* label, int64_t* out, uint64_t V,
-                             unsigned long long* nscc) {
-  const uint64_t v = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  bool root = false;
-  if (v < V) {
-    out[v] = int64_t(label[v]);
-    root = label[v] == uint32_t(v);
+__global__ __launch_bounds__(kReduceBlock) void k_labels_out(const uint32_t* label, int64_t* out,
+                                                             uint64_t V,
+                                                             unsigned long long* nscc) {
+  unsigned long long roots = 0;
+  for (uint64_t v = uint64_t(blockIdx.x) * kReduceBlock + threadIdx.x; v < V;
+       v += uint64_t(gridDim.x) * kReduceBlock) {
+    const uint32_t l = label[v];
+    out[v] = int64_t(l);
+    roots += l == uint32_t(v) ? 1ull : 0ull;
   }
-  block_atomic_add(nscc, root ? 1ull : 0ull);
+  block_atomic_add(nscc, roots);
 }
 
 // One counter through the context's pinned read-back buffer.
@@ -222,7 +250,10 @@ SccResult scc_device(ll_gpu_graph* g, int level, DevBuf& labels64) {
   trim(4);
   // forward-backward from the pivot
   k_zero<<<1, 1, 0, s>>>(cnt.as<unsigned long long>());
-  k_pivot<<<gV, 256, 0, s>>>(label.as<uint32_t>(), dout, din, V, cnt.as<unsigned long long>());
+  // reductions over every vertex: a fixed grid, one atomic per workgroup
+  const unsigned gR = std::min(gV, 4u * unsigned(ctx->num_cus));
+  k_pivot<<<gR, kReduceBlock, 0, s>>>(label.as<uint32_t>(), dout, din, V,
+                                      cnt.as<unsigned long long>());
   LLG_HIP(hipGetLastError());
   const unsigned long long best = read_ull(g->ctx, cnt);
   if (best) {
@@ -237,8 +268,8 @@ SccResult scc_device(ll_gpu_graph* g, int level, DevBuf& labels64) {
     DevBuf mn;
     mn.alloc(8, s);
     k_fill_u32<<<1, 2, 0, s>>>(mn.as<uint32_t>(), 2, kNone);
-    k_fwbw_min<<<gV, 256, 0, s>>>(fw.as<uint64_t>(), bw.as<uint64_t>(), excl.as<uint64_t>(), V,
-                                  mn.as<unsigned long long>());
+    k_fwbw_min<<<gR, kReduceBlock, 0, s>>>(fw.as<uint64_t>(), bw.as<uint64_t>(),
+                                           excl.as<uint64_t>(), V, mn.as<unsigned long long>());
     k_fwbw_assign<<<gV, 256, 0, s>>>(fw.as<uint64_t>(), bw.as<uint64_t>(), excl.as<uint64_t>(),
                                      V, mn.as<unsigned long long>(), label.as<uint32_t>());
     LLG_HIP(hipGetLastError());
@@ -286,8 +317,8 @@ SccResult scc_device(ll_gpu_graph* g, int level, DevBuf& labels64) {
   }
   labels64.alloc(V * 8, s);
   k_zero<<<1, 1, 0, s>>>(cnt.as<unsigned long long>());
-  k_labels_out<<<gV, 256, 0, s>>>(label.as<uint32_t>(), labels64.as<int64_t>(), V,
-                                  cnt.as<unsigned long long>());
+  k_labels_out<<<gR, kReduceBlock, 0, s>>>(label.as<uint32_t>(), labels64.as<int64_t>(), V,
+                                           cnt.as<unsigned long long>());
   LLG_HIP(hipGetLastError());
   r.num_scc = read_ull(g->ctx, cnt);
   return r;
