@@ -32,23 +32,40 @@ __global__ void k_fill_u32(uint32_t* p, uint64_t n, uint32_t x) {
 
 __global__ void k_zero(unsigned long long* p) { *p = 0; }
 
-__device__ inline bool has_active(const uint64_t* ofs, const uint32_t* adj, uint64_t v,
-                                  const uint32_t* label) {
+// Active (unlabelled) vertices as a bitmap; wave w builds word w.
+__global__ void k_active_bitmap(const uint32_t* label, uint64_t V, uint64_t* act) {
+  const uint64_t v = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const uint64_t words = (V + 63) / 64;
+  if ((v >> 6) >= words) return;
+  const uint64_t m = __ballot(v < V && label[v] == kNone);
+  if ((threadIdx.x & 63) == 0) act[v >> 6] = m;
+}
+
+__device__ inline bool has_active_bm(const uint64_t* ofs, const uint32_t* adj, uint64_t v,
+                                     const uint64_t* act) {
   for (uint64_t e = ofs[v], end = ofs[v + 1]; e < end; e++) {
     const uint32_t u = adj[e];
-    if (u != v && label[u] == kNone) return true;
+    if (u != v && ((act[u >> 6] >> (u & 63)) & 1)) return true;
   }
   return false;
 }
 
-__global__ void k_trim(const uint64_t* iofs, const uint32_t* iadj, const uint64_t* oofs,
-                       const uint32_t* oadj, uint32_t* label, uint64_t V,
-                       unsigned long long* changed) {
+// Neighbours are tested in the active bitmap (V/8 bytes, L2-resident) rather
+// than in the labels; a wave's vertices share one bitmap word, whose trimmed
+// bits the wave clears with one atomic.  Vertices trimmed earlier in the same
+// sweep may or may not be seen as inactive: either way a trimmed vertex is a
+// singleton component, and the sweeps repeat until none trims.
+__global__ __launch_bounds__(256) void k_trim(const uint64_t* iofs, const uint32_t* iadj,
+                                              const uint64_t* oofs, const uint32_t* oadj,
+                                              uint32_t* label, uint64_t* act, uint64_t V,
+                                              unsigned long long* changed) {
   const uint64_t v = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   bool t = false;
-  if (v < V && label[v] == kNone)
-    t = !has_active(iofs, iadj, v, label) || !has_active(oofs, oadj, v, label);
+  if (v < V && ((act[v >> 6] >> (v & 63)) & 1))
+    t = !has_active_bm(iofs, iadj, v, act) || !has_active_bm(oofs, oadj, v, act);
   if (t) label[v] = uint32_t(v);
+  const uint64_t m = __ballot(t);  // the wave's vertices: word v >> 6 (blocks of 256)
+  if ((threadIdx.x & 63) == 0 && m) atomicAnd((unsigned long long*)&act[v >> 6], ~m);
   block_atomic_add(changed, t ? 1ull : 0ull);
 }
 
@@ -237,10 +254,16 @@ SccResult scc_device(ll_gpu_graph* g, int level, DevBuf& labels64) {
   k_fill_u32<<<grid_for(V, 256), 256, 0, s>>>(label.as<uint32_t>(), V, kNone);
   SccResult r;
   const unsigned gV = grid_for(V, 256);
+  DevBuf act;
+  act.alloc(words * 8, s);
   auto trim = [&](int rounds) {
+    // the active bitmap from the labels as they stand (fw-bw and colouring
+    // label whole components)
+    k_active_bitmap<<<grid_for(words * 64, 256), 256, 0, s>>>(label.as<uint32_t>(), V,
+                                                             act.as<uint64_t>());
     for (int i = 0; i < rounds; i++) {
       k_zero<<<1, 1, 0, s>>>(cnt.as<unsigned long long>());
-      k_trim<<<gV, 256, 0, s>>>(io, ia, oo, oa, label.as<uint32_t>(), V,
+      k_trim<<<gV, 256, 0, s>>>(io, ia, oo, oa, label.as<uint32_t>(), act.as<uint64_t>(), V,
                                 cnt.as<unsigned long long>());
       LLG_HIP(hipGetLastError());
       r.sweeps++;
