@@ -416,33 +416,70 @@ __global__ __launch_bounds__(kBlock) void k_bu_short(const uint32_t* rows, const
   block_counters(n_new, m_new, exam, c);
 }
 
+// Rows of more than kLongRow in-edges: kSubLanes lanes per row, 64 /
+// kSubLanes rows of a wave walked side by side (each sub-group kSubLanes
+// neighbours per step), so a wave's rows are not one chain of dependent loads
+// after another; the wave steps until every one of its rows has found a
+// parent or run out of in-edges. Widths 4 to 32 give identical counts run
+// after run; the whole-wave form (64) gave run-to-run different counts in
+// measurement and its cause is not found, so it is not allowed.
+constexpr int kSubLanes = 8;
+static_assert(kSubLanes >= 4 && kSubLanes <= 32 && (kSubLanes & (kSubLanes - 1)) == 0,
+              "sub-group width must be a power of two in [4, 32]");
+
 __global__ __launch_bounds__(kBlock) void k_bu_long(const uint32_t* rows, const uint64_t* ofs,
                                                     const uint32_t* adj, const uint32_t* outdeg,
                                                     const uint64_t* fbm, uint64_t* nbm,
                                                     int32_t* lv, int32_t next_lv, Counters* c) {
+  constexpr int kPerWave = 64 / kSubLanes;
+  constexpr uint64_t kSubMask = (uint64_t(1) << kSubLanes) - 1;
   const uint64_t n = c->n_long;  // stripe 0
   const int lane = threadIdx.x & 63;
+  const int sub = lane / kSubLanes, sl = lane % kSubLanes;
   const uint64_t nw = uint64_t(gridDim.x) * (kBlock / 64);
+  const uint64_t wid = uint64_t(blockIdx.x) * (kBlock / 64) + (threadIdx.x >> 6);
   unsigned long long n_new = 0, m_new = 0, exam = 0;
-  for (uint64_t i = uint64_t(blockIdx.x) * (kBlock / 64) + (threadIdx.x >> 6); i < n; i += nw) {
-    const uint32_t r = rows[i];
-    const uint64_t lo = ofs[r], hi = ofs[r + 1];
-    bool found = false;
-    for (uint64_t e0 = lo; e0 < hi && !found; e0 += 64) {
-      const uint64_t e = e0 + uint64_t(lane);
-      bool f = false;
-      if (e < hi) {
-        const uint32_t u = adj[e];
-        f = (fbm[u >> 6] >> (u & 63)) & 1;
-      }
-      const uint64_t m = __ballot(f);
-      found = m != 0;
-      if (lane == 0) exam += found ? uint64_t(__ffsll((long long)m)) : min(uint64_t(64), hi - e0);
+  // rounds the same for every lane of a wave
+  for (uint64_t i0 = wid * kPerWave; i0 < n; i0 += nw * kPerWave) {
+    const uint64_t i = i0 + uint64_t(sub);
+    uint32_t r = 0;
+    uint64_t e0 = 0, hi = 0;
+    if (i < n) {
+      r = rows[i];
+      e0 = ofs[r];
+      hi = ofs[r + 1];
     }
-    if (found && lane == 0) {
-      found_row(r, lv, next_lv, nbm);
-      n_new++;
-      m_new += outdeg[r];
+    bool live = e0 < hi, found = false;
+    unsigned long long ex = 0;  // the same in every lane of the sub-group
+    while (__ballot(live)) {
+      bool f = false;
+      if (live) {
+        const uint64_t e = e0 + uint64_t(sl);
+        if (e < hi) {
+          const uint32_t u = adj[e];
+          f = (fbm[u >> 6] >> (u & 63)) & 1;
+        }
+      }
+      const uint64_t mine = (__ballot(f) >> (sub * kSubLanes)) & kSubMask;
+      if (live) {
+        if (mine) {
+          found = true;
+          ex += uint64_t(__ffsll((long long)mine));
+          live = false;
+        } else {
+          ex += min(uint64_t(kSubLanes), hi - e0);
+          e0 += kSubLanes;
+          live = e0 < hi;
+        }
+      }
+    }
+    if (sl == 0) {
+      exam += ex;
+      if (found) {
+        found_row(r, lv, next_lv, nbm);
+        n_new++;
+        m_new += outdeg[r];
+      }
     }
   }
   block_counters(n_new, m_new, exam, c);
