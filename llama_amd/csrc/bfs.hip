@@ -377,18 +377,35 @@ __global__ __launch_bounds__(kBlock) void k_bu_short(const uint32_t* rows, const
                                                      const uint64_t* fbm, uint64_t* nbm,
                                                      int32_t* lv, int32_t next_lv, Counters* c) {
   const uint64_t n = c->n_short;  // stripe 0
+  const uint64_t stride = uint64_t(gridDim.x) * kBlock;
   unsigned long long n_new = 0, m_new = 0, exam = 0;
+  // Rows are walked one after another per thread; the list entry two rows
+  // ahead and the offsets of the next row are loaded before the current row
+  // is walked, so they arrive while its neighbours are tested.
+  const uint64_t i0 = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
+  uint32_t r_next = 0, r_after = 0;
+  uint64_t lo_next = 0, hi_next = 0;
+  if (i0 < n) {
+    r_next = rows[i0];
+    lo_next = ofs[r_next];
+    hi_next = ofs[r_next + 1];
+  }
+  if (i0 + stride < n) r_after = rows[i0 + stride];
   // rounds the same for every lane of a wave (wave_set_bits)
-  for (uint64_t base = uint64_t(blockIdx.x) * kBlock; base < n;
-       base += uint64_t(gridDim.x) * kBlock) {
+  for (uint64_t base = uint64_t(blockIdx.x) * kBlock; base < n; base += stride) {
     const uint64_t i = base + threadIdx.x;
     const bool valid = i < n;
-    uint32_t r = 0;
+    const uint32_t r = r_next;
+    uint64_t e = lo_next;
+    const uint64_t hi = hi_next;
+    if (i + stride < n) {
+      r_next = r_after;
+      lo_next = ofs[r_next];
+      hi_next = ofs[r_next + 1];
+    }
+    if (i + 2 * stride < n) r_after = rows[i + 2 * stride];
     bool found = false;
     if (valid) {
-      r = rows[i];
-      uint64_t e = ofs[r];
-      const uint64_t hi = ofs[r + 1];
       while (e < hi && !found) {
         const uint32_t m = uint32_t(min(uint64_t(4), hi - e));
         uint32_t u[4];
