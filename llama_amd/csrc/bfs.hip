@@ -15,10 +15,11 @@
 //     listed in ascending order from the visited bitmap before each level:
 //     rows of at most 32 in-edges a thread each, four neighbour IDs and then
 //     their four frontier words at a time, stopping at the first neighbour in
-//     the frontier (the V/8-byte bitmap stays in L2); longer rows a wave
-//     each, 64 neighbours per step, stopping once a ballot finds one; a
-//     wave's found rows OR their next-frontier bits per bitmap word before
-//     one atomic per word.
+//     the frontier (the V/8-byte bitmap stays in L2); longer rows eight
+//     lanes each, eight rows of a wave side by side, eight neighbours per
+//     row and step, a row stopping once its lanes find one; a wave's found
+//     rows OR their next-frontier bits per bitmap word before one atomic per
+//     word.
 // Rows no search from the root can enter start as visited: rows without a
 // live in-edge always, and, when the root's successors all lie in the
 // snapshot's reach set (the vertices reachable from its highest-degree
@@ -65,7 +66,7 @@ struct alignas(128) Counters {
   unsigned long long aux;        // stripe 0: 1 when the search excludes the reach complement
   unsigned long long found;      // vertices a top-down level claimed (queued or not)
   unsigned long long n_short;    // stripe 0: open rows walked by a thread each
-  unsigned long long n_long;     // stripe 0: open rows walked by a wave each
+  unsigned long long n_long;     // stripe 0: open rows walked by a lane group each
 };
 static_assert(sizeof(Counters) == 128, "one stripe per 128-byte line");
 
@@ -232,8 +233,8 @@ __global__ void k_queue_to_bitmap(const uint32_t* q, uint64_t n, uint64_t* bm) {
 // excluded), each walking its in-neighbours until one is in the frontier
 // (bitmap of V/8 bytes, L2-resident) -- most rows that are found stop after a
 // few edges.  Rows of in-degree <= kLongRow: a thread per row, four
-// neighbours loaded before their frontier words; longer rows: a wave per row,
-// 64 neighbours per step, stopping once a ballot finds one.  Found rows take
+// neighbours loaded before their frontier words; longer rows: kSubLanes lanes
+// per row, stopping once the row's lanes find one.  Found rows take
 // the next level and their bit of the next frontier.  The row lists are
 // rebuilt from the visited bitmap before each bottom-up level, their lengths
 // stay on the device (stripe 0 of the counters), and both kernels run a

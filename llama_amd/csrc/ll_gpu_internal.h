@@ -315,7 +315,7 @@ struct ll_gpu_graph {
   llgpu::DevBuf sym_rows;
   uint64_t sym_nblock = 0, sym_nglobal = 0, sym_work = 0, sym_maxrow = 0;
   // rows of the in-CSR with no live in-edge (bitmap, excluded from BFS), then
-  // the rows with more than 32 (bitmap, walked a wave per row bottom-up)
+  // the rows with more than 32 (bitmap, walked by lane groups bottom-up)
   int dead_level = -1;
   llgpu::DevBuf dead_in;
   // complement of the vertices reachable from the hub (bitmap), their out-edges
