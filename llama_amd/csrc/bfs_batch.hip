@@ -295,7 +295,7 @@ __global__ __launch_bounds__(kBlock) void k_ms_pull(MsArgs<W> a) {
     __shared__ W wred[kBlock];
     wred[threadIdx.x] = acc;
     __syncthreads();
-    for (int st = kBlock / 2; st > 0; st >>= 1) {
+    for (unsigned st = kBlock / 2; st > 0; st >>= 1) {
       if (threadIdx.x < st) wred[threadIdx.x] = wred[threadIdx.x] | wred[threadIdx.x + st];
       __syncthreads();
     }
